@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""
+Forward-pass benchmark on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload 2] [--precision fp64|mixed|both]
+
+A step is one complete forward-pass fit (every epoch: candidate scoring, hinge
+sweep, device argmax, append) of the workload's synthetic inputs, with the
+inputs already resident in HBM when the timed region starts (the upload and
+device-side constructor run before the clock starts; their time is reported
+separately as `upload_s`).  `value` is candidates scored per second
+(sum over epochs of the (X column, parent) candidates, divided by the fit
+time).  The line also carries the fit seconds of each form, whether the
+forms chose the same term sequence, the roofline of the dominant kernel
+(HIP events on the library's stream, live in the timed region) and the CPU
+baseline: the reference's own CPU path (compiled from its sources into
+oracle/_ref) or, where that is absent, the oracle's restatement, timed on
+this host in the same run.
+"""
+
+import argparse
+import glob
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64 = 78.6e12     # MI355X FP64 (vector and matrix), FLOP/s
+PEAK_HBM = 8.0e12       # MI355X HBM3E, B/s
+CPU_SAMPLE_S = 20.0
+
+
+def workload_inputs(k):
+    from mars_amd import datagen
+
+    if k == 4:
+        raise SystemExit("workload 4 (batched fits) is benchmarked with --workload 4 once the batch path exists")
+    return datagen.GENERATORS[k](), dict(datagen.FIT_KW[k])
+
+
+def run_form(X, y, fit_kw, precision, steps, warmup):
+    import numpy as np
+
+    import mars_amd
+    from mars_amd import marslib
+
+    w = np.ones(len(X), dtype=np.float32)
+    max_terms = fit_kw["max_terms"]
+
+    def one(timing):
+        algo = marslib.MarsAlgo(X, y, w, max_terms, precision=precision)
+        algo.set_timing(timing)
+        return algo
+
+    for _ in range(warmup):
+        mars_amd.fit(X, y, w, algo=one(False), precision=precision, **fit_kw)
+    t0 = time.perf_counter()
+    algos = [one(True) for _ in range(steps)]
+    upload_s = (time.perf_counter() - t0) / steps
+    traces = []
+    t0 = time.perf_counter()
+    models = []
+    for a in algos:
+        tr = []
+        models.append(mars_amd.fit(X, y, w, algo=a, trace=tr, precision=precision, **fit_kw))
+        traces.append(tr)
+    wall = time.perf_counter() - t0
+    cands = sum(s["n_cand"] for s in traces[0])
+    ks = {}
+    for a in algos:
+        for r in a.kernel_stats():
+            d = ks.setdefault(r["name"], dict(launches=0, ms=0.0, flops=0.0, bytes=0.0))
+            for f in ("launches", "ms", "flops", "bytes"):
+                d[f] += r[f]
+    seq = [(r["type"].decode(), int(r["basis"]), int(r["input"]), float(r["hinge"])) for r in models[0]]
+    return dict(wall=wall, fit_s=wall / steps, upload_s=upload_s, cands=cands, kstats=ks, seq=seq,
+                trace=traces[0], epochs=len(traces[0]), dgks=algos[0].stats()["dgks_fired"])
+
+
+def roofline(ks):
+    name, d = max(ks.items(), key=lambda kv: kv[1]["ms"])
+    avg_s = d["ms"] / 1e3 / d["launches"]
+    if d["flops"] > 0:
+        ach = d["flops"] / d["launches"] / avg_s
+        return name, dict(bound="mfma", achieved=ach / 1e12, peak=PEAK_FP64 / 1e12, unit="TFLOP/s",
+                          frac=ach / PEAK_FP64, traffic=None, avg_launch_ms=d["ms"] / d["launches"],
+                          launches=d["launches"])
+    ach = d["bytes"] / d["launches"] / avg_s
+    return name, dict(bound="hbm", achieved=ach / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s", frac=ach / PEAK_HBM,
+                      traffic=None, avg_launch_ms=d["ms"] / d["launches"], launches=d["launches"])
+
+
+def cpu_baseline(X, y, fit_kw, budget_s):
+    """Reference CPU path (oracle/_ref) if built, else the oracle restatement, same inputs."""
+    import importlib.util
+
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import fit as F
+
+    ref = glob.glob(os.path.join(ROOT, "oracle", "_ref", "marslib*.so"))
+    if ref:
+        spec = importlib.util.spec_from_file_location("marslib", ref[0])
+        lib = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(lib)
+        kind = "reference"
+    else:
+        import oracle_marslib as lib
+
+        kind = "port"
+    cores = min(16, os.cpu_count() or 1)
+    w = np.ones(len(X), dtype=np.float32)
+    # bounded sample: the first epochs of the same fit, grown until about budget_s of CPU time
+    epochs, spent, result = 2, 0.0, None
+    while True:
+        t0 = time.perf_counter()
+        _, tr = F.fit(lib, X, y, w, threads=cores, max_epochs=epochs, **fit_kw)
+        spent = time.perf_counter() - t0
+        result = (tr, spent)
+        if spent > budget_s / 3 or len(tr) < epochs:
+            break
+        epochs *= 2
+    tr, spent = result
+    cands = sum(int(s.get("n_cand", 0)) for s in tr)
+    return dict(value=cands / spent if spent > 0 else None, unit="candidates/s", cores=cores, kind=kind,
+                sample="first %d epochs of the same fit (%d candidates, %.1f s)" % (len(tr), cands, spent))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", type=int, default=2)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=CPU_SAMPLE_S)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design "
+              "is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    (X, y), fit_kw = workload_inputs(args.workload)
+    forms = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    res = {f: run_form(X, y, fit_kw, f, args.steps, args.warmup) for f in forms}
+    head = res[forms[-1]]
+    kname, roof = roofline(head["kstats"])
+    out = {
+        "metric": "forward-pass fit s + candidates/s, mixed vs FP64 GPU at same term sequence",
+        "value": head["cands"] / head["fit_s"],
+        "unit": "candidates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["fit_s"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if forms[-1] == "fp64" else "bf16x3/f32 + f64",
+        "data": "synthetic (mars_amd.datagen, seeded)",
+        "config": dict(workload="workload %d: N=%d, p=%d, %s" % (args.workload, X.shape[0], X.shape[1], fit_kw),
+                       precision=forms[-1]),
+        "fit_s": {f: r["fit_s"] for f, r in res.items()},
+        "upload_s": {f: r["upload_s"] for f, r in res.items()},
+        "candidates_per_fit": head["cands"],
+        "epochs": head["epochs"],
+        "terms": len(head["seq"]),
+        "dgks_fired": head["dgks"],
+        "same_term_sequence": (len({json.dumps(r["seq"]) for r in res.values()}) == 1) if len(res) > 1 else None,
+        "roofline_kernel": kname,
+        "roofline": roof,
+        "kernels": {k: dict(v, avg_ms=v["ms"] / max(v["launches"], 1)) for k, v in head["kstats"].items()},
+    }
+    if len(res) > 1:
+        out["speedup_mixed_vs_fp64"] = res["fp64"]["fit_s"] / res["mixed"]["fit_s"]
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(X, y, fit_kw, args.cpu_budget)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,158 @@
+/*
+ *  mars_hip.h -- C ABI of the MI355X (gfx950) forward-pass library libmars_hip.so.
+ *
+ *  Plain C types only (no HIP, torch or C++ types), so the reference's
+ *  C++ host (marsalgo.cc / marslib.cc) or any FFI can bind it directly.
+ *
+ *  Two ways to drive it:
+ *
+ *  1. Mirror mode -- drop-in for the reference's GPU boundary
+ *     (cuda/mars_cuda.h:43-154).  The caller owns the fit state on the host
+ *     (B col-major, Bo row-major, the scaled candidate columns) and the
+ *     library keeps device copies in sync, exactly as the reference's
+ *     mars::cuda::* functions do.  Each mars_hip_* function below names the
+ *     mars::cuda function it replaces.
+ *
+ *  2. Resident mode -- the whole fit state lives in HBM.  mars_hip_fit_create
+ *     performs the MarsAlgo constructor on the device (marsalgo.cc:266-370),
+ *     mars_hip_fit_eval scores every (X column, parent basis) candidate --
+ *     orthonormalization, linear delta-SSE, hinge sweep, device argmax -- and
+ *     mars_hip_fit_append adds a basis function on the device
+ *     (marsalgo.cc:760-875).  Only the (p_x, m) result arrays and a few
+ *     scalars cross PCIe per forward step.
+ *
+ *  Every function returns MARS_HIP_OK (0) or a negative status; the message
+ *  of the last failure on the calling thread is mars_hip_last_error().
+ *  Calls on one context are synchronous and not thread-safe (one host thread
+ *  per context), like the reference (mars_cuda.h:26-33).
+ */
+#ifndef MARS_HIP_H
+#define MARS_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MARS_HIP_OK 0
+#define MARS_HIP_EINVAL (-1)   /* bad argument / shape / state            */
+#define MARS_HIP_EDEVICE (-2)  /* HIP runtime error (allocation, launch)  */
+#define MARS_HIP_EFULL (-3)    /* basis matrix is full                    */
+
+/* Precision forms of the candidate scoring. */
+#define MARS_HIP_FORM_PLAIN 0  /* all FP64 arithmetic on f32 storage, CPU rounding points */
+#define MARS_HIP_FORM_MIXED 1  /* BF16-split / FP32 MFMA bulk + FP64 refinement + FP64 near-tie recheck */
+
+typedef struct mars_hip_ctx mars_hip_ctx;
+
+/* Winner of one forward step, as chosen on the device (mars.py:342-343). */
+typedef struct mars_hip_best {
+    double sse1;   /* best linear delta-SSE (first maximum in row-major (xcol, bcol) order) */
+    int xcol1, bcol1;
+    double sse2;   /* best hinge-pair delta-SSE */
+    int xcol2, bcol2;
+    int knot2;     /* sorted position of the winning knot (-1 if none) */
+    double cut2;   /* raw X value at that knot (NaN if none) */
+    double sse0;   /* delta-SSE of the current basis (sum of ybo^2) */
+} mars_hip_best;
+
+/* Text of the last error on this thread ("" if none). */
+const char *mars_hip_last_error(void);
+
+/* ---------------------------------------------------------------------- */
+/*  Mirror mode: the reference's mars::cuda API                           */
+/* ---------------------------------------------------------------------- */
+
+/* replaces mars::cuda::context_create (cuda/mars_cuda.h:43) */
+int mars_hip_context_create(size_t n, size_t max_terms, double tol, mars_hip_ctx **out);
+
+/* replaces mars::cuda::context_destroy (cuda/mars_cuda.h:48); NULL is a no-op */
+int mars_hip_context_destroy(mars_hip_ctx *ctx);
+
+/* replaces mars::cuda::context_batch_capacity (cuda/mars_cuda.h:55) */
+int mars_hip_context_batch_capacity(mars_hip_ctx *ctx, size_t *capacity);
+
+/* replaces mars::cuda::context_sync_basis (cuda/mars_cuda.h:71):
+ * B (n, *) col-major ld ldB; Bo (n, m) row-major ld ldBo; uploads new columns only. */
+int mars_hip_context_sync_basis(mars_hip_ctx *ctx, size_t m, const float *B, size_t ldB,
+                                const float *Bo, size_t ldBo);
+
+/* replaces mars::cuda::context_set_target (cuda/mars_cuda.h:80): the normalized f32 target */
+int mars_hip_context_set_target(mars_hip_ctx *ctx, const float *y);
+
+/* replaces mars::cuda::context_sync_xcols (cuda/mars_cuda.h:95):
+ * make columns `cols` of X (n, p_x) col-major ld ldX, scaled by s, resident. */
+int mars_hip_context_sync_xcols(mars_hip_ctx *ctx, size_t p_x, const float *X, size_t ldX,
+                                const float *s, const int *cols, size_t ncols);
+
+/* replaces mars::cuda::orthonormalize (cuda/mars_cuda.h:127): x (n) scaled candidate,
+ * mask (p) parent columns; Bx (n, p) col-major ld ldBx [out, may be NULL];
+ * ybx (p) [out, may be NULL]; dgks_fired [in/out, may be NULL] += retries. */
+int mars_hip_orthonormalize(mars_hip_ctx *ctx, size_t m, size_t p, const float *x,
+                            const int *mask, float *Bx, size_t ldBx, double *ybx,
+                            long *dgks_fired);
+
+/* replaces mars::cuda::orthonormalize_batch (cuda/mars_cuda.h:152): P candidates
+ * (src_xcol[j], src_basis[j]) over resident columns; ybx (P) [out]. */
+int mars_hip_orthonormalize_batch(mars_hip_ctx *ctx, size_t m, size_t P, const int *src_xcol,
+                                  const int *src_basis, double *ybx, long *dgks_fired);
+
+/* ---------------------------------------------------------------------- */
+/*  Resident mode: MarsAlgo on the device                                 */
+/* ---------------------------------------------------------------------- */
+
+/* Device counterpart of MarsAlgo::MarsAlgo (marsalgo.cc:266-370) and
+ * new_algo (marslib.cc:12-37): X (n, p_x) col-major ld ldX, y (n), w (n).
+ * X, y, w are copied; the caller need not keep them.  `form` is a
+ * MARS_HIP_FORM_* value. */
+int mars_hip_fit_create(const float *X, size_t ldX, const float *y, const float *w, size_t n,
+                        size_t p_x, size_t max_terms, int form, mars_hip_ctx **out);
+
+/* MarsAlgo::nbasis / yvar / dsse (marsalgo.cc:381-400) */
+int mars_hip_fit_info(mars_hip_ctx *ctx, int *nbasis, double *yvar, double *dsse);
+
+/* Select the precision form for subsequent evals. */
+int mars_hip_fit_set_form(mars_hip_ctx *ctx, int form);
+
+/* Score all candidates of one forward step -- the device counterpart of the
+ * binding's eval (marslib.cc:41-148) / MarsAlgo::eval (marsalgo.cc:402-662)
+ * for every X column at once, plus the argmax (mars.py:342-343).
+ *   mask        : (mask_rows, m) bool, row stride mask_stride bytes, m = nbasis
+ *   linear/hinge/cut : (mask_rows, m) row-major [out, each may be NULL]
+ *   best        : device-chosen winners [out, may be NULL]            */
+int mars_hip_fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t mask_rows,
+                      size_t mask_stride, int min_span, int end_span, int linear_only,
+                      double *linear, double *hinge, double *cut, mars_hip_best *best);
+
+/* Device counterpart of MarsAlgo::append (marsalgo.cc:760-875): type 'l',
+ * '+' or '-'.  *mse receives the mse after the append, or -1 when the new
+ * column is numerically degenerate (the basis is then unchanged). */
+int mars_hip_fit_append(mars_hip_ctx *ctx, char type, int xcol, int bcol, float h, double *mse);
+
+/* Copy the live basis out: B and Bo as (n, m) row-major f32 (either may be NULL). */
+int mars_hip_fit_basis(mars_hip_ctx *ctx, float *B, float *Bo, size_t *m_out);
+
+/* Cumulative DGKS retries (orthonormalize + append) and device milliseconds
+ * spent in the scoring kernels since the context was created. */
+int mars_hip_stats(mars_hip_ctx *ctx, long *dgks_fired, double *eval_ms);
+
+/* Per-kernel-class timing with HIP events on the context's stream (off by
+ * default).  Each record: launches, summed device milliseconds and the
+ * summed algorithmic flops / HBM bytes of those launches. */
+typedef struct mars_hip_kstat {
+    char name[24];
+    long launches;
+    double ms;
+    double flops;
+    double bytes;
+} mars_hip_kstat;
+
+int mars_hip_timing(mars_hip_ctx *ctx, int enable);
+int mars_hip_kernel_stats(mars_hip_ctx *ctx, mars_hip_kstat *out, int capacity, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MARS_HIP_H */

@@ -1,0 +1,227 @@
+/*
+ *  marslib -- Python binding of the device-resident MarsAlgo.
+ *
+ *  Same module surface as the reference binding (marslib.cc:152-179):
+ *  MarsAlgo(X, y, w, max_terms) with eval / nbasis / yvar / append, the same
+ *  argument names, dtypes, layout checks and error texts.  The work is done by
+ *  libmars_hip.so through its C ABI (include/mars_hip.h); this file holds no
+ *  arithmetic.  `cuda=True` is the device route, as in the reference; this
+ *  library has no CPU scoring path, so `cuda=False` raises.
+ */
+#include <pybind11/pybind11.h>
+#include <pybind11/numpy.h>
+
+#include <cmath>
+#include <stdexcept>
+#include <string>
+
+#include "mars_hip.h"
+
+namespace py = pybind11;
+
+namespace {
+
+void verify(bool ok, const char *msg)
+{
+    if (!ok) throw std::runtime_error(msg);
+}
+
+void check(int rc)
+{
+    if (rc != MARS_HIP_OK) throw std::runtime_error(mars_hip_last_error());
+}
+
+int parse_form(const std::string &f)
+{
+    if (f == "fp64" || f == "plain") return MARS_HIP_FORM_PLAIN;
+    if (f == "mixed") return MARS_HIP_FORM_MIXED;
+    throw std::runtime_error("unknown precision form '" + f + "' (expected 'fp64' or 'mixed')");
+}
+
+class MarsAlgo {
+public:
+    /* new_algo (marslib.cc:12-37) + MarsAlgo::MarsAlgo (marsalgo.cc:266-370) on the device */
+    MarsAlgo(py::array_t<float> X, py::array_t<float> y, py::array_t<float> w, int max_terms,
+             const std::string &precision)
+    {
+        py::buffer_info xi = X.request(), yi = y.request(), wi = w.request();
+        verify(xi.ndim == 2, "expected 2D array for X");
+        verify(yi.ndim == 1, "expected 1D array for y");
+        verify(wi.ndim == 1, "expected 1D array for w");
+        verify(xi.strides[0] == sizeof(float), "X must be column-major");
+        verify(yi.strides[0] == sizeof(float), "y must be column-major");
+        verify(wi.strides[0] == sizeof(float), "w must be column-major");
+        verify(xi.shape[0] == yi.shape[0], "invalid dimension for y");
+        verify(xi.shape[0] == wi.shape[0], "invalid dimension for w");
+        verify(max_terms >= 1, "max_terms must be >= 1");
+        n_ = (size_t)xi.shape[0];
+        px_ = (size_t)xi.shape[1];
+        const size_t ldx = xi.shape[1] > 1 ? (size_t)(xi.strides[1] / sizeof(float)) : n_;
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = mars_hip_fit_create((const float *)xi.ptr, ldx, (const float *)yi.ptr,
+                                     (const float *)wi.ptr, n_, px_, (size_t)max_terms,
+                                     parse_form(precision), &ctx_);
+        }
+        check(rc);
+    }
+
+    ~MarsAlgo()
+    {
+        if (ctx_) mars_hip_context_destroy(ctx_);
+    }
+
+    int nbasis() const
+    {
+        int m = 0;
+        check(mars_hip_fit_info(ctx_, &m, nullptr, nullptr));
+        return m;
+    }
+
+    double yvar() const
+    {
+        double v = 0;
+        check(mars_hip_fit_info(ctx_, nullptr, &v, nullptr));
+        return v;
+    }
+
+    double dsse() const
+    {
+        double v = 0;
+        check(mars_hip_fit_info(ctx_, nullptr, nullptr, &v));
+        return v;
+    }
+
+    void set_precision(const std::string &f) { check(mars_hip_fit_set_form(ctx_, parse_form(f))); }
+
+    /* binding eval (marslib.cc:41-148); `best` adds the device argmax */
+    py::tuple eval(py::array_t<bool> mask, int min_span, int endspan, bool linear, int threads,
+                   bool cuda, bool with_best)
+    {
+        (void)threads;  // one host thread drives the device, as the reference's cuda route
+        verify(cuda, "mars_amd scores candidates on the GPU only; pass cuda=True");
+        py::buffer_info mi = mask.request();
+        verify(mi.ndim == 2, "expected 2D array for mask");
+        verify(mi.strides[1] == sizeof(bool), "mask must be row-major");
+        verify(mi.shape[1] == nbasis(), "invalid dimension for mask");
+        verify(min_span >= 1, "min_span must be >= 1");
+        verify(mi.shape[0] == (ssize_t)px_, "invalid dimension for mask");
+        const ssize_t rows = mi.shape[0], cols = mi.shape[1];
+        py::array_t<double> d1({rows, cols}), d2({rows, cols}), hc({rows, cols});
+        mars_hip_best b{};
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = mars_hip_fit_eval(ctx_, (const unsigned char *)mi.ptr, (size_t)rows,
+                                   (size_t)(mi.strides[0] / sizeof(bool)), min_span, endspan,
+                                   linear ? 1 : 0, d1.mutable_data(), d2.mutable_data(),
+                                   hc.mutable_data(), &b);
+        }
+        check(rc);
+        if (!with_best) return py::make_tuple(b.sse0, d1, d2, hc);
+        py::dict best;
+        best["sse1"] = b.sse1;
+        best["j1"] = py::make_tuple(b.xcol1, b.bcol1);
+        best["sse2"] = b.sse2;
+        best["j2"] = py::make_tuple(b.xcol2, b.bcol2);
+        best["knot2"] = b.knot2;
+        best["cut2"] = b.cut2;
+        return py::make_tuple(b.sse0, d1, d2, hc, best);
+    }
+
+    /* MarsAlgo::append (marsalgo.cc:760-875) */
+    double append(char type, int xcol, int bcol, float h)
+    {
+        double mse = -1.0;
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = mars_hip_fit_append(ctx_, type, xcol, bcol, h, &mse);
+        }
+        check(rc);
+        return mse;
+    }
+
+    py::tuple basis() const
+    {
+        size_t m = 0;
+        check(mars_hip_fit_info(ctx_, nullptr, nullptr, nullptr));
+        int mm = nbasis();
+        py::array_t<float> B({(ssize_t)n_, (ssize_t)mm}), Bo({(ssize_t)n_, (ssize_t)mm});
+        check(mars_hip_fit_basis(ctx_, B.mutable_data(), Bo.mutable_data(), &m));
+        return py::make_tuple(B, Bo);
+    }
+
+    py::dict stats() const
+    {
+        long fired = 0;
+        double ms = 0;
+        check(mars_hip_stats(ctx_, &fired, &ms));
+        py::dict d;
+        d["dgks_fired"] = fired;
+        d["eval_ms"] = ms;
+        return d;
+    }
+
+    void set_timing(bool on) { check(mars_hip_timing(ctx_, on ? 1 : 0)); }
+
+    py::list kernel_stats() const
+    {
+        mars_hip_kstat ks[32];
+        int cnt = 0;
+        check(mars_hip_kernel_stats(ctx_, ks, 32, &cnt));
+        py::list out;
+        for (int i = 0; i < cnt && i < 32; ++i) {
+            py::dict d;
+            d["name"] = std::string(ks[i].name);
+            d["launches"] = ks[i].launches;
+            d["ms"] = ks[i].ms;
+            d["flops"] = ks[i].flops;
+            d["bytes"] = ks[i].bytes;
+            out.append(d);
+        }
+        return out;
+    }
+
+private:
+    mars_hip_ctx *ctx_ = nullptr;
+    size_t n_ = 0, px_ = 0;
+};
+
+} // namespace
+
+PYBIND11_MODULE(_marslib, m)
+{
+    m.doc() = "Multivariate Adaptive Regression Splines -- MI355X forward pass";
+    m.attr("__version__") = "0.12";
+    py::class_<MarsAlgo>(m, "MarsAlgo")
+        .def(py::init<py::array_t<float>, py::array_t<float>, py::array_t<float>, int,
+                      const std::string &>(),
+             py::arg("X").noconvert(), py::arg("y").noconvert(), py::arg("w").noconvert(),
+             py::arg("max_terms"), py::arg("precision") = "fp64", py::keep_alive<1, 2>())
+        .def(
+            "eval",
+            [](MarsAlgo &a, py::array_t<bool> mask, int min_span, int endspan, bool linear_only,
+               int threads, bool cuda) {
+                return a.eval(mask, min_span, endspan, linear_only, threads, cuda, false);
+            },
+            py::arg("mask").noconvert(), py::arg("min_span"), py::arg("endspan"),
+            py::arg("linear_only"), py::arg("threads"), py::arg("cuda") = false)
+        .def(
+            "eval_best",
+            [](MarsAlgo &a, py::array_t<bool> mask, int min_span, int endspan, bool linear_only) {
+                return a.eval(mask, min_span, endspan, linear_only, 1, true, true);
+            },
+            py::arg("mask").noconvert(), py::arg("min_span"), py::arg("endspan"),
+            py::arg("linear_only"))
+        .def("nbasis", &MarsAlgo::nbasis)
+        .def("yvar", &MarsAlgo::yvar)
+        .def("dsse", &MarsAlgo::dsse)
+        .def("append", &MarsAlgo::append)
+        .def("set_precision", &MarsAlgo::set_precision)
+        .def("basis", &MarsAlgo::basis)
+        .def("stats", &MarsAlgo::stats)
+        .def("set_timing", &MarsAlgo::set_timing)
+        .def("kernel_stats", &MarsAlgo::kernel_stats);
+}
