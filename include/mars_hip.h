@@ -137,6 +137,46 @@ int mars_hip_fit_basis(mars_hip_ctx *ctx, float *B, float *Bo, size_t *m_out);
  * spent in the scoring kernels since the context was created. */
 int mars_hip_stats(mars_hip_ctx *ctx, long *dgks_fired, double *eval_ms);
 
+/* ---------------------------------------------------------------------- */
+/*  Batching form: many independent linear_only fits advanced together    */
+/* ---------------------------------------------------------------------- */
+
+/* Forward-pass options of mars.fit (mars.py:190-214) that the batch driver honours. */
+typedef struct mars_hip_batch_opts {
+    int max_degree;     /* default 3 */
+    double penalty;     /* default 3.0 */
+    int self_interact;  /* default 0 */
+    int r2_window;      /* default 32 */
+    double r2_thresh;   /* default 3e-5 */
+} mars_hip_batch_opts;
+
+/* One model row (mars.py:261-275); type 'i' (intercept) or 'l'. */
+typedef struct mars_hip_term {
+    int type;
+    int basis;
+    int input;
+    int order;
+    double mse;     /* append mse (normalized units); 0 for the intercept */
+    float r2;
+    float r2_cv;
+} mars_hip_term;
+
+/* Create `nfits` resident fits: fit i has X[i] (n[i], p_x[i]) col-major ld ldX[i],
+ * y[i], w[i] (w[i] may be NULL for unit weights). */
+int mars_hip_batch_create(int nfits, const float *const *X, const size_t *ldX, const float *const *y,
+                          const float *const *w, const size_t *n, const size_t *p_x, size_t max_terms,
+                          int form, mars_hip_ctx **out);
+
+/* Run every fit's linear_only forward pass to completion, all fits advancing
+ * one term per step together (mars.py:285-406 restated in C++): one batched
+ * scoring pass, per-fit device argmax and one batched append per step.
+ * terms: (nfits, max_terms) rows; nterms[i]: rows of fit i; steps: step count. */
+int mars_hip_batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts *opts, mars_hip_term *terms, int *nterms,
+                       int *steps);
+
+/* Mixed form: candidates scored in the bulk pass and those rescored exactly in FP64. */
+int mars_hip_mixed_stats(mars_hip_ctx *ctx, long *scored, long *rescored);
+
 /* Per-kernel-class timing with HIP events on the context's stream (off by
  * default).  Each record: launches, summed device milliseconds and the
  * summed algorithmic flops / HBM bytes of those launches. */

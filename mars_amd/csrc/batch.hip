@@ -1,0 +1,222 @@
+// Batching form: many independent linear_only MARS fits advanced together.
+//
+// The forward-pass driver of mars.fit (mars.py:285-406) restated in C++ for
+// the linear_only case with Fast-MARS off (every input and every parent is a
+// candidate each step).  One step = one batched scoring pass over every
+// active fit's candidates (plain FP64 or mixed form), one per-fit device
+// argmax, the GCV choice, and one batched device append; fits stop on their
+// own (no progress, R^2 threshold, GCV patience window, term budget).
+#include "internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace mh {
+namespace {
+
+template <typename F>
+int guarded_b(F &&f)
+{
+    try {
+        f();
+        g_last_error.clear();
+        return MARS_HIP_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return MARS_HIP_EDEVICE;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return MARS_HIP_EDEVICE;
+    }
+}
+
+struct FitDriver {
+    size_t max_terms = 0;            // after the dof clamp (mars.py:241-243)
+    std::vector<std::vector<int>> vars;  // variables of each basis function
+    std::vector<mars_hip_term> rows;
+    bool active = true;
+};
+
+void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *terms, int *nterms, int *steps)
+{
+    require(ctx->resident, "not a resident context");
+    const size_t F = ctx->fits.size();
+    std::vector<FitDriver> D(F);
+    auto dof = [&](double m) { return m + o.penalty * (m - 1); };
+    for (size_t f = 0; f < F; ++f) {
+        const FitState &fs = ctx->fits[f];
+        const double nt = (double)fs.n;
+        size_t mt = 0;
+        for (size_t t = 1; t <= ctx->max_terms; ++t)
+            if (dof((double)t) < nt) mt = t;
+        require(mt >= 1, "too few rows for any term");
+        D[f].max_terms = mt;
+        D[f].vars.assign(1, std::vector<int>());
+        mars_hip_term t0{'i', 0, 0, 0, 0.0, 0.0f, 0.0f};
+        D[f].rows.assign(1, t0);
+        D[f].active = fs.m < mt;
+    }
+    int step = 0;
+    std::vector<AppendArg> args(F);
+    std::vector<BestOut> best;
+    while (true) {
+        Cands &cs = ctx->cand;
+        cs.fit.clear();
+        cs.xc.clear();
+        cs.par.clear();
+        bool any = false;
+        for (size_t f = 0; f < F; ++f) {
+            FitDriver &d = D[f];
+            if (d.active && ctx->fits[f].m >= d.max_terms) d.active = false;
+            if (!d.active) continue;
+            any = true;
+            const FitState &fs = ctx->fits[f];
+            for (size_t i = 0; i < fs.px; ++i)
+                for (size_t b = 0; b < fs.m; ++b) {
+                    const std::vector<int> &v = d.vars[b];
+                    const bool has = std::find(v.begin(), v.end(), (int)i) != v.end();
+                    if ((int)v.size() < o.max_degree && (o.self_interact || !has)) {
+                        cs.fit.push_back((int)f);
+                        cs.xc.push_back((int)i);
+                        cs.par.push_back((int)b);
+                    }
+                }
+        }
+        if (!any) break;
+        ++step;
+        index_cands(ctx, cs);
+        const size_t P = cs.fit.size();
+        ctx->lin.ensure(std::max<size_t>(P, 1));
+        ctx->ybx.ensure(std::max<size_t>(P, 1));
+        if (P) {
+            if (ctx->form == MARS_HIP_FORM_MIXED)
+                mixed_score(ctx);
+            else
+                plain_score(ctx, cs, false, ctx->lin.p, ctx->ybx.p);
+        }
+        argmax_fits(ctx, cs, ctx->lin.p, best);
+        for (size_t f = 0; f < F; ++f) {
+            args[f] = AppendArg{0, 0, 0, 0, 0.0f, 0};
+            FitDriver &d = D[f];
+            if (!d.active) continue;
+            const FitState &fs = ctx->fits[f];
+            const double s1 = cs.fit_P[f] ? best[f].v1 : 0.0;
+            if (s1 <= 0.0) {  // all input data is filtered or zero (mars.py:345-346)
+                d.active = false;
+                continue;
+            }
+            const double n = (double)fs.n, m = (double)fs.m;
+            const double sse0 = host_dsse(fs);
+            auto gcv = [&](double mse, double mm) { return mse / std::pow(1.0 - dof(mm) / n, 2); };
+            const double mse1 = gcv((1.0 - sse0 - s1) / n, m + 1);
+            const double mse2 = gcv((1.0 - sse0 - 0.0) / n, m + 2);
+            require(mse1 <= mse2, "linear_only batch fit chose a hinge pair");
+            const int c = best[f].c1;  // position within the fit's candidates
+            const int g = cs.fit_c0[f] + c;
+            args[f] = AppendArg{0, cs.xc[g], cs.par[g], 1, 0.0f, 0};
+        }
+        std::vector<AppendOut> out;
+        fit_append_batch(ctx, args, out);
+        for (size_t f = 0; f < F; ++f) {
+            if (!args[f].active) continue;
+            FitDriver &d = D[f];
+            const FitState &fs = ctx->fits[f];
+            const double mse = out[f].mse;
+            if (mse < 0) {  // degenerate append: no progress (mars.py:393-394)
+                d.active = false;
+                continue;
+            }
+            const double n = (double)fs.n;
+            const double m = (double)(d.rows.size());
+            const double gm = mse / std::pow(1.0 - dof(m + 1) / n, 2);
+            std::vector<int> v = d.vars[args[f].bcol];
+            v.push_back(args[f].xcol);
+            mars_hip_term t;
+            t.type = 'l';
+            t.basis = args[f].bcol;
+            t.input = args[f].xcol;
+            t.order = (int)v.size();
+            t.mse = mse;
+            t.r2 = (float)(1.0 - mse / fs.yvar);
+            t.r2_cv = (float)(1.0 - gm / fs.yvar);
+            d.vars.push_back(v);
+            d.rows.push_back(t);
+            // stopping rules (mars.py:395-406)
+            // numpy compares the float32 model fields against Python floats in float32
+            if (t.r2 > (float)(1.0 - o.r2_thresh)) {
+                d.active = false;
+                continue;
+            }
+            const size_t R = d.rows.size();
+            if ((int)R > o.r2_window) {
+                float all = -INFINITY, last = -INFINITY;
+                for (size_t r = 0; r < R; ++r) {
+                    all = std::max(all, d.rows[r].r2_cv);
+                    if (r + o.r2_window >= R) last = std::max(last, d.rows[r].r2_cv);
+                }
+                if (last < all - (float)o.r2_thresh) d.active = false;
+            }
+        }
+    }
+    for (size_t f = 0; f < F; ++f) {
+        nterms[f] = (int)D[f].rows.size();
+        for (size_t r = 0; r < D[f].rows.size(); ++r) terms[f * ctx->max_terms + r] = D[f].rows[r];
+    }
+    if (steps) *steps = step;
+}
+
+} // namespace
+} // namespace mh
+
+using namespace mh;
+
+extern "C" {
+
+int mars_hip_batch_create(int nfits, const float *const *X, const size_t *ldX, const float *const *y,
+                          const float *const *w, const size_t *n, const size_t *p_x, size_t max_terms, int form,
+                          mars_hip_ctx **out)
+{
+    return guarded_b([&] {
+        require(out != nullptr, "null output pointer");
+        require(nfits >= 1, "need at least one fit");
+        require(X && ldX && y && n && p_x, "null argument");
+        require(form == MARS_HIP_FORM_PLAIN || form == MARS_HIP_FORM_MIXED, "unknown precision form");
+        *out = nullptr;
+        auto *ctx = new mars_hip_ctx;
+        try {
+            ctx_base_init(ctx, max_terms);
+            ctx->resident = true;
+            ctx->form = form;
+            ctx->fits.resize(nfits);
+            for (int f = 0; f < nfits; ++f) {
+                std::vector<float> ones;
+                const float *wf = w ? w[f] : nullptr;
+                if (!wf) {
+                    ones.assign(n[f], 1.0f);
+                    wf = ones.data();
+                }
+                fit_init(ctx, ctx->fits[f], X[f], ldX[f], y[f], wf, n[f], p_x[f]);
+            }
+        } catch (...) {
+            ctx_free(ctx);
+            throw;
+        }
+        *out = ctx;
+    });
+}
+
+int mars_hip_batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts *opts, mars_hip_term *terms, int *nterms,
+                       int *steps)
+{
+    return guarded_b([&] {
+        require(ctx && opts && terms && nterms, "null argument");
+        require(opts->max_degree >= 1, "max_degree must be >= 1");
+        batch_fit(ctx, *opts, terms, nterms, steps);
+    });
+}
+
+} // extern "C"

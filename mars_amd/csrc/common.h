@@ -1,4 +1,4 @@
-// Shared definitions of libmars_hip: context state, error plumbing, device helpers.
+// Shared definitions of libmars_hip: fit / context state, error plumbing, timing.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -58,10 +58,50 @@ struct DBuf {
     }
 };
 
-/* Per-step candidate block: candidates c in [0, P), grouped by X column. */
-struct Cands {
-    DBuf<int> xc, par;   // X column and parent basis column per candidate
-    std::vector<int> h_xc, h_par;
+/* Device view of one fit, read by every batched kernel. */
+struct FitDev {
+    const float *X;    // raw X, column-major (ldx)
+    const float *s;    // per-column 1/rms scale
+    const float *B;    // basis, row-major (ldb)
+    const float *Bo;   // orthonormal basis, row-major (ldb)
+    const float *y;    // normalized target (f32)
+    const double *ybo; // Bo^T y per live column
+    float *rbuf;       // residual / normalized candidate columns, row-major (ldr)
+    double *v;         // append: f64 residual of the new column (n)
+    long long n, ldx, ldb, ldr;
+    int m, c0;         // live basis count, first candidate of this fit
+    double tol;
+};
+
+/* One fit's resident state. */
+struct FitState {
+    size_t n = 0, px = 0;
+    size_t ldx = 0;   // device column stride of X (n rounded up to 32: aligned vector loads)
+    double tol = 0, yvar = 0;
+    size_t m = 0;
+    std::vector<double> ybo_h;
+    DBuf<float> X, s, B, Bo, y, rbuf;
+    DBuf<double> ybo, v;
+    size_t ldr = 0;
+    // hinge sort cache
+    DBuf<int> order;
+    DBuf<float> delta;
+    std::vector<char> sorted;
+    // mirror-mode bookkeeping
+    size_t m_synced = 0;
+    const float *y_key = nullptr;
+    std::vector<char> xres;
+    void release()
+    {
+        X.release(); s.release(); B.release(); Bo.release(); y.release(); rbuf.release();
+        ybo.release(); v.release(); order.release(); delta.release();
+    }
+};
+
+/* Work item of the batched kernels: candidates [c0, c0+nc) of fit `fit`, rows [r0, r1). */
+struct Item {
+    int fit, c0, nc, pad;
+    long long r0, r1;
 };
 
 /* Result slot returned by the device append. */
@@ -72,12 +112,28 @@ struct AppendOut {
     int fired;
 };
 
+struct AppendArg {
+    int type, xcol, bcol, active;
+    float h;
+    int pad;
+};
+
+/* Per-fit winner of one step (device argmax). */
+struct BestOut {
+    double v1;
+    int c1;
+    int pad1;
+    double v2;
+    int c2;
+    int knot2;
+};
+
 /* Kernel classes timed by mars_hip_timing. */
-enum KClass { K_TGEMM = 0, K_PROJECT, K_FINALIZE, K_DGKS, K_SWEEP, K_SORT, K_APPEND, K_MIXED_T,
-              K_MIXED_P, K_BATCH, K_NCLASS };
+enum KClass { K_TGEMM = 0, K_PROJECT, K_FINALIZE, K_DGKS, K_SWEEP, K_SORT, K_APPEND, K_MIXED_A,
+              K_MIXED_B, K_RECHECK, K_ARGMAX, K_NCLASS };
 constexpr const char *kClassName[K_NCLASS] = {"tgemm_f64", "project_f64", "finalize", "dgks_retry",
-                                             "hinge_sweep", "sort", "append", "tgemm_mixed",
-                                             "project_mixed", "batch_step"};
+                                             "hinge_sweep", "sort", "append", "mixed_pass_a",
+                                             "mixed_pass_b", "mixed_recheck", "argmax"};
 
 struct KAcc {
     long launches = 0;
@@ -90,15 +146,12 @@ struct KRec {
     double flops, bytes;
 };
 
-struct BestOut {
-    double v1;
-    int c1;
-    int pad1;
-    double v2;
-    int c2;
-    int knot2;
-    int row2;
-    int pad2;
+/* Candidate list of one scoring call: candidates of all fits, grouped by fit,
+ * then by X column, then parent. */
+struct Cands {
+    std::vector<int> fit, xc, par;     // host
+    std::vector<int> fit_c0, fit_P;    // per fit: first candidate, count
+    DBuf<int> d_fit, d_xc, d_par;      // device copies
 };
 
 } // namespace mh
@@ -108,51 +161,35 @@ struct mars_hip_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
-    size_t n = 0;          // rows
-    size_t max_terms = 0;  // basis capacity
+    size_t max_terms = 0;  // basis capacity (all fits)
     size_t ldb = 0;        // row stride of B and Bo (>= max_terms, multiple of 4)
-    size_t px = 0;         // resident X columns
-    double tol = 0;
     int form = MARS_HIP_FORM_PLAIN;
-
-    // Resident fit state (row-major B / Bo, column-major raw X)
-    mh::DBuf<float> X, s, B, Bo, y;
-    mh::DBuf<double> ybo;
-    std::vector<double> ybo_h;
-    size_t m = 0;          // live basis count
-    double yvar = 0;
     bool resident = false;
+    std::vector<mh::FitState> fits;
 
-    // Mirror-mode bookkeeping
-    size_t m_synced = 0;
-    const float *y_key = nullptr;
-    std::vector<char> xres;  // resident flag per X column
-
-    // Hinge sort cache (per X column: stable descending order and f32 deltas)
-    mh::DBuf<int> order;
-    mh::DBuf<float> delta;
-    std::vector<char> sorted;
-
-    // Scoring workspace
+    // scoring workspace (shared by all fits)
     mh::Cands cand;
-    mh::DBuf<double> tpart, T, tnorm, spart, ssum, ybx, lin, hin;
-    mh::DBuf<float> rbuf;
-    mh::DBuf<int> fired_list, fired_count, knot, krow;
-    mh::DBuf<long long> dgks;       // device DGKS counter
-    mh::DBuf<double> red;           // reduction scratch
-    mh::DBuf<double> vcol;          // append: f64 residual column
-    mh::DBuf<double> scal;          // append / init scalars
+    mh::DBuf<mh::FitDev> d_fits;
+    mh::DBuf<mh::Item> d_items;
+    mh::DBuf<int> d_cmap;            // per-candidate first item / item count per pass
+    mh::DBuf<double> tpart, T, tnorm, spart, ssum, ybx, lin, hin, aux;
+    mh::DBuf<int> flag, fired_list, fired_count, knot, krow, ival;
+    mh::DBuf<long long> dgks;        // device DGKS counter
+    mh::DBuf<double> red;            // reduction scratch
+    mh::DBuf<double> vcol;           // append: f64 residual columns
+    mh::DBuf<double> scal;           // append / init scalars
     mh::DBuf<mh::AppendOut> aout;
+    mh::DBuf<mh::AppendArg> aarg;
     mh::DBuf<mh::BestOut> bout;
-    mh::DBuf<double> dense;         // (p_x, m) x 3 result arrays
-    mh::DBuf<int> hstate_i;
-    mh::DBuf<double> hstate;        // hinge chunk summaries / carries
+    mh::DBuf<double> dense;          // (p_x, m) x 3 result arrays
+    mh::DBuf<int> dknot;
+    mh::DBuf<double> hstate;         // hinge chunk summaries / carries
     mh::DBuf<double> hbest;
     mh::DBuf<int> hbest_i;
-    mh::DBuf<double> mixed;         // mixed-form scratch
 
     long dgks_host = 0;
     double eval_ms = 0;
+    long scored = 0, rescored = 0;  // mixed form: candidates scored / rescored in FP64
 
     // optional per-kernel timing
     bool timing = false;

@@ -1,5 +1,5 @@
-// Resident fit state on the device: constructor, append, step evaluation and
-// the C ABI of libmars_hip (include/mars_hip.h).
+// Resident fit state on the device: constructor, batched append, step
+// evaluation, mirror mode and the C ABI of libmars_hip (include/mars_hip.h).
 #include "internal.h"
 #include "kernels.h"
 
@@ -8,22 +8,26 @@
 #include <cstring>
 
 namespace mh {
-namespace {
 
 thread_local std::string g_last_error;
 
-/* ---------------------------------------------------------------------- */
-/* Constructor kernels (marsalgo.cc:266-370)                              */
-/* ---------------------------------------------------------------------- */
+namespace {
+
+/* per-fit scalar slots */
 enum { S_YN = 0, S_WN, S_YBO0, S_MEAN, S_YVAR, S_BADX, S_PROJ, S_PROJ2, S_VV, S_COUNT };
 
+constexpr int kInitBlocks = 256;
+
+/* ---------------------------------------------------------------------- */
+/* Constructor kernels (marsalgo.cc:266-370), one fit at a time            */
+/* ---------------------------------------------------------------------- */
 __global__ __launch_bounds__(256) void k_init1(const float *__restrict__ y, const float *__restrict__ w,
                                                size_t n, double *__restrict__ yd,
                                                double *__restrict__ sw, double *__restrict__ part)
 {
     __shared__ double sh[256];
     double ay = 0.0, aw = 0.0;
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)kRedBlocks * 256) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)kInitBlocks * 256) {
         double yi = (double)y[i];
         double si = sqrt((double)w[i]);
         if (!isfinite(yi)) yi = si = 0.0;
@@ -37,7 +41,7 @@ __global__ __launch_bounds__(256) void k_init1(const float *__restrict__ y, cons
     aw = block_sum_256(aw, sh);
     if (threadIdx.x == 0) {
         part[blockIdx.x] = ay;
-        part[kRedBlocks + blockIdx.x] = aw;
+        part[kInitBlocks + blockIdx.x] = aw;
     }
 }
 
@@ -48,15 +52,15 @@ __global__ __launch_bounds__(256) void k_init2(double *__restrict__ yd, const do
                                                double *__restrict__ part2, double *__restrict__ scal)
 {
     __shared__ double sh[256];
-    const double yn = sqrt(sum_partials(part, sh));
-    const double wn = sqrt(sum_partials(part + kRedBlocks, sh));
+    const double yn = sqrt(sum_partials(part, kInitBlocks, sh));
+    const double wn = sqrt(sum_partials(part + kInitBlocks, kInitBlocks, sh));
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         scal[S_YN] = yn;
         scal[S_WN] = wn;
     }
     double ab = 0.0, am = 0.0;
     if (yn > 0.0 && wn > 0.0) {
-        for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)kRedBlocks * 256) {
+        for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)kInitBlocks * 256) {
             const double d = yd[i] / yn;
             const double s = sw[i] / wn;
             yd[i] = d;
@@ -73,7 +77,7 @@ __global__ __launch_bounds__(256) void k_init2(double *__restrict__ yd, const do
     am = block_sum_256(am, sh);
     if (threadIdx.x == 0) {
         part2[blockIdx.x] = ab;
-        part2[kRedBlocks + blockIdx.x] = am;
+        part2[kInitBlocks + blockIdx.x] = am;
     }
 }
 
@@ -82,14 +86,14 @@ __global__ __launch_bounds__(256) void k_init3(const double *__restrict__ yd, si
                                                double *__restrict__ part3, double *__restrict__ scal)
 {
     __shared__ double sh[256];
-    const double ybo0 = sum_partials(part2, sh);
-    const double mean = sum_partials(part2 + kRedBlocks, sh) / (double)n;
+    const double ybo0 = sum_partials(part2, kInitBlocks, sh);
+    const double mean = sum_partials(part2 + kInitBlocks, kInitBlocks, sh) / (double)n;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         scal[S_YBO0] = ybo0;
         scal[S_MEAN] = mean;
     }
     double av = 0.0;
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)kRedBlocks * 256) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)kInitBlocks * 256) {
         const double dv = yd[i] - mean;
         av = fma(dv, dv, av);
     }
@@ -101,7 +105,7 @@ __global__ __launch_bounds__(256) void k_init4(const double *__restrict__ part3,
                                                double *__restrict__ scal, double *__restrict__ ybo)
 {
     __shared__ double sh[256];
-    const double v = sum_partials(part3, sh);
+    const double v = sum_partials(part3, kInitBlocks, sh);
     if (threadIdx.x == 0) {
         scal[S_YVAR] = v / (double)n;
         ybo[0] = scal[S_YBO0];
@@ -123,8 +127,8 @@ __global__ __launch_bounds__(256) void k_colsq(const float *__restrict__ X, size
     if (threadIdx.x == 0) part[(size_t)blockIdx.y * 64 + blockIdx.x] = a;
 }
 
-__global__ void k_colscale(const double *__restrict__ part, int px, size_t n,
-                           float *__restrict__ s, double *__restrict__ scal)
+__global__ void k_colscale(const double *__restrict__ part, int px, size_t n, float *__restrict__ s,
+                           double *__restrict__ scal)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= px) return;
@@ -136,149 +140,175 @@ __global__ void k_colscale(const double *__restrict__ part, int px, size_t n,
 }
 
 /* ---------------------------------------------------------------------- */
-/* Append kernels (marsalgo.cc:760-875, kernels.cc:239-281)               */
+/* Append kernels (marsalgo.cc:760-875, kernels.cc:239-281), batched: grid */
+/* (nb, fits); per-fit partials at part[f * nb + b], scalars at scal[f*S]   */
 /* ---------------------------------------------------------------------- */
 __device__ inline float relu0(float a) { return a < 0.0f ? 0.0f : a; }
 
-__global__ __launch_bounds__(256) void k_app_build(const float *__restrict__ X, size_t ldx,
-                                                   const float *__restrict__ sc,
-                                                   const float *__restrict__ B, size_t ldb,
-                                                   int xcol, int bcol, int type, float h, size_t n,
-                                                   double *__restrict__ v, double *__restrict__ part,
-                                                   double *__restrict__ scal)
+__global__ __launch_bounds__(256) void k_app_build(const FitDev *__restrict__ fits,
+                                                   const AppendArg *__restrict__ args, int nb,
+                                                   double *__restrict__ part, double *__restrict__ scal)
 {
     __shared__ double sh[256];
-    const float s = sc[xcol];
-    const float *xp = X + (size_t)xcol * ldx;
+    const int f = blockIdx.y;
+    const AppendArg A = args[f];
+    if (!A.active) return;
+    const FitDev F = fits[f];
+    const float s = F.s[A.xcol];
+    const float *xp = F.X + (size_t)A.xcol * F.ldx;
     double a = 0.0;
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)kRedBlocks * 256) {
-        const float sb = __fmul_rn(s, B[i * ldb + bcol]);
-        float f;
-        if (type == 0)
-            f = __fmul_rn(sb, xp[i]);
-        else if (type == 1)
-            f = __fmul_rn(sb, relu0(__fsub_rn(xp[i], h)));
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < F.n; i += (long long)nb * 256) {
+        const float sb = __fmul_rn(s, F.B[i * F.ldb + A.bcol]);
+        float v;
+        if (A.type == 0)
+            v = __fmul_rn(sb, xp[i]);
+        else if (A.type == 1)
+            v = __fmul_rn(sb, relu0(__fsub_rn(xp[i], A.h)));
         else
-            f = __fmul_rn(sb, relu0(__fsub_rn(h, xp[i])));
-        const double d = (double)f;
-        v[i] = d;
+            v = __fmul_rn(sb, relu0(__fsub_rn(A.h, xp[i])));
+        const double d = (double)v;
+        F.v[i] = d;
         a = fma(d, d, a);
     }
     a = block_sum_256(a, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = a;
+    if (threadIdx.x == 0) part[(size_t)f * nb + blockIdx.x] = a;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        scal[S_PROJ] = 0.0;
-        scal[S_PROJ2] = 0.0;
+        scal[f * S_COUNT + S_PROJ] = 0.0;
+        scal[f * S_COUNT + S_PROJ2] = 0.0;
     }
 }
 
 /*
  * One modified Gram-Schmidt step j of the residual v against Bo's column j:
  * apply the update of column j-1 (coefficient = sum of the previous step's
- * partials) and accumulate the partials of column j's coefficient; the final
- * step (j == m) accumulates |v|^2 instead.  On the first sweep, step 0 also
- * stores the raw-norm-normalized B column.  `gate` (retry sweep) makes the
- * launch a no-op unless the DGKS gate fired.
+ * partials) and accumulate the partials of column j's coefficient; step
+ * j == m accumulates |v|^2 instead.  The first sweep's step 0 also stores the
+ * raw-norm-normalized B column (marsalgo.cc:829-836).  `gate` (retry sweep)
+ * makes the launch a no-op for fits whose DGKS gate did not fire.
  */
-__global__ __launch_bounds__(256) void k_app_mgs(const float *__restrict__ Bo, float *__restrict__ B,
-                                                 size_t ldb, int mcol, size_t n,
-                                                 double *__restrict__ v,
+__global__ __launch_bounds__(256) void k_app_mgs(const FitDev *__restrict__ fits,
+                                                 const AppendArg *__restrict__ args, int nb,
                                                  const double *__restrict__ prev,
-                                                 double *__restrict__ next, int j, int m,
-                                                 int first, const int *__restrict__ gate,
-                                                 double *__restrict__ scal)
+                                                 double *__restrict__ next, int j, int first,
+                                                 const int *__restrict__ gate,
+                                                 double *__restrict__ scal, float *const *__restrict__ Bw)
 {
     __shared__ double sh[256];
-    if (gate && !*gate) return;
+    const int f = blockIdx.y;
+    if (!args[f].active) return;
+    const FitDev F = fits[f];
+    const int m = F.m;
+    if (j > m) return;
+    if (gate && !gate[f]) return;
     double c = 0.0;
     float rn = 1.0f;
-    if (j > 0) c = sum_partials(prev, sh);
-    if (j == 0 && first) rn = (float)sqrt(sum_partials(prev, sh));
+    if (j > 0) c = sum_partials(prev + (size_t)f * nb, nb, sh);
+    if (j == 0 && first) rn = (float)sqrt(sum_partials(prev + (size_t)f * nb, nb, sh));
+    float *B = Bw[f];
     double a = 0.0;
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)kRedBlocks * 256) {
-        double vi = v[i];
-        if (j == 0 && first) B[i * ldb + mcol] = __fdiv_rn((float)vi, rn);
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < F.n; i += (long long)nb * 256) {
+        double vi = F.v[i];
+        if (j == 0 && first) B[i * F.ldb + m] = __fdiv_rn((float)vi, rn);
         if (j > 0) {
-            vi = fma(-c, (double)Bo[i * ldb + (j - 1)], vi);
-            v[i] = vi;
+            vi = fma(-c, (double)F.Bo[i * F.ldb + (j - 1)], vi);
+            F.v[i] = vi;
         }
         if (j < m)
-            a = fma((double)Bo[i * ldb + j], vi, a);
+            a = fma((double)F.Bo[i * F.ldb + j], vi, a);
         else
             a = fma(vi, vi, a);
     }
     a = block_sum_256(a, sh);
-    if (threadIdx.x == 0) next[blockIdx.x] = a;
+    if (threadIdx.x == 0) next[(size_t)f * nb + blockIdx.x] = a;
     if (j > 0 && blockIdx.x == 0 && threadIdx.x == 0) {
-        double *acc = first ? &scal[S_PROJ] : &scal[S_PROJ2];
+        double *acc = &scal[f * S_COUNT + (first ? S_PROJ : S_PROJ2)];
         *acc = fma(c, c, *acc);
     }
 }
 
-__global__ __launch_bounds__(256) void k_app_gate(const double *__restrict__ part, double tol,
+/* DGKS gate of the append (kernels.cc:376-385); the second call records |v|^2 after the retry. */
+__global__ __launch_bounds__(256) void k_app_gate(const FitDev *__restrict__ fits,
+                                                  const AppendArg *__restrict__ args, int nb,
+                                                  const double *__restrict__ pa,
+                                                  const double *__restrict__ pb,
                                                   int *__restrict__ gate, double *__restrict__ scal,
                                                   long long *__restrict__ dgks, int retry_done)
 {
     __shared__ double sh[256];
-    const double vv = sum_partials(part, sh);
-    if (threadIdx.x == 0) {
-        if (!retry_done) {
-            const int fire = (vv > tol && vv * kGate < scal[S_PROJ]) ? 1 : 0;
-            gate[0] = fire;
-            if (fire) *dgks += 1;
-            scal[S_VV] = vv;
-        } else if (gate[0]) {
-            scal[S_VV] = vv;
-        }
+    const int f = blockIdx.x;
+    if (!args[f].active) return;
+    const FitDev F = fits[f];
+    const double *part = ((F.m + 1) % 2 == 0 ? pa : pb) + (size_t)f * nb;
+    const double vv = sum_partials(part, nb, sh);
+    if (threadIdx.x != 0) return;
+    double *sc = scal + f * S_COUNT;
+    if (!retry_done) {
+        const int fire = (vv > F.tol && vv * kGate < sc[S_PROJ]) ? 1 : 0;
+        gate[f] = fire;
+        if (fire) atomicAdd((unsigned long long *)dgks, 1ull);
+        sc[S_VV] = vv;
+    } else if (gate[f]) {
+        sc[S_VV] = vv;
     }
 }
 
-__global__ __launch_bounds__(256) void k_app_store(float *__restrict__ Bo, size_t ldb, int mcol,
-                                                   size_t n, const double *__restrict__ v,
-                                                   const float *__restrict__ y, double tol,
+__global__ __launch_bounds__(256) void k_app_store(const FitDev *__restrict__ fits,
+                                                   const AppendArg *__restrict__ args, int nb,
+                                                   float *const *__restrict__ Bow,
                                                    const double *__restrict__ scal,
                                                    double *__restrict__ part)
 {
     __shared__ double sh[256];
-    const double w = sqrt(scal[S_VV]);
-    const bool ok = w * w > tol;
+    const int f = blockIdx.y;
+    if (!args[f].active) return;
+    const FitDev F = fits[f];
+    const double w = sqrt(scal[f * S_COUNT + S_VV]);
+    const bool ok = w * w > F.tol;
+    float *Bo = Bow[f];
     double a = 0.0;
     if (ok) {
-        for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)kRedBlocks * 256) {
-            const float b = (float)(v[i] / w);
-            Bo[i * ldb + mcol] = b;
-            a = fma((double)b, (double)y[i], a);
+        for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < F.n; i += (long long)nb * 256) {
+            const float b = (float)(F.v[i] / w);
+            Bo[i * F.ldb + F.m] = b;
+            a = fma((double)b, (double)F.y[i], a);
         }
     }
     a = block_sum_256(a, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = a;
+    if (threadIdx.x == 0) part[(size_t)f * nb + blockIdx.x] = a;
 }
 
-__global__ __launch_bounds__(256) void k_app_final(const double *__restrict__ part, int m, size_t n,
-                                                   double tol, const double *__restrict__ scal,
-                                                   double *__restrict__ ybo, const int *__restrict__ gate,
+__global__ __launch_bounds__(256) void k_app_final(const FitDev *__restrict__ fits,
+                                                   const AppendArg *__restrict__ args, int nb,
+                                                   const double *__restrict__ part,
+                                                   const double *__restrict__ scal,
+                                                   double *const *__restrict__ ybow,
+                                                   const int *__restrict__ gate,
                                                    AppendOut *__restrict__ out)
 {
     __shared__ double sh[256];
-    const double yb = sum_partials(part, sh);
+    const int f = blockIdx.x;
+    if (!args[f].active) return;
+    const FitDev F = fits[f];
+    const double yb = sum_partials(part + (size_t)f * nb, nb, sh);
     if (threadIdx.x != 0) return;
-    const double w = sqrt(scal[S_VV]);
+    const double w = sqrt(scal[f * S_COUNT + S_VV]);
+    double *ybo = ybow[f];
     AppendOut o;
-    o.fired = gate[0];
+    o.fired = gate[f];
     o.ybo_m = yb;
     o.status = 0;
     o.mse = -1.0;
-    if (w * w > tol) {
+    if (w * w > F.tol) {
         double sq = yb * yb;
-        for (int k = 0; k < m; ++k) sq = fma(ybo[k], ybo[k], sq);
-        const double mse = (1.0 - sq) / (double)n;
-        if (mse >= -tol) {
+        for (int k = 0; k < F.m; ++k) sq = fma(ybo[k], ybo[k], sq);
+        const double mse = (1.0 - sq) / (double)F.n;
+        if (mse >= -F.tol) {
             o.status = 1;
             o.mse = mse > 0.0 ? mse : 0.0;
-            ybo[m] = yb;
+            ybo[F.m] = yb;
         }
     }
-    *out = o;
+    out[f] = o;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -313,23 +343,27 @@ __global__ void k_scatter(const int *__restrict__ cxc, const int *__restrict__ c
     }
 }
 
-/* First maximum in row-major order, like numpy.argmax (values are >= 0). */
+/* First maximum in row-major order, like numpy.argmax over values >= 0.
+ * One block per fit over its contiguous range [off[f], off[f] + len[f]);
+ * `hin` may be null.  Index 0 is returned when every value is 0. */
 __global__ __launch_bounds__(1024) void k_argmax(const double *__restrict__ lin,
                                                  const double *__restrict__ hin,
                                                  const int *__restrict__ knot,
-                                                 const double *__restrict__ cut, size_t total,
+                                                 const int *__restrict__ off,
+                                                 const int *__restrict__ len,
                                                  BestOut *__restrict__ out)
 {
     __shared__ double sv[2][1024];
-    __shared__ long long si[2][1024];
-    const int t = threadIdx.x;
-    const size_t chunk = (total + 1023) / 1024;
-    const size_t b = t * chunk, e = b + chunk < total ? b + chunk : total;
+    __shared__ int si[2][1024];
+    const int f = blockIdx.x, t = threadIdx.x;
+    const int base = off[f], total = len[f];
+    const int chunk = (total + 1023) / 1024;
+    const int b = t * chunk, e = min(total, b + chunk);
     double v1 = -1.0, v2 = -1.0;
-    long long i1 = -1, i2 = -1;
-    for (size_t i = b; i < e; ++i) {
-        if (lin[i] > v1) { v1 = lin[i]; i1 = (long long)i; }
-        if (hin[i] > v2) { v2 = hin[i]; i2 = (long long)i; }
+    int i1 = -1, i2 = -1;
+    for (int i = b; i < e; ++i) {
+        if (lin[base + i] > v1) { v1 = lin[base + i]; i1 = i; }
+        if (hin && hin[base + i] > v2) { v2 = hin[base + i]; i2 = i; }
     }
     sv[0][t] = v1; si[0][t] = i1;
     sv[1][t] = v2; si[1][t] = i2;
@@ -338,8 +372,7 @@ __global__ __launch_bounds__(1024) void k_argmax(const double *__restrict__ lin,
         if (t < s) {
             for (int q = 0; q < 2; ++q) {
                 const double a = sv[q][t], bb = sv[q][t + s];
-                const long long ia = si[q][t], ib = si[q][t + s];
-                // larger value wins; on equal values the earlier index wins
+                const int ia = si[q][t], ib = si[q][t + s];
                 if (bb > a || (bb == a && ib >= 0 && (ia < 0 || ib < ia))) {
                     sv[q][t] = bb;
                     si[q][t] = ib;
@@ -351,60 +384,83 @@ __global__ __launch_bounds__(1024) void k_argmax(const double *__restrict__ lin,
     if (t == 0) {
         BestOut o;
         o.v1 = sv[0][0] > 0.0 ? sv[0][0] : 0.0;
-        o.c1 = sv[0][0] > 0.0 ? (int)si[0][0] : 0;
+        o.c1 = sv[0][0] > 0.0 ? si[0][0] : 0;
         o.v2 = sv[1][0] > 0.0 ? sv[1][0] : 0.0;
-        o.c2 = sv[1][0] > 0.0 ? (int)si[1][0] : 0;
-        o.knot2 = total ? knot[o.c2] : -1;
-        o.row2 = 0;
-        o.pad1 = o.pad2 = 0;
-        *out = o;
+        o.c2 = sv[1][0] > 0.0 ? si[1][0] : 0;
+        o.knot2 = (knot && total) ? knot[base + o.c2] : -1;
+        o.pad1 = 0;
+        out[f] = o;
     }
 }
+
+__global__ void k_put_bcol(float *__restrict__ dst, size_t ldb, int col, const float *__restrict__ src,
+                           size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i * ldb + col] = src[i];
+}
+
+__global__ void k_gather_cols(const float *__restrict__ rbuf, size_t ldr, size_t n,
+                              float *__restrict__ out, size_t ldo)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < n) out[(size_t)j * ldo + i] = rbuf[i * ldr + j];
+}
+
+} // namespace
 
 /* ---------------------------------------------------------------------- */
 /* Host side                                                              */
 /* ---------------------------------------------------------------------- */
-void ctx_base_init(mars_hip_ctx *ctx, size_t n, size_t max_terms, double tol)
+void ctx_base_init(mars_hip_ctx *ctx, size_t max_terms)
 {
-    require(n > 0, "empty dataset");
     require(max_terms >= 1, "max_terms must be >= 1");
     MH_CHECK(hipGetDevice(&ctx->device));
     MH_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     MH_CHECK(hipEventCreate(&ctx->ev0));
     MH_CHECK(hipEventCreate(&ctx->ev1));
-    ctx->n = n;
     ctx->max_terms = max_terms;
     ctx->ldb = (max_terms + 3) / 4 * 4;
-    ctx->tol = tol;
-    ctx->B.ensure(n * ctx->ldb);
-    ctx->Bo.ensure(n * ctx->ldb);
-    MH_CHECK(hipMemsetAsync(ctx->B.p, 0, n * ctx->ldb * sizeof(float), ctx->stream));
-    MH_CHECK(hipMemsetAsync(ctx->Bo.p, 0, n * ctx->ldb * sizeof(float), ctx->stream));
-    ctx->y.ensure(n);
-    ctx->ybo.ensure(max_terms);
-    ctx->scal.ensure(S_COUNT);
-    ctx->red.ensure(4 * kRedBlocks);
     ctx->dgks.ensure(1);
     ctx->fired_count.ensure(2);
     MH_CHECK(hipMemsetAsync(ctx->dgks.p, 0, sizeof(long long), ctx->stream));
-    MH_CHECK(hipMemsetAsync(ctx->scal.p, 0, S_COUNT * sizeof(double), ctx->stream));
+}
+
+void fit_state_alloc(mars_hip_ctx *ctx, FitState &fs, size_t n, double tol)
+{
+    require(n > 0, "empty dataset");
+    fs.n = n;
+    fs.tol = tol;
+    fs.B.ensure(n * ctx->ldb);
+    fs.Bo.ensure(n * ctx->ldb);
+    MH_CHECK(hipMemsetAsync(fs.B.p, 0, n * ctx->ldb * sizeof(float), ctx->stream));
+    MH_CHECK(hipMemsetAsync(fs.Bo.p, 0, n * ctx->ldb * sizeof(float), ctx->stream));
+    fs.ldx = (n + 31) / 32 * 32;
+    fs.y.ensure(fs.ldx + 32);
+    MH_CHECK(hipMemsetAsync(fs.y.p, 0, (fs.ldx + 32) * sizeof(float), ctx->stream));
+    fs.ybo.ensure(ctx->max_terms);
+    fs.v.ensure(n);
 }
 
 void ctx_free(mars_hip_ctx *ctx)
 {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    DBuf<float> *fb[] = {&ctx->X, &ctx->s, &ctx->B, &ctx->Bo, &ctx->y, &ctx->delta, &ctx->rbuf};
-    for (auto *b : fb) b->release();
-    DBuf<double> *db[] = {&ctx->ybo, &ctx->tpart, &ctx->T, &ctx->tnorm, &ctx->spart, &ctx->ssum,
-                          &ctx->ybx, &ctx->lin, &ctx->hin, &ctx->red, &ctx->vcol, &ctx->scal,
-                          &ctx->dense, &ctx->hstate, &ctx->hbest, &ctx->mixed};
+    for (auto &fs : ctx->fits) fs.release();
+    DBuf<double> *db[] = {&ctx->tpart, &ctx->T, &ctx->tnorm, &ctx->spart, &ctx->ssum, &ctx->ybx,
+                          &ctx->lin, &ctx->hin, &ctx->aux, &ctx->red, &ctx->vcol, &ctx->scal,
+                          &ctx->dense, &ctx->hstate, &ctx->hbest};
     for (auto *b : db) b->release();
-    DBuf<int> *ib[] = {&ctx->order, &ctx->fired_list, &ctx->fired_count, &ctx->knot, &ctx->krow,
-                       &ctx->hstate_i, &ctx->hbest_i, &ctx->cand.xc, &ctx->cand.par};
+    DBuf<int> *ib[] = {&ctx->flag, &ctx->fired_list, &ctx->fired_count, &ctx->knot, &ctx->krow,
+                       &ctx->ival, &ctx->d_cmap, &ctx->dknot, &ctx->hbest_i, &ctx->cand.d_fit,
+                       &ctx->cand.d_xc, &ctx->cand.d_par};
     for (auto *b : ib) b->release();
+    ctx->d_fits.release();
+    ctx->d_items.release();
     ctx->dgks.release();
     ctx->aout.release();
+    ctx->aarg.release();
     ctx->bout.release();
     for (auto &r : ctx->pend) {
         (void)hipEventDestroy(r.a);
@@ -417,191 +473,256 @@ void ctx_free(mars_hip_ctx *ctx)
     delete ctx;
 }
 
-void fit_create(mars_hip_ctx *ctx, const float *X, size_t ldX, const float *y, const float *w,
-                size_t n, size_t px, size_t max_terms)
+/* MarsAlgo constructor on the device for one fit (marsalgo.cc:266-370). */
+void fit_init(mars_hip_ctx *ctx, FitState &fs, const float *X, size_t ldX, const float *y, const float *w,
+              size_t n, size_t px)
 {
     require(X && y && w, "null input array");
-    require(ldX >= n, "invalid leading dimension for X");
+    require(ldX >= n || px <= 1, "invalid leading dimension for X");
     require(px >= 1, "X has no columns");
-    ctx_base_init(ctx, n, max_terms, (n * 0.02) * 2.220446049250313e-16);
-    ctx->px = px;
-    ctx->resident = true;
-    ctx->X.ensure(n * px);
-    ctx->s.ensure(px);
+    fit_state_alloc(ctx, fs, n, (n * 0.02) * 2.220446049250313e-16);
+    fs.px = px;
+    fs.X.ensure(fs.ldx * px + 32);
+    fs.s.ensure(px);
     hipStream_t st = ctx->stream;
-    MH_CHECK(hipMemcpy2DAsync(ctx->X.p, n * sizeof(float), X, ldX * sizeof(float), n * sizeof(float),
+    MH_CHECK(hipMemsetAsync(fs.X.p, 0, (fs.ldx * px + 32) * sizeof(float), st));
+    MH_CHECK(hipMemcpy2DAsync(fs.X.p, fs.ldx * sizeof(float), X, std::max(ldX, n) * sizeof(float), n * sizeof(float),
                               px, hipMemcpyHostToDevice, st));
     DBuf<float> yw;
     yw.ensure(2 * n);
     MH_CHECK(hipMemcpyAsync(yw.p, y, n * sizeof(float), hipMemcpyHostToDevice, st));
     MH_CHECK(hipMemcpyAsync(yw.p + n, w, n * sizeof(float), hipMemcpyHostToDevice, st));
-    DBuf<double> tmp;
+    DBuf<double> tmp, part, scal, cpart;
     tmp.ensure(2 * n);
-    double *yd = tmp.p, *sw = tmp.p + n;
-    double *part = ctx->red.p, *part2 = ctx->red.p + 2 * kRedBlocks;
-    hipLaunchKernelGGL(k_init1, dim3(kRedBlocks), dim3(256), 0, st, yw.p, yw.p + n, n, yd, sw, part);
-    MH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_init2, dim3(kRedBlocks), dim3(256), 0, st, yd, sw, n, part, ctx->y.p,
-                       ctx->B.p, ctx->Bo.p, ctx->ldb, part2, ctx->scal.p);
-    MH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_init3, dim3(kRedBlocks), dim3(256), 0, st, yd, n, part2, part, ctx->scal.p);
-    MH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_init4, dim3(1), dim3(256), 0, st, part, n, ctx->scal.p, ctx->ybo.p);
-    MH_LAUNCH_CHECK();
-    DBuf<double> cpart;
+    part.ensure(4 * kInitBlocks);
+    scal.ensure(S_COUNT);
     cpart.ensure(px * 64);
-    hipLaunchKernelGGL(k_colsq, dim3(64, px), dim3(256), 0, st, ctx->X.p, n, n, cpart.p);
+    MH_CHECK(hipMemsetAsync(scal.p, 0, S_COUNT * sizeof(double), st));
+    double *yd = tmp.p, *sw = tmp.p + n, *p1 = part.p, *p2 = part.p + 2 * kInitBlocks;
+    hipLaunchKernelGGL(k_init1, dim3(kInitBlocks), dim3(256), 0, st, yw.p, yw.p + n, n, yd, sw, p1);
     MH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_colscale, dim3((px + 255) / 256), dim3(256), 0, st, cpart.p, (int)px, n,
-                       ctx->s.p, ctx->scal.p);
+    hipLaunchKernelGGL(k_init2, dim3(kInitBlocks), dim3(256), 0, st, yd, sw, n, p1, fs.y.p, fs.B.p, fs.Bo.p,
+                       ctx->ldb, p2, scal.p);
+    MH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_init3, dim3(kInitBlocks), dim3(256), 0, st, yd, n, p2, p1, scal.p);
+    MH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_init4, dim3(1), dim3(256), 0, st, p1, n, scal.p, fs.ybo.p);
+    MH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_colsq, dim3(64, px), dim3(256), 0, st, fs.X.p, fs.ldx, n, cpart.p);
+    MH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_colscale, dim3((px + 255) / 256), dim3(256), 0, st, cpart.p, (int)px, n, fs.s.p,
+                       scal.p);
     MH_LAUNCH_CHECK();
     double sc[S_COUNT];
-    MH_CHECK(hipMemcpyAsync(sc, ctx->scal.p, sizeof(sc), hipMemcpyDeviceToHost, st));
+    MH_CHECK(hipMemcpyAsync(sc, scal.p, sizeof(sc), hipMemcpyDeviceToHost, st));
     MH_CHECK(hipStreamSynchronize(st));
     yw.release();
     tmp.release();
+    part.release();
+    scal.release();
     cpart.release();
     require(sc[S_YN] > 0.0 && sc[S_WN] > 0.0, "target Y is all zero or NANs");
     require(sc[S_BADX] == 0.0, "not all columns in X are finite");
-    ctx->yvar = sc[S_YVAR];
-    ctx->ybo_h.assign(1, sc[S_YBO0]);
-    ctx->m = 1;
-    ctx->sorted.assign(px, 0);
+    fs.yvar = sc[S_YVAR];
+    fs.ybo_h.assign(1, sc[S_YBO0]);
+    fs.m = 1;
+    fs.sorted.assign(px, 0);
 }
 
-void fit_append(mars_hip_ctx *ctx, char type, int xcol, int bcol, float h, double *mse_out)
+/*
+ * Append one basis function to every fit with args[f].active (device
+ * counterpart of MarsAlgo::append, marsalgo.cc:760-875).  Validation happens
+ * on the host with the reference's messages; out[f] receives the result.
+ */
+void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std::vector<AppendOut> &out)
 {
     require(ctx->resident, "context was not created by mars_hip_fit_create");
-    if (ctx->m >= ctx->max_terms) throw Error(MARS_HIP_EFULL, "basis matrix is full");
-    if (bcol < 0 || (size_t)bcol >= ctx->m) {
-        char msg[80];
-        snprintf(msg, sizeof(msg), "invalid basis column number: %d", bcol);
-        throw Error(MARS_HIP_EINVAL, msg);
+    const size_t F = ctx->fits.size();
+    require(args.size() == F, "one append argument per fit");
+    size_t mmax = 0, nmax = 0;
+    for (size_t f = 0; f < F; ++f) {
+        if (!args[f].active) continue;
+        const FitState &fs = ctx->fits[f];
+        if (fs.m >= ctx->max_terms) throw Error(MARS_HIP_EFULL, "basis matrix is full");
+        if (args[f].bcol < 0 || (size_t)args[f].bcol >= fs.m) {
+            char msg[80];
+            snprintf(msg, sizeof(msg), "invalid basis column number: %d", args[f].bcol);
+            throw Error(MARS_HIP_EINVAL, msg);
+        }
+        require(args[f].xcol >= 0 && (size_t)args[f].xcol < fs.px, "invalid X column index");
+        require(args[f].type >= 0 && args[f].type <= 2, "invalid basis type");
+        mmax = std::max(mmax, fs.m);
+        nmax = std::max(nmax, fs.n);
     }
-    require(xcol >= 0 && (size_t)xcol < ctx->px, "invalid X column index");
-    int t;
-    if (type == 'l') t = 0;
-    else if (type == '+') t = 1;
-    else if (type == '-') t = 2;
-    else throw Error(MARS_HIP_EINVAL, "invalid basis type");
-
-    const size_t n = ctx->n, m = ctx->m;
+    out.assign(F, AppendOut{-1.0, 0.0, 0, 0});
+    if (mmax == 0) return;
     hipStream_t st = ctx->stream;
-    ctx->vcol.ensure(n);
-    ctx->aout.ensure(1);
-    ctx->knot.ensure(std::max<size_t>(ctx->knot.cap, 1));
-    double *pa = ctx->red.p, *pb = ctx->red.p + kRedBlocks, *pc = ctx->red.p + 2 * kRedBlocks;
+    const int nb = (int)std::min<size_t>(256, std::max<size_t>(1, (nmax + 4095) / 4096));
+    Cands none;
+    none.fit_c0.assign(F, 0);
+    upload_fits(ctx, none);
+    ctx->aarg.ensure(F);
+    ctx->aout.ensure(F);
+    ctx->red.ensure(3 * F * nb);
+    ctx->scal.ensure(F * S_COUNT);
+    ctx->flag.ensure(F);
+    MH_CHECK(hipMemcpyAsync(ctx->aarg.p, args.data(), F * sizeof(AppendArg), hipMemcpyHostToDevice, st));
+    std::vector<float *> bw(F), bow(F);
+    std::vector<double *> yw(F);
+    for (size_t f = 0; f < F; ++f) {
+        bw[f] = ctx->fits[f].B.p;
+        bow[f] = ctx->fits[f].Bo.p;
+        yw[f] = ctx->fits[f].ybo.p;
+    }
+    DBuf<float *> dbw, dbow;
+    DBuf<double *> dyw;
+    dbw.ensure(F);
+    dbow.ensure(F);
+    dyw.ensure(F);
+    MH_CHECK(hipMemcpyAsync(dbw.p, bw.data(), F * sizeof(float *), hipMemcpyHostToDevice, st));
+    MH_CHECK(hipMemcpyAsync(dbow.p, bow.data(), F * sizeof(float *), hipMemcpyHostToDevice, st));
+    MH_CHECK(hipMemcpyAsync(dyw.p, yw.data(), F * sizeof(double *), hipMemcpyHostToDevice, st));
+    double *pa = ctx->red.p, *pb = pa + F * nb, *pc = pb + F * nb;
     double *pp[2] = {pa, pb};
-    int *gate = ctx->fired_count.p;  // slot 0: DGKS gate of this append
-
+    int *gate = ctx->flag.p;
+    double nrows = 0, mm = 0;
+    for (size_t f = 0; f < F; ++f)
+        if (args[f].active) {
+            nrows += (double)ctx->fits[f].n;
+            mm = std::max(mm, (double)ctx->fits[f].m);
+        }
     {
-        KTimer kt(ctx, K_APPEND, 0.0, 8.0 * n * (3.0 * (m + 1) + 4));
-        hipLaunchKernelGGL(k_app_build, dim3(kRedBlocks), dim3(256), 0, st, ctx->X.p, n, ctx->s.p,
-                           ctx->B.p, ctx->ldb, xcol, bcol, t, h, n, ctx->vcol.p, pa, ctx->scal.p);
+        KTimer kt(ctx, K_APPEND, 0.0, 8.0 * nrows * (3.0 * (mm + 1) + 4));
+        const dim3 grid(nb, F);
+        hipLaunchKernelGGL(k_app_build, grid, dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, pa, ctx->scal.p);
         MH_LAUNCH_CHECK();
         for (int pass = 0; pass < 2; ++pass) {
-            for (size_t j = 0; j <= m; ++j) {
-                hipLaunchKernelGGL(k_app_mgs, dim3(kRedBlocks), dim3(256), 0, st, ctx->Bo.p, ctx->B.p,
-                                   ctx->ldb, (int)m, n, ctx->vcol.p, pp[j % 2], pp[(j + 1) % 2], (int)j,
-                                   (int)m, pass == 0 ? 1 : 0, pass == 0 ? nullptr : gate, ctx->scal.p);
+            for (size_t j = 0; j <= mmax; ++j) {
+                hipLaunchKernelGGL(k_app_mgs, grid, dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, pp[j % 2],
+                                   pp[(j + 1) % 2], (int)j, pass == 0 ? 1 : 0, pass == 0 ? nullptr : gate,
+                                   ctx->scal.p, dbw.p);
                 MH_LAUNCH_CHECK();
             }
-            hipLaunchKernelGGL(k_app_gate, dim3(1), dim3(256), 0, st, pp[(m + 1) % 2], ctx->tol, gate,
-                               ctx->scal.p, ctx->dgks.p, pass);
+            hipLaunchKernelGGL(k_app_gate, dim3(F), dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, pa, pb,
+                               gate, ctx->scal.p, ctx->dgks.p, pass);
             MH_LAUNCH_CHECK();
         }
-        hipLaunchKernelGGL(k_app_store, dim3(kRedBlocks), dim3(256), 0, st, ctx->Bo.p, ctx->ldb,
-                           (int)m, n, ctx->vcol.p, ctx->y.p, ctx->tol, ctx->scal.p, pc);
+        hipLaunchKernelGGL(k_app_store, grid, dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, dbow.p,
+                           ctx->scal.p, pc);
         MH_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_app_final, dim3(1), dim3(256), 0, st, pc, (int)m, n, ctx->tol,
-                           ctx->scal.p, ctx->ybo.p, gate, ctx->aout.p);
+        hipLaunchKernelGGL(k_app_final, dim3(F), dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, pc,
+                           ctx->scal.p, dyw.p, gate, ctx->aout.p);
         MH_LAUNCH_CHECK();
     }
-    AppendOut o;
-    MH_CHECK(hipMemcpyAsync(&o, ctx->aout.p, sizeof(o), hipMemcpyDeviceToHost, st));
+    MH_CHECK(hipMemcpyAsync(out.data(), ctx->aout.p, F * sizeof(AppendOut), hipMemcpyDeviceToHost, st));
     MH_CHECK(hipStreamSynchronize(st));
+    dbw.release();
+    dbow.release();
+    dyw.release();
     resolve_timing(ctx);
-    if (o.fired) ctx->dgks_host += 1;
-    if (o.status == 1) {
-        ctx->ybo_h.push_back(o.ybo_m);
-        ctx->m += 1;
-        *mse_out = o.mse;
-    } else {
-        *mse_out = -1.0;
+    for (size_t f = 0; f < F; ++f) {
+        if (!args[f].active) continue;
+        if (out[f].fired) ctx->dgks_host += 1;
+        if (out[f].status == 1) {
+            ctx->fits[f].ybo_h.push_back(out[f].ybo_m);
+            ctx->fits[f].m += 1;
+        } else {
+            out[f].mse = -1.0;
+        }
     }
 }
 
-double host_dsse(const mars_hip_ctx *ctx)
+double host_dsse(const FitState &fs)
 {
     double s = 0.0;
-    for (double v : ctx->ybo_h) s = std::fma(v, v, s);
+    for (double v : fs.ybo_h) s = std::fma(v, v, s);
     return s;
 }
 
-void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t stride,
-              int min_span, int end_span, int linear_only, double *lin_out, double *hin_out,
-              double *cut_out, mars_hip_best *best)
+/* Per-fit best linear candidate (first maximum in row-major (xcol, bcol) order). */
+void argmax_fits(mars_hip_ctx *ctx, const Cands &cs, const double *lin, std::vector<BestOut> &best)
+{
+    const size_t F = ctx->fits.size();
+    ctx->ival.ensure(2 * F);
+    ctx->bout.ensure(F);
+    std::vector<int> ol(2 * F);
+    for (size_t f = 0; f < F; ++f) {
+        ol[f] = cs.fit_c0[f];
+        ol[F + f] = cs.fit_P[f];
+    }
+    MH_CHECK(hipMemcpyAsync(ctx->ival.p, ol.data(), 2 * F * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    {
+        KTimer kt(ctx, K_ARGMAX, 0.0, 8.0 * cs.fit.size());
+        hipLaunchKernelGGL(k_argmax, dim3(F), dim3(1024), 0, ctx->stream, lin, nullptr, nullptr, ctx->ival.p,
+                           ctx->ival.p + F, ctx->bout.p);
+        MH_LAUNCH_CHECK();
+    }
+    best.resize(F);
+    MH_CHECK(hipMemcpyAsync(best.data(), ctx->bout.p, F * sizeof(BestOut), hipMemcpyDeviceToHost, ctx->stream));
+    MH_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
+void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t stride, int min_span,
+              int end_span, int linear_only, double *lin_out, double *hin_out, double *cut_out,
+              mars_hip_best *best)
 {
     require(ctx->resident, "context was not created by mars_hip_fit_create");
+    require(ctx->fits.size() == 1, "use mars_hip_batch_fit on batch contexts");
+    FitState &fs = ctx->fits[0];
     require(mask != nullptr, "null mask");
-    require(rows == ctx->px, "invalid dimension for mask");
-    require(stride >= ctx->m, "invalid mask row stride");
+    require(rows == fs.px, "invalid dimension for mask");
+    require(stride >= fs.m, "invalid mask row stride");
     require(min_span >= 1, "min_span must be >= 1");
-    require(end_span >= 0 && (size_t)end_span < ctx->n, "end_span must be in [0, n)");
-    const size_t m = ctx->m, total = rows * m;
+    require(end_span >= 0 && (size_t)end_span < fs.n, "end_span must be in [0, n)");
+    require(linear_only || ctx->form == MARS_HIP_FORM_PLAIN,
+            "the mixed-precision form scores linear_only fits (use precision='fp64' for hinges)");
+    const size_t m = fs.m, total = rows * m;
     hipStream_t st = ctx->stream;
 
-    // candidates in row-major (xcol, parent) order
-    std::vector<int> all_xc, all_par;
+    Cands &cs = ctx->cand;
+    cs.fit.clear();
+    cs.xc.clear();
+    cs.par.clear();
     for (size_t i = 0; i < rows; ++i)
         for (size_t j = 0; j < m; ++j)
             if (mask[i * stride + j]) {
-                all_xc.push_back((int)i);
-                all_par.push_back((int)j);
+                cs.fit.push_back(0);
+                cs.xc.push_back((int)i);
+                cs.par.push_back((int)j);
             }
+    const int P = (int)cs.fit.size();
+    index_cands(ctx, cs);
 
     ctx->dense.ensure(3 * std::max<size_t>(total, 1));
-    ctx->hstate_i.ensure(std::max<size_t>(total, 1));
-    ctx->bout.ensure(1);
+    ctx->dknot.ensure(std::max<size_t>(total, 1));
     double *dlin = ctx->dense.p, *dhin = dlin + total, *dcut = dhin + total;
-    int *dknot = ctx->hstate_i.p;
+    int *dknot = ctx->dknot.p;
     if (total) {
-        hipLaunchKernelGGL(k_dense_init, dim3((total + 255) / 256), dim3(256), 0, st, dlin, dhin,
-                           dcut, dknot, total);
+        hipLaunchKernelGGL(k_dense_init, dim3((total + 255) / 256), dim3(256), 0, st, dlin, dhin, dcut, dknot,
+                           total);
         MH_LAUNCH_CHECK();
     }
     MH_CHECK(hipEventRecord(ctx->ev0, st));
-    const XSrc xs{ctx->X.p, ctx->n, ctx->s.p};
-    const size_t cap = block_capacity(ctx);
-    size_t a = 0;
-    while (a < all_xc.size()) {
-        // whole X columns per block
-        size_t b = a;
-        while (b < all_xc.size()) {
-            size_t e = b;
-            while (e < all_xc.size() && all_xc[e] == all_xc[b]) ++e;
-            if (b > a && e - a > cap) break;
-            b = e;
-        }
-        ctx->cand.h_xc.assign(all_xc.begin() + a, all_xc.begin() + b);
-        ctx->cand.h_par.assign(all_par.begin() + a, all_par.begin() + b);
-        upload_cands(ctx);
-        const int P = (int)(b - a);
+    if (P > 0) {
+        ctx->lin.ensure(P);
+        ctx->ybx.ensure(P);
         if (ctx->form == MARS_HIP_FORM_MIXED) {
-            mixed_orthonormalize(ctx, xs, P);
-            require(linear_only, "the mixed-precision form currently scores linear_only fits");
+            mixed_score(ctx);
         } else {
-            PlainPlan pl = plain_orthonormalize(ctx, xs, P, !linear_only);
-            if (!linear_only) hinge_sweep(ctx, xs, pl, min_span, end_span);
+            plain_score(ctx, cs, !linear_only, ctx->lin.p, ctx->ybx.p);
+            if (!linear_only) hinge_sweep(ctx, min_span, end_span);
         }
-        hipLaunchKernelGGL(k_scatter, dim3((P + 255) / 256), dim3(256), 0, st, ctx->cand.xc.p,
-                           ctx->cand.par.p, P, (int)m, ctx->lin.p, ctx->hin.p, ctx->knot.p,
-                           ctx->krow.p, ctx->X.p, ctx->n, linear_only ? 0 : 1, dlin, dhin, dcut,
-                           dknot);
+        upload_cands(ctx, cs);
+        hipLaunchKernelGGL(k_scatter, dim3((P + 255) / 256), dim3(256), 0, st, cs.d_xc.p, cs.d_par.p, P, (int)m,
+                           ctx->lin.p, ctx->hin.p, ctx->knot.p, ctx->krow.p, fs.X.p, fs.ldx, linear_only ? 0 : 1,
+                           dlin, dhin, dcut, dknot);
         MH_LAUNCH_CHECK();
-        a = b;
     }
-    hipLaunchKernelGGL(k_argmax, dim3(1), dim3(1024), 0, st, dlin, dhin, dknot, dcut, total,
+    ctx->ival.ensure(2);
+    ctx->bout.ensure(1);
+    const int ol[2] = {0, (int)total};
+    MH_CHECK(hipMemcpyAsync(ctx->ival.p, ol, sizeof(ol), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_argmax, dim3(1), dim3(1024), 0, st, dlin, dhin, dknot, ctx->ival.p, ctx->ival.p + 1,
                        ctx->bout.p);
     MH_LAUNCH_CHECK();
     MH_CHECK(hipEventRecord(ctx->ev1, st));
@@ -618,7 +739,7 @@ void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t 
     ctx->eval_ms += ms;
     resolve_timing(ctx);
     if (best) {
-        best->sse0 = host_dsse(ctx);
+        best->sse0 = host_dsse(fs);
         best->sse1 = bo.v1;
         best->xcol1 = m ? bo.c1 / (int)m : 0;
         best->bcol1 = m ? bo.c1 % (int)m : 0;
@@ -633,153 +754,143 @@ void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t 
 /* ---------------------------------------------------------------------- */
 /* Mirror mode (the reference's mars::cuda boundary)                       */
 /* ---------------------------------------------------------------------- */
-__global__ void k_put_bcol(float *__restrict__ dst, size_t ldb, int col, const float *__restrict__ src, size_t n)
-{
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i * ldb + col] = src[i];
-}
-
-void mirror_sync_basis(mars_hip_ctx *ctx, size_t m, const float *B, size_t ldB, const float *Bo,
-                       size_t ldBo)
+void mirror_sync_basis(mars_hip_ctx *ctx, size_t m, const float *B, size_t ldB, const float *Bo, size_t ldBo)
 {
     require(!ctx->resident, "resident contexts own their basis");
     require(m <= ctx->max_terms, "m exceeds max_terms");
     require(m == 0 || (B && Bo), "null basis pointer");
-    const size_t n = ctx->n;
+    FitState &fs = ctx->fits[0];
+    const size_t n = fs.n;
     hipStream_t st = ctx->stream;
-    if (m > ctx->m_synced) {
+    if (m > fs.m_synced) {
         DBuf<float> col;
         col.ensure(n);
         std::vector<float> h(n);
-        for (size_t j = ctx->m_synced; j < m; ++j) {
+        for (size_t j = fs.m_synced; j < m; ++j) {
             MH_CHECK(hipMemcpyAsync(col.p, B + j * ldB, n * sizeof(float), hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_put_bcol, dim3((n + 255) / 256), dim3(256), 0, st, ctx->B.p, ctx->ldb,
-                               (int)j, col.p, n);
+            hipLaunchKernelGGL(k_put_bcol, dim3((n + 255) / 256), dim3(256), 0, st, fs.B.p, ctx->ldb, (int)j, col.p,
+                               n);
             MH_LAUNCH_CHECK();
             MH_CHECK(hipStreamSynchronize(st));
             for (size_t i = 0; i < n; ++i) h[i] = Bo[i * ldBo + j];
             MH_CHECK(hipMemcpyAsync(col.p, h.data(), n * sizeof(float), hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_put_bcol, dim3((n + 255) / 256), dim3(256), 0, st, ctx->Bo.p, ctx->ldb,
-                               (int)j, col.p, n);
+            hipLaunchKernelGGL(k_put_bcol, dim3((n + 255) / 256), dim3(256), 0, st, fs.Bo.p, ctx->ldb, (int)j, col.p,
+                               n);
             MH_LAUNCH_CHECK();
             MH_CHECK(hipStreamSynchronize(st));
         }
         col.release();
-        ctx->m_synced = m;
+        fs.m_synced = m;
     }
-    ctx->m = m;
+    fs.m = m;
 }
 
 void mirror_set_target(mars_hip_ctx *ctx, const float *y)
 {
     require(y != nullptr, "null target");
-    if (ctx->y_key == y) return;
-    MH_CHECK(hipMemcpyAsync(ctx->y.p, y, ctx->n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    FitState &fs = ctx->fits[0];
+    if (fs.y_key == y) return;
+    MH_CHECK(hipMemcpyAsync(fs.y.p, y, fs.n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     MH_CHECK(hipStreamSynchronize(ctx->stream));
-    ctx->y_key = y;
+    fs.y_key = y;
 }
 
-void mirror_sync_xcols(mars_hip_ctx *ctx, size_t px, const float *X, size_t ldX, const float *s,
-                       const int *cols, size_t ncols)
+void mirror_sync_xcols(mars_hip_ctx *ctx, size_t px, const float *X, size_t ldX, const float *s, const int *cols,
+                       size_t ncols)
 {
     require(X && s, "null X or scale pointer");
-    const size_t n = ctx->n;
-    if (ctx->px == 0) {
-        ctx->px = px;
-        ctx->X.ensure(n * px + n);  // + one scratch column for single-column calls
-        ctx->s.ensure(px + 1);
-        ctx->xres.assign(px, 0);
+    FitState &fs = ctx->fits[0];
+    const size_t n = fs.n;
+    if (fs.px == 0) {
+        fs.px = px;
+        fs.X.ensure(fs.ldx * px + 32);
+        MH_CHECK(hipMemsetAsync(fs.X.p, 0, (fs.ldx * px + 32) * sizeof(float), ctx->stream));
+        fs.s.ensure(px);
+        fs.xres.assign(px, 0);
     }
-    require(px == ctx->px, "p_x changed between calls");
-    MH_CHECK(hipMemcpyAsync(ctx->s.p, s, px * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    require(px == fs.px, "p_x changed between calls");
+    require(ncols == 0 || cols != nullptr, "null column list");
+    MH_CHECK(hipMemcpyAsync(fs.s.p, s, px * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     for (size_t k = 0; k < ncols; ++k) {
         const int c = cols[k];
         require(c >= 0 && (size_t)c < px, "X column out of range");
-        if (ctx->xres[c]) continue;
-        MH_CHECK(hipMemcpyAsync(ctx->X.p + (size_t)c * n, X + (size_t)c * ldX, n * sizeof(float),
-                                hipMemcpyHostToDevice, ctx->stream));
-        ctx->xres[c] = 1;
+        if (fs.xres[c]) continue;
+        MH_CHECK(hipMemcpyAsync(fs.X.p + (size_t)c * fs.ldx, X + (size_t)c * ldX, n * sizeof(float), hipMemcpyHostToDevice,
+                                ctx->stream));
+        fs.xres[c] = 1;
     }
     MH_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
-__global__ void k_gather_cols(const float *__restrict__ rbuf, size_t ldr, int p, size_t n,
-                              float *__restrict__ out, size_t ldo)
-{
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    if (i < n) out[(size_t)j * ldo + i] = rbuf[i * ldr + j];
-}
-
-void mirror_orthonormalize(mars_hip_ctx *ctx, size_t m, size_t p, const float *x, const int *mask,
-                           float *Bx, size_t ldBx, double *ybx, long *fired)
+void mirror_orthonormalize(mars_hip_ctx *ctx, size_t m, size_t p, const float *x, const int *mask, float *Bx,
+                           size_t ldBx, double *ybx, long *fired)
 {
     require(!ctx->resident, "use mars_hip_fit_eval on resident contexts");
-    require(m == ctx->m && m <= ctx->m_synced, "basis not synced up to m");
+    FitState &fs = ctx->fits[0];
+    require(m == fs.m && m <= fs.m_synced && m >= 1, "basis not synced up to m");
     require(p <= ctx->max_terms, "p exceeds max_terms");
     require(x && mask, "null x or mask");
-    require(Bx == nullptr || ldBx >= ctx->n, "invalid leading dimension for Bx");
+    require(Bx == nullptr || ldBx >= fs.n, "invalid leading dimension for Bx");
     if (p == 0) return;
-    const size_t n = ctx->n;
+    const size_t n = fs.n;
     hipStream_t st = ctx->stream;
     DBuf<float> xs1;
-    xs1.ensure(n + 1);
+    xs1.ensure(fs.ldx + 32);
+    MH_CHECK(hipMemsetAsync(xs1.p, 0, (fs.ldx + 32) * sizeof(float), st));
     const float one = 1.0f;
     MH_CHECK(hipMemcpyAsync(xs1.p, x, n * sizeof(float), hipMemcpyHostToDevice, st));
-    MH_CHECK(hipMemcpyAsync(xs1.p + n, &one, sizeof(float), hipMemcpyHostToDevice, st));
-    ctx->cand.h_xc.assign(p, 0);
-    ctx->cand.h_par.assign(mask, mask + p);
-    for (int v : ctx->cand.h_par) require(v >= 0 && (size_t)v < m, "mask entry out of range");
-    upload_cands(ctx);
-    long long d0 = 0;
-    MH_CHECK(hipMemcpyAsync(&d0, ctx->dgks.p, sizeof(d0), hipMemcpyDeviceToHost, st));
-    const XSrc src{xs1.p, n, xs1.p + n};
-    PlainPlan pl = plain_orthonormalize(ctx, src, (int)p, Bx != nullptr);
+    MH_CHECK(hipMemcpyAsync(xs1.p + fs.ldx, &one, sizeof(float), hipMemcpyHostToDevice, st));
+    Cands &cs = ctx->cand;
+    cs.fit.assign(p, 0);
+    cs.xc.assign(p, 0);
+    cs.par.assign(mask, mask + p);
+    for (int v : cs.par) require(v >= 0 && (size_t)v < m, "mask entry out of range");
+    const long d0 = ctx->dgks_host;
+    ctx->lin.ensure(p);
+    ctx->ybx.ensure(p);
+    plain_score(ctx, cs, Bx != nullptr, ctx->lin.p, ctx->ybx.p, xs1.p, xs1.p + fs.ldx);
     if (Bx) {
         DBuf<float> out;
         out.ensure(n * p);
-        hipLaunchKernelGGL(k_gather_cols, dim3((n + 255) / 256, p), dim3(256), 0, st, ctx->rbuf.p,
-                           pl.ldr, (int)p, n, out.p, n);
+        hipLaunchKernelGGL(k_gather_cols, dim3((n + 255) / 256, p), dim3(256), 0, st, fs.rbuf.p, fs.ldr, n, out.p, n);
         MH_LAUNCH_CHECK();
-        MH_CHECK(hipMemcpy2DAsync(Bx, ldBx * sizeof(float), out.p, n * sizeof(float), n * sizeof(float),
-                                  p, hipMemcpyDeviceToHost, st));
+        MH_CHECK(hipMemcpy2DAsync(Bx, ldBx * sizeof(float), out.p, n * sizeof(float), n * sizeof(float), p,
+                                  hipMemcpyDeviceToHost, st));
         MH_CHECK(hipStreamSynchronize(st));
         out.release();
     }
     if (ybx) MH_CHECK(hipMemcpyAsync(ybx, ctx->ybx.p, p * sizeof(double), hipMemcpyDeviceToHost, st));
-    long long d1 = 0;
-    MH_CHECK(hipMemcpyAsync(&d1, ctx->dgks.p, sizeof(d1), hipMemcpyDeviceToHost, st));
     MH_CHECK(hipStreamSynchronize(st));
     xs1.release();
-    if (fired) *fired += (long)(d1 - d0);
+    resolve_timing(ctx);
+    if (fired) *fired += ctx->dgks_host - d0;
 }
 
-void mirror_orthonormalize_batch(mars_hip_ctx *ctx, size_t m, size_t P, const int *src_xcol,
-                                 const int *src_basis, double *ybx, long *fired)
+void mirror_orthonormalize_batch(mars_hip_ctx *ctx, size_t m, size_t P, const int *src_xcol, const int *src_basis,
+                                 double *ybx, long *fired)
 {
     require(!ctx->resident, "use mars_hip_fit_eval on resident contexts");
-    require(m == ctx->m && m <= ctx->m_synced, "basis not synced up to m");
-    require(P <= block_capacity(ctx), "P exceeds the batch capacity");
-    require(ctx->px > 0, "no X columns resident (call mars_hip_context_sync_xcols)");
+    FitState &fs = ctx->fits[0];
+    require(m == fs.m && m <= fs.m_synced && m >= 1, "basis not synced up to m");
+    require(fs.px > 0, "no X columns resident (call mars_hip_context_sync_xcols)");
     if (P == 0) return;
     require(src_xcol && src_basis && ybx, "null argument");
-    ctx->cand.h_xc.assign(src_xcol, src_xcol + P);
-    ctx->cand.h_par.assign(src_basis, src_basis + P);
+    Cands &cs = ctx->cand;
+    cs.fit.assign(P, 0);
+    cs.xc.assign(src_xcol, src_xcol + P);
+    cs.par.assign(src_basis, src_basis + P);
     for (size_t j = 0; j < P; ++j) {
-        require(src_xcol[j] >= 0 && (size_t)src_xcol[j] < ctx->px && ctx->xres[src_xcol[j]],
-                "src_xcol not resident");
+        require(src_xcol[j] >= 0 && (size_t)src_xcol[j] < fs.px && fs.xres[src_xcol[j]], "src_xcol not resident");
         require(src_basis[j] >= 0 && (size_t)src_basis[j] < m, "src_basis out of range");
     }
-    upload_cands(ctx);
-    hipStream_t st = ctx->stream;
-    long long d0 = 0, d1 = 0;
-    MH_CHECK(hipMemcpyAsync(&d0, ctx->dgks.p, sizeof(d0), hipMemcpyDeviceToHost, st));
-    const XSrc src{ctx->X.p, ctx->n, ctx->s.p};
-    plain_orthonormalize(ctx, src, (int)P, false);
-    MH_CHECK(hipMemcpyAsync(ybx, ctx->ybx.p, P * sizeof(double), hipMemcpyDeviceToHost, st));
-    MH_CHECK(hipMemcpyAsync(&d1, ctx->dgks.p, sizeof(d1), hipMemcpyDeviceToHost, st));
-    MH_CHECK(hipStreamSynchronize(st));
-    if (fired) *fired += (long)(d1 - d0);
+    const long d0 = ctx->dgks_host;
+    ctx->lin.ensure(P);
+    ctx->ybx.ensure(P);
+    plain_score(ctx, cs, false, ctx->lin.p, ctx->ybx.p);
+    MH_CHECK(hipMemcpyAsync(ybx, ctx->ybx.p, P * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    MH_CHECK(hipStreamSynchronize(ctx->stream));
+    resolve_timing(ctx);
+    if (fired) *fired += ctx->dgks_host - d0;
 }
 
 template <typename F>
@@ -801,7 +912,6 @@ int guarded(F &&f)
     }
 }
 
-} // namespace
 } // namespace mh
 
 /* ---------------------------------------------------------------------- */
@@ -820,7 +930,9 @@ int mars_hip_context_create(size_t n, size_t max_terms, double tol, mars_hip_ctx
         *out = nullptr;
         auto *ctx = new mars_hip_ctx;
         try {
-            ctx_base_init(ctx, n, max_terms, tol);
+            ctx_base_init(ctx, max_terms);
+            ctx->fits.resize(1);
+            fit_state_alloc(ctx, ctx->fits[0], n, tol);
             MH_CHECK(hipStreamSynchronize(ctx->stream));
         } catch (...) {
             ctx_free(ctx);
@@ -839,12 +951,15 @@ int mars_hip_context_batch_capacity(mars_hip_ctx *ctx, size_t *capacity)
 {
     return guarded([&] {
         require(ctx && capacity, "null argument");
-        *capacity = block_capacity(ctx);
+        // the residual buffer is (n x P) f32 and grows on demand; report the
+        // candidate count whose buffer fits in 8 GiB
+        const size_t n = std::max<size_t>(ctx->fits[0].n, 1);
+        *capacity = std::max<size_t>(ctx->max_terms, ((size_t)8 << 30) / (4 * n));
     });
 }
 
-int mars_hip_context_sync_basis(mars_hip_ctx *ctx, size_t m, const float *B, size_t ldB,
-                                const float *Bo, size_t ldBo)
+int mars_hip_context_sync_basis(mars_hip_ctx *ctx, size_t m, const float *B, size_t ldB, const float *Bo,
+                                size_t ldBo)
 {
     return guarded([&] {
         require(ctx != nullptr, "null context");
@@ -860,20 +975,19 @@ int mars_hip_context_set_target(mars_hip_ctx *ctx, const float *y)
     });
 }
 
-int mars_hip_context_sync_xcols(mars_hip_ctx *ctx, size_t p_x, const float *X, size_t ldX,
-                                const float *s, const int *cols, size_t ncols)
+int mars_hip_context_sync_xcols(mars_hip_ctx *ctx, size_t p_x, const float *X, size_t ldX, const float *s,
+                                const int *cols, size_t ncols)
 {
     return guarded([&] {
         require(ctx != nullptr, "null context");
         require(!ctx->resident, "resident contexts own their X");
-        require(ldX >= ctx->n, "invalid leading dimension for X");
+        require(ldX >= ctx->fits[0].n, "invalid leading dimension for X");
         mirror_sync_xcols(ctx, p_x, X, ldX, s, cols, ncols);
     });
 }
 
-int mars_hip_orthonormalize(mars_hip_ctx *ctx, size_t m, size_t p, const float *x,
-                            const int *mask, float *Bx, size_t ldBx, double *ybx,
-                            long *dgks_fired)
+int mars_hip_orthonormalize(mars_hip_ctx *ctx, size_t m, size_t p, const float *x, const int *mask, float *Bx,
+                            size_t ldBx, double *ybx, long *dgks_fired)
 {
     return guarded([&] {
         require(ctx != nullptr, "null context");
@@ -890,8 +1004,8 @@ int mars_hip_orthonormalize_batch(mars_hip_ctx *ctx, size_t m, size_t P, const i
     });
 }
 
-int mars_hip_fit_create(const float *X, size_t ldX, const float *y, const float *w, size_t n,
-                        size_t p_x, size_t max_terms, int form, mars_hip_ctx **out)
+int mars_hip_fit_create(const float *X, size_t ldX, const float *y, const float *w, size_t n, size_t p_x,
+                        size_t max_terms, int form, mars_hip_ctx **out)
 {
     return guarded([&] {
         require(out != nullptr, "null output pointer");
@@ -899,8 +1013,11 @@ int mars_hip_fit_create(const float *X, size_t ldX, const float *y, const float 
         *out = nullptr;
         auto *ctx = new mars_hip_ctx;
         try {
-            fit_create(ctx, X, ldX, y, w, n, p_x, max_terms);
+            ctx_base_init(ctx, max_terms);
+            ctx->fits.resize(1);
+            ctx->resident = true;
             ctx->form = form;
+            fit_init(ctx, ctx->fits[0], X, ldX, y, w, n, p_x);
         } catch (...) {
             ctx_free(ctx);
             throw;
@@ -913,9 +1030,11 @@ int mars_hip_fit_info(mars_hip_ctx *ctx, int *nbasis, double *yvar, double *dsse
 {
     return guarded([&] {
         require(ctx != nullptr, "null context");
-        if (nbasis) *nbasis = (int)ctx->m;
-        if (yvar) *yvar = ctx->yvar;
-        if (dsse) *dsse = host_dsse(ctx);
+        require(!ctx->fits.empty(), "empty context");
+        const FitState &fs = ctx->fits[0];
+        if (nbasis) *nbasis = (int)fs.m;
+        if (yvar) *yvar = fs.yvar;
+        if (dsse) *dsse = host_dsse(fs);
     });
 }
 
@@ -928,14 +1047,13 @@ int mars_hip_fit_set_form(mars_hip_ctx *ctx, int form)
     });
 }
 
-int mars_hip_fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t mask_rows,
-                      size_t mask_stride, int min_span, int end_span, int linear_only,
-                      double *linear, double *hinge, double *cut, mars_hip_best *best)
+int mars_hip_fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t mask_rows, size_t mask_stride,
+                      int min_span, int end_span, int linear_only, double *linear, double *hinge, double *cut,
+                      mars_hip_best *best)
 {
     return guarded([&] {
         require(ctx != nullptr, "null context");
-        fit_eval(ctx, mask, mask_rows, mask_stride, min_span, end_span, linear_only, linear,
-                 hinge, cut, best);
+        fit_eval(ctx, mask, mask_rows, mask_stride, min_span, end_span, linear_only, linear, hinge, cut, best);
     });
 }
 
@@ -943,7 +1061,16 @@ int mars_hip_fit_append(mars_hip_ctx *ctx, char type, int xcol, int bcol, float 
 {
     return guarded([&] {
         require(ctx != nullptr && mse != nullptr, "null argument");
-        fit_append(ctx, type, xcol, bcol, h, mse);
+        require(ctx->fits.size() == 1, "use mars_hip_batch_fit on batch contexts");
+        if (ctx->fits[0].m >= ctx->max_terms) throw Error(MARS_HIP_EFULL, "basis matrix is full");
+        AppendArg a{0, xcol, bcol, 1, h, 0};
+        if (type == 'l') a.type = 0;
+        else if (type == '+') a.type = 1;
+        else if (type == '-') a.type = 2;
+        else throw Error(MARS_HIP_EINVAL, "invalid basis type");
+        std::vector<AppendOut> o;
+        fit_append_batch(ctx, {a}, o);
+        *mse = o[0].mse;
     });
 }
 
@@ -951,15 +1078,15 @@ int mars_hip_fit_basis(mars_hip_ctx *ctx, float *B, float *Bo, size_t *m_out)
 {
     return guarded([&] {
         require(ctx != nullptr, "null context");
-        const size_t n = ctx->n, m = ctx->m, ldb = ctx->ldb;
+        const FitState &fs = ctx->fits[0];
+        const size_t n = fs.n, m = fs.m, ldb = ctx->ldb;
         if (m_out) *m_out = m;
         std::vector<float> h(n * ldb);
         float *dst[2] = {B, Bo};
-        const float *src[2] = {ctx->B.p, ctx->Bo.p};
+        const float *src[2] = {fs.B.p, fs.Bo.p};
         for (int q = 0; q < 2; ++q) {
             if (!dst[q]) continue;
-            MH_CHECK(hipMemcpyAsync(h.data(), src[q], n * ldb * sizeof(float), hipMemcpyDeviceToHost,
-                                    ctx->stream));
+            MH_CHECK(hipMemcpyAsync(h.data(), src[q], n * ldb * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
             MH_CHECK(hipStreamSynchronize(ctx->stream));
             for (size_t i = 0; i < n; ++i)
                 for (size_t j = 0; j < m; ++j) dst[q][i * m + j] = h[i * ldb + j];
@@ -1001,14 +1128,20 @@ int mars_hip_kernel_stats(mars_hip_ctx *ctx, mars_hip_kstat *out, int capacity, 
     });
 }
 
+int mars_hip_mixed_stats(mars_hip_ctx *ctx, long *scored, long *rescored)
+{
+    return guarded([&] {
+        require(ctx != nullptr, "null context");
+        if (scored) *scored = ctx->scored;
+        if (rescored) *rescored = ctx->rescored;
+    });
+}
+
 int mars_hip_stats(mars_hip_ctx *ctx, long *dgks_fired, double *eval_ms)
 {
     return guarded([&] {
         require(ctx != nullptr, "null context");
-        long long d = 0;
-        MH_CHECK(hipMemcpyAsync(&d, ctx->dgks.p, sizeof(d), hipMemcpyDeviceToHost, ctx->stream));
-        MH_CHECK(hipStreamSynchronize(ctx->stream));
-        if (dgks_fired) *dgks_fired = (long)d;
+        if (dgks_fired) *dgks_fired = ctx->dgks_host;
         if (eval_ms) *eval_ms = ctx->eval_ms;
     });
 }

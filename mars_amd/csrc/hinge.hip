@@ -395,25 +395,27 @@ void launch_sweep(const SweepArgs &a, dim3 grid, size_t lds, hipStream_t st, boo
 
 } // namespace
 
-void ensure_sorted(mars_hip_ctx *ctx, const XSrc &xs, int xc)
+/* Make the stable descending order and f32 deltas of X column xc of fit 0 resident. */
+static void ensure_sorted(mars_hip_ctx *ctx, int xc)
 {
-    if (ctx->sorted.size() != ctx->px) ctx->sorted.assign(ctx->px, 0);
-    if (ctx->sorted[xc]) return;
-    const size_t n = ctx->n;
+    FitState &fs = ctx->fits[0];
+    if (fs.sorted.size() != fs.px) fs.sorted.assign(fs.px, 0);
+    if (fs.sorted[xc]) return;
+    const size_t n = fs.n;
     hipStream_t st = ctx->stream;
-    ctx->order.ensure(ctx->px * n);
-    ctx->delta.ensure(ctx->px * n);
+    fs.order.ensure(fs.px * n);
+    fs.delta.ensure(fs.px * n);
     const int nblocks = (int)((n + kSortTile - 1) / kSortTile);
     DBuf<unsigned> keys, hist;
     DBuf<int> vals;
     keys.ensure(2 * n);
     vals.ensure(n);
     hist.ensure((size_t)256 * nblocks);
-    int *out = ctx->order.p + (size_t)xc * n;
+    int *out = fs.order.p + (size_t)xc * n;
     unsigned *ka = keys.p, *kb = keys.p + n;
     int *va = vals.p, *vb = out;
     KTimer kt(ctx, K_SORT, 0.0, 16.0 * 4 * n);
-    hipLaunchKernelGGL(k_sort_keys, dim3((n + 255) / 256), dim3(256), 0, st, xs.X + (size_t)xc * xs.ldx,
+    hipLaunchKernelGGL(k_sort_keys, dim3((n + 255) / 256), dim3(256), 0, st, fs.X.p + (size_t)xc * fs.ldx,
                        n, ka, va);
     MH_LAUNCH_CHECK();
     for (int pass = 0; pass < 4; ++pass) {
@@ -431,23 +433,25 @@ void ensure_sorted(mars_hip_ctx *ctx, const XSrc &xs, int xc)
     // after 4 passes the values are back in `vals` (even pass count): copy into the cache
     MH_CHECK(hipMemcpyAsync(out, va, n * sizeof(int), hipMemcpyDeviceToDevice, st));
     float sc = 1.0f;
-    MH_CHECK(hipMemcpyAsync(&sc, xs.s + xc, sizeof(float), hipMemcpyDeviceToHost, st));
+    MH_CHECK(hipMemcpyAsync(&sc, fs.s.p + xc, sizeof(float), hipMemcpyDeviceToHost, st));
     MH_CHECK(hipStreamSynchronize(st));
-    hipLaunchKernelGGL(k_deltas, dim3((n + 255) / 256), dim3(256), 0, st, xs.X + (size_t)xc * xs.ldx,
-                       sc, out, n, ctx->delta.p + (size_t)xc * n);
+    hipLaunchKernelGGL(k_deltas, dim3((n + 255) / 256), dim3(256), 0, st, fs.X.p + (size_t)xc * fs.ldx,
+                       sc, out, n, fs.delta.p + (size_t)xc * n);
     MH_LAUNCH_CHECK();
     MH_CHECK(hipStreamSynchronize(st));
     keys.release();
     vals.release();
     hist.release();
-    ctx->sorted[xc] = 1;
+    fs.sorted[xc] = 1;
 }
 
-void hinge_sweep(mars_hip_ctx *ctx, const XSrc &xs, const PlainPlan &pl, int min_span, int end_span)
+void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span)
 {
-    const int P = pl.P;
-    const size_t n = ctx->n;
-    const int m = (int)ctx->m;
+    FitState &fs = ctx->fits[0];
+    Cands &cs = ctx->cand;
+    const int P = (int)cs.fit.size();
+    const size_t n = fs.n;
+    const int m = (int)fs.m;
     hipStream_t st = ctx->stream;
     ctx->hin.ensure(P);
     ctx->knot.ensure(P);
@@ -458,13 +462,13 @@ void hinge_sweep(mars_hip_ctx *ctx, const XSrc &xs, const PlainPlan &pl, int min
     int maxnp = 0;
     for (int c = 0; c < P;) {
         int e = c;
-        while (e < P && ctx->cand.h_xc[e] == ctx->cand.h_xc[c]) ++e;
+        while (e < P && cs.xc[e] == cs.xc[c]) ++e;
         for (int q = c; q < e; ++q) cgrp[q] = (int)(grp.size() / 3);
-        grp.push_back(ctx->cand.h_xc[c]);
+        grp.push_back(cs.xc[c]);
         grp.push_back(c);
         grp.push_back(e - c);
         maxnp = std::max(maxnp, e - c);
-        ensure_sorted(ctx, xs, ctx->cand.h_xc[c]);
+        ensure_sorted(ctx, cs.xc[c]);
         c = e;
     }
     const int ng = (int)(grp.size() / 3);
@@ -488,21 +492,21 @@ void hinge_sweep(mars_hip_ctx *ctx, const XSrc &xs, const PlainPlan &pl, int min
     ctx->hbest_i.ensure((size_t)P * C);
 
     SweepArgs a;
-    a.Bo = ctx->Bo.p;
-    a.B = ctx->B.p;
-    a.rbuf = ctx->rbuf.p;
-    a.y = ctx->y.p;
-    a.X = xs.X;
-    a.s = xs.s;
+    a.Bo = fs.Bo.p;
+    a.B = fs.B.p;
+    a.rbuf = fs.rbuf.p;
+    a.y = fs.y.p;
+    a.X = fs.X.p;
+    a.s = fs.s.p;
     a.ldb = ctx->ldb;
-    a.ldr = pl.ldr;
+    a.ldr = fs.ldr;
     a.n = n;
-    a.ybo = ctx->ybo.p;
+    a.ybo = fs.ybo.p;
     a.ybx = ctx->ybx.p;
     a.grp = dgrp.p;
-    a.cpar = ctx->cand.par.p;
-    a.order = ctx->order.p;
-    a.delta = ctx->delta.p;
+    a.cpar = cs.d_par.p;
+    a.order = fs.order.p;
+    a.delta = fs.delta.p;
     a.m = m;
     a.L2 = L2;
     a.C = C;
@@ -510,7 +514,7 @@ void hinge_sweep(mars_hip_ctx *ctx, const XSrc &xs, const PlainPlan &pl, int min
     a.head = (size_t)end_span;
     a.span = (size_t)min_span;
     a.clen = clen;
-    a.tol = ctx->tol;
+    a.tol = fs.tol;
     a.sum = ctx->hstate.p;
     a.ksum = a.sum + (size_t)P * C * L2 * 2;
     a.dsum = a.ksum + (size_t)P * C;
@@ -538,7 +542,7 @@ void hinge_sweep(mars_hip_ctx *ctx, const XSrc &xs, const PlainPlan &pl, int min
             default: launch_sweep<4>(a, grid, lds, st, eval); break;
             }
         }
-        hipLaunchKernelGGL(k_sweep_final, dim3((P + 255) / 256), dim3(256), 0, st, a, P, ctx->cand.xc.p,
+        hipLaunchKernelGGL(k_sweep_final, dim3((P + 255) / 256), dim3(256), 0, st, a, P, cs.d_xc.p,
                            ctx->lin.p, ctx->hin.p, ctx->knot.p, ctx->krow.p);
         MH_LAUNCH_CHECK();
     }

@@ -5,49 +5,56 @@
 
 namespace mh {
 
-/* Candidate-column source for the scoring kernels: column xc of X (ld ldx)
- * scaled by s[xc].  Mirror-mode single-column calls point X at one column
- * with scale 1. */
-struct XSrc {
-    const float *X;
-    size_t ldx;
-    const float *s;
+/* Work list of one batched pass plus, per candidate, where its partial sums live. */
+struct Pass {
+    std::vector<Item> items;
+    std::vector<int> cmap;  // per candidate (or list position): first item, item count, slot
 };
 
-/* Rows per split / groups used by the plain orthonormalize pipeline. */
-struct PlainPlan {
-    int P = 0;          // candidates in this block
-    int mt = 0;         // 16-wide basis tiles covering m
-    int ldT = 0;        // leading dimension of T (mt * 16)
-    int ks = 0;         // row splits of the T contraction
-    size_t rps = 0;     // rows per split (multiple of 16)
-    int groups = 0;     // row groups of the projection / finalize passes
-    size_t rpg = 0;     // rows per group (multiple of 64)
-    size_t ldr = 0;     // row stride of the residual buffer
-};
+/*
+ * Score the candidate columns `cs` (all fits) with the reference's FP64
+ * arithmetic and f32 rounding points (kernels.cc:283-417,
+ * marsalgo.cc:500-505).  Writes per candidate c: ybx[c] and lin[c] = ybx^2
+ * (linear delta-SSE); with keep_bxn, the normalized f32 column stays in its
+ * fit's rbuf (row-major, column c - c0).  `xover`/`sover`, when set, replace
+ * fit 0's X and scales (mirror-mode single-column calls).
+ */
+void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, double *ybx,
+                 const float *xover = nullptr, const float *sover = nullptr);
 
-/* Orthonormalize the current candidate block (ctx->cand, P entries) against
- * the live basis with the reference's FP64 arithmetic and f32 rounding points
- * (kernels.cc:283-417), then ybx = Bxn.y (marsalgo.cc:500-505).  Leaves:
- *   ctx->ybx[c], ctx->lin[c] = ybx^2, and, if keep_bxn, the normalized f32
- *   columns in ctx->rbuf (row-major, row stride plan.ldr). */
-PlainPlan plain_orthonormalize(mars_hip_ctx *ctx, const XSrc &xs, int P, bool keep_bxn);
+/* Mixed-precision scoring of ctx->cand (linear delta-SSE only): BF16-split
+ * MFMA projections with an FP64-accumulated refinement, then an exact FP64
+ * rescoring of the flagged and near-winning candidates.  Writes ctx->lin. */
+void mixed_score(mars_hip_ctx *ctx);
 
-/* Mixed-precision scoring of the current candidate block (linear part only). */
-void mixed_orthonormalize(mars_hip_ctx *ctx, const XSrc &xs, int P);
+/* Hinge sweep of fit 0's candidates, whose normalized columns sit in its rbuf. */
+void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span);
 
-/* Hinge sweep over the candidates of the current block whose normalized
- * columns sit in ctx->rbuf (plan from plain_orthonormalize). Fills ctx->hin,
- * ctx->knot, ctx->krow. */
-void hinge_sweep(mars_hip_ctx *ctx, const XSrc &xs, const PlainPlan &plan, int min_span,
-                 int end_span);
+/* Upload the per-fit device views (current m, rbuf, first candidate of `cs`). */
+void upload_fits(mars_hip_ctx *ctx, const Cands &cs, const float *xover = nullptr,
+                 const float *sover = nullptr);
 
-/* Make the stable descending sort order and f32 deltas of X column xc resident. */
-void ensure_sorted(mars_hip_ctx *ctx, const XSrc &xs, int xc);
+/* Upload the candidate arrays of `cs`. */
+void upload_cands(mars_hip_ctx *ctx, Cands &cs);
 
-/* Largest candidate block whose residual buffer fits the scratch budget. */
-size_t block_capacity(const mars_hip_ctx *ctx);
+/* Fill cs.fit_c0 / fit_P from cs.fit (candidates grouped by fit). */
+void index_cands(const mars_hip_ctx *ctx, Cands &cs);
 
-void upload_cands(mars_hip_ctx *ctx);
+/* Make sure each fit's residual buffer can hold its candidates of `cs`. */
+void ensure_rbuf(mars_hip_ctx *ctx, const Cands &cs);
+
+} // namespace mh
+
+namespace mh {
+
+/* fit.hip */
+void ctx_base_init(mars_hip_ctx *ctx, size_t max_terms);
+void ctx_free(mars_hip_ctx *ctx);
+void fit_init(mars_hip_ctx *ctx, FitState &fs, const float *X, size_t ldX, const float *y, const float *w,
+              size_t n, size_t px);
+void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std::vector<AppendOut> &out);
+void argmax_fits(mars_hip_ctx *ctx, const Cands &cs, const double *lin, std::vector<BestOut> &best);
+double host_dsse(const FitState &fs);
+extern thread_local std::string g_last_error;
 
 } // namespace mh

@@ -10,7 +10,6 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int kWave = 64;
-constexpr int kRedBlocks = 256;   // blocks of the fixed-shape grid-stride reductions
 constexpr int kRedThreads = 256;
 
 /*
@@ -46,18 +45,10 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
-/* Sum of kRedBlocks partials, identical in every block that calls it. */
-__device__ inline double sum_partials(const double *part, double *sh)
+/* Sum of `nb` (<= 256) partials, identical in every block that calls it. */
+__device__ inline double sum_partials(const double *part, int nb, double *sh)
 {
-    return block_sum_256(part[threadIdx.x], sh);
-}
-
-/* f32 candidate value B[row, par] * f32(X[row, xc] * s[xc]) (marsalgo.cc:436-441, kernels.cc:298-312). */
-__device__ inline float cand_value(const float *B, size_t ldb, const float *X, size_t ldx,
-                                   const float *s, size_t row, int xc, int par)
-{
-    const float xs = __fmul_rn(X[(size_t)xc * ldx + row], s[xc]);
-    return __fmul_rn(B[row * ldb + par], xs);
+    return block_sum_256((int)threadIdx.x < nb ? part[threadIdx.x] : 0.0, sh);
 }
 
 } // namespace mh

@@ -158,9 +158,13 @@ public:
         long fired = 0;
         double ms = 0;
         check(mars_hip_stats(ctx_, &fired, &ms));
+        long sc = 0, re = 0;
+        check(mars_hip_mixed_stats(ctx_, &sc, &re));
         py::dict d;
         d["dgks_fired"] = fired;
         d["eval_ms"] = ms;
+        d["mixed_scored"] = sc;
+        d["mixed_rescored"] = re;
         return d;
     }
 
@@ -187,6 +191,122 @@ public:
 private:
     mars_hip_ctx *ctx_ = nullptr;
     size_t n_ = 0, px_ = 0;
+};
+
+/* Many independent linear_only fits advanced together (batching form). */
+class Batch {
+public:
+    Batch(py::list Xs, py::list ys, py::object ws, int max_terms, const std::string &precision)
+    {
+        const size_t F = Xs.size();
+        verify(F >= 1 && ys.size() == F, "need one y per X");
+        std::vector<const float *> xp(F), yp(F), wp(F, nullptr);
+        std::vector<size_t> ld(F), n(F), px(F);
+        for (size_t f = 0; f < F; ++f) {
+            auto X = py::cast<py::array_t<float>>(Xs[f]);
+            auto y = py::cast<py::array_t<float>>(ys[f]);
+            py::buffer_info xi = X.request(), yi = y.request();
+            verify(xi.ndim == 2, "expected 2D array for X");
+            verify(yi.ndim == 1, "expected 1D array for y");
+            verify(xi.strides[0] == sizeof(float), "X must be column-major");
+            verify(yi.strides[0] == sizeof(float), "y must be column-major");
+            verify(xi.shape[0] == yi.shape[0], "invalid dimension for y");
+            keep_.append(X);
+            keep_.append(y);
+            xp[f] = (const float *)xi.ptr;
+            yp[f] = (const float *)yi.ptr;
+            n[f] = (size_t)xi.shape[0];
+            px[f] = (size_t)xi.shape[1];
+            ld[f] = xi.shape[1] > 1 ? (size_t)(xi.strides[1] / sizeof(float)) : n[f];
+            if (!ws.is_none()) {
+                auto w = py::cast<py::array_t<float>>(py::list(ws)[f]);
+                py::buffer_info wi = w.request();
+                verify(wi.ndim == 1 && wi.strides[0] == sizeof(float) && wi.shape[0] == xi.shape[0],
+                       "invalid dimension for w");
+                keep_.append(w);
+                wp[f] = (const float *)wi.ptr;
+            }
+        }
+        max_terms_ = (size_t)max_terms;
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = mars_hip_batch_create((int)F, xp.data(), ld.data(), yp.data(), wp.data(), n.data(), px.data(),
+                                       max_terms_, parse_form(precision), &ctx_);
+        }
+        check(rc);
+        nfits_ = F;
+    }
+
+    ~Batch()
+    {
+        if (ctx_) mars_hip_context_destroy(ctx_);
+    }
+
+    /* returns (terms list of per-fit lists of tuples, steps) */
+    py::tuple fit(int max_degree, double penalty, bool self_interact, int r2_window, double r2_thresh)
+    {
+        mars_hip_batch_opts o{max_degree, penalty, self_interact ? 1 : 0, r2_window, r2_thresh};
+        std::vector<mars_hip_term> t(nfits_ * max_terms_);
+        std::vector<int> nt(nfits_);
+        int steps = 0, rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = mars_hip_batch_fit(ctx_, &o, t.data(), nt.data(), &steps);
+        }
+        check(rc);
+        py::list out;
+        for (size_t f = 0; f < nfits_; ++f) {
+            py::list rows;
+            for (int r = 0; r < nt[f]; ++r) {
+                const mars_hip_term &x = t[f * max_terms_ + r];
+                rows.append(py::make_tuple(std::string(1, (char)x.type), x.basis, x.input, x.order, x.mse,
+                                           x.r2, x.r2_cv));
+            }
+            out.append(rows);
+        }
+        return py::make_tuple(out, steps);
+    }
+
+    void set_timing(bool on) { check(mars_hip_timing(ctx_, on ? 1 : 0)); }
+
+    py::list kernel_stats() const
+    {
+        mars_hip_kstat ks[32];
+        int cnt = 0;
+        check(mars_hip_kernel_stats(ctx_, ks, 32, &cnt));
+        py::list out;
+        for (int i = 0; i < cnt && i < 32; ++i) {
+            py::dict d;
+            d["name"] = std::string(ks[i].name);
+            d["launches"] = ks[i].launches;
+            d["ms"] = ks[i].ms;
+            d["flops"] = ks[i].flops;
+            d["bytes"] = ks[i].bytes;
+            out.append(d);
+        }
+        return out;
+    }
+
+    py::dict stats() const
+    {
+        long fired = 0;
+        double ms = 0;
+        check(mars_hip_stats(ctx_, &fired, &ms));
+        long sc = 0, re = 0;
+        check(mars_hip_mixed_stats(ctx_, &sc, &re));
+        py::dict d;
+        d["dgks_fired"] = fired;
+        d["eval_ms"] = ms;
+        d["mixed_scored"] = sc;
+        d["mixed_rescored"] = re;
+        return d;
+    }
+
+private:
+    mars_hip_ctx *ctx_ = nullptr;
+    size_t nfits_ = 0, max_terms_ = 0;
+    py::list keep_;
 };
 
 } // namespace
@@ -224,4 +344,12 @@ PYBIND11_MODULE(_marslib, m)
         .def("stats", &MarsAlgo::stats)
         .def("set_timing", &MarsAlgo::set_timing)
         .def("kernel_stats", &MarsAlgo::kernel_stats);
+    py::class_<Batch>(m, "Batch")
+        .def(py::init<py::list, py::list, py::object, int, const std::string &>(), py::arg("Xs"), py::arg("ys"),
+             py::arg("ws") = py::none(), py::arg("max_terms"), py::arg("precision") = "fp64")
+        .def("fit", &Batch::fit, py::arg("max_degree") = 3, py::arg("penalty") = 3.0,
+             py::arg("self_interact") = false, py::arg("r2_window") = 32, py::arg("r2_thresh") = 3e-5)
+        .def("set_timing", &Batch::set_timing)
+        .def("kernel_stats", &Batch::kernel_stats)
+        .def("stats", &Batch::stats);
 }

@@ -1,125 +1,49 @@
-// Plain (FP64) candidate orthonormalization on gfx950.
+// Plain (FP64) candidate orthonormalization on gfx950, batched over fits.
 //
-// Restates mars::orthonormalize (kernels.cc:283-417) for a whole block of
-// candidates (X column, parent basis) at once, keeping the reference's f32
-// rounding points (SURVEY Appendix A R5-R10 in the design notes):
-//   Bx  = f32(B[:,par] * f32(X[:,xc] * s[xc]))          -- rebuilt on the fly, never stored
-//   T   = Bo^T Bx                     (FP64 MFMA, split over rows, fixed-order reduce)
-//   r   = f32(Bx - Bo T),  s = sum r^2  (FP64 MFMA; r kept row-major in HBM)
+// Restates mars::orthonormalize (kernels.cc:283-417) for every candidate
+// (fit, X column, parent basis) of one forward step at once, keeping the
+// reference's f32 rounding points:
+//   Bx  = f32(B[:,par] * f32(X[:,xc] * s[xc]))          rebuilt on the fly, never stored
+//   T   = Bo^T Bx                     FP64 MFMA, split over rows, fixed-order reduce
+//   r   = f32(Bx - Bo T),  s = sum r^2  FP64 MFMA; r kept row-major in HBM (rbuf)
 //   DGKS: s > tol and 9 s < |T|^2  ->  T2 = Bo^T r, r = f32(r - Bo T2), s = sum r^2
-//         (flagged columns compacted on the device; no host round trip)
 //   Bxn = f32(r * f32(1/sqrt(s + tol))) (0 if s <= tol),  ybx = sum Bxn y
-// All reductions run in a fixed order, so results are run-to-run identical.
+// Work is described by items (fit, candidate tile, row range) consumed by
+// persistent grids; every reduction runs in a fixed order, so results are
+// run-to-run identical and independent of the other fits in the batch.
 #include "internal.h"
 #include "kernels.h"
+#include "tgemm.h"
 
 #include <algorithm>
+#include <numeric>
 
 namespace mh {
 
 namespace {
 
-/* ---------------------------------------------------------------------- */
-/* T = Bo^T Bx (or Bo^T r for the DGKS retry): per-(split, wave) partials  */
-/* grid (cand tiles of 32, row splits, basis-tile groups), 4 waves          */
-/* ---------------------------------------------------------------------- */
-template <int MT, bool RETRY>
-__global__ __launch_bounds__(256) void k_tgemm(
-    const float *__restrict__ Bo, const float *__restrict__ B, size_t ldb, int m, size_t n,
-    const float *__restrict__ X, size_t ldx, const float *__restrict__ sc,
-    const int *__restrict__ cxc, const int *__restrict__ cpar, int P,
-    const float *__restrict__ rbuf, size_t ldr, const int *__restrict__ list,
-    const int *__restrict__ count, size_t rps, int ldT, double *__restrict__ tpart)
-{
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int kr = lane >> 4, kc = lane & 15;
-    const int ct = blockIdx.x, split = blockIdx.y, mt0 = blockIdx.z * 8;
-    const int Pc = RETRY ? *count : P;
-    if (ct * 32 >= Pc) return;
-    const size_t r0 = (size_t)split * rps;
-    const size_t r1 = r0 + rps < n ? r0 + rps : n;
+constexpr int kTileT = 64;   // candidates per T-GEMM item (4 waves x 16)
+constexpr int kTileP = 64;   // candidates per projection item
+constexpr int kTileF = 64;   // candidates per finalize item (x 4 row phases)
 
-    bool cok[2];
-    int cid[2] = {0, 0}, par[2] = {0, 0};
-    float sx[2] = {0.f, 0.f};
-    const float *xcol[2] = {X, X};
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        const int c = ct * 32 + u * 16 + kc;
-        cok[u] = c < Pc;
-        if (cok[u]) {
-            if (RETRY) {
-                cid[u] = list[c];
-            } else {
-                const int xc = cxc[c];
-                par[u] = cpar[c];
-                sx[u] = sc[xc];
-                xcol[u] = X + (size_t)xc * ldx;
-            }
-        }
-    }
-
-    v4d acc[MT][2];
-#pragma unroll
-    for (int t = 0; t < MT; ++t)
-#pragma unroll
-        for (int u = 0; u < 2; ++u) acc[t][u] = v4d{0, 0, 0, 0};
-
-    for (size_t q = r0 + 4 * w; q < r1; q += 16) {
-        const size_t row = q + kr;
-        const bool rok = row < r1;
-        double a[MT];
-#pragma unroll
-        for (int t = 0; t < MT; ++t) {
-            const int k = (mt0 + t) * 16 + kc;
-            a[t] = (rok && k < m) ? (double)Bo[row * ldb + k] : 0.0;
-        }
-        double b[2];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            float v = 0.f;
-            if (rok && cok[u]) {
-                if (RETRY)
-                    v = rbuf[row * ldr + cid[u]];
-                else
-                    v = __fmul_rn(B[row * ldb + par[u]], __fmul_rn(xcol[u][row], sx[u]));
-            }
-            b[u] = (double)v;
-        }
-#pragma unroll
-        for (int t = 0; t < MT; ++t)
-#pragma unroll
-            for (int u = 0; u < 2; ++u) acc[t][u] = mfma_f64(a[t], b[u], acc[t][u]);
-    }
-
-    double *dst = tpart + (size_t)(split * 4 + w) * P * ldT;
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        const int c = ct * 32 + u * 16 + kc;
-        if (!cok[u]) continue;
-#pragma unroll
-        for (int t = 0; t < MT; ++t) {
-            if ((mt0 + t) * 16 >= ldT) continue;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int k = (mt0 + t) * 16 + kr + 4 * v;
-                dst[(size_t)c * ldT + k] = acc[t][u][v];
-            }
-        }
-    }
-}
-
-/* T[c][k] = sum over partials in order; tnorm[c] = |T[c][:m]|^2.  One wave per candidate. */
-__global__ __launch_bounds__(64) void k_treduce(const double *__restrict__ tpart, int nparts, int P,
-                                                int ldT, int m, const int *__restrict__ count,
+/* T[c] = sum of the candidate's item partials in order; tnorm[c] = |T[c][:m]|^2.
+ * One wave per candidate (or list position). */
+__global__ __launch_bounds__(64) void k_treduce(const FitDev *__restrict__ fits,
+                                                const int *__restrict__ cfit,
+                                                const int *__restrict__ list, int count,
+                                                const int *__restrict__ cmap,
+                                                const double *__restrict__ tpart, int ldT,
                                                 double *__restrict__ T, double *__restrict__ tnorm)
 {
     const int c = blockIdx.x, lane = threadIdx.x;
-    if (count && c >= *count) return;
+    if (c >= count) return;
+    const int cid = list ? list[c] : c;
+    const int m = fits[cfit[cid]].m;
+    const int first = cmap[3 * c], cnt = cmap[3 * c + 1], slot = cmap[3 * c + 2];
     double tt = 0.0;
     for (int k = lane; k < ldT; k += 64) {
         double s = 0.0;
-        for (int p = 0; p < nparts; ++p) s += tpart[((size_t)p * P + c) * ldT + k];
+        for (int i = 0; i < cnt; ++i) s += tpart[((size_t)(first + i) * 64 + slot) * ldT + k];
         T[(size_t)c * ldT + k] = s;
         if (k < m) tt = fma(s, s, tt);
     }
@@ -128,315 +52,515 @@ __global__ __launch_bounds__(64) void k_treduce(const double *__restrict__ tpart
 }
 
 /* ---------------------------------------------------------------------- */
-/* r = f32(src - Bo T) with s partials per row group.                      */
-/* grid (row groups, cand tiles of 64), 4 waves x 16-row tiles             */
+/* r = f32(src - Bo T) and sum r^2 per item.  4 waves x 16-row tiles, 64 cands */
 /* ---------------------------------------------------------------------- */
 template <bool RETRY>
-__global__ __launch_bounds__(256) void k_project(
-    const float *__restrict__ Bo, const float *__restrict__ B, size_t ldb, int m, size_t n,
-    const float *__restrict__ X, size_t ldx, const float *__restrict__ sc,
-    const int *__restrict__ cxc, const int *__restrict__ cpar, int P,
-    const double *__restrict__ T, int ldT, float *__restrict__ rbuf, size_t ldr,
-    const int *__restrict__ list, const int *__restrict__ count, size_t rpg,
-    double *__restrict__ spart)
+__global__ __launch_bounds__(256) void k_project(const FitDev *__restrict__ fits,
+                                                 const Item *__restrict__ items, int nitems,
+                                                 const int *__restrict__ cxc,
+                                                 const int *__restrict__ cpar,
+                                                 const int *__restrict__ list,
+                                                 const double *__restrict__ T, int ldT,
+                                                 double *__restrict__ spart)
 {
     __shared__ double sh[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int kr = lane >> 4, kc = lane & 15;
-    const int g = blockIdx.x, ct = blockIdx.y;
-    const int Pc = RETRY ? *count : P;
-    if (ct * 64 >= Pc) return;
-    const size_t g0 = (size_t)g * rpg;
-    const size_t g1 = g0 + rpg < n ? g0 + rpg : n;
-
-    bool cok[4];
-    int slot[4], col[4], par[4] = {0, 0, 0, 0};
-    float sx[4] = {0.f, 0.f, 0.f, 0.f};
-    const float *xcol[4] = {X, X, X, X};
-    double sacc[4] = {0, 0, 0, 0};
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const Item I = items[it];
+        const FitDev F = fits[I.fit];
+        bool cok[4];
+        int tpos[4], col[4], par[4] = {0, 0, 0, 0};
+        float sx[4] = {0.f, 0.f, 0.f, 0.f};
+        const float *xcol[4] = {F.X, F.X, F.X, F.X};
+        double sacc[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        slot[u] = ct * 64 + u * 16 + kc;
-        cok[u] = slot[u] < Pc;
-        col[u] = slot[u];
-        if (cok[u]) {
-            if (RETRY) {
-                col[u] = list[slot[u]];
-            } else {
-                const int xc = cxc[slot[u]];
-                par[u] = cpar[slot[u]];
-                sx[u] = sc[xc];
-                xcol[u] = X + (size_t)xc * ldx;
+        for (int u = 0; u < 4; ++u) {
+            const int slot = u * 16 + kc;
+            cok[u] = slot < I.nc;
+            tpos[u] = I.c0 + slot;  // row of T (candidate or list position)
+            col[u] = 0;
+            if (cok[u]) {
+                if (RETRY) {
+                    col[u] = list[tpos[u]] - F.c0;
+                } else {
+                    const int xc = cxc[tpos[u]];
+                    col[u] = tpos[u] - F.c0;
+                    par[u] = cpar[tpos[u]];
+                    sx[u] = F.s[xc];
+                    xcol[u] = F.X + (size_t)xc * F.ldx;
+                }
             }
         }
-    }
-
-    for (size_t t0 = g0 + 16 * w; t0 < g1; t0 += 64) {
-        v4d acc[4];
+        for (long long t0 = I.r0 + 16 * w; t0 < I.r1; t0 += 64) {
+            v4d acc[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc[u] = v4d{0, 0, 0, 0};
-        const size_t arow = t0 + kc;
-        const bool aok = arow < g1;
-        for (int k0 = 0; k0 < m; k0 += 4) {
-            const int k = k0 + kr;
-            const bool kok = k < m;
-            const double a = (aok && kok) ? (double)Bo[arow * ldb + k] : 0.0;
+            for (int u = 0; u < 4; ++u) acc[u] = v4d{0, 0, 0, 0};
+            const long long arow = t0 + kc;
+            const bool aok = arow < I.r1;
+            for (int k0 = 0; k0 < F.m; k0 += 4) {
+                const int k = k0 + kr;
+                const bool kok = k < F.m;
+                const double a = (aok && kok) ? (double)F.Bo[arow * F.ldb + k] : 0.0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double b = (cok[u] && kok) ? T[(size_t)tpos[u] * ldT + k] : 0.0;
+                    acc[u] = mfma_f64(a, b, acc[u]);
+                }
+            }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const double b = (cok[u] && kok) ? T[(size_t)slot[u] * ldT + k] : 0.0;
-                acc[u] = mfma_f64(a, b, acc[u]);
+                if (!cok[u]) continue;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const long long row = t0 + kr + 4 * v;
+                    if (row >= I.r1) continue;
+                    float src;
+                    if (RETRY)
+                        src = F.rbuf[row * F.ldr + col[u]];
+                    else
+                        src = __fmul_rn(F.B[row * F.ldb + par[u]], __fmul_rn(xcol[u][row], sx[u]));
+                    const float rf = (float)((double)src - acc[u][v]);
+                    F.rbuf[row * F.ldr + col[u]] = rf;
+                    const double rd = (double)rf;
+                    sacc[u] = fma(rd, rd, sacc[u]);
+                }
             }
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            if (!cok[u]) continue;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const size_t row = t0 + kr + 4 * v;
-                if (row >= g1) continue;
-                float src;
-                if (RETRY)
-                    src = rbuf[row * ldr + col[u]];
-                else
-                    src = __fmul_rn(B[row * ldb + par[u]], __fmul_rn(xcol[u][row], sx[u]));
-                const float rf = (float)((double)src - acc[u][v]);
-                rbuf[row * ldr + col[u]] = rf;
-                const double rd = (double)rf;
-                sacc[u] = fma(rd, rd, sacc[u]);
-            }
+            double s = sacc[u];
+            s = s + __shfl_xor(s, 16, 64);
+            s = s + __shfl_xor(s, 32, 64);
+            if (kr == 0) sh[w][u * 16 + kc] = s;
         }
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        double s = sacc[u];
-        s = s + __shfl_xor(s, 16, 64);
-        s = s + __shfl_xor(s, 32, 64);
-        if (kr == 0) sh[w][u * 16 + kc] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        const int t = threadIdx.x;
-        const double tot = ((sh[0][t] + sh[1][t]) + sh[2][t]) + sh[3][t];
-        const int c = ct * 64 + t;
-        if (c < Pc) spart[(size_t)g * P + c] = tot;
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const int t = threadIdx.x;
+            if (t < I.nc) spart[(size_t)it * 64 + t] = ((sh[0][t] + sh[1][t]) + sh[2][t]) + sh[3][t];
+        }
+        __syncthreads();
     }
 }
 
-/* s[c] = sum of group partials; first pass also evaluates the DGKS gate. */
-__global__ void k_sreduce(const double *__restrict__ spart, int groups, int P,
-                          const int *__restrict__ list, const int *__restrict__ count,
-                          const double *__restrict__ tnorm, double tol, double gate,
-                          double *__restrict__ ssum, int *__restrict__ flag)
+/* s[c] = sum of item partials; the first pass also evaluates the DGKS gate. */
+__global__ void k_sreduce(const int *__restrict__ list, int count, const int *__restrict__ cmap,
+                          const double *__restrict__ spart, int tile,
+                          const double *__restrict__ tnorm, const FitDev *__restrict__ fits,
+                          const int *__restrict__ cfit, double *__restrict__ ssum,
+                          int *__restrict__ flag)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int Pc = list ? *count : P;
-    if (c >= Pc) return;
+    if (c >= count) return;
+    const int first = cmap[3 * c], cnt = cmap[3 * c + 1], slot = cmap[3 * c + 2];
     double s = 0.0;
-    for (int g = 0; g < groups; ++g) s += spart[(size_t)g * P + c];
+    for (int i = 0; i < cnt; ++i) s += spart[(size_t)(first + i) * tile + slot];
     if (list) {
         ssum[list[c]] = s;
     } else {
+        const double tol = fits[cfit[c]].tol;
         ssum[c] = s;
-        flag[c] = (s > tol && s * gate < tnorm[c]) ? 1 : 0;
+        flag[c] = (s > tol && s * kGate < tnorm[c]) ? 1 : 0;
     }
 }
 
-/* Ordered compaction of the DGKS flags (single block of 1024 threads). */
-__global__ __launch_bounds__(1024) void k_compact(const int *__restrict__ flag, int P,
-                                                  int *__restrict__ list, int *__restrict__ count,
-                                                  long long *__restrict__ dgks)
-{
-    __shared__ int sh[1024];
-    const int t = threadIdx.x;
-    const int chunk = (P + 1023) / 1024;
-    const int b = t * chunk, e = min(P, b + chunk);
-    int cnt = 0;
-    for (int i = b; i < e; ++i) cnt += flag[i];
-    sh[t] = cnt;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const int v = t >= o ? sh[t - o] : 0;
-        __syncthreads();
-        sh[t] += v;
-        __syncthreads();
-    }
-    int pos = sh[t] - cnt;
-    for (int i = b; i < e; ++i)
-        if (flag[i]) list[pos++] = i;
-    if (t == 1023) {
-        *count = sh[1023];
-        *dgks += sh[1023];
-    }
-}
-
-/* Normalize and project on y: Bxn = f32(r * scale), ybx partial = sum Bxn y. */
-__global__ __launch_bounds__(256) void k_finalize(float *__restrict__ rbuf, size_t ldr, int P,
-                                                  const double *__restrict__ ssum, double tol,
-                                                  const float *__restrict__ y, size_t n,
-                                                  size_t rpg, int store,
+/*
+ * Normalize and project on y: Bxn = f32(r * scale), ybx partial = sum Bxn y.
+ * 256 threads = 4 row phases x 64 candidates: thread (g, slot) takes rows
+ * r0 + g, r0 + g + 4, ... (eight loads in flight), and the four phase sums
+ * are added in a fixed order.
+ */
+template <bool STORE>
+__global__ __launch_bounds__(256) void k_finalize(const FitDev *__restrict__ fits,
+                                                  const Item *__restrict__ items, int nitems,
+                                                  const double *__restrict__ ssum,
                                                   double *__restrict__ ypart)
 {
-    const int c = blockIdx.y * 256 + threadIdx.x;
-    if (c >= P) return;
-    const size_t g0 = (size_t)blockIdx.x * rpg;
-    const size_t g1 = g0 + rpg < n ? g0 + rpg : n;
-    const double s = ssum[c];
-    const float scale = (s > tol) ? (float)(1.0 / sqrt(s + tol)) : 0.0f;
-    double acc = 0.0;
-    for (size_t row = g0; row < g1; ++row) {
-        const float bxn = __fmul_rn(rbuf[row * ldr + c], scale);
-        if (store) rbuf[row * ldr + c] = bxn;
-        acc = fma((double)bxn, (double)y[row], acc);
+    __shared__ double sh[4][kTileF];
+    const int slot = threadIdx.x & (kTileF - 1), g = threadIdx.x / kTileF;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const Item I = items[it];
+        double acc = 0.0;
+        if (slot < I.nc) {
+            const FitDev F = fits[I.fit];
+            const int c = I.c0 + slot;
+            const long long col = c - F.c0;
+            const double s = ssum[c];
+            const float scale = (s > F.tol) ? (float)(1.0 / sqrt(s + F.tol)) : 0.0f;
+            float *base = F.rbuf + col;
+            long long row = I.r0 + g;
+            for (; row + 28 < I.r1; row += 32) {
+                float r[8], yv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    r[u] = base[(row + 4 * u) * F.ldr];
+                    yv[u] = F.y[row + 4 * u];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const float bxn = __fmul_rn(r[u], scale);
+                    if (STORE) base[(row + 4 * u) * F.ldr] = bxn;
+                    acc = fma((double)bxn, (double)yv[u], acc);
+                }
+            }
+            for (; row < I.r1; row += 4) {
+                const float bxn = __fmul_rn(base[row * F.ldr], scale);
+                if (STORE) base[row * F.ldr] = bxn;
+                acc = fma((double)bxn, (double)F.y[row], acc);
+            }
+        }
+        sh[g][slot] = acc;
+        __syncthreads();
+        if (g == 0 && slot < I.nc)
+            ypart[(size_t)it * kTileF + slot] = ((sh[0][slot] + sh[1][slot]) + sh[2][slot]) + sh[3][slot];
+        __syncthreads();
     }
-    ypart[(size_t)blockIdx.x * P + c] = acc;
 }
 
-__global__ void k_yreduce(const double *__restrict__ ypart, int groups, int P,
+__global__ void k_yreduce(int count, const int *__restrict__ cmap, const double *__restrict__ ypart,
                           double *__restrict__ ybx, double *__restrict__ lin)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= P) return;
+    if (c >= count) return;
+    const int first = cmap[3 * c], cnt = cmap[3 * c + 1], slot = cmap[3 * c + 2];
     double s = 0.0;
-    for (int g = 0; g < groups; ++g) s += ypart[(size_t)g * P + c];
-    ybx[c] = s;
+    for (int i = 0; i < cnt; ++i) s += ypart[(size_t)(first + i) * kTileF + slot];
+    if (ybx) ybx[c] = s;
     lin[c] = s * s;
 }
 
-template <bool RETRY>
-void launch_tgemm(mars_hip_ctx *ctx, const XSrc &xs, const PlainPlan &pl, const int *list,
-                  const int *count)
+/*
+ * Work list: for each fit, its (list) positions [b_f, e_f) cut into tiles of
+ * `tile`, each tile cut into row ranges of about `rows_f` rows (multiple of
+ * `align`).  cmap gets (first item, item count, slot) per position.
+ */
+Pass make_pass(const std::vector<int> &fb, const std::vector<int> &fe, const std::vector<long long> &n,
+               const std::vector<long long> &rows, int tile, long long align)
 {
-    const int zt = (pl.mt + 7) / 8;
-    dim3 grid((pl.P + 31) / 32, pl.ks, zt);
-    const int mt = std::min(pl.mt, 8);
-    auto args = [&](auto kern) {
-        hipLaunchKernelGGL(kern, grid, dim3(256), 0, ctx->stream, ctx->Bo.p, ctx->B.p, ctx->ldb,
-                           (int)ctx->m, ctx->n, xs.X, xs.ldx, xs.s, ctx->cand.xc.p,
-                           ctx->cand.par.p, pl.P, ctx->rbuf.p, pl.ldr, list, count, pl.rps,
-                           pl.ldT, ctx->tpart.p);
+    Pass p;
+    const int total = fe.empty() ? 0 : *std::max_element(fe.begin(), fe.end());
+    p.cmap.assign(3 * (size_t)std::max(total, 0), 0);
+    for (size_t f = 0; f < fb.size(); ++f) {
+        if (fe[f] <= fb[f]) continue;
+        long long r = std::max<long long>(align, (rows[f] + align - 1) / align * align);
+        for (int c0 = fb[f]; c0 < fe[f]; c0 += tile) {
+            const int nc = std::min(tile, fe[f] - c0);
+            const int first = (int)p.items.size();
+            for (long long r0 = 0; r0 < n[f]; r0 += r) {
+                Item it;
+                it.fit = (int)f;
+                it.c0 = c0;
+                it.nc = nc;
+                it.pad = 0;
+                it.r0 = r0;
+                it.r1 = std::min(n[f], r0 + r);
+                p.items.push_back(it);
+            }
+            const int cnt = (int)p.items.size() - first;
+            for (int c = c0; c < c0 + nc; ++c) {
+                p.cmap[3 * (size_t)c] = first;
+                p.cmap[3 * (size_t)c + 1] = cnt;
+                p.cmap[3 * (size_t)c + 2] = c - c0;
+            }
+        }
+    }
+    return p;
+}
+
+/* Rows per item so that the whole pass has roughly `target` items. */
+std::vector<long long> split_rows(const std::vector<long long> &n, const std::vector<int> &cnt,
+                                  int tile, double target, long long minrows)
+{
+    double work = 0;
+    for (size_t f = 0; f < n.size(); ++f) work += (double)n[f] * ((cnt[f] + tile - 1) / tile);
+    std::vector<long long> r(n.size());
+    for (size_t f = 0; f < n.size(); ++f) {
+        const double tiles = (double)((cnt[f] + tile - 1) / tile);
+        const double want = work > 0 ? target * n[f] * tiles / work / std::max(tiles, 1.0) : 1;
+        long long splits = std::max<long long>(1, (long long)want);
+        splits = std::min<long long>(splits, std::max<long long>(1, n[f] / minrows));
+        r[f] = (n[f] + splits - 1) / splits;
+    }
+    return r;
+}
+
+struct Uploaded {
+    Item *items = nullptr;
+    int *cmap = nullptr;
+};
+
+/* Upload items and cmap into a contiguous scratch region of ctx->d_items / d_cmap at offsets. */
+void upload_pass(mars_hip_ctx *ctx, const Pass &p, size_t item_off, size_t cmap_off, Uploaded &u)
+{
+    ctx->d_items.ensure(item_off + std::max<size_t>(p.items.size(), 1));
+    ctx->d_cmap.ensure(cmap_off + std::max<size_t>(p.cmap.size(), 1));
+    u.items = ctx->d_items.p + item_off;
+    u.cmap = ctx->d_cmap.p + cmap_off;
+    if (!p.items.empty())
+        MH_CHECK(hipMemcpyAsync(u.items, p.items.data(), p.items.size() * sizeof(Item),
+                                hipMemcpyHostToDevice, ctx->stream));
+    if (!p.cmap.empty())
+        MH_CHECK(hipMemcpyAsync(u.cmap, p.cmap.data(), p.cmap.size() * sizeof(int), hipMemcpyHostToDevice,
+                                ctx->stream));
+}
+
+int grid_for(size_t items)
+{
+    return (int)std::max<size_t>(1, std::min<size_t>(items, 4096));
+}
+
+template <bool RETRY>
+void launch_tgemm(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, const Cands &cs,
+                  const int *list, int ldT)
+{
+    if (nitems == 0) return;
+    const dim3 grid(grid_for(nitems)), block(256);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, grid, block, 0, ctx->stream, ctx->d_fits.p, u.items, nitems, cs.d_xc.p,
+                           cs.d_par.p, list, ldT, ctx->tpart.p, nullptr);
     };
     switch (mt) {
-    case 1: args(k_tgemm<1, RETRY>); break;
-    case 2: args(k_tgemm<2, RETRY>); break;
-    case 3: args(k_tgemm<3, RETRY>); break;
-    case 4: args(k_tgemm<4, RETRY>); break;
-    case 5: args(k_tgemm<5, RETRY>); break;
-    case 6: args(k_tgemm<6, RETRY>); break;
-    case 7: args(k_tgemm<7, RETRY>); break;
-    default: args(k_tgemm<8, RETRY>); break;
+    case 1: go(k_tgemm_lds<1, false, RETRY, false>); break;
+    case 2: go(k_tgemm_lds<2, false, RETRY, false>); break;
+    case 3: go(k_tgemm_lds<3, false, RETRY, false>); break;
+    case 4: go(k_tgemm_lds<4, false, RETRY, false>); break;
+    case 5: go(k_tgemm_lds<5, false, RETRY, false>); break;
+    case 6: go(k_tgemm_lds<6, false, RETRY, false>); break;
+    case 7: go(k_tgemm_lds<7, false, RETRY, false>); break;
+    case 8: go(k_tgemm_lds<8, false, RETRY, false>); break;
+    case 9: go(k_tgemm_lds<9, false, RETRY, false>); break;
+    case 10: go(k_tgemm_lds<10, false, RETRY, false>); break;
+    case 11: go(k_tgemm_lds<11, false, RETRY, false>); break;
+    case 12: go(k_tgemm_lds<12, false, RETRY, false>); break;
+    case 13: go(k_tgemm_lds<13, false, RETRY, false>); break;
+    case 14: go(k_tgemm_lds<14, false, RETRY, false>); break;
+    case 15: go(k_tgemm_lds<15, false, RETRY, false>); break;
+    default: go(k_tgemm_lds<16, false, RETRY, false>); break;
     }
     MH_LAUNCH_CHECK();
 }
 
 } // namespace
 
-size_t block_capacity(const mars_hip_ctx *ctx)
+/* Work list over the candidates of `cs`: tiles of `tile` candidates per fit,
+ * each cut into row ranges (multiple of `align`) so the pass has about
+ * `target` items. */
+Pass make_items(const Cands &cs, const mars_hip_ctx *ctx, int tile, double target, long long align)
 {
-    /* residual buffer budget: 8 GiB, at least one full X column of parents */
-    const size_t budget = (size_t)8 << 30;
-    size_t cap = budget / (sizeof(float) * std::max<size_t>(ctx->n, 1));
-    cap = cap / 64 * 64;
-    return std::max<size_t>(cap, ((ctx->max_terms + 63) / 64) * 64);
+    const size_t F = ctx->fits.size();
+    std::vector<long long> n(F);
+    std::vector<int> fb(F), fe(F), cnt(F);
+    for (size_t f = 0; f < F; ++f) {
+        n[f] = (long long)ctx->fits[f].n;
+        fb[f] = cs.fit_c0[f];
+        cnt[f] = cs.fit_P[f];
+        fe[f] = fb[f] + cnt[f];
+    }
+    return make_pass(fb, fe, n, split_rows(n, cnt, tile, target, std::max<long long>(align, 256)), tile, align);
 }
 
-void upload_cands(mars_hip_ctx *ctx)
+void index_cands(const mars_hip_ctx *ctx, Cands &cs)
 {
-    const size_t P = ctx->cand.h_xc.size();
-    ctx->cand.xc.ensure(P);
-    ctx->cand.par.ensure(P);
+    const size_t F = ctx->fits.size();
+    cs.fit_c0.assign(F, 0);
+    cs.fit_P.assign(F, 0);
+    for (size_t c = 0; c < cs.fit.size(); ++c) {
+        const int f = cs.fit[c];
+        if (cs.fit_P[f] == 0) cs.fit_c0[f] = (int)c;
+        cs.fit_P[f] += 1;
+    }
+    for (size_t f = 0; f < F; ++f)
+        if (cs.fit_P[f] == 0) cs.fit_c0[f] = (int)cs.fit.size();
+}
+
+void upload_cands(mars_hip_ctx *ctx, Cands &cs)
+{
+    const size_t P = cs.fit.size();
+    cs.d_fit.ensure(P);
+    cs.d_xc.ensure(P);
+    cs.d_par.ensure(P);
     if (!P) return;
-    MH_CHECK(hipMemcpyAsync(ctx->cand.xc.p, ctx->cand.h_xc.data(), P * sizeof(int),
-                            hipMemcpyHostToDevice, ctx->stream));
-    MH_CHECK(hipMemcpyAsync(ctx->cand.par.p, ctx->cand.h_par.data(), P * sizeof(int),
-                            hipMemcpyHostToDevice, ctx->stream));
+    MH_CHECK(hipMemcpyAsync(cs.d_fit.p, cs.fit.data(), P * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    MH_CHECK(hipMemcpyAsync(cs.d_xc.p, cs.xc.data(), P * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    MH_CHECK(hipMemcpyAsync(cs.d_par.p, cs.par.data(), P * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
 }
 
-PlainPlan plain_orthonormalize(mars_hip_ctx *ctx, const XSrc &xs, int P, bool keep_bxn)
+void ensure_rbuf(mars_hip_ctx *ctx, const Cands &cs)
 {
-    PlainPlan pl;
-    const size_t n = ctx->n;
-    pl.P = P;
-    pl.mt = std::max<int>(1, (int)((ctx->m + 15) / 16));
-    pl.ldT = pl.mt * 16;
-    const int ctiles = (P + 31) / 32;
-    pl.ks = std::max(1, std::min<int>((1024 + ctiles - 1) / ctiles, (int)std::max<size_t>(1, n / 256)));
-    pl.rps = ((n + pl.ks - 1) / pl.ks + 15) / 16 * 16;
-    pl.ks = (int)((n + pl.rps - 1) / pl.rps);
-    pl.rpg = std::max<size_t>(256, ((n + 1023) / 1024 + 63) / 64 * 64);
-    pl.groups = (int)((n + pl.rpg - 1) / pl.rpg);
-    pl.ldr = ((size_t)P + 63) / 64 * 64;
+    for (size_t f = 0; f < ctx->fits.size(); ++f) {
+        FitState &fs = ctx->fits[f];
+        const size_t need = ((size_t)std::max(cs.fit_P[f], 1) + 63) / 64 * 64;
+        if (need > fs.ldr) {
+            // grow geometrically in columns so the buffer is reallocated O(log) times per fit
+            const size_t ldr = std::max(need, std::min<size_t>(2 * fs.ldr, ctx->max_terms * fs.px + 64));
+            fs.rbuf.release();
+            fs.rbuf.ensure(fs.n * ldr);
+            fs.ldr = ldr;
+        }
+    }
+}
 
-    const int nparts = pl.ks * 4;
-    ctx->tpart.ensure((size_t)nparts * P * pl.ldT);
-    ctx->T.ensure((size_t)P * pl.ldT);
-    ctx->tnorm.ensure(P);
-    ctx->spart.ensure((size_t)pl.groups * P);
-    ctx->ssum.ensure(P);
-    ctx->ybx.ensure(P);
-    ctx->lin.ensure(P);
-    ctx->rbuf.ensure(n * pl.ldr);
-    ctx->fired_list.ensure(P + 1);
-    ctx->fired_count.ensure(2);
-    ctx->knot.ensure(P);   // reused as the DGKS flag array here
-    ctx->dgks.ensure(1);
+void upload_fits(mars_hip_ctx *ctx, const Cands &cs, const float *xover, const float *sover)
+{
+    std::vector<FitDev> h(ctx->fits.size());
+    for (size_t f = 0; f < ctx->fits.size(); ++f) {
+        const FitState &fs = ctx->fits[f];
+        FitDev &d = h[f];
+        d.X = fs.X.p;
+        d.s = fs.s.p;
+        d.ldx = (long long)fs.ldx;
+        if (f == 0 && xover) {
+            d.X = xover;
+            d.s = sover;
+        }
+        d.B = fs.B.p;
+        d.Bo = fs.Bo.p;
+        d.y = fs.y.p;
+        d.ybo = fs.ybo.p;
+        d.rbuf = fs.rbuf.p;
+        d.v = fs.v.p;
+        d.n = (long long)fs.n;
+        d.ldb = (long long)ctx->ldb;
+        d.ldr = (long long)fs.ldr;
+        d.m = (int)fs.m;
+        d.c0 = cs.fit_c0.empty() ? 0 : cs.fit_c0[f];
+        d.tol = fs.tol;
+    }
+    ctx->d_fits.ensure(h.size());
+    MH_CHECK(hipMemcpyAsync(ctx->d_fits.p, h.data(), h.size() * sizeof(FitDev), hipMemcpyHostToDevice,
+                            ctx->stream));
+}
 
+void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, double *ybx,
+                 const float *xover, const float *sover)
+{
+    const int P = (int)cs.fit.size();
+    if (P == 0) return;
     hipStream_t st = ctx->stream;
-    int *flag = ctx->knot.p;
+    const size_t F = ctx->fits.size();
+    index_cands(ctx, cs);
+    ensure_rbuf(ctx, cs);
+    upload_fits(ctx, cs, xover, sover);
+    upload_cands(ctx, cs);
 
-    const double m_live = (double)ctx->m, nd = (double)n;
+    size_t mmax = 1;
+    std::vector<long long> n(F);
+    std::vector<int> fb(F), fe(F), cnt(F);
+    double flops = 0, bytes = 0;
+    for (size_t f = 0; f < F; ++f) {
+        mmax = std::max(mmax, ctx->fits[f].m);
+        n[f] = (long long)ctx->fits[f].n;
+        fb[f] = cs.fit_c0[f];
+        cnt[f] = cs.fit_P[f];
+        fe[f] = fb[f] + cnt[f];
+        flops += 2.0 * n[f] * ctx->fits[f].m * cnt[f];
+        bytes += 4.0 * n[f] * (2.0 * ctx->fits[f].m + 1);
+    }
+    const int mt = (int)std::min<size_t>(16, (mmax + 15) / 16);
+    require(mmax <= 256, "the FP64 scoring path supports up to 256 basis functions");
+    const int ldT = mt * 16;
+
+    Pass pT = make_pass(fb, fe, n, split_rows(n, cnt, kTileT, 2048, 256), kTileT, kRC);
+    Pass pP = make_pass(fb, fe, n, split_rows(n, cnt, kTileP, 2048, 256), kTileP, 64);
+    Pass pF = make_pass(fb, fe, n, split_rows(n, cnt, kTileF, 2048, 512), kTileF, 32);
+    Uploaded uT, uP, uF;
+    size_t io = 0, co = 0;
+    const size_t total_items = pT.items.size() + pP.items.size() + pF.items.size();
+    const size_t total_cmap = pT.cmap.size() + pP.cmap.size() + pF.cmap.size();
+    ctx->d_items.ensure(total_items + 2 * (size_t)P + 16);
+    ctx->d_cmap.ensure(total_cmap + 6 * (size_t)P + 16);
+    upload_pass(ctx, pT, io, co, uT);
+    io += pT.items.size(); co += pT.cmap.size();
+    upload_pass(ctx, pP, io, co, uP);
+    io += pP.items.size(); co += pP.cmap.size();
+    upload_pass(ctx, pF, io, co, uF);
+    io += pF.items.size(); co += pF.cmap.size();
+
+    ctx->tpart.ensure(std::max<size_t>(pT.items.size(), 1) * 64 * ldT);
+    ctx->T.ensure((size_t)P * ldT);
+    ctx->tnorm.ensure(P);
+    ctx->spart.ensure(std::max(pP.items.size() * kTileP, pF.items.size() * kTileF) + 1);
+    ctx->ssum.ensure(P);
+    ctx->flag.ensure(P);
+    ctx->fired_list.ensure(P);
+
     // Pass 1: T = Bo^T Bx
     {
-        KTimer kt(ctx, K_TGEMM, 2.0 * nd * m_live * P, 4.0 * nd * (2 * m_live + 1));
-        launch_tgemm<false>(ctx, xs, pl, nullptr, nullptr);
-        hipLaunchKernelGGL(k_treduce, dim3(P), dim3(64), 0, st, ctx->tpart.p, nparts, P, pl.ldT,
-                           (int)ctx->m, nullptr, ctx->T.p, ctx->tnorm.p);
+        KTimer kt(ctx, K_TGEMM, flops, bytes);
+        launch_tgemm<false>(ctx, mt, uT, (int)pT.items.size(), cs, nullptr, ldT);
+        hipLaunchKernelGGL(k_treduce, dim3(P), dim3(64), 0, st, ctx->d_fits.p, cs.d_fit.p, nullptr, P,
+                           uT.cmap, ctx->tpart.p, ldT, ctx->T.p, ctx->tnorm.p);
         MH_LAUNCH_CHECK();
     }
-    // Pass 2: r = f32(Bx - Bo T), s
+    // Pass 2: r = f32(Bx - Bo T), s, DGKS gate
     {
-    KTimer kt(ctx, K_PROJECT, 2.0 * nd * m_live * P, 4.0 * nd * (2 * m_live + 1) + 4.0 * nd * P);
-    hipLaunchKernelGGL(k_project<false>, dim3(pl.groups, (P + 63) / 64), dim3(256), 0, st,
-                       ctx->Bo.p, ctx->B.p, ctx->ldb, (int)ctx->m, n, xs.X, xs.ldx, xs.s,
-                       ctx->cand.xc.p, ctx->cand.par.p, P, ctx->T.p, pl.ldT, ctx->rbuf.p, pl.ldr,
-                       nullptr, nullptr, pl.rpg, ctx->spart.p);
-    MH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_sreduce, dim3((P + 255) / 256), dim3(256), 0, st, ctx->spart.p,
-                       pl.groups, P, nullptr, nullptr, ctx->tnorm.p, ctx->tol, kGate, ctx->ssum.p,
-                       flag);
-    MH_LAUNCH_CHECK();
+        KTimer kt(ctx, K_PROJECT, flops, bytes + 0.0);
+        hipLaunchKernelGGL(k_project<false>, dim3(grid_for(pP.items.size())), dim3(256), 0, st,
+                           ctx->d_fits.p, uP.items, (int)pP.items.size(), cs.d_xc.p, cs.d_par.p,
+                           nullptr, ctx->T.p, ldT, ctx->spart.p);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_sreduce, dim3((P + 255) / 256), dim3(256), 0, st, nullptr, P, uP.cmap,
+                           ctx->spart.p, kTileP, ctx->tnorm.p, ctx->d_fits.p, cs.d_fit.p,
+                           ctx->ssum.p, ctx->flag.p);
+        MH_LAUNCH_CHECK();
     }
-    // DGKS: compact flagged columns, retry them (grids sized for P, idle beyond the count)
-    {
-    KTimer kt(ctx, K_DGKS, 0.0, 0.0);
-    hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, st, flag, P, ctx->fired_list.p,
-                       ctx->fired_count.p, ctx->dgks.p);
-    MH_LAUNCH_CHECK();
-    launch_tgemm<true>(ctx, xs, pl, ctx->fired_list.p, ctx->fired_count.p);
-    hipLaunchKernelGGL(k_treduce, dim3(P), dim3(64), 0, st, ctx->tpart.p, nparts, P, pl.ldT,
-                       (int)ctx->m, ctx->fired_count.p, ctx->T.p, nullptr);
-    MH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_project<true>, dim3(pl.groups, (P + 63) / 64), dim3(256), 0, st,
-                       ctx->Bo.p, ctx->B.p, ctx->ldb, (int)ctx->m, n, xs.X, xs.ldx, xs.s,
-                       ctx->cand.xc.p, ctx->cand.par.p, P, ctx->T.p, pl.ldT, ctx->rbuf.p, pl.ldr,
-                       ctx->fired_list.p, ctx->fired_count.p, pl.rpg, ctx->spart.p);
-    MH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_sreduce, dim3((P + 255) / 256), dim3(256), 0, st, ctx->spart.p,
-                       pl.groups, P, ctx->fired_list.p, ctx->fired_count.p, nullptr, ctx->tol,
-                       kGate, ctx->ssum.p, nullptr);
-    MH_LAUNCH_CHECK();
+    // DGKS retry of the flagged columns, grouped by fit
+    std::vector<int> flags(P);
+    MH_CHECK(hipMemcpyAsync(flags.data(), ctx->flag.p, P * sizeof(int), hipMemcpyDeviceToHost, st));
+    MH_CHECK(hipStreamSynchronize(st));
+    std::vector<int> fired;
+    std::vector<int> rb(F, 0), re(F, 0);
+    for (size_t f = 0; f < F; ++f) {
+        rb[f] = (int)fired.size();
+        for (int c = fb[f]; c < fe[f]; ++c)
+            if (flags[c]) fired.push_back(c);
+        re[f] = (int)fired.size();
+    }
+    if (!fired.empty()) {
+        const int nf = (int)fired.size();
+        ctx->dgks_host += nf;
+        KTimer kt(ctx, K_DGKS, 0.0, 0.0);
+        MH_CHECK(hipMemcpyAsync(ctx->fired_list.p, fired.data(), nf * sizeof(int), hipMemcpyHostToDevice, st));
+        std::vector<int> fcnt(F);
+        for (size_t f = 0; f < F; ++f) fcnt[f] = re[f] - rb[f];
+        Pass rT = make_pass(rb, re, n, split_rows(n, fcnt, kTileT, 1024, 256), kTileT, kRC);
+        Pass rP = make_pass(rb, re, n, split_rows(n, fcnt, kTileP, 1024, 256), kTileP, 64);
+        Uploaded vT, vP;
+        ctx->d_items.ensure(io + rT.items.size() + rP.items.size() + 2);
+        ctx->d_cmap.ensure(co + rT.cmap.size() + rP.cmap.size() + 2);
+        upload_pass(ctx, rT, io, co, vT);
+        upload_pass(ctx, rP, io + rT.items.size(), co + rT.cmap.size(), vP);
+        ctx->tpart.ensure(std::max<size_t>(rT.items.size(), 1) * 64 * ldT);
+        ctx->spart.ensure(rP.items.size() * kTileP + 1);
+        launch_tgemm<true>(ctx, mt, vT, (int)rT.items.size(), cs, ctx->fired_list.p, ldT);
+        hipLaunchKernelGGL(k_treduce, dim3(nf), dim3(64), 0, st, ctx->d_fits.p, cs.d_fit.p,
+                           ctx->fired_list.p, nf, vT.cmap, ctx->tpart.p, ldT, ctx->T.p, nullptr);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_project<true>, dim3(grid_for(rP.items.size())), dim3(256), 0, st,
+                           ctx->d_fits.p, vP.items, (int)rP.items.size(), cs.d_xc.p, cs.d_par.p,
+                           ctx->fired_list.p, ctx->T.p, ldT, ctx->spart.p);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_sreduce, dim3((nf + 255) / 256), dim3(256), 0, st, ctx->fired_list.p, nf,
+                           vP.cmap, ctx->spart.p, kTileP, nullptr, ctx->d_fits.p, cs.d_fit.p,
+                           ctx->ssum.p, nullptr);
+        MH_LAUNCH_CHECK();
     }
     // Pass 3: normalize, ybx
     {
-    KTimer kt(ctx, K_FINALIZE, 2.0 * nd * P, (keep_bxn ? 8.0 : 4.0) * nd * P + 4.0 * nd);
-    hipLaunchKernelGGL(k_finalize, dim3(pl.groups, (P + 255) / 256), dim3(256), 0, st,
-                       ctx->rbuf.p, pl.ldr, P, ctx->ssum.p, ctx->tol, ctx->y.p, n, pl.rpg,
-                       keep_bxn ? 1 : 0, ctx->spart.p);
-    MH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, ctx->spart.p,
-                       pl.groups, P, ctx->ybx.p, ctx->lin.p);
-    MH_LAUNCH_CHECK();
+        double fb2 = 0;
+        for (size_t f = 0; f < F; ++f) fb2 += (keep_bxn ? 8.0 : 4.0) * n[f] * cnt[f] + 4.0 * n[f];
+        KTimer kt(ctx, K_FINALIZE, 0.0, fb2);
+        if (keep_bxn)
+            hipLaunchKernelGGL(k_finalize<true>, dim3(grid_for(pF.items.size())), dim3(256), 0, st,
+                               ctx->d_fits.p, uF.items, (int)pF.items.size(), ctx->ssum.p, ctx->spart.p);
+        else
+            hipLaunchKernelGGL(k_finalize<false>, dim3(grid_for(pF.items.size())), dim3(256), 0, st,
+                               ctx->d_fits.p, uF.items, (int)pF.items.size(), ctx->ssum.p, ctx->spart.p);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, P, uF.cmap, ctx->spart.p,
+                           ybx, lin);
+        MH_LAUNCH_CHECK();
     }
-    return pl;
 }
 
 } // namespace mh

@@ -248,3 +248,58 @@ def expand(X, model):
         else:
             raise ValueError("unknown basis type %r" % t)
     return B
+
+
+def fit_batch(Xs, ys, ws=None, **kwargs):
+    """
+    Batching form: fit many independent problems (each its own X, y, w of any
+    N and p) with one batched scoring pass, one device argmax and one device
+    append per forward step for all of them (driver restated in C++).
+
+    Supported keywords (same meaning as `fit`): max_terms (shared by all fits;
+    default: the reference default of the largest problem), max_degree,
+    penalty, self_interact, r2_window, r2_thresh, precision ("fp64"/"mixed"),
+    linear_only (must be True: the batch scores linear terms), stats (a dict
+    that receives step count, DGKS retries and per-kernel timing when given).
+    Returns one model array per problem, as `fit` would.
+    """
+    Xs = [np.asarray(X) for X in Xs]
+    ys = [np.ascontiguousarray(np.asarray(y).squeeze(), dtype="f") for y in ys]
+    if ws is not None:
+        ws = [np.ascontiguousarray(np.asarray(w).squeeze(), dtype="f") for w in ws]
+    for X in Xs:
+        assert X.dtype == "f", "X data must be 32-bit float"
+        assert X.strides[0] == X.itemsize, "X data must be column-major"
+    default_terms = max(1 + 2 * min(X.shape[1] + len(X) // 20, 15) for X in Xs)
+    max_terms = kwargs.pop("max_terms", default_terms)
+    max_degree = kwargs.pop("max_degree", 3)
+    penalty = kwargs.pop("penalty", 3.0)
+    self_interact = kwargs.pop("self_interact", False)
+    r2_window = kwargs.pop("r2_window", 32)
+    r2_thresh = kwargs.pop("r2_thresh", 3e-5)
+    precision = kwargs.pop("precision", "fp64")
+    linear_only = kwargs.pop("linear_only", True)
+    stats = kwargs.pop("stats", None)
+    timing = kwargs.pop("timing", False)
+    if kwargs:
+        raise TypeError("unknown argument: %s" % kwargs)
+    if not linear_only:
+        raise ValueError("fit_batch scores linear_only fits; use fit() for hinge sweeps")
+    batch = marslib.Batch(Xs, ys, ws, int(max_terms), precision=precision)
+    batch.set_timing(bool(timing))
+    t0 = time.perf_counter()
+    rows, steps = batch.fit(int(max_degree), float(penalty), bool(self_interact), int(r2_window), float(r2_thresh))
+    dt = time.perf_counter() - t0
+    models = []
+    for r in rows:
+        model = np.zeros(len(r), dtype=MODEL_DTYPE)
+        for i, (t, b, x, o, mse, r2, r2cv) in enumerate(r):
+            model[i] = (t, b, x, np.nan, np.nan, r2, r2cv, o, 0.0)
+        models.append(model)
+    if stats is not None:
+        stats.update(batch.stats())
+        stats["steps"] = steps
+        stats["seconds"] = dt
+        stats["mse"] = [[row[4] for row in r[1:]] for r in rows]
+        stats["kernels"] = batch.kernel_stats()
+    return models

@@ -225,3 +225,62 @@ def test_append_basis_vs_oracle(oracle):
     assert np.mean(B1 == B0) > 0.999 and np.abs(B1 - B0).max() < 1e-6
     assert np.abs(Bo1 - Bo0).max() < 1e-6
     assert abs(dev.yvar() - cpu.yvar()) <= 1e-13 * cpu.yvar()
+
+
+def test_batch_fits_match_reference():
+    """Batching form: 8 independent workload-4 fits in one batch reproduce each reference fit."""
+    import mars_amd
+
+    say("batch parity")
+    golds = [json.load(open(os.path.join(HERE, "golden", "w4_fit%d.json" % i))) for i in range(8)]
+    inputs = [make_inputs(CASES[g["name"]]) for g in golds]
+    stats = {}
+    models = mars_amd.fit_batch([x for x, _, _ in inputs], [y for _, y, _ in inputs], max_terms=33,
+                                stats=stats)
+    for gold, model, mse in zip(golds, models, stats["mse"]):
+        got = [(r["type"].decode(), int(r["basis"]), int(r["input"])) for r in model]
+        want = [(r["type"], r["basis"], r["input"]) for r in gold["model"]]
+        assert got == want, gold["name"]
+        want_mse = [a["mse"] for a in gold["appends"]]
+        assert len(mse) == len(want_mse)
+        for a, b in zip(mse, want_mse):
+            assert abs(a - b) <= 1e-10 * abs(b), (gold["name"], a, b)
+
+
+def test_batch_of_one_equals_single_fit():
+    import mars_amd
+
+    say("batch of one")
+    X, y, _ = make_inputs(CASES["w5_small"])
+    single = mars_amd.fit(X, y, max_terms=41, linear_only=True)
+    (batch,) = mars_amd.fit_batch([X], [y], max_terms=41)
+    assert [tuple(r)[:3] for r in single] == [tuple(r)[:3] for r in batch]
+    np.testing.assert_array_equal(single["r2"], batch["r2"])
+
+
+@pytest.mark.parametrize("case", ["w1", "w5_small", "w4_fit2"])
+def test_mixed_form_matches_reference(case):
+    """Mixed-precision form: same term sequence and per-step mse as the reference."""
+    path = os.path.join(HERE, "golden", case + ".json")
+    gold = json.load(open(path))
+    say("mixed fit", case)
+    model, trace = _fit(CASES[case], precision="mixed")
+    _compare_to_golden(gold, model, trace, 1e-10)
+
+
+def test_batch_mixed_matches_reference():
+    import mars_amd
+
+    say("batch mixed")
+    golds = [json.load(open(os.path.join(HERE, "golden", "w4_fit%d.json" % i))) for i in range(8)]
+    inputs = [make_inputs(CASES[g["name"]]) for g in golds]
+    stats = {}
+    models = mars_amd.fit_batch([x for x, _, _ in inputs], [y for _, y, _ in inputs], max_terms=33,
+                                precision="mixed", stats=stats)
+    for gold, model, mse in zip(golds, models, stats["mse"]):
+        got = [(r["type"].decode(), int(r["basis"]), int(r["input"])) for r in model]
+        want = [(r["type"], r["basis"], r["input"]) for r in gold["model"]]
+        assert got == want, gold["name"]
+        for a, b in zip(mse, [a["mse"] for a in gold["appends"]]):
+            assert abs(a - b) <= 1e-10 * abs(b)
+    assert 0 < stats["mixed_rescored"] < stats["mixed_scored"]
