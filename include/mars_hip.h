@@ -174,8 +174,9 @@ int mars_hip_batch_create(int nfits, const float *const *X, const size_t *ldX, c
 int mars_hip_batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts *opts, mars_hip_term *terms, int *nterms,
                        int *steps);
 
-/* Mixed form: candidates scored in the bulk pass and those rescored exactly in FP64. */
-int mars_hip_mixed_stats(mars_hip_ctx *ctx, long *scored, long *rescored);
+/* Candidates scored since creation (every form), and those the mixed form rescored
+ * exactly in FP64 (0 for the plain form). */
+int mars_hip_score_stats(mars_hip_ctx *ctx, long *scored, long *rescored);
 
 /* Per-kernel-class timing with HIP events on the context's stream (off by
  * default).  Each record: launches, summed device milliseconds and the

@@ -90,6 +90,7 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
         ++step;
         index_cands(ctx, cs);
         const size_t P = cs.fit.size();
+        ctx->scored += (long)P;
         ctx->lin.ensure(std::max<size_t>(P, 1));
         ctx->ybx.ensure(std::max<size_t>(P, 1));
         if (P) {

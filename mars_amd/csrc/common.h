@@ -64,10 +64,13 @@ struct FitDev {
     const float *s;    // per-column 1/rms scale
     const float *B;    // basis, row-major (ldb)
     const float *Bo;   // orthonormal basis, row-major (ldb)
+    const float *BoC;  // the same basis, column-major (ldx)
     const float *y;    // normalized target (f32)
     const double *ybo; // Bo^T y per live column
     float *rbuf;       // residual / normalized candidate columns, row-major (ldr)
     double *v;         // append: f64 residual of the new column (n)
+    double *tc;        // mixed form: cached T, [xc][par][k] (ldb x ldb per X column)
+    double *tcn;       // mixed form: cached |Bx|^2, Bx.y per (xc, par)
     long long n, ldx, ldb, ldr;
     int m, c0;         // live basis count, first candidate of this fit
     double tol;
@@ -80,9 +83,12 @@ struct FitState {
     double tol = 0, yvar = 0;
     size_t m = 0;
     std::vector<double> ybo_h;
-    DBuf<float> X, s, B, Bo, y, rbuf;
+    DBuf<float> X, s, B, Bo, BoC, y, rbuf;
     DBuf<double> ybo, v;
     size_t ldr = 0;
+    // mixed form: T cache valid for basis columns and parents < mc
+    DBuf<double> tc, tcn;
+    size_t mc = 0;
     // hinge sort cache
     DBuf<int> order;
     DBuf<float> delta;
@@ -93,8 +99,8 @@ struct FitState {
     std::vector<char> xres;
     void release()
     {
-        X.release(); s.release(); B.release(); Bo.release(); y.release(); rbuf.release();
-        ybo.release(); v.release(); order.release(); delta.release();
+        X.release(); s.release(); B.release(); Bo.release(); BoC.release(); y.release(); rbuf.release();
+        ybo.release(); v.release(); order.release(); delta.release(); tc.release(); tcn.release();
     }
 };
 
@@ -102,6 +108,12 @@ struct FitState {
 struct Item {
     int fit, c0, nc, pad;
     long long r0, r1;
+};
+
+/* Writable basis storage of one fit (append). */
+struct FitW {
+    float *B, *Bo, *BoC;
+    double *ybo;
 };
 
 /* Result slot returned by the device append. */
@@ -132,8 +144,8 @@ struct BestOut {
 enum KClass { K_TGEMM = 0, K_PROJECT, K_FINALIZE, K_DGKS, K_SWEEP, K_SORT, K_APPEND, K_MIXED_A,
               K_MIXED_B, K_RECHECK, K_ARGMAX, K_NCLASS };
 constexpr const char *kClassName[K_NCLASS] = {"tgemm_f64", "project_f64", "finalize", "dgks_retry",
-                                             "hinge_sweep", "sort", "append", "mixed_pass_a",
-                                             "mixed_pass_b", "mixed_recheck", "argmax"};
+                                             "hinge_sweep", "sort", "append", "mixed_update",
+                                             "mixed_score", "mixed_recheck", "argmax"};
 
 struct KAcc {
     long launches = 0;
@@ -169,6 +181,8 @@ struct mars_hip_ctx {
 
     // scoring workspace (shared by all fits)
     mh::Cands cand;
+    mh::Cands rcand;                 // mixed form: candidates rescored exactly
+    mh::DBuf<double> rlin, rybx;
     mh::DBuf<mh::FitDev> d_fits;
     mh::DBuf<mh::Item> d_items;
     mh::DBuf<int> d_cmap;            // per-candidate first item / item count per pass
@@ -180,6 +194,7 @@ struct mars_hip_ctx {
     mh::DBuf<double> scal;           // append / init scalars
     mh::DBuf<mh::AppendOut> aout;
     mh::DBuf<mh::AppendArg> aarg;
+    mh::DBuf<mh::FitW> d_fitw;       // per-fit writable basis pointers
     mh::DBuf<mh::BestOut> bout;
     mh::DBuf<double> dense;          // (p_x, m) x 3 result arrays
     mh::DBuf<int> dknot;

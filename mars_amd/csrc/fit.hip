@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void k_init1(const float *__restrict__ y, cons
 __global__ __launch_bounds__(256) void k_init2(double *__restrict__ yd, const double *__restrict__ sw,
                                                size_t n, const double *__restrict__ part,
                                                float *__restrict__ y32, float *__restrict__ B,
-                                               float *__restrict__ Bo, size_t ldb,
+                                               float *__restrict__ Bo, size_t ldb, float *__restrict__ BoC,
                                                double *__restrict__ part2, double *__restrict__ scal)
 {
     __shared__ double sh[256];
@@ -69,6 +69,7 @@ __global__ __launch_bounds__(256) void k_init2(double *__restrict__ yd, const do
             y32[i] = yf;
             B[i * ldb] = b;
             Bo[i * ldb] = b;
+            BoC[i] = b;
             ab = fma((double)b, (double)yf, ab);
             am += d;
         }
@@ -191,7 +192,7 @@ __global__ __launch_bounds__(256) void k_app_mgs(const FitDev *__restrict__ fits
                                                  const double *__restrict__ prev,
                                                  double *__restrict__ next, int j, int first,
                                                  const int *__restrict__ gate,
-                                                 double *__restrict__ scal, float *const *__restrict__ Bw)
+                                                 double *__restrict__ scal, const FitW *__restrict__ W)
 {
     __shared__ double sh[256];
     const int f = blockIdx.y;
@@ -204,17 +205,17 @@ __global__ __launch_bounds__(256) void k_app_mgs(const FitDev *__restrict__ fits
     float rn = 1.0f;
     if (j > 0) c = sum_partials(prev + (size_t)f * nb, nb, sh);
     if (j == 0 && first) rn = (float)sqrt(sum_partials(prev + (size_t)f * nb, nb, sh));
-    float *B = Bw[f];
+    float *B = W[f].B;
     double a = 0.0;
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < F.n; i += (long long)nb * 256) {
         double vi = F.v[i];
         if (j == 0 && first) B[i * F.ldb + m] = __fdiv_rn((float)vi, rn);
         if (j > 0) {
-            vi = fma(-c, (double)F.Bo[i * F.ldb + (j - 1)], vi);
+            vi = fma(-c, (double)F.BoC[(j - 1) * F.ldx + i], vi);
             F.v[i] = vi;
         }
         if (j < m)
-            a = fma((double)F.Bo[i * F.ldb + j], vi, a);
+            a = fma((double)F.BoC[j * F.ldx + i], vi, a);
         else
             a = fma(vi, vi, a);
     }
@@ -254,7 +255,7 @@ __global__ __launch_bounds__(256) void k_app_gate(const FitDev *__restrict__ fit
 
 __global__ __launch_bounds__(256) void k_app_store(const FitDev *__restrict__ fits,
                                                    const AppendArg *__restrict__ args, int nb,
-                                                   float *const *__restrict__ Bow,
+                                                   const FitW *__restrict__ W,
                                                    const double *__restrict__ scal,
                                                    double *__restrict__ part)
 {
@@ -264,12 +265,13 @@ __global__ __launch_bounds__(256) void k_app_store(const FitDev *__restrict__ fi
     const FitDev F = fits[f];
     const double w = sqrt(scal[f * S_COUNT + S_VV]);
     const bool ok = w * w > F.tol;
-    float *Bo = Bow[f];
+    float *Bo = W[f].Bo, *BoC = W[f].BoC + (size_t)F.m * F.ldx;
     double a = 0.0;
     if (ok) {
         for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < F.n; i += (long long)nb * 256) {
             const float b = (float)(F.v[i] / w);
             Bo[i * F.ldb + F.m] = b;
+            BoC[i] = b;
             a = fma((double)b, (double)F.y[i], a);
         }
     }
@@ -281,7 +283,7 @@ __global__ __launch_bounds__(256) void k_app_final(const FitDev *__restrict__ fi
                                                    const AppendArg *__restrict__ args, int nb,
                                                    const double *__restrict__ part,
                                                    const double *__restrict__ scal,
-                                                   double *const *__restrict__ ybow,
+                                                   const FitW *__restrict__ W,
                                                    const int *__restrict__ gate,
                                                    AppendOut *__restrict__ out)
 {
@@ -292,7 +294,7 @@ __global__ __launch_bounds__(256) void k_app_final(const FitDev *__restrict__ fi
     const double yb = sum_partials(part + (size_t)f * nb, nb, sh);
     if (threadIdx.x != 0) return;
     const double w = sqrt(scal[f * S_COUNT + S_VV]);
-    double *ybo = ybow[f];
+    double *ybo = W[f].ybo;
     AppendOut o;
     o.fired = gate[f];
     o.ybo_m = yb;
@@ -437,6 +439,8 @@ void fit_state_alloc(mars_hip_ctx *ctx, FitState &fs, size_t n, double tol)
     MH_CHECK(hipMemsetAsync(fs.B.p, 0, n * ctx->ldb * sizeof(float), ctx->stream));
     MH_CHECK(hipMemsetAsync(fs.Bo.p, 0, n * ctx->ldb * sizeof(float), ctx->stream));
     fs.ldx = (n + 31) / 32 * 32;
+    fs.BoC.ensure(fs.ldx * ctx->ldb);
+    MH_CHECK(hipMemsetAsync(fs.BoC.p, 0, fs.ldx * ctx->ldb * sizeof(float), ctx->stream));
     fs.y.ensure(fs.ldx + 32);
     MH_CHECK(hipMemsetAsync(fs.y.p, 0, (fs.ldx + 32) * sizeof(float), ctx->stream));
     fs.ybo.ensure(ctx->max_terms);
@@ -450,17 +454,19 @@ void ctx_free(mars_hip_ctx *ctx)
     for (auto &fs : ctx->fits) fs.release();
     DBuf<double> *db[] = {&ctx->tpart, &ctx->T, &ctx->tnorm, &ctx->spart, &ctx->ssum, &ctx->ybx,
                           &ctx->lin, &ctx->hin, &ctx->aux, &ctx->red, &ctx->vcol, &ctx->scal,
-                          &ctx->dense, &ctx->hstate, &ctx->hbest};
+                          &ctx->dense, &ctx->hstate, &ctx->hbest, &ctx->rlin, &ctx->rybx};
     for (auto *b : db) b->release();
     DBuf<int> *ib[] = {&ctx->flag, &ctx->fired_list, &ctx->fired_count, &ctx->knot, &ctx->krow,
                        &ctx->ival, &ctx->d_cmap, &ctx->dknot, &ctx->hbest_i, &ctx->cand.d_fit,
-                       &ctx->cand.d_xc, &ctx->cand.d_par};
+                       &ctx->cand.d_xc, &ctx->cand.d_par, &ctx->rcand.d_fit, &ctx->rcand.d_xc,
+                       &ctx->rcand.d_par};
     for (auto *b : ib) b->release();
     ctx->d_fits.release();
     ctx->d_items.release();
     ctx->dgks.release();
     ctx->aout.release();
     ctx->aarg.release();
+    ctx->d_fitw.release();
     ctx->bout.release();
     for (auto &r : ctx->pend) {
         (void)hipEventDestroy(r.a);
@@ -502,7 +508,7 @@ void fit_init(mars_hip_ctx *ctx, FitState &fs, const float *X, size_t ldX, const
     hipLaunchKernelGGL(k_init1, dim3(kInitBlocks), dim3(256), 0, st, yw.p, yw.p + n, n, yd, sw, p1);
     MH_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_init2, dim3(kInitBlocks), dim3(256), 0, st, yd, sw, n, p1, fs.y.p, fs.B.p, fs.Bo.p,
-                       ctx->ldb, p2, scal.p);
+                       ctx->ldb, fs.BoC.p, p2, scal.p);
     MH_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_init3, dim3(kInitBlocks), dim3(256), 0, st, yd, n, p2, p1, scal.p);
     MH_LAUNCH_CHECK();
@@ -567,21 +573,13 @@ void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std
     ctx->scal.ensure(F * S_COUNT);
     ctx->flag.ensure(F);
     MH_CHECK(hipMemcpyAsync(ctx->aarg.p, args.data(), F * sizeof(AppendArg), hipMemcpyHostToDevice, st));
-    std::vector<float *> bw(F), bow(F);
-    std::vector<double *> yw(F);
+    std::vector<FitW> fw(F);
     for (size_t f = 0; f < F; ++f) {
-        bw[f] = ctx->fits[f].B.p;
-        bow[f] = ctx->fits[f].Bo.p;
-        yw[f] = ctx->fits[f].ybo.p;
+        FitState &fs = ctx->fits[f];
+        fw[f] = FitW{fs.B.p, fs.Bo.p, fs.BoC.p, fs.ybo.p};
     }
-    DBuf<float *> dbw, dbow;
-    DBuf<double *> dyw;
-    dbw.ensure(F);
-    dbow.ensure(F);
-    dyw.ensure(F);
-    MH_CHECK(hipMemcpyAsync(dbw.p, bw.data(), F * sizeof(float *), hipMemcpyHostToDevice, st));
-    MH_CHECK(hipMemcpyAsync(dbow.p, bow.data(), F * sizeof(float *), hipMemcpyHostToDevice, st));
-    MH_CHECK(hipMemcpyAsync(dyw.p, yw.data(), F * sizeof(double *), hipMemcpyHostToDevice, st));
+    ctx->d_fitw.ensure(F);
+    MH_CHECK(hipMemcpyAsync(ctx->d_fitw.p, fw.data(), F * sizeof(FitW), hipMemcpyHostToDevice, st));
     double *pa = ctx->red.p, *pb = pa + F * nb, *pc = pb + F * nb;
     double *pp[2] = {pa, pb};
     int *gate = ctx->flag.p;
@@ -600,25 +598,22 @@ void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std
             for (size_t j = 0; j <= mmax; ++j) {
                 hipLaunchKernelGGL(k_app_mgs, grid, dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, pp[j % 2],
                                    pp[(j + 1) % 2], (int)j, pass == 0 ? 1 : 0, pass == 0 ? nullptr : gate,
-                                   ctx->scal.p, dbw.p);
+                                   ctx->scal.p, ctx->d_fitw.p);
                 MH_LAUNCH_CHECK();
             }
             hipLaunchKernelGGL(k_app_gate, dim3(F), dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, pa, pb,
                                gate, ctx->scal.p, ctx->dgks.p, pass);
             MH_LAUNCH_CHECK();
         }
-        hipLaunchKernelGGL(k_app_store, grid, dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, dbow.p,
+        hipLaunchKernelGGL(k_app_store, grid, dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, ctx->d_fitw.p,
                            ctx->scal.p, pc);
         MH_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_app_final, dim3(F), dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, pc,
-                           ctx->scal.p, dyw.p, gate, ctx->aout.p);
+                           ctx->scal.p, ctx->d_fitw.p, gate, ctx->aout.p);
         MH_LAUNCH_CHECK();
     }
     MH_CHECK(hipMemcpyAsync(out.data(), ctx->aout.p, F * sizeof(AppendOut), hipMemcpyDeviceToHost, st));
     MH_CHECK(hipStreamSynchronize(st));
-    dbw.release();
-    dbow.release();
-    dyw.release();
     resolve_timing(ctx);
     for (size_t f = 0; f < F; ++f) {
         if (!args[f].active) continue;
@@ -703,6 +698,7 @@ void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t 
         MH_LAUNCH_CHECK();
     }
     MH_CHECK(hipEventRecord(ctx->ev0, st));
+    ctx->scored += P;
     if (P > 0) {
         ctx->lin.ensure(P);
         ctx->ybx.ensure(P);
@@ -777,6 +773,7 @@ void mirror_sync_basis(mars_hip_ctx *ctx, size_t m, const float *B, size_t ldB, 
             hipLaunchKernelGGL(k_put_bcol, dim3((n + 255) / 256), dim3(256), 0, st, fs.Bo.p, ctx->ldb, (int)j, col.p,
                                n);
             MH_LAUNCH_CHECK();
+            MH_CHECK(hipMemcpyAsync(fs.BoC.p + j * fs.ldx, col.p, n * sizeof(float), hipMemcpyDeviceToDevice, st));
             MH_CHECK(hipStreamSynchronize(st));
         }
         col.release();
@@ -1128,7 +1125,7 @@ int mars_hip_kernel_stats(mars_hip_ctx *ctx, mars_hip_kstat *out, int capacity, 
     });
 }
 
-int mars_hip_mixed_stats(mars_hip_ctx *ctx, long *scored, long *rescored)
+int mars_hip_score_stats(mars_hip_ctx *ctx, long *scored, long *rescored)
 {
     return guarded([&] {
         require(ctx != nullptr, "null context");

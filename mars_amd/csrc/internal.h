@@ -22,9 +22,10 @@ struct Pass {
 void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, double *ybx,
                  const float *xover = nullptr, const float *sover = nullptr);
 
-/* Mixed-precision scoring of ctx->cand (linear delta-SSE only): BF16-split
- * MFMA projections with an FP64-accumulated refinement, then an exact FP64
- * rescoring of the flagged and near-winning candidates.  Writes ctx->lin. */
+/* Mixed-precision scoring of ctx->cand (linear delta-SSE only) from each
+ * fit's incrementally maintained T cache (FP32 MFMA, FP64 across row chunks),
+ * then an exact FP64 rescoring of the DGKS-gated and near-winning
+ * candidates.  Writes ctx->lin. */
 void mixed_score(mars_hip_ctx *ctx);
 
 /* Hinge sweep of fit 0's candidates, whose normalized columns sit in its rbuf. */

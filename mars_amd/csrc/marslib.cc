@@ -159,12 +159,12 @@ public:
         double ms = 0;
         check(mars_hip_stats(ctx_, &fired, &ms));
         long sc = 0, re = 0;
-        check(mars_hip_mixed_stats(ctx_, &sc, &re));
+        check(mars_hip_score_stats(ctx_, &sc, &re));
         py::dict d;
         d["dgks_fired"] = fired;
         d["eval_ms"] = ms;
-        d["mixed_scored"] = sc;
-        d["mixed_rescored"] = re;
+        d["scored"] = sc;
+        d["rescored"] = re;
         return d;
     }
 
@@ -294,12 +294,12 @@ public:
         double ms = 0;
         check(mars_hip_stats(ctx_, &fired, &ms));
         long sc = 0, re = 0;
-        check(mars_hip_mixed_stats(ctx_, &sc, &re));
+        check(mars_hip_score_stats(ctx_, &sc, &re));
         py::dict d;
         d["dgks_fired"] = fired;
         d["eval_ms"] = ms;
-        d["mixed_scored"] = sc;
-        d["mixed_rescored"] = re;
+        d["scored"] = sc;
+        d["rescored"] = re;
         return d;
     }
 

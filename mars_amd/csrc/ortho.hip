@@ -415,10 +415,13 @@ void upload_fits(mars_hip_ctx *ctx, const Cands &cs, const float *xover, const f
         }
         d.B = fs.B.p;
         d.Bo = fs.Bo.p;
+        d.BoC = fs.BoC.p;
         d.y = fs.y.p;
         d.ybo = fs.ybo.p;
         d.rbuf = fs.rbuf.p;
         d.v = fs.v.p;
+        d.tc = fs.tc.p;
+        d.tcn = fs.tcn.p;
         d.n = (long long)fs.n;
         d.ldb = (long long)ctx->ldb;
         d.ldr = (long long)fs.ldr;

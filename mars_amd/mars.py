@@ -260,7 +260,9 @@ def fit_batch(Xs, ys, ws=None, **kwargs):
     default: the reference default of the largest problem), max_degree,
     penalty, self_interact, r2_window, r2_thresh, precision ("fp64"/"mixed"),
     linear_only (must be True: the batch scores linear terms), stats (a dict
-    that receives step count, DGKS retries and per-kernel timing when given).
+    that receives step count, DGKS retries and per-kernel timing when given),
+    batch (a marslib.Batch already built on these same inputs, max_terms and
+    precision, so that device upload happens outside the call).
     Returns one model array per problem, as `fit` would.
     """
     Xs = [np.asarray(X) for X in Xs]
@@ -281,11 +283,13 @@ def fit_batch(Xs, ys, ws=None, **kwargs):
     linear_only = kwargs.pop("linear_only", True)
     stats = kwargs.pop("stats", None)
     timing = kwargs.pop("timing", False)
+    batch = kwargs.pop("batch", None)
     if kwargs:
         raise TypeError("unknown argument: %s" % kwargs)
     if not linear_only:
         raise ValueError("fit_batch scores linear_only fits; use fit() for hinge sweeps")
-    batch = marslib.Batch(Xs, ys, ws, int(max_terms), precision=precision)
+    if batch is None:
+        batch = marslib.Batch(Xs, ys, ws, int(max_terms), precision=precision)
     batch.set_timing(bool(timing))
     t0 = time.perf_counter()
     rows, steps = batch.fit(int(max_degree), float(penalty), bool(self_interact), int(r2_window), float(r2_thresh))

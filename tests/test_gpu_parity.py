@@ -283,4 +283,4 @@ def test_batch_mixed_matches_reference():
         assert got == want, gold["name"]
         for a, b in zip(mse, [a["mse"] for a in gold["appends"]]):
             assert abs(a - b) <= 1e-10 * abs(b)
-    assert 0 < stats["mixed_rescored"] < stats["mixed_scored"]
+    assert 0 < stats["rescored"] < stats["scored"]
