@@ -65,6 +65,7 @@ struct FitDev {
     const float *B;    // basis, row-major (ldb)
     const float *Bo;   // orthonormal basis, row-major (ldb)
     const float *BoC;  // the same basis, column-major (ldx)
+    const float *BC;   // basis B, column-major (ldx)
     const float *y;    // normalized target (f32)
     const double *ybo; // Bo^T y per live column
     float *rbuf;       // residual / normalized candidate columns, row-major (ldr)
@@ -83,7 +84,7 @@ struct FitState {
     double tol = 0, yvar = 0;
     size_t m = 0;
     std::vector<double> ybo_h;
-    DBuf<float> X, s, B, Bo, BoC, y, rbuf;
+    DBuf<float> X, s, B, Bo, BoC, BC, y, rbuf;
     DBuf<double> ybo, v;
     size_t ldr = 0;
     // mixed form: T cache valid for basis columns and parents < mc
@@ -99,7 +100,8 @@ struct FitState {
     std::vector<char> xres;
     void release()
     {
-        X.release(); s.release(); B.release(); Bo.release(); BoC.release(); y.release(); rbuf.release();
+        X.release(); s.release(); B.release(); Bo.release(); BoC.release(); BC.release(); y.release();
+        rbuf.release();
         ybo.release(); v.release(); order.release(); delta.release(); tc.release(); tcn.release();
     }
 };
@@ -112,7 +114,7 @@ struct Item {
 
 /* Writable basis storage of one fit (append). */
 struct FitW {
-    float *B, *Bo, *BoC;
+    float *B, *Bo, *BoC, *BC;
     double *ybo;
 };
 
@@ -186,6 +188,8 @@ struct mars_hip_ctx {
     mh::DBuf<mh::FitDev> d_fits;
     mh::DBuf<mh::Item> d_items;
     mh::DBuf<int> d_cmap;            // per-candidate first item / item count per pass
+    mh::DBuf<mh::Item> d_ritems;     // DGKS retry passes (separate: the first passes' lists stay live)
+    mh::DBuf<int> d_rcmap;
     mh::DBuf<double> tpart, T, tnorm, spart, ssum, ybx, lin, hin, aux;
     mh::DBuf<int> flag, fired_list, fired_count, knot, krow, ival;
     mh::DBuf<long long> dgks;        // device DGKS counter

@@ -49,6 +49,7 @@ __global__ __launch_bounds__(256) void k_init2(double *__restrict__ yd, const do
                                                size_t n, const double *__restrict__ part,
                                                float *__restrict__ y32, float *__restrict__ B,
                                                float *__restrict__ Bo, size_t ldb, float *__restrict__ BoC,
+                                               float *__restrict__ BC,
                                                double *__restrict__ part2, double *__restrict__ scal)
 {
     __shared__ double sh[256];
@@ -70,6 +71,7 @@ __global__ __launch_bounds__(256) void k_init2(double *__restrict__ yd, const do
             B[i * ldb] = b;
             Bo[i * ldb] = b;
             BoC[i] = b;
+            BC[i] = b;
             ab = fma((double)b, (double)yf, ab);
             am += d;
         }
@@ -205,11 +207,15 @@ __global__ __launch_bounds__(256) void k_app_mgs(const FitDev *__restrict__ fits
     float rn = 1.0f;
     if (j > 0) c = sum_partials(prev + (size_t)f * nb, nb, sh);
     if (j == 0 && first) rn = (float)sqrt(sum_partials(prev + (size_t)f * nb, nb, sh));
-    float *B = W[f].B;
+    float *B = W[f].B, *BC = W[f].BC + (size_t)m * F.ldx;
     double a = 0.0;
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < F.n; i += (long long)nb * 256) {
         double vi = F.v[i];
-        if (j == 0 && first) B[i * F.ldb + m] = __fdiv_rn((float)vi, rn);
+        if (j == 0 && first) {
+            const float b = __fdiv_rn((float)vi, rn);
+            B[i * F.ldb + m] = b;
+            BC[i] = b;
+        }
         if (j > 0) {
             vi = fma(-c, (double)F.BoC[(j - 1) * F.ldx + i], vi);
             F.v[i] = vi;
@@ -441,6 +447,8 @@ void fit_state_alloc(mars_hip_ctx *ctx, FitState &fs, size_t n, double tol)
     fs.ldx = (n + 31) / 32 * 32;
     fs.BoC.ensure(fs.ldx * ctx->ldb);
     MH_CHECK(hipMemsetAsync(fs.BoC.p, 0, fs.ldx * ctx->ldb * sizeof(float), ctx->stream));
+    fs.BC.ensure(fs.ldx * ctx->ldb);
+    MH_CHECK(hipMemsetAsync(fs.BC.p, 0, fs.ldx * ctx->ldb * sizeof(float), ctx->stream));
     fs.y.ensure(fs.ldx + 32);
     MH_CHECK(hipMemsetAsync(fs.y.p, 0, (fs.ldx + 32) * sizeof(float), ctx->stream));
     fs.ybo.ensure(ctx->max_terms);
@@ -463,6 +471,8 @@ void ctx_free(mars_hip_ctx *ctx)
     for (auto *b : ib) b->release();
     ctx->d_fits.release();
     ctx->d_items.release();
+    ctx->d_ritems.release();
+    ctx->d_rcmap.release();
     ctx->dgks.release();
     ctx->aout.release();
     ctx->aarg.release();
@@ -508,7 +518,7 @@ void fit_init(mars_hip_ctx *ctx, FitState &fs, const float *X, size_t ldX, const
     hipLaunchKernelGGL(k_init1, dim3(kInitBlocks), dim3(256), 0, st, yw.p, yw.p + n, n, yd, sw, p1);
     MH_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_init2, dim3(kInitBlocks), dim3(256), 0, st, yd, sw, n, p1, fs.y.p, fs.B.p, fs.Bo.p,
-                       ctx->ldb, fs.BoC.p, p2, scal.p);
+                       ctx->ldb, fs.BoC.p, fs.BC.p, p2, scal.p);
     MH_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_init3, dim3(kInitBlocks), dim3(256), 0, st, yd, n, p2, p1, scal.p);
     MH_LAUNCH_CHECK();
@@ -576,7 +586,7 @@ void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std
     std::vector<FitW> fw(F);
     for (size_t f = 0; f < F; ++f) {
         FitState &fs = ctx->fits[f];
-        fw[f] = FitW{fs.B.p, fs.Bo.p, fs.BoC.p, fs.ybo.p};
+        fw[f] = FitW{fs.B.p, fs.Bo.p, fs.BoC.p, fs.BC.p, fs.ybo.p};
     }
     ctx->d_fitw.ensure(F);
     MH_CHECK(hipMemcpyAsync(ctx->d_fitw.p, fw.data(), F * sizeof(FitW), hipMemcpyHostToDevice, st));
@@ -595,6 +605,15 @@ void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std
         hipLaunchKernelGGL(k_app_build, grid, dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, pa, ctx->scal.p);
         MH_LAUNCH_CHECK();
         for (int pass = 0; pass < 2; ++pass) {
+            if (pass == 1) {
+                // the retry sweep only runs when some fit's DGKS gate fired
+                std::vector<int> g(F);
+                MH_CHECK(hipMemcpyAsync(g.data(), gate, F * sizeof(int), hipMemcpyDeviceToHost, st));
+                MH_CHECK(hipStreamSynchronize(st));
+                bool any = false;
+                for (size_t f = 0; f < F; ++f) any = any || (args[f].active && g[f]);
+                if (!any) break;
+            }
             for (size_t j = 0; j <= mmax; ++j) {
                 hipLaunchKernelGGL(k_app_mgs, grid, dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, pp[j % 2],
                                    pp[(j + 1) % 2], (int)j, pass == 0 ? 1 : 0, pass == 0 ? nullptr : gate,
@@ -727,9 +746,12 @@ void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t 
     if (lin_out && total) MH_CHECK(hipMemcpyAsync(lin_out, dlin, total * 8, hipMemcpyDeviceToHost, st));
     if (hin_out && total) MH_CHECK(hipMemcpyAsync(hin_out, dhin, total * 8, hipMemcpyDeviceToHost, st));
     if (cut_out && total) MH_CHECK(hipMemcpyAsync(cut_out, dcut, total * 8, hipMemcpyDeviceToHost, st));
-    double cut2 = NAN;
-    if (total) MH_CHECK(hipMemcpyAsync(&cut2, dcut + bo.c2, 8, hipMemcpyDeviceToHost, st));
     MH_CHECK(hipStreamSynchronize(st));
+    double cut2 = NAN;
+    if (total) {
+        require(bo.c2 >= 0 && (size_t)bo.c2 < total, "internal: argmax index out of range");
+        MH_CHECK(hipMemcpy(&cut2, dcut + bo.c2, 8, hipMemcpyDeviceToHost));
+    }
     float ms = 0.f;
     MH_CHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->eval_ms += ms;
@@ -767,6 +789,7 @@ void mirror_sync_basis(mars_hip_ctx *ctx, size_t m, const float *B, size_t ldB, 
             hipLaunchKernelGGL(k_put_bcol, dim3((n + 255) / 256), dim3(256), 0, st, fs.B.p, ctx->ldb, (int)j, col.p,
                                n);
             MH_LAUNCH_CHECK();
+            MH_CHECK(hipMemcpyAsync(fs.BC.p + j * fs.ldx, col.p, n * sizeof(float), hipMemcpyDeviceToDevice, st));
             MH_CHECK(hipStreamSynchronize(st));
             for (size_t i = 0; i < n; ++i) h[i] = Bo[i * ldBo + j];
             MH_CHECK(hipMemcpyAsync(col.p, h.data(), n * sizeof(float), hipMemcpyHostToDevice, st));

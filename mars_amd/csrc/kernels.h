@@ -23,6 +23,15 @@ __device__ inline v4d mfma_f64(double a, double b, v4d c)
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+/*
+ * MFMA f32 16x16x4 (measured on gfx950): A and B as for f64;
+ *   D (16x16): lane l, element v holds D[4(l/16) + v][l%16]
+ */
+__device__ inline v4f mfma_f32(float a, float b, v4f c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
 /* Fixed-order block sum of one double per thread (blockDim.x == 256). */
 __device__ inline double block_sum_256(double v, double *sh)
 {

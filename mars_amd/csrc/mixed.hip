@@ -23,7 +23,6 @@
 // Per step the cache costs O(n p m) per fit instead of O(n p m^2).
 #include "internal.h"
 #include "kernels.h"
-#include "tgemm.h"
 
 #include <algorithm>
 #include <cmath>
@@ -32,7 +31,7 @@ namespace mh {
 
 namespace {
 
-constexpr int kChunk = kRC;        // rows per FP32 accumulation chunk (folded into FP64)
+constexpr int kChunk = 32;         // rows per FP32 accumulation chunk (folded into FP64)
 constexpr int kXT = 64;            // X columns per update item (4 waves x 16)
 constexpr double kWindow = 1e-4;   // relative near-tie window rescored in FP64
 constexpr int kGroupInts = 6;      // fit, xc0, nc, j, first item, item count
@@ -45,9 +44,10 @@ constexpr int kGroupInts = 6;      // fit, xc0, nc, j, first item, item count
  *           = f32(Bo[i][j] B[i][a-j-1])      (a >  j:  k = j,  parent a-j-1)
  * and of |Bx|^2, Bx.y for Bx = f32(B[:,j] xs[:,xc]).
  * Per 32-row chunk the block stages the Bo and B rows (columns <= j) and the
- * 64 scaled X columns with 16-byte loads; wave w then runs the MFMAs of X
- * columns 16w..16w+15.  Rows r0 are multiples of 32; X and y are padded to a
- * multiple of 32 rows, so the vector loads stay inside their buffers.
+ * 64 scaled X columns with 16-byte loads, forms the chunk's A tile once in
+ * LDS, and wave w runs the MFMAs of X columns 16w..16w+15.  Rows r0 are
+ * multiples of 32; X and y are padded to a multiple of 32 rows, so the vector
+ * loads stay inside their buffers.
  */
 template <int MT>
 __global__ __launch_bounds__(256) void k_tc_update(const FitDev *__restrict__ fits,
@@ -55,8 +55,10 @@ __global__ __launch_bounds__(256) void k_tc_update(const FitDev *__restrict__ fi
                                                    double *__restrict__ tpart, double *__restrict__ npart)
 {
     constexpr int SW = 8 * MT;  // staged basis columns: j + 1 <= SW since 2j + 1 <= 16 MT
+    constexpr int KW = 16 * MT;
     __shared__ float sBo[kChunk][SW + 4];
     __shared__ float sB[kChunk][SW + 4];
+    __shared__ float sA[kChunk][KW + 4];
     __shared__ float sX[kChunk][kXT + 4];
     __shared__ double sN[2][4][kXT];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -106,6 +108,15 @@ __global__ __launch_bounds__(256) void k_tc_update(const FitDev *__restrict__ fi
                 }
             }
             __syncthreads();
+            for (int e = t; e < kChunk * na; e += 256) {
+                const int i = e / na, a = e - i * na;
+                const int ka = a <= j ? a : j, pa = a <= j ? j : a - j - 1;
+                sA[i][a] = __fmul_rn(sBo[i][ka], sB[i][pa]);
+            }
+            for (int e = t; e < kChunk * (16 * mtf - na); e += 256) {
+                const int i = e / (16 * mtf - na), a = na + e - i * (16 * mtf - na);
+                sA[i][a] = 0.0f;
+            }
             if (sok) {
                 const float4 *yp = (const float4 *)(F.y + r + 8 * g);
                 const float4 y0 = yp[0], y1 = yp[1];
@@ -117,6 +128,7 @@ __global__ __launch_bounds__(256) void k_tc_update(const FitDev *__restrict__ fi
                     bxy = fma(bx, (double)yv[u], bxy);
                 }
             }
+            __syncthreads();
             if (wok) {
                 v4f part[MT];
 #pragma unroll
@@ -125,17 +137,8 @@ __global__ __launch_bounds__(256) void k_tc_update(const FitDev *__restrict__ fi
                 for (int q = 0; q < kChunk; q += 4) {
                     const float b = sX[q + kr][16 * w + kc];
 #pragma unroll
-                    for (int u = 0; u < MT; ++u) {
-                        if (u < mtf) {
-                            const int a = 16 * u + kc;
-                            float av = 0.0f;
-                            if (a < na) {
-                                const int ka = a <= j ? a : j, pa = a <= j ? j : a - j - 1;
-                                av = __fmul_rn(sBo[q + kr][ka], sB[q + kr][pa]);
-                            }
-                            part[u] = mfma_f32(av, b, part[u]);
-                        }
-                    }
+                    for (int u = 0; u < MT; ++u)
+                        if (u < mtf) part[u] = mfma_f32(sA[q + kr][16 * u + kc], b, part[u]);
                 }
 #pragma unroll
                 for (int u = 0; u < MT; ++u)

@@ -13,7 +13,7 @@
 // run-to-run identical and independent of the other fits in the batch.
 #include "internal.h"
 #include "kernels.h"
-#include "tgemm.h"
+#include "stream.h"
 
 #include <algorithm>
 #include <numeric>
@@ -22,8 +22,6 @@ namespace mh {
 
 namespace {
 
-constexpr int kTileT = 64;   // candidates per T-GEMM item (4 waves x 16)
-constexpr int kTileP = 64;   // candidates per projection item
 constexpr int kTileF = 64;   // candidates per finalize item (x 4 row phases)
 
 /* T[c] = sum of the candidate's item partials in order; tnorm[c] = |T[c][:m]|^2.
@@ -49,98 +47,6 @@ __global__ __launch_bounds__(64) void k_treduce(const FitDev *__restrict__ fits,
     }
     tt = wave_sum(tt);
     if (lane == 0 && tnorm) tnorm[c] = tt;
-}
-
-/* ---------------------------------------------------------------------- */
-/* r = f32(src - Bo T) and sum r^2 per item.  4 waves x 16-row tiles, 64 cands */
-/* ---------------------------------------------------------------------- */
-template <bool RETRY>
-__global__ __launch_bounds__(256) void k_project(const FitDev *__restrict__ fits,
-                                                 const Item *__restrict__ items, int nitems,
-                                                 const int *__restrict__ cxc,
-                                                 const int *__restrict__ cpar,
-                                                 const int *__restrict__ list,
-                                                 const double *__restrict__ T, int ldT,
-                                                 double *__restrict__ spart)
-{
-    __shared__ double sh[4][64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int kr = lane >> 4, kc = lane & 15;
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-        const Item I = items[it];
-        const FitDev F = fits[I.fit];
-        bool cok[4];
-        int tpos[4], col[4], par[4] = {0, 0, 0, 0};
-        float sx[4] = {0.f, 0.f, 0.f, 0.f};
-        const float *xcol[4] = {F.X, F.X, F.X, F.X};
-        double sacc[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int slot = u * 16 + kc;
-            cok[u] = slot < I.nc;
-            tpos[u] = I.c0 + slot;  // row of T (candidate or list position)
-            col[u] = 0;
-            if (cok[u]) {
-                if (RETRY) {
-                    col[u] = list[tpos[u]] - F.c0;
-                } else {
-                    const int xc = cxc[tpos[u]];
-                    col[u] = tpos[u] - F.c0;
-                    par[u] = cpar[tpos[u]];
-                    sx[u] = F.s[xc];
-                    xcol[u] = F.X + (size_t)xc * F.ldx;
-                }
-            }
-        }
-        for (long long t0 = I.r0 + 16 * w; t0 < I.r1; t0 += 64) {
-            v4d acc[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) acc[u] = v4d{0, 0, 0, 0};
-            const long long arow = t0 + kc;
-            const bool aok = arow < I.r1;
-            for (int k0 = 0; k0 < F.m; k0 += 4) {
-                const int k = k0 + kr;
-                const bool kok = k < F.m;
-                const double a = (aok && kok) ? (double)F.Bo[arow * F.ldb + k] : 0.0;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const double b = (cok[u] && kok) ? T[(size_t)tpos[u] * ldT + k] : 0.0;
-                    acc[u] = mfma_f64(a, b, acc[u]);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (!cok[u]) continue;
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const long long row = t0 + kr + 4 * v;
-                    if (row >= I.r1) continue;
-                    float src;
-                    if (RETRY)
-                        src = F.rbuf[row * F.ldr + col[u]];
-                    else
-                        src = __fmul_rn(F.B[row * F.ldb + par[u]], __fmul_rn(xcol[u][row], sx[u]));
-                    const float rf = (float)((double)src - acc[u][v]);
-                    F.rbuf[row * F.ldr + col[u]] = rf;
-                    const double rd = (double)rf;
-                    sacc[u] = fma(rd, rd, sacc[u]);
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            double s = sacc[u];
-            s = s + __shfl_xor(s, 16, 64);
-            s = s + __shfl_xor(s, 32, 64);
-            if (kr == 0) sh[w][u * 16 + kc] = s;
-        }
-        __syncthreads();
-        if (threadIdx.x < 64) {
-            const int t = threadIdx.x;
-            if (t < I.nc) spart[(size_t)it * 64 + t] = ((sh[0][t] + sh[1][t]) + sh[2][t]) + sh[3][t];
-        }
-        __syncthreads();
-    }
 }
 
 /* s[c] = sum of item partials; the first pass also evaluates the DGKS gate. */
@@ -289,13 +195,15 @@ struct Uploaded {
     int *cmap = nullptr;
 };
 
-/* Upload items and cmap into a contiguous scratch region of ctx->d_items / d_cmap at offsets. */
-void upload_pass(mars_hip_ctx *ctx, const Pass &p, size_t item_off, size_t cmap_off, Uploaded &u)
+/* Upload items and cmap at offsets of `items` / `cmap`, which the caller has sized already (no
+ * reallocation here: pointers into earlier passes of the same buffers stay valid). */
+void upload_pass(mars_hip_ctx *ctx, const Pass &p, DBuf<Item> &items, DBuf<int> &cmap, size_t item_off,
+                 size_t cmap_off, Uploaded &u)
 {
-    ctx->d_items.ensure(item_off + std::max<size_t>(p.items.size(), 1));
-    ctx->d_cmap.ensure(cmap_off + std::max<size_t>(p.cmap.size(), 1));
-    u.items = ctx->d_items.p + item_off;
-    u.cmap = ctx->d_cmap.p + cmap_off;
+    require(item_off + p.items.size() <= items.cap && cmap_off + p.cmap.size() <= cmap.cap,
+            "internal: work list buffer too small");
+    u.items = items.p + item_off;
+    u.cmap = cmap.p + cmap_off;
     if (!p.items.empty())
         MH_CHECK(hipMemcpyAsync(u.items, p.items.data(), p.items.size() * sizeof(Item),
                                 hipMemcpyHostToDevice, ctx->stream));
@@ -309,55 +217,50 @@ int grid_for(size_t items)
     return (int)std::max<size_t>(1, std::min<size_t>(items, 4096));
 }
 
+#define MH_MT_SWITCH(mt, GO)                                                                 \
+    switch (mt) {                                                                            \
+    case 1: GO(1); break;   case 2: GO(2); break;   case 3: GO(3); break;   case 4: GO(4); break;   \
+    case 5: GO(5); break;   case 6: GO(6); break;   case 7: GO(7); break;   case 8: GO(8); break;   \
+    case 9: GO(9); break;   case 10: GO(10); break; case 11: GO(11); break; case 12: GO(12); break; \
+    case 13: GO(13); break; case 14: GO(14); break; case 15: GO(15); break; default: GO(16); break;  \
+    }
+
+int grid_w(size_t items)
+{
+    return (int)std::max<size_t>(1, std::min<size_t>(items, 65536));
+}
+
 template <bool RETRY>
 void launch_tgemm(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, const Cands &cs,
                   const int *list, int ldT)
 {
     if (nitems == 0) return;
-    const dim3 grid(grid_for(nitems)), block(256);
-    auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, grid, block, 0, ctx->stream, ctx->d_fits.p, u.items, nitems, cs.d_xc.p,
-                           cs.d_par.p, list, ldT, ctx->tpart.p, nullptr);
-    };
-    switch (mt) {
-    case 1: go(k_tgemm_lds<1, false, RETRY, false>); break;
-    case 2: go(k_tgemm_lds<2, false, RETRY, false>); break;
-    case 3: go(k_tgemm_lds<3, false, RETRY, false>); break;
-    case 4: go(k_tgemm_lds<4, false, RETRY, false>); break;
-    case 5: go(k_tgemm_lds<5, false, RETRY, false>); break;
-    case 6: go(k_tgemm_lds<6, false, RETRY, false>); break;
-    case 7: go(k_tgemm_lds<7, false, RETRY, false>); break;
-    case 8: go(k_tgemm_lds<8, false, RETRY, false>); break;
-    case 9: go(k_tgemm_lds<9, false, RETRY, false>); break;
-    case 10: go(k_tgemm_lds<10, false, RETRY, false>); break;
-    case 11: go(k_tgemm_lds<11, false, RETRY, false>); break;
-    case 12: go(k_tgemm_lds<12, false, RETRY, false>); break;
-    case 13: go(k_tgemm_lds<13, false, RETRY, false>); break;
-    case 14: go(k_tgemm_lds<14, false, RETRY, false>); break;
-    case 15: go(k_tgemm_lds<15, false, RETRY, false>); break;
-    default: go(k_tgemm_lds<16, false, RETRY, false>); break;
-    }
+    const dim3 grid(grid_w(nitems)), block(64);
+#define MH_GO(M)                                                                                      \
+    hipLaunchKernelGGL((k_tgemm_w<M, cg_tgemm(M), RETRY>), grid, block, 0, ctx->stream, (const FitDev *)ctx->d_fits.p, \
+                       (const Item *)u.items, nitems, (const int *)cs.d_xc.p, (const int *)cs.d_par.p, list, ldT, \
+                       ctx->tpart.p)
+    MH_MT_SWITCH(mt, MH_GO)
+#undef MH_GO
+    MH_LAUNCH_CHECK();
+}
+
+template <bool RETRY>
+void launch_project(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, const Cands &cs,
+                    const int *list, int ldT)
+{
+    if (nitems == 0) return;
+    const dim3 grid(grid_w(nitems)), block(64);
+#define MH_GO(M)                                                                                        \
+    hipLaunchKernelGGL((k_project_w<M, cg_project(M), RETRY>), grid, block, 0, ctx->stream, (const FitDev *)ctx->d_fits.p, \
+                       (const Item *)u.items, nitems, (const int *)cs.d_xc.p, (const int *)cs.d_par.p, list,   \
+                       (const double *)ctx->T.p, ldT, ctx->spart.p)
+    MH_MT_SWITCH(mt, MH_GO)
+#undef MH_GO
     MH_LAUNCH_CHECK();
 }
 
 } // namespace
-
-/* Work list over the candidates of `cs`: tiles of `tile` candidates per fit,
- * each cut into row ranges (multiple of `align`) so the pass has about
- * `target` items. */
-Pass make_items(const Cands &cs, const mars_hip_ctx *ctx, int tile, double target, long long align)
-{
-    const size_t F = ctx->fits.size();
-    std::vector<long long> n(F);
-    std::vector<int> fb(F), fe(F), cnt(F);
-    for (size_t f = 0; f < F; ++f) {
-        n[f] = (long long)ctx->fits[f].n;
-        fb[f] = cs.fit_c0[f];
-        cnt[f] = cs.fit_P[f];
-        fe[f] = fb[f] + cnt[f];
-    }
-    return make_pass(fb, fe, n, split_rows(n, cnt, tile, target, std::max<long long>(align, 256)), tile, align);
-}
 
 void index_cands(const mars_hip_ctx *ctx, Cands &cs)
 {
@@ -416,6 +319,7 @@ void upload_fits(mars_hip_ctx *ctx, const Cands &cs, const float *xover, const f
         d.B = fs.B.p;
         d.Bo = fs.Bo.p;
         d.BoC = fs.BoC.p;
+        d.BC = fs.BC.p;
         d.y = fs.y.p;
         d.ybo = fs.ybo.p;
         d.rbuf = fs.rbuf.p;
@@ -462,27 +366,27 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     const int mt = (int)std::min<size_t>(16, (mmax + 15) / 16);
     require(mmax <= 256, "the FP64 scoring path supports up to 256 basis functions");
     const int ldT = mt * 16;
+    const int tileT = 16 * cg_tgemm(mt), tileP = 16 * cg_project(mt);
 
-    Pass pT = make_pass(fb, fe, n, split_rows(n, cnt, kTileT, 2048, 256), kTileT, kRC);
-    Pass pP = make_pass(fb, fe, n, split_rows(n, cnt, kTileP, 2048, 256), kTileP, 64);
+    Pass pT = make_pass(fb, fe, n, split_rows(n, cnt, tileT, 8192, 256), tileT, kWR);
+    Pass pP = make_pass(fb, fe, n, split_rows(n, cnt, tileP, 8192, 256), tileP, kWR);
     Pass pF = make_pass(fb, fe, n, split_rows(n, cnt, kTileF, 2048, 512), kTileF, 32);
     Uploaded uT, uP, uF;
     size_t io = 0, co = 0;
     const size_t total_items = pT.items.size() + pP.items.size() + pF.items.size();
     const size_t total_cmap = pT.cmap.size() + pP.cmap.size() + pF.cmap.size();
-    ctx->d_items.ensure(total_items + 2 * (size_t)P + 16);
-    ctx->d_cmap.ensure(total_cmap + 6 * (size_t)P + 16);
-    upload_pass(ctx, pT, io, co, uT);
+    ctx->d_items.ensure(total_items + 1);
+    ctx->d_cmap.ensure(total_cmap + 1);
+    upload_pass(ctx, pT, ctx->d_items, ctx->d_cmap, io, co, uT);
     io += pT.items.size(); co += pT.cmap.size();
-    upload_pass(ctx, pP, io, co, uP);
+    upload_pass(ctx, pP, ctx->d_items, ctx->d_cmap, io, co, uP);
     io += pP.items.size(); co += pP.cmap.size();
-    upload_pass(ctx, pF, io, co, uF);
-    io += pF.items.size(); co += pF.cmap.size();
+    upload_pass(ctx, pF, ctx->d_items, ctx->d_cmap, io, co, uF);
 
-    ctx->tpart.ensure(std::max<size_t>(pT.items.size(), 1) * 64 * ldT);
+    ctx->tpart.ensure(std::max<size_t>(pT.items.size(), 1) * kSlots * ldT);
     ctx->T.ensure((size_t)P * ldT);
     ctx->tnorm.ensure(P);
-    ctx->spart.ensure(std::max(pP.items.size() * kTileP, pF.items.size() * kTileF) + 1);
+    ctx->spart.ensure(std::max(pP.items.size() * kSlots, pF.items.size() * kTileF) + 1);
     ctx->ssum.ensure(P);
     ctx->flag.ensure(P);
     ctx->fired_list.ensure(P);
@@ -498,12 +402,9 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     // Pass 2: r = f32(Bx - Bo T), s, DGKS gate
     {
         KTimer kt(ctx, K_PROJECT, flops, bytes + 0.0);
-        hipLaunchKernelGGL(k_project<false>, dim3(grid_for(pP.items.size())), dim3(256), 0, st,
-                           ctx->d_fits.p, uP.items, (int)pP.items.size(), cs.d_xc.p, cs.d_par.p,
-                           nullptr, ctx->T.p, ldT, ctx->spart.p);
-        MH_LAUNCH_CHECK();
+        launch_project<false>(ctx, mt, uP, (int)pP.items.size(), cs, nullptr, ldT);
         hipLaunchKernelGGL(k_sreduce, dim3((P + 255) / 256), dim3(256), 0, st, nullptr, P, uP.cmap,
-                           ctx->spart.p, kTileP, ctx->tnorm.p, ctx->d_fits.p, cs.d_fit.p,
+                           ctx->spart.p, kSlots, ctx->tnorm.p, ctx->d_fits.p, cs.d_fit.p,
                            ctx->ssum.p, ctx->flag.p);
         MH_LAUNCH_CHECK();
     }
@@ -526,25 +427,22 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         MH_CHECK(hipMemcpyAsync(ctx->fired_list.p, fired.data(), nf * sizeof(int), hipMemcpyHostToDevice, st));
         std::vector<int> fcnt(F);
         for (size_t f = 0; f < F; ++f) fcnt[f] = re[f] - rb[f];
-        Pass rT = make_pass(rb, re, n, split_rows(n, fcnt, kTileT, 1024, 256), kTileT, kRC);
-        Pass rP = make_pass(rb, re, n, split_rows(n, fcnt, kTileP, 1024, 256), kTileP, 64);
+        Pass rT = make_pass(rb, re, n, split_rows(n, fcnt, tileT, 4096, 256), tileT, kWR);
+        Pass rP = make_pass(rb, re, n, split_rows(n, fcnt, tileP, 4096, 256), tileP, kWR);
         Uploaded vT, vP;
-        ctx->d_items.ensure(io + rT.items.size() + rP.items.size() + 2);
-        ctx->d_cmap.ensure(co + rT.cmap.size() + rP.cmap.size() + 2);
-        upload_pass(ctx, rT, io, co, vT);
-        upload_pass(ctx, rP, io + rT.items.size(), co + rT.cmap.size(), vP);
-        ctx->tpart.ensure(std::max<size_t>(rT.items.size(), 1) * 64 * ldT);
-        ctx->spart.ensure(rP.items.size() * kTileP + 1);
+        ctx->d_ritems.ensure(rT.items.size() + rP.items.size() + 1);
+        ctx->d_rcmap.ensure(rT.cmap.size() + rP.cmap.size() + 1);
+        upload_pass(ctx, rT, ctx->d_ritems, ctx->d_rcmap, 0, 0, vT);
+        upload_pass(ctx, rP, ctx->d_ritems, ctx->d_rcmap, rT.items.size(), rT.cmap.size(), vP);
+        ctx->tpart.ensure(std::max<size_t>(rT.items.size(), 1) * kSlots * ldT);
+        ctx->spart.ensure(rP.items.size() * kSlots + 1);
         launch_tgemm<true>(ctx, mt, vT, (int)rT.items.size(), cs, ctx->fired_list.p, ldT);
         hipLaunchKernelGGL(k_treduce, dim3(nf), dim3(64), 0, st, ctx->d_fits.p, cs.d_fit.p,
                            ctx->fired_list.p, nf, vT.cmap, ctx->tpart.p, ldT, ctx->T.p, nullptr);
         MH_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_project<true>, dim3(grid_for(rP.items.size())), dim3(256), 0, st,
-                           ctx->d_fits.p, vP.items, (int)rP.items.size(), cs.d_xc.p, cs.d_par.p,
-                           ctx->fired_list.p, ctx->T.p, ldT, ctx->spart.p);
-        MH_LAUNCH_CHECK();
+        launch_project<true>(ctx, mt, vP, (int)rP.items.size(), cs, ctx->fired_list.p, ldT);
         hipLaunchKernelGGL(k_sreduce, dim3((nf + 255) / 256), dim3(256), 0, st, ctx->fired_list.p, nf,
-                           vP.cmap, ctx->spart.p, kTileP, nullptr, ctx->d_fits.p, cs.d_fit.p,
+                           vP.cmap, ctx->spart.p, kSlots, nullptr, ctx->d_fits.p, cs.d_fit.p,
                            ctx->ssum.p, nullptr);
         MH_LAUNCH_CHECK();
     }

@@ -1,0 +1,243 @@
+// Wave-streaming FP64 contractions of the plain candidate path (gfx950).
+//
+// Each workgroup is a single wave64 that owns one item (fit, tile of at most
+// 16*CG candidates, row range starting at a multiple of 32) and walks its rows
+// in 32-row chunks, feeding v_mfma_f64_16x16x4f64 straight from registers.
+// Which memory row an MFMA K (T-GEMM) or M (projection) index stands for is
+// free, so it is chosen to make every lane's operands contiguous runs of the
+// column-major copies of Bo and B (BoC, BC: ld = the 32-padded row count) and
+// of X: 16-byte loads, no barriers inside the row walk, workgroups fully
+// independent.
+// 16-candidate groups past the tile's end are skipped, so small candidate
+// sets (the mixed form's rescoring) cost in proportion.
+//
+//   k_tgemm_w:   T = Bo^T Bx over the item's rows                  (kernels.cc:332-345)
+//   k_project_w: r = f32(Bx - Bo T), stored row-major, and sum r^2  (kernels.cc:347-372)
+//                (T tile staged in LDS once per item)
+// with Bx = f32(B[:,par] * f32(X[:,xc] * s[xc])) (kernels.cc:300-318), or, in
+// the DGKS retry, the stored residual columns (kernels.cc:387-399).
+//
+// Rows past the fit's end inside the last chunk read as zero: X, BC and BoC
+// are zero-padded to a multiple of 32 rows.
+#pragma once
+
+#include "common.h"
+#include "kernels.h"
+
+namespace mh {
+
+constexpr int kWR = 32;      // rows per chunk (item row ranges start at multiples of 32)
+constexpr int kSlots = 64;   // candidate slots per item in the partial-sum layouts
+
+/* Candidate groups of 16 per item: bounds registers (T-GEMM accumulators) and
+ * LDS (projection T tile, at most 2048 doubles for m <= 128). */
+constexpr int cg_tgemm(int mt) { return mt <= 2 ? 4 : (mt <= 8 ? 2 : 1); }
+constexpr int cg_project(int mt) { return mt <= 2 ? 4 : (mt <= 4 ? 2 : 1); }
+constexpr int waves_for(int mt) { return mt == 1 ? 2 : (mt <= 4 ? 3 : 2); }
+
+/* Per-lane candidate of an item slot. */
+struct LaneCand {
+    bool ok;
+    float sx;
+    const float *bc;   // B column of the parent (column-major)
+    const float *xp;   // X column
+    long long col;     // rbuf column
+};
+
+template <bool RETRY>
+__device__ inline LaneCand lane_cand(const FitDev &F, const Item &I, int slot, const int *cxc, const int *cpar,
+                                     const int *list)
+{
+    LaneCand c{slot < I.nc, 0.0f, F.BC, F.X, 0};
+    if (!c.ok) return c;
+    const int pos = I.c0 + slot;
+    if (RETRY) {
+        c.col = list[pos] - F.c0;
+    } else {
+        c.col = pos - F.c0;
+        const int xc = cxc[pos];
+        c.sx = F.s[xc];
+        c.bc = F.BC + (size_t)cpar[pos] * F.ldx;
+        c.xp = F.X + (size_t)xc * F.ldx;
+    }
+    return c;
+}
+
+/* Bx of rows row..row+3 (row a multiple of 4, all < r1 or zero-padded). */
+template <bool RETRY>
+__device__ inline float4 bx4(const FitDev &F, const LaneCand &c, long long row, long long r1)
+{
+    float4 v = {0.f, 0.f, 0.f, 0.f};
+    if (!c.ok) return v;
+    if (RETRY) {
+        const float *p = F.rbuf + c.col;
+        if (row < r1) v.x = p[row * F.ldr];
+        if (row + 1 < r1) v.y = p[(row + 1) * F.ldr];
+        if (row + 2 < r1) v.z = p[(row + 2) * F.ldr];
+        if (row + 3 < r1) v.w = p[(row + 3) * F.ldr];
+    } else {
+        const float4 b = *(const float4 *)(c.bc + row);
+        const float4 x = *(const float4 *)(c.xp + row);
+        v.x = __fmul_rn(b.x, __fmul_rn(x.x, c.sx));
+        v.y = __fmul_rn(b.y, __fmul_rn(x.y, c.sx));
+        v.z = __fmul_rn(b.z, __fmul_rn(x.z, c.sx));
+        v.w = __fmul_rn(b.w, __fmul_rn(x.w, c.sx));
+    }
+    return v;
+}
+
+/*
+ * T-GEMM partials: tpart[(item * 64 + slot) * ldT + k] = sum over the item's
+ * rows of Bo[i][k] Bx[i][slot].  MFMA step q of a 4-row half-step at row r
+ * takes rows r + 8 kr + q (kr = lane / 16), so each lane reads 4 consecutive
+ * rows of its Bo columns 16u + lane%16 and of its candidates' Bx.
+ */
+template <int MT, int CG, bool RETRY>
+__global__ __launch_bounds__(64, waves_for(MT)) void k_tgemm_w(const FitDev *__restrict__ fits, const Item *__restrict__ items,
+                                                int nitems, const int *__restrict__ cxc,
+                                                const int *__restrict__ cpar, const int *__restrict__ list,
+                                                int ldT, double *__restrict__ tpart)
+{
+    const int lane = threadIdx.x, kr = lane >> 4, kc = lane & 15;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const Item I = items[it];
+        const FitDev F = fits[I.fit];
+        LaneCand c[CG];
+#pragma unroll
+        for (int g = 0; g < CG; ++g) c[g] = lane_cand<RETRY>(F, I, 16 * g + kc, cxc, cpar, list);
+        const float *bo[MT];
+        bool kok[MT];
+#pragma unroll
+        for (int u = 0; u < MT; ++u) {
+            const int k = 16 * u + kc;
+            kok[u] = k < F.m;
+            bo[u] = F.BoC + (size_t)(kok[u] ? k : 0) * F.ldx;
+        }
+        v4d acc[CG][MT];
+#pragma unroll
+        for (int g = 0; g < CG; ++g)
+#pragma unroll
+            for (int u = 0; u < MT; ++u) acc[g][u] = v4d{0, 0, 0, 0};
+#pragma unroll 1
+        for (long long r = I.r0; r < I.r1; r += kWR) {
+#pragma unroll 1
+            for (int hf = 0; hf < 2; ++hf) {
+                const long long row = r + 8 * kr + 4 * hf;
+                double a[MT][4], b[CG][4];
+#pragma unroll
+                for (int u = 0; u < MT; ++u) {
+                    float4 v = {0.f, 0.f, 0.f, 0.f};
+                    if (kok[u]) v = *(const float4 *)(bo[u] + row);
+                    a[u][0] = v.x; a[u][1] = v.y; a[u][2] = v.z; a[u][3] = v.w;
+                }
+#pragma unroll
+                for (int g = 0; g < CG; ++g) {
+                    float4 v = {0.f, 0.f, 0.f, 0.f};
+                    if (16 * g < I.nc) v = bx4<RETRY>(F, c[g], row, I.r1);
+                    b[g][0] = v.x; b[g][1] = v.y; b[g][2] = v.z; b[g][3] = v.w;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int g = 0; g < CG; ++g)
+                        if (16 * g < I.nc)
+#pragma unroll
+                            for (int u = 0; u < MT; ++u) acc[g][u] = mfma_f64(a[u][q], b[g][q], acc[g][u]);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < CG; ++g) {
+            const int slot = 16 * g + kc;
+            if (slot >= I.nc) continue;
+            double *dst = tpart + ((size_t)it * kSlots + slot) * ldT;
+#pragma unroll
+            for (int u = 0; u < MT; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) dst[16 * u + kr + 4 * v] = acc[g][u][v];  // D row l/16 + 4v
+        }
+    }
+}
+
+/*
+ * Projection: r = f32(Bx - Bo T) for the item's rows, stored at
+ * rbuf[row * ldr + column], and spart[item * 64 + slot] = sum r^2.
+ * The item's T tile is staged once in LDS (MFMA B operand).  The A operand's
+ * M index M stands for row 4 (M % 4) + M / 4 of the 16-row block, so the D
+ * elements a lane holds (M = l/16 + 4v) are the 4 consecutive rows
+ * 4 (l/16) + v, whose Bx it loads as one float4 each of B and X.
+ */
+template <int MT, int CG, bool RETRY>
+__global__ __launch_bounds__(64, waves_for(MT)) void k_project_w(const FitDev *__restrict__ fits,
+                                                                 const Item *__restrict__ items, int nitems,
+                                                                 const int *__restrict__ cxc,
+                                                                 const int *__restrict__ cpar,
+                                                                 const int *__restrict__ list,
+                                                                 const double *__restrict__ T, int ldT,
+                                                                 double *__restrict__ spart)
+{
+    constexpr int KQ = 4 * MT;           // k steps of 4
+    constexpr int NC = 16 * CG, ST = 16 * MT + 1;
+    __shared__ double sT[NC][ST];
+    const int lane = threadIdx.x, kr = lane >> 4, kc = lane & 15;
+    const int prow = 4 * (kc & 3) + (kc >> 2);  // memory row of this lane's A-operand M index
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const Item I = items[it];
+        const FitDev F = fits[I.fit];
+        __syncthreads();
+        for (int e = lane; e < NC * 16 * MT; e += 64) {
+            const int sl = e / (16 * MT), k = e - sl * (16 * MT);
+            sT[sl][k] = (sl < I.nc && k < F.m) ? T[(size_t)(I.c0 + sl) * ldT + k] : 0.0;
+        }
+        __syncthreads();
+        LaneCand c[CG];
+        double sacc[CG];
+#pragma unroll
+        for (int g = 0; g < CG; ++g) {
+            c[g] = lane_cand<RETRY>(F, I, 16 * g + kc, cxc, cpar, list);
+            sacc[g] = 0.0;
+        }
+#pragma unroll 1
+        for (long long r = I.r0; r < I.r1; r += kWR) {
+#pragma unroll 1
+            for (int h = 0; h < kWR / 16; ++h) {
+                const long long r16 = r + 16 * h;
+                float av[KQ];
+#pragma unroll
+                for (int q = 0; q < KQ; ++q) {
+                    const int k = 4 * q + kr;
+                    av[q] = k < F.m ? F.BoC[(size_t)k * F.ldx + r16 + prow] : 0.0f;
+                }
+#pragma unroll
+                for (int g = 0; g < CG; ++g) {
+                    if (16 * g >= I.nc) continue;
+                    v4d d = v4d{0, 0, 0, 0};
+#pragma unroll
+                    for (int q = 0; q < KQ; ++q) d = mfma_f64((double)av[q], sT[16 * g + kc][4 * q + kr], d);
+                    const long long row = r16 + 4 * kr;
+                    const float4 bx = bx4<RETRY>(F, c[g], row, I.r1);
+                    const float bv[4] = {bx.x, bx.y, bx.z, bx.w};
+                    if (c[g].ok) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            if (row + v < I.r1) {
+                                const float rf = (float)((double)bv[v] - d[v]);
+                                F.rbuf[(row + v) * F.ldr + c[g].col] = rf;
+                                sacc[g] = fma((double)rf, (double)rf, sacc[g]);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < CG; ++g) {
+            double s = sacc[g];
+            s = s + __shfl_xor(s, 16, 64);
+            s = s + __shfl_xor(s, 32, 64);
+            const int slot = 16 * g + kc;
+            if (kr == 0 && slot < I.nc) spart[(size_t)it * kSlots + slot] = s;
+        }
+    }
+}
+
+} // namespace mh
