@@ -40,7 +40,11 @@ sys.path.insert(0, ROOT)
 PEAK_FP64 = 78.6e12     # MI355X FP64 (vector and matrix), FLOP/s
 PEAK_FP32 = 157.3e12    # MI355X FP32 matrix, FLOP/s
 PEAK_HBM = 8.0e12       # MI355X HBM3E, B/s
-FP32_KERNELS = ("mixed_pass_a", "mixed_update")
+FP32_KERNELS = ("mixed_update",)
+# Kernel classes whose event brackets span several kernels and host work (no roofline).
+COMPOSITE = ("mixed_recheck", "dgks_retry")
+# Classes bound by HBM traffic by design (arithmetic intensity below the MFMA ridge point).
+HBM_BOUND = ("mixed_update", "append", "finalize", "sort", "argmax", "mixed_score")
 CPU_SAMPLE_S = 20.0
 
 
@@ -88,7 +92,7 @@ def run_batch(fits, fit_kw, precision, steps, warmup):
     seq = [[(r["type"].decode(), int(r["basis"]), int(r["input"])) for r in m] for m in models[0]]
     st = stats[0]
     return dict(wall=wall, fit_s=wall / steps, upload_s=upload_s, cands=st["scored"], kstats=ks, seq=seq,
-                mse=st["mse"], epochs=st["steps"], dgks=st["dgks_fired"], rescored=st["rescored"],
+                mse=st["mse"], epochs=st["steps"], dgks=st["dgks_fired"], rescored=st["rescored"], gated=st["gated"],
                 terms=sum(len(m) for m in models[0]))
 
 
@@ -132,9 +136,9 @@ def run_form(X, y, fit_kw, precision, steps, warmup):
 
 
 def roofline(ks):
-    name, d = max(ks.items(), key=lambda kv: kv[1]["ms"])
+    name, d = max(((k, v) for k, v in ks.items() if k not in COMPOSITE), key=lambda kv: kv[1]["ms"])
     avg_s = d["ms"] / 1e3 / d["launches"]
-    if d["flops"] > 0:
+    if d["flops"] > 0 and name not in HBM_BOUND:
         peak = PEAK_FP32 if name in FP32_KERNELS else PEAK_FP64
         ach = d["flops"] / d["launches"] / avg_s
         return name, dict(bound="mfma", achieved=ach / 1e12, peak=peak / 1e12, unit="TFLOP/s",
@@ -215,6 +219,34 @@ def cpu_baseline_batch(fits, fit_kw, budget_s):
                 sample="fits 0..%d of the 256, complete (%d candidates, %.1f s)" % (k - 1, cands, spent))
 
 
+def copy_probe(mib=1024, reps=4):
+    """A plain device copy of known size (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read_per_dispatch": n * 4, "copy_probe_bytes_written_per_dispatch": n * 4,
+                      "dispatches": reps}))
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if there is one."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    e = t.get("kernels", {}).get(kernel)
+    if not e or e.get("workload") != workload:
+        return None
+    return e["hbm_bytes_per_launch"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,7 +256,11 @@ def main():
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=CPU_SAMPLE_S)
+    ap.add_argument("--copy-probe", action="store_true", help="run only a 1 GiB device copy (PMC calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design "
               "is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
@@ -242,6 +278,7 @@ def main():
         workload = "one fit: N=%d, p=%d, %s" % (X.shape[0], X.shape[1], fit_kw)
     head = res[forms[-1]]
     kname, roof = roofline(head["kstats"])
+    roof["traffic"] = measured_traffic(kname, args.workload)
     out = {
         "metric": "forward-pass fit s + candidates/s, mixed vs FP64 GPU at same term sequence",
         "value": head["cands"] / head["fit_s"],
@@ -274,6 +311,7 @@ def main():
                                       for a, b in zip(ma, mb))
     if "rescored" in head:
         out["rescored"] = head["rescored"]
+        out["rescored_gated"] = head["gated"]
     if not args.no_cpu:
         if args.workload == 4:
             out["cpu_baseline"] = cpu_baseline_batch(data, fit_kw, args.cpu_budget)

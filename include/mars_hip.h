@@ -174,9 +174,10 @@ int mars_hip_batch_create(int nfits, const float *const *X, const size_t *ldX, c
 int mars_hip_batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts *opts, mars_hip_term *terms, int *nterms,
                        int *steps);
 
-/* Candidates scored since creation (every form), and those the mixed form rescored
- * exactly in FP64 (0 for the plain form). */
-int mars_hip_score_stats(mars_hip_ctx *ctx, long *scored, long *rescored);
+/* Candidates scored since creation (every form); those the mixed form rescored
+ * exactly in FP64, and how many of them because their DGKS gate tripped in the
+ * bulk pass (both 0 for the plain form).  Any pointer may be NULL. */
+int mars_hip_score_stats(mars_hip_ctx *ctx, long *scored, long *rescored, long *gated);
 
 /* Per-kernel-class timing with HIP events on the context's stream (off by
  * default).  Each record: launches, summed device milliseconds and the

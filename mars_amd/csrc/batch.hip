@@ -191,6 +191,7 @@ int mars_hip_batch_create(int nfits, const float *const *X, const size_t *ldX, c
         try {
             ctx_base_init(ctx, max_terms);
             ctx->resident = true;
+            ctx->row_major = false;  // the batch scores linear terms only: column-major copies suffice
             ctx->form = form;
             ctx->fits.resize(nfits);
             for (int f = 0; f < nfits; ++f) {

@@ -62,8 +62,8 @@ struct DBuf {
 struct FitDev {
     const float *X;    // raw X, column-major (ldx)
     const float *s;    // per-column 1/rms scale
-    const float *B;    // basis, row-major (ldb)
-    const float *Bo;   // orthonormal basis, row-major (ldb)
+    const float *B;    // basis, row-major (ldb); null in batch contexts
+    const float *Bo;   // orthonormal basis, row-major (ldb); null in batch contexts
     const float *BoC;  // the same basis, column-major (ldx)
     const float *BC;   // basis B, column-major (ldx)
     const float *y;    // normalized target (f32)
@@ -112,7 +112,7 @@ struct Item {
     long long r0, r1;
 };
 
-/* Writable basis storage of one fit (append). */
+/* Writable basis storage of one fit (append); B and Bo are null without row-major copies. */
 struct FitW {
     float *B, *Bo, *BoC, *BC;
     double *ybo;
@@ -179,6 +179,7 @@ struct mars_hip_ctx {
     size_t ldb = 0;        // row stride of B and Bo (>= max_terms, multiple of 4)
     int form = MARS_HIP_FORM_PLAIN;
     bool resident = false;
+    bool row_major = true;  // keep row-major B and Bo (hinge sweep, basis download); batch contexts do not
     std::vector<mh::FitState> fits;
 
     // scoring workspace (shared by all fits)
@@ -190,7 +191,7 @@ struct mars_hip_ctx {
     mh::DBuf<int> d_cmap;            // per-candidate first item / item count per pass
     mh::DBuf<mh::Item> d_ritems;     // DGKS retry passes (separate: the first passes' lists stay live)
     mh::DBuf<int> d_rcmap;
-    mh::DBuf<double> tpart, T, tnorm, spart, ssum, ybx, lin, hin, aux;
+    mh::DBuf<double> tpart, T, T2, tnorm, spart, ssum, ybx, lin, hin, aux;
     mh::DBuf<int> flag, fired_list, fired_count, knot, krow, ival;
     mh::DBuf<long long> dgks;        // device DGKS counter
     mh::DBuf<double> red;            // reduction scratch
@@ -208,7 +209,8 @@ struct mars_hip_ctx {
 
     long dgks_host = 0;
     double eval_ms = 0;
-    long scored = 0, rescored = 0;  // mixed form: candidates scored / rescored in FP64
+    long scored = 0, rescored = 0;  // candidates scored / rescored in FP64 by the mixed form
+    long gated = 0;                 // mixed form: rescored because the bulk DGKS gate tripped
 
     // optional per-kernel timing
     bool timing = false;

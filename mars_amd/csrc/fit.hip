@@ -68,8 +68,8 @@ __global__ __launch_bounds__(256) void k_init2(double *__restrict__ yd, const do
             const float yf = (float)d;
             const float b = (float)s;
             y32[i] = yf;
-            B[i * ldb] = b;
-            Bo[i * ldb] = b;
+            if (B) B[i * ldb] = b;
+            if (Bo) Bo[i * ldb] = b;
             BoC[i] = b;
             BC[i] = b;
             ab = fma((double)b, (double)yf, ab);
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void k_app_build(const FitDev *__restrict__ fi
     const float *xp = F.X + (size_t)A.xcol * F.ldx;
     double a = 0.0;
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < F.n; i += (long long)nb * 256) {
-        const float sb = __fmul_rn(s, F.B[i * F.ldb + A.bcol]);
+        const float sb = __fmul_rn(s, F.BC[(size_t)A.bcol * F.ldx + i]);
         float v;
         if (A.type == 0)
             v = __fmul_rn(sb, xp[i]);
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(256) void k_app_mgs(const FitDev *__restrict__ fits
         double vi = F.v[i];
         if (j == 0 && first) {
             const float b = __fdiv_rn((float)vi, rn);
-            B[i * F.ldb + m] = b;
+            if (B) B[i * F.ldb + m] = b;
             BC[i] = b;
         }
         if (j > 0) {
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(256) void k_app_store(const FitDev *__restrict__ fi
     if (ok) {
         for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < F.n; i += (long long)nb * 256) {
             const float b = (float)(F.v[i] / w);
-            Bo[i * F.ldb + F.m] = b;
+            if (Bo) Bo[i * F.ldb + F.m] = b;
             BoC[i] = b;
             a = fma((double)b, (double)F.y[i], a);
         }
@@ -440,10 +440,12 @@ void fit_state_alloc(mars_hip_ctx *ctx, FitState &fs, size_t n, double tol)
     require(n > 0, "empty dataset");
     fs.n = n;
     fs.tol = tol;
-    fs.B.ensure(n * ctx->ldb);
-    fs.Bo.ensure(n * ctx->ldb);
-    MH_CHECK(hipMemsetAsync(fs.B.p, 0, n * ctx->ldb * sizeof(float), ctx->stream));
-    MH_CHECK(hipMemsetAsync(fs.Bo.p, 0, n * ctx->ldb * sizeof(float), ctx->stream));
+    if (ctx->row_major) {
+        fs.B.ensure(n * ctx->ldb);
+        fs.Bo.ensure(n * ctx->ldb);
+        MH_CHECK(hipMemsetAsync(fs.B.p, 0, n * ctx->ldb * sizeof(float), ctx->stream));
+        MH_CHECK(hipMemsetAsync(fs.Bo.p, 0, n * ctx->ldb * sizeof(float), ctx->stream));
+    }
     fs.ldx = (n + 31) / 32 * 32;
     fs.BoC.ensure(fs.ldx * ctx->ldb);
     MH_CHECK(hipMemsetAsync(fs.BoC.p, 0, fs.ldx * ctx->ldb * sizeof(float), ctx->stream));
@@ -460,7 +462,7 @@ void ctx_free(mars_hip_ctx *ctx)
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &fs : ctx->fits) fs.release();
-    DBuf<double> *db[] = {&ctx->tpart, &ctx->T, &ctx->tnorm, &ctx->spart, &ctx->ssum, &ctx->ybx,
+    DBuf<double> *db[] = {&ctx->tpart, &ctx->T, &ctx->T2, &ctx->tnorm, &ctx->spart, &ctx->ssum, &ctx->ybx,
                           &ctx->lin, &ctx->hin, &ctx->aux, &ctx->red, &ctx->vcol, &ctx->scal,
                           &ctx->dense, &ctx->hstate, &ctx->hbest, &ctx->rlin, &ctx->rybx};
     for (auto *b : db) b->release();
@@ -724,6 +726,7 @@ void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t 
         if (ctx->form == MARS_HIP_FORM_MIXED) {
             mixed_score(ctx);
         } else {
+            require(linear_only || ctx->row_major, "hinge sweeps need the row-major basis copies");
             plain_score(ctx, cs, !linear_only, ctx->lin.p, ctx->ybx.p);
             if (!linear_only) hinge_sweep(ctx, min_span, end_span);
         }
@@ -1098,18 +1101,19 @@ int mars_hip_fit_basis(mars_hip_ctx *ctx, float *B, float *Bo, size_t *m_out)
 {
     return guarded([&] {
         require(ctx != nullptr, "null context");
+        require(!ctx->fits.empty(), "empty context");
         const FitState &fs = ctx->fits[0];
-        const size_t n = fs.n, m = fs.m, ldb = ctx->ldb;
+        const size_t n = fs.n, m = fs.m, ld = fs.ldx;
         if (m_out) *m_out = m;
-        std::vector<float> h(n * ldb);
+        std::vector<float> h(ld * std::max<size_t>(m, 1));
         float *dst[2] = {B, Bo};
-        const float *src[2] = {fs.B.p, fs.Bo.p};
+        const float *src[2] = {fs.BC.p, fs.BoC.p};  // column-major copies (always present)
         for (int q = 0; q < 2; ++q) {
-            if (!dst[q]) continue;
-            MH_CHECK(hipMemcpyAsync(h.data(), src[q], n * ldb * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+            if (!dst[q] || m == 0) continue;
+            MH_CHECK(hipMemcpyAsync(h.data(), src[q], ld * m * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
             MH_CHECK(hipStreamSynchronize(ctx->stream));
             for (size_t i = 0; i < n; ++i)
-                for (size_t j = 0; j < m; ++j) dst[q][i * m + j] = h[i * ldb + j];
+                for (size_t j = 0; j < m; ++j) dst[q][i * m + j] = h[j * ld + i];
         }
     });
 }
@@ -1148,12 +1152,13 @@ int mars_hip_kernel_stats(mars_hip_ctx *ctx, mars_hip_kstat *out, int capacity, 
     });
 }
 
-int mars_hip_score_stats(mars_hip_ctx *ctx, long *scored, long *rescored)
+int mars_hip_score_stats(mars_hip_ctx *ctx, long *scored, long *rescored, long *gated)
 {
     return guarded([&] {
         require(ctx != nullptr, "null context");
         if (scored) *scored = ctx->scored;
         if (rescored) *rescored = ctx->rescored;
+        if (gated) *gated = ctx->gated;
     });
 }
 

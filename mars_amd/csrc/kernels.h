@@ -13,6 +13,26 @@ constexpr int kWave = 64;
 constexpr int kRedThreads = 256;
 
 /*
+ * Load through the global address space.  Pointers read from device structs
+ * are generic to the compiler, which then emits flat loads; every pointer the
+ * kernels dereference this way points at hipMalloc'd memory, where generic and
+ * global addresses coincide.
+ */
+template <typename T>
+__device__ inline T gld(const T *p)
+{
+    return *__builtin_bit_cast(const __attribute__((address_space(1))) T *, p);
+}
+
+/* 16-byte global load (p 16-byte aligned). */
+__device__ inline float4 gld4(const float *p)
+{
+    typedef float nf4 __attribute__((ext_vector_type(4)));
+    const nf4 v = *__builtin_bit_cast(const __attribute__((address_space(1))) nf4 *, p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+/*
  * MFMA f64 16x16x4 (measured on gfx950):
  *   A (16x4):  lane l holds A[l%16][l/16]
  *   B (4x16):  lane l holds B[l/16][l%16]

@@ -158,13 +158,14 @@ public:
         long fired = 0;
         double ms = 0;
         check(mars_hip_stats(ctx_, &fired, &ms));
-        long sc = 0, re = 0;
-        check(mars_hip_score_stats(ctx_, &sc, &re));
+        long sc = 0, re = 0, ga = 0;
+        check(mars_hip_score_stats(ctx_, &sc, &re, &ga));
         py::dict d;
         d["dgks_fired"] = fired;
         d["eval_ms"] = ms;
         d["scored"] = sc;
         d["rescored"] = re;
+        d["gated"] = ga;
         return d;
     }
 
@@ -293,13 +294,14 @@ public:
         long fired = 0;
         double ms = 0;
         check(mars_hip_stats(ctx_, &fired, &ms));
-        long sc = 0, re = 0;
-        check(mars_hip_score_stats(ctx_, &sc, &re));
+        long sc = 0, re = 0, ga = 0;
+        check(mars_hip_score_stats(ctx_, &sc, &re, &ga));
         py::dict d;
         d["dgks_fired"] = fired;
         d["eval_ms"] = ms;
         d["scored"] = sc;
         d["rescored"] = re;
+        d["gated"] = ga;
         return d;
     }
 

@@ -32,158 +32,157 @@ namespace mh {
 namespace {
 
 constexpr int kChunk = 32;         // rows per FP32 accumulation chunk (folded into FP64)
-constexpr int kXT = 64;            // X columns per update item (4 waves x 16)
+constexpr int kXT = 64;            // X columns per update item (4 groups of 16)
 constexpr double kWindow = 1e-4;   // relative near-tie window rescored in FP64
-constexpr int kGroupInts = 6;      // fit, xc0, nc, j, first item, item count
+constexpr int kGroupInts = 7;      // fit, xc0, nc, j, a-tile, first item, item count
 
 /*
- * Cache update for new basis column j = item.pad of one fit: for the item's
- * X columns [c0, c0+nc) and rows [r0, r1), partial sums of
- *   U[a][xc] = sum_i A[i][a] xs[i][xc],   a < 2j+1,
+ * Cache update for new basis column j of one fit.  Item = (fit, A tile u of
+ * 16 columns, X columns [c0, c0+nc), rows [r0, r1)), item.pad = j << 8 | u;
+ * partial sums
+ *   U[a][xc] = sum_i A[i][a] xs[i][xc],   a = 16u + (0..15) < 2j+1,
  *   A[i][a] = f32(Bo[i][a] B[i][j])          (a <= j:  k = a,  parent j)
  *           = f32(Bo[i][j] B[i][a-j-1])      (a >  j:  k = j,  parent a-j-1)
- * and of |Bx|^2, Bx.y for Bx = f32(B[:,j] xs[:,xc]).
- * Per 32-row chunk the block stages the Bo and B rows (columns <= j) and the
- * 64 scaled X columns with 16-byte loads, forms the chunk's A tile once in
- * LDS, and wave w runs the MFMAs of X columns 16w..16w+15.  Rows r0 are
- * multiples of 32; X and y are padded to a multiple of 32 rows, so the vector
- * loads stay inside their buffers.
+ * and, in the u == 0 items, of |Bx|^2 and Bx.y for Bx = f32(B[:,j] xs[:,xc]).
+ * One wave per item, no LDS: MFMA step q of a 4-row half-step at row r takes
+ * row r + 8 (lane/16) + q, so each lane's operands are float4 runs of the
+ * column-major BoC, BC and X (zero-padded to a multiple of 32 rows).
+ * v_mfma_f32_16x16x4f32 accumulates each 32-row chunk in FP32; the chunk sums
+ * are folded into FP64.
  */
-template <int MT>
-__global__ __launch_bounds__(256) void k_tc_update(const FitDev *__restrict__ fits,
-                                                   const Item *__restrict__ items, int nitems, int ldA,
-                                                   double *__restrict__ tpart, double *__restrict__ npart)
+__global__ __launch_bounds__(64, 3) void k_tc_update(const FitDev *__restrict__ fits,
+                                                     const Item *__restrict__ items, int nitems,
+                                                     double *__restrict__ tpart, double *__restrict__ npart)
 {
-    constexpr int SW = 8 * MT;  // staged basis columns: j + 1 <= SW since 2j + 1 <= 16 MT
-    constexpr int KW = 16 * MT;
-    __shared__ float sBo[kChunk][SW + 4];
-    __shared__ float sB[kChunk][SW + 4];
-    __shared__ float sA[kChunk][KW + 4];
-    __shared__ float sX[kChunk][kXT + 4];
-    __shared__ double sN[2][4][kXT];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int kr = lane >> 4, kc = lane & 15;
-    const int sc = t & (kXT - 1), g = t >> 6;  // staging role: X column sc, rows 8g..8g+7
-    const float4 zero4 = {0.f, 0.f, 0.f, 0.f};
+    const int lane = threadIdx.x, kr = lane >> 4, kc = lane & 15;
+    const float4 z4 = {0.f, 0.f, 0.f, 0.f};
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const Item I = items[it];
         const FitDev F = fits[I.fit];
-        const int j = I.pad, na = 2 * j + 1, mtf = (na + 15) / 16, q4 = (j + 4) / 4;
-        const bool sok = sc < I.nc;
-        const int xc = I.c0 + (sok ? sc : 0);
-        const float sx = F.s[xc];
-        const float *xp = F.X + (size_t)xc * F.ldx;
-        v4d acc[MT];
+        const int j = I.pad >> 8, u = I.pad & 255, na = 2 * j + 1;
+        const int a = 16 * u + kc;
+        const bool aok = a < na;
+        const int ka = a <= j ? a : j, pa = a <= j ? j : a - j - 1;
+        const float *pbo = F.BoC + (size_t)(aok ? ka : 0) * F.ldx;
+        const float *pb = F.BC + (size_t)(aok ? pa : 0) * F.ldx;
+        const float *pbj = F.BC + (size_t)j * F.ldx;
+        const int G = (I.nc + 15) >> 4;
+        const float *px[4];
+        float sx[4];
+        bool xok[4];
 #pragma unroll
-        for (int u = 0; u < MT; ++u) acc[u] = v4d{0, 0, 0, 0};
-        double bxx = 0.0, bxy = 0.0;
-        const bool wok = 16 * w < I.nc;
+        for (int g = 0; g < 4; ++g) {
+            const int slot = 16 * g + kc;
+            xok[g] = slot < I.nc;
+            const int xc = I.c0 + (xok[g] ? slot : 0);
+            px[g] = F.X + (size_t)xc * F.ldx;
+            sx[g] = xok[g] ? F.s[xc] : 0.0f;
+        }
+        const bool norms = u == 0;
+        v4d acc[4];
+        double bxx[4], bxy[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            acc[g] = v4d{0, 0, 0, 0};
+            bxx[g] = 0.0;
+            bxy[g] = 0.0;
+        }
+#pragma unroll 1
         for (long long r = I.r0; r < I.r1; r += kChunk) {
-            const int rc = (int)(I.r1 - r < kChunk ? I.r1 - r : kChunk);
-            __syncthreads();
-            for (int e = t; e < kChunk * q4; e += 256) {
-                const int i = e / q4, q = e - i * q4;
-                float4 a = zero4, b = zero4;
-                if (i < rc) {
-                    a = *(const float4 *)(F.Bo + (r + i) * F.ldb + 4 * q);
-                    b = *(const float4 *)(F.B + (r + i) * F.ldb + 4 * q);
-                }
-                *(float4 *)&sBo[i][4 * q] = a;
-                *(float4 *)&sB[i][4 * q] = b;
-            }
-            float xv[8];
-            {
-                float4 x0 = zero4, x1 = zero4;
-                if (sok) {
-                    const float4 *src = (const float4 *)(xp + r + 8 * g);
-                    x0 = src[0];
-                    x1 = src[1];
-                }
-                xv[0] = x0.x; xv[1] = x0.y; xv[2] = x0.z; xv[3] = x0.w;
-                xv[4] = x1.x; xv[5] = x1.y; xv[6] = x1.z; xv[7] = x1.w;
+            // every load of the chunk first (clamped addresses, masked by selects below), so they
+            // are all in flight together
+            float4 o[2], bb[2], bj[2], yv[2], xv[2][4];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    xv[u] = (8 * g + u < rc) ? __fmul_rn(xv[u], sx) : 0.0f;
-                    sX[8 * g + u][sc] = xv[u];
-                }
-            }
-            __syncthreads();
-            for (int e = t; e < kChunk * na; e += 256) {
-                const int i = e / na, a = e - i * na;
-                const int ka = a <= j ? a : j, pa = a <= j ? j : a - j - 1;
-                sA[i][a] = __fmul_rn(sBo[i][ka], sB[i][pa]);
-            }
-            for (int e = t; e < kChunk * (16 * mtf - na); e += 256) {
-                const int i = e / (16 * mtf - na), a = na + e - i * (16 * mtf - na);
-                sA[i][a] = 0.0f;
-            }
-            if (sok) {
-                const float4 *yp = (const float4 *)(F.y + r + 8 * g);
-                const float4 y0 = yp[0], y1 = yp[1];
-                const float yv[8] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w};
+            for (int hf = 0; hf < 2; ++hf) {
+                const long long rb = r + 8 * kr + 4 * hf;
+                // no branches here: a branch join would make the compiler wait for the loads
+                o[hf] = gld4(pbo + rb);
+                bb[hf] = gld4(pb + rb);
+                bj[hf] = gld4(pbj + rb);
+                yv[hf] = gld4(F.y + rb);
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const double bx = (double)__fmul_rn(sB[8 * g + u][j], xv[u]);
-                    bxx = fma(bx, bx, bxx);
-                    bxy = fma(bx, (double)yv[u], bxy);
+                for (int g = 0; g < 4; ++g) xv[hf][g] = gld4(px[g] + rb);
+            }
+            v4f part[4];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) part[g] = v4f{0, 0, 0, 0};
+#pragma unroll
+            for (int hf = 0; hf < 2; ++hf) {
+                const float ov[4] = {o[hf].x, o[hf].y, o[hf].z, o[hf].w};
+                const float bv[4] = {bb[hf].x, bb[hf].y, bb[hf].z, bb[hf].w};
+                const float jv[4] = {bj[hf].x, bj[hf].y, bj[hf].z, bj[hf].w};
+                const float yy[4] = {yv[hf].x, yv[hf].y, yv[hf].z, yv[hf].w};
+                float av[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) av[q] = aok ? __fmul_rn(ov[q], bv[q]) : 0.0f;
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    if (g >= G) continue;
+                    const float xx[4] = {xv[hf][g].x, xv[hf][g].y, xv[hf][g].z, xv[hf][g].w};
+                    float xs[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) xs[q] = xok[g] ? __fmul_rn(xx[q], sx[g]) : 0.0f;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) part[g] = mfma_f32(av[q], xs[q], part[g]);
+                    if (norms) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const double bx = (double)__fmul_rn(jv[q], xs[q]);
+                            bxx[g] = fma(bx, bx, bxx[g]);
+                            bxy[g] = fma(bx, (double)yy[q], bxy[g]);
+                        }
+                    }
                 }
             }
-            __syncthreads();
-            if (wok) {
-                v4f part[MT];
 #pragma unroll
-                for (int u = 0; u < MT; ++u) part[u] = v4f{0, 0, 0, 0};
+            for (int g = 0; g < 4; ++g)
 #pragma unroll
-                for (int q = 0; q < kChunk; q += 4) {
-                    const float b = sX[q + kr][16 * w + kc];
-#pragma unroll
-                    for (int u = 0; u < MT; ++u)
-                        if (u < mtf) part[u] = mfma_f32(sA[q + kr][16 * u + kc], b, part[u]);
-                }
-#pragma unroll
-                for (int u = 0; u < MT; ++u)
-                    if (u < mtf)
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) acc[u][v] += (double)part[u][v];
-            }
+                for (int v = 0; v < 4; ++v) acc[g][v] += (double)part[g][v];
         }
-        const int slot = 16 * w + kc;
-        if (wok && slot < I.nc) {
-            double *dst = tpart + ((size_t)it * kXT + slot) * ldA;
 #pragma unroll
-            for (int u = 0; u < MT; ++u)
-                if (u < mtf)
+        for (int g = 0; g < 4; ++g) {
+            if (g >= G) continue;
+            const int slot = 16 * g + kc;
+            // D row 4(l/16)+v is A column 16u + 4 kr + v
+            double *dst = tpart + ((size_t)it * kXT + slot) * 16 + 4 * kr;
+            if (slot < I.nc)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) dst[16 * u + 4 * kr + v] = acc[u][v];  // D row 4(l/16)+v
-        }
-        sN[0][g][sc] = bxx;
-        sN[1][g][sc] = bxy;
-        __syncthreads();
-        if (g == 0 && sok) {
-            npart[((size_t)it * kXT + sc) * 2] = ((sN[0][0][sc] + sN[0][1][sc]) + sN[0][2][sc]) + sN[0][3][sc];
-            npart[((size_t)it * kXT + sc) * 2 + 1] = ((sN[1][0][sc] + sN[1][1][sc]) + sN[1][2][sc]) + sN[1][3][sc];
+                for (int v = 0; v < 4; ++v) dst[v] = acc[g][v];
+            if (norms) {
+                double s0 = bxx[g], s1 = bxy[g];
+                s0 = s0 + __shfl_xor(s0, 16, 64);
+                s0 = s0 + __shfl_xor(s0, 32, 64);
+                s1 = s1 + __shfl_xor(s1, 16, 64);
+                s1 = s1 + __shfl_xor(s1, 32, 64);
+                if (kr == 0 && slot < I.nc) {
+                    npart[((size_t)it * kXT + slot) * 2] = s0;
+                    npart[((size_t)it * kXT + slot) * 2 + 1] = s1;
+                }
+            }
         }
     }
 }
 
-/* Sum each (fit, X-column tile) group's row partials in item order and write the new cache entries. */
+/* Sum each (fit, A tile, X-column block) group's row partials in item order and write the new cache entries. */
 __global__ __launch_bounds__(256) void k_tc_reduce(const FitDev *__restrict__ fits, const int *__restrict__ groups,
-                                                   int ldA, const double *__restrict__ tpart,
+                                                   const double *__restrict__ tpart,
                                                    const double *__restrict__ npart)
 {
     const int *G = groups + (size_t)blockIdx.x * kGroupInts;
-    const int fit = G[0], xc0 = G[1], nc = G[2], j = G[3], first = G[4], count = G[5];
+    const int fit = G[0], xc0 = G[1], nc = G[2], j = G[3], u = G[4], first = G[5], count = G[6];
     const FitDev F = fits[fit];
     const int na = 2 * j + 1;
     const long long ldb = F.ldb;
-    for (int e = threadIdx.x; e < nc * na; e += 256) {
-        const int xl = e / na, a = e - xl * na;
+    for (int e = threadIdx.x; e < nc * 16; e += 256) {
+        const int xl = e >> 4, al = e & 15, a = 16 * u + al;
+        if (a >= na) continue;
         double s = 0.0;
-        for (int i = 0; i < count; ++i) s += tpart[((size_t)(first + i) * kXT + xl) * ldA + a];
+        for (int i = 0; i < count; ++i) s += tpart[((size_t)(first + i) * kXT + xl) * 16 + al];
         const long long xc = xc0 + xl;
         const int k = a <= j ? a : j, par = a <= j ? j : a - j - 1;
         F.tc[(xc * ldb + par) * ldb + k] = s;
     }
+    if (u != 0) return;
     for (int xl = threadIdx.x; xl < nc; xl += 256) {
         double s0 = 0.0, s1 = 0.0;
         for (int i = 0; i < count; ++i) {
@@ -228,37 +227,13 @@ __global__ __launch_bounds__(256) void k_tc_score(const FitDev *__restrict__ fit
     score[c] = gate ? 0.0 : (ry * ry) / (s + F->tol);
 }
 
-template <typename... A>
-void launch_update(int mt, dim3 g, hipStream_t st, A... args)
-{
-    switch (mt) {
-    case 1: hipLaunchKernelGGL(k_tc_update<1>, g, dim3(256), 0, st, args...); break;
-    case 2: hipLaunchKernelGGL(k_tc_update<2>, g, dim3(256), 0, st, args...); break;
-    case 3: hipLaunchKernelGGL(k_tc_update<3>, g, dim3(256), 0, st, args...); break;
-    case 4: hipLaunchKernelGGL(k_tc_update<4>, g, dim3(256), 0, st, args...); break;
-    case 5: hipLaunchKernelGGL(k_tc_update<5>, g, dim3(256), 0, st, args...); break;
-    case 6: hipLaunchKernelGGL(k_tc_update<6>, g, dim3(256), 0, st, args...); break;
-    case 7: hipLaunchKernelGGL(k_tc_update<7>, g, dim3(256), 0, st, args...); break;
-    case 8: hipLaunchKernelGGL(k_tc_update<8>, g, dim3(256), 0, st, args...); break;
-    case 9: hipLaunchKernelGGL(k_tc_update<9>, g, dim3(256), 0, st, args...); break;
-    case 10: hipLaunchKernelGGL(k_tc_update<10>, g, dim3(256), 0, st, args...); break;
-    case 11: hipLaunchKernelGGL(k_tc_update<11>, g, dim3(256), 0, st, args...); break;
-    case 12: hipLaunchKernelGGL(k_tc_update<12>, g, dim3(256), 0, st, args...); break;
-    case 13: hipLaunchKernelGGL(k_tc_update<13>, g, dim3(256), 0, st, args...); break;
-    case 14: hipLaunchKernelGGL(k_tc_update<14>, g, dim3(256), 0, st, args...); break;
-    case 15: hipLaunchKernelGGL(k_tc_update<15>, g, dim3(256), 0, st, args...); break;
-    default: hipLaunchKernelGGL(k_tc_update<16>, g, dim3(256), 0, st, args...); break;
-    }
-    MH_LAUNCH_CHECK();
-}
-
-constexpr size_t kMaxMixedTerms = 128;  // 2j+1 <= 255 cache columns per update: MT <= 16
+constexpr size_t kMaxMixedTerms = 256;  // cache: p_x * ldb^2 doubles per fit
 
 /* Allocate the caches of fits that have none yet. */
 void ensure_tcache(mars_hip_ctx *ctx)
 {
     for (FitState &fs : ctx->fits) {
-        require(fs.m <= kMaxMixedTerms, "the mixed-precision form supports up to 128 basis functions");
+        require(fs.m <= kMaxMixedTerms, "the mixed-precision form supports up to 256 basis functions");
         if (fs.tc.p) continue;
         fs.tc.ensure(fs.px * ctx->ldb * ctx->ldb);
         fs.tcn.ensure(fs.px * ctx->ldb * 2);
@@ -273,37 +248,36 @@ void tc_update(mars_hip_ctx *ctx)
     hipStream_t st = ctx->stream;
     while (true) {
         std::vector<int> jf(F, -1);
-        size_t jmax = 0;
         double work = 0, flops = 0, bytes = 0;
         for (size_t f = 0; f < F; ++f) {
             const FitState &fs = ctx->fits[f];
             if (fs.mc >= fs.m) continue;
             jf[f] = (int)fs.mc;
-            jmax = std::max(jmax, fs.mc);
-            work += (double)fs.n * ((fs.px + kXT - 1) / kXT);
+            const size_t tiles = (2 * fs.mc + 1 + 15) / 16;
+            work += (double)fs.n * tiles * ((fs.px + kXT - 1) / kXT);
             flops += 2.0 * fs.n * (2.0 * fs.mc + 1) * fs.px;
             bytes += 4.0 * fs.n * (2.0 * (fs.mc + 1) + fs.px + 1);
         }
         if (work == 0) break;
-        const int mt = (int)((2 * jmax + 1 + 15) / 16);
-        const int ldA = 16 * mt;
         std::vector<Item> items;
         std::vector<int> groups;
         for (size_t f = 0; f < F; ++f) {
             if (jf[f] < 0) continue;
             const FitState &fs = ctx->fits[f];
             const long long n = (long long)fs.n;
-            long long splits = std::max<long long>(1, std::llround(2048.0 * n / work));
+            const int tiles = (2 * jf[f] + 1 + 15) / 16;
+            long long splits = std::max<long long>(1, std::llround(8192.0 * n / work));
             splits = std::min<long long>(splits, std::max<long long>(1, n / 256));
             const long long rows = ((n + splits - 1) / splits + kChunk - 1) / kChunk * kChunk;
-            for (size_t xc0 = 0; xc0 < fs.px; xc0 += kXT) {
-                const int nc = (int)std::min<size_t>(kXT, fs.px - xc0);
-                const int first = (int)items.size();
-                for (long long r0 = 0; r0 < n; r0 += rows)
-                    items.push_back(Item{(int)f, (int)xc0, nc, jf[f], r0, std::min(n, r0 + rows)});
-                const int g[kGroupInts] = {(int)f, (int)xc0, nc, jf[f], first, (int)items.size() - first};
-                groups.insert(groups.end(), g, g + kGroupInts);
-            }
+            for (int u = 0; u < tiles; ++u)
+                for (size_t xc0 = 0; xc0 < fs.px; xc0 += kXT) {
+                    const int nc = (int)std::min<size_t>(kXT, fs.px - xc0);
+                    const int first = (int)items.size();
+                    for (long long r0 = 0; r0 < n; r0 += rows)
+                        items.push_back(Item{(int)f, (int)xc0, nc, (jf[f] << 8) | u, r0, std::min(n, r0 + rows)});
+                    const int g[kGroupInts] = {(int)f, (int)xc0, nc, jf[f], u, first, (int)items.size() - first};
+                    groups.insert(groups.end(), g, g + kGroupInts);
+                }
         }
         const size_t ni = items.size(), ng = groups.size() / kGroupInts;
         ctx->d_items.ensure(ni);
@@ -311,18 +285,18 @@ void tc_update(mars_hip_ctx *ctx)
         MH_CHECK(hipMemcpyAsync(ctx->d_items.p, items.data(), ni * sizeof(Item), hipMemcpyHostToDevice, st));
         MH_CHECK(hipMemcpyAsync(ctx->d_cmap.p, groups.data(), groups.size() * sizeof(int), hipMemcpyHostToDevice,
                                 st));
-        ctx->tpart.ensure(ni * kXT * ldA);
+        ctx->tpart.ensure(ni * kXT * 16);
         ctx->aux.ensure(ni * kXT * 2);
         {
-            KTimer kt(ctx, K_MIXED_A, flops, bytes);
-            const dim3 grid((unsigned)std::max<size_t>(1, std::min<size_t>(ni, 4096)));
-            launch_update(mt, grid, st, (const FitDev *)ctx->d_fits.p, (const Item *)ctx->d_items.p, (int)ni, ldA,
-                          ctx->tpart.p, ctx->aux.p);
-            hipLaunchKernelGGL(k_tc_reduce, dim3((unsigned)ng), dim3(256), 0, st, (const FitDev *)ctx->d_fits.p,
-                               (const int *)ctx->d_cmap.p, ldA, (const double *)ctx->tpart.p,
-                               (const double *)ctx->aux.p);
+            KTimer kt(ctx, K_MIXED_A, flops, bytes);  // the update kernel alone
+            const dim3 grid((unsigned)std::max<size_t>(1, std::min<size_t>(ni, 65536)));
+            hipLaunchKernelGGL(k_tc_update, grid, dim3(64), 0, st, (const FitDev *)ctx->d_fits.p,
+                               (const Item *)ctx->d_items.p, (int)ni, ctx->tpart.p, ctx->aux.p);
             MH_LAUNCH_CHECK();
         }
+        hipLaunchKernelGGL(k_tc_reduce, dim3((unsigned)ng), dim3(256), 0, st, (const FitDev *)ctx->d_fits.p,
+                           (const int *)ctx->d_cmap.p, (const double *)ctx->tpart.p, (const double *)ctx->aux.p);
+        MH_LAUNCH_CHECK();
         for (size_t f = 0; f < F; ++f)
             if (jf[f] >= 0) ctx->fits[f].mc += 1;
     }
@@ -369,8 +343,10 @@ void mixed_score(mars_hip_ctx *ctx)
         double mx = 0.0;
         for (int c = b; c < e; ++c)
             if (!flags[c]) mx = std::max(mx, approx[c]);
-        for (int c = b; c < e; ++c)
+        for (int c = b; c < e; ++c) {
+            if (flags[c]) ctx->gated += 1;
             if (flags[c] || (mx > 0.0 && approx[c] >= mx * (1.0 - kWindow))) sub.push_back(c);
+        }
     }
     if (!sub.empty()) {
         const size_t R = sub.size();

@@ -245,16 +245,17 @@ void launch_tgemm(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, cons
     MH_LAUNCH_CHECK();
 }
 
-template <bool RETRY>
+template <int MODE, bool RB, bool TCAND>
 void launch_project(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, const Cands &cs,
-                    const int *list, int ldT)
+                    const int *list, const double *T, int ldT)
 {
     if (nitems == 0) return;
     const dim3 grid(grid_w(nitems)), block(64);
-#define MH_GO(M)                                                                                        \
-    hipLaunchKernelGGL((k_project_w<M, cg_project(M), RETRY>), grid, block, 0, ctx->stream, (const FitDev *)ctx->d_fits.p, \
-                       (const Item *)u.items, nitems, (const int *)cs.d_xc.p, (const int *)cs.d_par.p, list,   \
-                       (const double *)ctx->T.p, ldT, ctx->spart.p)
+#define MH_GO(M)                                                                                          \
+    hipLaunchKernelGGL((k_project_w<M, cg_project(M), MODE, RB, TCAND>), grid, block, 0, ctx->stream,     \
+                       (const FitDev *)ctx->d_fits.p, (const Item *)u.items, nitems, (const int *)cs.d_xc.p, \
+                       (const int *)cs.d_par.p, list, T, ldT, (const double *)ctx->ssum.p,                \
+                       (const int *)ctx->flag.p, ctx->spart.p)
     MH_MT_SWITCH(mt, MH_GO)
 #undef MH_GO
     MH_LAUNCH_CHECK();
@@ -370,7 +371,8 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
 
     Pass pT = make_pass(fb, fe, n, split_rows(n, cnt, tileT, 8192, 256), tileT, kWR);
     Pass pP = make_pass(fb, fe, n, split_rows(n, cnt, tileP, 8192, 256), tileP, kWR);
-    Pass pF = make_pass(fb, fe, n, split_rows(n, cnt, kTileF, 2048, 512), kTileF, 32);
+    Pass pF;  // finalize work list: only the hinge path stores normalized columns
+    if (keep_bxn) pF = make_pass(fb, fe, n, split_rows(n, cnt, kTileF, 2048, 512), kTileF, 32);
     Uploaded uT, uP, uF;
     size_t io = 0, co = 0;
     const size_t total_items = pT.items.size() + pP.items.size() + pF.items.size();
@@ -399,16 +401,20 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                            uT.cmap, ctx->tpart.p, ldT, ctx->T.p, ctx->tnorm.p);
         MH_LAUNCH_CHECK();
     }
-    // Pass 2: r = f32(Bx - Bo T), s, DGKS gate
+    // Pass 2: s = sum f32(Bx - Bo T)^2 and the DGKS gate; r itself is kept only for the
+    // hinge sweep (which needs the normalized columns)
     {
         KTimer kt(ctx, K_PROJECT, flops, bytes + 0.0);
-        launch_project<false>(ctx, mt, uP, (int)pP.items.size(), cs, nullptr, ldT);
+        if (keep_bxn)
+            launch_project<kStoreR, false, false>(ctx, mt, uP, (int)pP.items.size(), cs, nullptr, ctx->T.p, ldT);
+        else
+            launch_project<kSumSq, false, false>(ctx, mt, uP, (int)pP.items.size(), cs, nullptr, ctx->T.p, ldT);
         hipLaunchKernelGGL(k_sreduce, dim3((P + 255) / 256), dim3(256), 0, st, nullptr, P, uP.cmap,
                            ctx->spart.p, kSlots, ctx->tnorm.p, ctx->d_fits.p, cs.d_fit.p,
                            ctx->ssum.p, ctx->flag.p);
         MH_LAUNCH_CHECK();
     }
-    // DGKS retry of the flagged columns, grouped by fit
+    // DGKS retry of the flagged columns, grouped by fit (kernels.cc:387-399)
     std::vector<int> flags(P);
     MH_CHECK(hipMemcpyAsync(flags.data(), ctx->flag.p, P * sizeof(int), hipMemcpyDeviceToHost, st));
     MH_CHECK(hipStreamSynchronize(st));
@@ -436,30 +442,37 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         upload_pass(ctx, rP, ctx->d_ritems, ctx->d_rcmap, rT.items.size(), rT.cmap.size(), vP);
         ctx->tpart.ensure(std::max<size_t>(rT.items.size(), 1) * kSlots * ldT);
         ctx->spart.ensure(rP.items.size() * kSlots + 1);
+        ctx->T2.ensure((size_t)nf * ldT);
+        if (!keep_bxn)  // the first residual of the fired columns, for the retry
+            launch_project<kStoreR, false, true>(ctx, mt, vP, (int)rP.items.size(), cs, ctx->fired_list.p,
+                                                 ctx->T.p, ldT);
         launch_tgemm<true>(ctx, mt, vT, (int)rT.items.size(), cs, ctx->fired_list.p, ldT);
         hipLaunchKernelGGL(k_treduce, dim3(nf), dim3(64), 0, st, ctx->d_fits.p, cs.d_fit.p,
-                           ctx->fired_list.p, nf, vT.cmap, ctx->tpart.p, ldT, ctx->T.p, nullptr);
+                           ctx->fired_list.p, nf, vT.cmap, ctx->tpart.p, ldT, ctx->T2.p, nullptr);
         MH_LAUNCH_CHECK();
-        launch_project<true>(ctx, mt, vP, (int)rP.items.size(), cs, ctx->fired_list.p, ldT);
+        launch_project<kStoreR, true, false>(ctx, mt, vP, (int)rP.items.size(), cs, ctx->fired_list.p, ctx->T2.p,
+                                             ldT);
         hipLaunchKernelGGL(k_sreduce, dim3((nf + 255) / 256), dim3(256), 0, st, ctx->fired_list.p, nf,
                            vP.cmap, ctx->spart.p, kSlots, nullptr, ctx->d_fits.p, cs.d_fit.p,
                            ctx->ssum.p, nullptr);
         MH_LAUNCH_CHECK();
     }
-    // Pass 3: normalize, ybx
+    // Pass 3: Bxn = f32(r * scale), ybx = sum Bxn y
     {
         double fb2 = 0;
         for (size_t f = 0; f < F; ++f) fb2 += (keep_bxn ? 8.0 : 4.0) * n[f] * cnt[f] + 4.0 * n[f];
-        KTimer kt(ctx, K_FINALIZE, 0.0, fb2);
-        if (keep_bxn)
+        KTimer kt(ctx, K_FINALIZE, keep_bxn ? 0.0 : flops, fb2);
+        if (keep_bxn) {
             hipLaunchKernelGGL(k_finalize<true>, dim3(grid_for(pF.items.size())), dim3(256), 0, st,
                                ctx->d_fits.p, uF.items, (int)pF.items.size(), ctx->ssum.p, ctx->spart.p);
-        else
-            hipLaunchKernelGGL(k_finalize<false>, dim3(grid_for(pF.items.size())), dim3(256), 0, st,
-                               ctx->d_fits.p, uF.items, (int)pF.items.size(), ctx->ssum.p, ctx->spart.p);
-        MH_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, P, uF.cmap, ctx->spart.p,
-                           ybx, lin);
+            MH_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, P, uF.cmap, ctx->spart.p,
+                               ybx, lin);
+        } else {
+            launch_project<kFinalY, false, false>(ctx, mt, uP, (int)pP.items.size(), cs, nullptr, ctx->T.p, ldT);
+            hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, P, uP.cmap, ctx->spart.p,
+                               ybx, lin);
+        }
         MH_LAUNCH_CHECK();
     }
 }

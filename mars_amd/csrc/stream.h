@@ -12,7 +12,8 @@
 // sets (the mixed form's rescoring) cost in proportion.
 //
 //   k_tgemm_w:   T = Bo^T Bx over the item's rows                  (kernels.cc:332-345)
-//   k_project_w: r = f32(Bx - Bo T), stored row-major, and sum r^2  (kernels.cc:347-372)
+//   k_project_w: r = f32(Bx - Bo T) and sum r^2, r stored row-major or not,
+//                or the normalised column's projection on y    (kernels.cc:347-416)
 //                (T tile staged in LDS once per item)
 // with Bx = f32(B[:,par] * f32(X[:,xc] * s[xc])) (kernels.cc:300-318), or, in
 // the DGKS retry, the stored residual columns (kernels.cc:387-399).
@@ -44,45 +45,53 @@ struct LaneCand {
     long long col;     // rbuf column
 };
 
-template <bool RETRY>
+/* Lane setup for item slot `slot`: position I.c0 + slot, candidate list[pos] (or pos itself).
+ * RB: the candidate's values come from its stored column (rbuf), not from B and X. */
+template <bool RB>
 __device__ inline LaneCand lane_cand(const FitDev &F, const Item &I, int slot, const int *cxc, const int *cpar,
                                      const int *list)
 {
     LaneCand c{slot < I.nc, 0.0f, F.BC, F.X, 0};
     if (!c.ok) return c;
     const int pos = I.c0 + slot;
-    if (RETRY) {
-        c.col = list[pos] - F.c0;
-    } else {
-        c.col = pos - F.c0;
-        const int xc = cxc[pos];
+    const int cand = list ? list[pos] : pos;
+    c.col = cand - F.c0;
+    if (!RB) {
+        const int xc = cxc[cand];
         c.sx = F.s[xc];
-        c.bc = F.BC + (size_t)cpar[pos] * F.ldx;
+        c.bc = F.BC + (size_t)cpar[cand] * F.ldx;
         c.xp = F.X + (size_t)xc * F.ldx;
     }
     return c;
 }
 
-/* Bx of rows row..row+3 (row a multiple of 4, all < r1 or zero-padded). */
-template <bool RETRY>
+/* Bx of rows row..row+3 (row a multiple of 4).  The loads are unconditional (addresses
+ * clamped to valid memory) and masked afterwards, so a wave's loads stay in flight together;
+ * rows past the fit's end read the zero padding of BC and X. */
+template <bool RB>
 __device__ inline float4 bx4(const FitDev &F, const LaneCand &c, long long row, long long r1)
 {
-    float4 v = {0.f, 0.f, 0.f, 0.f};
-    if (!c.ok) return v;
-    if (RETRY) {
+    float4 v;
+    if (RB) {
         const float *p = F.rbuf + c.col;
-        if (row < r1) v.x = p[row * F.ldr];
-        if (row + 1 < r1) v.y = p[(row + 1) * F.ldr];
-        if (row + 2 < r1) v.z = p[(row + 2) * F.ldr];
-        if (row + 3 < r1) v.w = p[(row + 3) * F.ldr];
+        const long long last = r1 - 1;
+        const float v0 = gld(p + (row < r1 ? row : last) * F.ldr);
+        const float v1 = gld(p + (row + 1 < r1 ? row + 1 : last) * F.ldr);
+        const float v2 = gld(p + (row + 2 < r1 ? row + 2 : last) * F.ldr);
+        const float v3 = gld(p + (row + 3 < r1 ? row + 3 : last) * F.ldr);
+        v.x = row < r1 ? v0 : 0.f;
+        v.y = row + 1 < r1 ? v1 : 0.f;
+        v.z = row + 2 < r1 ? v2 : 0.f;
+        v.w = row + 3 < r1 ? v3 : 0.f;
     } else {
-        const float4 b = *(const float4 *)(c.bc + row);
-        const float4 x = *(const float4 *)(c.xp + row);
+        const float4 b = gld4(c.bc + row);
+        const float4 x = gld4(c.xp + row);
         v.x = __fmul_rn(b.x, __fmul_rn(x.x, c.sx));
         v.y = __fmul_rn(b.y, __fmul_rn(x.y, c.sx));
         v.z = __fmul_rn(b.z, __fmul_rn(x.z, c.sx));
         v.w = __fmul_rn(b.w, __fmul_rn(x.w, c.sx));
     }
+    if (!c.ok) v = float4{0.f, 0.f, 0.f, 0.f};
     return v;
 }
 
@@ -123,18 +132,26 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_tgemm_w(const FitDev *__r
 #pragma unroll 1
             for (int hf = 0; hf < 2; ++hf) {
                 const long long row = r + 8 * kr + 4 * hf;
+                // all loads of the half-step first, unconditionally (see bx4)
+                float4 av4[MT], bv4[CG];
+#pragma unroll
+                for (int u = 0; u < MT; ++u) av4[u] = gld4(bo[u] + row);
+#pragma unroll
+                for (int g = 0; g < CG; ++g) bv4[g] = bx4<RETRY>(F, c[g], row, I.r1);
                 double a[MT][4], b[CG][4];
 #pragma unroll
                 for (int u = 0; u < MT; ++u) {
-                    float4 v = {0.f, 0.f, 0.f, 0.f};
-                    if (kok[u]) v = *(const float4 *)(bo[u] + row);
-                    a[u][0] = v.x; a[u][1] = v.y; a[u][2] = v.z; a[u][3] = v.w;
+                    a[u][0] = kok[u] ? av4[u].x : 0.f;
+                    a[u][1] = kok[u] ? av4[u].y : 0.f;
+                    a[u][2] = kok[u] ? av4[u].z : 0.f;
+                    a[u][3] = kok[u] ? av4[u].w : 0.f;
                 }
 #pragma unroll
                 for (int g = 0; g < CG; ++g) {
-                    float4 v = {0.f, 0.f, 0.f, 0.f};
-                    if (16 * g < I.nc) v = bx4<RETRY>(F, c[g], row, I.r1);
-                    b[g][0] = v.x; b[g][1] = v.y; b[g][2] = v.z; b[g][3] = v.w;
+                    b[g][0] = bv4[g].x;
+                    b[g][1] = bv4[g].y;
+                    b[g][2] = bv4[g].z;
+                    b[g][3] = bv4[g].w;
                 }
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
@@ -158,21 +175,32 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_tgemm_w(const FitDev *__r
     }
 }
 
+/* Projection modes. */
+enum { kStoreR = 0, kSumSq = 1, kFinalY = 2 };
+
 /*
- * Projection: r = f32(Bx - Bo T) for the item's rows, stored at
- * rbuf[row * ldr + column], and spart[item * 64 + slot] = sum r^2.
+ * Projection r = f32(Bx - Bo T) over the item's rows, then per MODE:
+ *   kStoreR  r -> rbuf[row * ldr + column], spart[item * 64 + slot] = sum r^2;
+ *   kSumSq   spart = sum r^2 only;
+ *   kFinalY  Bxn = f32(r * scale), scale = f32(1 / sqrt(s + tol)) (0 when s <= tol,
+ *            kernels.cc:401-416), spart = sum Bxn y (marsalgo.cc:500-505); candidates
+ *            whose DGKS gate fired take r from their stored retry residual instead.
+ * RB: Bx is the candidate's stored column (the DGKS retry).  TCAND: the item's
+ * T rows are indexed by candidate instead of by position.
  * The item's T tile is staged once in LDS (MFMA B operand).  The A operand's
  * M index M stands for row 4 (M % 4) + M / 4 of the 16-row block, so the D
  * elements a lane holds (M = l/16 + 4v) are the 4 consecutive rows
  * 4 (l/16) + v, whose Bx it loads as one float4 each of B and X.
  */
-template <int MT, int CG, bool RETRY>
+template <int MT, int CG, int MODE, bool RB, bool TCAND>
 __global__ __launch_bounds__(64, waves_for(MT)) void k_project_w(const FitDev *__restrict__ fits,
                                                                  const Item *__restrict__ items, int nitems,
                                                                  const int *__restrict__ cxc,
                                                                  const int *__restrict__ cpar,
                                                                  const int *__restrict__ list,
                                                                  const double *__restrict__ T, int ldT,
+                                                                 const double *__restrict__ ssum,
+                                                                 const int *__restrict__ fired,
                                                                  double *__restrict__ spart)
 {
     constexpr int KQ = 4 * MT;           // k steps of 4
@@ -184,46 +212,81 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_project_w(const FitDev *_
         const Item I = items[it];
         const FitDev F = fits[I.fit];
         __syncthreads();
-        for (int e = lane; e < NC * 16 * MT; e += 64) {
-            const int sl = e / (16 * MT), k = e - sl * (16 * MT);
-            sT[sl][k] = (sl < I.nc && k < F.m) ? T[(size_t)(I.c0 + sl) * ldT + k] : 0.0;
+        {
+            // only the live slots are loaded; eight loads in flight per lane
+            const int live = ((I.nc + 15) & ~15) * 16 * MT;
+#pragma unroll 8
+            for (int e = lane; e < NC * 16 * MT; e += 64) {
+                const int sl = e / (16 * MT), k = e - sl * (16 * MT);
+                const bool ok = e < live && sl < I.nc && k < F.m;
+                const int pos = I.c0 + (ok ? sl : 0);
+                const int row = TCAND ? (list ? gld(list + pos) : pos) : pos;
+                const double v = gld(T + (size_t)row * ldT + (ok ? k : 0));
+                sT[sl][k] = ok ? v : 0.0;
+            }
         }
         __syncthreads();
         LaneCand c[CG];
-        double sacc[CG];
+        double acc[CG];
+        float scale[CG];
+        bool fromr[CG];
+        const float *bo[KQ];
+#pragma unroll
+        for (int q = 0; q < KQ; ++q) {
+            const int k = 4 * q + kr;
+            bo[q] = F.BoC + (size_t)(k < F.m ? k : 0) * F.ldx;
+        }
 #pragma unroll
         for (int g = 0; g < CG; ++g) {
-            c[g] = lane_cand<RETRY>(F, I, 16 * g + kc, cxc, cpar, list);
-            sacc[g] = 0.0;
+            c[g] = lane_cand<RB>(F, I, 16 * g + kc, cxc, cpar, list);
+            acc[g] = 0.0;
+            scale[g] = 0.0f;
+            fromr[g] = false;
+            if (MODE == kFinalY && c[g].ok) {
+                const int cand = (int)(c[g].col + F.c0);
+                const double sv = ssum[cand];
+                scale[g] = sv > F.tol ? (float)(1.0 / sqrt(sv + F.tol)) : 0.0f;
+                fromr[g] = fired[cand] != 0;
+            }
         }
 #pragma unroll 1
         for (long long r = I.r0; r < I.r1; r += kWR) {
 #pragma unroll 1
             for (int h = 0; h < kWR / 16; ++h) {
                 const long long r16 = r + 16 * h;
+                const long long row = r16 + 4 * kr;
+                // all loads of the 16-row block first, unconditionally (see bx4)
                 float av[KQ];
 #pragma unroll
-                for (int q = 0; q < KQ; ++q) {
-                    const int k = 4 * q + kr;
-                    av[q] = k < F.m ? F.BoC[(size_t)k * F.ldx + r16 + prow] : 0.0f;
-                }
+                for (int q = 0; q < KQ; ++q) av[q] = gld(bo[q] + r16 + prow);
+                float4 bxv[CG];
+#pragma unroll
+                for (int g = 0; g < CG; ++g) bxv[g] = bx4<RB>(F, c[g], row, I.r1);
+                float4 yv = {0.f, 0.f, 0.f, 0.f};
+                if (MODE == kFinalY) yv = gld4(F.y + row);
+                const float yy[4] = {yv.x, yv.y, yv.z, yv.w};
+#pragma unroll
+                for (int q = 0; q < KQ; ++q)
+                    if (4 * q + kr >= F.m) av[q] = 0.0f;
 #pragma unroll
                 for (int g = 0; g < CG; ++g) {
                     if (16 * g >= I.nc) continue;
                     v4d d = v4d{0, 0, 0, 0};
 #pragma unroll
                     for (int q = 0; q < KQ; ++q) d = mfma_f64((double)av[q], sT[16 * g + kc][4 * q + kr], d);
-                    const long long row = r16 + 4 * kr;
-                    const float4 bx = bx4<RETRY>(F, c[g], row, I.r1);
-                    const float bv[4] = {bx.x, bx.y, bx.z, bx.w};
-                    if (c[g].ok) {
+                    if (!c[g].ok) continue;
+                    const float bv[4] = {bxv[g].x, bxv[g].y, bxv[g].z, bxv[g].w};
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) {
-                            if (row + v < I.r1) {
-                                const float rf = (float)((double)bv[v] - d[v]);
-                                F.rbuf[(row + v) * F.ldr + c[g].col] = rf;
-                                sacc[g] = fma((double)rf, (double)rf, sacc[g]);
-                            }
+                    for (int v = 0; v < 4; ++v) {
+                        if (row + v >= I.r1) continue;
+                        float rf = (float)((double)bv[v] - d[v]);
+                        if (MODE == kFinalY) {
+                            if (fromr[g]) rf = F.rbuf[(row + v) * F.ldr + c[g].col];
+                            const float bxn = __fmul_rn(rf, scale[g]);
+                            acc[g] = fma((double)bxn, (double)yy[v], acc[g]);
+                        } else {
+                            if (MODE == kStoreR) F.rbuf[(row + v) * F.ldr + c[g].col] = rf;
+                            acc[g] = fma((double)rf, (double)rf, acc[g]);
                         }
                     }
                 }
@@ -231,7 +294,7 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_project_w(const FitDev *_
         }
 #pragma unroll
         for (int g = 0; g < CG; ++g) {
-            double s = sacc[g];
+            double s = acc[g];
             s = s + __shfl_xor(s, 16, 64);
             s = s + __shfl_xor(s, 32, 64);
             const int slot = 16 * g + kc;
