@@ -42,7 +42,8 @@ extern "C" {
 
 /* Precision forms of the candidate scoring. */
 #define MARS_HIP_FORM_PLAIN 0  /* all FP64 arithmetic on f32 storage, CPU rounding points */
-#define MARS_HIP_FORM_MIXED 1  /* BF16-split / FP32 MFMA bulk + FP64 refinement + FP64 near-tie recheck */
+#define MARS_HIP_FORM_MIXED 1  /* FP32-MFMA incremental projection cache + exact FP64 rescoring of DGKS-gated
+                                  and near-tie candidates (linear delta-SSE only) */
 
 typedef struct mars_hip_ctx mars_hip_ctx;
 
