@@ -64,6 +64,8 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
     std::vector<AppendArg> args(F);
     std::vector<BestOut> best;
     while (true) {
+        HostTimer step_timer(ctx, K_H_STEP);
+        HostTimer cands_timer(ctx, K_H_CANDS);
         Cands &cs = ctx->cand;
         cs.fit.clear();
         cs.xc.clear();
@@ -86,20 +88,30 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
                     }
                 }
         }
-        if (!any) break;
+        if (!any) {
+            step_timer.cancel();
+            cands_timer.cancel();
+            break;
+        }
         ++step;
         index_cands(ctx, cs);
+        cands_timer.stop();
         const size_t P = cs.fit.size();
         ctx->scored += (long)P;
         ctx->lin.ensure(std::max<size_t>(P, 1));
         ctx->ybx.ensure(std::max<size_t>(P, 1));
         if (P) {
+            HostTimer t(ctx, K_H_SCORE);
             if (ctx->form == MARS_HIP_FORM_MIXED)
                 mixed_score(ctx);
             else
                 plain_score(ctx, cs, false, ctx->lin.p, ctx->ybx.p);
+            MH_CHECK(hipStreamSynchronize(ctx->stream));
         }
-        argmax_fits(ctx, cs, ctx->lin.p, best);
+        {
+            HostTimer t(ctx, K_H_ARGMAX);
+            argmax_fits(ctx, cs, ctx->lin.p, best);
+        }
         for (size_t f = 0; f < F; ++f) {
             args[f] = AppendArg{0, 0, 0, 0, 0.0f, 0};
             FitDriver &d = D[f];
@@ -121,7 +133,10 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
             args[f] = AppendArg{0, cs.xc[g], cs.par[g], 1, 0.0f, 0};
         }
         std::vector<AppendOut> out;
-        fit_append_batch(ctx, args, out);
+        {
+            HostTimer t(ctx, K_H_APPEND);
+            fit_append_batch(ctx, args, out);
+        }
         for (size_t f = 0; f < F; ++f) {
             if (!args[f].active) continue;
             FitDriver &d = D[f];

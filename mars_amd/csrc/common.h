@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstddef>
 #include <cstdint>
 #include <stdexcept>
@@ -144,10 +145,14 @@ struct BestOut {
 
 /* Kernel classes timed by mars_hip_timing. */
 enum KClass { K_TGEMM = 0, K_PROJECT, K_FINALIZE, K_DGKS, K_SWEEP, K_SORT, K_APPEND, K_MIXED_A,
-              K_MIXED_B, K_RECHECK, K_ARGMAX, K_NCLASS };
+              K_MIXED_B, K_RECHECK, K_ARGMAX,
+              // host wall-clock sections of a batch step (include the waits they contain)
+              K_H_CANDS, K_H_SCORE, K_H_ARGMAX, K_H_APPEND, K_H_STEP, K_NCLASS };
 constexpr const char *kClassName[K_NCLASS] = {"tgemm_f64", "project_f64", "finalize", "dgks_retry",
                                              "hinge_sweep", "sort", "append", "mixed_update",
-                                             "mixed_score", "mixed_recheck", "argmax"};
+                                             "mixed_score", "mixed_recheck", "argmax",
+                                             "host_cands", "host_score", "host_argmax", "host_append",
+                                             "host_step"};
 
 struct KAcc {
     long launches = 0;
@@ -186,11 +191,14 @@ struct mars_hip_ctx {
     mh::Cands cand;
     mh::Cands rcand;                 // mixed form: candidates rescored exactly
     mh::DBuf<double> rlin, rybx;
+    mh::DBuf<int> rsel;              // their positions in the scored list
     mh::DBuf<mh::FitDev> d_fits;
     mh::DBuf<mh::Item> d_items;
     mh::DBuf<int> d_cmap;            // per-candidate first item / item count per pass
     mh::DBuf<mh::Item> d_ritems;     // DGKS retry passes (separate: the first passes' lists stay live)
     mh::DBuf<int> d_rcmap;
+    mh::DBuf<mh::Item> d_litems;     // final pass over the candidates with s > tol
+    mh::DBuf<int> d_lcmap, live_list;
     mh::DBuf<double> tpart, T, T2, tnorm, spart, ssum, ybx, lin, hin, aux;
     mh::DBuf<int> flag, fired_list, fired_count, knot, krow, ival;
     mh::DBuf<long long> dgks;        // device DGKS counter
@@ -253,6 +261,28 @@ struct KTimer {
         MH_CHECK(hipEventCreate(&e));
         return e;
     }
+};
+
+/* Host wall-clock time of a section, when timing is on. */
+struct HostTimer {
+    mars_hip_ctx *ctx;
+    int cls;
+    std::chrono::steady_clock::time_point t0;
+    HostTimer(mars_hip_ctx *c, int k) : ctx(c), cls(k), t0(std::chrono::steady_clock::now()) {}
+    ~HostTimer() { stop(); }
+    /* Record now (once); cancel() drops the section instead. */
+    void stop()
+    {
+        if (!ctx) return;
+        if (ctx->timing) {
+            const double ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            ctx->kacc[cls].launches += 1;
+            ctx->kacc[cls].ms += ms;
+        }
+        ctx = nullptr;
+    }
+    void cancel() { ctx = nullptr; }
 };
 
 /* Fold completed timing records into the per-class sums (stream must be idle). */

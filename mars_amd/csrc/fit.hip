@@ -475,6 +475,10 @@ void ctx_free(mars_hip_ctx *ctx)
     ctx->d_items.release();
     ctx->d_ritems.release();
     ctx->d_rcmap.release();
+    ctx->d_litems.release();
+    ctx->d_lcmap.release();
+    ctx->live_list.release();
+    ctx->rsel.release();
     ctx->dgks.release();
     ctx->aout.release();
     ctx->aarg.release();

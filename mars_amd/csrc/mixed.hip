@@ -227,6 +227,73 @@ __global__ __launch_bounds__(256) void k_tc_score(const FitDev *__restrict__ fit
     score[c] = gate ? 0.0 : (ry * ry) / (s + F->tol);
 }
 
+/* mx[f] = largest ungated approximate score of fit f (0 if none); one block per fit. */
+__global__ __launch_bounds__(256) void k_fit_max(const int *__restrict__ c0, const int *__restrict__ cnt,
+                                                 const double *__restrict__ score, const int *__restrict__ flag,
+                                                 double *__restrict__ mx)
+{
+    __shared__ double sh[256];
+    const int f = blockIdx.x, t = threadIdx.x;
+    double v = 0.0;
+    for (int c = c0[f] + t; c < c0[f] + cnt[f]; c += 256)
+        if (!flag[c]) v = fmax(v, score[c]);
+    sh[t] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) sh[t] = fmax(sh[t], sh[t + s]);
+        __syncthreads();
+    }
+    if (t == 0) mx[f] = sh[0];
+}
+
+/*
+ * Rescoring list, in candidate order: every gated candidate and every ungated one whose
+ * approximate score is within the window of its fit's best ungated one.  One block of 1024
+ * threads, each owning a contiguous range; out_n[0] = list length, out_n[1] = gated count.
+ */
+__global__ __launch_bounds__(1024) void k_select(int P, const int *__restrict__ cfit,
+                                                 const double *__restrict__ score, const int *__restrict__ flag,
+                                                 const double *__restrict__ mx, double window,
+                                                 int *__restrict__ list, int *__restrict__ out_n)
+{
+    __shared__ int sc[1024], sg[1024];
+    const int t = threadIdx.x;
+    const int per = (P + 1023) / 1024, b = t * per, e = min(P, b + per);
+    auto take = [&](int c) {
+        const double m = mx[cfit[c]];
+        return flag[c] != 0 || (m > 0.0 && score[c] >= m * (1.0 - window));
+    };
+    int n = 0, g = 0;
+    for (int c = b; c < e; ++c) {
+        n += take(c) ? 1 : 0;
+        g += flag[c] != 0 ? 1 : 0;
+    }
+    sc[t] = n;
+    sg[t] = g;
+    __syncthreads();
+    for (int s = 1; s < 1024; s <<= 1) {  // inclusive scan (Hillis-Steele)
+        const int a = t >= s ? sc[t - s] : 0, ag = t >= s ? sg[t - s] : 0;
+        __syncthreads();
+        sc[t] += a;
+        sg[t] += ag;
+        __syncthreads();
+    }
+    int o = sc[t] - n;
+    for (int c = b; c < e; ++c)
+        if (take(c)) list[o++] = c;
+    if (t == 1023) {
+        out_n[0] = sc[1023];
+        out_n[1] = sg[1023];
+    }
+}
+
+/* lin[idx[i]] = val[i] */
+__global__ void k_put(const int *__restrict__ idx, const double *__restrict__ val, int n, double *__restrict__ lin)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) lin[idx[i]] = val[i];
+}
+
 constexpr size_t kMaxMixedTerms = 256;  // cache: p_x * ldb^2 doubles per fit
 
 /* Allocate the caches of fits that have none yet. */
@@ -327,48 +394,51 @@ void mixed_score(mars_hip_ctx *ctx)
                            (const int *)cs.d_par.p, P, ctx->lin.p, ctx->flag.p);
         MH_LAUNCH_CHECK();
     }
-    std::vector<double> approx(P);
-    std::vector<int> flags(P);
-    MH_CHECK(hipMemcpyAsync(approx.data(), ctx->lin.p, P * sizeof(double), hipMemcpyDeviceToHost, st));
-    MH_CHECK(hipMemcpyAsync(flags.data(), ctx->flag.p, P * sizeof(int), hipMemcpyDeviceToHost, st));
-    MH_CHECK(hipStreamSynchronize(st));
-
+    // rescoring list on the device (candidate order), then its exact FP64 scores
     KTimer kt(ctx, K_RECHECK, 0.0, 0.0);
-    // Rescore exactly: the gated candidates, and every ungated one within the window of
-    // its fit's best ungated approximation.  An exact gated score above that best only
-    // raises the maximum, so the window taken here contains the final one.
-    std::vector<int> sub;
+    std::vector<int> ranges(2 * F);
     for (size_t f = 0; f < F; ++f) {
-        const int b = cs.fit_c0[f], e = b + cs.fit_P[f];
-        double mx = 0.0;
-        for (int c = b; c < e; ++c)
-            if (!flags[c]) mx = std::max(mx, approx[c]);
-        for (int c = b; c < e; ++c) {
-            if (flags[c]) ctx->gated += 1;
-            if (flags[c] || (mx > 0.0 && approx[c] >= mx * (1.0 - kWindow))) sub.push_back(c);
-        }
+        ranges[f] = cs.fit_c0[f];
+        ranges[F + f] = cs.fit_P[f];
     }
-    if (!sub.empty()) {
-        const size_t R = sub.size();
-        Cands &rc = ctx->rcand;
-        rc.fit.resize(R);
-        rc.xc.resize(R);
-        rc.par.resize(R);
-        for (size_t i = 0; i < R; ++i) {
-            rc.fit[i] = cs.fit[sub[i]];
-            rc.xc[i] = cs.xc[sub[i]];
-            rc.par[i] = cs.par[sub[i]];
-        }
-        ctx->rlin.ensure(R);
-        ctx->rybx.ensure(R);
-        plain_score(ctx, rc, false, ctx->rlin.p, ctx->rybx.p);
-        std::vector<double> exact(R);
-        MH_CHECK(hipMemcpyAsync(exact.data(), ctx->rlin.p, R * sizeof(double), hipMemcpyDeviceToHost, st));
-        MH_CHECK(hipStreamSynchronize(st));
-        for (size_t i = 0; i < R; ++i) approx[sub[i]] = exact[i];
+    ctx->ival.ensure(2 * F + 2);
+    ctx->red.ensure(F);
+    ctx->rsel.ensure(P);
+    MH_CHECK(hipMemcpyAsync(ctx->ival.p, ranges.data(), 2 * F * sizeof(int), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_fit_max, dim3((unsigned)F), dim3(256), 0, st, (const int *)ctx->ival.p,
+                       (const int *)ctx->ival.p + F, (const double *)ctx->lin.p, (const int *)ctx->flag.p, ctx->red.p);
+    MH_LAUNCH_CHECK();
+    // An exact gated score above a fit's best ungated approximation only raises the maximum,
+    // so the window taken here contains the final one.
+    hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, st, P, (const int *)cs.d_fit.p, (const double *)ctx->lin.p,
+                       (const int *)ctx->flag.p, (const double *)ctx->red.p, kWindow, ctx->rsel.p,
+                       ctx->ival.p + 2 * F);
+    MH_LAUNCH_CHECK();
+    int nsel[2] = {0, 0};
+    MH_CHECK(hipMemcpyAsync(nsel, ctx->ival.p + 2 * F, sizeof(nsel), hipMemcpyDeviceToHost, st));
+    MH_CHECK(hipStreamSynchronize(st));
+    const int R = nsel[0];
+    ctx->gated += nsel[1];
+    ctx->rescored += R;
+    if (R == 0) return;
+    std::vector<int> sub(R);
+    MH_CHECK(hipMemcpyAsync(sub.data(), ctx->rsel.p, R * sizeof(int), hipMemcpyDeviceToHost, st));
+    MH_CHECK(hipStreamSynchronize(st));
+    Cands &rc = ctx->rcand;
+    rc.fit.resize(R);
+    rc.xc.resize(R);
+    rc.par.resize(R);
+    for (int i = 0; i < R; ++i) {
+        rc.fit[i] = cs.fit[sub[i]];
+        rc.xc[i] = cs.xc[sub[i]];
+        rc.par[i] = cs.par[sub[i]];
     }
-    MH_CHECK(hipMemcpyAsync(ctx->lin.p, approx.data(), P * sizeof(double), hipMemcpyHostToDevice, st));
-    ctx->rescored += (long)sub.size();
+    ctx->rlin.ensure(R);
+    ctx->rybx.ensure(R);
+    plain_score(ctx, rc, false, ctx->rlin.p, ctx->rybx.p);
+    hipLaunchKernelGGL(k_put, dim3((R + 255) / 256), dim3(256), 0, st, (const int *)ctx->rsel.p,
+                       (const double *)ctx->rlin.p, R, ctx->lin.p);
+    MH_LAUNCH_CHECK();
 }
 
 } // namespace mh

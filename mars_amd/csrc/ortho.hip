@@ -49,7 +49,8 @@ __global__ __launch_bounds__(64) void k_treduce(const FitDev *__restrict__ fits,
     if (lane == 0 && tnorm) tnorm[c] = tt;
 }
 
-/* s[c] = sum of item partials; the first pass also evaluates the DGKS gate. */
+/* s[c] = sum of item partials; the first pass also sets flag[c]: bit 0 the DGKS gate
+ * fired (kernels.cc:387-388), bit 1 s > tol (the candidate can score above 0). */
 __global__ void k_sreduce(const int *__restrict__ list, int count, const int *__restrict__ cmap,
                           const double *__restrict__ spart, int tile,
                           const double *__restrict__ tnorm, const FitDev *__restrict__ fits,
@@ -66,7 +67,7 @@ __global__ void k_sreduce(const int *__restrict__ list, int count, const int *__
     } else {
         const double tol = fits[cfit[c]].tol;
         ssum[c] = s;
-        flag[c] = (s > tol && s * kGate < tnorm[c]) ? 1 : 0;
+        flag[c] = ((s > tol && s * kGate < tnorm[c]) ? 1 : 0) | (s > tol ? 2 : 0);
     }
 }
 
@@ -123,16 +124,27 @@ __global__ __launch_bounds__(256) void k_finalize(const FitDev *__restrict__ fit
     }
 }
 
+/* ybx = sum of item partials, lin = ybx^2, for candidate c (or list[c]). */
 __global__ void k_yreduce(int count, const int *__restrict__ cmap, const double *__restrict__ ypart,
-                          double *__restrict__ ybx, double *__restrict__ lin)
+                          const int *__restrict__ list, double *__restrict__ ybx, double *__restrict__ lin)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= count) return;
     const int first = cmap[3 * c], cnt = cmap[3 * c + 1], slot = cmap[3 * c + 2];
     double s = 0.0;
     for (int i = 0; i < cnt; ++i) s += ypart[(size_t)(first + i) * kTileF + slot];
-    if (ybx) ybx[c] = s;
-    lin[c] = s * s;
+    const int cand = list ? list[c] : c;
+    if (ybx) ybx[cand] = s;
+    lin[cand] = s * s;
+}
+
+/* lin = ybx = 0 (candidates whose s <= tol: scale 0, kernels.cc:401-416). */
+__global__ void k_zero2(int count, double *__restrict__ a, double *__restrict__ b)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= count) return;
+    if (a) a[c] = 0.0;
+    b[c] = 0.0;
 }
 
 /*
@@ -418,13 +430,17 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     std::vector<int> flags(P);
     MH_CHECK(hipMemcpyAsync(flags.data(), ctx->flag.p, P * sizeof(int), hipMemcpyDeviceToHost, st));
     MH_CHECK(hipStreamSynchronize(st));
-    std::vector<int> fired;
-    std::vector<int> rb(F, 0), re(F, 0);
+    std::vector<int> fired, live;
+    std::vector<int> rb(F, 0), re(F, 0), lb(F, 0), le(F, 0);
     for (size_t f = 0; f < F; ++f) {
         rb[f] = (int)fired.size();
-        for (int c = fb[f]; c < fe[f]; ++c)
-            if (flags[c]) fired.push_back(c);
+        lb[f] = (int)live.size();
+        for (int c = fb[f]; c < fe[f]; ++c) {
+            if (flags[c] & 1) fired.push_back(c);
+            if (flags[c] & 2) live.push_back(c);
+        }
         re[f] = (int)fired.size();
+        le[f] = (int)live.size();
     }
     if (!fired.empty()) {
         const int nf = (int)fired.size();
@@ -467,11 +483,28 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                                ctx->d_fits.p, uF.items, (int)pF.items.size(), ctx->ssum.p, ctx->spart.p);
             MH_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, P, uF.cmap, ctx->spart.p,
-                               ybx, lin);
+                               nullptr, ybx, lin);
         } else {
-            launch_project<kFinalY, false, false>(ctx, mt, uP, (int)pP.items.size(), cs, nullptr, ctx->T.p, ldT);
-            hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, P, uP.cmap, ctx->spart.p,
-                               ybx, lin);
+            // only candidates with s > tol can score above 0; the others are 0 exactly
+            hipLaunchKernelGGL(k_zero2, dim3((P + 255) / 256), dim3(256), 0, st, P, ybx, lin);
+            MH_LAUNCH_CHECK();
+            const int nl = (int)live.size();
+            if (nl > 0) {
+                std::vector<int> lcnt(F);
+                for (size_t f = 0; f < F; ++f) lcnt[f] = le[f] - lb[f];
+                Pass pL = make_pass(lb, le, n, split_rows(n, lcnt, tileP, 8192, 256), tileP, kWR);
+                Uploaded uL;
+                ctx->live_list.ensure(nl);
+                ctx->d_litems.ensure(pL.items.size() + 1);
+                ctx->d_lcmap.ensure(pL.cmap.size() + 1);
+                MH_CHECK(hipMemcpyAsync(ctx->live_list.p, live.data(), nl * sizeof(int), hipMemcpyHostToDevice, st));
+                upload_pass(ctx, pL, ctx->d_litems, ctx->d_lcmap, 0, 0, uL);
+                ctx->spart.ensure(pL.items.size() * kSlots + 1);
+                launch_project<kFinalY, false, true>(ctx, mt, uL, (int)pL.items.size(), cs, ctx->live_list.p,
+                                                     ctx->T.p, ldT);
+                hipLaunchKernelGGL(k_yreduce, dim3((nl + 255) / 256), dim3(256), 0, st, nl, uL.cmap, ctx->spart.p,
+                                   ctx->live_list.p, ybx, lin);
+            }
         }
         MH_LAUNCH_CHECK();
     }

@@ -246,7 +246,7 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_project_w(const FitDev *_
                 const int cand = (int)(c[g].col + F.c0);
                 const double sv = ssum[cand];
                 scale[g] = sv > F.tol ? (float)(1.0 / sqrt(sv + F.tol)) : 0.0f;
-                fromr[g] = fired[cand] != 0;
+                fromr[g] = (fired[cand] & 1) != 0;
             }
         }
 #pragma unroll 1
