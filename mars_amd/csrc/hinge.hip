@@ -2,13 +2,13 @@
 // order it walks (marsalgo.cc:33-39, 521-529).
 //
 // Sweep state per candidate (X column xc, parent b, normalized column bx) after
-// sorted row i, over lanes q = [Bo row (m) ; bx ; y ; b]:
+// sorted row i, over lanes q = [b ; y ; bx ; Bo row (m)]:
 //     g_i = sum_{l<=i} b_l q_l                   f_i = sum_{l<i} b_l (x_l - x_i) q_l
 //     k_i = sum_{l<i} b_l^2 (x_l - x_i)^2
 // advanced row by row as f += d g, g += b q, k += d^2 g_b + 2 d f_b with
-// d = x_{i-1} - x_i.  The lane m+1 pair (g, f) is the reference's (vb, w) and
-// the lane m+2 pair is (b2, bd).  At grid rows i = head+1 + t*min_span < tail:
-//     ff = |f[0..m]|^2,  fy = f[0..m-1].ybo + f[m] ybx,
+// d = x_{i-1} - x_i.  The y lane's pair (g, f) is the reference's (vb, w) and
+// the b lane's pair is (b2, bd).  At grid rows i = head+1 + t*min_span < tail:
+//     ff = |f[bx, Bo]|^2,  fy = f[Bo].ybo + f[bx] ybx,
 //     sse = (fy - w)^2 / (k - ff + tol)  if k - ff > tol.
 // Rows are split into chunks scanned in parallel: pass 1 runs each chunk from
 // the zero state, a carry scan combines the chunk summaries
@@ -166,166 +166,292 @@ struct SweepArgs {
     int *besti;          // [P][C]
 };
 
-constexpr int kTileRows = 64;
+constexpr int kTile = 32;   // sorted rows per tile
+constexpr int kPitch = 33;  // LDS floats per tile column (odd: column-major tile without bank conflicts)
+constexpr int kSlots = 8;   // candidates per block
 
-template <int KPL, bool EVAL>
-__global__ __launch_bounds__(1024) void k_sweep(SweepArgs a)
+/* LDS bytes of one k_sweep block for `nc` tile columns (two buffers of everything). */
+__host__ __device__ constexpr size_t sweep_lds(int nc)
 {
-    extern __shared__ float lds[];
-    float *sbo = lds;                                   // [64][m]
-    int *srow = (int *)(sbo + kTileRows * a.m);         // [64]
-    float *sd = (float *)(srow + kTileRows);            // [64]
-    float *sy = sd + kTileRows;                         // [64]
+    return 2 * (size_t)kSlots * kTile * sizeof(double)   // b per slot and row (f64)
+         + 2 * (size_t)kTile * 2 * sizeof(double)        // (d, d*d) per row
+         + 2 * (size_t)kTile * sizeof(int)               // sorted-order row indices
+         + 2 * (size_t)nc * kPitch * sizeof(float);      // the tile, column-major
+}
 
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+/*
+ * One block = W = 8/PK waves sweeping kSlots candidates (the parents c0.. of one
+ * X column) over the sorted rows [r0, r1) of its chunk.  PK = 2 packs two
+ * candidates of at most 29 basis functions into the halves of a wave (L = 32
+ * lanes each).  Lane slot u = lane % L + L s (s < KPL) of a candidate holds
+ * the running (f, g) of one covariate column: u = 0 the parent b, u = 1 y,
+ * u = 2 the candidate's Bx column, u = 3 .. m+2 the orthonormal basis; the
+ * b lane also carries k = |h|^2 (marsalgo.cc:576-649).
+ *
+ * Tiles of 32 sorted rows are gathered into LDS column-major, columns
+ *   [0, m) Bo row | [m, m+8) Bx per slot | m+8 y | m+9 zero | [m+10, m+18) b per slot,
+ * double-buffered: the global gathers of tile t+1 are in flight while the
+ * unrolled recurrence runs over tile t.  Rows past the chunk end are zero
+ * (d = b = 0 leaves every accumulator unchanged), so every tile runs 32 rows.
+ * Knot rows come as a per-tile bit mask (grid position) plus a mask of nonzero
+ * deltas for the tie-run test, both in scalar registers.
+ */
+template <int KPL, int PK, bool EVAL>
+__global__ __launch_bounds__(512) void k_sweep(SweepArgs a)
+{
+    constexpr int L = 64 / PK;
+    constexpr int W = kSlots / PK;                  // waves per block
+    constexpr int NT = 64 * W;
+    constexpr int RPW = kTile / W;                  // gathered rows per wave and tile
+    constexpr int UM = PK == 2 ? 1 : KPL + 1;       // 64-column groups of the tile
+    extern __shared__ double lds_d[];
+    const int m = a.m;
+    const int nc = m + 2 * kSlots + 2;
+    double *sb = lds_d;                                       // [2][kSlots][32]
+    double2 *sdd = (double2 *)(sb + 2 * kSlots * kTile);      // [2][32]
+    int *srow = (int *)(sdd + 2 * kTile);                     // [2][32]
+    float *sq = (float *)(srow + 2 * kTile);                  // [2][nc][33]
+
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int half = PK == 2 ? lane >> 5 : 0, kl = lane & (L - 1);
     const int chunk = blockIdx.x, pg = blockIdx.y, gi = blockIdx.z;
     const int xc = a.grp[3 * gi], c0 = a.grp[3 * gi + 1], np = a.grp[3 * gi + 2];
-    const int j = pg * 16 + wv;             // parent slot within the group
-    if (pg * 16 >= np) return;              // whole block idle (uniform)
-    const bool active = j < np;
-    const int c = c0 + (active ? j : 0);
-    const int par = a.cpar[c];
-    const int m = a.m;
+    const int sbase = pg * kSlots;                  // first parent slot of this block
+    if (sbase >= np) return;                        // whole block idle (uniform)
+    const int nact = min(kSlots, np - sbase);
+    const int slot = wv * PK + half;                // this lane's candidate slot
+    const bool active = slot < nact;
+    const bool wave_active = wv * PK < nact;
+    const int c = c0 + sbase + (active ? slot : 0);
     const size_t r0 = (size_t)chunk * a.clen;
     const size_t r1 = min(a.tail, r0 + a.clen);
     const int *ord = a.order + (size_t)xc * a.n;
     const float *dl = a.delta + (size_t)xc * a.n;
+    const int T = r1 > r0 ? (int)((r1 - r0 + kTile - 1) / kTile) : 0;
+    auto tile_rows = [&](int t) { return (int)min((size_t)kTile, r1 - (r0 + (size_t)t * kTile)); };
 
+    // ---- gather sources of this thread's tile columns (fixed over tiles)
+    const float *cp[UM];
+    size_t cld[UM];
+    int ccol[UM], cbs[UM];
+#pragma unroll
+    for (int u = 0; u < UM; ++u) {
+        const int col = lane + 64 * u;
+        cp[u] = nullptr;
+        cld[u] = 0;
+        ccol[u] = col < nc ? col : -1;
+        cbs[u] = -1;
+        if (col < m) {
+            cp[u] = a.Bo + col;
+            cld[u] = a.ldb;
+        } else if (col < m + kSlots) {
+            const int j = col - m;
+            if (j < nact) {
+                cp[u] = a.rbuf + (c0 + sbase + j);
+                cld[u] = a.ldr;
+            }
+        } else if (col == m + kSlots) {
+            cp[u] = a.y;
+            cld[u] = 1;
+        } else if (col >= m + kSlots + 2 && col < nc) {
+            const int j = col - m - kSlots - 2;
+            cbs[u] = j;
+            if (j < nact) {
+                cp[u] = a.B + a.cpar[c0 + sbase + j];
+                cld[u] = a.ldb;
+            }
+        }
+    }
+
+    // ---- this lane's covariate columns and state
+    int qoff[KPL];
     double f[KPL], g[KPL], Y[KPL];
+    bool inff[KPL];
     double k2 = 0.0;
-    const int sb = (m + 2) / 64, tb = (m + 2) % 64;
-    const int sy_ = (m + 1) / 64, ty_ = (m + 1) % 64;
 #pragma unroll
     for (int s = 0; s < KPL; ++s) {
-        const int k = lane + 64 * s;
+        const int u = kl + L * s;
+        int col;
+        if (u == 0) col = m + kSlots + 2 + slot;
+        else if (u == 1) col = m + kSlots;
+        else if (u == 2) col = m + slot;
+        else if (u <= m + 2) col = u - 3;
+        else col = m + kSlots + 1;
+        qoff[s] = col * kPitch;
         f[s] = g[s] = 0.0;
-        Y[s] = k < m ? a.ybo[k] : (k == m ? a.ybx[c] : 0.0);
+        Y[s] = (u == 2) ? a.ybx[c] : ((u >= 3 && u <= m + 2) ? a.ybo[u - 3] : 0.0);
+        inff[s] = u >= 2 && u <= m + 2;
     }
     if (EVAL && active) {
         const double *src = a.sum + ((size_t)c * a.C + chunk) * a.L2 * 2;
 #pragma unroll
         for (int s = 0; s < KPL; ++s) {
-            f[s] = src[2 * (lane + 64 * s)];
-            g[s] = src[2 * (lane + 64 * s) + 1];
+            f[s] = src[2 * (kl + L * s)];
+            g[s] = src[2 * (kl + L * s) + 1];
         }
         k2 = a.ksum[(size_t)c * a.C + chunk];
     }
 
-    // grid rows inside this chunk and the tie-run tracker
+    // ---- grid rows and the tie-run tracker
     const size_t g0 = a.head + 1;
-    size_t next = g0;
-    if (r0 > g0) next = g0 + ((r0 - g0 + a.span - 1) / a.span) * a.span;
-    long long last_nz = -1;  // last sorted row <= current with a nonzero delta
+    int lastnz = -1;  // last sorted row before the current tile with a nonzero delta
     if (EVAL) {
         const size_t lo = r0 > a.span ? r0 - a.span : 0;
-        for (size_t l = lo; l < r0; ++l)
-            if (dl[l] != 0.0f) last_nz = (long long)l;
+        int mine = -1;
+        for (size_t l = lo + lane; l < r0; l += 64)
+            if (dl[l] != 0.0f) mine = (int)l;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) mine = max(mine, __shfl_xor(mine, o, 64));
+        lastnz = mine;
     }
-    double bsse = 0.0;
+    double bsse = 0.0, dacc = 0.0;
     int bidx = -1;
-    double dacc = 0.0;
 
-    for (size_t t0 = r0; t0 < r1; t0 += kTileRows) {
-        const int nr = (int)min((size_t)kTileRows, r1 - t0);
-        __syncthreads();
-        if ((int)threadIdx.x < nr) {
-            const int row = ord[t0 + threadIdx.x];
-            srow[threadIdx.x] = row;
-            sd[threadIdx.x] = dl[t0 + threadIdx.x];
-            sy[threadIdx.x] = a.y[row];
-        }
-        __syncthreads();
-        for (int e = threadIdx.x; e < nr * m; e += blockDim.x) {
-            const int rr = e / m, k = e - rr * m;
-            sbo[rr * m + k] = a.Bo[(size_t)srow[rr] * a.ldb + k];
-        }
-        __syncthreads();
-        if (!active) continue;
-        // per-lane preload of this parent's b and bx for the tile rows
-        float bt = 0.f, bxt = 0.f;
-        if (lane < nr) {
-            const size_t row = (size_t)srow[lane];
-            bt = a.B[row * a.ldb + par];
-            bxt = a.rbuf[row * a.ldr + c];
-        }
-        for (int rr = 0; rr < nr; ++rr) {
-            const size_t i = t0 + rr;
-            const double d = (double)sd[rr];
-            const double b = (double)__shfl(bt, rr, 64);
-            const double bx = (double)__shfl(bxt, rr, 64);
-            const double yv = (double)sy[rr];
-            dacc += d;
-            // k uses the b-lane (f, g) before this row's update
-            double fb = 0.0, gb = 0.0;
+    // tile masks of the wave: bit rr = row rr is a grid row / has a nonzero delta
+    auto tile_masks = [&](int t, float dv, unsigned &km, unsigned &nzm) {
+        const int nr = tile_rows(t);
+        const size_t i = r0 + (size_t)t * kTile + lane;
+        const bool in = lane < nr;
+        const bool grid = in && i >= g0 && ((i - g0) % a.span) == 0;
+        km = (unsigned)__ballot(grid);
+        nzm = (unsigned)__ballot(in && dv != 0.0f);
+    };
+
+    // ---- prologue: row indices of tiles 0 and 1, tile 0 into buffer 0
+    if (tid < kTile) {
+        if (T > 0 && tid < tile_rows(0)) srow[tid] = ord[r0 + tid];
+        if (T > 1 && tid < tile_rows(1)) srow[kTile + tid] = ord[r0 + kTile + tid];
+    }
+    __syncthreads();
+    unsigned kmask = 0, nzmask = 0;
+    if (T > 0) {
+        const int nr = tile_rows(0);
 #pragma unroll
-            for (int s = 0; s < KPL; ++s)
-                if (s == sb) {
-                    fb = f[s];
-                    gb = g[s];
-                }
-            k2 = fma(d * d, gb, k2);
-            k2 = fma(d * 2, fb, k2);
+        for (int i = 0; i < RPW; ++i) {
+            const int rr = wv + W * i;
+            const bool in = rr < nr;
+            const size_t row = in ? (size_t)srow[rr] : 0;
 #pragma unroll
-            for (int s = 0; s < KPL; ++s) {
-                const int k = lane + 64 * s;
-                double q;
-                if (k < m) q = (double)sbo[rr * m + k];
-                else if (k == m) q = bx;
-                else if (k == m + 1) q = yv;
-                else if (k == m + 2) q = b;
-                else q = 0.0;
-                f[s] = fma(d, g[s], f[s]);
-                g[s] = fma(b, q, g[s]);
+            for (int u = 0; u < UM; ++u) {
+                const float v = (in && cp[u]) ? cp[u][row * cld[u]] : 0.0f;
+                if (ccol[u] >= 0) sq[ccol[u] * kPitch + rr] = v;
+                if (cbs[u] >= 0) sb[cbs[u] * kTile + rr] = (double)v;
             }
-            if (EVAL) {
-                if (d != 0.0) last_nz = (long long)i;
-                if (i == next) {
-                    next += a.span;
-                    const bool dup = (i >= g0 + a.span) && (last_nz <= (long long)(i - a.span));
-                    double ff = 0.0, fy = 0.0;
+        }
+        const float dv = lane < nr ? dl[r0 + lane] : 0.0f;
+        if (tid < kTile) {
+            const double d = (double)dv;
+            sdd[tid] = make_double2(d, d * d);
+        }
+        if (EVAL) tile_masks(0, dv, kmask, nzmask);
+    }
+    __syncthreads();
+
+    for (int t = 0; t < T; ++t) {
+        const int cur = t & 1, nxt = cur ^ 1;
+        const size_t tb0 = r0 + (size_t)t * kTile;
+        const int nr1 = t + 1 < T ? tile_rows(t + 1) : 0;
+        const int nr2 = t + 2 < T ? tile_rows(t + 2) : 0;
+        // ---- prefetch tile t+1 (its row indices are in srow[nxt]) and tile t+2's indices
+        float gv[RPW][UM];
 #pragma unroll
-                    for (int s = 0; s < KPL; ++s) {
-                        const int k = lane + 64 * s;
-                        if (k <= m) {
-                            ff = fma(f[s], f[s], ff);
-                            fy = fma(f[s], Y[s], fy);
+        for (int i = 0; i < RPW; ++i) {
+            const int rr = wv + W * i;
+            const bool in = rr < nr1;
+            const size_t row = in ? (size_t)srow[nxt * kTile + rr] : 0;
+#pragma unroll
+            for (int u = 0; u < UM; ++u) gv[i][u] = (in && cp[u]) ? cp[u][row * cld[u]] : 0.0f;
+        }
+        const float nd = lane < nr1 ? dl[tb0 + kTile + lane] : 0.0f;
+        int nrow = 0;
+        if (tid < nr2) nrow = ord[tb0 + 2 * kTile + tid];
+
+        // ---- the recurrence over tile t (marsalgo.cc:576-649)
+        if (wave_active) {
+            const float *tq = sq + (size_t)cur * nc * kPitch;
+            const double *tbv = sb + (cur * kSlots + slot) * kTile;
+            const double2 *td = sdd + cur * kTile;
+#pragma unroll
+            for (int rr = 0; rr < kTile; ++rr) {
+                const double2 dd = td[rr];
+                const double b = tbv[rr];
+                // k uses the b lane's (f, g) before this row's update
+                k2 = fma(dd.y, g[0], k2);
+                k2 = fma(dd.x * 2, f[0], k2);
+#pragma unroll
+                for (int s = 0; s < KPL; ++s) {
+                    const double q = (double)tq[qoff[s] + rr];
+                    f[s] = fma(dd.x, g[s], f[s]);
+                    g[s] = fma(b, q, g[s]);
+                }
+                if (!EVAL) dacc += dd.x;
+                if (EVAL && ((kmask >> rr) & 1u)) {
+                    const size_t i = tb0 + rr;
+                    const unsigned nzb = nzmask & (0xffffffffu >> (31 - rr));
+                    const long long last = nzb ? (long long)(tb0 + 31 - __clz(nzb)) : (long long)lastnz;
+                    const bool dup = (i >= g0 + a.span) && (last <= (long long)(i - a.span));
+                    if (!dup) {
+                        double ff = 0.0, fy = 0.0;
+#pragma unroll
+                        for (int s = 0; s < KPL; ++s)
+                            if (inff[s]) {
+                                ff = fma(f[s], f[s], ff);
+                                fy = fma(f[s], Y[s], fy);
+                            }
+#pragma unroll
+                        for (int o = L / 2; o > 0; o >>= 1) {
+                            ff = ff + __shfl_xor(ff, o, 64);
+                            fy = fy + __shfl_xor(fy, o, 64);
+                        }
+                        const double w = __shfl(f[0], half * L + 1, 64);
+                        const double kk = __shfl(k2, half * L, 64);
+                        const double uw = fy - w;
+                        const double den = kk - ff;
+                        const double sse = den > a.tol ? (uw * uw) / (den + a.tol) : 0.0;
+                        if (sse > bsse) {
+                            bsse = sse;
+                            bidx = (int)i;
                         }
                     }
-                    ff = wave_sum(ff);
-                    fy = wave_sum(fy);
-                    double fw = 0.0;
-#pragma unroll
-                    for (int s = 0; s < KPL; ++s)
-                        if (s == sy_) fw = f[s];
-                    const double w = __shfl(fw, ty_, 64);
-                    const double kk = __shfl(k2, tb, 64);
-                    const double uw = fy - w;
-                    const double den = kk - ff;
-                    const double sse = den > a.tol ? (uw * uw) / (den + a.tol) : 0.0;
-                    if (!dup && sse > bsse) {
-                        bsse = sse;
-                        bidx = (int)i;
-                    }
                 }
             }
         }
+        if (EVAL && nzmask) lastnz = (int)(tb0 + 31 - __clz(nzmask));
+        if (EVAL && t + 1 < T) tile_masks(t + 1, nd, kmask, nzmask);
+        // buffer nxt last held tile t-1, which every wave finished before the
+        // previous iteration's barrier: store tile t+1 into it
+#pragma unroll
+        for (int i = 0; i < RPW; ++i) {
+            const int rr = wv + W * i;
+#pragma unroll
+            for (int u = 0; u < UM; ++u) {
+                if (ccol[u] >= 0) sq[((size_t)nxt * nc + ccol[u]) * kPitch + rr] = gv[i][u];
+                if (cbs[u] >= 0) sb[(nxt * kSlots + cbs[u]) * kTile + rr] = (double)gv[i][u];
+            }
+        }
+        if (tid < kTile) {
+            const double d = (double)nd;
+            sdd[nxt * kTile + tid] = make_double2(d, d * d);
+            if (tid < nr2) srow[cur * kTile + tid] = nrow;
+        }
+        __syncthreads();
     }
     if (!active) return;
-    const size_t slot = (size_t)c * a.C + chunk;
+    const size_t cs = (size_t)c * a.C + chunk;
     if (EVAL) {
-        if (lane == 0) {
-            a.best[slot] = bsse;
-            a.besti[slot] = bidx;
+        if (kl == 0) {
+            a.best[cs] = bsse;
+            a.besti[cs] = bidx;
         }
     } else {
-        double *dst = a.sum + slot * a.L2 * 2;
+        double *dst = a.sum + cs * a.L2 * 2;
 #pragma unroll
         for (int s = 0; s < KPL; ++s) {
-            dst[2 * (lane + 64 * s)] = f[s];
-            dst[2 * (lane + 64 * s) + 1] = g[s];
+            dst[2 * (kl + L * s)] = f[s];
+            dst[2 * (kl + L * s) + 1] = g[s];
         }
-        if (lane == tb) a.ksum[slot] = k2;
-        if (lane == 0 && j == 0) a.dsum[(size_t)gi * a.C + chunk] = dacc;
+        if (kl == 0) a.ksum[cs] = k2;
+        if (kl == 0 && pg == 0 && slot == 0) a.dsum[(size_t)gi * a.C + chunk] = dacc;
     }
 }
 
@@ -338,14 +464,14 @@ __global__ void k_sweep_carry(SweepArgs a, const int *__restrict__ cgrp, int P)
     const int gi = cgrp[c];
     const int m = a.m;
     double f = 0.0, g = 0.0, kk = 0.0;
-    // the thread of lane m+2 also carries k and needs its own (f, g) = (bd, b2)
+    // the thread of lane slot 0 (the parent b) also carries k and needs its own (f, g) = (bd, b2)
     for (int ch = 0; ch < a.C; ++ch) {
         double *s = a.sum + (((size_t)c * a.C + ch) * a.L2 + k) * 2;
         const double F = s[0], U = s[1];
         const double D = a.dsum[(size_t)gi * a.C + ch];
         s[0] = f;
         s[1] = g;
-        if (k == m + 2) {
+        if (k == 0) {
             double *ks = a.ksum + (size_t)c * a.C + ch;
             const double K = *ks;
             *ks = kk;
@@ -384,12 +510,19 @@ __global__ void k_sweep_final(SweepArgs a, int P, const int *__restrict__ cxc,
 }
 
 template <int KPL>
-void launch_sweep(const SweepArgs &a, dim3 grid, size_t lds, hipStream_t st, bool eval)
+void launch_sweep(const SweepArgs &a, dim3 grid, size_t lds, hipStream_t st, bool eval, int pk)
 {
-    if (eval)
-        hipLaunchKernelGGL((k_sweep<KPL, true>), grid, dim3(1024), lds, st, a);
-    else
-        hipLaunchKernelGGL((k_sweep<KPL, false>), grid, dim3(1024), lds, st, a);
+    if (pk == 2) {
+        if (eval)
+            hipLaunchKernelGGL((k_sweep<1, 2, true>), grid, dim3(64 * kSlots / 2), lds, st, a);
+        else
+            hipLaunchKernelGGL((k_sweep<1, 2, false>), grid, dim3(64 * kSlots / 2), lds, st, a);
+    } else {
+        if (eval)
+            hipLaunchKernelGGL((k_sweep<KPL, 1, true>), grid, dim3(64 * kSlots), lds, st, a);
+        else
+            hipLaunchKernelGGL((k_sweep<KPL, 1, false>), grid, dim3(64 * kSlots), lds, st, a);
+    }
     MH_LAUNCH_CHECK();
 }
 
@@ -473,14 +606,23 @@ void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span)
     }
     const int ng = (int)(grp.size() / 3);
     const size_t tail = n - (size_t)end_span;
-    const int pgroups = (maxnp + 15) / 16;
-    const int want = std::max(1, (512 + ng * pgroups - 1) / (ng * pgroups));
-    const int maxc = (int)std::max<size_t>(1, tail / 4096);
+    require(m <= 253, "hinge sweep supports up to 253 basis functions");
+    const int PK = m + 3 <= 32 ? 2 : 1;  // candidates per wave
+    const int KPL = PK == 2 ? 1 : (m + 3 + 63) / 64;
+    const int L2 = KPL * (64 / PK);      // state lanes per candidate
+    const size_t lds = sweep_lds(m + 2 * kSlots + 2);
+    {
+        int dev = 0, lmax = 0;
+        MH_CHECK(hipGetDevice(&dev));
+        MH_CHECK(hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        require(lds <= (size_t)lmax, "hinge sweep: basis too large for the device's LDS");
+    }
+    // about 4096 blocks (16 per CU), chunks of at least 512 sorted rows
+    const int pgroups = (maxnp + kSlots - 1) / kSlots;
+    const int want = std::max(1, (4096 + ng * pgroups - 1) / (ng * pgroups));
+    const int maxc = (int)std::max<size_t>(1, tail / 512);
     const int C = std::min(want, maxc);
     const size_t clen = (tail + C - 1) / C;
-    const int KPL = (m + 3 + 63) / 64;
-    require(KPL <= 4, "hinge sweep supports up to 253 basis functions");
-    const int L2 = KPL * 64;
 
     DBuf<int> dgrp, dcgrp;
     dgrp.ensure(grp.size());
@@ -526,7 +668,6 @@ void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span)
     {
         KTimer kt(ctx, K_SWEEP, (double)P * (double)tail * (m + 1) * (4.0 + 4.0 / min_span),
                   (double)ng * tail * (4.0 * m + 12.0) + (double)P * tail * 8.0);
-        const size_t lds = (size_t)kTileRows * m * sizeof(float) + 3 * kTileRows * sizeof(float);
         for (int pass = 0; pass < 2; ++pass) {
             const bool eval = pass == 1;
             if (eval) {
@@ -536,10 +677,10 @@ void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span)
                 MH_LAUNCH_CHECK();
             }
             switch (KPL) {
-            case 1: launch_sweep<1>(a, grid, lds, st, eval); break;
-            case 2: launch_sweep<2>(a, grid, lds, st, eval); break;
-            case 3: launch_sweep<3>(a, grid, lds, st, eval); break;
-            default: launch_sweep<4>(a, grid, lds, st, eval); break;
+            case 1: launch_sweep<1>(a, grid, lds, st, eval, PK); break;
+            case 2: launch_sweep<2>(a, grid, lds, st, eval, PK); break;
+            case 3: launch_sweep<3>(a, grid, lds, st, eval, PK); break;
+            default: launch_sweep<4>(a, grid, lds, st, eval, PK); break;
             }
         }
         hipLaunchKernelGGL(k_sweep_final, dim3((P + 255) / 256), dim3(256), 0, st, a, P, cs.d_xc.p,
