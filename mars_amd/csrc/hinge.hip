@@ -156,7 +156,7 @@ struct SweepArgs {
     const int *cpar;     // parent per candidate
     const int *order;    // [px][n]
     const float *delta;  // [px][n]
-    int m, L2, C;        // basis count, padded lane count (KPL*64), chunks
+    int m, L2, C;        // basis count, covariate slots per candidate, chunks
     size_t tail, head, span, clen;
     double tol;
     double *sum;         // [P][C][L2][2]   chunk summaries, then carry-ins (f, g)
@@ -166,135 +166,153 @@ struct SweepArgs {
     int *besti;          // [P][C]
 };
 
-constexpr int kTile = 32;   // sorted rows per tile
-constexpr int kPitch = 33;  // LDS floats per tile column (odd: column-major tile without bank conflicts)
-constexpr int kSlots = 8;   // candidates per block
+constexpr int kTR = 16;    // sorted rows per tile
+constexpr int kCand = 8;   // candidates per block
 
-/* LDS bytes of one k_sweep block for `nc` tile columns (two buffers of everything). */
-__host__ __device__ constexpr size_t sweep_lds(int nc)
+/*
+ * Lane layout of one candidate: G lanes, each holding J consecutive covariate
+ * slots u = t J + s (t = lane in the group): u = 0 the parent b, u = 1 y,
+ * u = 2 the candidate's Bx column, u = 3 .. m+2 the orthonormal basis, the
+ * rest zero padding up to PT = G J slots.
+ */
+template <int PT>
+struct SweepShape {
+    static constexpr int J = PT == 32 ? 4 : 8;          // slots per lane
+    static constexpr int G = PT / J;                    // lanes per candidate
+    static constexpr int CW = 64 / G;                   // candidates per wave
+    static constexpr int NW = kCand / CW;               // waves per block
+    static constexpr int RPI = PT < 64 ? 64 / PT : 1;   // tile rows per wave-wide load
+};
+
+/* v from the lane DPP control CTRL names (a permutation within each row of 16 lanes). */
+template <int CTRL>
+__device__ __forceinline__ double dpp_perm(double v)
 {
-    return 2 * (size_t)kSlots * kTile * sizeof(double)   // b per slot and row (f64)
-         + 2 * (size_t)kTile * 2 * sizeof(double)        // (d, d*d) per row
-         + 2 * (size_t)kTile * sizeof(int)               // sorted-order row indices
-         + 2 * (size_t)nc * kPitch * sizeof(float);      // the tile, column-major
+    const unsigned long long x = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)x, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(x >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+/* Sum over aligned groups of G lanes; every lane of a group gets the same bits. */
+template <int G>
+__device__ __forceinline__ double group_sum(double v)
+{
+    v = v + dpp_perm<0xB1>(v);   // quad_perm [1,0,3,2]
+    v = v + dpp_perm<0x4E>(v);   // quad_perm [2,3,0,1]
+    v = v + dpp_perm<0x141>(v);  // row_half_mirror: lane i <-> 7-i
+    if (G >= 16) v = v + dpp_perm<0x140>(v);  // row_mirror: lane i <-> 15-i
+    if (G >= 32) v = v + __shfl_xor(v, 16, 64);
+    return v;
 }
 
 /*
- * One block = W = 8/PK waves sweeping kSlots candidates (the parents c0.. of one
- * X column) over the sorted rows [r0, r1) of its chunk.  PK = 2 packs two
- * candidates of at most 29 basis functions into the halves of a wave (L = 32
- * lanes each).  Lane slot u = lane % L + L s (s < KPL) of a candidate holds
- * the running (f, g) of one covariate column: u = 0 the parent b, u = 1 y,
- * u = 2 the candidate's Bx column, u = 3 .. m+2 the orthonormal basis; the
- * b lane also carries k = |h|^2 (marsalgo.cc:576-649).
- *
- * Tiles of 32 sorted rows are gathered into LDS column-major, columns
- *   [0, m) Bo row | [m, m+8) Bx per slot | m+8 y | m+9 zero | [m+10, m+18) b per slot,
- * double-buffered: the global gathers of tile t+1 are in flight while the
+ * One block = kCand candidates (parents c0.. of one X column) swept over the
+ * sorted rows [r0, r1) of its chunk (marsalgo.cc:576-649).  Tiles of 16 sorted
+ * rows are gathered into LDS: row-major [0 | y | 0 | Bo row | 0...] (PT
+ * floats), plus b (f64) and Bx per candidate, and the deltas (d, d^2).  The
+ * buffers are doubled: the global gathers of tile t+1 are in flight while the
  * unrolled recurrence runs over tile t.  Rows past the chunk end are zero
- * (d = b = 0 leaves every accumulator unchanged), so every tile runs 32 rows.
- * Knot rows come as a per-tile bit mask (grid position) plus a mask of nonzero
- * deltas for the tie-run test, both in scalar registers.
+ * (d = b = 0 leaves every accumulator unchanged), so every tile runs 16 rows.
+ * Grid rows and nonzero deltas of a tile are bit masks in scalar registers.
+ * At a grid row each lane sums its slots' f.Y (Y = -1 on the y slot, so the
+ * sum is fy - w) and k - |f|^2 over the bx and basis slots, and the group
+ * reduces both with DPP.
  */
-template <int KPL, int PK, bool EVAL>
-__global__ __launch_bounds__(512) void k_sweep(SweepArgs a)
+template <int PT, bool EVAL>
+__global__ __launch_bounds__(64 * SweepShape<PT>::NW) void k_sweep(SweepArgs a)
 {
-    constexpr int L = 64 / PK;
-    constexpr int W = kSlots / PK;                  // waves per block
-    constexpr int NT = 64 * W;
-    constexpr int RPW = kTile / W;                  // gathered rows per wave and tile
-    constexpr int UM = PK == 2 ? 1 : KPL + 1;       // 64-column groups of the tile
-    extern __shared__ double lds_d[];
-    const int m = a.m;
-    const int nc = m + 2 * kSlots + 2;
-    double *sb = lds_d;                                       // [2][kSlots][32]
-    double2 *sdd = (double2 *)(sb + 2 * kSlots * kTile);      // [2][32]
-    int *srow = (int *)(sdd + 2 * kTile);                     // [2][32]
-    float *sq = (float *)(srow + 2 * kTile);                  // [2][nc][33]
+    using S = SweepShape<PT>;
+    constexpr int J = S::J, G = S::G, CW = S::CW, NW = S::NW, RPI = S::RPI;
+    constexpr int NB = kTR / RPI;                          // Bo-row loads per lane and tile
+    constexpr int NPC = 2 * kCand * kTR / (64 * NW);       // per-candidate loads per lane and tile
+    __shared__ double sbv[2][kTR][kCand];                  // b (f64)
+    __shared__ double2 sdd[2][kTR];                        // (d, d*d)
+    __shared__ __attribute__((aligned(16))) float sq[2][kTR][PT];
+    __shared__ float sbx[2][kTR][kCand];
+    __shared__ int srow[2][kTR];
 
+    const int m = a.m;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int half = PK == 2 ? lane >> 5 : 0, kl = lane & (L - 1);
+    const int t = lane % G, grp = lane / G;
     const int chunk = blockIdx.x, pg = blockIdx.y, gi = blockIdx.z;
     const int xc = a.grp[3 * gi], c0 = a.grp[3 * gi + 1], np = a.grp[3 * gi + 2];
-    const int sbase = pg * kSlots;                  // first parent slot of this block
-    if (sbase >= np) return;                        // whole block idle (uniform)
-    const int nact = min(kSlots, np - sbase);
-    const int slot = wv * PK + half;                // this lane's candidate slot
+    const int sbase = pg * kCand;                  // first parent slot of this block
+    if (sbase >= np) return;                       // whole block idle (uniform)
+    const int nact = min(kCand, np - sbase);
+    const int slot = wv * CW + grp;                // this lane's candidate slot
     const bool active = slot < nact;
-    const bool wave_active = wv * PK < nact;
+    const bool wave_active = wv * CW < nact;
     const int c = c0 + sbase + (active ? slot : 0);
     const size_t r0 = (size_t)chunk * a.clen;
     const size_t r1 = min(a.tail, r0 + a.clen);
     const int *ord = a.order + (size_t)xc * a.n;
     const float *dl = a.delta + (size_t)xc * a.n;
-    const int T = r1 > r0 ? (int)((r1 - r0 + kTile - 1) / kTile) : 0;
-    auto tile_rows = [&](int t) { return (int)min((size_t)kTile, r1 - (r0 + (size_t)t * kTile)); };
+    const int T = r1 > r0 ? (int)((r1 - r0 + kTR - 1) / kTR) : 0;
+    auto tile_rows = [&](int tt) { return (int)min((size_t)kTR, r1 - (r0 + (size_t)tt * kTR)); };
 
-    // ---- gather sources of this thread's tile columns (fixed over tiles)
-    const float *cp[UM];
-    size_t cld[UM];
-    int ccol[UM], cbs[UM];
+    // ---- gather sources (fixed over tiles)
+    const int pos = PT < 64 ? (lane & (PT - 1)) : (wv * 64 + lane);   // tile column
+    const int ro = PT < 64 ? lane / PT : 0;                           // row within a load
+    const float *bp = nullptr;
+    size_t bld = 0;
+    if (pos == 1) {
+        bp = a.y;
+        bld = 1;
+    } else if (pos >= 3 && pos < m + 3) {
+        bp = a.Bo + (pos - 3);
+        bld = a.ldb;
+    }
+    const float *pp[NPC];
+    size_t pld[NPC];
+    int prr[NPC], pj[NPC];
+    bool pisb[NPC];
 #pragma unroll
-    for (int u = 0; u < UM; ++u) {
-        const int col = lane + 64 * u;
-        cp[u] = nullptr;
-        cld[u] = 0;
-        ccol[u] = col < nc ? col : -1;
-        cbs[u] = -1;
-        if (col < m) {
-            cp[u] = a.Bo + col;
-            cld[u] = a.ldb;
-        } else if (col < m + kSlots) {
-            const int j = col - m;
-            if (j < nact) {
-                cp[u] = a.rbuf + (c0 + sbase + j);
-                cld[u] = a.ldr;
-            }
-        } else if (col == m + kSlots) {
-            cp[u] = a.y;
-            cld[u] = 1;
-        } else if (col >= m + kSlots + 2 && col < nc) {
-            const int j = col - m - kSlots - 2;
-            cbs[u] = j;
-            if (j < nact) {
-                cp[u] = a.B + a.cpar[c0 + sbase + j];
-                cld[u] = a.ldb;
+    for (int i = 0; i < NPC; ++i) {
+        const int e = tid + 64 * NW * i;
+        pisb[i] = e < kCand * kTR;
+        const int rem = e % (kCand * kTR);
+        prr[i] = rem / kCand;
+        pj[i] = rem % kCand;
+        pp[i] = nullptr;
+        pld[i] = 0;
+        if (pj[i] < nact) {
+            if (pisb[i]) {
+                pp[i] = a.B + a.cpar[c0 + sbase + pj[i]];
+                pld[i] = a.ldb;
+            } else {
+                pp[i] = a.rbuf + (c0 + sbase + pj[i]);
+                pld[i] = a.ldr;
             }
         }
     }
 
-    // ---- this lane's covariate columns and state
-    int qoff[KPL];
-    double f[KPL], g[KPL], Y[KPL];
-    bool inff[KPL];
+    // ---- lane state
+    double f[J], g[J], Y[J];
+    const double isb = t == 0 ? 1.0 : 0.0;   // this lane holds the b, y and bx slots
+    const float isbf = t == 0 ? 1.0f : 0.0f;
     double k2 = 0.0;
 #pragma unroll
-    for (int s = 0; s < KPL; ++s) {
-        const int u = kl + L * s;
-        int col;
-        if (u == 0) col = m + kSlots + 2 + slot;
-        else if (u == 1) col = m + kSlots;
-        else if (u == 2) col = m + slot;
-        else if (u <= m + 2) col = u - 3;
-        else col = m + kSlots + 1;
-        qoff[s] = col * kPitch;
+    for (int s = 0; s < J; ++s) {
+        const int u = t * J + s;
         f[s] = g[s] = 0.0;
-        Y[s] = (u == 2) ? a.ybx[c] : ((u >= 3 && u <= m + 2) ? a.ybo[u - 3] : 0.0);
-        inff[s] = u >= 2 && u <= m + 2;
+        Y[s] = u == 1 ? -1.0 : (u == 2 ? a.ybx[c] : ((u >= 3 && u <= m + 2) ? a.ybo[u - 3] : 0.0));
     }
     if (EVAL && active) {
         const double *src = a.sum + ((size_t)c * a.C + chunk) * a.L2 * 2;
 #pragma unroll
-        for (int s = 0; s < KPL; ++s) {
-            f[s] = src[2 * (kl + L * s)];
-            g[s] = src[2 * (kl + L * s) + 1];
+        for (int s = 0; s < J; ++s) {
+            f[s] = src[2 * (t * J + s)];
+            g[s] = src[2 * (t * J + s) + 1];
         }
         k2 = a.ksum[(size_t)c * a.C + chunk];
     }
 
     // ---- grid rows and the tie-run tracker
     const size_t g0 = a.head + 1;
-    int lastnz = -1;  // last sorted row before the current tile with a nonzero delta
+    size_t next = g0;  // next grid row at or after the current tile
+    if (r0 > g0) next = g0 + ((r0 - g0 + a.span - 1) / a.span) * a.span;
+    int lastnz = -1;   // last sorted row before the current tile with a nonzero delta
     if (EVAL) {
         const size_t lo = r0 > a.span ? r0 - a.span : 0;
         int mine = -1;
@@ -306,81 +324,91 @@ __global__ __launch_bounds__(512) void k_sweep(SweepArgs a)
     }
     double bsse = 0.0, dacc = 0.0;
     int bidx = -1;
-
-    // tile masks of the wave: bit rr = row rr is a grid row / has a nonzero delta
-    auto tile_masks = [&](int t, float dv, unsigned &km, unsigned &nzm) {
-        const int nr = tile_rows(t);
-        const size_t i = r0 + (size_t)t * kTile + lane;
-        const bool in = lane < nr;
-        const bool grid = in && i >= g0 && ((i - g0) % a.span) == 0;
-        km = (unsigned)__ballot(grid);
-        nzm = (unsigned)__ballot(in && dv != 0.0f);
+    auto grid_mask = [&](int tt) {
+        const size_t tb = r0 + (size_t)tt * kTR, te = tb + tile_rows(tt);
+        unsigned km = 0;
+        for (; next < te; next += a.span) km |= 1u << (unsigned)(next - tb);
+        return km;
     };
 
     // ---- prologue: row indices of tiles 0 and 1, tile 0 into buffer 0
-    if (tid < kTile) {
-        if (T > 0 && tid < tile_rows(0)) srow[tid] = ord[r0 + tid];
-        if (T > 1 && tid < tile_rows(1)) srow[kTile + tid] = ord[r0 + kTile + tid];
+    if (tid < kTR) {
+        if (T > 0 && tid < tile_rows(0)) srow[0][tid] = ord[r0 + tid];
+        if (T > 1 && tid < tile_rows(1)) srow[1][tid] = ord[r0 + kTR + tid];
     }
     __syncthreads();
     unsigned kmask = 0, nzmask = 0;
     if (T > 0) {
         const int nr = tile_rows(0);
 #pragma unroll
-        for (int i = 0; i < RPW; ++i) {
-            const int rr = wv + W * i;
-            const bool in = rr < nr;
-            const size_t row = in ? (size_t)srow[rr] : 0;
+        for (int i = 0; i < NB; ++i) {
+            const int rr = i * RPI + ro;
+            const bool in = rr < nr && bp;
+            sq[0][rr][pos] = in ? bp[(size_t)srow[0][rr] * bld] : 0.0f;
+        }
 #pragma unroll
-            for (int u = 0; u < UM; ++u) {
-                const float v = (in && cp[u]) ? cp[u][row * cld[u]] : 0.0f;
-                if (ccol[u] >= 0) sq[ccol[u] * kPitch + rr] = v;
-                if (cbs[u] >= 0) sb[cbs[u] * kTile + rr] = (double)v;
-            }
+        for (int i = 0; i < NPC; ++i) {
+            const bool in = prr[i] < nr && pp[i];
+            const float v = in ? pp[i][(size_t)srow[0][prr[i]] * pld[i]] : 0.0f;
+            if (pisb[i]) sbv[0][prr[i]][pj[i]] = (double)v;
+            else sbx[0][prr[i]][pj[i]] = v;
         }
         const float dv = lane < nr ? dl[r0 + lane] : 0.0f;
-        if (tid < kTile) {
+        if (tid < kTR) {
             const double d = (double)dv;
-            sdd[tid] = make_double2(d, d * d);
+            sdd[0][tid] = make_double2(d, d * d);
         }
-        if (EVAL) tile_masks(0, dv, kmask, nzmask);
+        if (EVAL) {
+            kmask = grid_mask(0);
+            nzmask = (unsigned)__ballot(dv != 0.0f);
+        }
     }
     __syncthreads();
 
-    for (int t = 0; t < T; ++t) {
-        const int cur = t & 1, nxt = cur ^ 1;
-        const size_t tb0 = r0 + (size_t)t * kTile;
-        const int nr1 = t + 1 < T ? tile_rows(t + 1) : 0;
-        const int nr2 = t + 2 < T ? tile_rows(t + 2) : 0;
+    for (int tt = 0; tt < T; ++tt) {
+        const int cur = tt & 1, nxt = cur ^ 1;
+        const size_t tb0 = r0 + (size_t)tt * kTR;
+        const int nr1 = tt + 1 < T ? tile_rows(tt + 1) : 0;
+        const int nr2 = tt + 2 < T ? tile_rows(tt + 2) : 0;
         // ---- prefetch tile t+1 (its row indices are in srow[nxt]) and tile t+2's indices
-        float gv[RPW][UM];
+        float bov[NB], pcv[NPC];
 #pragma unroll
-        for (int i = 0; i < RPW; ++i) {
-            const int rr = wv + W * i;
-            const bool in = rr < nr1;
-            const size_t row = in ? (size_t)srow[nxt * kTile + rr] : 0;
-#pragma unroll
-            for (int u = 0; u < UM; ++u) gv[i][u] = (in && cp[u]) ? cp[u][row * cld[u]] : 0.0f;
+        for (int i = 0; i < NB; ++i) {
+            const int rr = i * RPI + ro;
+            const bool in = rr < nr1 && bp;
+            bov[i] = in ? bp[(size_t)srow[nxt][rr] * bld] : 0.0f;
         }
-        const float nd = lane < nr1 ? dl[tb0 + kTile + lane] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < NPC; ++i) {
+            const bool in = prr[i] < nr1 && pp[i];
+            pcv[i] = in ? pp[i][(size_t)srow[nxt][prr[i]] * pld[i]] : 0.0f;
+        }
+        const float nd = lane < nr1 ? dl[tb0 + kTR + lane] : 0.0f;
         int nrow = 0;
-        if (tid < nr2) nrow = ord[tb0 + 2 * kTile + tid];
+        if (tid < nr2) nrow = ord[tb0 + 2 * kTR + tid];
 
         // ---- the recurrence over tile t (marsalgo.cc:576-649)
         if (wave_active) {
-            const float *tq = sq + (size_t)cur * nc * kPitch;
-            const double *tbv = sb + (cur * kSlots + slot) * kTile;
-            const double2 *td = sdd + cur * kTile;
 #pragma unroll
-            for (int rr = 0; rr < kTile; ++rr) {
-                const double2 dd = td[rr];
-                const double b = tbv[rr];
-                // k uses the b lane's (f, g) before this row's update
+            for (int rr = 0; rr < kTR; ++rr) {
+                const double2 dd = sdd[cur][rr];
+                const double b = sbv[cur][rr][slot];
+                float qf[J];
+#pragma unroll
+                for (int s = 0; s < J; s += 4) {
+                    const float4 v4 = *(const float4 *)&sq[cur][rr][t * J + s];
+                    qf[s] = v4.x;
+                    qf[s + 1] = v4.y;
+                    qf[s + 2] = v4.z;
+                    qf[s + 3] = v4.w;
+                }
+                qf[2] = fmaf(isbf, sbx[cur][rr][slot], qf[2]);   // bx on the group's first lane
+                // k uses the b slot's (f, g) before this row's update
                 k2 = fma(dd.y, g[0], k2);
                 k2 = fma(dd.x * 2, f[0], k2);
 #pragma unroll
-                for (int s = 0; s < KPL; ++s) {
-                    const double q = (double)tq[qoff[s] + rr];
+                for (int s = 0; s < J; ++s) {
+                    const double q = s == 0 ? fma(isb, b, (double)qf[0]) : (double)qf[s];
                     f[s] = fma(dd.x, g[s], f[s]);
                     g[s] = fma(b, q, g[s]);
                 }
@@ -391,22 +419,17 @@ __global__ __launch_bounds__(512) void k_sweep(SweepArgs a)
                     const long long last = nzb ? (long long)(tb0 + 31 - __clz(nzb)) : (long long)lastnz;
                     const bool dup = (i >= g0 + a.span) && (last <= (long long)(i - a.span));
                     if (!dup) {
-                        double ff = 0.0, fy = 0.0;
+                        // fy - w over the y, bx and basis slots; k - |f|^2 over the bx and basis slots
+                        const double f0 = f[0] * (1.0 - isb), f1 = f[1] * (1.0 - isb);
+                        double uw = 0.0, ff = f0 * f0;
+                        ff = fma(f1, f1, ff);
 #pragma unroll
-                        for (int s = 0; s < KPL; ++s)
-                            if (inff[s]) {
-                                ff = fma(f[s], f[s], ff);
-                                fy = fma(f[s], Y[s], fy);
-                            }
+                        for (int s = 2; s < J; ++s) ff = fma(f[s], f[s], ff);
 #pragma unroll
-                        for (int o = L / 2; o > 0; o >>= 1) {
-                            ff = ff + __shfl_xor(ff, o, 64);
-                            fy = fy + __shfl_xor(fy, o, 64);
-                        }
-                        const double w = __shfl(f[0], half * L + 1, 64);
-                        const double kk = __shfl(k2, half * L, 64);
-                        const double uw = fy - w;
-                        const double den = kk - ff;
+                        for (int s = 0; s < J; ++s) uw = fma(f[s], Y[s], uw);
+                        double den = fma(isb, k2, -ff);
+                        uw = group_sum<G>(uw);
+                        den = group_sum<G>(den);
                         const double sse = den > a.tol ? (uw * uw) / (den + a.tol) : 0.0;
                         if (sse > bsse) {
                             bsse = sse;
@@ -416,42 +439,45 @@ __global__ __launch_bounds__(512) void k_sweep(SweepArgs a)
                 }
             }
         }
-        if (EVAL && nzmask) lastnz = (int)(tb0 + 31 - __clz(nzmask));
-        if (EVAL && t + 1 < T) tile_masks(t + 1, nd, kmask, nzmask);
+        if (EVAL) {
+            if (nzmask) lastnz = (int)(tb0 + 31 - __clz(nzmask));
+            if (tt + 1 < T) {
+                kmask = grid_mask(tt + 1);
+                nzmask = (unsigned)__ballot(nd != 0.0f);
+            }
+        }
         // buffer nxt last held tile t-1, which every wave finished before the
         // previous iteration's barrier: store tile t+1 into it
 #pragma unroll
-        for (int i = 0; i < RPW; ++i) {
-            const int rr = wv + W * i;
+        for (int i = 0; i < NB; ++i) sq[nxt][i * RPI + ro][pos] = bov[i];
 #pragma unroll
-            for (int u = 0; u < UM; ++u) {
-                if (ccol[u] >= 0) sq[((size_t)nxt * nc + ccol[u]) * kPitch + rr] = gv[i][u];
-                if (cbs[u] >= 0) sb[(nxt * kSlots + cbs[u]) * kTile + rr] = (double)gv[i][u];
-            }
+        for (int i = 0; i < NPC; ++i) {
+            if (pisb[i]) sbv[nxt][prr[i]][pj[i]] = (double)pcv[i];
+            else sbx[nxt][prr[i]][pj[i]] = pcv[i];
         }
-        if (tid < kTile) {
+        if (tid < kTR) {
             const double d = (double)nd;
-            sdd[nxt * kTile + tid] = make_double2(d, d * d);
-            if (tid < nr2) srow[cur * kTile + tid] = nrow;
+            sdd[nxt][tid] = make_double2(d, d * d);
+            if (tid < nr2) srow[cur][tid] = nrow;
         }
         __syncthreads();
     }
     if (!active) return;
     const size_t cs = (size_t)c * a.C + chunk;
     if (EVAL) {
-        if (kl == 0) {
+        if (t == 0) {
             a.best[cs] = bsse;
             a.besti[cs] = bidx;
         }
     } else {
         double *dst = a.sum + cs * a.L2 * 2;
 #pragma unroll
-        for (int s = 0; s < KPL; ++s) {
-            dst[2 * (kl + L * s)] = f[s];
-            dst[2 * (kl + L * s) + 1] = g[s];
+        for (int s = 0; s < J; ++s) {
+            dst[2 * (t * J + s)] = f[s];
+            dst[2 * (t * J + s) + 1] = g[s];
         }
-        if (kl == 0) a.ksum[cs] = k2;
-        if (kl == 0 && pg == 0 && slot == 0) a.dsum[(size_t)gi * a.C + chunk] = dacc;
+        if (t == 0) a.ksum[cs] = k2;
+        if (t == 0 && pg == 0 && slot == 0) a.dsum[(size_t)gi * a.C + chunk] = dacc;
     }
 }
 
@@ -509,20 +535,14 @@ __global__ void k_sweep_final(SweepArgs a, int P, const int *__restrict__ cxc,
     }
 }
 
-template <int KPL>
-void launch_sweep(const SweepArgs &a, dim3 grid, size_t lds, hipStream_t st, bool eval, int pk)
+template <int PT>
+void launch_sweep(const SweepArgs &a, dim3 grid, hipStream_t st, bool eval)
 {
-    if (pk == 2) {
-        if (eval)
-            hipLaunchKernelGGL((k_sweep<1, 2, true>), grid, dim3(64 * kSlots / 2), lds, st, a);
-        else
-            hipLaunchKernelGGL((k_sweep<1, 2, false>), grid, dim3(64 * kSlots / 2), lds, st, a);
-    } else {
-        if (eval)
-            hipLaunchKernelGGL((k_sweep<KPL, 1, true>), grid, dim3(64 * kSlots), lds, st, a);
-        else
-            hipLaunchKernelGGL((k_sweep<KPL, 1, false>), grid, dim3(64 * kSlots), lds, st, a);
-    }
+    const dim3 block(64 * SweepShape<PT>::NW);
+    if (eval)
+        hipLaunchKernelGGL((k_sweep<PT, true>), grid, block, 0, st, a);
+    else
+        hipLaunchKernelGGL((k_sweep<PT, false>), grid, block, 0, st, a);
     MH_LAUNCH_CHECK();
 }
 
@@ -607,20 +627,13 @@ void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span)
     const int ng = (int)(grp.size() / 3);
     const size_t tail = n - (size_t)end_span;
     require(m <= 253, "hinge sweep supports up to 253 basis functions");
-    const int PK = m + 3 <= 32 ? 2 : 1;  // candidates per wave
-    const int KPL = PK == 2 ? 1 : (m + 3 + 63) / 64;
-    const int L2 = KPL * (64 / PK);      // state lanes per candidate
-    const size_t lds = sweep_lds(m + 2 * kSlots + 2);
-    {
-        int dev = 0, lmax = 0;
-        MH_CHECK(hipGetDevice(&dev));
-        MH_CHECK(hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        require(lds <= (size_t)lmax, "hinge sweep: basis too large for the device's LDS");
-    }
-    // about 4096 blocks (16 per CU), chunks of at least 512 sorted rows
-    const int pgroups = (maxnp + kSlots - 1) / kSlots;
+    // covariate slots per candidate: b, y, bx and the m basis columns, padded
+    const int PT = m + 3 <= 32 ? 32 : (m + 3 <= 64 ? 64 : (m + 3 <= 128 ? 128 : 256));
+    const int L2 = PT;
+    // about 4096 blocks (16 per CU), chunks of at least 256 sorted rows
+    const int pgroups = (maxnp + kCand - 1) / kCand;
     const int want = std::max(1, (4096 + ng * pgroups - 1) / (ng * pgroups));
-    const int maxc = (int)std::max<size_t>(1, tail / 512);
+    const int maxc = (int)std::max<size_t>(1, tail / 256);
     const int C = std::min(want, maxc);
     const size_t clen = (tail + C - 1) / C;
 
@@ -676,11 +689,11 @@ void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span)
                                    dcgrp.p, P);
                 MH_LAUNCH_CHECK();
             }
-            switch (KPL) {
-            case 1: launch_sweep<1>(a, grid, lds, st, eval, PK); break;
-            case 2: launch_sweep<2>(a, grid, lds, st, eval, PK); break;
-            case 3: launch_sweep<3>(a, grid, lds, st, eval, PK); break;
-            default: launch_sweep<4>(a, grid, lds, st, eval, PK); break;
+            switch (PT) {
+            case 32: launch_sweep<32>(a, grid, st, eval); break;
+            case 64: launch_sweep<64>(a, grid, st, eval); break;
+            case 128: launch_sweep<128>(a, grid, st, eval); break;
+            default: launch_sweep<256>(a, grid, st, eval); break;
             }
         }
         hipLaunchKernelGGL(k_sweep_final, dim3((P + 255) / 256), dim3(256), 0, st, a, P, cs.d_xc.p,
