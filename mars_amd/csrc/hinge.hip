@@ -167,7 +167,6 @@ struct SweepArgs {
 };
 
 constexpr int kTR = 16;    // sorted rows per tile
-constexpr int kCand = 8;   // candidates per block
 
 /*
  * Lane layout of one candidate: G lanes, each holding J consecutive covariate
@@ -175,13 +174,13 @@ constexpr int kCand = 8;   // candidates per block
  * u = 2 the candidate's Bx column, u = 3 .. m+2 the orthonormal basis, the
  * rest zero padding up to PT = G J slots.
  */
-template <int PT>
+template <int PT, int NC>
 struct SweepShape {
     static constexpr int J = PT == 32 ? 4 : 8;          // slots per lane
     static constexpr int G = PT / J;                    // lanes per candidate
     static constexpr int CW = 64 / G;                   // candidates per wave
-    static constexpr int NW = kCand / CW;               // waves per block
-    static constexpr int RPI = PT < 64 ? 64 / PT : 1;   // tile rows per wave-wide load
+    static constexpr int NW = NC / CW;                  // waves per block
+    static constexpr int RPS = 64 * NW / PT;            // tile rows per block-wide load
 };
 
 /* v from the lane DPP control CTRL names (a permutation within each row of 16 lanes). */
@@ -207,7 +206,7 @@ __device__ __forceinline__ double group_sum(double v)
 }
 
 /*
- * One block = kCand candidates (parents c0.. of one X column) swept over the
+ * One block = NC candidates (parents c0.. of one X column) swept over the
  * sorted rows [r0, r1) of its chunk (marsalgo.cc:576-649).  Tiles of 16 sorted
  * rows are gathered into LDS: row-major [0 | y | 0 | Bo row | 0...] (PT
  * floats), plus b (f64) and Bx per candidate, and the deltas (d, d^2).  The
@@ -219,17 +218,18 @@ __device__ __forceinline__ double group_sum(double v)
  * sum is fy - w) and k - |f|^2 over the bx and basis slots, and the group
  * reduces both with DPP.
  */
-template <int PT, bool EVAL>
-__global__ __launch_bounds__(64 * SweepShape<PT>::NW) void k_sweep(SweepArgs a)
+template <int PT, int NC, bool EVAL>
+__global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepArgs a)
 {
-    using S = SweepShape<PT>;
-    constexpr int J = S::J, G = S::G, CW = S::CW, NW = S::NW, RPI = S::RPI;
-    constexpr int NB = kTR / RPI;                          // Bo-row loads per lane and tile
-    constexpr int NPC = 2 * kCand * kTR / (64 * NW);       // per-candidate loads per lane and tile
-    __shared__ double sbv[2][kTR][kCand];                  // b (f64)
+    using S = SweepShape<PT, NC>;
+    constexpr int J = S::J, G = S::G, CW = S::CW, NW = S::NW, RPS = S::RPS;
+    static_assert(kTR % RPS == 0, "tile rows");
+    constexpr int NB = kTR / RPS;                          // Bo-row loads per lane and tile
+    constexpr int NPC = 2 * NC * kTR / (64 * NW);          // per-candidate loads per lane and tile
+    __shared__ double sbv[2][kTR][NC];                     // b (f64)
     __shared__ double2 sdd[2][kTR];                        // (d, d*d)
     __shared__ __attribute__((aligned(16))) float sq[2][kTR][PT];
-    __shared__ float sbx[2][kTR][kCand];
+    __shared__ float sbx[2][kTR][NC];
     __shared__ int srow[2][kTR];
 
     const int m = a.m;
@@ -237,9 +237,9 @@ __global__ __launch_bounds__(64 * SweepShape<PT>::NW) void k_sweep(SweepArgs a)
     const int t = lane % G, grp = lane / G;
     const int chunk = blockIdx.x, pg = blockIdx.y, gi = blockIdx.z;
     const int xc = a.grp[3 * gi], c0 = a.grp[3 * gi + 1], np = a.grp[3 * gi + 2];
-    const int sbase = pg * kCand;                  // first parent slot of this block
+    const int sbase = pg * NC;                     // first parent slot of this block
     if (sbase >= np) return;                       // whole block idle (uniform)
-    const int nact = min(kCand, np - sbase);
+    const int nact = min(NC, np - sbase);
     const int slot = wv * CW + grp;                // this lane's candidate slot
     const bool active = slot < nact;
     const bool wave_active = wv * CW < nact;
@@ -252,8 +252,8 @@ __global__ __launch_bounds__(64 * SweepShape<PT>::NW) void k_sweep(SweepArgs a)
     auto tile_rows = [&](int tt) { return (int)min((size_t)kTR, r1 - (r0 + (size_t)tt * kTR)); };
 
     // ---- gather sources (fixed over tiles)
-    const int pos = PT < 64 ? (lane & (PT - 1)) : (wv * 64 + lane);   // tile column
-    const int ro = PT < 64 ? lane / PT : 0;                           // row within a load
+    const int pos = tid % PT;   // tile column
+    const int ro = tid / PT;    // row within a block-wide load
     const float *bp = nullptr;
     size_t bld = 0;
     if (pos == 1) {
@@ -270,10 +270,10 @@ __global__ __launch_bounds__(64 * SweepShape<PT>::NW) void k_sweep(SweepArgs a)
 #pragma unroll
     for (int i = 0; i < NPC; ++i) {
         const int e = tid + 64 * NW * i;
-        pisb[i] = e < kCand * kTR;
-        const int rem = e % (kCand * kTR);
-        prr[i] = rem / kCand;
-        pj[i] = rem % kCand;
+        pisb[i] = e < NC * kTR;
+        const int rem = e % (NC * kTR);
+        prr[i] = rem / NC;
+        pj[i] = rem % NC;
         pp[i] = nullptr;
         pld[i] = 0;
         if (pj[i] < nact) {
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(64 * SweepShape<PT>::NW) void k_sweep(SweepArgs a)
     size_t next = g0;  // next grid row at or after the current tile
     if (r0 > g0) next = g0 + ((r0 - g0 + a.span - 1) / a.span) * a.span;
     int lastnz = -1;   // last sorted row before the current tile with a nonzero delta
-    if (EVAL) {
+    if (EVAL && r0 < r1) {   // (chunks past the tail are empty: r0 may exceed n)
         const size_t lo = r0 > a.span ? r0 - a.span : 0;
         int mine = -1;
         for (size_t l = lo + lane; l < r0; l += 64)
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(64 * SweepShape<PT>::NW) void k_sweep(SweepArgs a)
         const int nr = tile_rows(0);
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
-            const int rr = i * RPI + ro;
+            const int rr = i * RPS + ro;
             const bool in = rr < nr && bp;
             sq[0][rr][pos] = in ? bp[(size_t)srow[0][rr] * bld] : 0.0f;
         }
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(64 * SweepShape<PT>::NW) void k_sweep(SweepArgs a)
         float bov[NB], pcv[NPC];
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
-            const int rr = i * RPI + ro;
+            const int rr = i * RPS + ro;
             const bool in = rr < nr1 && bp;
             bov[i] = in ? bp[(size_t)srow[nxt][rr] * bld] : 0.0f;
         }
@@ -449,7 +449,7 @@ __global__ __launch_bounds__(64 * SweepShape<PT>::NW) void k_sweep(SweepArgs a)
         // buffer nxt last held tile t-1, which every wave finished before the
         // previous iteration's barrier: store tile t+1 into it
 #pragma unroll
-        for (int i = 0; i < NB; ++i) sq[nxt][i * RPI + ro][pos] = bov[i];
+        for (int i = 0; i < NB; ++i) sq[nxt][i * RPS + ro][pos] = bov[i];
 #pragma unroll
         for (int i = 0; i < NPC; ++i) {
             if (pisb[i]) sbv[nxt][prr[i]][pj[i]] = (double)pcv[i];
@@ -535,15 +535,25 @@ __global__ void k_sweep_final(SweepArgs a, int P, const int *__restrict__ cxc,
     }
 }
 
-template <int PT>
-void launch_sweep(const SweepArgs &a, dim3 grid, hipStream_t st, bool eval)
+template <int PT, int NC>
+void launch_sweep_nc(const SweepArgs &a, dim3 grid, hipStream_t st, bool eval)
 {
-    const dim3 block(64 * SweepShape<PT>::NW);
+    const dim3 block(64 * SweepShape<PT, NC>::NW);
     if (eval)
-        hipLaunchKernelGGL((k_sweep<PT, true>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((k_sweep<PT, NC, true>), grid, block, 0, st, a);
     else
-        hipLaunchKernelGGL((k_sweep<PT, false>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((k_sweep<PT, NC, false>), grid, block, 0, st, a);
     MH_LAUNCH_CHECK();
+}
+
+template <int PT>
+void launch_sweep(const SweepArgs &a, dim3 grid, int nc, hipStream_t st, bool eval)
+{
+    switch (nc) {
+    case 8: launch_sweep_nc<PT, 8>(a, grid, st, eval); break;
+    case 16: launch_sweep_nc<PT, 16>(a, grid, st, eval); break;
+    default: launch_sweep_nc<PT, 32>(a, grid, st, eval); break;
+    }
 }
 
 } // namespace
@@ -630,11 +640,16 @@ void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span)
     // covariate slots per candidate: b, y, bx and the m basis columns, padded
     const int PT = m + 3 <= 32 ? 32 : (m + 3 <= 64 ? 64 : (m + 3 <= 128 ? 128 : 256));
     const int L2 = PT;
-    // about 4096 blocks (16 per CU), chunks of at least 256 sorted rows
-    const int pgroups = (maxnp + kCand - 1) / kCand;
-    const int want = std::max(1, (4096 + ng * pgroups - 1) / (ng * pgroups));
+    // candidates per block: one block takes all parents of an X column, up to 32
+    const int NC = maxnp <= 8 ? 8 : (maxnp <= 16 ? 16 : 32);
+    const int pgroups = (maxnp + NC - 1) / NC;
+    const int wpb = NC / (64 / (PT / (PT == 32 ? 4 : 8)));   // waves per block
+    // about 8192 waves, chunks of at least 256 sorted rows; a multiple of 8
+    // chunks puts the blocks of one (column, chunk) on one XCD (shared L2)
+    const int want = std::max(1, (8192 + ng * pgroups * wpb - 1) / (ng * pgroups * wpb));
     const int maxc = (int)std::max<size_t>(1, tail / 256);
-    const int C = std::min(want, maxc);
+    int C = std::min(want, maxc);
+    if (C >= 8) C -= C % 8;
     const size_t clen = (tail + C - 1) / C;
 
     DBuf<int> dgrp, dcgrp;
@@ -690,10 +705,10 @@ void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span)
                 MH_LAUNCH_CHECK();
             }
             switch (PT) {
-            case 32: launch_sweep<32>(a, grid, st, eval); break;
-            case 64: launch_sweep<64>(a, grid, st, eval); break;
-            case 128: launch_sweep<128>(a, grid, st, eval); break;
-            default: launch_sweep<256>(a, grid, st, eval); break;
+            case 32: launch_sweep<32>(a, grid, NC, st, eval); break;
+            case 64: launch_sweep<64>(a, grid, NC, st, eval); break;
+            case 128: launch_sweep<128>(a, grid, NC, st, eval); break;
+            default: launch_sweep<256>(a, grid, NC, st, eval); break;
             }
         }
         hipLaunchKernelGGL(k_sweep_final, dim3((P + 255) / 256), dim3(256), 0, st, a, P, cs.d_xc.p,
