@@ -244,32 +244,40 @@ int grid_w(size_t items)
 
 template <bool RETRY>
 void launch_tgemm(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, const Cands &cs,
-                  const int *list, int ldT)
+                  const int *list, int ldT, bool narrow)
 {
     if (nitems == 0) return;
     const dim3 grid(grid_w(nitems)), block(64);
-#define MH_GO(M)                                                                                      \
-    hipLaunchKernelGGL((k_tgemm_w<M, cg_tgemm(M), RETRY>), grid, block, 0, ctx->stream, (const FitDev *)ctx->d_fits.p, \
+#define MH_GO1(M, CG)                                                                                 \
+    hipLaunchKernelGGL((k_tgemm_w<M, CG, RETRY>), grid, block, 0, ctx->stream, (const FitDev *)ctx->d_fits.p, \
                        (const Item *)u.items, nitems, (const int *)cs.d_xc.p, (const int *)cs.d_par.p, list, ldT, \
                        ctx->tpart.p)
+#define MH_GO(M)                      \
+    if (narrow) MH_GO1(M, 1);         \
+    else MH_GO1(M, cg_tgemm(M))
     MH_MT_SWITCH(mt, MH_GO)
 #undef MH_GO
+#undef MH_GO1
     MH_LAUNCH_CHECK();
 }
 
 template <int MODE, bool RB, bool TCAND>
 void launch_project(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, const Cands &cs,
-                    const int *list, const double *T, int ldT)
+                    const int *list, const double *T, int ldT, bool narrow)
 {
     if (nitems == 0) return;
     const dim3 grid(grid_w(nitems)), block(64);
-#define MH_GO(M)                                                                                          \
-    hipLaunchKernelGGL((k_project_w<M, cg_project(M), MODE, RB, TCAND>), grid, block, 0, ctx->stream,     \
+#define MH_GO1(M, CG)                                                                                     \
+    hipLaunchKernelGGL((k_project_w<M, CG, MODE, RB, TCAND>), grid, block, 0, ctx->stream,                \
                        (const FitDev *)ctx->d_fits.p, (const Item *)u.items, nitems, (const int *)cs.d_xc.p, \
                        (const int *)cs.d_par.p, list, T, ldT, (const double *)ctx->ssum.p,                \
                        (const int *)ctx->flag.p, ctx->spart.p)
+#define MH_GO(M)                      \
+    if (narrow) MH_GO1(M, 1);         \
+    else MH_GO1(M, cg_project(M))
     MH_MT_SWITCH(mt, MH_GO)
 #undef MH_GO
+#undef MH_GO1
     MH_LAUNCH_CHECK();
 }
 
@@ -379,7 +387,19 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     const int mt = (int)std::min<size_t>(16, (mmax + 15) / 16);
     require(mmax <= 256, "the FP64 scoring path supports up to 256 basis functions");
     const int ldT = mt * 16;
-    const int tileT = 16 * cg_tgemm(mt), tileP = 16 * cg_project(mt);
+    // a few candidates per fit (the mixed form's rescoring, DGKS retries) run on
+    // 16-wide candidate tiles: the wide tiles would be mostly zero padding in
+    // the FP64 MFMA work that bounds these kernels
+    auto narrow_for = [&](const std::vector<int> &c) {
+        long long tot = 0, act = 0;
+        for (int v : c) {
+            tot += v;
+            act += v > 0;
+        }
+        return tot <= 24 * act;
+    };
+    const bool narrow = narrow_for(cnt);
+    const int tileT = 16 * (narrow ? 1 : cg_tgemm(mt)), tileP = 16 * (narrow ? 1 : cg_project(mt));
 
     Pass pT = make_pass(fb, fe, n, split_rows(n, cnt, tileT, 8192, 256), tileT, kWR);
     Pass pP = make_pass(fb, fe, n, split_rows(n, cnt, tileP, 8192, 256), tileP, kWR);
@@ -408,7 +428,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     // Pass 1: T = Bo^T Bx
     {
         KTimer kt(ctx, K_TGEMM, flops, bytes);
-        launch_tgemm<false>(ctx, mt, uT, (int)pT.items.size(), cs, nullptr, ldT);
+        launch_tgemm<false>(ctx, mt, uT, (int)pT.items.size(), cs, nullptr, ldT, narrow);
         hipLaunchKernelGGL(k_treduce, dim3(P), dim3(64), 0, st, ctx->d_fits.p, cs.d_fit.p, nullptr, P,
                            uT.cmap, ctx->tpart.p, ldT, ctx->T.p, ctx->tnorm.p);
         MH_LAUNCH_CHECK();
@@ -418,9 +438,11 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     {
         KTimer kt(ctx, K_PROJECT, flops, bytes + 0.0);
         if (keep_bxn)
-            launch_project<kStoreR, false, false>(ctx, mt, uP, (int)pP.items.size(), cs, nullptr, ctx->T.p, ldT);
+            launch_project<kStoreR, false, false>(ctx, mt, uP, (int)pP.items.size(), cs, nullptr, ctx->T.p, ldT,
+                                                  narrow);
         else
-            launch_project<kSumSq, false, false>(ctx, mt, uP, (int)pP.items.size(), cs, nullptr, ctx->T.p, ldT);
+            launch_project<kSumSq, false, false>(ctx, mt, uP, (int)pP.items.size(), cs, nullptr, ctx->T.p, ldT,
+                                                 narrow);
         hipLaunchKernelGGL(k_sreduce, dim3((P + 255) / 256), dim3(256), 0, st, nullptr, P, uP.cmap,
                            ctx->spart.p, kSlots, ctx->tnorm.p, ctx->d_fits.p, cs.d_fit.p,
                            ctx->ssum.p, ctx->flag.p);
@@ -449,8 +471,10 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         MH_CHECK(hipMemcpyAsync(ctx->fired_list.p, fired.data(), nf * sizeof(int), hipMemcpyHostToDevice, st));
         std::vector<int> fcnt(F);
         for (size_t f = 0; f < F; ++f) fcnt[f] = re[f] - rb[f];
-        Pass rT = make_pass(rb, re, n, split_rows(n, fcnt, tileT, 4096, 256), tileT, kWR);
-        Pass rP = make_pass(rb, re, n, split_rows(n, fcnt, tileP, 4096, 256), tileP, kWR);
+        const bool rnarrow = narrow_for(fcnt);
+        const int rtT = 16 * (rnarrow ? 1 : cg_tgemm(mt)), rtP = 16 * (rnarrow ? 1 : cg_project(mt));
+        Pass rT = make_pass(rb, re, n, split_rows(n, fcnt, rtT, 4096, 256), rtT, kWR);
+        Pass rP = make_pass(rb, re, n, split_rows(n, fcnt, rtP, 4096, 256), rtP, kWR);
         Uploaded vT, vP;
         ctx->d_ritems.ensure(rT.items.size() + rP.items.size() + 1);
         ctx->d_rcmap.ensure(rT.cmap.size() + rP.cmap.size() + 1);
@@ -461,13 +485,13 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         ctx->T2.ensure((size_t)nf * ldT);
         if (!keep_bxn)  // the first residual of the fired columns, for the retry
             launch_project<kStoreR, false, true>(ctx, mt, vP, (int)rP.items.size(), cs, ctx->fired_list.p,
-                                                 ctx->T.p, ldT);
-        launch_tgemm<true>(ctx, mt, vT, (int)rT.items.size(), cs, ctx->fired_list.p, ldT);
+                                                 ctx->T.p, ldT, rnarrow);
+        launch_tgemm<true>(ctx, mt, vT, (int)rT.items.size(), cs, ctx->fired_list.p, ldT, rnarrow);
         hipLaunchKernelGGL(k_treduce, dim3(nf), dim3(64), 0, st, ctx->d_fits.p, cs.d_fit.p,
                            ctx->fired_list.p, nf, vT.cmap, ctx->tpart.p, ldT, ctx->T2.p, nullptr);
         MH_LAUNCH_CHECK();
         launch_project<kStoreR, true, false>(ctx, mt, vP, (int)rP.items.size(), cs, ctx->fired_list.p, ctx->T2.p,
-                                             ldT);
+                                             ldT, rnarrow);
         hipLaunchKernelGGL(k_sreduce, dim3((nf + 255) / 256), dim3(256), 0, st, ctx->fired_list.p, nf,
                            vP.cmap, ctx->spart.p, kSlots, nullptr, ctx->d_fits.p, cs.d_fit.p,
                            ctx->ssum.p, nullptr);
@@ -492,7 +516,9 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
             if (nl > 0) {
                 std::vector<int> lcnt(F);
                 for (size_t f = 0; f < F; ++f) lcnt[f] = le[f] - lb[f];
-                Pass pL = make_pass(lb, le, n, split_rows(n, lcnt, tileP, 8192, 256), tileP, kWR);
+                const bool lnarrow = narrow_for(lcnt);
+                const int ltP = 16 * (lnarrow ? 1 : cg_project(mt));
+                Pass pL = make_pass(lb, le, n, split_rows(n, lcnt, ltP, 8192, 256), ltP, kWR);
                 Uploaded uL;
                 ctx->live_list.ensure(nl);
                 ctx->d_litems.ensure(pL.items.size() + 1);
@@ -501,7 +527,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                 upload_pass(ctx, pL, ctx->d_litems, ctx->d_lcmap, 0, 0, uL);
                 ctx->spart.ensure(pL.items.size() * kSlots + 1);
                 launch_project<kFinalY, false, true>(ctx, mt, uL, (int)pL.items.size(), cs, ctx->live_list.p,
-                                                     ctx->T.p, ldT);
+                                                     ctx->T.p, ldT, lnarrow);
                 hipLaunchKernelGGL(k_yreduce, dim3((nl + 255) / 256), dim3(256), 0, st, nl, uL.cmap, ctx->spart.p,
                                    ctx->live_list.p, ybx, lin);
             }
