@@ -43,19 +43,24 @@ constexpr int kGroupInts = 7;      // fit, xc0, nc, j, a-tile, first item, item 
  *   U[a][xc] = sum_i A[i][a] xs[i][xc],   a = 16u + (0..15) < 2j+1,
  *   A[i][a] = f32(Bo[i][a] B[i][j])          (a <= j:  k = a,  parent j)
  *           = f32(Bo[i][j] B[i][a-j-1])      (a >  j:  k = j,  parent a-j-1)
- * and, in the u == 0 items, of |Bx|^2 and Bx.y for Bx = f32(B[:,j] xs[:,xc]).
+ * and, in the u == 0 items (NORMS), of |Bx|^2 and Bx.y for Bx = f32(B[:,j] xs[:,xc]).
  * One wave per item, no LDS: MFMA step q of a 4-row half-step at row r takes
  * row r + 8 (lane/16) + q, so each lane's operands are float4 runs of the
  * column-major BoC, BC and X (zero-padded to a multiple of 32 rows).
  * v_mfma_f32_16x16x4f32 accumulates each 32-row chunk in FP32; the chunk sums
- * are folded into FP64.
+ * are folded into FP64.  The loads of chunk c+1 are issued before the MFMAs
+ * of chunk c (two register sets, alternating).
  */
-__global__ __launch_bounds__(64, 3) void k_tc_update(const FitDev *__restrict__ fits,
+struct TcChunk {
+    float4 o[2], bb[2], bj[2], yv[2], xv[2][4];
+};
+
+template <bool NORMS>
+__global__ __launch_bounds__(64, 2) void k_tc_update(const FitDev *__restrict__ fits,
                                                      const Item *__restrict__ items, int nitems,
                                                      double *__restrict__ tpart, double *__restrict__ npart)
 {
     const int lane = threadIdx.x, kr = lane >> 4, kc = lane & 15;
-    const float4 z4 = {0.f, 0.f, 0.f, 0.f};
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const Item I = items[it];
         const FitDev F = fits[I.fit];
@@ -78,7 +83,6 @@ __global__ __launch_bounds__(64, 3) void k_tc_update(const FitDev *__restrict__ 
             px[g] = F.X + (size_t)xc * F.ldx;
             sx[g] = xok[g] ? F.s[xc] : 0.0f;
         }
-        const bool norms = u == 0;
         v4d acc[4];
         double bxx[4], bxy[4];
 #pragma unroll
@@ -87,44 +91,44 @@ __global__ __launch_bounds__(64, 3) void k_tc_update(const FitDev *__restrict__ 
             bxx[g] = 0.0;
             bxy[g] = 0.0;
         }
-#pragma unroll 1
-        for (long long r = I.r0; r < I.r1; r += kChunk) {
-            // every load of the chunk first (clamped addresses, masked by selects below), so they
-            // are all in flight together
-            float4 o[2], bb[2], bj[2], yv[2], xv[2][4];
+        // every load of a chunk (no branches: a branch join would make the compiler wait for them)
+        auto load = [&](TcChunk &c, long long r) {
 #pragma unroll
             for (int hf = 0; hf < 2; ++hf) {
                 const long long rb = r + 8 * kr + 4 * hf;
-                // no branches here: a branch join would make the compiler wait for the loads
-                o[hf] = gld4(pbo + rb);
-                bb[hf] = gld4(pb + rb);
-                bj[hf] = gld4(pbj + rb);
-                yv[hf] = gld4(F.y + rb);
+                c.o[hf] = gld4(pbo + rb);
+                c.bb[hf] = gld4(pb + rb);
+                if (NORMS) {
+                    c.bj[hf] = gld4(pbj + rb);
+                    c.yv[hf] = gld4(F.y + rb);
+                }
 #pragma unroll
-                for (int g = 0; g < 4; ++g) xv[hf][g] = gld4(px[g] + rb);
+                for (int g = 0; g < 4; ++g) c.xv[hf][g] = gld4(px[g] + rb);
             }
+        };
+        auto compute = [&](const TcChunk &c) {
             v4f part[4];
 #pragma unroll
             for (int g = 0; g < 4; ++g) part[g] = v4f{0, 0, 0, 0};
 #pragma unroll
             for (int hf = 0; hf < 2; ++hf) {
-                const float ov[4] = {o[hf].x, o[hf].y, o[hf].z, o[hf].w};
-                const float bv[4] = {bb[hf].x, bb[hf].y, bb[hf].z, bb[hf].w};
-                const float jv[4] = {bj[hf].x, bj[hf].y, bj[hf].z, bj[hf].w};
-                const float yy[4] = {yv[hf].x, yv[hf].y, yv[hf].z, yv[hf].w};
+                const float ov[4] = {c.o[hf].x, c.o[hf].y, c.o[hf].z, c.o[hf].w};
+                const float bv[4] = {c.bb[hf].x, c.bb[hf].y, c.bb[hf].z, c.bb[hf].w};
                 float av[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) av[q] = aok ? __fmul_rn(ov[q], bv[q]) : 0.0f;
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
                     if (g >= G) continue;
-                    const float xx[4] = {xv[hf][g].x, xv[hf][g].y, xv[hf][g].z, xv[hf][g].w};
+                    const float xx[4] = {c.xv[hf][g].x, c.xv[hf][g].y, c.xv[hf][g].z, c.xv[hf][g].w};
                     float xs[4];
 #pragma unroll
                     for (int q = 0; q < 4; ++q) xs[q] = xok[g] ? __fmul_rn(xx[q], sx[g]) : 0.0f;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) part[g] = mfma_f32(av[q], xs[q], part[g]);
-                    if (norms) {
+                    if (NORMS) {
+                        const float jv[4] = {c.bj[hf].x, c.bj[hf].y, c.bj[hf].z, c.bj[hf].w};
+                        const float yy[4] = {c.yv[hf].x, c.yv[hf].y, c.yv[hf].z, c.yv[hf].w};
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
                             const double bx = (double)__fmul_rn(jv[q], xs[q]);
@@ -138,6 +142,21 @@ __global__ __launch_bounds__(64, 3) void k_tc_update(const FitDev *__restrict__ 
             for (int g = 0; g < 4; ++g)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) acc[g][v] += (double)part[g][v];
+        };
+        // chunks alternate between two register sets; a missing next chunk reloads the current
+        // one (in bounds, unused)
+        TcChunk ca, cb;
+        long long r = I.r0;
+        if (r < I.r1) load(ca, r);
+#pragma unroll 1
+        while (r < I.r1) {
+            load(cb, r + kChunk < I.r1 ? r + kChunk : r);
+            compute(ca);
+            r += kChunk;
+            if (r >= I.r1) break;
+            load(ca, r + kChunk < I.r1 ? r + kChunk : r);
+            compute(cb);
+            r += kChunk;
         }
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
@@ -148,7 +167,7 @@ __global__ __launch_bounds__(64, 3) void k_tc_update(const FitDev *__restrict__ 
             if (slot < I.nc)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) dst[v] = acc[g][v];
-            if (norms) {
+            if (NORMS) {
                 double s0 = bxx[g], s1 = bxy[g];
                 s0 = s0 + __shfl_xor(s0, 16, 64);
                 s0 = s0 + __shfl_xor(s0, 32, 64);
@@ -326,25 +345,30 @@ void tc_update(mars_hip_ctx *ctx)
             bytes += 4.0 * fs.n * (2.0 * (fs.mc + 1) + fs.px + 1);
         }
         if (work == 0) break;
+        // the A tile 0 items (which also form |Bx|^2 and Bx.y) first, then the others
         std::vector<Item> items;
         std::vector<int> groups;
-        for (size_t f = 0; f < F; ++f) {
-            if (jf[f] < 0) continue;
-            const FitState &fs = ctx->fits[f];
-            const long long n = (long long)fs.n;
-            const int tiles = (2 * jf[f] + 1 + 15) / 16;
-            long long splits = std::max<long long>(1, std::llround(8192.0 * n / work));
-            splits = std::min<long long>(splits, std::max<long long>(1, n / 256));
-            const long long rows = ((n + splits - 1) / splits + kChunk - 1) / kChunk * kChunk;
-            for (int u = 0; u < tiles; ++u)
-                for (size_t xc0 = 0; xc0 < fs.px; xc0 += kXT) {
-                    const int nc = (int)std::min<size_t>(kXT, fs.px - xc0);
-                    const int first = (int)items.size();
-                    for (long long r0 = 0; r0 < n; r0 += rows)
-                        items.push_back(Item{(int)f, (int)xc0, nc, (jf[f] << 8) | u, r0, std::min(n, r0 + rows)});
-                    const int g[kGroupInts] = {(int)f, (int)xc0, nc, jf[f], u, first, (int)items.size() - first};
-                    groups.insert(groups.end(), g, g + kGroupInts);
-                }
+        size_t n0 = 0;
+        for (int pass = 0; pass < 2; ++pass) {
+            for (size_t f = 0; f < F; ++f) {
+                if (jf[f] < 0) continue;
+                const FitState &fs = ctx->fits[f];
+                const long long n = (long long)fs.n;
+                const int tiles = (2 * jf[f] + 1 + 15) / 16;
+                long long splits = std::max<long long>(1, std::llround(8192.0 * n / work));
+                splits = std::min<long long>(splits, std::max<long long>(1, n / 256));
+                const long long rows = ((n + splits - 1) / splits + kChunk - 1) / kChunk * kChunk;
+                for (int u = pass == 0 ? 0 : 1; u < (pass == 0 ? 1 : tiles); ++u)
+                    for (size_t xc0 = 0; xc0 < fs.px; xc0 += kXT) {
+                        const int nc = (int)std::min<size_t>(kXT, fs.px - xc0);
+                        const int first = (int)items.size();
+                        for (long long r0 = 0; r0 < n; r0 += rows)
+                            items.push_back(Item{(int)f, (int)xc0, nc, (jf[f] << 8) | u, r0, std::min(n, r0 + rows)});
+                        const int g[kGroupInts] = {(int)f, (int)xc0, nc, jf[f], u, first, (int)items.size() - first};
+                        groups.insert(groups.end(), g, g + kGroupInts);
+                    }
+            }
+            if (pass == 0) n0 = items.size();
         }
         const size_t ni = items.size(), ng = groups.size() / kGroupInts;
         ctx->d_items.ensure(ni);
@@ -356,10 +380,19 @@ void tc_update(mars_hip_ctx *ctx)
         ctx->aux.ensure(ni * kXT * 2);
         {
             KTimer kt(ctx, K_MIXED_A, flops, bytes);  // the update kernel alone
-            const dim3 grid((unsigned)std::max<size_t>(1, std::min<size_t>(ni, 65536)));
-            hipLaunchKernelGGL(k_tc_update, grid, dim3(64), 0, st, (const FitDev *)ctx->d_fits.p,
-                               (const Item *)ctx->d_items.p, (int)ni, ctx->tpart.p, ctx->aux.p);
-            MH_LAUNCH_CHECK();
+            const Item *di = (const Item *)ctx->d_items.p;
+            if (n0 > 0) {
+                const dim3 grid((unsigned)std::min<size_t>(n0, 65536));
+                hipLaunchKernelGGL(k_tc_update<true>, grid, dim3(64), 0, st, (const FitDev *)ctx->d_fits.p, di,
+                                   (int)n0, ctx->tpart.p, ctx->aux.p);
+                MH_LAUNCH_CHECK();
+            }
+            if (ni > n0) {
+                const dim3 grid((unsigned)std::min<size_t>(ni - n0, 65536));
+                hipLaunchKernelGGL(k_tc_update<false>, grid, dim3(64), 0, st, (const FitDev *)ctx->d_fits.p,
+                                   di + n0, (int)(ni - n0), ctx->tpart.p + n0 * kXT * 16, ctx->aux.p + n0 * kXT * 2);
+                MH_LAUNCH_CHECK();
+            }
         }
         hipLaunchKernelGGL(k_tc_reduce, dim3((unsigned)ng), dim3(256), 0, st, (const FitDev *)ctx->d_fits.p,
                            (const int *)ctx->d_cmap.p, (const double *)ctx->tpart.p, (const double *)ctx->aux.p);
