@@ -194,6 +194,28 @@ typedef struct mars_hip_kstat {
 int mars_hip_timing(mars_hip_ctx *ctx, int enable);
 int mars_hip_kernel_stats(mars_hip_ctx *ctx, mars_hip_kstat *out, int capacity, int *count);
 
+/* ---------------------------------------------------------------------- */
+/* After the forward pass (stateless; host pointers, synchronous).         */
+/* ---------------------------------------------------------------------- */
+
+/* Basis matrix of a model: replaces mars.expand (mars.py:423-450).  X (n, p_x)
+ * and B (n, M) column-major; term i has type[i] in 'i' (B_i = 1), 'l'
+ * (B_basis * X_input), '+' (B_basis * max(X_input - hinge, 0)), '-'
+ * (B_basis * max(hinge - X_input, 0)), with basis[i] < i for every non-'i'
+ * term.  '+' / '-' form the hinge and its product in f64 and round once. */
+int mars_hip_expand(const float *X, size_t ldX, size_t n, size_t p_x, size_t M, const char *type, const int *basis,
+                    const int *input, const double *hinge, float *B, size_t ldB);
+int mars_hip_expand_f64(const double *X, size_t ldX, size_t n, size_t p_x, size_t M, const char *type,
+                        const int *basis, const int *input, const double *hinge, double *B, size_t ldB);
+
+/* out (ma, mb) column-major = A^T B with f64 accumulation: replaces mars.gram
+ * (mars.py:456-490).  A (n, ma), B (n, mb) column-major; B == NULL gives
+ * A^T A (exactly symmetric).  f32 inputs: exact f64 products. */
+int mars_hip_gram(const float *A, size_t ldA, size_t n, size_t ma, const float *B, size_t ldB, size_t mb,
+                  double *out, size_t ldOut);
+int mars_hip_gram_f64(const double *A, size_t ldA, size_t n, size_t ma, const double *B, size_t ldB, size_t mb,
+                      double *out, size_t ldOut);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,7 +10,8 @@ include/mars_hip.h (libmars_hip.so).
 """
 
 from . import marslib  # noqa: F401  (raises ImportError if the native parts are missing)
-from .mars import MODEL_DTYPE, expand, fit, fit_batch  # noqa: F401
+from .mars import MODEL_DTYPE, compact, expand, fit, fit_batch, gram, pprint, prune  # noqa: F401
 
 __version__ = marslib.__version__
-__all__ = ["fit", "fit_batch", "expand", "marslib", "MODEL_DTYPE", "__version__"]
+__all__ = ["fit", "fit_batch", "expand", "gram", "prune", "compact", "pprint", "marslib", "MODEL_DTYPE",
+           "__version__"]

@@ -14,7 +14,7 @@ import sysconfig
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-HIP_SOURCES = ["ortho.hip", "hinge.hip", "mixed.hip", "fit.hip", "batch.hip"]
+HIP_SOURCES = ["ortho.hip", "hinge.hip", "mixed.hip", "fit.hip", "batch.hip", "post.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MARS_HIP_ARCH", "gfx950")
 

@@ -15,25 +15,6 @@
 namespace mh {
 namespace {
 
-template <typename F>
-int guarded_b(F &&f)
-{
-    try {
-        f();
-        g_last_error.clear();
-        return MARS_HIP_OK;
-    } catch (const Error &e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const std::exception &e) {
-        g_last_error = e.what();
-        return MARS_HIP_EDEVICE;
-    } catch (...) {
-        g_last_error = "unknown error";
-        return MARS_HIP_EDEVICE;
-    }
-}
-
 struct FitDriver {
     size_t max_terms = 0;            // after the dof clamp (mars.py:241-243)
     std::vector<std::vector<int>> vars;  // variables of each basis function
@@ -196,7 +177,7 @@ int mars_hip_batch_create(int nfits, const float *const *X, const size_t *ldX, c
                           const float *const *w, const size_t *n, const size_t *p_x, size_t max_terms, int form,
                           mars_hip_ctx **out)
 {
-    return guarded_b([&] {
+    return guarded([&] {
         require(out != nullptr, "null output pointer");
         require(nfits >= 1, "need at least one fit");
         require(X && ldX && y && n && p_x, "null argument");
@@ -229,7 +210,7 @@ int mars_hip_batch_create(int nfits, const float *const *X, const size_t *ldX, c
 int mars_hip_batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts *opts, mars_hip_term *terms, int *nterms,
                        int *steps)
 {
-    return guarded_b([&] {
+    return guarded([&] {
         require(ctx && opts && terms && nterms, "null argument");
         require(opts->max_degree >= 1, "max_degree must be >= 1");
         batch_fit(ctx, *opts, terms, nterms, steps);

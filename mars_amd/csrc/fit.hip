@@ -920,25 +920,6 @@ void mirror_orthonormalize_batch(mars_hip_ctx *ctx, size_t m, size_t P, const in
     if (fired) *fired += ctx->dgks_host - d0;
 }
 
-template <typename F>
-int guarded(F &&f)
-{
-    try {
-        f();
-        g_last_error.clear();
-        return MARS_HIP_OK;
-    } catch (const Error &e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const std::exception &e) {
-        g_last_error = e.what();
-        return MARS_HIP_EDEVICE;
-    } catch (...) {
-        g_last_error = "unknown error";
-        return MARS_HIP_EDEVICE;
-    }
-}
-
 } // namespace mh
 
 /* ---------------------------------------------------------------------- */

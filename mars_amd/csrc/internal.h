@@ -58,4 +58,24 @@ void argmax_fits(mars_hip_ctx *ctx, const Cands &cs, const double *lin, std::vec
 double host_dsse(const FitState &fs);
 extern thread_local std::string g_last_error;
 
+/* Run f, turning exceptions into a status code and the thread's last error text. */
+template <typename F>
+int guarded(F &&f)
+{
+    try {
+        f();
+        g_last_error.clear();
+        return MARS_HIP_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return MARS_HIP_EDEVICE;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return MARS_HIP_EDEVICE;
+    }
+}
+
 } // namespace mh

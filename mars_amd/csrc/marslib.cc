@@ -14,6 +14,8 @@
 #include <cmath>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <vector>
 
 #include "mars_hip.h"
 
@@ -311,6 +313,69 @@ private:
     py::list keep_;
 };
 
+/* mars.expand (mars.py:423-450) on the device.  X (n, p_x) f32 or f64; the
+ * model as type codes ('i', 'l', '+', '-'), parent, input and knot arrays. */
+template <typename T>
+py::array_t<T> expand_impl(py::array_t<T, py::array::f_style | py::array::forcecast> X, const std::string &types,
+                           py::array_t<int, py::array::c_style | py::array::forcecast> basis,
+                           py::array_t<int, py::array::c_style | py::array::forcecast> input,
+                           py::array_t<double, py::array::c_style | py::array::forcecast> hinge)
+{
+    py::buffer_info xi = X.request();
+    verify(xi.ndim == 2, "expected 2D array for X");
+    const size_t n = (size_t)xi.shape[0], px = (size_t)xi.shape[1], M = types.size();
+    verify((size_t)basis.size() == M && (size_t)input.size() == M && (size_t)hinge.size() == M,
+           "model arrays differ in length");
+    py::array_t<T, py::array::f_style> B(std::vector<py::ssize_t>{(py::ssize_t)n, (py::ssize_t)M});
+    const T *xp = (const T *)xi.ptr;
+    T *bp = B.mutable_data();
+    int rc;
+    {
+        py::gil_scoped_release nogil;
+        if constexpr (std::is_same<T, float>::value)
+            rc = mars_hip_expand(xp, n, n, px, M, types.data(), basis.data(), input.data(), hinge.data(), bp, n);
+        else
+            rc = mars_hip_expand_f64(xp, n, n, px, M, types.data(), basis.data(), input.data(), hinge.data(), bp, n);
+    }
+    check(rc);
+    return B;
+}
+
+/* mars.gram (mars.py:456-490) on the device: A^T B (A^T A when B is None), f64 result. */
+template <typename T>
+py::array_t<double> gram_impl(py::array_t<T, py::array::f_style | py::array::forcecast> A, py::object Bobj)
+{
+    py::buffer_info ai = A.request();
+    verify(ai.ndim == 2, "expected 2D array for A");
+    const size_t n = (size_t)ai.shape[0], ma = (size_t)ai.shape[1];
+    py::array_t<T, py::array::f_style | py::array::forcecast> B;
+    size_t mb = ma;
+    const T *bp = nullptr;
+    if (!Bobj.is_none()) {
+        B = py::array_t<T, py::array::f_style | py::array::forcecast>::ensure(Bobj);
+        verify(B.ptr() != nullptr, "B must be a numeric array");
+        py::buffer_info bi = B.request();
+        verify(bi.ndim == 2, "expected 2D array for B");
+        verify((size_t)bi.shape[0] == n, "shapes are not aligned");
+        mb = (size_t)bi.shape[1];
+        bp = (const T *)bi.ptr;
+    }
+    py::array_t<double, py::array::f_style> out(std::vector<py::ssize_t>{(py::ssize_t)ma, (py::ssize_t)mb});
+    if (ma == 0 || mb == 0) return out;
+    const T *ap = (const T *)ai.ptr;
+    double *op = out.mutable_data();
+    int rc;
+    {
+        py::gil_scoped_release nogil;
+        if constexpr (std::is_same<T, float>::value)
+            rc = mars_hip_gram(ap, n, n, ma, bp, n, mb, op, ma);
+        else
+            rc = mars_hip_gram_f64(ap, n, n, ma, bp, n, mb, op, ma);
+    }
+    check(rc);
+    return out;
+}
+
 } // namespace
 
 PYBIND11_MODULE(_marslib, m)
@@ -354,4 +419,10 @@ PYBIND11_MODULE(_marslib, m)
         .def("set_timing", &Batch::set_timing)
         .def("kernel_stats", &Batch::kernel_stats)
         .def("stats", &Batch::stats);
+    m.def("expand_f32", &expand_impl<float>, py::arg("X"), py::arg("types"), py::arg("basis"), py::arg("input"),
+          py::arg("hinge"));
+    m.def("expand_f64", &expand_impl<double>, py::arg("X"), py::arg("types"), py::arg("basis"), py::arg("input"),
+          py::arg("hinge"));
+    m.def("gram_f32", &gram_impl<float>, py::arg("A"), py::arg("B") = py::none());
+    m.def("gram_f64", &gram_impl<double>, py::arg("A"), py::arg("B") = py::none());
 }

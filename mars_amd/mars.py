@@ -231,23 +231,247 @@ def fit(X, y, w=None, **kwargs):
 
 
 def expand(X, model):
-    """Rebuild the basis matrix of a model (reference mars.py:423-450); test helper."""
+    """
+    Expand a feature set X into the bases of a MARS model (reference
+    mars.py:423-450), on the device (mars_hip_expand).  X is float32 or
+    float64; B has X's dtype and Fortran order.  Hinge terms form
+    max(x - h, 0) and its product in float64 and round once to X's dtype, as
+    numpy does for an f32 array against the model's f64 knot.
+    """
     X = np.asarray(X)
-    B = np.empty((len(X), len(model)), dtype=X.dtype, order="F")
+    if X.ndim != 2:
+        raise ValueError("expand: X must be 2-D")
+    if X.dtype == np.float32:
+        kernel = marslib.expand_f32
+    elif X.dtype == np.float64:
+        kernel = marslib.expand_f64
+    else:
+        raise TypeError("expand: X must be float32 or float64, got %s" % X.dtype)
+    M = len(model)
+    codes = bytearray(M)
+    parent = np.zeros(M, dtype=np.int32)
+    column = np.zeros(M, dtype=np.int32)
+    knot = np.zeros(M, dtype=np.float64)
     for i, node in enumerate(model):
-        t = node["type"].decode() if isinstance(node["type"], bytes) else node["type"]
-        b, x, h = node["basis"], node["input"], node["hinge"]
-        if t == "i":
-            B[:, i] = 1.0
-        elif t == "l":
-            B[:, i] = B[:, b] * X[:, x]
-        elif t == "+":
-            B[:, i] = B[:, b] * np.maximum(X[:, x] - h, 0)
-        elif t == "-":
-            B[:, i] = B[:, b] * np.maximum(h - X[:, x], 0)
+        t = node["type"]
+        t = t.decode("utf8") if isinstance(t, bytes) else str(t)
+        if t != "i":
+            assert 0 <= node["input"] < X.shape[1]
+            assert 0 <= node["basis"] < i
+            assert t in ("l", "+", "-")
+        codes[i] = ord(t)
+        parent[i] = node["basis"]
+        column[i] = node["input"]
+        knot[i] = node["hinge"]
+    return kernel(X, bytes(codes), parent, column, knot)
+
+
+def gram(A, B=None, chunk_size=128):
+    """
+    A.T @ B (A.T @ A when B is None) with float64 accumulation (reference
+    mars.py:456-490), on the device (mars_hip_gram).  float32 inputs are
+    multiplied exactly in float64; `chunk_size` is accepted for compatibility
+    and has no effect (the device sums in float64 throughout).
+
+    1-D inputs are column vectors and the result is squeezed like numpy's
+    `@`: a float if both are 1-D, 1-D if exactly one is, otherwise 2-D.
+    """
+    del chunk_size
+    same = B is None
+    A = np.asarray(A)
+    B = A if same else np.asarray(B)
+    if len(A) != len(B):
+        raise ValueError("shapes are not aligned: %s x %s" % (A.shape, B.shape))
+    A2 = A[:, np.newaxis] if A.ndim == 1 else A
+    B2 = B[:, np.newaxis] if B.ndim == 1 else B
+    if A2.dtype == np.float32 and B2.dtype == np.float32:
+        out = marslib.gram_f32(A2, None if same else B2)
+    else:
+        A2 = A2.astype(np.float64, copy=False)
+        B2 = B2.astype(np.float64, copy=False)
+        out = marslib.gram_f64(A2, None if same else B2)
+    if A.ndim == 1 and B.ndim == 1:
+        return float(out[0, 0])
+    if A.ndim == 1:
+        return out[0]
+    if B.ndim == 1:
+        return out[:, 0]
+    return out
+
+
+def _init_inverse_state(XX, XY, YY, active_idx, ridge=0.0):
+    """
+    (H, beta, SSE) of the active columns from a fresh Cholesky factor
+    (reference mars.py:493-534): H = (XX_a + ridge I)^-1 explicitly (the
+    backward pass downdates it), beta = H XY_a, SSE = max(YY - XY_a.beta, 0).
+    A factorization failure retries once with the diagonal loaded by
+    1e-8 (trace/m + 1).
+    """
+    import scipy.linalg
+
+    m = len(active_idx)
+    if m == 0:
+        return np.zeros((0, 0)), np.zeros(0), float(YY)
+    G = XX[np.ix_(active_idx, active_idx)]
+    b = XY[active_idx]
+    if ridge > 0:
+        G = G + ridge * np.eye(m)
+    try:
+        L = np.linalg.cholesky(G)
+    except np.linalg.LinAlgError:
+        L = np.linalg.cholesky(G + 1e-8 * (np.trace(G) / m + 1.0) * np.eye(m))
+    factor = (L, True)
+    beta = scipy.linalg.cho_solve(factor, b)
+    H = scipy.linalg.cho_solve(factor, np.eye(m))
+    return H, beta, max(float(YY - b @ beta), 0.0)
+
+
+def _backward_eliminate(XX, XY, YY, mask, penalty, n_true, ridge=0.0):
+    """
+    Backward stepwise elimination by rank-1 Schur downdates (reference
+    mars.py:537-604): each step drops the active column with the smallest
+    beta_j^2 / H_jj (the intercept, global column 0, is never dropped), adds
+    that to the SSE and downdates H and beta; every max(64, M/4) steps H is
+    rebuilt from scratch.  Returns the mask of the lowest-GCV subset seen
+    (strictly lower replaces, so the larger subset wins a tie).
+    """
+    M = len(XX)
+    idx = np.flatnonzero(mask)
+    m = len(idx)
+    if m == 0:
+        return mask.copy()
+    H, beta, SSE = _init_inverse_state(XX, XY, YY, idx, ridge)
+
+    def gcv(sse, k):
+        denom = 1.0 - (k + penalty * (k - 1)) / n_true
+        return sse / (denom * denom) if denom > 0 else np.inf
+
+    best = gcv(SSE, m)
+    best_mask = mask.copy()
+    period = max(64, M // 4)
+    steps = 0
+    while m > 0:
+        h = np.diag(H)
+        floor = 1e-12 * max(float(h.max()), 1.0)
+        cost = np.full_like(beta, np.inf)
+        np.divide(beta * beta, h, out=cost, where=h > floor)
+        if idx[0] == 0:
+            cost[0] = np.inf
+        j = int(np.argmin(cost))
+        if not np.isfinite(cost[j]):
+            break
+        SSE += max(float(cost[j]), 0.0)
+        col = H[:, j]
+        v = col / H[j, j]
+        beta = beta - beta[j] * v
+        H = H - np.outer(v, col)
+        H = 0.5 * (H + H.T)
+        np.fill_diagonal(H, np.maximum(np.diag(H), 0.0))
+        keep = np.arange(m) != j
+        beta = beta[keep]
+        H = H[np.ix_(keep, keep)]
+        mask[idx[j]] = False
+        idx = idx[keep]
+        m -= 1
+        steps += 1
+        if steps >= period and m > 0:
+            H, beta, SSE = _init_inverse_state(XX, XY, YY, idx, ridge)
+            steps = 0
+        g = gcv(SSE, m)
+        if g < best:
+            best = g
+            best_mask = mask.copy()
+    return best_mask
+
+
+def prune(B, y, w=None, n_true=None, penalty=3.0, ridge=0.0, mask=None):
+    """
+    Linear coefficients by backward stepwise selection with GCV (Friedman
+    1991, section 3.4), as the reference's mars.prune (mars.py:607-698): the
+    intercept (column 0) is protected, columns are standardized to a unit
+    Gram diagonal (so `ridge` acts on that scale), non-finite y rows get
+    weight 0, weights fold in as sqrt(w).  The Gram products run on the
+    device (gram); the M x M elimination runs on the host.
+
+    Returns beta (M,), zero for pruned terms, in B's original scale.
+    """
+    n, M = B.shape
+    if n_true is None:
+        n_true = n
+    ok = np.isfinite(y)
+    if not ok.all():
+        if w is None:
+            w = np.ones(len(y), dtype=B.dtype)
+        w = np.where(ok, w, 0)
+        y = np.where(ok, y, 0)
+    if w is not None:
+        sw = np.sqrt(np.asarray(w, dtype=B.dtype))
+        B = B * sw[:, np.newaxis]
+        y = y * sw
+    XX = gram(B)
+    XY = gram(B, y)
+    YY = gram(y, y)
+    d = np.sqrt(np.diag(XX))
+    active = d > 0
+    scale = np.where(active, d, 1.0)
+    XX = XX / scale / scale[:, np.newaxis]
+    XY = XY / scale
+    mask = (np.array(mask) if mask is not None else np.ones(M, dtype="bool")) & active
+    chosen = _backward_eliminate(XX, XY, YY, mask, penalty, n_true, ridge)
+    if ridge > 0:
+        XX = XX + ridge * np.eye(len(XX))
+    beta = np.zeros(M)
+    if chosen.any():
+        beta[chosen] = np.linalg.lstsq(XX[np.ix_(chosen, chosen)], XY[chosen], rcond=None)[0]
+    return beta / scale
+
+
+def compact(model):
+    """
+    Drop the nodes whose `beta` is zero, keeping every ancestor a surviving
+    node needs (kept with its zero beta), and remap the parent indices
+    (reference mars.py:701-738).
+    """
+    keep = model["beta"] != 0
+    # a parent always precedes its children: one backward sweep marks every ancestor
+    for i in range(len(model) - 1, 0, -1):
+        if keep[i] and model[i]["type"] != b"i":
+            keep[model[i]["basis"]] = True
+    renum = np.full(len(model), -1, dtype="i4")
+    renum[keep] = np.arange(int(keep.sum()))
+    out = model[keep].copy()
+    out["basis"] = renum[out["basis"]]
+    return out
+
+
+def pprint(model, beta=None, labels=None):
+    """
+    Print one line per node: its coefficient and the product of its inputs
+    (reference mars.py:741-776).  Coefficients default to model["beta"].
+    """
+    if beta is None:
+        beta = model["beta"]
+
+    def name(i):
+        return labels[i] if labels is not None else "X[%d]" % i
+
+    def factors(i):
+        row = model[i]
+        if row["type"] == b"+":
+            f = "MAX(%s-%g,0)" % (name(row["input"]), row["hinge"])
+        elif row["type"] == b"-":
+            f = "MAX(%g-%s,0)" % (row["hinge"], name(row["input"]))
         else:
-            raise ValueError("unknown basis type %r" % t)
-    return B
+            f = name(row["input"])
+        return [f] + factors(row["basis"]) if row["basis"] > 0 else [f]
+
+    for i in range(len(model)):
+        line = "  "
+        if beta is not None and hasattr(beta, "__getitem__"):
+            line += "%+9.4g" % beta[i]
+        if model[i]["type"] != b"i":
+            line += " * " + " * ".join(factors(i))
+        print(line)
 
 
 def fit_batch(Xs, ys, ws=None, **kwargs):

@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
 try:
-    from ._marslib import Batch, MarsAlgo, __version__  # noqa: F401
+    from ._marslib import Batch, MarsAlgo, __version__, expand_f32, expand_f64, gram_f32, gram_f64  # noqa: F401
 except ImportError as e:  # pragma: no cover - exercised only on unbuilt trees
     raise ImportError(
         "mars_amd native extension is not built (expected %s and libmars_hip.so next to it); "

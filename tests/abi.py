@@ -42,6 +42,10 @@ def load():
     lib.mars_hip_fit_append.argtypes = [P, C.c_char, i, i, C.c_float, C.POINTER(d)]
     lib.mars_hip_fit_basis.argtypes = [P, P, P, C.POINTER(sz)]
     lib.mars_hip_stats.argtypes = [P, C.POINTER(C.c_long), C.POINTER(d)]
+    lib.mars_hip_expand.argtypes = [P, sz, sz, sz, sz, C.c_char_p, P, P, P, P, sz]
+    lib.mars_hip_expand_f64.argtypes = [P, sz, sz, sz, sz, C.c_char_p, P, P, P, P, sz]
+    lib.mars_hip_gram.argtypes = [P, sz, sz, sz, P, sz, sz, P, sz]
+    lib.mars_hip_gram_f64.argtypes = [P, sz, sz, sz, P, sz, sz, P, sz]
     del fp
     return lib
 
