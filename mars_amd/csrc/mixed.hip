@@ -34,7 +34,7 @@ namespace {
 constexpr int kChunk = 32;         // rows per FP32 accumulation chunk (folded into FP64)
 constexpr int kXT = 64;            // X columns per update item (4 groups of 16)
 constexpr double kWindow = 1e-4;   // relative near-tie window rescored in FP64
-constexpr int kGroupInts = 7;      // fit, xc0, nc, j, a-tile, first item, item count
+constexpr int kGroupInts = 7;      // fit, xc0, nc, j, a-tile, first list entry, item count
 
 /*
  * Cache update for new basis column j of one fit.  Item = (fit, A tile u of
@@ -56,14 +56,11 @@ struct TcChunk {
 };
 
 template <bool NORMS>
-__global__ __launch_bounds__(64, 2) void k_tc_update(const FitDev *__restrict__ fits,
-                                                     const Item *__restrict__ items, int nitems,
-                                                     double *__restrict__ tpart, double *__restrict__ npart)
+__device__ __forceinline__ void tc_item(const FitDev &F, const Item &I, int it, double *__restrict__ tpart,
+                                        double *__restrict__ npart)
 {
     const int lane = threadIdx.x, kr = lane >> 4, kc = lane & 15;
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-        const Item I = items[it];
-        const FitDev F = fits[I.fit];
+    {
         const int j = I.pad >> 8, u = I.pad & 255, na = 2 * j + 1;
         const int a = 16 * u + kc;
         const bool aok = a < na;
@@ -182,9 +179,24 @@ __global__ __launch_bounds__(64, 2) void k_tc_update(const FitDev *__restrict__ 
     }
 }
 
-/* Sum each (fit, A tile, X-column block) group's row partials in item order and write the new cache entries. */
+/* One wave per item; the A tile 0 items also form |Bx|^2 and Bx.y (a uniform branch per item). */
+__global__ __launch_bounds__(64, 2) void k_tc_update(const FitDev *__restrict__ fits,
+                                                     const Item *__restrict__ items, int nitems,
+                                                     double *__restrict__ tpart, double *__restrict__ npart)
+{
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const Item I = items[it];
+        const FitDev F = fits[I.fit];
+        if ((I.pad & 255) == 0)
+            tc_item<true>(F, I, it, tpart, npart);
+        else
+            tc_item<false>(F, I, it, tpart, npart);
+    }
+}
+
+/* Sum each (fit, A tile, X-column block) group's row partials in row order and write the new cache entries. */
 __global__ __launch_bounds__(256) void k_tc_reduce(const FitDev *__restrict__ fits, const int *__restrict__ groups,
-                                                   const double *__restrict__ tpart,
+                                                   const int *__restrict__ glist, const double *__restrict__ tpart,
                                                    const double *__restrict__ npart)
 {
     const int *G = groups + (size_t)blockIdx.x * kGroupInts;
@@ -196,7 +208,7 @@ __global__ __launch_bounds__(256) void k_tc_reduce(const FitDev *__restrict__ fi
         const int xl = e >> 4, al = e & 15, a = 16 * u + al;
         if (a >= na) continue;
         double s = 0.0;
-        for (int i = 0; i < count; ++i) s += tpart[((size_t)(first + i) * kXT + xl) * 16 + al];
+        for (int i = 0; i < count; ++i) s += tpart[((size_t)glist[first + i] * kXT + xl) * 16 + al];
         const long long xc = xc0 + xl;
         const int k = a <= j ? a : j, par = a <= j ? j : a - j - 1;
         F.tc[(xc * ldb + par) * ldb + k] = s;
@@ -205,8 +217,8 @@ __global__ __launch_bounds__(256) void k_tc_reduce(const FitDev *__restrict__ fi
     for (int xl = threadIdx.x; xl < nc; xl += 256) {
         double s0 = 0.0, s1 = 0.0;
         for (int i = 0; i < count; ++i) {
-            s0 += npart[((size_t)(first + i) * kXT + xl) * 2];
-            s1 += npart[((size_t)(first + i) * kXT + xl) * 2 + 1];
+            s0 += npart[((size_t)glist[first + i] * kXT + xl) * 2];
+            s1 += npart[((size_t)glist[first + i] * kXT + xl) * 2 + 1];
         }
         F.tcn[((xc0 + xl) * ldb + j) * 2] = s0;
         F.tcn[((xc0 + xl) * ldb + j) * 2 + 1] = s1;
@@ -345,32 +357,63 @@ void tc_update(mars_hip_ctx *ctx)
             bytes += 4.0 * fs.n * (2.0 * (fs.mc + 1) + fs.px + 1);
         }
         if (work == 0) break;
-        // the A tile 0 items (which also form |Bx|^2 and Bx.y) first, then the others
-        std::vector<Item> items;
-        std::vector<int> groups;
-        size_t n0 = 0;
-        for (int pass = 0; pass < 2; ++pass) {
-            for (size_t f = 0; f < F; ++f) {
-                if (jf[f] < 0) continue;
-                const FitState &fs = ctx->fits[f];
-                const long long n = (long long)fs.n;
-                const int tiles = (2 * jf[f] + 1 + 15) / 16;
-                long long splits = std::max<long long>(1, std::llround(8192.0 * n / work));
-                splits = std::min<long long>(splits, std::max<long long>(1, n / 256));
-                const long long rows = ((n + splits - 1) / splits + kChunk - 1) / kChunk * kChunk;
-                for (int u = pass == 0 ? 0 : 1; u < (pass == 0 ? 1 : tiles); ++u)
-                    for (size_t xc0 = 0; xc0 < fs.px; xc0 += kXT) {
-                        const int nc = (int)std::min<size_t>(kXT, fs.px - xc0);
-                        const int first = (int)items.size();
-                        for (long long r0 = 0; r0 < n; r0 += rows)
-                            items.push_back(Item{(int)f, (int)xc0, nc, (jf[f] << 8) | u, r0, std::min(n, r0 + rows)});
-                        const int g[kGroupInts] = {(int)f, (int)xc0, nc, jf[f], u, first, (int)items.size() - first};
-                        groups.insert(groups.end(), g, g + kGroupInts);
-                    }
-            }
-            if (pass == 0) n0 = items.size();
+        // Row groups (fit, X-column block, row range) each carry one item per A tile.  Work-group i
+        // runs on XCD i % 8, so all tiles of a row group get item indices of one residue mod 8,
+        // close together in dispatch order: the later tiles find that row range of X in the XCD's
+        // L2.  Queues are filled shortest-first and padded with empty items.
+        struct RowGroup {
+            int f, nc, tiles;
+            size_t xc0;
+            long long r0, r1;
+        };
+        std::vector<RowGroup> rgs;
+        for (size_t f = 0; f < F; ++f) {
+            if (jf[f] < 0) continue;
+            const FitState &fs = ctx->fits[f];
+            const long long n = (long long)fs.n;
+            const int tiles = (2 * jf[f] + 1 + 15) / 16;
+            long long splits = std::max<long long>(1, std::llround(8192.0 * n / work));
+            splits = std::min<long long>(splits, std::max<long long>(1, n / 256));
+            const long long rows = ((n + splits - 1) / splits + kChunk - 1) / kChunk * kChunk;
+            for (size_t xc0 = 0; xc0 < fs.px; xc0 += kXT)
+                for (long long r0 = 0; r0 < n; r0 += rows)
+                    rgs.push_back(RowGroup{(int)f, (int)std::min<size_t>(kXT, fs.px - xc0), tiles, xc0, r0,
+                                           std::min(n, r0 + rows)});
         }
-        const size_t ni = items.size(), ng = groups.size() / kGroupInts;
+        constexpr int kXcd = 8;
+        std::vector<size_t> qlen(kXcd, 0);
+        std::vector<int> rq(rgs.size());
+        std::vector<size_t> rpos(rgs.size());
+        for (size_t g = 0; g < rgs.size(); ++g) {
+            const int q = (int)(std::min_element(qlen.begin(), qlen.end()) - qlen.begin());
+            rq[g] = q;
+            rpos[g] = qlen[q];
+            qlen[q] += rgs[g].tiles;
+        }
+        const size_t qmax = *std::max_element(qlen.begin(), qlen.end());
+        std::vector<Item> items(kXcd * qmax, Item{0, 0, 0, 0, 0, 0});  // padding: no rows, no columns
+        for (size_t g = 0; g < rgs.size(); ++g) {
+            const RowGroup &R = rgs[g];
+            for (int u = 0; u < R.tiles; ++u)
+                items[kXcd * (rpos[g] + u) + rq[g]] = Item{R.f, (int)R.xc0, R.nc, (jf[R.f] << 8) | u, R.r0, R.r1};
+        }
+        // reduce groups (fit, X-column block, A tile): their items in row order, listed after the headers
+        std::vector<int> groups, glist;
+        for (size_t g0 = 0; g0 < rgs.size();) {
+            size_t g1 = g0;
+            while (g1 < rgs.size() && rgs[g1].f == rgs[g0].f && rgs[g1].xc0 == rgs[g0].xc0) ++g1;
+            for (int u = 0; u < rgs[g0].tiles; ++u) {
+                const int first = (int)glist.size();
+                for (size_t g = g0; g < g1; ++g) glist.push_back((int)(kXcd * (rpos[g] + u) + rq[g]));
+                const int h[kGroupInts] = {rgs[g0].f, (int)rgs[g0].xc0, rgs[g0].nc, jf[rgs[g0].f], u, first,
+                                           (int)(g1 - g0)};
+                groups.insert(groups.end(), h, h + kGroupInts);
+            }
+            g0 = g1;
+        }
+        const size_t nh = groups.size();
+        groups.insert(groups.end(), glist.begin(), glist.end());
+        const size_t ni = items.size(), ng = nh / kGroupInts;
         ctx->d_items.ensure(ni);
         ctx->d_cmap.ensure(groups.size());
         MH_CHECK(hipMemcpyAsync(ctx->d_items.p, items.data(), ni * sizeof(Item), hipMemcpyHostToDevice, st));
@@ -380,22 +423,14 @@ void tc_update(mars_hip_ctx *ctx)
         ctx->aux.ensure(ni * kXT * 2);
         {
             KTimer kt(ctx, K_MIXED_A, flops, bytes);  // the update kernel alone
-            const Item *di = (const Item *)ctx->d_items.p;
-            if (n0 > 0) {
-                const dim3 grid((unsigned)std::min<size_t>(n0, 65536));
-                hipLaunchKernelGGL(k_tc_update<true>, grid, dim3(64), 0, st, (const FitDev *)ctx->d_fits.p, di,
-                                   (int)n0, ctx->tpart.p, ctx->aux.p);
-                MH_LAUNCH_CHECK();
-            }
-            if (ni > n0) {
-                const dim3 grid((unsigned)std::min<size_t>(ni - n0, 65536));
-                hipLaunchKernelGGL(k_tc_update<false>, grid, dim3(64), 0, st, (const FitDev *)ctx->d_fits.p,
-                                   di + n0, (int)(ni - n0), ctx->tpart.p + n0 * kXT * 16, ctx->aux.p + n0 * kXT * 2);
-                MH_LAUNCH_CHECK();
-            }
+            const dim3 grid((unsigned)std::max<size_t>(1, std::min<size_t>(ni, 65536)));
+            hipLaunchKernelGGL(k_tc_update, grid, dim3(64), 0, st, (const FitDev *)ctx->d_fits.p,
+                               (const Item *)ctx->d_items.p, (int)ni, ctx->tpart.p, ctx->aux.p);
+            MH_LAUNCH_CHECK();
         }
         hipLaunchKernelGGL(k_tc_reduce, dim3((unsigned)ng), dim3(256), 0, st, (const FitDev *)ctx->d_fits.p,
-                           (const int *)ctx->d_cmap.p, (const double *)ctx->tpart.p, (const double *)ctx->aux.p);
+                           (const int *)ctx->d_cmap.p, (const int *)ctx->d_cmap.p + nh, (const double *)ctx->tpart.p,
+                           (const double *)ctx->aux.p);
         MH_LAUNCH_CHECK();
         for (size_t f = 0; f < F; ++f)
             if (jf[f] >= 0) ctx->fits[f].mc += 1;
