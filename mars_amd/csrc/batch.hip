@@ -87,7 +87,7 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
                 mixed_score(ctx);
             else
                 plain_score(ctx, cs, false, ctx->lin.p, ctx->ybx.p);
-            MH_CHECK(hipStreamSynchronize(ctx->stream));
+            ctx_sync(ctx);
         }
         {
             HostTimer t(ctx, K_H_ARGMAX);

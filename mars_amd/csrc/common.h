@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cstddef>
 #include <cstdint>
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -37,7 +38,8 @@ inline void require(bool ok, const char *msg)
 /* DGKS gate: retry when residual energy * 9 < projected energy (kernels.h:17). */
 constexpr double kGate = 9.0;
 
-/* Device buffer that only grows. */
+/* Device buffer that only grows: the first allocation is exact, a regrowth takes at least
+ * 1.5x (lists that grow every step reallocate O(log) times; hipFree waits for the device). */
 template <typename T>
 struct DBuf {
     T *p = nullptr;
@@ -45,11 +47,12 @@ struct DBuf {
     void ensure(size_t count)
     {
         if (count <= cap) return;
+        const size_t want = cap ? std::max(count, cap + cap / 2) : count;
         if (p) MH_CHECK(hipFree(p));
         p = nullptr;
         cap = 0;
-        MH_CHECK(hipMalloc((void **)&p, sizeof(T) * (count ? count : 1)));
-        cap = count;
+        MH_CHECK(hipMalloc((void **)&p, sizeof(T) * (want ? want : 1)));
+        cap = want;
     }
     void release()
     {
@@ -179,6 +182,9 @@ struct mars_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // pinned staging for host-to-device copies: bump-allocated, reset after a stream sync
+    char *stage = nullptr;
+    size_t stage_cap = 0, stage_off = 0;
 
     size_t max_terms = 0;  // basis capacity (all fits)
     size_t ldb = 0;        // row stride of B and Bo (>= max_terms, multiple of 4)

@@ -457,10 +457,45 @@ void fit_state_alloc(mars_hip_ctx *ctx, FitState &fs, size_t n, double tol)
     fs.v.ensure(n);
 }
 
+constexpr size_t kStageMax = 16u << 20;  // larger copies (one-time data uploads) go direct
+
+void h2d(mars_hip_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return;
+    if (bytes > kStageMax) {
+        MH_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        return;
+    }
+    const size_t need = (bytes + 255) / 256 * 256;
+    if (ctx->stage_off + need > ctx->stage_cap) {
+        ctx_sync(ctx);  // every copy out of the staging space has completed
+        if (need > ctx->stage_cap) {
+            if (ctx->stage) MH_CHECK(hipHostFree(ctx->stage));
+            ctx->stage = nullptr;
+            ctx->stage_cap = 0;
+            const size_t cap = std::max<size_t>(need, 4u << 20);
+            MH_CHECK(hipHostMalloc((void **)&ctx->stage, cap, hipHostMallocDefault));
+            ctx->stage_cap = cap;
+        }
+    }
+    char *h = ctx->stage + ctx->stage_off;
+    std::memcpy(h, src, bytes);
+    ctx->stage_off += need;
+    MH_CHECK(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+}
+
+void ctx_sync(mars_hip_ctx *ctx)
+{
+    MH_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->stage_off = 0;
+}
+
 void ctx_free(mars_hip_ctx *ctx)
 {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stage) (void)hipHostFree(ctx->stage);
+    ctx->stage = nullptr;
     for (auto &fs : ctx->fits) fs.release();
     DBuf<double> *db[] = {&ctx->tpart, &ctx->T, &ctx->T2, &ctx->tnorm, &ctx->spart, &ctx->ssum, &ctx->ybx,
                           &ctx->lin, &ctx->hin, &ctx->aux, &ctx->red, &ctx->vcol, &ctx->scal,
@@ -512,8 +547,8 @@ void fit_init(mars_hip_ctx *ctx, FitState &fs, const float *X, size_t ldX, const
                               px, hipMemcpyHostToDevice, st));
     DBuf<float> yw;
     yw.ensure(2 * n);
-    MH_CHECK(hipMemcpyAsync(yw.p, y, n * sizeof(float), hipMemcpyHostToDevice, st));
-    MH_CHECK(hipMemcpyAsync(yw.p + n, w, n * sizeof(float), hipMemcpyHostToDevice, st));
+    h2d(ctx, yw.p, y, n * sizeof(float));
+    h2d(ctx, yw.p + n, w, n * sizeof(float));
     DBuf<double> tmp, part, scal, cpart;
     tmp.ensure(2 * n);
     part.ensure(4 * kInitBlocks);
@@ -537,7 +572,7 @@ void fit_init(mars_hip_ctx *ctx, FitState &fs, const float *X, size_t ldX, const
     MH_LAUNCH_CHECK();
     double sc[S_COUNT];
     MH_CHECK(hipMemcpyAsync(sc, scal.p, sizeof(sc), hipMemcpyDeviceToHost, st));
-    MH_CHECK(hipStreamSynchronize(st));
+    ctx_sync(ctx);
     yw.release();
     tmp.release();
     part.release();
@@ -588,14 +623,14 @@ void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std
     ctx->red.ensure(3 * F * nb);
     ctx->scal.ensure(F * S_COUNT);
     ctx->flag.ensure(F);
-    MH_CHECK(hipMemcpyAsync(ctx->aarg.p, args.data(), F * sizeof(AppendArg), hipMemcpyHostToDevice, st));
+    h2d(ctx, ctx->aarg.p, args.data(), F * sizeof(AppendArg));
     std::vector<FitW> fw(F);
     for (size_t f = 0; f < F; ++f) {
         FitState &fs = ctx->fits[f];
         fw[f] = FitW{fs.B.p, fs.Bo.p, fs.BoC.p, fs.BC.p, fs.ybo.p};
     }
     ctx->d_fitw.ensure(F);
-    MH_CHECK(hipMemcpyAsync(ctx->d_fitw.p, fw.data(), F * sizeof(FitW), hipMemcpyHostToDevice, st));
+    h2d(ctx, ctx->d_fitw.p, fw.data(), F * sizeof(FitW));
     double *pa = ctx->red.p, *pb = pa + F * nb, *pc = pb + F * nb;
     double *pp[2] = {pa, pb};
     int *gate = ctx->flag.p;
@@ -615,7 +650,7 @@ void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std
                 // the retry sweep only runs when some fit's DGKS gate fired
                 std::vector<int> g(F);
                 MH_CHECK(hipMemcpyAsync(g.data(), gate, F * sizeof(int), hipMemcpyDeviceToHost, st));
-                MH_CHECK(hipStreamSynchronize(st));
+                ctx_sync(ctx);
                 bool any = false;
                 for (size_t f = 0; f < F; ++f) any = any || (args[f].active && g[f]);
                 if (!any) break;
@@ -638,7 +673,7 @@ void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std
         MH_LAUNCH_CHECK();
     }
     MH_CHECK(hipMemcpyAsync(out.data(), ctx->aout.p, F * sizeof(AppendOut), hipMemcpyDeviceToHost, st));
-    MH_CHECK(hipStreamSynchronize(st));
+    ctx_sync(ctx);
     resolve_timing(ctx);
     for (size_t f = 0; f < F; ++f) {
         if (!args[f].active) continue;
@@ -670,7 +705,7 @@ void argmax_fits(mars_hip_ctx *ctx, const Cands &cs, const double *lin, std::vec
         ol[f] = cs.fit_c0[f];
         ol[F + f] = cs.fit_P[f];
     }
-    MH_CHECK(hipMemcpyAsync(ctx->ival.p, ol.data(), 2 * F * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    h2d(ctx, ctx->ival.p, ol.data(), 2 * F * sizeof(int));
     {
         KTimer kt(ctx, K_ARGMAX, 0.0, 8.0 * cs.fit.size());
         hipLaunchKernelGGL(k_argmax, dim3(F), dim3(1024), 0, ctx->stream, lin, nullptr, nullptr, ctx->ival.p,
@@ -679,7 +714,7 @@ void argmax_fits(mars_hip_ctx *ctx, const Cands &cs, const double *lin, std::vec
     }
     best.resize(F);
     MH_CHECK(hipMemcpyAsync(best.data(), ctx->bout.p, F * sizeof(BestOut), hipMemcpyDeviceToHost, ctx->stream));
-    MH_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx_sync(ctx);
 }
 
 void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t stride, int min_span,
@@ -743,7 +778,7 @@ void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t 
     ctx->ival.ensure(2);
     ctx->bout.ensure(1);
     const int ol[2] = {0, (int)total};
-    MH_CHECK(hipMemcpyAsync(ctx->ival.p, ol, sizeof(ol), hipMemcpyHostToDevice, st));
+    h2d(ctx, ctx->ival.p, ol, sizeof(ol));
     hipLaunchKernelGGL(k_argmax, dim3(1), dim3(1024), 0, st, dlin, dhin, dknot, ctx->ival.p, ctx->ival.p + 1,
                        ctx->bout.p);
     MH_LAUNCH_CHECK();
@@ -753,7 +788,7 @@ void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t 
     if (lin_out && total) MH_CHECK(hipMemcpyAsync(lin_out, dlin, total * 8, hipMemcpyDeviceToHost, st));
     if (hin_out && total) MH_CHECK(hipMemcpyAsync(hin_out, dhin, total * 8, hipMemcpyDeviceToHost, st));
     if (cut_out && total) MH_CHECK(hipMemcpyAsync(cut_out, dcut, total * 8, hipMemcpyDeviceToHost, st));
-    MH_CHECK(hipStreamSynchronize(st));
+    ctx_sync(ctx);
     double cut2 = NAN;
     if (total) {
         require(bo.c2 >= 0 && (size_t)bo.c2 < total, "internal: argmax index out of range");
@@ -792,19 +827,19 @@ void mirror_sync_basis(mars_hip_ctx *ctx, size_t m, const float *B, size_t ldB, 
         col.ensure(n);
         std::vector<float> h(n);
         for (size_t j = fs.m_synced; j < m; ++j) {
-            MH_CHECK(hipMemcpyAsync(col.p, B + j * ldB, n * sizeof(float), hipMemcpyHostToDevice, st));
+            h2d(ctx, col.p, B + j * ldB, n * sizeof(float));
             hipLaunchKernelGGL(k_put_bcol, dim3((n + 255) / 256), dim3(256), 0, st, fs.B.p, ctx->ldb, (int)j, col.p,
                                n);
             MH_LAUNCH_CHECK();
             MH_CHECK(hipMemcpyAsync(fs.BC.p + j * fs.ldx, col.p, n * sizeof(float), hipMemcpyDeviceToDevice, st));
-            MH_CHECK(hipStreamSynchronize(st));
+            ctx_sync(ctx);
             for (size_t i = 0; i < n; ++i) h[i] = Bo[i * ldBo + j];
-            MH_CHECK(hipMemcpyAsync(col.p, h.data(), n * sizeof(float), hipMemcpyHostToDevice, st));
+            h2d(ctx, col.p, h.data(), n * sizeof(float));
             hipLaunchKernelGGL(k_put_bcol, dim3((n + 255) / 256), dim3(256), 0, st, fs.Bo.p, ctx->ldb, (int)j, col.p,
                                n);
             MH_LAUNCH_CHECK();
             MH_CHECK(hipMemcpyAsync(fs.BoC.p + j * fs.ldx, col.p, n * sizeof(float), hipMemcpyDeviceToDevice, st));
-            MH_CHECK(hipStreamSynchronize(st));
+            ctx_sync(ctx);
         }
         col.release();
         fs.m_synced = m;
@@ -817,8 +852,8 @@ void mirror_set_target(mars_hip_ctx *ctx, const float *y)
     require(y != nullptr, "null target");
     FitState &fs = ctx->fits[0];
     if (fs.y_key == y) return;
-    MH_CHECK(hipMemcpyAsync(fs.y.p, y, fs.n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    MH_CHECK(hipStreamSynchronize(ctx->stream));
+    h2d(ctx, fs.y.p, y, fs.n * sizeof(float));
+    ctx_sync(ctx);
     fs.y_key = y;
 }
 
@@ -837,16 +872,15 @@ void mirror_sync_xcols(mars_hip_ctx *ctx, size_t px, const float *X, size_t ldX,
     }
     require(px == fs.px, "p_x changed between calls");
     require(ncols == 0 || cols != nullptr, "null column list");
-    MH_CHECK(hipMemcpyAsync(fs.s.p, s, px * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    h2d(ctx, fs.s.p, s, px * sizeof(float));
     for (size_t k = 0; k < ncols; ++k) {
         const int c = cols[k];
         require(c >= 0 && (size_t)c < px, "X column out of range");
         if (fs.xres[c]) continue;
-        MH_CHECK(hipMemcpyAsync(fs.X.p + (size_t)c * fs.ldx, X + (size_t)c * ldX, n * sizeof(float), hipMemcpyHostToDevice,
-                                ctx->stream));
+        h2d(ctx, fs.X.p + (size_t)c * fs.ldx, X + (size_t)c * ldX, n * sizeof(float));
         fs.xres[c] = 1;
     }
-    MH_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx_sync(ctx);
 }
 
 void mirror_orthonormalize(mars_hip_ctx *ctx, size_t m, size_t p, const float *x, const int *mask, float *Bx,
@@ -865,8 +899,8 @@ void mirror_orthonormalize(mars_hip_ctx *ctx, size_t m, size_t p, const float *x
     xs1.ensure(fs.ldx + 32);
     MH_CHECK(hipMemsetAsync(xs1.p, 0, (fs.ldx + 32) * sizeof(float), st));
     const float one = 1.0f;
-    MH_CHECK(hipMemcpyAsync(xs1.p, x, n * sizeof(float), hipMemcpyHostToDevice, st));
-    MH_CHECK(hipMemcpyAsync(xs1.p + fs.ldx, &one, sizeof(float), hipMemcpyHostToDevice, st));
+    h2d(ctx, xs1.p, x, n * sizeof(float));
+    h2d(ctx, xs1.p + fs.ldx, &one, sizeof(float));
     Cands &cs = ctx->cand;
     cs.fit.assign(p, 0);
     cs.xc.assign(p, 0);
@@ -883,11 +917,11 @@ void mirror_orthonormalize(mars_hip_ctx *ctx, size_t m, size_t p, const float *x
         MH_LAUNCH_CHECK();
         MH_CHECK(hipMemcpy2DAsync(Bx, ldBx * sizeof(float), out.p, n * sizeof(float), n * sizeof(float), p,
                                   hipMemcpyDeviceToHost, st));
-        MH_CHECK(hipStreamSynchronize(st));
+        ctx_sync(ctx);
         out.release();
     }
     if (ybx) MH_CHECK(hipMemcpyAsync(ybx, ctx->ybx.p, p * sizeof(double), hipMemcpyDeviceToHost, st));
-    MH_CHECK(hipStreamSynchronize(st));
+    ctx_sync(ctx);
     xs1.release();
     resolve_timing(ctx);
     if (fired) *fired += ctx->dgks_host - d0;
@@ -915,7 +949,7 @@ void mirror_orthonormalize_batch(mars_hip_ctx *ctx, size_t m, size_t P, const in
     ctx->ybx.ensure(P);
     plain_score(ctx, cs, false, ctx->lin.p, ctx->ybx.p);
     MH_CHECK(hipMemcpyAsync(ybx, ctx->ybx.p, P * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    MH_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx_sync(ctx);
     resolve_timing(ctx);
     if (fired) *fired += ctx->dgks_host - d0;
 }
@@ -941,7 +975,7 @@ int mars_hip_context_create(size_t n, size_t max_terms, double tol, mars_hip_ctx
             ctx_base_init(ctx, max_terms);
             ctx->fits.resize(1);
             fit_state_alloc(ctx, ctx->fits[0], n, tol);
-            MH_CHECK(hipStreamSynchronize(ctx->stream));
+            ctx_sync(ctx);
         } catch (...) {
             ctx_free(ctx);
             throw;
@@ -1096,7 +1130,7 @@ int mars_hip_fit_basis(mars_hip_ctx *ctx, float *B, float *Bo, size_t *m_out)
         for (int q = 0; q < 2; ++q) {
             if (!dst[q] || m == 0) continue;
             MH_CHECK(hipMemcpyAsync(h.data(), src[q], ld * m * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-            MH_CHECK(hipStreamSynchronize(ctx->stream));
+            ctx_sync(ctx);
             for (size_t i = 0; i < n; ++i)
                 for (size_t j = 0; j < m; ++j) dst[q][i * m + j] = h[j * ld + i];
         }
@@ -1107,7 +1141,7 @@ int mars_hip_timing(mars_hip_ctx *ctx, int enable)
 {
     return guarded([&] {
         require(ctx != nullptr, "null context");
-        MH_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx_sync(ctx);
         resolve_timing(ctx);
         ctx->timing = enable != 0;
     });
@@ -1117,7 +1151,7 @@ int mars_hip_kernel_stats(mars_hip_ctx *ctx, mars_hip_kstat *out, int capacity, 
 {
     return guarded([&] {
         require(ctx != nullptr && count != nullptr, "null argument");
-        MH_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx_sync(ctx);
         resolve_timing(ctx);
         int k = 0;
         for (int c = 0; c < K_NCLASS; ++c) {

@@ -597,11 +597,11 @@ static void ensure_sorted(mars_hip_ctx *ctx, int xc)
     MH_CHECK(hipMemcpyAsync(out, va, n * sizeof(int), hipMemcpyDeviceToDevice, st));
     float sc = 1.0f;
     MH_CHECK(hipMemcpyAsync(&sc, fs.s.p + xc, sizeof(float), hipMemcpyDeviceToHost, st));
-    MH_CHECK(hipStreamSynchronize(st));
+    ctx_sync(ctx);
     hipLaunchKernelGGL(k_deltas, dim3((n + 255) / 256), dim3(256), 0, st, fs.X.p + (size_t)xc * fs.ldx,
                        sc, out, n, fs.delta.p + (size_t)xc * n);
     MH_LAUNCH_CHECK();
-    MH_CHECK(hipStreamSynchronize(st));
+    ctx_sync(ctx);
     keys.release();
     vals.release();
     hist.release();
@@ -655,8 +655,8 @@ void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span)
     DBuf<int> dgrp, dcgrp;
     dgrp.ensure(grp.size());
     dcgrp.ensure(P);
-    MH_CHECK(hipMemcpyAsync(dgrp.p, grp.data(), grp.size() * sizeof(int), hipMemcpyHostToDevice, st));
-    MH_CHECK(hipMemcpyAsync(dcgrp.p, cgrp.data(), P * sizeof(int), hipMemcpyHostToDevice, st));
+    h2d(ctx, dgrp.p, grp.data(), grp.size() * sizeof(int));
+    h2d(ctx, dcgrp.p, cgrp.data(), P * sizeof(int));
     ctx->hstate.ensure((size_t)P * C * L2 * 2 + (size_t)P * C + (size_t)ng * C);
     ctx->hbest.ensure((size_t)P * C);
     ctx->hbest_i.ensure((size_t)P * C);
@@ -715,7 +715,7 @@ void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span)
                            ctx->lin.p, ctx->hin.p, ctx->knot.p, ctx->krow.p);
         MH_LAUNCH_CHECK();
     }
-    MH_CHECK(hipStreamSynchronize(st));
+    ctx_sync(ctx);
     dgrp.release();
     dcgrp.release();
 }

@@ -58,6 +58,13 @@ void argmax_fits(mars_hip_ctx *ctx, const Cands &cs, const double *lin, std::vec
 double host_dsse(const FitState &fs);
 extern thread_local std::string g_last_error;
 
+/* Asynchronous host-to-device copy on the context's stream.  Small copies go through pinned
+ * staging, so the call does not wait for the stream (a pageable copy does); the source may be
+ * reused as soon as it returns. */
+void h2d(mars_hip_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* hipStreamSynchronize on the context's stream; the staging space is free again afterwards. */
+void ctx_sync(mars_hip_ctx *ctx);
+
 /* Run f, turning exceptions into a status code and the thread's last error text. */
 template <typename F>
 int guarded(F &&f)

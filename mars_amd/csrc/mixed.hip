@@ -416,9 +416,8 @@ void tc_update(mars_hip_ctx *ctx)
         const size_t ni = items.size(), ng = nh / kGroupInts;
         ctx->d_items.ensure(ni);
         ctx->d_cmap.ensure(groups.size());
-        MH_CHECK(hipMemcpyAsync(ctx->d_items.p, items.data(), ni * sizeof(Item), hipMemcpyHostToDevice, st));
-        MH_CHECK(hipMemcpyAsync(ctx->d_cmap.p, groups.data(), groups.size() * sizeof(int), hipMemcpyHostToDevice,
-                                st));
+        h2d(ctx, ctx->d_items.p, items.data(), ni * sizeof(Item));
+        h2d(ctx, ctx->d_cmap.p, groups.data(), groups.size() * sizeof(int));
         ctx->tpart.ensure(ni * kXT * 16);
         ctx->aux.ensure(ni * kXT * 2);
         {
@@ -472,7 +471,7 @@ void mixed_score(mars_hip_ctx *ctx)
     ctx->ival.ensure(2 * F + 2);
     ctx->red.ensure(F);
     ctx->rsel.ensure(P);
-    MH_CHECK(hipMemcpyAsync(ctx->ival.p, ranges.data(), 2 * F * sizeof(int), hipMemcpyHostToDevice, st));
+    h2d(ctx, ctx->ival.p, ranges.data(), 2 * F * sizeof(int));
     hipLaunchKernelGGL(k_fit_max, dim3((unsigned)F), dim3(256), 0, st, (const int *)ctx->ival.p,
                        (const int *)ctx->ival.p + F, (const double *)ctx->lin.p, (const int *)ctx->flag.p, ctx->red.p);
     MH_LAUNCH_CHECK();
@@ -484,14 +483,14 @@ void mixed_score(mars_hip_ctx *ctx)
     MH_LAUNCH_CHECK();
     int nsel[2] = {0, 0};
     MH_CHECK(hipMemcpyAsync(nsel, ctx->ival.p + 2 * F, sizeof(nsel), hipMemcpyDeviceToHost, st));
-    MH_CHECK(hipStreamSynchronize(st));
+    ctx_sync(ctx);
     const int R = nsel[0];
     ctx->gated += nsel[1];
     ctx->rescored += R;
     if (R == 0) return;
     std::vector<int> sub(R);
     MH_CHECK(hipMemcpyAsync(sub.data(), ctx->rsel.p, R * sizeof(int), hipMemcpyDeviceToHost, st));
-    MH_CHECK(hipStreamSynchronize(st));
+    ctx_sync(ctx);
     Cands &rc = ctx->rcand;
     rc.fit.resize(R);
     rc.xc.resize(R);

@@ -217,11 +217,9 @@ void upload_pass(mars_hip_ctx *ctx, const Pass &p, DBuf<Item> &items, DBuf<int> 
     u.items = items.p + item_off;
     u.cmap = cmap.p + cmap_off;
     if (!p.items.empty())
-        MH_CHECK(hipMemcpyAsync(u.items, p.items.data(), p.items.size() * sizeof(Item),
-                                hipMemcpyHostToDevice, ctx->stream));
+        h2d(ctx, u.items, p.items.data(), p.items.size() * sizeof(Item));
     if (!p.cmap.empty())
-        MH_CHECK(hipMemcpyAsync(u.cmap, p.cmap.data(), p.cmap.size() * sizeof(int), hipMemcpyHostToDevice,
-                                ctx->stream));
+        h2d(ctx, u.cmap, p.cmap.data(), p.cmap.size() * sizeof(int));
 }
 
 int grid_for(size_t items)
@@ -304,9 +302,9 @@ void upload_cands(mars_hip_ctx *ctx, Cands &cs)
     cs.d_xc.ensure(P);
     cs.d_par.ensure(P);
     if (!P) return;
-    MH_CHECK(hipMemcpyAsync(cs.d_fit.p, cs.fit.data(), P * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-    MH_CHECK(hipMemcpyAsync(cs.d_xc.p, cs.xc.data(), P * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-    MH_CHECK(hipMemcpyAsync(cs.d_par.p, cs.par.data(), P * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    h2d(ctx, cs.d_fit.p, cs.fit.data(), P * sizeof(int));
+    h2d(ctx, cs.d_xc.p, cs.xc.data(), P * sizeof(int));
+    h2d(ctx, cs.d_par.p, cs.par.data(), P * sizeof(int));
 }
 
 void ensure_rbuf(mars_hip_ctx *ctx, const Cands &cs)
@@ -355,8 +353,7 @@ void upload_fits(mars_hip_ctx *ctx, const Cands &cs, const float *xover, const f
         d.tol = fs.tol;
     }
     ctx->d_fits.ensure(h.size());
-    MH_CHECK(hipMemcpyAsync(ctx->d_fits.p, h.data(), h.size() * sizeof(FitDev), hipMemcpyHostToDevice,
-                            ctx->stream));
+    h2d(ctx, ctx->d_fits.p, h.data(), h.size() * sizeof(FitDev));
 }
 
 void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, double *ybx,
@@ -451,7 +448,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     // DGKS retry of the flagged columns, grouped by fit (kernels.cc:387-399)
     std::vector<int> flags(P);
     MH_CHECK(hipMemcpyAsync(flags.data(), ctx->flag.p, P * sizeof(int), hipMemcpyDeviceToHost, st));
-    MH_CHECK(hipStreamSynchronize(st));
+    ctx_sync(ctx);
     std::vector<int> fired, live;
     std::vector<int> rb(F, 0), re(F, 0), lb(F, 0), le(F, 0);
     for (size_t f = 0; f < F; ++f) {
@@ -468,7 +465,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         const int nf = (int)fired.size();
         ctx->dgks_host += nf;
         KTimer kt(ctx, K_DGKS, 0.0, 0.0);
-        MH_CHECK(hipMemcpyAsync(ctx->fired_list.p, fired.data(), nf * sizeof(int), hipMemcpyHostToDevice, st));
+        h2d(ctx, ctx->fired_list.p, fired.data(), nf * sizeof(int));
         std::vector<int> fcnt(F);
         for (size_t f = 0; f < F; ++f) fcnt[f] = re[f] - rb[f];
         const bool rnarrow = narrow_for(fcnt);
@@ -523,7 +520,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                 ctx->live_list.ensure(nl);
                 ctx->d_litems.ensure(pL.items.size() + 1);
                 ctx->d_lcmap.ensure(pL.cmap.size() + 1);
-                MH_CHECK(hipMemcpyAsync(ctx->live_list.p, live.data(), nl * sizeof(int), hipMemcpyHostToDevice, st));
+                h2d(ctx, ctx->live_list.p, live.data(), nl * sizeof(int));
                 upload_pass(ctx, pL, ctx->d_litems, ctx->d_lcmap, 0, 0, uL);
                 ctx->spart.ensure(pL.items.size() * kSlots + 1);
                 launch_project<kFinalY, false, true>(ctx, mt, uL, (int)pL.items.size(), cs, ctx->live_list.p,
