@@ -41,8 +41,10 @@ PEAK_FP64 = 78.6e12     # MI355X FP64 (vector and matrix), FLOP/s
 PEAK_FP32 = 157.3e12    # MI355X FP32 matrix, FLOP/s
 PEAK_HBM = 8.0e12       # MI355X HBM3E, B/s
 FP32_KERNELS = ("mixed_update",)
-# Kernel classes whose event brackets span several kernels and host work (no roofline).
-COMPOSITE = ("mixed_recheck", "dgks_retry")
+# Kernel classes whose event brackets span several kernels and host work (no single-kernel
+# roofline): the exact rescoring, the DGKS retry, and the append (build, one MGS kernel per
+# basis column, gate, store, final).
+COMPOSITE = ("mixed_recheck", "dgks_retry", "append")
 # Classes bound by HBM traffic by design (arithmetic intensity below the MFMA ridge point).
 HBM_BOUND = ("mixed_update", "append", "finalize", "sort", "argmax", "mixed_score")
 CPU_SAMPLE_S = 20.0

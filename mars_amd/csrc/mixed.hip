@@ -37,49 +37,88 @@ constexpr double kWindow = 1e-4;   // relative near-tie window rescored in FP64
 constexpr int kGroupInts = 7;      // fit, xc0, nc, j, a-tile, first list entry, item count
 
 /*
- * Cache update for new basis column j of one fit.  Item = (fit, A tile u of
- * 16 columns, X columns [c0, c0+nc), rows [r0, r1)), item.pad = j << 8 | u;
- * partial sums
+ * Cache update for new basis column j of one fit.  Block item = (fit, X
+ * columns [c0, c0+nc), rows [r0, r1), A tiles u0 .. u0+W-1), item.pad =
+ * j << 8 | u0; wave w computes A tile u = u0 + w:
  *   U[a][xc] = sum_i A[i][a] xs[i][xc],   a = 16u + (0..15) < 2j+1,
  *   A[i][a] = f32(Bo[i][a] B[i][j])          (a <= j:  k = a,  parent j)
  *           = f32(Bo[i][j] B[i][a-j-1])      (a >  j:  k = j,  parent a-j-1)
- * and, in the u == 0 items (NORMS), of |Bx|^2 and Bx.y for Bx = f32(B[:,j] xs[:,xc]).
- * One wave per item, no LDS: MFMA step q of a 4-row half-step at row r takes
- * row r + 8 (lane/16) + q, so each lane's operands are float4 runs of the
- * column-major BoC, BC and X (zero-padded to a multiple of 32 rows).
- * v_mfma_f32_16x16x4f32 accumulates each 32-row chunk in FP32; the chunk sums
- * are folded into FP64.  The loads of chunk c+1 are issued before the MFMAs
- * of chunk c (two register sets, alternating).
+ * and the u == 0 wave also |Bx|^2 and Bx.y for Bx = f32(B[:,j] xs[:,xc]).
+ * Every 32-row chunk of xs = f32(X s) is staged once per block in LDS
+ * (column-major, 36-float pitch: each 16-lane phase of a ds_read_b128 hits
+ * distinct banks), double-buffered: the next chunk's X loads and each wave's
+ * next A columns are in flight during the current chunk's MFMAs.  MFMA step q
+ * of a 4-row half-step at row r takes row r + 8 (lane/16) + q, so every
+ * operand is a float4 run of one column.  v_mfma_f32_16x16x4f32 accumulates
+ * each chunk in FP32; the chunk sums are folded into FP64.
  */
-struct TcChunk {
-    float4 o[2], bb[2], bj[2], yv[2], xv[2][4];
-};
+constexpr int kTW = 8;         // A tiles (waves) per block at most
+constexpr int kXP = 36;        // LDS floats per staged X column (32 rows + pad)
 
-template <bool NORMS>
-__device__ __forceinline__ void tc_item(const FitDev &F, const Item &I, int it, double *__restrict__ tpart,
-                                        double *__restrict__ npart)
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_tc_update(const FitDev *__restrict__ fits,
+                                                      const Item *__restrict__ items, int nitems,
+                                                      double *__restrict__ tpart, double *__restrict__ npart)
 {
-    const int lane = threadIdx.x, kr = lane >> 4, kc = lane & 15;
-    {
-        const int j = I.pad >> 8, u = I.pad & 255, na = 2 * j + 1;
+    constexpr int NXL = (kXT * kChunk / 4 + 64 * W - 1) / (64 * W);  // X float4 loads per thread and chunk
+    __shared__ __attribute__((aligned(16))) float sx[2][kXT][kXP];
+    __shared__ __attribute__((aligned(16))) float sjy[2][2][kChunk];  // B[:, j] and y of the chunk
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, kr = lane >> 4, kc = lane & 15;
+    for (int bi = blockIdx.x; bi < nitems; bi += gridDim.x) {
+        const Item I = items[bi];
+        const FitDev F = fits[I.fit];
+        const int j = I.pad >> 8, u = (I.pad & 255) + wv, na = 2 * j + 1;
         const int a = 16 * u + kc;
         const bool aok = a < na;
+        const bool wact = 16 * u < na;   // this wave's tile holds A columns (uniform)
+        const bool norms = u == 0;
         const int ka = a <= j ? a : j, pa = a <= j ? j : a - j - 1;
         const float *pbo = F.BoC + (size_t)(aok ? ka : 0) * F.ldx;
         const float *pb = F.BC + (size_t)(aok ? pa : 0) * F.ldx;
         const float *pbj = F.BC + (size_t)j * F.ldx;
         const int G = (I.nc + 15) >> 4;
-        const float *px[4];
-        float sx[4];
-        bool xok[4];
+        // this thread's share of the X chunk: float4 e -> column e/8, rows 4 (e%8) ..
+        const float *xp[NXL];
+        float xsc[NXL];
+        int xcol[NXL], xrow[NXL];
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const int slot = 16 * g + kc;
-            xok[g] = slot < I.nc;
-            const int xc = I.c0 + (xok[g] ? slot : 0);
-            px[g] = F.X + (size_t)xc * F.ldx;
-            sx[g] = xok[g] ? F.s[xc] : 0.0f;
+        for (int q = 0; q < NXL; ++q) {
+            const int e = tid + 64 * W * q;
+            xcol[q] = e < kXT * kChunk / 4 ? e >> 3 : -1;
+            xrow[q] = 4 * (e & 7);
+            const bool ok = xcol[q] >= 0 && xcol[q] < I.nc;
+            const int xc = I.c0 + (ok ? xcol[q] : 0);
+            xp[q] = F.X + (size_t)xc * F.ldx + xrow[q];
+            xsc[q] = ok ? F.s[xc] : 0.0f;
         }
+        // threads 0..15 also stage B[:, j] (0..7) and y (8..15) of the chunk, 4 rows each
+        const bool jyl = tid < 16;
+        const float *jyp = (tid < 8 ? pbj : F.y) + 4 * (tid & 7);
+        auto load_x = [&](float4 (&v)[NXL], float4 &jy, long long r) {
+#pragma unroll
+            for (int q = 0; q < NXL; ++q) v[q] = gld4(xp[q] + r);
+            jy = gld4(jyp + r);
+        };
+        auto store_x = [&](const float4 (&v)[NXL], const float4 &jy, int buf) {
+#pragma unroll
+            for (int q = 0; q < NXL; ++q) {
+                if (xcol[q] < 0) continue;
+                const float s = xsc[q];
+                const float4 w4 = make_float4(__fmul_rn(v[q].x, s), __fmul_rn(v[q].y, s), __fmul_rn(v[q].z, s),
+                                              __fmul_rn(v[q].w, s));
+                *(float4 *)&sx[buf][xcol[q]][xrow[q]] = w4;
+            }
+            if (jyl) *(float4 *)&sjy[buf][tid >> 3][4 * (tid & 7)] = jy;
+        };
+        float4 ao[2][2], ab[2][2];   // [register set][half]
+        auto load_a = [&](int set, long long r) {
+#pragma unroll
+            for (int hf = 0; hf < 2; ++hf) {
+                const long long rb = r + 8 * kr + 4 * hf;
+                ao[set][hf] = gld4(pbo + rb);
+                ab[set][hf] = gld4(pb + rb);
+            }
+        };
         v4d acc[4];
         double bxx[4], bxy[4];
 #pragma unroll
@@ -88,44 +127,29 @@ __device__ __forceinline__ void tc_item(const FitDev &F, const Item &I, int it, 
             bxx[g] = 0.0;
             bxy[g] = 0.0;
         }
-        // every load of a chunk (no branches: a branch join would make the compiler wait for them)
-        auto load = [&](TcChunk &c, long long r) {
-#pragma unroll
-            for (int hf = 0; hf < 2; ++hf) {
-                const long long rb = r + 8 * kr + 4 * hf;
-                c.o[hf] = gld4(pbo + rb);
-                c.bb[hf] = gld4(pb + rb);
-                if (NORMS) {
-                    c.bj[hf] = gld4(pbj + rb);
-                    c.yv[hf] = gld4(F.y + rb);
-                }
-#pragma unroll
-                for (int g = 0; g < 4; ++g) c.xv[hf][g] = gld4(px[g] + rb);
-            }
-        };
-        auto compute = [&](const TcChunk &c) {
+        auto compute = [&](int set, int buf) {
             v4f part[4];
 #pragma unroll
             for (int g = 0; g < 4; ++g) part[g] = v4f{0, 0, 0, 0};
 #pragma unroll
             for (int hf = 0; hf < 2; ++hf) {
-                const float ov[4] = {c.o[hf].x, c.o[hf].y, c.o[hf].z, c.o[hf].w};
-                const float bv[4] = {c.bb[hf].x, c.bb[hf].y, c.bb[hf].z, c.bb[hf].w};
+                const float ov[4] = {ao[set][hf].x, ao[set][hf].y, ao[set][hf].z, ao[set][hf].w};
+                const float bv[4] = {ab[set][hf].x, ab[set][hf].y, ab[set][hf].z, ab[set][hf].w};
                 float av[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) av[q] = aok ? __fmul_rn(ov[q], bv[q]) : 0.0f;
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
                     if (g >= G) continue;
-                    const float xx[4] = {c.xv[hf][g].x, c.xv[hf][g].y, c.xv[hf][g].z, c.xv[hf][g].w};
-                    float xs[4];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) xs[q] = xok[g] ? __fmul_rn(xx[q], sx[g]) : 0.0f;
+                    const float4 x4 = *(const float4 *)&sx[buf][16 * g + kc][8 * kr + 4 * hf];
+                    const float xs[4] = {x4.x, x4.y, x4.z, x4.w};
 #pragma unroll
                     for (int q = 0; q < 4; ++q) part[g] = mfma_f32(av[q], xs[q], part[g]);
-                    if (NORMS) {
-                        const float jv[4] = {c.bj[hf].x, c.bj[hf].y, c.bj[hf].z, c.bj[hf].w};
-                        const float yy[4] = {c.yv[hf].x, c.yv[hf].y, c.yv[hf].z, c.yv[hf].w};
+                    if (norms) {
+                        const float4 j4 = *(const float4 *)&sjy[buf][0][8 * kr + 4 * hf];
+                        const float4 y4 = *(const float4 *)&sjy[buf][1][8 * kr + 4 * hf];
+                        const float jv[4] = {j4.x, j4.y, j4.z, j4.w};
+                        const float yy[4] = {y4.x, y4.y, y4.z, y4.w};
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
                             const double bx = (double)__fmul_rn(jv[q], xs[q]);
@@ -140,21 +164,37 @@ __device__ __forceinline__ void tc_item(const FitDev &F, const Item &I, int it, 
 #pragma unroll
                 for (int v = 0; v < 4; ++v) acc[g][v] += (double)part[g][v];
         };
-        // chunks alternate between two register sets; a missing next chunk reloads the current
-        // one (in bounds, unused)
-        TcChunk ca, cb;
-        long long r = I.r0;
-        if (r < I.r1) load(ca, r);
-#pragma unroll 1
-        while (r < I.r1) {
-            load(cb, r + kChunk < I.r1 ? r + kChunk : r);
-            compute(ca);
-            r += kChunk;
-            if (r >= I.r1) break;
-            load(ca, r + kChunk < I.r1 ? r + kChunk : r);
-            compute(cb);
-            r += kChunk;
+        // prologue: chunk 0 into buffer 0 and register set 0
+        float4 xv[NXL], jy;
+        const long long r0 = I.r0, r1 = I.r1;
+        if (r0 < r1) {
+            load_x(xv, jy, r0);
+            if (wact) load_a(0, r0);
+            store_x(xv, jy, 0);
         }
+        __syncthreads();
+        int c = 0;
+#pragma unroll 1
+        for (long long r = r0; r < r1; r += kChunk, ++c) {
+            const int cur = c & 1, nxt = cur ^ 1;
+            const bool more = r + kChunk < r1;
+            // a missing next chunk reloads the current one (in bounds, unused)
+            const long long rn = more ? r + kChunk : r;
+            load_x(xv, jy, rn);
+            if (wact) {
+                if (cur == 0) {
+                    load_a(1, rn);
+                    compute(0, cur);
+                } else {
+                    load_a(0, rn);
+                    compute(1, cur);
+                }
+            }
+            // buffer nxt last held chunk c-1, finished by every wave before the previous barrier
+            store_x(xv, jy, nxt);
+            __syncthreads();
+        }
+        const int it = bi * W + wv;   // sub-item of this wave
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             if (g >= G) continue;
@@ -164,7 +204,7 @@ __device__ __forceinline__ void tc_item(const FitDev &F, const Item &I, int it, 
             if (slot < I.nc)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) dst[v] = acc[g][v];
-            if (NORMS) {
+            if (norms) {
                 double s0 = bxx[g], s1 = bxy[g];
                 s0 = s0 + __shfl_xor(s0, 16, 64);
                 s0 = s0 + __shfl_xor(s0, 32, 64);
@@ -176,21 +216,7 @@ __device__ __forceinline__ void tc_item(const FitDev &F, const Item &I, int it, 
                 }
             }
         }
-    }
-}
-
-/* One wave per item; the A tile 0 items also form |Bx|^2 and Bx.y (a uniform branch per item). */
-__global__ __launch_bounds__(64, 2) void k_tc_update(const FitDev *__restrict__ fits,
-                                                     const Item *__restrict__ items, int nitems,
-                                                     double *__restrict__ tpart, double *__restrict__ npart)
-{
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-        const Item I = items[it];
-        const FitDev F = fits[I.fit];
-        if ((I.pad & 255) == 0)
-            tc_item<true>(F, I, it, tpart, npart);
-        else
-            tc_item<false>(F, I, it, tpart, npart);
+        __syncthreads();   // the next item restages sx
     }
 }
 
@@ -357,74 +383,65 @@ void tc_update(mars_hip_ctx *ctx)
             bytes += 4.0 * fs.n * (2.0 * (fs.mc + 1) + fs.px + 1);
         }
         if (work == 0) break;
-        // Row groups (fit, X-column block, row range) each carry one item per A tile.  Work-group i
-        // runs on XCD i % 8, so all tiles of a row group get item indices of one residue mod 8,
-        // close together in dispatch order: the later tiles find that row range of X in the XCD's
-        // L2.  Queues are filled shortest-first and padded with empty items.
-        struct RowGroup {
-            int f, nc, tiles;
-            size_t xc0;
-            long long r0, r1;
-        };
-        std::vector<RowGroup> rgs;
+        // Block items (fit, X-column block, group of W A tiles, row range), one wave per tile;
+        // every X chunk is staged once per block.  W: the largest tile count of the round,
+        // rounded up to a power of two (at most kTW; larger counts take several groups).
+        int umax = 1;
+        for (size_t f = 0; f < F; ++f)
+            if (jf[f] >= 0) umax = std::max(umax, (2 * jf[f] + 1 + 15) / 16);
+        const int W = umax <= 1 ? 1 : (umax <= 2 ? 2 : (umax <= 4 ? 4 : kTW));
+        double bwork = 0;
+        for (size_t f = 0; f < F; ++f) {
+            if (jf[f] < 0) continue;
+            const int tiles = (2 * jf[f] + 1 + 15) / 16;
+            bwork += (double)ctx->fits[f].n * ((tiles + W - 1) / W) * ((ctx->fits[f].px + kXT - 1) / kXT);
+        }
+        const double target = 8192.0 / W;   // about 8192 waves
+        std::vector<Item> items;
+        std::vector<int> groups, glist;
         for (size_t f = 0; f < F; ++f) {
             if (jf[f] < 0) continue;
             const FitState &fs = ctx->fits[f];
             const long long n = (long long)fs.n;
             const int tiles = (2 * jf[f] + 1 + 15) / 16;
-            long long splits = std::max<long long>(1, std::llround(8192.0 * n / work));
+            long long splits = std::max<long long>(1, std::llround(target * n / bwork));
             splits = std::min<long long>(splits, std::max<long long>(1, n / 256));
             const long long rows = ((n + splits - 1) / splits + kChunk - 1) / kChunk * kChunk;
-            for (size_t xc0 = 0; xc0 < fs.px; xc0 += kXT)
-                for (long long r0 = 0; r0 < n; r0 += rows)
-                    rgs.push_back(RowGroup{(int)f, (int)std::min<size_t>(kXT, fs.px - xc0), tiles, xc0, r0,
-                                           std::min(n, r0 + rows)});
-        }
-        constexpr int kXcd = 8;
-        std::vector<size_t> qlen(kXcd, 0);
-        std::vector<int> rq(rgs.size());
-        std::vector<size_t> rpos(rgs.size());
-        for (size_t g = 0; g < rgs.size(); ++g) {
-            const int q = (int)(std::min_element(qlen.begin(), qlen.end()) - qlen.begin());
-            rq[g] = q;
-            rpos[g] = qlen[q];
-            qlen[q] += rgs[g].tiles;
-        }
-        const size_t qmax = *std::max_element(qlen.begin(), qlen.end());
-        std::vector<Item> items(kXcd * qmax, Item{0, 0, 0, 0, 0, 0});  // padding: no rows, no columns
-        for (size_t g = 0; g < rgs.size(); ++g) {
-            const RowGroup &R = rgs[g];
-            for (int u = 0; u < R.tiles; ++u)
-                items[kXcd * (rpos[g] + u) + rq[g]] = Item{R.f, (int)R.xc0, R.nc, (jf[R.f] << 8) | u, R.r0, R.r1};
-        }
-        // reduce groups (fit, X-column block, A tile): their items in row order, listed after the headers
-        std::vector<int> groups, glist;
-        for (size_t g0 = 0; g0 < rgs.size();) {
-            size_t g1 = g0;
-            while (g1 < rgs.size() && rgs[g1].f == rgs[g0].f && rgs[g1].xc0 == rgs[g0].xc0) ++g1;
-            for (int u = 0; u < rgs[g0].tiles; ++u) {
-                const int first = (int)glist.size();
-                for (size_t g = g0; g < g1; ++g) glist.push_back((int)(kXcd * (rpos[g] + u) + rq[g]));
-                const int h[kGroupInts] = {rgs[g0].f, (int)rgs[g0].xc0, rgs[g0].nc, jf[rgs[g0].f], u, first,
-                                           (int)(g1 - g0)};
-                groups.insert(groups.end(), h, h + kGroupInts);
+            for (size_t xc0 = 0; xc0 < fs.px; xc0 += kXT) {
+                const int nc = (int)std::min<size_t>(kXT, fs.px - xc0);
+                for (int u0 = 0; u0 < tiles; u0 += W) {
+                    const int first_block = (int)items.size();
+                    for (long long r0 = 0; r0 < n; r0 += rows)
+                        items.push_back(Item{(int)f, (int)xc0, nc, (jf[f] << 8) | u0, r0, std::min(n, r0 + rows)});
+                    const int nb = (int)items.size() - first_block;
+                    for (int u = u0; u < std::min(tiles, u0 + W); ++u) {
+                        const int h[kGroupInts] = {(int)f, (int)xc0, nc, jf[f], u, (int)glist.size(), nb};
+                        groups.insert(groups.end(), h, h + kGroupInts);
+                        for (int bb = 0; bb < nb; ++bb) glist.push_back((first_block + bb) * W + (u - u0));
+                    }
+                }
             }
-            g0 = g1;
         }
         const size_t nh = groups.size();
         groups.insert(groups.end(), glist.begin(), glist.end());
-        const size_t ni = items.size(), ng = nh / kGroupInts;
+        const size_t ni = items.size(), ng = nh / kGroupInts, nsub = ni * W;
         ctx->d_items.ensure(ni);
         ctx->d_cmap.ensure(groups.size());
         h2d(ctx, ctx->d_items.p, items.data(), ni * sizeof(Item));
         h2d(ctx, ctx->d_cmap.p, groups.data(), groups.size() * sizeof(int));
-        ctx->tpart.ensure(ni * kXT * 16);
-        ctx->aux.ensure(ni * kXT * 2);
+        ctx->tpart.ensure(nsub * kXT * 16);
+        ctx->aux.ensure(nsub * kXT * 2);
         {
             KTimer kt(ctx, K_MIXED_A, flops, bytes);  // the update kernel alone
             const dim3 grid((unsigned)std::max<size_t>(1, std::min<size_t>(ni, 65536)));
-            hipLaunchKernelGGL(k_tc_update, grid, dim3(64), 0, st, (const FitDev *)ctx->d_fits.p,
-                               (const Item *)ctx->d_items.p, (int)ni, ctx->tpart.p, ctx->aux.p);
+            const Item *di = (const Item *)ctx->d_items.p;
+            const FitDev *dfit = (const FitDev *)ctx->d_fits.p;
+            switch (W) {
+            case 1: hipLaunchKernelGGL(k_tc_update<1>, grid, dim3(64), 0, st, dfit, di, (int)ni, ctx->tpart.p, ctx->aux.p); break;
+            case 2: hipLaunchKernelGGL(k_tc_update<2>, grid, dim3(128), 0, st, dfit, di, (int)ni, ctx->tpart.p, ctx->aux.p); break;
+            case 4: hipLaunchKernelGGL(k_tc_update<4>, grid, dim3(256), 0, st, dfit, di, (int)ni, ctx->tpart.p, ctx->aux.p); break;
+            default: hipLaunchKernelGGL(k_tc_update<kTW>, grid, dim3(64 * kTW), 0, st, dfit, di, (int)ni, ctx->tpart.p, ctx->aux.p); break;
+            }
             MH_LAUNCH_CHECK();
         }
         hipLaunchKernelGGL(k_tc_reduce, dim3((unsigned)ng), dim3(256), 0, st, (const FitDev *)ctx->d_fits.p,
