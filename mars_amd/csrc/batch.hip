@@ -18,6 +18,7 @@ namespace {
 struct FitDriver {
     size_t max_terms = 0;            // after the dof clamp (mars.py:241-243)
     std::vector<std::vector<int>> vars;  // variables of each basis function
+    std::vector<unsigned char> has;  // has[b * p_x + i]: variable i occurs in basis function b
     std::vector<mars_hip_term> rows;
     bool active = true;
 };
@@ -37,6 +38,7 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
         require(mt >= 1, "too few rows for any term");
         D[f].max_terms = mt;
         D[f].vars.assign(1, std::vector<int>());
+        D[f].has.assign(fs.px, 0);
         mars_hip_term t0{'i', 0, 0, 0, 0.0, 0.0f, 0.0f};
         D[f].rows.assign(1, t0);
         D[f].active = fs.m < mt;
@@ -46,6 +48,10 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
     std::vector<BestOut> best;
     while (true) {
         HostTimer step_timer(ctx, K_H_STEP);
+        // the mixed form's cache update runs on the device while the host builds the candidates
+        bool live = false;
+        for (size_t f = 0; f < F; ++f) live = live || (D[f].active && ctx->fits[f].m < D[f].max_terms);
+        if (live && ctx->form == MARS_HIP_FORM_MIXED) mixed_prepare(ctx);
         HostTimer cands_timer(ctx, K_H_CANDS);
         Cands &cs = ctx->cand;
         cs.fit.clear();
@@ -58,16 +64,14 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
             if (!d.active) continue;
             any = true;
             const FitState &fs = ctx->fits[f];
-            for (size_t i = 0; i < fs.px; ++i)
-                for (size_t b = 0; b < fs.m; ++b) {
-                    const std::vector<int> &v = d.vars[b];
-                    const bool has = std::find(v.begin(), v.end(), (int)i) != v.end();
-                    if ((int)v.size() < o.max_degree && (o.self_interact || !has)) {
+            const size_t px = fs.px;
+            for (size_t i = 0; i < px; ++i)
+                for (size_t b = 0; b < fs.m; ++b)
+                    if ((int)d.vars[b].size() < o.max_degree && (o.self_interact || !d.has[b * px + i])) {
                         cs.fit.push_back((int)f);
                         cs.xc.push_back((int)i);
                         cs.par.push_back((int)b);
                     }
-                }
         }
         if (!any) {
             step_timer.cancel();
@@ -141,6 +145,8 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
             t.r2 = (float)(1.0 - mse / fs.yvar);
             t.r2_cv = (float)(1.0 - gm / fs.yvar);
             d.vars.push_back(v);
+            d.has.resize(d.has.size() + fs.px, 0);
+            for (int x : v) d.has[(d.vars.size() - 1) * fs.px + x] = 1;
             d.rows.push_back(t);
             // stopping rules (mars.py:395-406)
             // numpy compares the float32 model fields against Python floats in float32

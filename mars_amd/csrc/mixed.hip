@@ -455,6 +455,14 @@ void tc_update(mars_hip_ctx *ctx)
 
 } // namespace
 
+void mixed_prepare(mars_hip_ctx *ctx)
+{
+    ensure_tcache(ctx);
+    Cands none;  // the cache update reads no candidate list
+    upload_fits(ctx, none);
+    tc_update(ctx);
+}
+
 void mixed_score(mars_hip_ctx *ctx)
 {
     Cands &cs = ctx->cand;
