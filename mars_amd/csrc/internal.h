@@ -17,10 +17,13 @@ struct Pass {
  * marsalgo.cc:500-505).  Writes per candidate c: ybx[c] and lin[c] = ybx^2
  * (linear delta-SSE); with keep_bxn, the normalized f32 column stays in its
  * fit's rbuf (row-major, column c - c0).  `xover`/`sover`, when set, replace
- * fit 0's X and scales (mirror-mode single-column calls).
+ * fit 0's X and scales (mirror-mode single-column calls).  `rescore` (a few
+ * candidates per fit, the mixed form's exact rescoring): the Sum r^2 pass
+ * keeps every residual, so the DGKS retry and the final pass read r instead
+ * of recomputing it from Bo.
  */
 void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, double *ybx,
-                 const float *xover = nullptr, const float *sover = nullptr);
+                 const float *xover = nullptr, const float *sover = nullptr, bool rescore = false);
 
 /* Mixed-precision scoring of ctx->cand (linear delta-SSE only) from each
  * fit's incrementally maintained T cache (FP32 MFMA, FP64 across row chunks),

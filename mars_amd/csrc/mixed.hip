@@ -527,7 +527,7 @@ void mixed_score(mars_hip_ctx *ctx)
     }
     ctx->rlin.ensure(R);
     ctx->rybx.ensure(R);
-    plain_score(ctx, rc, false, ctx->rlin.p, ctx->rybx.p);
+    plain_score(ctx, rc, false, ctx->rlin.p, ctx->rybx.p, nullptr, nullptr, true);
     hipLaunchKernelGGL(k_put, dim3((R + 255) / 256), dim3(256), 0, st, (const int *)ctx->rsel.p,
                        (const double *)ctx->rlin.p, R, ctx->lin.p);
     MH_LAUNCH_CHECK();

@@ -357,8 +357,9 @@ void upload_fits(mars_hip_ctx *ctx, const Cands &cs, const float *xover, const f
 }
 
 void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, double *ybx,
-                 const float *xover, const float *sover)
+                 const float *xover, const float *sover, bool rescore)
 {
+    const bool store_r = keep_bxn || rescore;  // every residual stays in rbuf
     const int P = (int)cs.fit.size();
     if (P == 0) return;
     hipStream_t st = ctx->stream;
@@ -400,8 +401,8 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
 
     Pass pT = make_pass(fb, fe, n, split_rows(n, cnt, tileT, 8192, 256), tileT, kWR);
     Pass pP = make_pass(fb, fe, n, split_rows(n, cnt, tileP, 8192, 256), tileP, kWR);
-    Pass pF;  // finalize work list: only the hinge path stores normalized columns
-    if (keep_bxn) pF = make_pass(fb, fe, n, split_rows(n, cnt, kTileF, 2048, 512), kTileF, 32);
+    Pass pF;  // finalize work list, over stored residuals (hinge path, rescoring)
+    if (store_r) pF = make_pass(fb, fe, n, split_rows(n, cnt, kTileF, 2048, 512), kTileF, 32);
     Uploaded uT, uP, uF;
     size_t io = 0, co = 0;
     const size_t total_items = pT.items.size() + pP.items.size() + pF.items.size();
@@ -434,7 +435,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     // hinge sweep (which needs the normalized columns)
     {
         KTimer kt(ctx, K_PROJECT, flops, bytes + 0.0);
-        if (keep_bxn)
+        if (store_r)
             launch_project<kStoreR, false, false>(ctx, mt, uP, (int)pP.items.size(), cs, nullptr, ctx->T.p, ldT,
                                                   narrow);
         else
@@ -480,7 +481,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         ctx->tpart.ensure(std::max<size_t>(rT.items.size(), 1) * kSlots * ldT);
         ctx->spart.ensure(rP.items.size() * kSlots + 1);
         ctx->T2.ensure((size_t)nf * ldT);
-        if (!keep_bxn)  // the first residual of the fired columns, for the retry
+        if (!store_r)  // the first residual of the fired columns, for the retry
             launch_project<kStoreR, false, true>(ctx, mt, vP, (int)rP.items.size(), cs, ctx->fired_list.p,
                                                  ctx->T.p, ldT, rnarrow);
         launch_tgemm<true>(ctx, mt, vT, (int)rT.items.size(), cs, ctx->fired_list.p, ldT, rnarrow);
@@ -498,10 +499,15 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     {
         double fb2 = 0;
         for (size_t f = 0; f < F; ++f) fb2 += (keep_bxn ? 8.0 : 4.0) * n[f] * cnt[f] + 4.0 * n[f];
-        KTimer kt(ctx, K_FINALIZE, keep_bxn ? 0.0 : flops, fb2);
-        if (keep_bxn) {
-            hipLaunchKernelGGL(k_finalize<true>, dim3(grid_for(pF.items.size())), dim3(256), 0, st,
-                               ctx->d_fits.p, uF.items, (int)pF.items.size(), ctx->ssum.p, ctx->spart.p);
+        KTimer kt(ctx, K_FINALIZE, store_r ? 0.0 : flops, fb2);
+        if (store_r) {
+            // Bxn = f32(r scale), ybx = sum Bxn y; a candidate with s <= tol has scale 0, so ybx = 0
+            if (keep_bxn)
+                hipLaunchKernelGGL(k_finalize<true>, dim3(grid_for(pF.items.size())), dim3(256), 0, st,
+                                   ctx->d_fits.p, uF.items, (int)pF.items.size(), ctx->ssum.p, ctx->spart.p);
+            else
+                hipLaunchKernelGGL(k_finalize<false>, dim3(grid_for(pF.items.size())), dim3(256), 0, st,
+                                   ctx->d_fits.p, uF.items, (int)pF.items.size(), ctx->ssum.p, ctx->spart.p);
             MH_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, P, uF.cmap, ctx->spart.p,
                                nullptr, ybx, lin);
