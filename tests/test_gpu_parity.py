@@ -206,6 +206,43 @@ def test_eval_arrays_vs_oracle(oracle):
     assert best["knot2"] == k0[j2]
 
 
+@pytest.mark.parametrize("n,p,mt,widest", [(6000, 6, 61, 32), (8000, 6, 118, 64)])
+def test_wide_parent_blocks_vs_oracle(oracle, n, p, mt, widest):
+    """Hinge fits whose late steps score more than 32 parents per X column (two 32-candidate
+    sweep blocks per column; beyond 64, three) against the oracle: same terms and knots, mse
+    within 1e-10.  The stopping rules are off so that the fit runs to max_terms.  The second
+    fit stops at 118 terms: term 118 would be x2*B80, a column the basis already spans (its
+    residual is 1.2e-7 of its norm, the f32 rounding of the product), whose delta-SSE fits
+    rounding noise; and the step after ranks x1*B20 against x2*B27, one product (x1 x2 x3 x4 h)
+    formed in two orders, so that f32 roundings alone separate the two scores."""
+    import mars_amd
+    from mars_amd import datagen
+    from oracle import fit as F
+
+    say("wide parent blocks n=%d p=%d max_terms=%d" % (n, p, mt))
+    X, y = datagen.workload2(n=n, p=p)
+    kw = dict(max_terms=mt, linear_only=False, max_degree=5, r2_thresh=0.0, r2_window=1000)
+    want, wtrace = F.fit(oracle, X, y, None, **kw)
+    trace = []
+    got = mars_amd.fit(X, y, trace=trace, **kw)
+    variables = [set()]
+    for r in got[1:]:
+        variables.append(variables[int(r["basis"])] | {int(r["input"])})
+    # parents scored at the last step: every term but the last two (one step appends at most two)
+    eligible = max(sum(1 for v in variables[:-2] if len(v) < 5 and i not in v) for i in range(p))
+    assert eligible > widest, eligible
+    seq = lambda mdl: [(r["type"], int(r["basis"]), int(r["input"]),
+                        None if np.isnan(r["hinge"]) else float(r["hinge"])) for r in mdl]
+    assert seq(got) == seq(want)
+    gm = [v for t in trace for v in t["mse"]]
+    wm = [v for t in wtrace for v in t["mse"]]
+    assert len(gm) == len(wm)
+    # mse is (1 - sum of squared projections of y) / n with y scaled to unit sum of squares: its
+    # rounding scales with 1/n, and late in the fit mse falls to 6% of that
+    for a, b in zip(gm, wm):
+        assert abs(a - b) <= 1e-10 * (abs(b) + 1.0 / n), (a, b)
+
+
 def test_append_basis_vs_oracle(oracle):
     import mars_amd
 
