@@ -72,11 +72,13 @@ struct FitDev {
     const float *BC;   // basis B, column-major (ldx)
     const float *y;    // normalized target (f32)
     const double *ybo; // Bo^T y per live column
-    float *rbuf;       // residual / normalized candidate columns, row-major (ldr)
+    float *rbuf;       // residual / normalized candidate columns: element (row, col) at
+                       // row * rsr + col * rsc (row-major, rsr = ldr, for the hinge sweep and
+                       // column downloads; column-major, rsc = ldx, for linear scoring)
     double *v;         // append: f64 residual of the new column (n)
     double *tc;        // mixed form: cached T, [xc][par][k] (ldb x ldb per X column)
     double *tcn;       // mixed form: cached |Bx|^2, Bx.y per (xc, par)
-    long long n, ldx, ldb, ldr;
+    long long n, ldx, ldb, rsr, rsc;
     int m, c0;         // live basis count, first candidate of this fit
     double tol;
 };
@@ -198,6 +200,8 @@ struct mars_hip_ctx {
     mh::Cands rcand;                 // mixed form: candidates rescored exactly
     mh::DBuf<double> rlin, rybx;
     mh::DBuf<int> rsel;              // their positions in the scored list
+    bool rbuf_cm = false;            // candidate columns in rbuf are column-major (linear scoring)
+    int rsel_hint = 0;               // the previous rescoring list's length
     mh::DBuf<mh::FitDev> d_fits;
     mh::DBuf<mh::Item> d_items;
     mh::DBuf<int> d_cmap;            // per-candidate first item / item count per pass

@@ -305,42 +305,97 @@ __global__ __launch_bounds__(256) void k_fit_max(const int *__restrict__ c0, con
 
 /*
  * Rescoring list, in candidate order: every gated candidate and every ungated one whose
- * approximate score is within the window of its fit's best ungated one.  One block of 1024
- * threads, each owning a contiguous range; out_n[0] = list length, out_n[1] = gated count.
+ * approximate score is within the window of its fit's best ungated one.  Two passes over
+ * tiles of kSelTile candidates (256 threads, 4 coalesced candidates each): k_select_count
+ * writes each tile's list and gated counts, k_select_write places the tile's entries after
+ * those of the tiles before it; out_n[0] = list length, out_n[1] = gated count.
  */
-__global__ __launch_bounds__(1024) void k_select(int P, const int *__restrict__ cfit,
-                                                 const double *__restrict__ score, const int *__restrict__ flag,
-                                                 const double *__restrict__ mx, double window,
-                                                 int *__restrict__ list, int *__restrict__ out_n)
+constexpr int kSelTile = 1024;
+
+__device__ inline bool sel_take(int c, const int *cfit, const double *score, const int *flag, const double *mx,
+                                double window)
 {
-    __shared__ int sc[1024], sg[1024];
-    const int t = threadIdx.x;
-    const int per = (P + 1023) / 1024, b = t * per, e = min(P, b + per);
-    auto take = [&](int c) {
-        const double m = mx[cfit[c]];
-        return flag[c] != 0 || (m > 0.0 && score[c] >= m * (1.0 - window));
-    };
+    const double m = mx[cfit[c]];
+    return flag[c] != 0 || (m > 0.0 && score[c] >= m * (1.0 - window));
+}
+
+__device__ inline int wave_sum_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_select_count(int P, const int *__restrict__ cfit,
+                                                      const double *__restrict__ score,
+                                                      const int *__restrict__ flag, const double *__restrict__ mx,
+                                                      double window, int *__restrict__ blk)
+{
+    __shared__ int sn[4], sg[4];
+    const int t = threadIdx.x, base = blockIdx.x * kSelTile;
     int n = 0, g = 0;
-    for (int c = b; c < e; ++c) {
-        n += take(c) ? 1 : 0;
-        g += flag[c] != 0 ? 1 : 0;
+#pragma unroll
+    for (int i = 0; i < kSelTile / 256; ++i) {
+        const int c = base + 256 * i + t;
+        if (c < P) {
+            n += sel_take(c, cfit, score, flag, mx, window) ? 1 : 0;
+            g += flag[c] != 0 ? 1 : 0;
+        }
     }
-    sc[t] = n;
-    sg[t] = g;
+    n = wave_sum_i(n);
+    g = wave_sum_i(g);
+    if ((t & 63) == 0) {
+        sn[t >> 6] = n;
+        sg[t >> 6] = g;
+    }
     __syncthreads();
-    for (int s = 1; s < 1024; s <<= 1) {  // inclusive scan (Hillis-Steele)
-        const int a = t >= s ? sc[t - s] : 0, ag = t >= s ? sg[t - s] : 0;
-        __syncthreads();
-        sc[t] += a;
-        sg[t] += ag;
-        __syncthreads();
+    if (t == 0) {
+        blk[2 * blockIdx.x] = sn[0] + sn[1] + sn[2] + sn[3];
+        blk[2 * blockIdx.x + 1] = sg[0] + sg[1] + sg[2] + sg[3];
     }
-    int o = sc[t] - n;
-    for (int c = b; c < e; ++c)
-        if (take(c)) list[o++] = c;
-    if (t == 1023) {
-        out_n[0] = sc[1023];
-        out_n[1] = sg[1023];
+}
+
+__global__ __launch_bounds__(256) void k_select_write(int P, const int *__restrict__ cfit,
+                                                      const double *__restrict__ score,
+                                                      const int *__restrict__ flag, const double *__restrict__ mx,
+                                                      double window, const int *__restrict__ blk,
+                                                      int *__restrict__ list, int *__restrict__ out_n)
+{
+    __shared__ int sw[4], sg[4];
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63, b = blockIdx.x, base = b * kSelTile;
+    // this tile's offset: the list counts of the tiles before it (the last tile also sums the gated counts)
+    int off = 0, gat = 0;
+    const bool last = b == (int)gridDim.x - 1;
+    for (int j = t; j < b; j += 256) {
+        off += blk[2 * j];
+        if (last) gat += blk[2 * j + 1];
+    }
+    off = wave_sum_i(off);
+    gat = wave_sum_i(gat);
+    if (lane == 0) {
+        sw[w] = off;
+        sg[w] = gat;
+    }
+    __syncthreads();
+    int run = sw[0] + sw[1] + sw[2] + sw[3];
+    if (last && t == 0) {
+        out_n[0] = run + blk[2 * b];
+        out_n[1] = sg[0] + sg[1] + sg[2] + sg[3] + blk[2 * b + 1];
+    }
+    __syncthreads();
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll 1
+    for (int i = 0; i < kSelTile / 256; ++i) {
+        const int c = base + 256 * i + t;
+        const bool tk = c < P && sel_take(c, cfit, score, flag, mx, window);
+        const unsigned long long m = __ballot(tk);
+        if (lane == 0) sw[w] = __popcll(m);
+        __syncthreads();
+        int woff = run;
+        for (int k = 0; k < w; ++k) woff += sw[k];
+        if (tk) list[woff + __popcll(m & below)] = c;
+        run += sw[0] + sw[1] + sw[2] + sw[3];
+        __syncthreads();
     }
 }
 
@@ -493,29 +548,45 @@ void mixed_score(mars_hip_ctx *ctx)
         ranges[f] = cs.fit_c0[f];
         ranges[F + f] = cs.fit_P[f];
     }
-    ctx->ival.ensure(2 * F + 2);
+    const int nblk = (P + kSelTile - 1) / kSelTile;
+    ctx->ival.ensure(2 * F + 2 + 2 * (size_t)nblk);
     ctx->red.ensure(F);
     ctx->rsel.ensure(P);
+    int *d_nsel = ctx->ival.p + 2 * F, *d_blk = d_nsel + 2;
     h2d(ctx, ctx->ival.p, ranges.data(), 2 * F * sizeof(int));
     hipLaunchKernelGGL(k_fit_max, dim3((unsigned)F), dim3(256), 0, st, (const int *)ctx->ival.p,
                        (const int *)ctx->ival.p + F, (const double *)ctx->lin.p, (const int *)ctx->flag.p, ctx->red.p);
     MH_LAUNCH_CHECK();
     // An exact gated score above a fit's best ungated approximation only raises the maximum,
     // so the window taken here contains the final one.
-    hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, st, P, (const int *)cs.d_fit.p, (const double *)ctx->lin.p,
-                       (const int *)ctx->flag.p, (const double *)ctx->red.p, kWindow, ctx->rsel.p,
-                       ctx->ival.p + 2 * F);
+    hipLaunchKernelGGL(k_select_count, dim3((unsigned)nblk), dim3(256), 0, st, P, (const int *)cs.d_fit.p,
+                       (const double *)ctx->lin.p, (const int *)ctx->flag.p, (const double *)ctx->red.p, kWindow,
+                       d_blk);
     MH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_select_write, dim3((unsigned)nblk), dim3(256), 0, st, P, (const int *)cs.d_fit.p,
+                       (const double *)ctx->lin.p, (const int *)ctx->flag.p, (const double *)ctx->red.p, kWindow,
+                       (const int *)d_blk, ctx->rsel.p, d_nsel);
+    MH_LAUNCH_CHECK();
+    // the list length and a prefix of the list sized from the previous step come back together;
+    // a longer list takes a second copy
     int nsel[2] = {0, 0};
-    MH_CHECK(hipMemcpyAsync(nsel, ctx->ival.p + 2 * F, sizeof(nsel), hipMemcpyDeviceToHost, st));
+    const int guess = std::min(P, std::max(2 * ctx->rsel_hint, 1024));
+    std::vector<int> sub(guess);
+    MH_CHECK(hipMemcpyAsync(nsel, d_nsel, sizeof(nsel), hipMemcpyDeviceToHost, st));
+    MH_CHECK(hipMemcpyAsync(sub.data(), ctx->rsel.p, guess * sizeof(int), hipMemcpyDeviceToHost, st));
     ctx_sync(ctx);
     const int R = nsel[0];
+    require(R >= 0 && R <= P, "rescoring list length out of range");
+    ctx->rsel_hint = R;
     ctx->gated += nsel[1];
     ctx->rescored += R;
     if (R == 0) return;
-    std::vector<int> sub(R);
-    MH_CHECK(hipMemcpyAsync(sub.data(), ctx->rsel.p, R * sizeof(int), hipMemcpyDeviceToHost, st));
-    ctx_sync(ctx);
+    if (R > guess) {
+        sub.resize(R);
+        MH_CHECK(hipMemcpyAsync(sub.data() + guess, ctx->rsel.p + guess, (R - guess) * sizeof(int),
+                                hipMemcpyDeviceToHost, st));
+        ctx_sync(ctx);
+    }
     Cands &rc = ctx->rcand;
     rc.fit.resize(R);
     rc.xc.resize(R);

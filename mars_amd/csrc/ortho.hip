@@ -73,18 +73,20 @@ __global__ void k_sreduce(const int *__restrict__ list, int count, const int *__
 
 /*
  * Normalize and project on y: Bxn = f32(r * scale), ybx partial = sum Bxn y.
- * 256 threads = 4 row phases x 64 candidates: thread (g, slot) takes rows
- * r0 + g, r0 + g + 4, ... (eight loads in flight), and the four phase sums
- * are added in a fixed order.
+ * 256 threads = 256/TF row phases x TF candidates: thread (g, slot) takes rows
+ * r0 + g, r0 + g + 256/TF, ... (eight loads in flight), and the phase sums are
+ * added in a fixed order.  TF = 16 serves small candidate sets (the mixed form's
+ * rescoring), TF = kTileF the others.
  */
-template <bool STORE>
+template <bool STORE, int TF>
 __global__ __launch_bounds__(256) void k_finalize(const FitDev *__restrict__ fits,
                                                   const Item *__restrict__ items, int nitems,
                                                   const double *__restrict__ ssum,
                                                   double *__restrict__ ypart)
 {
-    __shared__ double sh[4][kTileF];
-    const int slot = threadIdx.x & (kTileF - 1), g = threadIdx.x / kTileF;
+    constexpr int NPH = 256 / TF;
+    __shared__ double sh[NPH][TF];
+    const int slot = threadIdx.x % TF, g = threadIdx.x / TF;
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const Item I = items[it];
         double acc = 0.0;
@@ -94,33 +96,84 @@ __global__ __launch_bounds__(256) void k_finalize(const FitDev *__restrict__ fit
             const long long col = c - F.c0;
             const double s = ssum[c];
             const float scale = (s > F.tol) ? (float)(1.0 / sqrt(s + F.tol)) : 0.0f;
-            float *base = F.rbuf + col;
+            float *base = F.rbuf + col * F.rsc;
             long long row = I.r0 + g;
-            for (; row + 28 < I.r1; row += 32) {
+            for (; row + 7 * NPH < I.r1; row += 8 * NPH) {
                 float r[8], yv[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    r[u] = base[(row + 4 * u) * F.ldr];
-                    yv[u] = F.y[row + 4 * u];
+                    r[u] = base[(row + NPH * u) * F.rsr];
+                    yv[u] = F.y[row + NPH * u];
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     const float bxn = __fmul_rn(r[u], scale);
-                    if (STORE) base[(row + 4 * u) * F.ldr] = bxn;
+                    if (STORE) base[(row + NPH * u) * F.rsr] = bxn;
                     acc = fma((double)bxn, (double)yv[u], acc);
                 }
             }
-            for (; row < I.r1; row += 4) {
-                const float bxn = __fmul_rn(base[row * F.ldr], scale);
-                if (STORE) base[row * F.ldr] = bxn;
+            for (; row < I.r1; row += NPH) {
+                const float bxn = __fmul_rn(base[row * F.rsr], scale);
+                if (STORE) base[row * F.rsr] = bxn;
                 acc = fma((double)bxn, (double)F.y[row], acc);
             }
         }
         sh[g][slot] = acc;
         __syncthreads();
-        if (g == 0 && slot < I.nc)
-            ypart[(size_t)it * kTileF + slot] = ((sh[0][slot] + sh[1][slot]) + sh[2][slot]) + sh[3][slot];
+        if (g == 0 && slot < I.nc) {
+            double t = sh[0][slot];
+#pragma unroll
+            for (int q = 1; q < NPH; ++q) t += sh[q][slot];
+            ypart[(size_t)it * kTileF + slot] = t;
+        }
         __syncthreads();
+    }
+}
+
+/*
+ * The same over column-major residuals (linear scoring): wave w of 4 takes the item's slots
+ * w, w + 4, ...; its lanes read the slot's column as float4 (rows r0 + 4 lane + 256 u, four
+ * in flight), and the wave's sum is formed in a fixed order.
+ */
+__global__ __launch_bounds__(256) void k_finalize_cm(const FitDev *__restrict__ fits,
+                                                     const Item *__restrict__ items, int nitems,
+                                                     const double *__restrict__ ssum,
+                                                     double *__restrict__ ypart)
+{
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const Item I = items[it];
+        const FitDev F = fits[I.fit];
+        for (int slot = w; slot < I.nc; slot += 4) {
+            const int c = I.c0 + slot;
+            const double s = ssum[c];
+            const float scale = (s > F.tol) ? (float)(1.0 / sqrt(s + F.tol)) : 0.0f;
+            const float *col = F.rbuf + (c - F.c0) * F.rsc;
+            double acc = 0.0;
+            for (long long r = I.r0 + 4 * lane; r < I.r1; r += 1024) {
+                float4 rv[4], yv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const long long rr = r + 256 * u;
+                    const long long ra = rr < I.r1 ? rr : r;  // clamped: stays inside the column
+                    rv[u] = gld4(col + ra);
+                    yv[u] = gld4(F.y + ra);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const long long rr = r + 256 * u;
+                    if (rr >= I.r1) continue;
+                    const float a[4] = {rv[u].x, rv[u].y, rv[u].z, rv[u].w};
+                    const float b[4] = {yv[u].x, yv[u].y, yv[u].z, yv[u].w};
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        if (rr + v < I.r1) acc = fma((double)__fmul_rn(a[v], scale), (double)b[v], acc);
+                }
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+            if (lane == 0) ypart[(size_t)it * kTileF + slot] = acc;
+        }
     }
 }
 
@@ -316,7 +369,7 @@ void ensure_rbuf(mars_hip_ctx *ctx, const Cands &cs)
             // grow geometrically in columns so the buffer is reallocated O(log) times per fit
             const size_t ldr = std::max(need, std::min<size_t>(2 * fs.ldr, ctx->max_terms * fs.px + 64));
             fs.rbuf.release();
-            fs.rbuf.ensure(fs.n * ldr);
+            fs.rbuf.ensure(fs.ldx * ldr);  // room for either layout (column-major uses ldx rows)
             fs.ldr = ldr;
         }
     }
@@ -347,7 +400,8 @@ void upload_fits(mars_hip_ctx *ctx, const Cands &cs, const float *xover, const f
         d.tcn = fs.tcn.p;
         d.n = (long long)fs.n;
         d.ldb = (long long)ctx->ldb;
-        d.ldr = (long long)fs.ldr;
+        d.rsr = ctx->rbuf_cm ? 1 : (long long)fs.ldr;
+        d.rsc = ctx->rbuf_cm ? (long long)fs.ldx : 1;
         d.m = (int)fs.m;
         d.c0 = cs.fit_c0.empty() ? 0 : cs.fit_c0[f];
         d.tol = fs.tol;
@@ -366,6 +420,9 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     const size_t F = ctx->fits.size();
     index_cands(ctx, cs);
     ensure_rbuf(ctx, cs);
+    // normalised columns kept for the hinge sweep or a download stay row-major; residuals that
+    // only linear scoring reads are column-major (contiguous per candidate)
+    ctx->rbuf_cm = !keep_bxn;
     upload_fits(ctx, cs, xover, sover);
     upload_cands(ctx, cs);
 
@@ -402,7 +459,8 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     Pass pT = make_pass(fb, fe, n, split_rows(n, cnt, tileT, 8192, 256), tileT, kWR);
     Pass pP = make_pass(fb, fe, n, split_rows(n, cnt, tileP, 8192, 256), tileP, kWR);
     Pass pF;  // finalize work list, over stored residuals (hinge path, rescoring)
-    if (store_r) pF = make_pass(fb, fe, n, split_rows(n, cnt, kTileF, 2048, 512), kTileF, 32);
+    const int tileF = narrow ? 16 : kTileF;
+    if (store_r) pF = make_pass(fb, fe, n, split_rows(n, cnt, tileF, 2048, 512), tileF, 32);
     Uploaded uT, uP, uF;
     size_t io = 0, co = 0;
     const size_t total_items = pT.items.size() + pP.items.size() + pF.items.size();
@@ -502,12 +560,23 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         KTimer kt(ctx, K_FINALIZE, store_r ? 0.0 : flops, fb2);
         if (store_r) {
             // Bxn = f32(r scale), ybx = sum Bxn y; a candidate with s <= tol has scale 0, so ybx = 0
-            if (keep_bxn)
-                hipLaunchKernelGGL(k_finalize<true>, dim3(grid_for(pF.items.size())), dim3(256), 0, st,
-                                   ctx->d_fits.p, uF.items, (int)pF.items.size(), ctx->ssum.p, ctx->spart.p);
+            const dim3 gF(grid_for(pF.items.size()));
+            const int nF = (int)pF.items.size();
+            if (ctx->rbuf_cm)
+                hipLaunchKernelGGL(k_finalize_cm, gF, dim3(256), 0, st, ctx->d_fits.p, uF.items, nF, ctx->ssum.p,
+                                   ctx->spart.p);
+            else if (keep_bxn && narrow)
+                hipLaunchKernelGGL((k_finalize<true, 16>), gF, dim3(256), 0, st, ctx->d_fits.p, uF.items, nF,
+                                   ctx->ssum.p, ctx->spart.p);
+            else if (keep_bxn)
+                hipLaunchKernelGGL((k_finalize<true, kTileF>), gF, dim3(256), 0, st, ctx->d_fits.p, uF.items, nF,
+                                   ctx->ssum.p, ctx->spart.p);
+            else if (narrow)
+                hipLaunchKernelGGL((k_finalize<false, 16>), gF, dim3(256), 0, st, ctx->d_fits.p, uF.items, nF,
+                                   ctx->ssum.p, ctx->spart.p);
             else
-                hipLaunchKernelGGL(k_finalize<false>, dim3(grid_for(pF.items.size())), dim3(256), 0, st,
-                                   ctx->d_fits.p, uF.items, (int)pF.items.size(), ctx->ssum.p, ctx->spart.p);
+                hipLaunchKernelGGL((k_finalize<false, kTileF>), gF, dim3(256), 0, st, ctx->d_fits.p, uF.items,
+                                   nF, ctx->ssum.p, ctx->spart.p);
             MH_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, P, uF.cmap, ctx->spart.p,
                                nullptr, ybx, lin);

@@ -35,6 +35,20 @@ constexpr int kSlots = 64;   // candidate slots per item in the partial-sum layo
 constexpr int cg_tgemm(int mt) { return mt <= 2 ? 4 : (mt <= 8 ? 2 : 1); }
 constexpr int cg_project(int mt) { return mt <= 2 ? 4 : (mt <= 4 ? 2 : 1); }
 constexpr int waves_for(int mt) { return mt == 1 ? 2 : (mt <= 4 ? 3 : 2); }
+/* Narrow T-GEMM tiles (one 16-candidate group, the mixed form's rescoring) issue the loads of a
+ * whole 32-row chunk before its MFMAs: their few loads per half-step alone leave HBM latency
+ * exposed. */
+constexpr bool kHoist(int mt, int cg) { return cg == 1 && mt <= 2; }
+
+/* Work items are listed fit by fit and tile by tile in row order, so neighbouring items read
+ * the same columns and share their address translations.  Workgroups are dispatched round-robin
+ * over the 8 XCDs; block b of a round of `cnt` blocks takes item xcd_item(b, cnt), which gives
+ * each XCD (and so each CU) a contiguous run of the list instead of every 8th item. */
+__device__ inline int xcd_item(int b, int cnt)
+{
+    const int x = b & 7, q = cnt >> 3, rm = cnt & 7;
+    return x * q + (x < rm ? x : rm) + (b >> 3);
+}
 
 /* Per-lane candidate of an item slot. */
 struct LaneCand {
@@ -73,12 +87,22 @@ __device__ inline float4 bx4(const FitDev &F, const LaneCand &c, long long row, 
 {
     float4 v;
     if (RB) {
-        const float *p = F.rbuf + c.col;
-        const long long last = r1 - 1;
-        const float v0 = gld(p + (row < r1 ? row : last) * F.ldr);
-        const float v1 = gld(p + (row + 1 < r1 ? row + 1 : last) * F.ldr);
-        const float v2 = gld(p + (row + 2 < r1 ? row + 2 : last) * F.ldr);
-        const float v3 = gld(p + (row + 3 < r1 ? row + 3 : last) * F.ldr);
+        const float *p = F.rbuf + c.col * F.rsc;
+        float v0, v1, v2, v3;
+        if (F.rsr == 1) {
+            // column-major: rows row..row+3 lie inside the column's ldx rows (row < r1 <= n)
+            const float4 q = gld4(p + row);
+            v0 = q.x;
+            v1 = q.y;
+            v2 = q.z;
+            v3 = q.w;
+        } else {
+            const long long last = r1 - 1;
+            v0 = gld(p + (row < r1 ? row : last) * F.rsr);
+            v1 = gld(p + (row + 1 < r1 ? row + 1 : last) * F.rsr);
+            v2 = gld(p + (row + 2 < r1 ? row + 2 : last) * F.rsr);
+            v3 = gld(p + (row + 3 < r1 ? row + 3 : last) * F.rsr);
+        }
         v.x = row < r1 ? v0 : 0.f;
         v.y = row + 1 < r1 ? v1 : 0.f;
         v.z = row + 2 < r1 ? v2 : 0.f;
@@ -107,8 +131,12 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_tgemm_w(const FitDev *__r
                                                 const int *__restrict__ cpar, const int *__restrict__ list,
                                                 int ldT, double *__restrict__ tpart)
 {
+    constexpr int HF = kHoist(MT, CG) ? 2 : 1;  // half-steps whose loads are issued together
     const int lane = threadIdx.x, kr = lane >> 4, kc = lane & 15;
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    for (int round = 0; round < nitems; round += gridDim.x) {
+        const int cnt = min((int)gridDim.x, nitems - round);
+        if ((int)blockIdx.x >= cnt) break;
+        const int it = round + xcd_item(blockIdx.x, cnt);
         const Item I = items[it];
         const FitDev F = fits[I.fit];
         LaneCand c[CG];
@@ -130,36 +158,42 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_tgemm_w(const FitDev *__r
 #pragma unroll 1
         for (long long r = I.r0; r < I.r1; r += kWR) {
 #pragma unroll 1
-            for (int hf = 0; hf < 2; ++hf) {
-                const long long row = r + 8 * kr + 4 * hf;
-                // all loads of the half-step first, unconditionally (see bx4)
-                float4 av4[MT], bv4[CG];
+            for (int h0 = 0; h0 < 2; h0 += HF) {
+                // all loads of HF half-steps first, unconditionally (see bx4)
+                float4 av4[HF][MT], bv4[HF][CG];
 #pragma unroll
-                for (int u = 0; u < MT; ++u) av4[u] = gld4(bo[u] + row);
+                for (int hh = 0; hh < HF; ++hh) {
+                    const long long row = r + 8 * kr + 4 * (h0 + hh);
 #pragma unroll
-                for (int g = 0; g < CG; ++g) bv4[g] = bx4<RETRY>(F, c[g], row, I.r1);
-                double a[MT][4], b[CG][4];
+                    for (int u = 0; u < MT; ++u) av4[hh][u] = gld4(bo[u] + row);
 #pragma unroll
-                for (int u = 0; u < MT; ++u) {
-                    a[u][0] = kok[u] ? av4[u].x : 0.f;
-                    a[u][1] = kok[u] ? av4[u].y : 0.f;
-                    a[u][2] = kok[u] ? av4[u].z : 0.f;
-                    a[u][3] = kok[u] ? av4[u].w : 0.f;
+                    for (int g = 0; g < CG; ++g) bv4[hh][g] = bx4<RETRY>(F, c[g], row, I.r1);
                 }
 #pragma unroll
-                for (int g = 0; g < CG; ++g) {
-                    b[g][0] = bv4[g].x;
-                    b[g][1] = bv4[g].y;
-                    b[g][2] = bv4[g].z;
-                    b[g][3] = bv4[g].w;
+                for (int hh = 0; hh < HF; ++hh) {
+                    double a[MT][4], b[CG][4];
+#pragma unroll
+                    for (int u = 0; u < MT; ++u) {
+                        a[u][0] = kok[u] ? av4[hh][u].x : 0.f;
+                        a[u][1] = kok[u] ? av4[hh][u].y : 0.f;
+                        a[u][2] = kok[u] ? av4[hh][u].z : 0.f;
+                        a[u][3] = kok[u] ? av4[hh][u].w : 0.f;
+                    }
+#pragma unroll
+                    for (int g = 0; g < CG; ++g) {
+                        b[g][0] = bv4[hh][g].x;
+                        b[g][1] = bv4[hh][g].y;
+                        b[g][2] = bv4[hh][g].z;
+                        b[g][3] = bv4[hh][g].w;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+#pragma unroll
+                        for (int g = 0; g < CG; ++g)
+                            if (16 * g < I.nc)
+#pragma unroll
+                                for (int u = 0; u < MT; ++u) acc[g][u] = mfma_f64(a[u][q], b[g][q], acc[g][u]);
                 }
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-#pragma unroll
-                    for (int g = 0; g < CG; ++g)
-                        if (16 * g < I.nc)
-#pragma unroll
-                            for (int u = 0; u < MT; ++u) acc[g][u] = mfma_f64(a[u][q], b[g][q], acc[g][u]);
             }
         }
 #pragma unroll
@@ -180,7 +214,7 @@ enum { kStoreR = 0, kSumSq = 1, kFinalY = 2 };
 
 /*
  * Projection r = f32(Bx - Bo T) over the item's rows, then per MODE:
- *   kStoreR  r -> rbuf[row * ldr + column], spart[item * 64 + slot] = sum r^2;
+ *   kStoreR  r -> rbuf (row * rsr + column * rsc), spart[item * 64 + slot] = sum r^2;
  *   kSumSq   spart = sum r^2 only;
  *   kFinalY  Bxn = f32(r * scale), scale = f32(1 / sqrt(s + tol)) (0 when s <= tol,
  *            kernels.cc:401-416), spart = sum Bxn y (marsalgo.cc:500-505); candidates
@@ -205,10 +239,14 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_project_w(const FitDev *_
 {
     constexpr int KQ = 4 * MT;           // k steps of 4
     constexpr int NC = 16 * CG, ST = 16 * MT + 1;
+    constexpr int HF = 1;  // 16-row blocks whose loads are issued together (two spill registers)
     __shared__ double sT[NC][ST];
     const int lane = threadIdx.x, kr = lane >> 4, kc = lane & 15;
     const int prow = 4 * (kc & 3) + (kc >> 2);  // memory row of this lane's A-operand M index
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    for (int round = 0; round < nitems; round += gridDim.x) {
+        const int cnt = min((int)gridDim.x, nitems - round);
+        if ((int)blockIdx.x >= cnt) break;
+        const int it = round + xcd_item(blockIdx.x, cnt);
         const Item I = items[it];
         const FitDev F = fits[I.fit];
         __syncthreads();
@@ -252,41 +290,75 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_project_w(const FitDev *_
 #pragma unroll 1
         for (long long r = I.r0; r < I.r1; r += kWR) {
 #pragma unroll 1
-            for (int h = 0; h < kWR / 16; ++h) {
-                const long long r16 = r + 16 * h;
-                const long long row = r16 + 4 * kr;
-                // all loads of the 16-row block first, unconditionally (see bx4)
-                float av[KQ];
+            for (int h0 = 0; h0 < kWR / 16; h0 += HF) {
+                // all loads of HF 16-row blocks first, unconditionally (see bx4)
+                float avh[HF][KQ];
+                float4 bxh[HF][CG], yh[HF];
 #pragma unroll
-                for (int q = 0; q < KQ; ++q) av[q] = gld(bo[q] + r16 + prow);
-                float4 bxv[CG];
+                for (int hh = 0; hh < HF; ++hh) {
+                    const long long r16 = r + 16 * (h0 + hh);
+                    const long long row = r16 + 4 * kr;
 #pragma unroll
-                for (int g = 0; g < CG; ++g) bxv[g] = bx4<RB>(F, c[g], row, I.r1);
-                float4 yv = {0.f, 0.f, 0.f, 0.f};
-                if (MODE == kFinalY) yv = gld4(F.y + row);
-                const float yy[4] = {yv.x, yv.y, yv.z, yv.w};
+                    for (int q = 0; q < KQ; ++q) avh[hh][q] = gld(bo[q] + r16 + prow);
 #pragma unroll
-                for (int q = 0; q < KQ; ++q)
-                    if (4 * q + kr >= F.m) av[q] = 0.0f;
+                    for (int g = 0; g < CG; ++g) bxh[hh][g] = bx4<RB>(F, c[g], row, I.r1);
+                    yh[hh] = float4{0.f, 0.f, 0.f, 0.f};
+                    if (MODE == kFinalY) yh[hh] = gld4(F.y + row);
+                }
 #pragma unroll
-                for (int g = 0; g < CG; ++g) {
-                    if (16 * g >= I.nc) continue;
-                    v4d d = v4d{0, 0, 0, 0};
+                for (int hh = 0; hh < HF; ++hh) {
+                    const long long row = r + 16 * (h0 + hh) + 4 * kr;
+                    float *av = avh[hh];
+                    const float4 *bxv = bxh[hh];
+                    const float yy[4] = {yh[hh].x, yh[hh].y, yh[hh].z, yh[hh].w};
 #pragma unroll
-                    for (int q = 0; q < KQ; ++q) d = mfma_f64((double)av[q], sT[16 * g + kc][4 * q + kr], d);
-                    if (!c[g].ok) continue;
-                    const float bv[4] = {bxv[g].x, bxv[g].y, bxv[g].z, bxv[g].w};
+                    for (int q = 0; q < KQ; ++q)
+                        if (4 * q + kr >= F.m) av[q] = 0.0f;
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        if (row + v >= I.r1) continue;
-                        float rf = (float)((double)bv[v] - d[v]);
-                        if (MODE == kFinalY) {
-                            if (fromr[g]) rf = F.rbuf[(row + v) * F.ldr + c[g].col];
-                            const float bxn = __fmul_rn(rf, scale[g]);
-                            acc[g] = fma((double)bxn, (double)yy[v], acc[g]);
-                        } else {
-                            if (MODE == kStoreR) F.rbuf[(row + v) * F.ldr + c[g].col] = rf;
-                            acc[g] = fma((double)rf, (double)rf, acc[g]);
+                    for (int g = 0; g < CG; ++g) {
+                        if (16 * g >= I.nc) continue;
+                        v4d d = v4d{0, 0, 0, 0};
+#pragma unroll
+                        for (int q = 0; q < KQ; ++q) d = mfma_f64((double)av[q], sT[16 * g + kc][4 * q + kr], d);
+                        if (!c[g].ok) continue;
+                        const float bv[4] = {bxv[g].x, bxv[g].y, bxv[g].z, bxv[g].w};
+                        float rf[4];
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) rf[v] = (float)((double)bv[v] - d[v]);
+                        // rows past the item's end occur only past the fit's end (item ranges
+                        // start at multiples of 32), where r is 0 from the zero padding
+                        float *rp = F.rbuf + c[g].col * F.rsc;
+                        if (MODE == kFinalY && fromr[g]) {
+                            if (F.rsr == 1) {
+                                const float4 q = gld4(rp + row);
+                                rf[0] = q.x;
+                                rf[1] = q.y;
+                                rf[2] = q.z;
+                                rf[3] = q.w;
+                            } else {
+#pragma unroll
+                                for (int v = 0; v < 4; ++v)
+                                    if (row + v < I.r1) rf[v] = rp[(row + v) * F.rsr];
+                            }
+                        }
+                        if (MODE == kStoreR) {
+                            if (F.rsr == 1) {
+                                *reinterpret_cast<float4 *>(rp + row) = float4{rf[0], rf[1], rf[2], rf[3]};
+                            } else {
+#pragma unroll
+                                for (int v = 0; v < 4; ++v)
+                                    if (row + v < I.r1) rp[(row + v) * F.rsr] = rf[v];
+                            }
+                        }
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            if (row + v >= I.r1) continue;
+                            if (MODE == kFinalY) {
+                                const float bxn = __fmul_rn(rf[v], scale[g]);
+                                acc[g] = fma((double)bxn, (double)yy[v], acc[g]);
+                            } else {
+                                acc[g] = fma((double)rf[v], (double)rf[v], acc[g]);
+                            }
                         }
                     }
                 }
