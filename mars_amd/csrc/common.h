@@ -92,6 +92,7 @@ struct FitState {
     std::vector<double> ybo_h;
     DBuf<float> X, s, B, Bo, BoC, BC, y, rbuf;
     DBuf<double> ybo, v;
+    DBuf<double> G;   // append: Gram rows of the stored Bo columns (ldb x ldb)
     size_t ldr = 0;
     // mixed form: T cache valid for basis columns and parents < mc
     DBuf<double> tc, tcn;
@@ -108,7 +109,7 @@ struct FitState {
     {
         X.release(); s.release(); B.release(); Bo.release(); BoC.release(); BC.release(); y.release();
         rbuf.release();
-        ybo.release(); v.release(); order.release(); delta.release(); tc.release(); tcn.release();
+        ybo.release(); v.release(); G.release(); order.release(); delta.release(); tc.release(); tcn.release();
     }
 };
 
@@ -122,6 +123,7 @@ struct Item {
 struct FitW {
     float *B, *Bo, *BoC, *BC;
     double *ybo;
+    double *G;  // Gram matrix of the stored Bo columns, G[j * ldb + l] for l < j
 };
 
 /* Result slot returned by the device append. */
@@ -214,6 +216,7 @@ struct mars_hip_ctx {
     mh::DBuf<long long> dgks;        // device DGKS counter
     mh::DBuf<double> red;            // reduction scratch
     mh::DBuf<double> vcol;           // append: f64 residual columns
+    mh::DBuf<double> adot, agram, acoef;  // append: partials of Q^T v and of a Gram row, MGS coefficients
     mh::DBuf<double> scal;           // append / init scalars
     mh::DBuf<mh::AppendOut> aout;
     mh::DBuf<mh::AppendArg> aarg;

@@ -182,55 +182,166 @@ __global__ __launch_bounds__(256) void k_app_build(const FitDev *__restrict__ fi
 }
 
 /*
- * One modified Gram-Schmidt step j of the residual v against Bo's column j:
- * apply the update of column j-1 (coefficient = sum of the previous step's
- * partials) and accumulate the partials of column j's coefficient; step
- * j == m accumulates |v|^2 instead.  The first sweep's step 0 also stores the
- * raw-norm-normalized B column (marsalgo.cc:829-836).  `gate` (retry sweep)
- * makes the launch a no-op for fits whose DGKS gate did not fire.
+ * Modified Gram-Schmidt of the residual v against Bo's m columns (kernels.cc:239-281), in
+ * two passes over the rows.  MGS's coefficient of column j is c_j = q_j . v_j with
+ * v_j = v - sum_{l<j} c_l q_l, which equals d_j - sum_{l<j} c_l G_jl for d = Q^T v and the
+ * Gram matrix G = Q^T Q of the stored f32 columns (all products exact in FP64).  So:
+ *   k_app_dots   partials of d (16 columns per block row), and on the first sweep of an
+ *                append also the Gram row of column m-1 (the newest one) and the stored
+ *                raw-norm-normalized B column (marsalgo.cc:829-836);
+ *   k_app_solve  per fit: d and the new Gram row from their partials, then the recurrence
+ *                for c in column order, and sum c^2 for the DGKS gate;
+ *   k_app_sub    v <- v - c_0 q_0 - c_1 q_1 - ... row by row (MGS's order of updates), and the
+ *                partials of |v|^2.
+ * `gate` (retry sweep) makes a launch a no-op for fits whose DGKS gate did not fire.
  */
-__global__ __launch_bounds__(256) void k_app_mgs(const FitDev *__restrict__ fits,
-                                                 const AppendArg *__restrict__ args, int nb,
-                                                 const double *__restrict__ prev,
-                                                 double *__restrict__ next, int j, int first,
-                                                 const int *__restrict__ gate,
-                                                 double *__restrict__ scal, const FitW *__restrict__ W)
+constexpr int kAppG = 16;             // Bo columns per k_app_dots block
+constexpr int kMaxAppendCols = 512;   // basis columns the device append orthogonalizes against
+
+__global__ __launch_bounds__(256) void k_app_dots(const FitDev *__restrict__ fits,
+                                                  const AppendArg *__restrict__ args, int nb,
+                                                  const double *__restrict__ bpart, int first,
+                                                  const int *__restrict__ gate, double *__restrict__ dpart,
+                                                  double *__restrict__ gpart, const FitW *__restrict__ W)
 {
-    __shared__ double sh[256];
-    const int f = blockIdx.y;
+    __shared__ double sh[4][2 * kAppG];
+    __shared__ double shr[256];
+    const int f = blockIdx.y, g0 = kAppG * (int)blockIdx.z;
     if (!args[f].active) return;
     const FitDev F = fits[f];
     const int m = F.m;
-    if (j > m) return;
     if (gate && !gate[f]) return;
-    double c = 0.0;
+    // the first group also writes the B column; the others only serve columns they hold
+    if (g0 >= m && !(first && blockIdx.z == 0)) return;
+    const int kn = min(kAppG, m - g0);                       // live columns of this group
+    const bool grow = first && m >= 2;                      // Gram row of column m - 1
     float rn = 1.0f;
-    if (j > 0) c = sum_partials(prev + (size_t)f * nb, nb, sh);
-    if (j == 0 && first) rn = (float)sqrt(sum_partials(prev + (size_t)f * nb, nb, sh));
+    if (first && blockIdx.z == 0) rn = (float)sqrt(sum_partials(bpart + (size_t)f * nb, nb, shr));
     float *B = W[f].B, *BC = W[f].BC + (size_t)m * F.ldx;
-    double a = 0.0;
+    const float *qn = F.BoC + (size_t)(m - 1) * F.ldx;
+    double d[kAppG], gr[kAppG];
+#pragma unroll
+    for (int k = 0; k < kAppG; ++k) d[k] = gr[k] = 0.0;
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < F.n; i += (long long)nb * 256) {
-        double vi = F.v[i];
-        if (j == 0 && first) {
+        const double vi = F.v[i];
+        if (first && blockIdx.z == 0) {
             const float b = __fdiv_rn((float)vi, rn);
             if (B) B[i * F.ldb + m] = b;
             BC[i] = b;
         }
-        if (j > 0) {
-            vi = fma(-c, (double)F.BoC[(j - 1) * F.ldx + i], vi);
-            F.v[i] = vi;
+        const double qm = grow ? (double)qn[i] : 0.0;
+#pragma unroll
+        for (int k = 0; k < kAppG; ++k) {
+            if (k < kn) {
+                const double q = (double)F.BoC[(size_t)(g0 + k) * F.ldx + i];
+                d[k] = fma(q, vi, d[k]);
+                if (grow) gr[k] = fma(qm, q, gr[k]);
+            }
         }
-        if (j < m)
-            a = fma((double)F.BoC[j * F.ldx + i], vi, a);
-        else
-            a = fma(vi, vi, a);
+    }
+    // fixed-order block sums: wave butterflies, then the four waves in order
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < kAppG; ++k) {
+        const double a = wave_sum(d[k]), b = wave_sum(gr[k]);
+        if (lane == 0) {
+            sh[w][k] = a;
+            sh[w][kAppG + k] = b;
+        }
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < kn) {
+        const double a = ((sh[0][t] + sh[1][t]) + sh[2][t]) + sh[3][t];
+        dpart[((size_t)f * F.ldb + g0 + t) * nb + blockIdx.x] = a;
+    } else if (grow && t >= kAppG && t - kAppG < kn) {
+        const int k = t - kAppG;
+        const double b = ((sh[0][t] + sh[1][t]) + sh[2][t]) + sh[3][t];
+        gpart[((size_t)f * F.ldb + g0 + k) * nb + blockIdx.x] = b;
+    }
+}
+
+/* One block of 256 per fit: reduce the partials, extend G, run the MGS recurrence. */
+__global__ __launch_bounds__(256) void k_app_solve(const FitDev *__restrict__ fits,
+                                                   const AppendArg *__restrict__ args, int nb, int first,
+                                                   const int *__restrict__ gate,
+                                                   const double *__restrict__ dpart,
+                                                   const double *__restrict__ gpart, const FitW *__restrict__ W,
+                                                   double *__restrict__ coef, double *__restrict__ scal)
+{
+    __shared__ double shr[256];
+    __shared__ double sd[kMaxAppendCols], sg[kMaxAppendCols];
+    const int f = blockIdx.x;
+    if (!args[f].active) return;
+    if (gate && !gate[f]) return;
+    const FitDev F = fits[f];
+    const int m = F.m;
+    double *G = W[f].G;  // G[j * ldb + l], l < j
+    for (int k = 0; k < m; ++k) {
+        const double a = sum_partials(dpart + ((size_t)f * F.ldb + k) * nb, nb, shr);
+        if (threadIdx.x == 0) sd[k] = a;
+    }
+    if (first && m >= 2)
+        for (int l = 0; l < m - 1; ++l) {
+            const double b = sum_partials(gpart + ((size_t)f * F.ldb + l) * nb, nb, shr);
+            if (threadIdx.x == 0) {
+                G[(size_t)(m - 1) * F.ldb + l] = b;
+                sg[l] = b;
+            }
+        }
+    __syncthreads();
+    if (threadIdx.x >= 64) return;
+    // c_j = d_j - sum_{l<j} c_l G_jl on one wave, without barriers: lane l % 64 keeps c_l in
+    // slot l / 64; every lane forms the same c_j and the same sum of c^2
+    const int lane = threadIdx.x;
+    double cown[kMaxAppendCols / 64];
+#pragma unroll
+    for (int u = 0; u < kMaxAppendCols / 64; ++u) cown[u] = 0.0;
+    double proj = 0.0;
+    for (int j = 0; j < m; ++j) {
+        double part = 0.0;
+#pragma unroll
+        for (int u = 0; u < kMaxAppendCols / 64; ++u) {
+            const int l = lane + 64 * u;
+            if (l < j) part = fma(cown[u], (first && j == m - 1) ? sg[l] : G[(size_t)j * F.ldb + l], part);
+        }
+        const double c = sd[j] - wave_sum(part);
+#pragma unroll
+        for (int u = 0; u < kMaxAppendCols / 64; ++u)
+            if (j == lane + 64 * u) cown[u] = c;
+        proj = fma(c, c, proj);
+    }
+#pragma unroll
+    for (int u = 0; u < kMaxAppendCols / 64; ++u) {
+        const int j = lane + 64 * u;
+        if (j < m) coef[(size_t)f * F.ldb + j] = cown[u];
+    }
+    if (lane == 0) scal[f * S_COUNT + (first ? S_PROJ : S_PROJ2)] = proj;
+}
+
+__global__ __launch_bounds__(256) void k_app_sub(const FitDev *__restrict__ fits,
+                                                 const AppendArg *__restrict__ args, int nb,
+                                                 const int *__restrict__ gate,
+                                                 const double *__restrict__ coef, double *__restrict__ part)
+{
+    __shared__ double sh[256];
+    __shared__ double sc[kMaxAppendCols];
+    const int f = blockIdx.y;
+    if (!args[f].active) return;
+    if (gate && !gate[f]) return;
+    const FitDev F = fits[f];
+    const int m = F.m;
+    for (int j = threadIdx.x; j < m; j += 256) sc[j] = coef[(size_t)f * F.ldb + j];
+    __syncthreads();
+    double a = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < F.n; i += (long long)nb * 256) {
+        double vi = F.v[i];
+        for (int j = 0; j < m; ++j) vi = fma(-sc[j], (double)F.BoC[(size_t)j * F.ldx + i], vi);
+        F.v[i] = vi;
+        a = fma(vi, vi, a);
     }
     a = block_sum_256(a, sh);
-    if (threadIdx.x == 0) next[(size_t)f * nb + blockIdx.x] = a;
-    if (j > 0 && blockIdx.x == 0 && threadIdx.x == 0) {
-        double *acc = &scal[f * S_COUNT + (first ? S_PROJ : S_PROJ2)];
-        *acc = fma(c, c, *acc);
-    }
+    if (threadIdx.x == 0) part[(size_t)f * nb + blockIdx.x] = a;
 }
 
 /* DGKS gate of the append (kernels.cc:376-385); the second call records |v|^2 after the retry. */
@@ -499,7 +610,8 @@ void ctx_free(mars_hip_ctx *ctx)
     for (auto &fs : ctx->fits) fs.release();
     DBuf<double> *db[] = {&ctx->tpart, &ctx->T, &ctx->T2, &ctx->tnorm, &ctx->spart, &ctx->ssum, &ctx->ybx,
                           &ctx->lin, &ctx->hin, &ctx->aux, &ctx->red, &ctx->vcol, &ctx->scal,
-                          &ctx->dense, &ctx->hstate, &ctx->hbest, &ctx->rlin, &ctx->rybx};
+                          &ctx->dense, &ctx->hstate, &ctx->hbest, &ctx->rlin, &ctx->rybx,
+                          &ctx->adot, &ctx->agram, &ctx->acoef};
     for (auto *b : db) b->release();
     DBuf<int> *ib[] = {&ctx->flag, &ctx->fired_list, &ctx->fired_count, &ctx->knot, &ctx->krow,
                        &ctx->ival, &ctx->d_cmap, &ctx->dknot, &ctx->hbest_i, &ctx->cand.d_fit,
@@ -608,6 +720,7 @@ void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std
         }
         require(args[f].xcol >= 0 && (size_t)args[f].xcol < fs.px, "invalid X column index");
         require(args[f].type >= 0 && args[f].type <= 2, "invalid basis type");
+        require(fs.m < (size_t)kMaxAppendCols, "the device append supports up to 512 basis functions");
         mmax = std::max(mmax, fs.m);
         nmax = std::max(nmax, fs.n);
     }
@@ -627,12 +740,15 @@ void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std
     std::vector<FitW> fw(F);
     for (size_t f = 0; f < F; ++f) {
         FitState &fs = ctx->fits[f];
-        fw[f] = FitW{fs.B.p, fs.Bo.p, fs.BoC.p, fs.BC.p, fs.ybo.p};
+        fs.G.ensure(ctx->ldb * ctx->ldb);  // Gram rows of the stored columns, grown one per append
+        fw[f] = FitW{fs.B.p, fs.Bo.p, fs.BoC.p, fs.BC.p, fs.ybo.p, fs.G.p};
     }
     ctx->d_fitw.ensure(F);
     h2d(ctx, ctx->d_fitw.p, fw.data(), F * sizeof(FitW));
     double *pa = ctx->red.p, *pb = pa + F * nb, *pc = pb + F * nb;
-    double *pp[2] = {pa, pb};
+    ctx->adot.ensure(F * ctx->ldb * nb);
+    ctx->agram.ensure(F * ctx->ldb * nb);
+    ctx->acoef.ensure(F * ctx->ldb);
     int *gate = ctx->flag.p;
     double nrows = 0, mm = 0;
     for (size_t f = 0; f < F; ++f)
@@ -655,13 +771,20 @@ void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std
                 for (size_t f = 0; f < F; ++f) any = any || (args[f].active && g[f]);
                 if (!any) break;
             }
-            for (size_t j = 0; j <= mmax; ++j) {
-                hipLaunchKernelGGL(k_app_mgs, grid, dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, pp[j % 2],
-                                   pp[(j + 1) % 2], (int)j, pass == 0 ? 1 : 0, pass == 0 ? nullptr : gate,
-                                   ctx->scal.p, ctx->d_fitw.p);
-                MH_LAUNCH_CHECK();
-            }
-            hipLaunchKernelGGL(k_app_gate, dim3(F), dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, pa, pb,
+            const int first = pass == 0 ? 1 : 0;
+            const int *gp = pass == 0 ? nullptr : gate;
+            const unsigned kg = (unsigned)((mmax + kAppG - 1) / kAppG);
+            hipLaunchKernelGGL(k_app_dots, dim3(nb, F, kg), dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb,
+                               (const double *)pa, first, gp, ctx->adot.p, ctx->agram.p, ctx->d_fitw.p);
+            MH_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_app_solve, dim3(F), dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, first, gp,
+                               (const double *)ctx->adot.p, (const double *)ctx->agram.p, ctx->d_fitw.p,
+                               ctx->acoef.p, ctx->scal.p);
+            MH_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_app_sub, grid, dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, gp,
+                               (const double *)ctx->acoef.p, pb);
+            MH_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_app_gate, dim3(F), dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, pb, pb,
                                gate, ctx->scal.p, ctx->dgks.p, pass);
             MH_LAUNCH_CHECK();
         }
