@@ -91,7 +91,7 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
                 mixed_score(ctx);
             else
                 plain_score(ctx, cs, false, ctx->lin.p, ctx->ybx.p);
-            ctx_sync(ctx);
+            // no synchronisation here: the argmax that follows on the same stream returns the result
         }
         {
             HostTimer t(ctx, K_H_ARGMAX);

@@ -186,6 +186,9 @@ struct mars_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_rb = nullptr;      // marks a readback into rb_pin as complete
+    int *rb_pin = nullptr;           // pinned host readback buffer (rb_cap ints)
+    size_t rb_cap = 0;
     // pinned staging for host-to-device copies: bump-allocated, reset after a stream sync
     char *stage = nullptr;
     size_t stage_cap = 0, stage_off = 0;

@@ -219,24 +219,34 @@ __global__ __launch_bounds__(256) void k_app_dots(const FitDev *__restrict__ fit
     if (first && blockIdx.z == 0) rn = (float)sqrt(sum_partials(bpart + (size_t)f * nb, nb, shr));
     float *B = W[f].B, *BC = W[f].BC + (size_t)m * F.ldx;
     const float *qn = F.BoC + (size_t)(m - 1) * F.ldx;
+    // every column load is issued unconditionally (columns past the group's end read column
+    // g0 again and are masked), so a row's loads are in flight together
+    const float *qc[kAppG];
+#pragma unroll
+    for (int k = 0; k < kAppG; ++k) qc[k] = F.BoC + (size_t)(g0 + (k < kn ? k : 0)) * F.ldx;
     double d[kAppG], gr[kAppG];
 #pragma unroll
     for (int k = 0; k < kAppG; ++k) d[k] = gr[k] = 0.0;
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < F.n; i += (long long)nb * 256) {
-        const double vi = F.v[i];
+    // a contiguous row range per block: its 16 columns span few pages
+    const long long span = (F.n + (long long)nb * 256 - 1) / ((long long)nb * 256) * 256;
+    const long long r1 = min(F.n, span * (blockIdx.x + 1));
+    for (long long i = span * blockIdx.x + threadIdx.x; i < r1; i += 256) {
+        const double vi = gld(F.v + i);
+        const double qm = (double)gld(qn + i);  // column m - 1 (only used when grow)
+        float q[kAppG];
+#pragma unroll
+        for (int k = 0; k < kAppG; ++k) q[k] = gld(qc[k] + i);
         if (first && blockIdx.z == 0) {
             const float b = __fdiv_rn((float)vi, rn);
             if (B) B[i * F.ldb + m] = b;
             BC[i] = b;
         }
-        const double qm = grow ? (double)qn[i] : 0.0;
 #pragma unroll
         for (int k = 0; k < kAppG; ++k) {
-            if (k < kn) {
-                const double q = (double)F.BoC[(size_t)(g0 + k) * F.ldx + i];
-                d[k] = fma(q, vi, d[k]);
-                if (grow) gr[k] = fma(qm, q, gr[k]);
-            }
+            const double dk = fma((double)q[k], vi, d[k]);
+            const double gk = fma(qm, (double)q[k], gr[k]);
+            d[k] = k < kn ? dk : d[k];
+            gr[k] = (grow && k < kn) ? gk : gr[k];
         }
     }
     // fixed-order block sums: wave butterflies, then the four waves in order
@@ -269,7 +279,6 @@ __global__ __launch_bounds__(256) void k_app_solve(const FitDev *__restrict__ fi
                                                    const double *__restrict__ gpart, const FitW *__restrict__ W,
                                                    double *__restrict__ coef, double *__restrict__ scal)
 {
-    __shared__ double shr[256];
     __shared__ double sd[kMaxAppendCols], sg[kMaxAppendCols];
     const int f = blockIdx.x;
     if (!args[f].active) return;
@@ -277,18 +286,20 @@ __global__ __launch_bounds__(256) void k_app_solve(const FitDev *__restrict__ fi
     const FitDev F = fits[f];
     const int m = F.m;
     double *G = W[f].G;  // G[j * ldb + l], l < j
-    for (int k = 0; k < m; ++k) {
-        const double a = sum_partials(dpart + ((size_t)f * F.ldb + k) * nb, nb, shr);
-        if (threadIdx.x == 0) sd[k] = a;
-    }
-    if (first && m >= 2)
-        for (int l = 0; l < m - 1; ++l) {
-            const double b = sum_partials(gpart + ((size_t)f * F.ldb + l) * nb, nb, shr);
-            if (threadIdx.x == 0) {
-                G[(size_t)(m - 1) * F.ldb + l] = b;
-                sg[l] = b;
-            }
+    // thread k sums the partials of d_k (and of the Gram row entry) in block order
+    for (int k = threadIdx.x; k < m; k += 256) {
+        const double *pd = dpart + ((size_t)f * F.ldb + k) * nb;
+        double a = 0.0;
+        for (int b = 0; b < nb; ++b) a += pd[b];
+        sd[k] = a;
+        if (first && k < m - 1) {
+            const double *pg = gpart + ((size_t)f * F.ldb + k) * nb;
+            double g = 0.0;
+            for (int b = 0; b < nb; ++b) g += pg[b];
+            G[(size_t)(m - 1) * F.ldb + k] = g;
+            sg[k] = g;
         }
+    }
     __syncthreads();
     if (threadIdx.x >= 64) return;
     // c_j = d_j - sum_{l<j} c_l G_jl on one wave, without barriers: lane l % 64 keeps c_l in
@@ -334,7 +345,9 @@ __global__ __launch_bounds__(256) void k_app_sub(const FitDev *__restrict__ fits
     for (int j = threadIdx.x; j < m; j += 256) sc[j] = coef[(size_t)f * F.ldb + j];
     __syncthreads();
     double a = 0.0;
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < F.n; i += (long long)nb * 256) {
+    const long long span = (F.n + (long long)nb * 256 - 1) / ((long long)nb * 256) * 256;
+    const long long r1 = min(F.n, span * (blockIdx.x + 1));
+    for (long long i = span * blockIdx.x + threadIdx.x; i < r1; i += 256) {
         double vi = F.v[i];
         for (int j = 0; j < m; ++j) vi = fma(-sc[j], (double)F.BoC[(size_t)j * F.ldx + i], vi);
         F.v[i] = vi;
@@ -539,6 +552,7 @@ void ctx_base_init(mars_hip_ctx *ctx, size_t max_terms)
     MH_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     MH_CHECK(hipEventCreate(&ctx->ev0));
     MH_CHECK(hipEventCreate(&ctx->ev1));
+    MH_CHECK(hipEventCreateWithFlags(&ctx->ev_rb, hipEventDisableTiming));
     ctx->max_terms = max_terms;
     ctx->ldb = (max_terms + 3) / 4 * 4;
     ctx->dgks.ensure(1);
@@ -638,6 +652,9 @@ void ctx_free(mars_hip_ctx *ctx)
     for (auto e : ctx->evpool) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->ev_rb) (void)hipEventDestroy(ctx->ev_rb);
+    if (ctx->rb_pin) (void)hipHostFree(ctx->rb_pin);
+    ctx->rb_pin = nullptr;
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

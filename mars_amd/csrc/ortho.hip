@@ -133,10 +133,12 @@ __global__ __launch_bounds__(256) void k_finalize(const FitDev *__restrict__ fit
 /*
  * The same over column-major residuals (linear scoring): wave w of 4 takes the item's slots
  * w, w + 4, ...; its lanes read the slot's column as float4 (rows r0 + 4 lane + 256 u, four
- * in flight), and the wave's sum is formed in a fixed order.
+ * in flight), and the wave's sum is formed in a fixed order.  With `list`, item positions
+ * index the list of candidates.
  */
 __global__ __launch_bounds__(256) void k_finalize_cm(const FitDev *__restrict__ fits,
                                                      const Item *__restrict__ items, int nitems,
+                                                     const int *__restrict__ list,
                                                      const double *__restrict__ ssum,
                                                      double *__restrict__ ypart)
 {
@@ -145,7 +147,7 @@ __global__ __launch_bounds__(256) void k_finalize_cm(const FitDev *__restrict__ 
         const Item I = items[it];
         const FitDev F = fits[I.fit];
         for (int slot = w; slot < I.nc; slot += 4) {
-            const int c = I.c0 + slot;
+            const int c = list ? list[I.c0 + slot] : I.c0 + slot;
             const double s = ssum[c];
             const float scale = (s > F.tol) ? (float)(1.0 / sqrt(s + F.tol)) : 0.0f;
             const float *col = F.rbuf + (c - F.c0) * F.rsc;
@@ -504,10 +506,45 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                            ctx->ssum.p, ctx->flag.p);
         MH_LAUNCH_CHECK();
     }
+    // Stored column-major residuals (the mixed form's rescoring) are finalized before the host
+    // has read the DGKS flags: the device runs pass 3 over every candidate while the flags come
+    // back, and the few fired candidates are finalized again after their retry.
+    const bool spec = store_r && ctx->rbuf_cm;
+    const int nF = (int)pF.items.size();
+    std::vector<int> flag_v;
+    const int *flags = nullptr;
+    if (spec) {
+        if (ctx->rb_cap < (size_t)P) {
+            ctx_sync(ctx);
+            if (ctx->rb_pin) MH_CHECK(hipHostFree(ctx->rb_pin));
+            ctx->rb_pin = nullptr;
+            ctx->rb_cap = 0;
+            const size_t cap = std::max<size_t>((size_t)P, 4096);
+            MH_CHECK(hipHostMalloc((void **)&ctx->rb_pin, cap * sizeof(int), hipHostMallocDefault));
+            ctx->rb_cap = cap;
+        }
+        MH_CHECK(hipMemcpyAsync(ctx->rb_pin, ctx->flag.p, P * sizeof(int), hipMemcpyDeviceToHost, st));
+        MH_CHECK(hipEventRecord(ctx->ev_rb, st));
+        {
+            double fb2 = 0;
+            for (size_t f = 0; f < F; ++f) fb2 += 4.0 * n[f] * cnt[f] + 4.0 * n[f];
+            KTimer kt(ctx, K_FINALIZE, 0.0, fb2);
+            hipLaunchKernelGGL(k_finalize_cm, dim3(grid_for(pF.items.size())), dim3(256), 0, st, ctx->d_fits.p,
+                               uF.items, nF, nullptr, ctx->ssum.p, ctx->spart.p);
+            MH_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, P, uF.cmap, ctx->spart.p,
+                               nullptr, ybx, lin);
+            MH_LAUNCH_CHECK();
+        }
+        MH_CHECK(hipEventSynchronize(ctx->ev_rb));
+        flags = ctx->rb_pin;
+    } else {
+        flag_v.resize(P);
+        MH_CHECK(hipMemcpyAsync(flag_v.data(), ctx->flag.p, P * sizeof(int), hipMemcpyDeviceToHost, st));
+        ctx_sync(ctx);
+        flags = flag_v.data();
+    }
     // DGKS retry of the flagged columns, grouped by fit (kernels.cc:387-399)
-    std::vector<int> flags(P);
-    MH_CHECK(hipMemcpyAsync(flags.data(), ctx->flag.p, P * sizeof(int), hipMemcpyDeviceToHost, st));
-    ctx_sync(ctx);
     std::vector<int> fired, live;
     std::vector<int> rb(F, 0), re(F, 0), lb(F, 0), le(F, 0);
     for (size_t f = 0; f < F; ++f) {
@@ -552,7 +589,23 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                            vP.cmap, ctx->spart.p, kSlots, nullptr, ctx->d_fits.p, cs.d_fit.p,
                            ctx->ssum.p, nullptr);
         MH_LAUNCH_CHECK();
+        if (spec) {  // pass 3 again for the fired candidates, over their retried residuals
+            Pass fF = make_pass(rb, re, n, split_rows(n, fcnt, 16, 2048, 512), 16, 32);
+            Uploaded wF;
+            ctx->d_litems.ensure(fF.items.size() + 1);
+            ctx->d_lcmap.ensure(fF.cmap.size() + 1);
+            upload_pass(ctx, fF, ctx->d_litems, ctx->d_lcmap, 0, 0, wF);
+            ctx->spart.ensure(fF.items.size() * kTileF + 1);
+            hipLaunchKernelGGL(k_finalize_cm, dim3(grid_for(fF.items.size())), dim3(256), 0, st, ctx->d_fits.p,
+                               wF.items, (int)fF.items.size(), (const int *)ctx->fired_list.p, ctx->ssum.p,
+                               ctx->spart.p);
+            MH_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_yreduce, dim3((nf + 255) / 256), dim3(256), 0, st, nf, wF.cmap, ctx->spart.p,
+                               ctx->fired_list.p, ybx, lin);
+            MH_LAUNCH_CHECK();
+        }
     }
+    if (spec) return;
     // Pass 3: Bxn = f32(r * scale), ybx = sum Bxn y
     {
         double fb2 = 0;
@@ -560,23 +613,14 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         KTimer kt(ctx, K_FINALIZE, store_r ? 0.0 : flops, fb2);
         if (store_r) {
             // Bxn = f32(r scale), ybx = sum Bxn y; a candidate with s <= tol has scale 0, so ybx = 0
+            // (row-major normalised columns, kept for the hinge sweep or a download)
             const dim3 gF(grid_for(pF.items.size()));
-            const int nF = (int)pF.items.size();
-            if (ctx->rbuf_cm)
-                hipLaunchKernelGGL(k_finalize_cm, gF, dim3(256), 0, st, ctx->d_fits.p, uF.items, nF, ctx->ssum.p,
-                                   ctx->spart.p);
-            else if (keep_bxn && narrow)
+            if (narrow)
                 hipLaunchKernelGGL((k_finalize<true, 16>), gF, dim3(256), 0, st, ctx->d_fits.p, uF.items, nF,
                                    ctx->ssum.p, ctx->spart.p);
-            else if (keep_bxn)
+            else
                 hipLaunchKernelGGL((k_finalize<true, kTileF>), gF, dim3(256), 0, st, ctx->d_fits.p, uF.items, nF,
                                    ctx->ssum.p, ctx->spart.p);
-            else if (narrow)
-                hipLaunchKernelGGL((k_finalize<false, 16>), gF, dim3(256), 0, st, ctx->d_fits.p, uF.items, nF,
-                                   ctx->ssum.p, ctx->spart.p);
-            else
-                hipLaunchKernelGGL((k_finalize<false, kTileF>), gF, dim3(256), 0, st, ctx->d_fits.p, uF.items,
-                                   nF, ctx->ssum.p, ctx->spart.p);
             MH_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, P, uF.cmap, ctx->spart.p,
                                nullptr, ybx, lin);
