@@ -19,8 +19,9 @@ struct Pass {
  * fit's rbuf (row-major, column c - c0).  `xover`/`sover`, when set, replace
  * fit 0's X and scales (mirror-mode single-column calls).  `rescore` (a few
  * candidates per fit, the mixed form's exact rescoring): the Sum r^2 pass
- * keeps every residual, so the DGKS retry and the final pass read r instead
- * of recomputing it from Bo.
+ * keeps every residual (column-major), so the DGKS retry and the final pass
+ * read r instead of recomputing it from Bo; the final pass runs while the
+ * DGKS flags come back, and the fired candidates get it again after retry.
  */
 void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, double *ybx,
                  const float *xover = nullptr, const float *sover = nullptr, bool rescore = false);

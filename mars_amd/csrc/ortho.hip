@@ -5,12 +5,14 @@
 // reference's f32 rounding points:
 //   Bx  = f32(B[:,par] * f32(X[:,xc] * s[xc]))          rebuilt on the fly, never stored
 //   T   = Bo^T Bx                     FP64 MFMA, split over rows, fixed-order reduce
-//   r   = f32(Bx - Bo T),  s = sum r^2  FP64 MFMA; r kept row-major in HBM (rbuf)
+//   r   = f32(Bx - Bo T),  s = sum r^2  FP64 MFMA; r, when kept, goes to rbuf
+//         (column-major for linear scoring, row-major for the hinge sweep)
 //   DGKS: s > tol and 9 s < |T|^2  ->  T2 = Bo^T r, r = f32(r - Bo T2), s = sum r^2
 //   Bxn = f32(r * f32(1/sqrt(s + tol))) (0 if s <= tol),  ybx = sum Bxn y
 // Work is described by items (fit, candidate tile, row range) consumed by
 // persistent grids; every reduction runs in a fixed order, so results are
-// run-to-run identical and independent of the other fits in the batch.
+// run-to-run identical for a given batch (row ranges are sized from the whole
+// batch's work, so a fit's summation order depends on the batch it is in).
 #include "internal.h"
 #include "kernels.h"
 #include "stream.h"
@@ -72,13 +74,13 @@ __global__ void k_sreduce(const int *__restrict__ list, int count, const int *__
 }
 
 /*
- * Normalize and project on y: Bxn = f32(r * scale), ybx partial = sum Bxn y.
+ * Normalize in place and project on y (row-major columns kept for the hinge sweep or a
+ * download): Bxn = f32(r * scale) replaces r, ybx partial = sum Bxn y.
  * 256 threads = 256/TF row phases x TF candidates: thread (g, slot) takes rows
  * r0 + g, r0 + g + 256/TF, ... (eight loads in flight), and the phase sums are
- * added in a fixed order.  TF = 16 serves small candidate sets (the mixed form's
- * rescoring), TF = kTileF the others.
+ * added in a fixed order.  TF = 16 serves small candidate sets, TF = kTileF the others.
  */
-template <bool STORE, int TF>
+template <int TF>
 __global__ __launch_bounds__(256) void k_finalize(const FitDev *__restrict__ fits,
                                                   const Item *__restrict__ items, int nitems,
                                                   const double *__restrict__ ssum,
@@ -108,13 +110,13 @@ __global__ __launch_bounds__(256) void k_finalize(const FitDev *__restrict__ fit
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     const float bxn = __fmul_rn(r[u], scale);
-                    if (STORE) base[(row + NPH * u) * F.rsr] = bxn;
+                    base[(row + NPH * u) * F.rsr] = bxn;
                     acc = fma((double)bxn, (double)yv[u], acc);
                 }
             }
             for (; row < I.r1; row += NPH) {
                 const float bxn = __fmul_rn(base[row * F.rsr], scale);
-                if (STORE) base[row * F.rsr] = bxn;
+                base[row * F.rsr] = bxn;
                 acc = fma((double)bxn, (double)F.y[row], acc);
             }
         }
@@ -616,10 +618,10 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
             // (row-major normalised columns, kept for the hinge sweep or a download)
             const dim3 gF(grid_for(pF.items.size()));
             if (narrow)
-                hipLaunchKernelGGL((k_finalize<true, 16>), gF, dim3(256), 0, st, ctx->d_fits.p, uF.items, nF,
+                hipLaunchKernelGGL((k_finalize<16>), gF, dim3(256), 0, st, ctx->d_fits.p, uF.items, nF,
                                    ctx->ssum.p, ctx->spart.p);
             else
-                hipLaunchKernelGGL((k_finalize<true, kTileF>), gF, dim3(256), 0, st, ctx->d_fits.p, uF.items, nF,
+                hipLaunchKernelGGL((k_finalize<kTileF>), gF, dim3(256), 0, st, ctx->d_fits.p, uF.items, nF,
                                    ctx->ssum.p, ctx->spart.p);
             MH_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, P, uF.cmap, ctx->spart.p,
