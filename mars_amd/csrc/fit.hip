@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void k_app_build(const FitDev *__restrict__ fi
  * two passes over the rows.  MGS's coefficient of column j is c_j = q_j . v_j with
  * v_j = v - sum_{l<j} c_l q_l, which equals d_j - sum_{l<j} c_l G_jl for d = Q^T v and the
  * Gram matrix G = Q^T Q of the stored f32 columns (all products exact in FP64).  So:
- *   k_app_dots   partials of d (16 columns per block row), and on the first sweep of an
+ *   k_app_dots   partials of d (kAppG columns per block), and on the first sweep of an
  *                append also the Gram row of column m-1 (the newest one) and the stored
  *                raw-norm-normalized B column (marsalgo.cc:829-836);
  *   k_app_solve  per fit: d and the new Gram row from their partials, then the recurrence
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256) void k_app_build(const FitDev *__restrict__ fi
  *                partials of |v|^2.
  * `gate` (retry sweep) makes a launch a no-op for fits whose DGKS gate did not fire.
  */
-constexpr int kAppG = 16;             // Bo columns per k_app_dots block
+constexpr int kAppG = 8;              // Bo columns per k_app_dots block
 constexpr int kMaxAppendCols = 512;   // basis columns the device append orthogonalizes against
 
 __global__ __launch_bounds__(256) void k_app_dots(const FitDev *__restrict__ fits,
