@@ -35,9 +35,9 @@ constexpr int kSlots = 64;   // candidate slots per item in the partial-sum layo
 constexpr int cg_tgemm(int mt) { return mt <= 2 ? 4 : (mt <= 8 ? 2 : 1); }
 constexpr int cg_project(int mt) { return mt <= 2 ? 4 : (mt <= 4 ? 2 : 1); }
 constexpr int waves_for(int mt) { return mt == 1 ? 2 : (mt <= 4 ? 3 : 2); }
-/* Narrow T-GEMM tiles (one 16-candidate group, the mixed form's rescoring) issue the loads of a
- * whole 32-row chunk before its MFMAs: their few loads per half-step alone leave HBM latency
- * exposed. */
+/* Narrow T-GEMM tiles (one 16-candidate group, the mixed form's rescoring) issue the loads of
+ * two whole 32-row chunks before their MFMAs: their few loads per half-step alone leave HBM
+ * latency exposed, and each column then streams 256-byte runs. */
 constexpr bool kHoist(int mt, int cg) { return cg == 1 && mt <= 2; }
 
 /* Work items are listed fit by fit and tile by tile in row order, so neighbouring items read
@@ -131,7 +131,8 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_tgemm_w(const FitDev *__r
                                                 const int *__restrict__ cpar, const int *__restrict__ list,
                                                 int ldT, double *__restrict__ tpart)
 {
-    constexpr int HF = kHoist(MT, CG) ? 2 : 1;  // half-steps whose loads are issued together
+    constexpr int HF = kHoist(MT, CG) ? 2 : 1;   // half-steps whose loads are issued together
+    constexpr int NCH = kHoist(MT, CG) ? 2 : 1;  // 32-row chunks per iteration
     const int lane = threadIdx.x, kr = lane >> 4, kc = lane & 15;
     for (int round = 0; round < nitems; round += gridDim.x) {
         const int cnt = min((int)gridDim.x, nitems - round);
@@ -156,43 +157,53 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_tgemm_w(const FitDev *__r
 #pragma unroll
             for (int u = 0; u < MT; ++u) acc[g][u] = v4d{0, 0, 0, 0};
 #pragma unroll 1
-        for (long long r = I.r0; r < I.r1; r += kWR) {
+        for (long long r = I.r0; r < I.r1; r += kWR * NCH) {
 #pragma unroll 1
             for (int h0 = 0; h0 < 2; h0 += HF) {
-                // all loads of HF half-steps first, unconditionally (see bx4)
-                float4 av4[HF][MT], bv4[HF][CG];
+                // all loads of HF half-steps of NCH chunks first, unconditionally within a chunk
+                // (see bx4); a chunk past the item's end is skipped whole (uniform branch)
+                float4 av4[NCH][HF][MT], bv4[NCH][HF][CG];
 #pragma unroll
-                for (int hh = 0; hh < HF; ++hh) {
-                    const long long row = r + 8 * kr + 4 * (h0 + hh);
+                for (int cc = 0; cc < NCH; ++cc) {
+                    if (cc > 0 && r + kWR * cc >= I.r1) continue;
 #pragma unroll
-                    for (int u = 0; u < MT; ++u) av4[hh][u] = gld4(bo[u] + row);
+                    for (int hh = 0; hh < HF; ++hh) {
+                        const long long row = r + kWR * cc + 8 * kr + 4 * (h0 + hh);
 #pragma unroll
-                    for (int g = 0; g < CG; ++g) bv4[hh][g] = bx4<RETRY>(F, c[g], row, I.r1);
+                        for (int u = 0; u < MT; ++u) av4[cc][hh][u] = gld4(bo[u] + row);
+#pragma unroll
+                        for (int g = 0; g < CG; ++g) bv4[cc][hh][g] = bx4<RETRY>(F, c[g], row, I.r1);
+                    }
                 }
 #pragma unroll
-                for (int hh = 0; hh < HF; ++hh) {
-                    double a[MT][4], b[CG][4];
+                for (int cc = 0; cc < NCH; ++cc) {
+                    if (cc > 0 && r + kWR * cc >= I.r1) continue;
 #pragma unroll
-                    for (int u = 0; u < MT; ++u) {
-                        a[u][0] = kok[u] ? av4[hh][u].x : 0.f;
-                        a[u][1] = kok[u] ? av4[hh][u].y : 0.f;
-                        a[u][2] = kok[u] ? av4[hh][u].z : 0.f;
-                        a[u][3] = kok[u] ? av4[hh][u].w : 0.f;
+                    for (int hh = 0; hh < HF; ++hh) {
+                        double a[MT][4], b[CG][4];
+#pragma unroll
+                        for (int u = 0; u < MT; ++u) {
+                            a[u][0] = kok[u] ? av4[cc][hh][u].x : 0.f;
+                            a[u][1] = kok[u] ? av4[cc][hh][u].y : 0.f;
+                            a[u][2] = kok[u] ? av4[cc][hh][u].z : 0.f;
+                            a[u][3] = kok[u] ? av4[cc][hh][u].w : 0.f;
+                        }
+#pragma unroll
+                        for (int g = 0; g < CG; ++g) {
+                            b[g][0] = bv4[cc][hh][g].x;
+                            b[g][1] = bv4[cc][hh][g].y;
+                            b[g][2] = bv4[cc][hh][g].z;
+                            b[g][3] = bv4[cc][hh][g].w;
+                        }
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+#pragma unroll
+                            for (int g = 0; g < CG; ++g)
+                                if (16 * g < I.nc)
+#pragma unroll
+                                    for (int u = 0; u < MT; ++u)
+                                        acc[g][u] = mfma_f64(a[u][q], b[g][q], acc[g][u]);
                     }
-#pragma unroll
-                    for (int g = 0; g < CG; ++g) {
-                        b[g][0] = bv4[hh][g].x;
-                        b[g][1] = bv4[hh][g].y;
-                        b[g][2] = bv4[hh][g].z;
-                        b[g][3] = bv4[hh][g].w;
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-#pragma unroll
-                        for (int g = 0; g < CG; ++g)
-                            if (16 * g < I.nc)
-#pragma unroll
-                                for (int u = 0; u < MT; ++u) acc[g][u] = mfma_f64(a[u][q], b[g][q], acc[g][u]);
                 }
             }
         }
