@@ -779,15 +779,8 @@ void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std
         hipLaunchKernelGGL(k_app_build, grid, dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, pa, ctx->scal.p);
         MH_LAUNCH_CHECK();
         for (int pass = 0; pass < 2; ++pass) {
-            if (pass == 1) {
-                // the retry sweep only runs when some fit's DGKS gate fired
-                std::vector<int> g(F);
-                MH_CHECK(hipMemcpyAsync(g.data(), gate, F * sizeof(int), hipMemcpyDeviceToHost, st));
-                ctx_sync(ctx);
-                bool any = false;
-                for (size_t f = 0; f < F; ++f) any = any || (args[f].active && g[f]);
-                if (!any) break;
-            }
+            // the retry sweep is launched without reading the gates back: its blocks return at
+            // once for fits whose DGKS gate did not fire
             const int first = pass == 0 ? 1 : 0;
             const int *gp = pass == 0 ? nullptr : gate;
             const unsigned kg = (unsigned)((mmax + kAppG - 1) / kAppG);

@@ -154,24 +154,27 @@ __global__ __launch_bounds__(256) void k_finalize_cm(const FitDev *__restrict__ 
             const float scale = (s > F.tol) ? (float)(1.0 / sqrt(s + F.tol)) : 0.0f;
             const float *col = F.rbuf + (c - F.c0) * F.rsc;
             double acc = 0.0;
-            for (long long r = I.r0 + 4 * lane; r < I.r1; r += 1024) {
+            // s <= tol: scale 0, so ybx = 0 exactly (kernels.cc:401-416) and r is not read; most
+            // rescored candidates (duplicates of basis functions) end here
+            const long long rend = s > F.tol ? I.r1 : I.r0;
+            for (long long r = I.r0 + 4 * lane; r < rend; r += 1024) {
                 float4 rv[4], yv[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const long long rr = r + 256 * u;
-                    const long long ra = rr < I.r1 ? rr : r;  // clamped: stays inside the column
+                    const long long ra = rr < rend ? rr : r;  // clamped: stays inside the column
                     rv[u] = gld4(col + ra);
                     yv[u] = gld4(F.y + ra);
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const long long rr = r + 256 * u;
-                    if (rr >= I.r1) continue;
+                    if (rr >= rend) continue;
                     const float a[4] = {rv[u].x, rv[u].y, rv[u].z, rv[u].w};
                     const float b[4] = {yv[u].x, yv[u].y, yv[u].z, yv[u].w};
 #pragma unroll
                     for (int v = 0; v < 4; ++v)
-                        if (rr + v < I.r1) acc = fma((double)__fmul_rn(a[v], scale), (double)b[v], acc);
+                        if (rr + v < rend) acc = fma((double)__fmul_rn(a[v], scale), (double)b[v], acc);
                 }
             }
 #pragma unroll
