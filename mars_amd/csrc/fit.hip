@@ -289,13 +289,11 @@ __global__ __launch_bounds__(256) void k_app_solve(const FitDev *__restrict__ fi
     // thread k sums the partials of d_k (and of the Gram row entry) in block order
     for (int k = threadIdx.x; k < m; k += 256) {
         const double *pd = dpart + ((size_t)f * F.ldb + k) * nb;
-        double a = 0.0;
-        for (int b = 0; b < nb; ++b) a += pd[b];
+        const double a = ordered_sum(pd, nb, [](int b) { return (size_t)b; });
         sd[k] = a;
         if (first && k < m - 1) {
             const double *pg = gpart + ((size_t)f * F.ldb + k) * nb;
-            double g = 0.0;
-            for (int b = 0; b < nb; ++b) g += pg[b];
+            const double g = ordered_sum(pg, nb, [](int b) { return (size_t)b; });
             G[(size_t)(m - 1) * F.ldb + k] = g;
             sg[k] = g;
         }

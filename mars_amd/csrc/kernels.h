@@ -52,6 +52,26 @@ __device__ inline v4f mfma_f32(float a, float b, v4f c)
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+/*
+ * p[idx(0)] + p[idx(1)] + ... + p[idx(cnt-1)], added left to right (the fixed order every
+ * reduce kernel uses), with eight loads in flight: a plain loop waits for each load in turn.
+ */
+template <typename Idx>
+__device__ inline double ordered_sum(const double *p, int cnt, Idx idx)
+{
+    double s = 0.0;
+    int i = 0;
+    for (; i + 8 <= cnt; i += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = gld(p + idx(i + u));
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; i < cnt; ++i) s += gld(p + idx(i));
+    return s;
+}
+
 /* Fixed-order block sum of one double per thread (blockDim.x == 256). */
 __device__ inline double block_sum_256(double v, double *sh)
 {

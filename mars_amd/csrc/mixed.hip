@@ -233,19 +233,18 @@ __global__ __launch_bounds__(256) void k_tc_reduce(const FitDev *__restrict__ fi
     for (int e = threadIdx.x; e < nc * 16; e += 256) {
         const int xl = e >> 4, al = e & 15, a = 16 * u + al;
         if (a >= na) continue;
-        double s = 0.0;
-        for (int i = 0; i < count; ++i) s += tpart[((size_t)glist[first + i] * kXT + xl) * 16 + al];
+        const double s = ordered_sum(
+            tpart, count, [&](int i) { return ((size_t)glist[first + i] * kXT + xl) * 16 + al; });
         const long long xc = xc0 + xl;
         const int k = a <= j ? a : j, par = a <= j ? j : a - j - 1;
         F.tc[(xc * ldb + par) * ldb + k] = s;
     }
     if (u != 0) return;
     for (int xl = threadIdx.x; xl < nc; xl += 256) {
-        double s0 = 0.0, s1 = 0.0;
-        for (int i = 0; i < count; ++i) {
-            s0 += npart[((size_t)glist[first + i] * kXT + xl) * 2];
-            s1 += npart[((size_t)glist[first + i] * kXT + xl) * 2 + 1];
-        }
+        const double s0 =
+            ordered_sum(npart, count, [&](int i) { return ((size_t)glist[first + i] * kXT + xl) * 2; });
+        const double s1 =
+            ordered_sum(npart, count, [&](int i) { return ((size_t)glist[first + i] * kXT + xl) * 2 + 1; });
         F.tcn[((xc0 + xl) * ldb + j) * 2] = s0;
         F.tcn[((xc0 + xl) * ldb + j) * 2 + 1] = s1;
     }

@@ -42,8 +42,8 @@ __global__ __launch_bounds__(64) void k_treduce(const FitDev *__restrict__ fits,
     const int first = cmap[3 * c], cnt = cmap[3 * c + 1], slot = cmap[3 * c + 2];
     double tt = 0.0;
     for (int k = lane; k < ldT; k += 64) {
-        double s = 0.0;
-        for (int i = 0; i < cnt; ++i) s += tpart[((size_t)(first + i) * 64 + slot) * ldT + k];
+        const double s =
+            ordered_sum(tpart, cnt, [&](int i) { return ((size_t)(first + i) * 64 + slot) * ldT + k; });
         T[(size_t)c * ldT + k] = s;
         if (k < m) tt = fma(s, s, tt);
     }
@@ -62,8 +62,7 @@ __global__ void k_sreduce(const int *__restrict__ list, int count, const int *__
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= count) return;
     const int first = cmap[3 * c], cnt = cmap[3 * c + 1], slot = cmap[3 * c + 2];
-    double s = 0.0;
-    for (int i = 0; i < cnt; ++i) s += spart[(size_t)(first + i) * tile + slot];
+    const double s = ordered_sum(spart, cnt, [&](int i) { return (size_t)(first + i) * tile + slot; });
     if (list) {
         ssum[list[c]] = s;
     } else {
@@ -191,8 +190,7 @@ __global__ void k_yreduce(int count, const int *__restrict__ cmap, const double 
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= count) return;
     const int first = cmap[3 * c], cnt = cmap[3 * c + 1], slot = cmap[3 * c + 2];
-    double s = 0.0;
-    for (int i = 0; i < cnt; ++i) s += ypart[(size_t)(first + i) * kTileF + slot];
+    const double s = ordered_sum(ypart, cnt, [&](int i) { return (size_t)(first + i) * kTileF + slot; });
     const int cand = list ? list[c] : c;
     if (ybx) ybx[cand] = s;
     lin[cand] = s * s;
