@@ -126,6 +126,18 @@ struct FitW {
     double *G;  // Gram matrix of the stored Bo columns, G[j * ldb + l] for l < j
 };
 
+/* Deferred small uploads: each is staged in pinned memory at `off`; one copy sends the staged
+ * range to the device mirror and one kernel scatters the pieces to their destinations. */
+struct UpDesc {
+    void *dst;
+    unsigned long long off, bytes;
+};
+constexpr int kUpMax = 16;  // pending uploads per flush (the table travels as a kernel argument)
+struct UpTable {
+    UpDesc d[kUpMax];
+    int n = 0;
+};
+
 /* Result slot returned by the device append. */
 struct AppendOut {
     double mse;
@@ -192,6 +204,8 @@ struct mars_hip_ctx {
     // pinned staging for host-to-device copies: bump-allocated, reset after a stream sync
     char *stage = nullptr;
     size_t stage_cap = 0, stage_off = 0;
+    mh::DBuf<char> mirror;           // device copy of the staging space, for deferred uploads
+    mh::UpTable up_pend;             // deferred uploads not yet sent
 
     size_t max_terms = 0;  // basis capacity (all fits)
     size_t ldb = 0;        // row stride of B and Bo (>= max_terms, multiple of 4)

@@ -582,6 +582,29 @@ void fit_state_alloc(mars_hip_ctx *ctx, FitState &fs, size_t n, double tol)
 
 constexpr size_t kStageMax = 16u << 20;  // larger copies (one-time data uploads) go direct
 
+/* Room for `bytes` in the pinned staging space; returns its offset.  A full space is recycled
+ * after a synchronisation (every copy out of it has completed). */
+size_t stage_take(mars_hip_ctx *ctx, size_t bytes)
+{
+    const size_t need = (bytes + 255) / 256 * 256;
+    if (ctx->stage_off + need > ctx->stage_cap) {
+        ctx_sync(ctx);
+        if (need > ctx->stage_cap) {
+            if (ctx->stage) MH_CHECK(hipHostFree(ctx->stage));
+            ctx->stage = nullptr;
+            ctx->stage_cap = 0;
+            ctx->mirror.release();
+            const size_t cap = std::max<size_t>(need, 4u << 20);
+            MH_CHECK(hipHostMalloc((void **)&ctx->stage, cap, hipHostMallocDefault));
+            ctx->stage_cap = cap;
+            ctx->mirror.ensure(cap);
+        }
+    }
+    const size_t off = ctx->stage_off;
+    ctx->stage_off += need;
+    return off;
+}
+
 void h2d(mars_hip_ctx *ctx, void *dst, const void *src, size_t bytes)
 {
     if (bytes == 0) return;
@@ -589,26 +612,73 @@ void h2d(mars_hip_ctx *ctx, void *dst, const void *src, size_t bytes)
         MH_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
         return;
     }
-    const size_t need = (bytes + 255) / 256 * 256;
-    if (ctx->stage_off + need > ctx->stage_cap) {
-        ctx_sync(ctx);  // every copy out of the staging space has completed
-        if (need > ctx->stage_cap) {
-            if (ctx->stage) MH_CHECK(hipHostFree(ctx->stage));
-            ctx->stage = nullptr;
-            ctx->stage_cap = 0;
-            const size_t cap = std::max<size_t>(need, 4u << 20);
-            MH_CHECK(hipHostMalloc((void **)&ctx->stage, cap, hipHostMallocDefault));
-            ctx->stage_cap = cap;
-        }
-    }
-    char *h = ctx->stage + ctx->stage_off;
+    const size_t off = stage_take(ctx, bytes);  // may (re)allocate ctx->stage: read it afterwards
+    char *h = ctx->stage + off;
     std::memcpy(h, src, bytes);
-    ctx->stage_off += need;
     MH_CHECK(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+}
+
+constexpr unsigned kScatterChunk = 16384;  // bytes per k_upload_scatter block
+
+/* Block (x, y) copies bytes [x * kScatterChunk, ...) of upload y from the device mirror. */
+__global__ __launch_bounds__(256) void k_upload_scatter(const char *__restrict__ mirror, UpTable t)
+{
+    const UpDesc u = t.d[blockIdx.y];
+    const unsigned long long b0 = (unsigned long long)blockIdx.x * kScatterChunk;
+    if (b0 >= u.bytes) return;
+    const unsigned long long b1 = min(u.bytes, b0 + kScatterChunk);
+    const unsigned *src = (const unsigned *)(mirror + u.off);
+    unsigned *dst = (unsigned *)u.dst;
+    for (unsigned long long i = b0 / 4 + threadIdx.x; i < b1 / 4; i += 256) dst[i] = src[i];
+}
+
+void h2d_defer(mars_hip_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return;
+    if (bytes > kStageMax || bytes % 4 != 0) {
+        h2d(ctx, dst, src, bytes);
+        return;
+    }
+    // a second upload to a pending destination replaces the first (the scatter kernel's blocks
+    // run in any order, so one flush never writes a destination twice)
+    int same = -1;
+    for (int i = 0; i < ctx->up_pend.n; ++i)
+        if (ctx->up_pend.d[i].dst == dst) same = i;
+    if (same >= 0 && ctx->up_pend.d[same].bytes != bytes) {
+        h2d_flush(ctx);
+        same = -1;
+    }
+    if (same < 0 && ctx->up_pend.n == kUpMax) h2d_flush(ctx);
+    const size_t off = stage_take(ctx, bytes);  // may synchronise, which flushes what is pending
+    std::memcpy(ctx->stage + off, src, bytes);
+    if (same >= 0 && ctx->up_pend.n > 0)
+        ctx->up_pend.d[same].off = off;
+    else
+        ctx->up_pend.d[ctx->up_pend.n++] = UpDesc{dst, off, bytes};
+}
+
+void h2d_flush(mars_hip_ctx *ctx)
+{
+    UpTable &t = ctx->up_pend;
+    if (t.n == 0) return;
+    // one copy covers every pending piece of the staging space
+    unsigned long long lo = t.d[0].off, hi = 0, most = 0;
+    for (int i = 0; i < t.n; ++i) {
+        lo = std::min(lo, t.d[i].off);
+        hi = std::max(hi, t.d[i].off + t.d[i].bytes);
+        most = std::max(most, t.d[i].bytes);
+    }
+    MH_CHECK(hipMemcpyAsync(ctx->mirror.p + lo, ctx->stage + lo, hi - lo, hipMemcpyHostToDevice, ctx->stream));
+    const UpTable send = t;
+    t.n = 0;
+    hipLaunchKernelGGL(k_upload_scatter, dim3((unsigned)((most + kScatterChunk - 1) / kScatterChunk), (unsigned)send.n),
+                       dim3(256), 0, ctx->stream, (const char *)ctx->mirror.p, send);
+    MH_LAUNCH_CHECK();
 }
 
 void ctx_sync(mars_hip_ctx *ctx)
 {
+    h2d_flush(ctx);
     MH_CHECK(hipStreamSynchronize(ctx->stream));
     ctx->stage_off = 0;
 }
@@ -619,6 +689,7 @@ void ctx_free(mars_hip_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->stage) (void)hipHostFree(ctx->stage);
     ctx->stage = nullptr;
+    ctx->mirror.release();
     for (auto &fs : ctx->fits) fs.release();
     DBuf<double> *db[] = {&ctx->tpart, &ctx->T, &ctx->T2, &ctx->tnorm, &ctx->spart, &ctx->ssum, &ctx->ybx,
                           &ctx->lin, &ctx->hin, &ctx->aux, &ctx->red, &ctx->vcol, &ctx->scal,
@@ -720,6 +791,7 @@ void fit_init(mars_hip_ctx *ctx, FitState &fs, const float *X, size_t ldX, const
  */
 void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std::vector<AppendOut> &out)
 {
+    PendingScope pending_scope(ctx);
     require(ctx->resident, "context was not created by mars_hip_fit_create");
     const size_t F = ctx->fits.size();
     require(args.size() == F, "one append argument per fit");
@@ -751,7 +823,7 @@ void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std
     ctx->red.ensure(3 * F * nb);
     ctx->scal.ensure(F * S_COUNT);
     ctx->flag.ensure(F);
-    h2d(ctx, ctx->aarg.p, args.data(), F * sizeof(AppendArg));
+    h2d_defer(ctx, ctx->aarg.p, args.data(), F * sizeof(AppendArg));
     std::vector<FitW> fw(F);
     for (size_t f = 0; f < F; ++f) {
         FitState &fs = ctx->fits[f];
@@ -759,7 +831,7 @@ void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std
         fw[f] = FitW{fs.B.p, fs.Bo.p, fs.BoC.p, fs.BC.p, fs.ybo.p, fs.G.p};
     }
     ctx->d_fitw.ensure(F);
-    h2d(ctx, ctx->d_fitw.p, fw.data(), F * sizeof(FitW));
+    h2d_defer(ctx, ctx->d_fitw.p, fw.data(), F * sizeof(FitW));
     double *pa = ctx->red.p, *pb = pa + F * nb, *pc = pb + F * nb;
     ctx->adot.ensure(F * ctx->ldb * nb);
     ctx->agram.ensure(F * ctx->ldb * nb);
@@ -772,6 +844,7 @@ void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std
             mm = std::max(mm, (double)ctx->fits[f].m);
         }
     {
+        h2d_flush(ctx);  // fits, arguments and column pointers in one copy
         KTimer kt(ctx, K_APPEND, 0.0, 8.0 * nrows * (3.0 * (mm + 1) + 4));
         const dim3 grid(nb, F);
         hipLaunchKernelGGL(k_app_build, grid, dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, pa, ctx->scal.p);
@@ -828,6 +901,7 @@ double host_dsse(const FitState &fs)
 /* Per-fit best linear candidate (first maximum in row-major (xcol, bcol) order). */
 void argmax_fits(mars_hip_ctx *ctx, const Cands &cs, const double *lin, std::vector<BestOut> &best)
 {
+    PendingScope pending_scope(ctx);
     const size_t F = ctx->fits.size();
     ctx->ival.ensure(2 * F);
     ctx->bout.ensure(F);
@@ -836,7 +910,8 @@ void argmax_fits(mars_hip_ctx *ctx, const Cands &cs, const double *lin, std::vec
         ol[f] = cs.fit_c0[f];
         ol[F + f] = cs.fit_P[f];
     }
-    h2d(ctx, ctx->ival.p, ol.data(), 2 * F * sizeof(int));
+    h2d_defer(ctx, ctx->ival.p, ol.data(), 2 * F * sizeof(int));
+    h2d_flush(ctx);
     {
         KTimer kt(ctx, K_ARGMAX, 0.0, 8.0 * cs.fit.size());
         hipLaunchKernelGGL(k_argmax, dim3(F), dim3(1024), 0, ctx->stream, lin, nullptr, nullptr, ctx->ival.p,
@@ -852,6 +927,7 @@ void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t 
               int end_span, int linear_only, double *lin_out, double *hin_out, double *cut_out,
               mars_hip_best *best)
 {
+    PendingScope pending_scope(ctx);
     require(ctx->resident, "context was not created by mars_hip_fit_create");
     require(ctx->fits.size() == 1, "use mars_hip_batch_fit on batch contexts");
     FitState &fs = ctx->fits[0];
@@ -901,6 +977,7 @@ void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t 
             if (!linear_only) hinge_sweep(ctx, min_span, end_span);
         }
         upload_cands(ctx, cs);
+        h2d_flush(ctx);
         hipLaunchKernelGGL(k_scatter, dim3((P + 255) / 256), dim3(256), 0, st, cs.d_xc.p, cs.d_par.p, P, (int)m,
                            ctx->lin.p, ctx->hin.p, ctx->knot.p, ctx->krow.p, fs.X.p, fs.ldx, linear_only ? 0 : 1,
                            dlin, dhin, dcut, dknot);

@@ -3,6 +3,8 @@
 
 #include "common.h"
 
+#include <exception>
+
 namespace mh {
 
 /* Work list of one batched pass plus, per candidate, where its partial sums live. */
@@ -69,6 +71,22 @@ extern thread_local std::string g_last_error;
  * staging, so the call does not wait for the stream (a pageable copy does); the source may be
  * reused as soon as it returns. */
 void h2d(mars_hip_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Deferred small upload: staged now, sent by the next h2d_flush (or ctx_sync) together with the
+ * other pending ones (one copy and one scatter kernel instead of a copy each).  The caller
+ * flushes before launching any kernel that reads `dst`, and does not reallocate `dst` first. */
+void h2d_defer(mars_hip_ctx *ctx, void *dst, const void *src, size_t bytes);
+void h2d_flush(mars_hip_ctx *ctx);
+/* Held by every function that defers uploads: an exception leaving it drops what is still
+ * pending, so no later flush writes into a buffer reallocated in the meantime. */
+struct PendingScope {
+    mars_hip_ctx *ctx;
+    int ex;
+    explicit PendingScope(mars_hip_ctx *c) : ctx(c), ex(std::uncaught_exceptions()) {}
+    ~PendingScope()
+    {
+        if (std::uncaught_exceptions() > ex) ctx->up_pend.n = 0;
+    }
+};
 /* hipStreamSynchronize on the context's stream; the staging space is free again afterwards. */
 void ctx_sync(mars_hip_ctx *ctx);
 

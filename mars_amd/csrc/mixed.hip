@@ -481,8 +481,9 @@ void tc_update(mars_hip_ctx *ctx)
         const size_t ni = items.size(), ng = nh / kGroupInts, nsub = ni * W;
         ctx->d_items.ensure(ni);
         ctx->d_cmap.ensure(groups.size());
-        h2d(ctx, ctx->d_items.p, items.data(), ni * sizeof(Item));
-        h2d(ctx, ctx->d_cmap.p, groups.data(), groups.size() * sizeof(int));
+        h2d_defer(ctx, ctx->d_items.p, items.data(), ni * sizeof(Item));
+        h2d_defer(ctx, ctx->d_cmap.p, groups.data(), groups.size() * sizeof(int));
+        h2d_flush(ctx);  // with the fits and candidates deferred before the update
         ctx->tpart.ensure(nsub * kXT * 16);
         ctx->aux.ensure(nsub * kXT * 2);
         {
@@ -511,14 +512,17 @@ void tc_update(mars_hip_ctx *ctx)
 
 void mixed_prepare(mars_hip_ctx *ctx)
 {
+    PendingScope pending_scope(ctx);
     ensure_tcache(ctx);
     Cands none;  // the cache update reads no candidate list
     upload_fits(ctx, none);
+    h2d_flush(ctx);
     tc_update(ctx);
 }
 
 void mixed_score(mars_hip_ctx *ctx)
 {
+    PendingScope pending_scope(ctx);
     Cands &cs = ctx->cand;
     const int P = (int)cs.fit.size();
     if (P == 0) return;
@@ -528,6 +532,7 @@ void mixed_score(mars_hip_ctx *ctx)
     ensure_tcache(ctx);
     upload_fits(ctx, cs);
     upload_cands(ctx, cs);
+    h2d_flush(ctx);  // k_tc_score reads them whether or not the update below has work
     tc_update(ctx);
     ctx->flag.ensure(P);
     ctx->lin.ensure(P);
@@ -552,7 +557,8 @@ void mixed_score(mars_hip_ctx *ctx)
     ctx->red.ensure(F);
     ctx->rsel.ensure(P);
     int *d_nsel = ctx->ival.p + 2 * F, *d_blk = d_nsel + 2;
-    h2d(ctx, ctx->ival.p, ranges.data(), 2 * F * sizeof(int));
+    h2d_defer(ctx, ctx->ival.p, ranges.data(), 2 * F * sizeof(int));
+    h2d_flush(ctx);
     hipLaunchKernelGGL(k_fit_max, dim3((unsigned)F), dim3(256), 0, st, (const int *)ctx->ival.p,
                        (const int *)ctx->ival.p + F, (const double *)ctx->lin.p, (const int *)ctx->flag.p, ctx->red.p);
     MH_LAUNCH_CHECK();

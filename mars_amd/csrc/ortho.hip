@@ -275,9 +275,9 @@ void upload_pass(mars_hip_ctx *ctx, const Pass &p, DBuf<Item> &items, DBuf<int> 
     u.items = items.p + item_off;
     u.cmap = cmap.p + cmap_off;
     if (!p.items.empty())
-        h2d(ctx, u.items, p.items.data(), p.items.size() * sizeof(Item));
+        h2d_defer(ctx, u.items, p.items.data(), p.items.size() * sizeof(Item));
     if (!p.cmap.empty())
-        h2d(ctx, u.cmap, p.cmap.data(), p.cmap.size() * sizeof(int));
+        h2d_defer(ctx, u.cmap, p.cmap.data(), p.cmap.size() * sizeof(int));
 }
 
 int grid_for(size_t items)
@@ -360,9 +360,9 @@ void upload_cands(mars_hip_ctx *ctx, Cands &cs)
     cs.d_xc.ensure(P);
     cs.d_par.ensure(P);
     if (!P) return;
-    h2d(ctx, cs.d_fit.p, cs.fit.data(), P * sizeof(int));
-    h2d(ctx, cs.d_xc.p, cs.xc.data(), P * sizeof(int));
-    h2d(ctx, cs.d_par.p, cs.par.data(), P * sizeof(int));
+    h2d_defer(ctx, cs.d_fit.p, cs.fit.data(), P * sizeof(int));
+    h2d_defer(ctx, cs.d_xc.p, cs.xc.data(), P * sizeof(int));
+    h2d_defer(ctx, cs.d_par.p, cs.par.data(), P * sizeof(int));
 }
 
 void ensure_rbuf(mars_hip_ctx *ctx, const Cands &cs)
@@ -412,12 +412,13 @@ void upload_fits(mars_hip_ctx *ctx, const Cands &cs, const float *xover, const f
         d.tol = fs.tol;
     }
     ctx->d_fits.ensure(h.size());
-    h2d(ctx, ctx->d_fits.p, h.data(), h.size() * sizeof(FitDev));
+    h2d_defer(ctx, ctx->d_fits.p, h.data(), h.size() * sizeof(FitDev));
 }
 
 void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, double *ybx,
                  const float *xover, const float *sover, bool rescore)
 {
+    PendingScope pending_scope(ctx);
     const bool store_r = keep_bxn || rescore;  // every residual stays in rbuf
     const int P = (int)cs.fit.size();
     if (P == 0) return;
@@ -477,6 +478,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     upload_pass(ctx, pP, ctx->d_items, ctx->d_cmap, io, co, uP);
     io += pP.items.size(); co += pP.cmap.size();
     upload_pass(ctx, pF, ctx->d_items, ctx->d_cmap, io, co, uF);
+    h2d_flush(ctx);  // fits, candidates and the three work lists in one copy
 
     ctx->tpart.ensure(std::max<size_t>(pT.items.size(), 1) * kSlots * ldT);
     ctx->T.ensure((size_t)P * ldT);
@@ -564,7 +566,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         const int nf = (int)fired.size();
         ctx->dgks_host += nf;
         KTimer kt(ctx, K_DGKS, 0.0, 0.0);
-        h2d(ctx, ctx->fired_list.p, fired.data(), nf * sizeof(int));
+        h2d_defer(ctx, ctx->fired_list.p, fired.data(), nf * sizeof(int));
         std::vector<int> fcnt(F);
         for (size_t f = 0; f < F; ++f) fcnt[f] = re[f] - rb[f];
         const bool rnarrow = narrow_for(fcnt);
@@ -576,6 +578,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         ctx->d_rcmap.ensure(rT.cmap.size() + rP.cmap.size() + 1);
         upload_pass(ctx, rT, ctx->d_ritems, ctx->d_rcmap, 0, 0, vT);
         upload_pass(ctx, rP, ctx->d_ritems, ctx->d_rcmap, rT.items.size(), rT.cmap.size(), vP);
+        h2d_flush(ctx);
         ctx->tpart.ensure(std::max<size_t>(rT.items.size(), 1) * kSlots * ldT);
         ctx->spart.ensure(rP.items.size() * kSlots + 1);
         ctx->T2.ensure((size_t)nf * ldT);
@@ -598,6 +601,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
             ctx->d_litems.ensure(fF.items.size() + 1);
             ctx->d_lcmap.ensure(fF.cmap.size() + 1);
             upload_pass(ctx, fF, ctx->d_litems, ctx->d_lcmap, 0, 0, wF);
+            h2d_flush(ctx);
             ctx->spart.ensure(fF.items.size() * kTileF + 1);
             hipLaunchKernelGGL(k_finalize_cm, dim3(grid_for(fF.items.size())), dim3(256), 0, st, ctx->d_fits.p,
                                wF.items, (int)fF.items.size(), (const int *)ctx->fired_list.p, ctx->ssum.p,
@@ -642,8 +646,9 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                 ctx->live_list.ensure(nl);
                 ctx->d_litems.ensure(pL.items.size() + 1);
                 ctx->d_lcmap.ensure(pL.cmap.size() + 1);
-                h2d(ctx, ctx->live_list.p, live.data(), nl * sizeof(int));
+                h2d_defer(ctx, ctx->live_list.p, live.data(), nl * sizeof(int));
                 upload_pass(ctx, pL, ctx->d_litems, ctx->d_lcmap, 0, 0, uL);
+                h2d_flush(ctx);
                 ctx->spart.ensure(pL.items.size() * kSlots + 1);
                 launch_project<kFinalY, false, true>(ctx, mt, uL, (int)pL.items.size(), cs, ctx->live_list.p,
                                                      ctx->T.p, ldT, lnarrow);
