@@ -38,8 +38,14 @@ inline void require(bool ok, const char *msg)
 /* DGKS gate: retry when residual energy * 9 < projected energy (kernels.h:17). */
 constexpr double kGate = 9.0;
 
+/* Buffers replaced while queued kernels may still read them.  hipFree waits for the whole
+ * device, so they are freed at the next ctx_sync, after the stream has drained. */
+void retire_device(void *p);
+void retire_host(void *p);
+void free_retired();
+
 /* Device buffer that only grows: the first allocation is exact, a regrowth takes at least
- * 1.5x (lists that grow every step reallocate O(log) times; hipFree waits for the device). */
+ * 2x (lists that grow every step reallocate O(log) times) and retires the old block. */
 template <typename T>
 struct DBuf {
     T *p = nullptr;
@@ -47,8 +53,8 @@ struct DBuf {
     void ensure(size_t count)
     {
         if (count <= cap) return;
-        const size_t want = cap ? std::max(count, cap + cap / 2) : count;
-        if (p) MH_CHECK(hipFree(p));
+        const size_t want = cap ? std::max(count, 2 * cap) : count;
+        if (p) retire_device(p);
         p = nullptr;
         cap = 0;
         MH_CHECK(hipMalloc((void **)&p, sizeof(T) * (want ? want : 1)));

@@ -676,10 +676,29 @@ void h2d_flush(mars_hip_ctx *ctx)
     MH_LAUNCH_CHECK();
 }
 
+namespace {
+thread_local std::vector<void *> g_retired_dev, g_retired_host;
+}
+
+void retire_device(void *p) { g_retired_dev.push_back(p); }
+void retire_host(void *p) { g_retired_host.push_back(p); }
+
+void free_retired()
+{
+    if (g_retired_dev.empty() && g_retired_host.empty()) return;
+    // another context on this thread may still have work queued on a retired block
+    MH_CHECK(hipDeviceSynchronize());
+    for (void *p : g_retired_dev) MH_CHECK(hipFree(p));
+    for (void *p : g_retired_host) MH_CHECK(hipHostFree(p));
+    g_retired_dev.clear();
+    g_retired_host.clear();
+}
+
 void ctx_sync(mars_hip_ctx *ctx)
 {
     h2d_flush(ctx);
     MH_CHECK(hipStreamSynchronize(ctx->stream));
+    free_retired();
     ctx->stage_off = 0;
 }
 
@@ -687,6 +706,10 @@ void ctx_free(mars_hip_ctx *ctx)
 {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    try {
+        free_retired();
+    } catch (...) {
+    }
     if (ctx->stage) (void)hipHostFree(ctx->stage);
     ctx->stage = nullptr;
     ctx->mirror.release();

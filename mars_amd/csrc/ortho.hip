@@ -520,11 +520,11 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     const int *flags = nullptr;
     if (spec) {
         if (ctx->rb_cap < (size_t)P) {
-            ctx_sync(ctx);
-            if (ctx->rb_pin) MH_CHECK(hipHostFree(ctx->rb_pin));
+            // an earlier readback may still be in flight into the old block: retire it
+            if (ctx->rb_pin) retire_host(ctx->rb_pin);
             ctx->rb_pin = nullptr;
+            const size_t cap = std::max<size_t>((size_t)P, std::max<size_t>(2 * ctx->rb_cap, 65536));
             ctx->rb_cap = 0;
-            const size_t cap = std::max<size_t>((size_t)P, 4096);
             MH_CHECK(hipHostMalloc((void **)&ctx->rb_pin, cap * sizeof(int), hipHostMallocDefault));
             ctx->rb_cap = cap;
         }
