@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 
 namespace mh {
 namespace {
@@ -120,7 +121,27 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
         std::vector<AppendOut> out;
         {
             HostTimer t(ctx, K_H_APPEND);
-            fit_append_batch(ctx, args, out);
+            // mixed form: the next step's cache update for the columns being appended is queued
+            // behind the append, assuming each succeeds; a degenerate append (rare) marks its
+            // column uncached again below, and the entries computed for it are never read
+            std::function<void()> ahead;
+            if (ctx->form == MARS_HIP_FORM_MIXED)
+                ahead = [&] {
+                    // (not for a fit this append fills: it takes no further step)
+                    std::vector<char> bump(F, 0);
+                    for (size_t f = 0; f < F; ++f)
+                        bump[f] = args[f].active && ctx->fits[f].m + 1 < D[f].max_terms;
+                    for (size_t f = 0; f < F; ++f)
+                        if (bump[f]) ctx->fits[f].m += 1;
+                    mixed_prepare(ctx);
+                    for (size_t f = 0; f < F; ++f)
+                        if (bump[f]) ctx->fits[f].m -= 1;
+                };
+            fit_append_batch(ctx, args, out, ahead);
+            for (size_t f = 0; f < F; ++f) {
+                FitState &fs = ctx->fits[f];
+                if (fs.mc > fs.m) fs.mc = fs.m;
+            }
         }
         for (size_t f = 0; f < F; ++f) {
             if (!args[f].active) continue;
@@ -165,6 +186,8 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
             }
         }
     }
+    ctx_sync(ctx);  // work queued ahead (an update no step consumed) finishes; every timer resolves
+    resolve_timing(ctx);
     for (size_t f = 0; f < F; ++f) {
         nterms[f] = (int)D[f].rows.size();
         for (size_t r = 0; r < D[f].rows.size(); ++r) terms[f * ctx->max_terms + r] = D[f].rows[r];

@@ -205,6 +205,9 @@ struct mars_hip_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t ev_rb = nullptr;      // marks a readback into rb_pin as complete
+    hipEvent_t ev_app = nullptr;     // marks the append results' readback as complete
+    mh::AppendOut *aout_pin = nullptr;  // pinned host copy of the append results (aout_cap)
+    size_t aout_cap = 0;
     int *rb_pin = nullptr;           // pinned host readback buffer (rb_cap ints)
     size_t rb_cap = 0;
     // pinned staging for host-to-device copies: bump-allocated, reset after a stream sync
@@ -322,9 +325,14 @@ struct HostTimer {
 };
 
 /* Fold completed timing records into the per-class sums (stream must be idle). */
+/* Fold the recorded timers whose work has completed (they complete in recording order; work
+ * queued behind the last synchronisation stays pending). */
 inline void resolve_timing(mars_hip_ctx *ctx)
 {
+    size_t done = 0;
     for (auto &r : ctx->pend) {
+        if (hipEventQuery(r.b) != hipSuccess) break;
+        ++done;
         float ms = 0.f;
         MH_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
         KAcc &a = ctx->kacc[r.cls];
@@ -335,7 +343,7 @@ inline void resolve_timing(mars_hip_ctx *ctx)
         ctx->evpool.push_back(r.a);
         ctx->evpool.push_back(r.b);
     }
-    ctx->pend.clear();
+    ctx->pend.erase(ctx->pend.begin(), ctx->pend.begin() + done);
 }
 
 } // namespace mh

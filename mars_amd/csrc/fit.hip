@@ -4,6 +4,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstring>
 
@@ -551,6 +552,7 @@ void ctx_base_init(mars_hip_ctx *ctx, size_t max_terms)
     MH_CHECK(hipEventCreate(&ctx->ev0));
     MH_CHECK(hipEventCreate(&ctx->ev1));
     MH_CHECK(hipEventCreateWithFlags(&ctx->ev_rb, hipEventDisableTiming));
+    MH_CHECK(hipEventCreateWithFlags(&ctx->ev_app, hipEventDisableTiming));
     ctx->max_terms = max_terms;
     ctx->ldb = (max_terms + 3) / 4 * 4;
     ctx->dgks.ensure(1);
@@ -745,6 +747,9 @@ void ctx_free(mars_hip_ctx *ctx)
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->ev_rb) (void)hipEventDestroy(ctx->ev_rb);
+    if (ctx->ev_app) (void)hipEventDestroy(ctx->ev_app);
+    if (ctx->aout_pin) (void)hipHostFree(ctx->aout_pin);
+    ctx->aout_pin = nullptr;
     if (ctx->rb_pin) (void)hipHostFree(ctx->rb_pin);
     ctx->rb_pin = nullptr;
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -812,7 +817,8 @@ void fit_init(mars_hip_ctx *ctx, FitState &fs, const float *X, size_t ldX, const
  * counterpart of MarsAlgo::append, marsalgo.cc:760-875).  Validation happens
  * on the host with the reference's messages; out[f] receives the result.
  */
-void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std::vector<AppendOut> &out)
+void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std::vector<AppendOut> &out,
+                      const std::function<void()> &before_readback)
 {
     PendingScope pending_scope(ctx);
     require(ctx->resident, "context was not created by mars_hip_fit_create");
@@ -899,8 +905,20 @@ void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std
                            ctx->scal.p, ctx->d_fitw.p, gate, ctx->aout.p);
         MH_LAUNCH_CHECK();
     }
-    MH_CHECK(hipMemcpyAsync(out.data(), ctx->aout.p, F * sizeof(AppendOut), hipMemcpyDeviceToHost, st));
-    ctx_sync(ctx);
+    // the results come back through pinned memory behind an event, so work the caller queues
+    // after them (before_readback) runs on the device while the host reads them
+    if (ctx->aout_cap < F) {
+        if (ctx->aout_pin) retire_host(ctx->aout_pin);
+        ctx->aout_pin = nullptr;
+        ctx->aout_cap = 0;
+        MH_CHECK(hipHostMalloc((void **)&ctx->aout_pin, F * sizeof(AppendOut), hipHostMallocDefault));
+        ctx->aout_cap = F;
+    }
+    MH_CHECK(hipMemcpyAsync(ctx->aout_pin, ctx->aout.p, F * sizeof(AppendOut), hipMemcpyDeviceToHost, st));
+    MH_CHECK(hipEventRecord(ctx->ev_app, st));
+    if (before_readback) before_readback();
+    MH_CHECK(hipEventSynchronize(ctx->ev_app));
+    std::copy(ctx->aout_pin, ctx->aout_pin + F, out.begin());
     resolve_timing(ctx);
     for (size_t f = 0; f < F; ++f) {
         if (!args[f].active) continue;

@@ -4,6 +4,7 @@
 #include "common.h"
 
 #include <exception>
+#include <functional>
 
 namespace mh {
 
@@ -62,7 +63,11 @@ void ctx_base_init(mars_hip_ctx *ctx, size_t max_terms);
 void ctx_free(mars_hip_ctx *ctx);
 void fit_init(mars_hip_ctx *ctx, FitState &fs, const float *X, size_t ldX, const float *y, const float *w,
               size_t n, size_t px);
-void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std::vector<AppendOut> &out);
+/* Append one basis function to every fit with args[f].active.  `before_readback`, when set,
+ * is called after the append's kernels and the copy of its results are queued and before the
+ * host waits for those results: work it queues overlaps the host's reading of them. */
+void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std::vector<AppendOut> &out,
+                      const std::function<void()> &before_readback = {});
 void argmax_fits(mars_hip_ctx *ctx, const Cands &cs, const double *lin, std::vector<BestOut> &best);
 double host_dsse(const FitState &fs);
 extern thread_local std::string g_last_error;
