@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace mh {
 
@@ -110,7 +111,7 @@ __global__ __launch_bounds__(64 * W) void k_tc_update(const FitDev *__restrict__
             }
             if (jyl) *(float4 *)&sjy[buf][tid >> 3][4 * (tid & 7)] = jy;
         };
-        float4 ao[2][2], ab[2][2];   // [register set][half]
+        float4 ao[3][2], ab[3][2];   // [register set][half]: A columns are fetched two chunks ahead
         auto load_a = [&](int set, long long r) {
 #pragma unroll
             for (int hf = 0; hf < 2; ++hf) {
@@ -164,35 +165,48 @@ __global__ __launch_bounds__(64 * W) void k_tc_update(const FitDev *__restrict__
 #pragma unroll
                 for (int v = 0; v < 4; ++v) acc[g][v] += (double)part[g][v];
         };
-        // prologue: chunk 0 into buffer 0 and register set 0
+        // prologue: chunk 0 into buffer 0, A of chunks 0 and 1 into register sets 0 and 1
         float4 xv[NXL], jy;
         const long long r0 = I.r0, r1 = I.r1;
         if (r0 < r1) {
             load_x(xv, jy, r0);
-            if (wact) load_a(0, r0);
+            if (wact) {
+                load_a(0, r0);
+                load_a(1, r0 + kChunk < r1 ? r0 + kChunk : r0);
+            }
             store_x(xv, jy, 0);
         }
         __syncthreads();
-        int c = 0;
-#pragma unroll 1
-        for (long long r = r0; r < r1; r += kChunk, ++c) {
+        // chunk c at row r: A in register set c % 3 (a constant in each unrolled copy)
+        auto step = [&](long long r, int c, auto setc) {
+            constexpr int set = decltype(setc)::value, ahead = (set + 2) % 3;
             const int cur = c & 1, nxt = cur ^ 1;
-            const bool more = r + kChunk < r1;
-            // a missing next chunk reloads the current one (in bounds, unused)
-            const long long rn = more ? r + kChunk : r;
+            // a missing chunk ahead reloads the current one (in bounds, unused)
+            const long long rn = r + kChunk < r1 ? r + kChunk : r;
+            const long long ra = r + 2 * kChunk < r1 ? r + 2 * kChunk : r;
             load_x(xv, jy, rn);
             if (wact) {
-                if (cur == 0) {
-                    load_a(1, rn);
-                    compute(0, cur);
-                } else {
-                    load_a(0, rn);
-                    compute(1, cur);
-                }
+                load_a(ahead, ra);
+                compute(set, cur);
             }
             // buffer nxt last held chunk c-1, finished by every wave before the previous barrier
             store_x(xv, jy, nxt);
             __syncthreads();
+        };
+        {
+            long long r = r0;
+            int c = 0;
+#pragma unroll 1
+            while (r < r1) {
+                step(r, c, std::integral_constant<int, 0>{});
+                r += kChunk, ++c;
+                if (r >= r1) break;
+                step(r, c, std::integral_constant<int, 1>{});
+                r += kChunk, ++c;
+                if (r >= r1) break;
+                step(r, c, std::integral_constant<int, 2>{});
+                r += kChunk, ++c;
+            }
         }
         const int it = bi * W + wv;   // sub-item of this wave
 #pragma unroll
