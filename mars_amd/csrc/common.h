@@ -228,6 +228,8 @@ struct mars_hip_ctx {
     mh::Cands rcand;                 // mixed form: candidates rescored exactly
     mh::DBuf<double> rlin, rybx;
     mh::DBuf<int> rsel;              // their positions in the scored list
+    mh::DBuf<int> rkeep;             // 1: the candidate was ungated in the cache approximation,
+                                     // so its residual is stored for the final pass
     bool rbuf_cm = false;            // candidate columns in rbuf are column-major (linear scoring)
     int rsel_hint = 0;               // the previous rescoring list's length
     mh::DBuf<mh::FitDev> d_fits;

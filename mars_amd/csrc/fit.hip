@@ -734,6 +734,7 @@ void ctx_free(mars_hip_ctx *ctx)
     ctx->d_lcmap.release();
     ctx->live_list.release();
     ctx->rsel.release();
+    ctx->rkeep.release();
     ctx->dgks.release();
     ctx->aout.release();
     ctx->aarg.release();

@@ -22,12 +22,17 @@ struct Pass {
  * fit's rbuf (row-major, column c - c0).  `xover`/`sover`, when set, replace
  * fit 0's X and scales (mirror-mode single-column calls).  `rescore` (a few
  * candidates per fit, the mixed form's exact rescoring): the Sum r^2 pass
- * keeps every residual (column-major), so the DGKS retry and the final pass
+ * keeps the residuals (column-major), so the DGKS retry and the final pass
  * read r instead of recomputing it from Bo; the final pass runs while the
  * DGKS flags come back, and the fired candidates get it again after retry.
+ * `keep` (device, one int per candidate, rescoring only): 0 = that candidate's
+ * residual is not stored, because it most likely ends with s <= tol and is
+ * never read; one that ends with s > tol after all is recomputed (before its
+ * DGKS retry, or by a final pass over the late candidates).
  */
 void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, double *ybx,
-                 const float *xover = nullptr, const float *sover = nullptr, bool rescore = false);
+                 const float *xover = nullptr, const float *sover = nullptr, bool rescore = false,
+                 const int *keep = nullptr);
 
 /* Mixed-precision scoring of ctx->cand (linear delta-SSE only) from each
  * fit's incrementally maintained T cache (FP32 MFMA, FP64 across row chunks),

@@ -26,6 +26,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <type_traits>
 
 namespace mh {
@@ -321,7 +323,9 @@ __global__ __launch_bounds__(256) void k_fit_max(const int *__restrict__ c0, con
  * approximate score is within the window of its fit's best ungated one.  Two passes over
  * tiles of kSelTile candidates (256 threads, 4 coalesced candidates each): k_select_count
  * writes each tile's list and gated counts, k_select_write places the tile's entries after
- * those of the tiles before it; out_n[0] = list length, out_n[1] = gated count.
+ * those of the tiles before it; out_n[0] = list length, out_n[1] = gated count.  keep[i] = 1
+ * when the rescoring stores list entry i's residual for the final pass: with store policy 0,
+ * when the entry is ungated (an exact s > tol is then almost certain); policy 1 all, 2 none.
  */
 constexpr int kSelTile = 1024;
 
@@ -372,7 +376,8 @@ __global__ __launch_bounds__(256) void k_select_write(int P, const int *__restri
                                                       const double *__restrict__ score,
                                                       const int *__restrict__ flag, const double *__restrict__ mx,
                                                       double window, const int *__restrict__ blk,
-                                                      int *__restrict__ list, int *__restrict__ out_n)
+                                                      int *__restrict__ list, int *__restrict__ keep,
+                                                      int policy, int *__restrict__ out_n)
 {
     __shared__ int sw[4], sg[4];
     const int t = threadIdx.x, w = t >> 6, lane = t & 63, b = blockIdx.x, base = b * kSelTile;
@@ -406,10 +411,26 @@ __global__ __launch_bounds__(256) void k_select_write(int P, const int *__restri
         __syncthreads();
         int woff = run;
         for (int k = 0; k < w; ++k) woff += sw[k];
-        if (tk) list[woff + __popcll(m & below)] = c;
+        if (tk) {
+            const int pos = woff + __popcll(m & below);
+            list[pos] = c;
+            keep[pos] = policy == 1 ? 1 : policy == 2 ? 0 : (flag[c] == 0 ? 1 : 0);
+        }
         run += sw[0] + sw[1] + sw[2] + sw[3];
         __syncthreads();
     }
+}
+
+/* Which rescored candidates keep their first residual in rbuf, from MARS_HIP_RESCORE_STORE:
+ * "ungated" (default, 0), "all" (1) or "none" (2).  The scores do not depend on it: a candidate
+ * without a stored residual that ends with s > tol has r recomputed from Bo and T. */
+int store_policy()
+{
+    const char *e = std::getenv("MARS_HIP_RESCORE_STORE");
+    if (!e || !*e || !std::strcmp(e, "ungated")) return 0;
+    if (!std::strcmp(e, "all")) return 1;
+    if (!std::strcmp(e, "none")) return 2;
+    throw Error(MARS_HIP_EINVAL, std::string("MARS_HIP_RESCORE_STORE must be ungated, all or none, not '") + e + "'");
 }
 
 /* lin[idx[i]] = val[i] */
@@ -570,6 +591,7 @@ void mixed_score(mars_hip_ctx *ctx)
     ctx->ival.ensure(2 * F + 2 + 2 * (size_t)nblk);
     ctx->red.ensure(F);
     ctx->rsel.ensure(P);
+    ctx->rkeep.ensure(P);
     int *d_nsel = ctx->ival.p + 2 * F, *d_blk = d_nsel + 2;
     h2d_defer(ctx, ctx->ival.p, ranges.data(), 2 * F * sizeof(int));
     h2d_flush(ctx);
@@ -584,7 +606,7 @@ void mixed_score(mars_hip_ctx *ctx)
     MH_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_select_write, dim3((unsigned)nblk), dim3(256), 0, st, P, (const int *)cs.d_fit.p,
                        (const double *)ctx->lin.p, (const int *)ctx->flag.p, (const double *)ctx->red.p, kWindow,
-                       (const int *)d_blk, ctx->rsel.p, d_nsel);
+                       (const int *)d_blk, ctx->rsel.p, ctx->rkeep.p, store_policy(), d_nsel);
     MH_LAUNCH_CHECK();
     // the list length and a prefix of the list sized from the previous step come back together;
     // a longer list takes a second copy
@@ -617,7 +639,7 @@ void mixed_score(mars_hip_ctx *ctx)
     }
     ctx->rlin.ensure(R);
     ctx->rybx.ensure(R);
-    plain_score(ctx, rc, false, ctx->rlin.p, ctx->rybx.p, nullptr, nullptr, true);
+    plain_score(ctx, rc, false, ctx->rlin.p, ctx->rybx.p, nullptr, nullptr, true, ctx->rkeep.p);
     hipLaunchKernelGGL(k_put, dim3((R + 255) / 256), dim3(256), 0, st, (const int *)ctx->rsel.p,
                        (const double *)ctx->rlin.p, R, ctx->lin.p);
     MH_LAUNCH_CHECK();

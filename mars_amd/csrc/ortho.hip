@@ -52,12 +52,13 @@ __global__ __launch_bounds__(64) void k_treduce(const FitDev *__restrict__ fits,
 }
 
 /* s[c] = sum of item partials; the first pass also sets flag[c]: bit 0 the DGKS gate
- * fired (kernels.cc:387-388), bit 1 s > tol (the candidate can score above 0). */
+ * fired (kernels.cc:387-388), bit 1 s > tol (the candidate can score above 0), bit 2 its
+ * residual was not stored (keep[c] == 0). */
 __global__ void k_sreduce(const int *__restrict__ list, int count, const int *__restrict__ cmap,
                           const double *__restrict__ spart, int tile,
                           const double *__restrict__ tnorm, const FitDev *__restrict__ fits,
-                          const int *__restrict__ cfit, double *__restrict__ ssum,
-                          int *__restrict__ flag)
+                          const int *__restrict__ cfit, const int *__restrict__ keep,
+                          double *__restrict__ ssum, int *__restrict__ flag)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= count) return;
@@ -68,7 +69,8 @@ __global__ void k_sreduce(const int *__restrict__ list, int count, const int *__
     } else {
         const double tol = fits[cfit[c]].tol;
         ssum[c] = s;
-        flag[c] = ((s > tol && s * kGate < tnorm[c]) ? 1 : 0) | (s > tol ? 2 : 0);
+        flag[c] = ((s > tol && s * kGate < tnorm[c]) ? 1 : 0) | (s > tol ? 2 : 0) |
+                  ((keep && keep[c] == 0) ? 4 : 0);
     }
 }
 
@@ -135,11 +137,13 @@ __global__ __launch_bounds__(256) void k_finalize(const FitDev *__restrict__ fit
  * The same over column-major residuals (linear scoring): wave w of 4 takes the item's slots
  * w, w + 4, ...; its lanes read the slot's column as float4 (rows r0 + 4 lane + 256 u, four
  * in flight), and the wave's sum is formed in a fixed order.  With `list`, item positions
- * index the list of candidates.
+ * index the list of candidates.  A candidate whose residual was not stored (keep entry 0)
+ * gets 0 here; if its s > tol, a later pass recomputes it.
  */
 __global__ __launch_bounds__(256) void k_finalize_cm(const FitDev *__restrict__ fits,
                                                      const Item *__restrict__ items, int nitems,
                                                      const int *__restrict__ list,
+                                                     const int *__restrict__ keep,
                                                      const double *__restrict__ ssum,
                                                      double *__restrict__ ypart)
 {
@@ -155,7 +159,7 @@ __global__ __launch_bounds__(256) void k_finalize_cm(const FitDev *__restrict__ 
             double acc = 0.0;
             // s <= tol: scale 0, so ybx = 0 exactly (kernels.cc:401-416) and r is not read; most
             // rescored candidates (duplicates of basis functions) end here
-            const long long rend = s > F.tol ? I.r1 : I.r0;
+            const long long rend = (s > F.tol && (!keep || keep[c] != 0)) ? I.r1 : I.r0;
             for (long long r = I.r0 + 4 * lane; r < rend; r += 1024) {
                 float4 rv[4], yv[4];
 #pragma unroll
@@ -319,7 +323,7 @@ void launch_tgemm(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, cons
 
 template <int MODE, bool RB, bool TCAND>
 void launch_project(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, const Cands &cs,
-                    const int *list, const double *T, int ldT, bool narrow)
+                    const int *list, const double *T, int ldT, bool narrow, const int *keep = nullptr)
 {
     if (nitems == 0) return;
     const dim3 grid(grid_w(nitems)), block(64);
@@ -327,7 +331,7 @@ void launch_project(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, co
     hipLaunchKernelGGL((k_project_w<M, CG, MODE, RB, TCAND>), grid, block, 0, ctx->stream,                \
                        (const FitDev *)ctx->d_fits.p, (const Item *)u.items, nitems, (const int *)cs.d_xc.p, \
                        (const int *)cs.d_par.p, list, T, ldT, (const double *)ctx->ssum.p,                \
-                       (const int *)ctx->flag.p, ctx->spart.p)
+                       (const int *)ctx->flag.p, keep, ctx->spart.p)
 #define MH_GO(M)                      \
     if (narrow) MH_GO1(M, 1);         \
     else MH_GO1(M, cg_project(M))
@@ -416,10 +420,11 @@ void upload_fits(mars_hip_ctx *ctx, const Cands &cs, const float *xover, const f
 }
 
 void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, double *ybx,
-                 const float *xover, const float *sover, bool rescore)
+                 const float *xover, const float *sover, bool rescore, const int *keep)
 {
     PendingScope pending_scope(ctx);
-    const bool store_r = keep_bxn || rescore;  // every residual stays in rbuf
+    const bool store_r = keep_bxn || rescore;  // residuals stay in rbuf (all, or those `keep` marks)
+    if (!rescore) keep = nullptr;
     const int P = (int)cs.fit.size();
     if (P == 0) return;
     hipStream_t st = ctx->stream;
@@ -502,12 +507,12 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         KTimer kt(ctx, K_PROJECT, flops, bytes + 0.0);
         if (store_r)
             launch_project<kStoreR, false, false>(ctx, mt, uP, (int)pP.items.size(), cs, nullptr, ctx->T.p, ldT,
-                                                  narrow);
+                                                  narrow, keep);
         else
             launch_project<kSumSq, false, false>(ctx, mt, uP, (int)pP.items.size(), cs, nullptr, ctx->T.p, ldT,
                                                  narrow);
         hipLaunchKernelGGL(k_sreduce, dim3((P + 255) / 256), dim3(256), 0, st, nullptr, P, uP.cmap,
-                           ctx->spart.p, kSlots, ctx->tnorm.p, ctx->d_fits.p, cs.d_fit.p,
+                           ctx->spart.p, kSlots, ctx->tnorm.p, ctx->d_fits.p, cs.d_fit.p, keep,
                            ctx->ssum.p, ctx->flag.p);
         MH_LAUNCH_CHECK();
     }
@@ -535,7 +540,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
             for (size_t f = 0; f < F; ++f) fb2 += 4.0 * n[f] * cnt[f] + 4.0 * n[f];
             KTimer kt(ctx, K_FINALIZE, 0.0, fb2);
             hipLaunchKernelGGL(k_finalize_cm, dim3(grid_for(pF.items.size())), dim3(256), 0, st, ctx->d_fits.p,
-                               uF.items, nF, nullptr, ctx->ssum.p, ctx->spart.p);
+                               uF.items, nF, nullptr, keep, ctx->ssum.p, ctx->spart.p);
             MH_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, P, uF.cmap, ctx->spart.p,
                                nullptr, ybx, lin);
@@ -550,17 +555,23 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         flags = flag_v.data();
     }
     // DGKS retry of the flagged columns, grouped by fit (kernels.cc:387-399)
-    std::vector<int> fired, live;
-    std::vector<int> rb(F, 0), re(F, 0), lb(F, 0), le(F, 0);
+    // late: candidates with s > tol whose residual was not stored and no retry will store
+    std::vector<int> fired, live, late;
+    std::vector<int> rb(F, 0), re(F, 0), lb(F, 0), le(F, 0), tb(F, 0), te(F, 0);
+    bool fired_unstored = false;
     for (size_t f = 0; f < F; ++f) {
         rb[f] = (int)fired.size();
         lb[f] = (int)live.size();
+        tb[f] = (int)late.size();
         for (int c = fb[f]; c < fe[f]; ++c) {
             if (flags[c] & 1) fired.push_back(c);
             if (flags[c] & 2) live.push_back(c);
+            if ((flags[c] & 7) == 6) late.push_back(c);
+            if ((flags[c] & 5) == 5) fired_unstored = true;
         }
         re[f] = (int)fired.size();
         le[f] = (int)live.size();
+        te[f] = (int)late.size();
     }
     if (!fired.empty()) {
         const int nf = (int)fired.size();
@@ -582,7 +593,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         ctx->tpart.ensure(std::max<size_t>(rT.items.size(), 1) * kSlots * ldT);
         ctx->spart.ensure(rP.items.size() * kSlots + 1);
         ctx->T2.ensure((size_t)nf * ldT);
-        if (!store_r)  // the first residual of the fired columns, for the retry
+        if (!store_r || fired_unstored)  // the first residual of the fired columns, for the retry
             launch_project<kStoreR, false, true>(ctx, mt, vP, (int)rP.items.size(), cs, ctx->fired_list.p,
                                                  ctx->T.p, ldT, rnarrow);
         launch_tgemm<true>(ctx, mt, vT, (int)rT.items.size(), cs, ctx->fired_list.p, ldT, rnarrow);
@@ -592,7 +603,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         launch_project<kStoreR, true, false>(ctx, mt, vP, (int)rP.items.size(), cs, ctx->fired_list.p, ctx->T2.p,
                                              ldT, rnarrow);
         hipLaunchKernelGGL(k_sreduce, dim3((nf + 255) / 256), dim3(256), 0, st, ctx->fired_list.p, nf,
-                           vP.cmap, ctx->spart.p, kSlots, nullptr, ctx->d_fits.p, cs.d_fit.p,
+                           vP.cmap, ctx->spart.p, kSlots, nullptr, ctx->d_fits.p, cs.d_fit.p, nullptr,
                            ctx->ssum.p, nullptr);
         MH_LAUNCH_CHECK();
         if (spec) {  // pass 3 again for the fired candidates, over their retried residuals
@@ -604,15 +615,41 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
             h2d_flush(ctx);
             ctx->spart.ensure(fF.items.size() * kTileF + 1);
             hipLaunchKernelGGL(k_finalize_cm, dim3(grid_for(fF.items.size())), dim3(256), 0, st, ctx->d_fits.p,
-                               wF.items, (int)fF.items.size(), (const int *)ctx->fired_list.p, ctx->ssum.p,
-                               ctx->spart.p);
+                               wF.items, (int)fF.items.size(), (const int *)ctx->fired_list.p, nullptr,
+                               ctx->ssum.p, ctx->spart.p);
             MH_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_yreduce, dim3((nf + 255) / 256), dim3(256), 0, st, nf, wF.cmap, ctx->spart.p,
                                ctx->fired_list.p, ybx, lin);
             MH_LAUNCH_CHECK();
         }
     }
-    if (spec) return;
+    if (spec) {
+        // candidates with s > tol whose residual was not stored: recompute r and finish them
+        // (the final pass above gave them 0)
+        const int nl = (int)late.size();
+        if (nl > 0) {
+            KTimer kt(ctx, K_FINALIZE, 0.0, 0.0);
+            std::vector<int> lcnt(F);
+            for (size_t f = 0; f < F; ++f) lcnt[f] = te[f] - tb[f];
+            const bool lnarrow = narrow_for(lcnt);
+            const int ltP = 16 * (lnarrow ? 1 : cg_project(mt));
+            Pass pL = make_pass(tb, te, n, split_rows(n, lcnt, ltP, 8192, 256), ltP, kWR);
+            Uploaded uL;
+            ctx->live_list.ensure(nl);
+            ctx->d_litems.ensure(pL.items.size() + 1);
+            ctx->d_lcmap.ensure(pL.cmap.size() + 1);
+            h2d_defer(ctx, ctx->live_list.p, late.data(), nl * sizeof(int));
+            upload_pass(ctx, pL, ctx->d_litems, ctx->d_lcmap, 0, 0, uL);
+            h2d_flush(ctx);
+            ctx->spart.ensure(pL.items.size() * kSlots + 1);
+            launch_project<kFinalY, false, true>(ctx, mt, uL, (int)pL.items.size(), cs, ctx->live_list.p,
+                                                 ctx->T.p, ldT, lnarrow);
+            hipLaunchKernelGGL(k_yreduce, dim3((nl + 255) / 256), dim3(256), 0, st, nl, uL.cmap, ctx->spart.p,
+                               ctx->live_list.p, ybx, lin);
+            MH_LAUNCH_CHECK();
+        }
+        return;
+    }
     // Pass 3: Bxn = f32(r * scale), ybx = sum Bxn y
     {
         double fb2 = 0;

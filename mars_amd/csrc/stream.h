@@ -226,6 +226,7 @@ enum { kStoreR = 0, kSumSq = 1, kFinalY = 2 };
 /*
  * Projection r = f32(Bx - Bo T) over the item's rows, then per MODE:
  *   kStoreR  r -> rbuf (row * rsr + column * rsc), spart[item * 64 + slot] = sum r^2;
+ *            with `keep`, only candidates whose keep entry is nonzero store r;
  *   kSumSq   spart = sum r^2 only;
  *   kFinalY  Bxn = f32(r * scale), scale = f32(1 / sqrt(s + tol)) (0 when s <= tol,
  *            kernels.cc:401-416), spart = sum Bxn y (marsalgo.cc:500-505); candidates
@@ -246,6 +247,7 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_project_w(const FitDev *_
                                                                  const double *__restrict__ T, int ldT,
                                                                  const double *__restrict__ ssum,
                                                                  const int *__restrict__ fired,
+                                                                 const int *__restrict__ keep,
                                                                  double *__restrict__ spart)
 {
     constexpr int KQ = 4 * MT;           // k steps of 4
@@ -278,7 +280,7 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_project_w(const FitDev *_
         LaneCand c[CG];
         double acc[CG];
         float scale[CG];
-        bool fromr[CG];
+        bool fromr[CG], store[CG];
         const float *bo[KQ];
 #pragma unroll
         for (int q = 0; q < KQ; ++q) {
@@ -291,6 +293,7 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_project_w(const FitDev *_
             acc[g] = 0.0;
             scale[g] = 0.0f;
             fromr[g] = false;
+            store[g] = MODE == kStoreR && c[g].ok && (!keep || keep[c[g].col + F.c0] != 0);
             if (MODE == kFinalY && c[g].ok) {
                 const int cand = (int)(c[g].col + F.c0);
                 const double sv = ssum[cand];
@@ -352,7 +355,7 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_project_w(const FitDev *_
                                     if (row + v < I.r1) rf[v] = rp[(row + v) * F.rsr];
                             }
                         }
-                        if (MODE == kStoreR) {
+                        if (MODE == kStoreR && store[g]) {
                             if (F.rsr == 1) {
                                 *reinterpret_cast<float4 *>(rp + row) = float4{rf[0], rf[1], rf[2], rf[3]};
                             } else {

@@ -88,10 +88,13 @@ def test_config3_full_properties():
     assert np.abs(Q.T @ Q - np.eye(Q.shape[1])).max() <= 1e-5
 
 
-def test_config4_full_mixed_equals_plain():
-    """256 independent fits (2k-200k rows, 5-64 columns), 33 linear terms each, batched."""
+@pytest.mark.parametrize("policy", ["ungated", "none"])
+def test_config4_full_mixed_equals_plain(policy, monkeypatch):
+    """256 independent fits (2k-200k rows, 5-64 columns), 33 linear terms each, batched; the
+    rescoring storing the residuals of its ungated candidates (the default) or of none."""
     import mars_amd
 
+    monkeypatch.setenv("MARS_HIP_RESCORE_STORE", policy)
     fits = datagen.workload4()
     say("config 4 full: %d fits, %d rows" % (len(fits), sum(len(x) for x, _ in fits)))
     Xs, ys = [x for x, _ in fits], [y for _, y in fits]

@@ -295,20 +295,79 @@ def test_batch_of_one_equals_single_fit():
     np.testing.assert_array_equal(single["r2"], batch["r2"])
 
 
+# residual storage policies of the exact rescoring: "none" sends every rescored candidate with
+# s > tol through the recomputation paths that the default policy takes only for a few
+STORE_POLICIES = ["ungated", "none"]
+
+
+@pytest.mark.parametrize("policy", STORE_POLICIES)
 @pytest.mark.parametrize("case", ["w1", "w5_small", "w4_fit2"])
-def test_mixed_form_matches_reference(case):
+def test_mixed_form_matches_reference(case, policy, monkeypatch):
     """Mixed-precision form: same term sequence and per-step mse as the reference."""
+    monkeypatch.setenv("MARS_HIP_RESCORE_STORE", policy)
     path = os.path.join(HERE, "golden", case + ".json")
     gold = json.load(open(path))
-    say("mixed fit", case)
+    say("mixed fit", case, policy)
     model, trace = _fit(CASES[case], precision="mixed")
     _compare_to_golden(gold, model, trace, 1e-10)
 
 
-def test_batch_mixed_matches_reference():
+@pytest.mark.parametrize("policy", STORE_POLICIES + ["all"])
+def test_mixed_eval_at_dgks_gate_boundary(policy, monkeypatch):
+    """Columns on the DGKS gate boundary: x_j = 3 + c_j z with z centred and of unit variance, so
+    at the intercept-only step s = n c_j^2 and |T|^2 = 9 n, and the gate 9 s < |T|^2
+    (kernels.cc:387-388) holds exactly when c_j < 1.  With c_j within 6e-7 of 1 the cache
+    approximation and the exact pass disagree on some columns' gate.  Every column has the same
+    score (all residuals are multiples of z), so all are rescored, and the mixed form's linear
+    scores equal the plain form's up to FP64 summation order, whichever residuals the
+    rescoring stores."""
+    from mars_amd import marslib
+
+    monkeypatch.setenv("MARS_HIP_RESCORE_STORE", policy)
+    say("gate boundary eval", policy)
+    n, p = 8192, 384
+    rng = np.random.default_rng(11)
+    z = rng.standard_normal(n)
+    z -= z.mean()
+    z /= np.sqrt(np.mean(z * z))
+    c = 1.0 + np.linspace(-6e-7, 6e-7, p)
+    X = np.asfortranarray((3.0 + np.outer(z, c)).astype(np.float32))
+    y = (z + 0.5 * rng.standard_normal(n)).astype(np.float32)
+    w = np.ones(n, dtype=np.float32)
+    mask = np.ones((p, 1), dtype=bool)
+    out = {}
+    for form in ("fp64", "mixed"):
+        algo = marslib.MarsAlgo(X, y, w, 4, precision=form)
+        _, lin, _, _ = algo.eval(mask, 1, 0, True, 1, cuda=True)
+        out[form] = (np.asarray(lin)[:, 0].copy(), algo.stats())
+        del algo
+    (lp, sp), (lm, sm) = out["fp64"], out["mixed"]
+    say("gated", sm["gated"], "fired", sm["dgks_fired"], "plain fired", sp["dgks_fired"])
+    assert sm["rescored"] == p
+    assert sm["gated"] > 0 and sm["dgks_fired"] == sp["dgks_fired"] > 0
+    assert np.all(lp > 0)
+    np.testing.assert_allclose(lm, lp, rtol=1e-12, atol=0)
+
+
+def test_rescore_store_policy_rejects_unknown(monkeypatch):
+    from mars_amd import marslib
+
+    X, y, _ = make_inputs(CASES["w1"])
+    algo = marslib.MarsAlgo(X, y, np.ones(len(X), dtype=np.float32), 8, precision="mixed")
+    monkeypatch.setenv("MARS_HIP_RESCORE_STORE", "some")
+    with pytest.raises(RuntimeError, match="MARS_HIP_RESCORE_STORE"):
+        algo.eval(np.ones((X.shape[1], 1), dtype=bool), 1, 0, True, 1, cuda=True)
+    monkeypatch.setenv("MARS_HIP_RESCORE_STORE", "all")
+    d0 = algo.eval(np.ones((X.shape[1], 1), dtype=bool), 1, 0, True, 1, cuda=True)[0]
+    assert d0 > 0
+
+
+@pytest.mark.parametrize("policy", STORE_POLICIES)
+def test_batch_mixed_matches_reference(policy, monkeypatch):
     import mars_amd
 
-    say("batch mixed")
+    monkeypatch.setenv("MARS_HIP_RESCORE_STORE", policy)
+    say("batch mixed", policy)
     golds = [json.load(open(os.path.join(HERE, "golden", "w4_fit%d.json" % i))) for i in range(8)]
     inputs = [make_inputs(CASES[g["name"]]) for g in golds]
     stats = {}
