@@ -132,8 +132,8 @@ struct FitW {
     double *G;  // Gram matrix of the stored Bo columns, G[j * ldb + l] for l < j
 };
 
-/* Deferred small uploads: each is staged in pinned memory at `off`; one copy sends the staged
- * range to the device mirror and one kernel scatters the pieces to their destinations. */
+/* Deferred small uploads: each is staged in pinned host memory at `off`; one kernel reads the
+ * pieces from there over the host link and scatters them to their destinations. */
 struct UpDesc {
     void *dst;
     unsigned long long off, bytes;
@@ -213,7 +213,7 @@ struct mars_hip_ctx {
     // pinned staging for host-to-device copies: bump-allocated, reset after a stream sync
     char *stage = nullptr;
     size_t stage_cap = 0, stage_off = 0;
-    mh::DBuf<char> mirror;           // device copy of the staging space, for deferred uploads
+    char *stage_dev = nullptr;       // device address of the staging space (deferred uploads)
     mh::UpTable up_pend;             // deferred uploads not yet sent
 
     size_t max_terms = 0;  // basis capacity (all fits)

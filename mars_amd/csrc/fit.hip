@@ -594,12 +594,13 @@ size_t stage_take(mars_hip_ctx *ctx, size_t bytes)
         if (need > ctx->stage_cap) {
             if (ctx->stage) MH_CHECK(hipHostFree(ctx->stage));
             ctx->stage = nullptr;
+            ctx->stage_dev = nullptr;
             ctx->stage_cap = 0;
-            ctx->mirror.release();
             const size_t cap = std::max<size_t>(need, 4u << 20);
-            MH_CHECK(hipHostMalloc((void **)&ctx->stage, cap, hipHostMallocDefault));
+            // coherent: the scatter kernel reads it directly, uncached
+            MH_CHECK(hipHostMalloc((void **)&ctx->stage, cap, hipHostMallocCoherent));
+            MH_CHECK(hipHostGetDevicePointer((void **)&ctx->stage_dev, ctx->stage, 0));
             ctx->stage_cap = cap;
-            ctx->mirror.ensure(cap);
         }
     }
     const size_t off = ctx->stage_off;
@@ -620,18 +621,25 @@ void h2d(mars_hip_ctx *ctx, void *dst, const void *src, size_t bytes)
     MH_CHECK(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, ctx->stream));
 }
 
-constexpr unsigned kScatterChunk = 16384;  // bytes per k_upload_scatter block
+constexpr unsigned kScatterChunk = 4096;  // bytes per k_upload_scatter block (4 words per thread)
 
-/* Block (x, y) copies bytes [x * kScatterChunk, ...) of upload y from the device mirror. */
-__global__ __launch_bounds__(256) void k_upload_scatter(const char *__restrict__ mirror, UpTable t)
+/* Block (x, y) copies bytes [x * kScatterChunk, ...) of upload y from the pinned staging space;
+ * each thread issues its four host reads before any store, so their latencies overlap. */
+__global__ __launch_bounds__(256) void k_upload_scatter(const char *__restrict__ stage, UpTable t)
 {
     const UpDesc u = t.d[blockIdx.y];
     const unsigned long long b0 = (unsigned long long)blockIdx.x * kScatterChunk;
     if (b0 >= u.bytes) return;
-    const unsigned long long b1 = min(u.bytes, b0 + kScatterChunk);
-    const unsigned *src = (const unsigned *)(mirror + u.off);
+    const unsigned long long w0 = b0 / 4 + threadIdx.x, w1 = min(u.bytes, b0 + kScatterChunk) / 4;
+    const unsigned *src = (const unsigned *)(stage + u.off);
     unsigned *dst = (unsigned *)u.dst;
-    for (unsigned long long i = b0 / 4 + threadIdx.x; i < b1 / 4; i += 256) dst[i] = src[i];
+    unsigned v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (w0 + 256 * k < w1) v[k] = src[w0 + 256 * k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (w0 + 256 * k < w1) dst[w0 + 256 * k] = v[k];
 }
 
 void h2d_defer(mars_hip_ctx *ctx, void *dst, const void *src, size_t bytes)
@@ -663,18 +671,12 @@ void h2d_flush(mars_hip_ctx *ctx)
 {
     UpTable &t = ctx->up_pend;
     if (t.n == 0) return;
-    // one copy covers every pending piece of the staging space
-    unsigned long long lo = t.d[0].off, hi = 0, most = 0;
-    for (int i = 0; i < t.n; ++i) {
-        lo = std::min(lo, t.d[i].off);
-        hi = std::max(hi, t.d[i].off + t.d[i].bytes);
-        most = std::max(most, t.d[i].bytes);
-    }
-    MH_CHECK(hipMemcpyAsync(ctx->mirror.p + lo, ctx->stage + lo, hi - lo, hipMemcpyHostToDevice, ctx->stream));
+    unsigned long long most = 0;
+    for (int i = 0; i < t.n; ++i) most = std::max(most, t.d[i].bytes);
     const UpTable send = t;
     t.n = 0;
     hipLaunchKernelGGL(k_upload_scatter, dim3((unsigned)((most + kScatterChunk - 1) / kScatterChunk), (unsigned)send.n),
-                       dim3(256), 0, ctx->stream, (const char *)ctx->mirror.p, send);
+                       dim3(256), 0, ctx->stream, (const char *)ctx->stage_dev, send);
     MH_LAUNCH_CHECK();
 }
 
@@ -714,7 +716,6 @@ void ctx_free(mars_hip_ctx *ctx)
     }
     if (ctx->stage) (void)hipHostFree(ctx->stage);
     ctx->stage = nullptr;
-    ctx->mirror.release();
     for (auto &fs : ctx->fits) fs.release();
     DBuf<double> *db[] = {&ctx->tpart, &ctx->T, &ctx->T2, &ctx->tnorm, &ctx->spart, &ctx->ssum, &ctx->ybx,
                           &ctx->lin, &ctx->hin, &ctx->aux, &ctx->red, &ctx->vcol, &ctx->scal,

@@ -82,7 +82,7 @@ extern thread_local std::string g_last_error;
  * reused as soon as it returns. */
 void h2d(mars_hip_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* Deferred small upload: staged now, sent by the next h2d_flush (or ctx_sync) together with the
- * other pending ones (one copy and one scatter kernel instead of a copy each).  The caller
+ * other pending ones (one scatter kernel reading pinned memory instead of a copy each).  The caller
  * flushes before launching any kernel that reads `dst`, and does not reallocate `dst` first. */
 void h2d_defer(mars_hip_ctx *ctx, void *dst, const void *src, size_t bytes);
 void h2d_flush(mars_hip_ctx *ctx);
