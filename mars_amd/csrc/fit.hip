@@ -580,6 +580,10 @@ void fit_state_alloc(mars_hip_ctx *ctx, FitState &fs, size_t n, double tol)
     MH_CHECK(hipMemsetAsync(fs.y.p, 0, (fs.ldx + 32) * sizeof(float), ctx->stream));
     fs.ybo.ensure(ctx->max_terms);
     fs.v.ensure(n);
+    // room for the residuals of 64 candidates (either layout), allocated with the fit so that a
+    // forward pass does not begin with one large allocation per fit; grown when a step needs more
+    fs.ldr = 64;
+    fs.rbuf.ensure(fs.ldx * fs.ldr);
 }
 
 constexpr size_t kStageMax = 16u << 20;  // larger copies (one-time data uploads) go direct
