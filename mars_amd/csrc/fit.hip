@@ -584,6 +584,7 @@ void fit_state_alloc(mars_hip_ctx *ctx, FitState &fs, size_t n, double tol)
     // forward pass does not begin with one large allocation per fit; grown when a step needs more
     fs.ldr = 64;
     fs.rbuf.ensure(fs.ldx * fs.ldr);
+    fs.G.ensure(ctx->ldb * ctx->ldb);
 }
 
 constexpr size_t kStageMax = 16u << 20;  // larger copies (one-time data uploads) go direct
@@ -1315,6 +1316,7 @@ int mars_hip_fit_create(const float *X, size_t ldX, const float *y, const float 
             ctx->resident = true;
             ctx->form = form;
             fit_init(ctx, ctx->fits[0], X, ldX, y, w, n, p_x);
+            if (form == MARS_HIP_FORM_MIXED) ensure_tcache(ctx);
         } catch (...) {
             ctx_free(ctx);
             throw;

@@ -42,6 +42,9 @@ void mixed_score(mars_hip_ctx *ctx);
 /* Launch the mixed form's cache update for every fit's newest basis column; it needs no candidate
  * list, so a caller can build the list while the update runs (mixed_score then finds it done). */
 void mixed_prepare(mars_hip_ctx *ctx);
+/* Allocate the mixed form's T cache of every fit that has none yet (at creation for the mixed
+ * form, else at its first use). */
+void ensure_tcache(mars_hip_ctx *ctx);
 
 /* Hinge sweep of fit 0's candidates, whose normalized columns sit in its rbuf. */
 void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span);

@@ -442,6 +442,8 @@ __global__ void k_put(const int *__restrict__ idx, const double *__restrict__ va
 
 constexpr size_t kMaxMixedTerms = 256;  // cache: p_x * ldb^2 doubles per fit
 
+} // namespace
+
 /* Allocate the caches of fits that have none yet. */
 void ensure_tcache(mars_hip_ctx *ctx)
 {
@@ -453,6 +455,8 @@ void ensure_tcache(mars_hip_ctx *ctx)
         fs.mc = 0;
     }
 }
+
+namespace {
 
 /* Bring every fit's cache up to its current basis, one new column per fit per round. */
 void tc_update(mars_hip_ctx *ctx)
