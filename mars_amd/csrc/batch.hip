@@ -229,6 +229,7 @@ int mars_hip_batch_create(int nfits, const float *const *X, const size_t *ldX, c
                 fit_init(ctx, ctx->fits[f], X[f], ldX[f], y[f], wf, n[f], p_x[f]);
             }
             if (form == MARS_HIP_FORM_MIXED) ensure_tcache(ctx);
+            reserve_step_buffers(ctx);
         } catch (...) {
             ctx_free(ctx);
             throw;

@@ -1317,6 +1317,7 @@ int mars_hip_fit_create(const float *X, size_t ldX, const float *y, const float 
             ctx->form = form;
             fit_init(ctx, ctx->fits[0], X, ldX, y, w, n, p_x);
             if (form == MARS_HIP_FORM_MIXED) ensure_tcache(ctx);
+            reserve_step_buffers(ctx);
         } catch (...) {
             ctx_free(ctx);
             throw;

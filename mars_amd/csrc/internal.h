@@ -45,6 +45,10 @@ void mixed_prepare(mars_hip_ctx *ctx);
 /* Allocate the mixed form's T cache of every fit that has none yet (at creation for the mixed
  * form, else at its first use). */
 void ensure_tcache(mars_hip_ctx *ctx);
+/* Size the buffers indexed by candidate or work item for the context's largest step (every
+ * fit's X columns times max_terms - 1 parents, capped at 4M candidates), so that a forward pass
+ * does not grow them step by step; called when a resident context is created. */
+void reserve_step_buffers(mars_hip_ctx *ctx);
 
 /* Hinge sweep of fit 0's candidates, whose normalized columns sit in its rbuf. */
 void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span);

@@ -419,6 +419,28 @@ void upload_fits(mars_hip_ctx *ctx, const Cands &cs, const float *xover, const f
     h2d_defer(ctx, ctx->d_fits.p, h.data(), h.size() * sizeof(FitDev));
 }
 
+void reserve_step_buffers(mars_hip_ctx *ctx)
+{
+    const size_t parents = ctx->max_terms > 1 ? ctx->max_terms - 1 : 1;
+    size_t P = 0;
+    for (const FitState &fs : ctx->fits) P += fs.px * parents;
+    P = std::min<size_t>(P, size_t(1) << 22);  // beyond this the buffers grow when needed
+    const size_t F = ctx->fits.size();
+    const size_t ldT = 16 * std::min<size_t>(16, (ctx->max_terms + 15) / 16);
+    const size_t items = 3 * (8192 + P / 16 + F) + 1;  // the three passes' work lists
+    for (Cands *c : {&ctx->cand, &ctx->rcand}) {
+        c->d_fit.ensure(P);
+        c->d_xc.ensure(P);
+        c->d_par.ensure(P);
+    }
+    for (DBuf<double> *b : {&ctx->lin, &ctx->ybx, &ctx->rlin, &ctx->rybx, &ctx->tnorm, &ctx->ssum}) b->ensure(P);
+    for (DBuf<int> *b : {&ctx->flag, &ctx->rsel, &ctx->rkeep, &ctx->fired_list}) b->ensure(P);
+    ctx->T.ensure(P * ldT);
+    ctx->d_items.ensure(items);
+    ctx->d_cmap.ensure(9 * P + 1);
+    ctx->spart.ensure(items * kSlots);
+}
+
 void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, double *ybx,
                  const float *xover, const float *sover, bool rescore, const int *keep)
 {
