@@ -68,6 +68,33 @@ struct DBuf {
     }
 };
 
+/* Pinned host block for readbacks: grows 2x, and a replaced block is retired (freed at the next
+ * synchronisation), since a copy into it may still be in flight. */
+struct PinBuf {
+    char *p = nullptr;
+    size_t cap = 0;
+    template <typename T>
+    T *get(size_t count)
+    {
+        const size_t bytes = std::max<size_t>(count * sizeof(T), 1);
+        if (bytes > cap) {
+            const size_t want = std::max(bytes, 2 * cap);
+            if (p) retire_host(p);
+            p = nullptr;
+            cap = 0;
+            MH_CHECK(hipHostMalloc((void **)&p, want, hipHostMallocDefault));
+            cap = want;
+        }
+        return reinterpret_cast<T *>(p);
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
 /* Device view of one fit, read by every batched kernel. */
 struct FitDev {
     const float *X;    // raw X, column-major (ldx)
@@ -209,6 +236,7 @@ struct mars_hip_ctx {
     mh::AppendOut *aout_pin = nullptr;  // pinned host copy of the append results (aout_cap)
     size_t aout_cap = 0;
     int *rb_pin = nullptr;           // pinned host readback buffer (rb_cap ints)
+    mh::PinBuf sel_pin, best_pin;    // readbacks of the rescoring list and of the per-fit argmax
     size_t rb_cap = 0;
     // pinned staging for host-to-device copies: bump-allocated, reset after a stream sync
     char *stage = nullptr;

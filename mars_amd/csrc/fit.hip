@@ -757,6 +757,8 @@ void ctx_free(mars_hip_ctx *ctx)
     if (ctx->ev_app) (void)hipEventDestroy(ctx->ev_app);
     if (ctx->aout_pin) (void)hipHostFree(ctx->aout_pin);
     ctx->aout_pin = nullptr;
+    ctx->sel_pin.release();
+    ctx->best_pin.release();
     if (ctx->rb_pin) (void)hipHostFree(ctx->rb_pin);
     ctx->rb_pin = nullptr;
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -966,9 +968,10 @@ void argmax_fits(mars_hip_ctx *ctx, const Cands &cs, const double *lin, std::vec
                            ctx->ival.p + F, ctx->bout.p);
         MH_LAUNCH_CHECK();
     }
-    best.resize(F);
-    MH_CHECK(hipMemcpyAsync(best.data(), ctx->bout.p, F * sizeof(BestOut), hipMemcpyDeviceToHost, ctx->stream));
+    BestOut *pin = ctx->best_pin.get<BestOut>(F);
+    MH_CHECK(hipMemcpyAsync(pin, ctx->bout.p, F * sizeof(BestOut), hipMemcpyDeviceToHost, ctx->stream));
     ctx_sync(ctx);
+    best.assign(pin, pin + F);
 }
 
 void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t stride, int min_span,

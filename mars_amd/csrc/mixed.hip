@@ -614,12 +614,12 @@ void mixed_score(mars_hip_ctx *ctx)
     MH_LAUNCH_CHECK();
     // the list length and a prefix of the list sized from the previous step come back together;
     // a longer list takes a second copy
-    int nsel[2] = {0, 0};
     const int guess = std::min(P, std::max(2 * ctx->rsel_hint, 1024));
-    std::vector<int> sub(guess);
-    MH_CHECK(hipMemcpyAsync(nsel, d_nsel, sizeof(nsel), hipMemcpyDeviceToHost, st));
-    MH_CHECK(hipMemcpyAsync(sub.data(), ctx->rsel.p, guess * sizeof(int), hipMemcpyDeviceToHost, st));
+    int *pin = ctx->sel_pin.get<int>(2 + (size_t)P);  // [count, gated count, list...]
+    MH_CHECK(hipMemcpyAsync(pin, d_nsel, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    MH_CHECK(hipMemcpyAsync(pin + 2, ctx->rsel.p, guess * sizeof(int), hipMemcpyDeviceToHost, st));
     ctx_sync(ctx);
+    const int nsel[2] = {pin[0], pin[1]};
     const int R = nsel[0];
     require(R >= 0 && R <= P, "rescoring list length out of range");
     ctx->rsel_hint = R;
@@ -627,11 +627,11 @@ void mixed_score(mars_hip_ctx *ctx)
     ctx->rescored += R;
     if (R == 0) return;
     if (R > guess) {
-        sub.resize(R);
-        MH_CHECK(hipMemcpyAsync(sub.data() + guess, ctx->rsel.p + guess, (R - guess) * sizeof(int),
+        MH_CHECK(hipMemcpyAsync(pin + 2 + guess, ctx->rsel.p + guess, (R - guess) * sizeof(int),
                                 hipMemcpyDeviceToHost, st));
         ctx_sync(ctx);
     }
+    const int *sub = pin + 2;
     Cands &rc = ctx->rcand;
     rc.fit.resize(R);
     rc.xc.resize(R);

@@ -220,6 +220,13 @@ Pass make_pass(const std::vector<int> &fb, const std::vector<int> &fe, const std
     Pass p;
     const int total = fe.empty() ? 0 : *std::max_element(fe.begin(), fe.end());
     p.cmap.assign(3 * (size_t)std::max(total, 0), 0);
+    size_t count = 0;
+    for (size_t f = 0; f < fb.size(); ++f) {
+        if (fe[f] <= fb[f]) continue;
+        const long long r = std::max<long long>(align, (rows[f] + align - 1) / align * align);
+        count += (size_t)((fe[f] - fb[f] + tile - 1) / tile) * (size_t)((n[f] + r - 1) / r);
+    }
+    p.items.reserve(count);
     for (size_t f = 0; f < fb.size(); ++f) {
         if (fe[f] <= fb[f]) continue;
         long long r = std::max<long long>(align, (rows[f] + align - 1) / align * align);
