@@ -59,7 +59,7 @@ constexpr int kTW = 8;         // A tiles (waves) per block at most
 constexpr int kXP = 36;        // LDS floats per staged X column (32 rows + pad)
 
 template <int W>
-__global__ __launch_bounds__(64 * W) void k_tc_update(const FitDev *__restrict__ fits,
+__global__ __launch_bounds__(64 * W, W >= 4 ? 3 : 1) void k_tc_update(const FitDev *__restrict__ fits,
                                                       const Item *__restrict__ items, int nitems,
                                                       double *__restrict__ tpart, double *__restrict__ npart)
 {
