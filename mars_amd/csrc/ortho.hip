@@ -496,21 +496,29 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     const bool narrow = narrow_for(cnt);
     const int tileT = 16 * (narrow ? 1 : cg_tgemm(mt)), tileP = 16 * (narrow ? 1 : cg_project(mt));
 
+    // equal tile widths give the two passes the same work list: it is built and sent once
+    const bool shared = tileT == tileP;
     Pass pT = make_pass(fb, fe, n, split_rows(n, cnt, tileT, 8192, 256), tileT, kWR);
-    Pass pP = make_pass(fb, fe, n, split_rows(n, cnt, tileP, 8192, 256), tileP, kWR);
+    Pass pPown;
+    if (!shared) pPown = make_pass(fb, fe, n, split_rows(n, cnt, tileP, 8192, 256), tileP, kWR);
+    const Pass &pP = shared ? pT : pPown;
     Pass pF;  // finalize work list, over stored residuals (hinge path, rescoring)
     const int tileF = narrow ? 16 : kTileF;
     if (store_r) pF = make_pass(fb, fe, n, split_rows(n, cnt, tileF, 2048, 512), tileF, 32);
     Uploaded uT, uP, uF;
     size_t io = 0, co = 0;
-    const size_t total_items = pT.items.size() + pP.items.size() + pF.items.size();
-    const size_t total_cmap = pT.cmap.size() + pP.cmap.size() + pF.cmap.size();
+    const size_t total_items = pT.items.size() + pPown.items.size() + pF.items.size();
+    const size_t total_cmap = pT.cmap.size() + pPown.cmap.size() + pF.cmap.size();
     ctx->d_items.ensure(total_items + 1);
     ctx->d_cmap.ensure(total_cmap + 1);
     upload_pass(ctx, pT, ctx->d_items, ctx->d_cmap, io, co, uT);
     io += pT.items.size(); co += pT.cmap.size();
-    upload_pass(ctx, pP, ctx->d_items, ctx->d_cmap, io, co, uP);
-    io += pP.items.size(); co += pP.cmap.size();
+    if (shared) {
+        uP = uT;
+    } else {
+        upload_pass(ctx, pP, ctx->d_items, ctx->d_cmap, io, co, uP);
+        io += pP.items.size(); co += pP.cmap.size();
+    }
     upload_pass(ctx, pF, ctx->d_items, ctx->d_cmap, io, co, uF);
     h2d_flush(ctx);  // fits, candidates and the three work lists in one copy
 
