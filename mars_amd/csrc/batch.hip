@@ -16,6 +16,25 @@
 namespace mh {
 namespace {
 
+/* Append arguments from the device argmax: fit f appends the linear term of its best candidate
+ * (X column, parent) when it had candidates and that candidate's score is above 0 (mars.py:345-346);
+ * otherwise its argument is inactive.  One thread per fit. */
+__global__ void k_batch_args(int F, const int *__restrict__ c0, const int *__restrict__ cnt,
+                             const BestOut *__restrict__ best, const int *__restrict__ cxc,
+                             const int *__restrict__ cpar, AppendArg *__restrict__ args)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    AppendArg a{0, 0, 0, 0, 0.0f, 0};
+    if (cnt[f] > 0 && best[f].v1 > 0.0) {
+        const int g = c0[f] + best[f].c1;
+        a.xcol = cxc[g];
+        a.bcol = cpar[g];
+        a.active = 1;
+    }
+    args[f] = a;
+}
+
 struct FitDriver {
     size_t max_terms = 0;            // after the dof clamp (mars.py:241-243)
     std::vector<std::vector<int>> vars;  // variables of each basis function
@@ -94,29 +113,27 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
                 plain_score(ctx, cs, false, ctx->lin.p, ctx->ybx.p);
             // no synchronisation here: the argmax that follows on the same stream returns the result
         }
+        // the argmax and the append arguments stay on the device: the append is queued at once,
+        // and the host reads the winners with the append's results
         {
             HostTimer t(ctx, K_H_ARGMAX);
-            argmax_fits(ctx, cs, ctx->lin.p, best);
+            argmax_launch(ctx, cs, ctx->lin.p);
+            ctx->aarg.ensure(F);  // before the kernel writes it: the append must not reallocate it
+            hipLaunchKernelGGL(k_batch_args, dim3((unsigned)((F + 255) / 256)), dim3(256), 0, ctx->stream, (int)F,
+                               (const int *)ctx->ival.p, (const int *)ctx->ival.p + F,
+                               (const BestOut *)ctx->bout.p, (const int *)cs.d_xc.p, (const int *)cs.d_par.p,
+                               ctx->aarg.p);
+            MH_LAUNCH_CHECK();
         }
+        std::vector<double> sse0(F, 0.0), m0(F, 0.0);
         for (size_t f = 0; f < F; ++f) {
-            args[f] = AppendArg{0, 0, 0, 0, 0.0f, 0};
-            FitDriver &d = D[f];
-            if (!d.active) continue;
-            const FitState &fs = ctx->fits[f];
-            const double s1 = cs.fit_P[f] ? best[f].v1 : 0.0;
-            if (s1 <= 0.0) {  // all input data is filtered or zero (mars.py:345-346)
-                d.active = false;
-                continue;
+            // fits that may append: the active ones with candidates (the device drops a fit whose
+            // best score is not above 0)
+            args[f] = AppendArg{0, 0, 0, (D[f].active && cs.fit_P[f] > 0) ? 1 : 0, 0.0f, 0};
+            if (args[f].active) {
+                sse0[f] = host_dsse(ctx->fits[f]);
+                m0[f] = (double)ctx->fits[f].m;
             }
-            const double n = (double)fs.n, m = (double)fs.m;
-            const double sse0 = host_dsse(fs);
-            auto gcv = [&](double mse, double mm) { return mse / std::pow(1.0 - dof(mm) / n, 2); };
-            const double mse1 = gcv((1.0 - sse0 - s1) / n, m + 1);
-            const double mse2 = gcv((1.0 - sse0 - 0.0) / n, m + 2);
-            require(mse1 <= mse2, "linear_only batch fit chose a hinge pair");
-            const int c = best[f].c1;  // position within the fit's candidates
-            const int g = cs.fit_c0[f] + c;
-            args[f] = AppendArg{0, cs.xc[g], cs.par[g], 1, 0.0f, 0};
         }
         std::vector<AppendOut> out;
         {
@@ -137,16 +154,28 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
                     for (size_t f = 0; f < F; ++f)
                         if (bump[f]) ctx->fits[f].m -= 1;
                 };
-            fit_append_batch(ctx, args, out, ahead);
+            fit_append_batch(ctx, args, out, ahead, true, &best);
             for (size_t f = 0; f < F; ++f) {
                 FitState &fs = ctx->fits[f];
                 if (fs.mc > fs.m) fs.mc = fs.m;
             }
         }
         for (size_t f = 0; f < F; ++f) {
-            if (!args[f].active) continue;
             FitDriver &d = D[f];
+            if (!d.active) continue;
+            if (!args[f].active) {  // no candidates, or all input data is filtered or zero (mars.py:345-346)
+                d.active = false;
+                continue;
+            }
             const FitState &fs = ctx->fits[f];
+            {
+                // the GCV choice (mars.py:349-357): the linear term, as the fit is linear_only
+                const double s1 = best[f].v1, n = (double)fs.n, m = m0[f];
+                auto gcv = [&](double e, double mm) { return e / std::pow(1.0 - dof(mm) / n, 2); };
+                const double mse1 = gcv((1.0 - sse0[f] - s1) / n, m + 1);
+                const double mse2 = gcv((1.0 - sse0[f] - 0.0) / n, m + 2);
+                require(mse1 <= mse2, "linear_only batch fit chose a hinge pair");
+            }
             const double mse = out[f].mse;
             if (mse < 0) {  // degenerate append: no progress (mars.py:393-394)
                 d.active = false;

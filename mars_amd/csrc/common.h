@@ -237,6 +237,7 @@ struct mars_hip_ctx {
     size_t aout_cap = 0;
     int *rb_pin = nullptr;           // pinned host readback buffer (rb_cap ints)
     mh::PinBuf sel_pin, best_pin;    // readbacks of the rescoring list and of the per-fit argmax
+    mh::PinBuf aarg_pin;             // readback of append arguments built on the device
     size_t rb_cap = 0;
     // pinned staging for host-to-device copies: bump-allocated, reset after a stream sync
     char *stage = nullptr;

@@ -759,6 +759,7 @@ void ctx_free(mars_hip_ctx *ctx)
     ctx->aout_pin = nullptr;
     ctx->sel_pin.release();
     ctx->best_pin.release();
+    ctx->aarg_pin.release();
     if (ctx->rb_pin) (void)hipHostFree(ctx->rb_pin);
     ctx->rb_pin = nullptr;
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -826,8 +827,8 @@ void fit_init(mars_hip_ctx *ctx, FitState &fs, const float *X, size_t ldX, const
  * counterpart of MarsAlgo::append, marsalgo.cc:760-875).  Validation happens
  * on the host with the reference's messages; out[f] receives the result.
  */
-void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std::vector<AppendOut> &out,
-                      const std::function<void()> &before_readback)
+void fit_append_batch(mars_hip_ctx *ctx, std::vector<AppendArg> &args, std::vector<AppendOut> &out,
+                      const std::function<void()> &before_readback, bool device_args, std::vector<BestOut> *best)
 {
     PendingScope pending_scope(ctx);
     require(ctx->resident, "context was not created by mars_hip_fit_create");
@@ -838,13 +839,15 @@ void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std
         if (!args[f].active) continue;
         const FitState &fs = ctx->fits[f];
         if (fs.m >= ctx->max_terms) throw Error(MARS_HIP_EFULL, "basis matrix is full");
-        if (args[f].bcol < 0 || (size_t)args[f].bcol >= fs.m) {
+        if (!device_args && (args[f].bcol < 0 || (size_t)args[f].bcol >= fs.m)) {
             char msg[80];
             snprintf(msg, sizeof(msg), "invalid basis column number: %d", args[f].bcol);
             throw Error(MARS_HIP_EINVAL, msg);
         }
-        require(args[f].xcol >= 0 && (size_t)args[f].xcol < fs.px, "invalid X column index");
-        require(args[f].type >= 0 && args[f].type <= 2, "invalid basis type");
+        if (!device_args) {  // device arguments come from the fit's own candidates
+            require(args[f].xcol >= 0 && (size_t)args[f].xcol < fs.px, "invalid X column index");
+            require(args[f].type >= 0 && args[f].type <= 2, "invalid basis type");
+        }
         require(fs.m < (size_t)kMaxAppendCols, "the device append supports up to 512 basis functions");
         mmax = std::max(mmax, fs.m);
         nmax = std::max(nmax, fs.n);
@@ -861,7 +864,7 @@ void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std
     ctx->red.ensure(3 * F * nb);
     ctx->scal.ensure(F * S_COUNT);
     ctx->flag.ensure(F);
-    h2d_defer(ctx, ctx->aarg.p, args.data(), F * sizeof(AppendArg));
+    if (!device_args) h2d_defer(ctx, ctx->aarg.p, args.data(), F * sizeof(AppendArg));
     std::vector<FitW> fw(F);
     for (size_t f = 0; f < F; ++f) {
         FitState &fs = ctx->fits[f];
@@ -924,10 +927,24 @@ void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std
         ctx->aout_cap = F;
     }
     MH_CHECK(hipMemcpyAsync(ctx->aout_pin, ctx->aout.p, F * sizeof(AppendOut), hipMemcpyDeviceToHost, st));
+    AppendArg *apin = nullptr;
+    BestOut *bpin = nullptr;
+    if (device_args) {
+        apin = ctx->aarg_pin.get<AppendArg>(F);
+        MH_CHECK(hipMemcpyAsync(apin, ctx->aarg.p, F * sizeof(AppendArg), hipMemcpyDeviceToHost, st));
+        if (best) {
+            bpin = ctx->best_pin.get<BestOut>(F);
+            MH_CHECK(hipMemcpyAsync(bpin, ctx->bout.p, F * sizeof(BestOut), hipMemcpyDeviceToHost, st));
+        }
+    }
     MH_CHECK(hipEventRecord(ctx->ev_app, st));
     if (before_readback) before_readback();
     MH_CHECK(hipEventSynchronize(ctx->ev_app));
     std::copy(ctx->aout_pin, ctx->aout_pin + F, out.begin());
+    if (device_args) {
+        args.assign(apin, apin + F);
+        if (best) best->assign(bpin, bpin + F);
+    }
     resolve_timing(ctx);
     for (size_t f = 0; f < F; ++f) {
         if (!args[f].active) continue;
@@ -949,7 +966,7 @@ double host_dsse(const FitState &fs)
 }
 
 /* Per-fit best linear candidate (first maximum in row-major (xcol, bcol) order). */
-void argmax_fits(mars_hip_ctx *ctx, const Cands &cs, const double *lin, std::vector<BestOut> &best)
+void argmax_launch(mars_hip_ctx *ctx, const Cands &cs, const double *lin)
 {
     PendingScope pending_scope(ctx);
     const size_t F = ctx->fits.size();
@@ -968,10 +985,6 @@ void argmax_fits(mars_hip_ctx *ctx, const Cands &cs, const double *lin, std::vec
                            ctx->ival.p + F, ctx->bout.p);
         MH_LAUNCH_CHECK();
     }
-    BestOut *pin = ctx->best_pin.get<BestOut>(F);
-    MH_CHECK(hipMemcpyAsync(pin, ctx->bout.p, F * sizeof(BestOut), hipMemcpyDeviceToHost, ctx->stream));
-    ctx_sync(ctx);
-    best.assign(pin, pin + F);
 }
 
 void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t stride, int min_span,
@@ -1372,7 +1385,8 @@ int mars_hip_fit_append(mars_hip_ctx *ctx, char type, int xcol, int bcol, float 
         else if (type == '-') a.type = 2;
         else throw Error(MARS_HIP_EINVAL, "invalid basis type");
         std::vector<AppendOut> o;
-        fit_append_batch(ctx, {a}, o);
+        std::vector<AppendArg> args{a};
+        fit_append_batch(ctx, args, o);
         *mse = o[0].mse;
     });
 }

@@ -77,10 +77,17 @@ void fit_init(mars_hip_ctx *ctx, FitState &fs, const float *X, size_t ldX, const
               size_t n, size_t px);
 /* Append one basis function to every fit with args[f].active.  `before_readback`, when set,
  * is called after the append's kernels and the copy of its results are queued and before the
- * host waits for those results: work it queues overlaps the host's reading of them. */
-void fit_append_batch(mars_hip_ctx *ctx, const std::vector<AppendArg> &args, std::vector<AppendOut> &out,
-                      const std::function<void()> &before_readback = {});
-void argmax_fits(mars_hip_ctx *ctx, const Cands &cs, const double *lin, std::vector<BestOut> &best);
+ * host waits for those results: work it queues overlaps the host's reading of them.
+ * `device_args`: the arguments are already in ctx->aarg on the device (queued before this call);
+ * `args` then only marks the fits that may be active (it sizes the launch), and on return holds
+ * the device's arguments, read back with the results (with `best`, when given, the per-fit
+ * argmax ctx->bout as well). */
+void fit_append_batch(mars_hip_ctx *ctx, std::vector<AppendArg> &args, std::vector<AppendOut> &out,
+                      const std::function<void()> &before_readback = {}, bool device_args = false,
+                      std::vector<BestOut> *best = nullptr);
+/* Per-fit argmax of `lin` over each fit's candidates (first maximum in candidate order), queued
+ * on the stream: ctx->bout receives it, ctx->ival holds the fits' candidate offsets and counts. */
+void argmax_launch(mars_hip_ctx *ctx, const Cands &cs, const double *lin);
 double host_dsse(const FitState &fs);
 extern thread_local std::string g_last_error;
 
