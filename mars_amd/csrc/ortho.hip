@@ -25,6 +25,9 @@ namespace mh {
 namespace {
 
 constexpr int kTileF = 64;   // candidates per finalize item (x 4 row phases)
+// work items per projection / T-GEMM pass: about 32 one-wave items per SIMD, so a pass keeps
+// every CU busy however few candidates each fit has (8192 measured 10% slower on the mixed form)
+constexpr double kPassItems = 32768;
 
 /* T[c] = sum of the candidate's item partials in order; tnorm[c] = |T[c][:m]|^2.
  * One wave per candidate (or list position). */
@@ -434,7 +437,7 @@ void reserve_step_buffers(mars_hip_ctx *ctx)
     P = std::min<size_t>(P, size_t(1) << 22);  // beyond this the buffers grow when needed
     const size_t F = ctx->fits.size();
     const size_t ldT = 16 * std::min<size_t>(16, (ctx->max_terms + 15) / 16);
-    const size_t items = 3 * (8192 + P / 16 + F) + 1;  // the three passes' work lists
+    const size_t items = 3 * ((size_t)kPassItems + P / 16 + F) + 1;  // the three passes' work lists
     for (Cands *c : {&ctx->cand, &ctx->rcand}) {
         c->d_fit.ensure(P);
         c->d_xc.ensure(P);
@@ -498,9 +501,9 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
 
     // equal tile widths give the two passes the same work list: it is built and sent once
     const bool shared = tileT == tileP;
-    Pass pT = make_pass(fb, fe, n, split_rows(n, cnt, tileT, 8192, 256), tileT, kWR);
+    Pass pT = make_pass(fb, fe, n, split_rows(n, cnt, tileT, kPassItems, 256), tileT, kWR);
     Pass pPown;
-    if (!shared) pPown = make_pass(fb, fe, n, split_rows(n, cnt, tileP, 8192, 256), tileP, kWR);
+    if (!shared) pPown = make_pass(fb, fe, n, split_rows(n, cnt, tileP, kPassItems, 256), tileP, kWR);
     const Pass &pP = shared ? pT : pPown;
     Pass pF;  // finalize work list, over stored residuals (hinge path, rescoring)
     const int tileF = narrow ? 16 : kTileF;
@@ -670,7 +673,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
             for (size_t f = 0; f < F; ++f) lcnt[f] = te[f] - tb[f];
             const bool lnarrow = narrow_for(lcnt);
             const int ltP = 16 * (lnarrow ? 1 : cg_project(mt));
-            Pass pL = make_pass(tb, te, n, split_rows(n, lcnt, ltP, 8192, 256), ltP, kWR);
+            Pass pL = make_pass(tb, te, n, split_rows(n, lcnt, ltP, kPassItems, 256), ltP, kWR);
             Uploaded uL;
             ctx->live_list.ensure(nl);
             ctx->d_litems.ensure(pL.items.size() + 1);
@@ -715,7 +718,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                 for (size_t f = 0; f < F; ++f) lcnt[f] = le[f] - lb[f];
                 const bool lnarrow = narrow_for(lcnt);
                 const int ltP = 16 * (lnarrow ? 1 : cg_project(mt));
-                Pass pL = make_pass(lb, le, n, split_rows(n, lcnt, ltP, 8192, 256), ltP, kWR);
+                Pass pL = make_pass(lb, le, n, split_rows(n, lcnt, ltP, kPassItems, 256), ltP, kWR);
                 Uploaded uL;
                 ctx->live_list.ensure(nl);
                 ctx->d_litems.ensure(pL.items.size() + 1);
