@@ -28,6 +28,7 @@ constexpr int kTileF = 64;   // candidates per finalize item (x 4 row phases)
 // work items per projection / T-GEMM pass: about 32 one-wave items per SIMD, so a pass keeps
 // every CU busy however few candidates each fit has (8192 measured 10% slower on the mixed form)
 constexpr double kPassItems = 32768;
+constexpr long long kMaxSplits = 2048;  // row ranges per candidate tile at most
 
 /* T[c] = sum of the candidate's item partials in order; tnorm[c] = |T[c][:m]|^2.
  * One wave per candidate (or list position). */
@@ -269,6 +270,8 @@ std::vector<long long> split_rows(const std::vector<long long> &n, const std::ve
         const double want = work > 0 ? target * n[f] * tiles / work / std::max(tiles, 1.0) : 1;
         long long splits = std::max<long long>(1, (long long)want);
         splits = std::min<long long>(splits, std::max<long long>(1, n[f] / minrows));
+        // a tile's partial sums are added by one wave per candidate: bound how many there are
+        splits = std::min<long long>(splits, kMaxSplits);
         r[f] = (n[f] + splits - 1) / splits;
     }
     return r;
