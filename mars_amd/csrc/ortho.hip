@@ -448,7 +448,8 @@ void reserve_step_buffers(mars_hip_ctx *ctx)
     }
     for (DBuf<double> *b : {&ctx->lin, &ctx->ybx, &ctx->rlin, &ctx->rybx, &ctx->tnorm, &ctx->ssum}) b->ensure(P);
     for (DBuf<int> *b : {&ctx->flag, &ctx->rsel, &ctx->rkeep, &ctx->fired_list}) b->ensure(P);
-    ctx->T.ensure(P * ldT);
+    // T rows: every candidate in the plain form, the rescored few in the mixed form
+    ctx->T.ensure((ctx->form == MARS_HIP_FORM_MIXED ? std::min<size_t>(P, 65536) : P) * ldT);
     ctx->d_items.ensure(items);
     ctx->d_cmap.ensure(9 * P + 1);
     ctx->spart.ensure(items * kSlots);
