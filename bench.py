@@ -74,18 +74,29 @@ def run_batch(fits, fit_kw, precision, steps, warmup):
         return marslib.Batch(Xs, ys, None, max_terms, precision=precision)
 
     for _ in range(warmup):
-        mars_amd.fit_batch(Xs, ys, batch=make(), max_terms=max_terms, precision=precision, **kw)
-    t0 = time.perf_counter()
-    batches = [make() for _ in range(steps)]
-    upload_s = (time.perf_counter() - t0) / steps
+        b = make()
+        mars_amd.fit_batch(Xs, ys, batch=b, max_terms=max_terms, precision=precision, **kw)
+        b.close()
+    # one context at a time: each step's inputs are uploaded (untimed) into a fresh context, its
+    # forward pass is timed, and the context is freed before the next, so device memory does not
+    # grow with the step count
     models, stats = [], []
-    t0 = time.perf_counter()
-    for b in batches:
+    wall = upload = 0.0
+    marslib.device_bytes(reset_peak=True)
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        b = make()
+        t1 = time.perf_counter()
         st = {}
         models.append(mars_amd.fit_batch(Xs, ys, batch=b, max_terms=max_terms, precision=precision,
                                          stats=st, timing=True, **kw))
+        t2 = time.perf_counter()  # fit_batch returns after the stream has drained
+        b.close()
         stats.append(st)
-    wall = time.perf_counter() - t0
+        upload += t1 - t0
+        wall += t2 - t1
+    upload_s = upload / steps
+    peak_bytes = marslib.device_bytes()[1]
     ks = {}
     for st in stats:
         for r in st["kernels"]:
@@ -96,7 +107,7 @@ def run_batch(fits, fit_kw, precision, steps, warmup):
     st = stats[0]
     return dict(wall=wall, fit_s=wall / steps, upload_s=upload_s, cands=st["scored"], kstats=ks, seq=seq,
                 mse=st["mse"], epochs=st["steps"], dgks=st["dgks_fired"], rescored=st["rescored"], gated=st["gated"],
-                terms=sum(len(m) for m in models[0]))
+                terms=sum(len(m) for m in models[0]), peak_device_bytes=peak_bytes)
 
 
 def run_form(X, y, fit_kw, precision, steps, warmup):
@@ -114,28 +125,34 @@ def run_form(X, y, fit_kw, precision, steps, warmup):
         return algo
 
     for _ in range(warmup):
-        mars_amd.fit(X, y, w, algo=one(False), precision=precision, **fit_kw)
-    t0 = time.perf_counter()
-    algos = [one(True) for _ in range(steps)]
-    upload_s = (time.perf_counter() - t0) / steps
-    traces = []
-    t0 = time.perf_counter()
-    models = []
-    for a in algos:
+        a = one(False)
+        mars_amd.fit(X, y, w, algo=a, precision=precision, **fit_kw)
+        a.close()
+    traces, models, ks = [], [], {}
+    wall = upload = 0.0
+    dgks = None
+    marslib.device_bytes(reset_peak=True)
+    for _ in range(steps):  # one context at a time (see run_batch)
+        t0 = time.perf_counter()
+        a = one(True)
+        t1 = time.perf_counter()
         tr = []
         models.append(mars_amd.fit(X, y, w, algo=a, trace=tr, precision=precision, **fit_kw))
+        t2 = time.perf_counter()
         traces.append(tr)
-    wall = time.perf_counter() - t0
-    cands = sum(s["n_cand"] for s in traces[0])
-    ks = {}
-    for a in algos:
         for r in a.kernel_stats():
             d = ks.setdefault(r["name"], dict(launches=0, ms=0.0, flops=0.0, bytes=0.0))
             for f in ("launches", "ms", "flops", "bytes"):
                 d[f] += r[f]
+        if dgks is None:
+            dgks = a.stats()["dgks_fired"]
+        a.close()
+        upload += t1 - t0
+        wall += t2 - t1
+    cands = sum(s["n_cand"] for s in traces[0])
     seq = [(r["type"].decode(), int(r["basis"]), int(r["input"]), float(r["hinge"])) for r in models[0]]
-    return dict(wall=wall, fit_s=wall / steps, upload_s=upload_s, cands=cands, kstats=ks, seq=seq,
-                trace=traces[0], epochs=len(traces[0]), dgks=algos[0].stats()["dgks_fired"])
+    return dict(wall=wall, fit_s=wall / steps, upload_s=upload / steps, cands=cands, kstats=ks, seq=seq,
+                trace=traces[0], epochs=len(traces[0]), dgks=dgks, peak_device_bytes=marslib.device_bytes()[1])
 
 
 def roofline(ks):
@@ -153,6 +170,49 @@ def roofline(ks):
     ach = d["bytes"] / d["launches"] / avg_s
     return name, dict(bound="hbm", achieved=ach / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s", frac=ach / PEAK_HBM,
                       traffic=None, avg_launch_ms=d["ms"] / d["launches"], launches=d["launches"])
+
+
+def cpu_threads():
+    """Threads the CPU path uses with threads=-1: min(64, processors available), as the reference's
+    binding picks them (marslib.cc:61-62)."""
+    try:
+        procs = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        procs = os.cpu_count() or 1
+    return min(64, procs)
+
+
+def host_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(nproc=os.cpu_count(), cpu_model=model)
+
+
+def class_rooflines(ks):
+    """Achieved rate of every timed kernel class: summed algorithmic flops (or bytes) over the
+    launches its timer brackets, divided by the bracketed device time.  Composite classes bracket
+    several kernels (a pass kernel and its reduction, the append's kernels), so their rate is that of
+    the whole group."""
+    out = {}
+    for name, d in ks.items():
+        if name.startswith("host_") or d["ms"] <= 0:
+            continue
+        t = d["ms"] / 1e3
+        e = dict(ms=d["ms"], launches=d["launches"], composite=name in COMPOSITE)
+        if d["flops"] > 0 and name not in HBM_BOUND:
+            peak = PEAK_FP32 if name in FP32_KERNELS else PEAK_FP64
+            e.update(bound="mfma", achieved_tflops=d["flops"] / t / 1e12, frac=d["flops"] / t / peak)
+        elif d["bytes"] > 0:
+            e.update(bound="hbm", achieved_gbs=d["bytes"] / t / 1e9, frac=d["bytes"] / t / PEAK_HBM)
+        out[name] = e
+    return out
 
 
 def cpu_baseline(X, y, fit_kw, budget_s):
@@ -174,13 +234,13 @@ def cpu_baseline(X, y, fit_kw, budget_s):
         import oracle_marslib as lib
 
         kind = "port"
-    cores = min(16, os.cpu_count() or 1)
+    cores = cpu_threads()
     w = np.ones(len(X), dtype=np.float32)
     # bounded sample: the first epochs of the same fit, grown until about budget_s of CPU time
     epochs, spent, result = 2, 0.0, None
     while True:
         t0 = time.perf_counter()
-        _, tr = F.fit(lib, X, y, w, threads=cores, max_epochs=epochs, **fit_kw)
+        _, tr = F.fit(lib, X, y, w, threads=-1, max_epochs=epochs, **fit_kw)
         spent = time.perf_counter() - t0
         result = (tr, spent)
         if spent > budget_s / 3 or len(tr) < epochs:
@@ -189,7 +249,8 @@ def cpu_baseline(X, y, fit_kw, budget_s):
     tr, spent = result
     cands = sum(int(s.get("n_cand", 0)) for s in tr)
     return dict(value=cands / spent if spent > 0 else None, unit="candidates/s", cores=cores, kind=kind,
-                sample="first %d epochs of the same fit (%d candidates, %.1f s)" % (len(tr), cands, spent))
+                sample="first %d epochs of the same fit (%d candidates, %.1f s)" % (len(tr), cands, spent),
+                **host_info())
 
 
 def cpu_baseline_batch(fits, fit_kw, budget_s):
@@ -211,18 +272,19 @@ def cpu_baseline_batch(fits, fit_kw, budget_s):
         import oracle_marslib as lib
 
         kind = "port"
-    cores = min(16, os.cpu_count() or 1)
+    cores = cpu_threads()
     spent, cands, k = 0.0, 0, 0
     while k < len(fits) and spent < budget_s:
         X, y = fits[k]
         w = np.ones(len(X), dtype=np.float32)
         t0 = time.perf_counter()
-        _, tr = F.fit(lib, X, y, w, threads=cores, **fit_kw)
+        _, tr = F.fit(lib, X, y, w, threads=-1, **fit_kw)
         spent += time.perf_counter() - t0
         cands += sum(int(s.get("n_cand", 0)) for s in tr)
         k += 1
     return dict(value=cands / spent if spent > 0 else None, unit="candidates/s", cores=cores, kind=kind,
-                sample="fits 0..%d of the 256, complete (%d candidates, %.1f s)" % (k - 1, cands, spent))
+                sample="fits 0..%d of the 256, complete (%d candidates, %.1f s)" % (k - 1, cands, spent),
+                **host_info())
 
 
 def copy_probe(mib=1024, reps=4):
@@ -310,6 +372,8 @@ def main():
         "roofline_rule": "largest kernel class whose timer brackets exactly one kernel launch",
         "roofline": roof,
         "kernels": {k: dict(v, avg_ms=v["ms"] / max(v["launches"], 1)) for k, v in head["kstats"].items()},
+        "class_rooflines": {f: class_rooflines(r["kstats"]) for f, r in res.items()},
+        "peak_device_bytes": {f: r["peak_device_bytes"] for f, r in res.items()},
     }
     if len(res) > 1:
         out["speedup_mixed_vs_fp64"] = res["fp64"]["fit_s"] / res["mixed"]["fit_s"]

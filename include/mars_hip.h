@@ -138,6 +138,13 @@ int mars_hip_fit_basis(mars_hip_ctx *ctx, float *B, float *Bo, size_t *m_out);
  * spent in the scoring kernels since the context was created. */
 int mars_hip_stats(mars_hip_ctx *ctx, long *dgks_fired, double *eval_ms);
 
+/* Device memory held by the library's buffers in this process (all contexts; blocks
+ * replaced but not yet freed included), and the most held since the last reset.
+ * reset_peak != 0 sets the peak to the current amount after reading it.  No
+ * counterpart in the reference (its context sizes scratch once, mars_cuda.h:43):
+ * lets a caller check the footprint of a workload. */
+int mars_hip_device_bytes(size_t *live, size_t *peak, int reset_peak);
+
 /* ---------------------------------------------------------------------- */
 /*  Batching form: many independent linear_only fits advanced together    */
 /* ---------------------------------------------------------------------- */
@@ -215,6 +222,20 @@ int mars_hip_gram(const float *A, size_t ldA, size_t n, size_t ma, const float *
                   double *out, size_t ldOut);
 int mars_hip_gram_f64(const double *A, size_t ldA, size_t n, size_t ma, const double *B, size_t ldB, size_t mb,
                       double *out, size_t ldOut);
+
+/* Backward stepwise selection and coefficient solve of the backward pass:
+ * replaces the host part of mars.prune (mars.py:493-698: _init_inverse_state,
+ * _backward_eliminate and the final lstsq).  Input: the raw Gram products of
+ * the (weight-folded) basis, XX = B^T B (M x M, row stride ldXX), XY = B^T y,
+ * YY = y^T y, as mars_hip_gram computes them on the device; mask (M) the
+ * candidate terms (NULL: all).  Columns are standardised to a unit Gram
+ * diagonal (zero-norm columns are dropped), `ridge` loads that diagonal, the
+ * intercept (column 0) is never removed.  Output: beta (M) in the original
+ * scale, 0 for pruned terms; chosen (M) [may be NULL] the kept subset.
+ * Host-only (no device work), synchronous. */
+int mars_hip_prune_gram(const double *XX, size_t ldXX, const double *XY, double YY, size_t M,
+                        const unsigned char *mask, double penalty, double n_true, double ridge,
+                        double *beta, unsigned char *chosen);
 
 #ifdef __cplusplus
 }

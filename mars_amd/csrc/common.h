@@ -40,9 +40,13 @@ constexpr double kGate = 9.0;
 
 /* Buffers replaced while queued kernels may still read them.  hipFree waits for the whole
  * device, so they are freed at the next ctx_sync, after the stream has drained. */
-void retire_device(void *p);
+void retire_device(void *p, size_t bytes);
 void retire_host(void *p);
 void free_retired();
+
+/* Device bytes held by this library's buffers in the process (retired blocks included until
+ * they are freed), and the most held at any time since the last reset. */
+void dev_account(long long delta);
 
 /* Device buffer that only grows: the first allocation is exact, a regrowth takes at least
  * 2x (lists that grow every step reallocate O(log) times) and retires the old block. */
@@ -54,15 +58,19 @@ struct DBuf {
     {
         if (count <= cap) return;
         const size_t want = cap ? std::max(count, 2 * cap) : count;
-        if (p) retire_device(p);
+        if (p) retire_device(p, sizeof(T) * (cap ? cap : 1));
         p = nullptr;
         cap = 0;
         MH_CHECK(hipMalloc((void **)&p, sizeof(T) * (want ? want : 1)));
+        dev_account((long long)(sizeof(T) * (want ? want : 1)));
         cap = want;
     }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            dev_account(-(long long)(sizeof(T) * (cap ? cap : 1)));
+        }
         p = nullptr;
         cap = 0;
     }
@@ -113,6 +121,9 @@ struct FitDev {
     double *tcn;       // mixed form: cached |Bx|^2, Bx.y per (xc, par)
     long long n, ldx, ldb, rsr, rsc;
     int m, c0;         // live basis count, first candidate of this fit
+    int rbyl;          // 1: a list pass addresses rbuf columns by position in the fit's part of the
+                       // list (compact residual storage), 0: by candidate (column cand - c0)
+    int pad_;
     double tol;
 };
 
@@ -146,7 +157,9 @@ struct FitState {
     }
 };
 
-/* Work item of the batched kernels: candidates [c0, c0+nc) of fit `fit`, rows [r0, r1). */
+/* Work item of the batched kernels: candidates [c0, c0+nc) of fit `fit`, rows [r0, r1).
+ * pad: in the orthonormalization passes, the list position where the fit's candidates begin
+ * (compact residual columns are numbered from there); in the cache update, its tile code. */
 struct Item {
     int fit, c0, nc, pad;
     long long r0, r1;
@@ -260,6 +273,7 @@ struct mars_hip_ctx {
     mh::DBuf<int> rkeep;             // 1: the candidate was ungated in the cache approximation,
                                      // so its residual is stored for the final pass
     bool rbuf_cm = false;            // candidate columns in rbuf are column-major (linear scoring)
+    mh::DBuf<float> rarena;          // plain form's DGKS retry: compact residual columns (bounded)
     int rsel_hint = 0;               // the previous rescoring list's length
     mh::DBuf<mh::FitDev> d_fits;
     mh::DBuf<mh::Item> d_items;

@@ -3,6 +3,7 @@
 #include "internal.h"
 #include "kernels.h"
 
+#include <atomic>
 #include <algorithm>
 #include <functional>
 #include <cmath>
@@ -11,6 +12,8 @@
 namespace mh {
 
 thread_local std::string g_last_error;
+
+void set_last_error(const char *msg) { g_last_error = msg; }
 
 namespace {
 
@@ -581,9 +584,12 @@ void fit_state_alloc(mars_hip_ctx *ctx, FitState &fs, size_t n, double tol)
     fs.ybo.ensure(ctx->max_terms);
     fs.v.ensure(n);
     // room for the residuals of 64 candidates (either layout), allocated with the fit so that a
-    // forward pass does not begin with one large allocation per fit; grown when a step needs more
-    fs.ldr = 64;
-    fs.rbuf.ensure(fs.ldx * fs.ldr);
+    // forward pass does not begin with one large allocation per fit; grown when a step needs more.
+    // A plain-form batch stores none there (its DGKS retry uses the context's bounded arena).
+    if (ctx->row_major || ctx->form == MARS_HIP_FORM_MIXED) {
+        fs.ldr = 64;
+        fs.rbuf.ensure(fs.ldx * fs.ldr);
+    }
     fs.G.ensure(ctx->ldb * ctx->ldb);
 }
 
@@ -686,10 +692,20 @@ void h2d_flush(mars_hip_ctx *ctx)
 }
 
 namespace {
-thread_local std::vector<void *> g_retired_dev, g_retired_host;
+thread_local std::vector<std::pair<void *, size_t>> g_retired_dev;
+thread_local std::vector<void *> g_retired_host;
+std::atomic<long long> g_dev_live{0}, g_dev_peak{0};
 }
 
-void retire_device(void *p) { g_retired_dev.push_back(p); }
+void dev_account(long long delta)
+{
+    const long long now = g_dev_live.fetch_add(delta) + delta;
+    long long peak = g_dev_peak.load();
+    while (now > peak && !g_dev_peak.compare_exchange_weak(peak, now)) {
+    }
+}
+
+void retire_device(void *p, size_t bytes) { g_retired_dev.emplace_back(p, bytes); }
 void retire_host(void *p) { g_retired_host.push_back(p); }
 
 void free_retired()
@@ -697,7 +713,10 @@ void free_retired()
     if (g_retired_dev.empty() && g_retired_host.empty()) return;
     // another context on this thread may still have work queued on a retired block
     MH_CHECK(hipDeviceSynchronize());
-    for (void *p : g_retired_dev) MH_CHECK(hipFree(p));
+    for (auto &pb : g_retired_dev) {
+        MH_CHECK(hipFree(pb.first));
+        dev_account(-(long long)pb.second);
+    }
     for (void *p : g_retired_host) MH_CHECK(hipHostFree(p));
     g_retired_dev.clear();
     g_retired_host.clear();
@@ -1465,6 +1484,16 @@ int mars_hip_stats(mars_hip_ctx *ctx, long *dgks_fired, double *eval_ms)
         require(ctx != nullptr, "null context");
         if (dgks_fired) *dgks_fired = ctx->dgks_host;
         if (eval_ms) *eval_ms = ctx->eval_ms;
+    });
+}
+
+int mars_hip_device_bytes(size_t *live, size_t *peak, int reset_peak)
+{
+    return guarded([&] {
+        const long long now = g_dev_live.load();
+        if (live) *live = (size_t)std::max<long long>(now, 0);
+        if (peak) *peak = (size_t)std::max<long long>(g_dev_peak.load(), 0);
+        if (reset_peak) g_dev_peak.store(now);
     });
 }
 

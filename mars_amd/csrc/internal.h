@@ -55,7 +55,7 @@ void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span);
 
 /* Upload the per-fit device views (current m, rbuf, first candidate of `cs`). */
 void upload_fits(mars_hip_ctx *ctx, const Cands &cs, const float *xover = nullptr,
-                 const float *sover = nullptr);
+                 const float *sover = nullptr, const std::vector<size_t> *arena_off = nullptr);
 
 /* Upload the candidate arrays of `cs`. */
 void upload_cands(mars_hip_ctx *ctx, Cands &cs);

@@ -69,9 +69,13 @@ public:
         check(rc);
     }
 
-    ~MarsAlgo()
+    ~MarsAlgo() { close(); }
+
+    /* Free the device state now (the object is unusable afterwards). */
+    void close()
     {
         if (ctx_) mars_hip_context_destroy(ctx_);
+        ctx_ = nullptr;
     }
 
     int nbasis() const
@@ -241,9 +245,13 @@ public:
         nfits_ = F;
     }
 
-    ~Batch()
+    ~Batch() { close(); }
+
+    /* Free the device state now (the object is unusable afterwards). */
+    void close()
     {
         if (ctx_) mars_hip_context_destroy(ctx_);
+        ctx_ = nullptr;
     }
 
     /* returns (terms list of per-fit lists of tuples, steps) */
@@ -410,7 +418,8 @@ PYBIND11_MODULE(_marslib, m)
         .def("basis", &MarsAlgo::basis)
         .def("stats", &MarsAlgo::stats)
         .def("set_timing", &MarsAlgo::set_timing)
-        .def("kernel_stats", &MarsAlgo::kernel_stats);
+        .def("kernel_stats", &MarsAlgo::kernel_stats)
+        .def("close", &MarsAlgo::close);
     py::class_<Batch>(m, "Batch")
         .def(py::init<py::list, py::list, py::object, int, const std::string &>(), py::arg("Xs"), py::arg("ys"),
              py::arg("ws") = py::none(), py::arg("max_terms"), py::arg("precision") = "fp64")
@@ -418,7 +427,40 @@ PYBIND11_MODULE(_marslib, m)
              py::arg("self_interact") = false, py::arg("r2_window") = 32, py::arg("r2_thresh") = 3e-5)
         .def("set_timing", &Batch::set_timing)
         .def("kernel_stats", &Batch::kernel_stats)
-        .def("stats", &Batch::stats);
+        .def("stats", &Batch::stats)
+        .def("close", &Batch::close);
+    m.def(
+        "prune_gram",
+        [](py::array_t<double, py::array::c_style | py::array::forcecast> XX,
+           py::array_t<double, py::array::c_style | py::array::forcecast> XY, double YY, py::object mask,
+           double penalty, double n_true, double ridge) {
+            verify(XX.ndim() == 2 && XX.shape(0) == XX.shape(1), "XX must be square");
+            const size_t M = (size_t)XX.shape(0);
+            verify(XY.ndim() == 1 && (size_t)XY.shape(0) == M, "XY must have one entry per column");
+            std::vector<unsigned char> mk;
+            if (!mask.is_none()) {
+                auto a = py::cast<py::array_t<bool, py::array::c_style | py::array::forcecast>>(mask);
+                verify(a.ndim() == 1 && (size_t)a.shape(0) == M, "mask must have one entry per column");
+                mk.assign(a.data(), a.data() + M);
+            }
+            py::array_t<double> beta(M);
+            py::array_t<bool> chosen(M);
+            std::vector<unsigned char> ch(M);
+            check(mars_hip_prune_gram(XX.data(), M, XY.data(), YY, M, mk.empty() ? nullptr : mk.data(), penalty,
+                                      n_true, ridge, beta.mutable_data(), ch.data()));
+            for (size_t i = 0; i < M; ++i) chosen.mutable_data()[i] = ch[i] != 0;
+            return py::make_tuple(beta, chosen);
+        },
+        py::arg("XX"), py::arg("XY"), py::arg("YY"), py::arg("mask") = py::none(), py::arg("penalty") = 3.0,
+        py::arg("n_true") = 1.0, py::arg("ridge") = 0.0);
+    m.def(
+        "device_bytes",
+        [](bool reset_peak) {
+            size_t live = 0, peak = 0;
+            check(mars_hip_device_bytes(&live, &peak, reset_peak ? 1 : 0));
+            return py::make_tuple(live, peak);
+        },
+        py::arg("reset_peak") = false);
     m.def("expand_f32", &expand_impl<float>, py::arg("X"), py::arg("types"), py::arg("basis"), py::arg("input"),
           py::arg("hinge"));
     m.def("expand_f64", &expand_impl<double>, py::arg("X"), py::arg("types"), py::arg("basis"), py::arg("input"),

@@ -29,6 +29,8 @@ constexpr int kTileF = 64;   // candidates per finalize item (x 4 row phases)
 // every CU busy however few candidates each fit has (8192 measured 10% slower on the mixed form)
 constexpr double kPassItems = 32768;
 constexpr long long kMaxSplits = 2048;  // row ranges per candidate tile at most
+// residual bytes that the plain form's DGKS retry keeps at once (one arena per context)
+constexpr size_t kRetryArena = size_t(2) << 30;
 
 /* T[c] = sum of the candidate's item partials in order; tnorm[c] = |T[c][:m]|^2.
  * One wave per candidate (or list position). */
@@ -159,7 +161,7 @@ __global__ __launch_bounds__(256) void k_finalize_cm(const FitDev *__restrict__ 
             const int c = list ? list[I.c0 + slot] : I.c0 + slot;
             const double s = ssum[c];
             const float scale = (s > F.tol) ? (float)(1.0 / sqrt(s + F.tol)) : 0.0f;
-            const float *col = F.rbuf + (c - F.c0) * F.rsc;
+            const float *col = F.rbuf + (F.rbyl ? I.c0 + slot - I.pad : c - F.c0) * F.rsc;
             double acc = 0.0;
             // s <= tol: scale 0, so ybx = 0 exactly (kernels.cc:401-416) and r is not read; most
             // rescored candidates (duplicates of basis functions) end here
@@ -242,7 +244,7 @@ Pass make_pass(const std::vector<int> &fb, const std::vector<int> &fe, const std
                 it.fit = (int)f;
                 it.c0 = c0;
                 it.nc = nc;
-                it.pad = 0;
+                it.pad = fb[f];
                 it.r0 = r0;
                 it.r1 = std::min(n[f], r0 + r);
                 p.items.push_back(it);
@@ -388,16 +390,17 @@ void ensure_rbuf(mars_hip_ctx *ctx, const Cands &cs)
         FitState &fs = ctx->fits[f];
         const size_t need = ((size_t)std::max(cs.fit_P[f], 1) + 63) / 64 * 64;
         if (need > fs.ldr) {
-            // grow geometrically in columns so the buffer is reallocated O(log) times per fit
+            // grow geometrically in columns so the buffer is reallocated O(log) times per fit; the
+            // old block is retired (queued kernels may still read it), not freed mid-step
             const size_t ldr = std::max(need, std::min<size_t>(2 * fs.ldr, ctx->max_terms * fs.px + 64));
-            fs.rbuf.release();
             fs.rbuf.ensure(fs.ldx * ldr);  // room for either layout (column-major uses ldx rows)
             fs.ldr = ldr;
         }
     }
 }
 
-void upload_fits(mars_hip_ctx *ctx, const Cands &cs, const float *xover, const float *sover)
+void upload_fits(mars_hip_ctx *ctx, const Cands &cs, const float *xover, const float *sover,
+                 const std::vector<size_t> *arena_off)
 {
     std::vector<FitDev> h(ctx->fits.size());
     for (size_t f = 0; f < ctx->fits.size(); ++f) {
@@ -424,6 +427,14 @@ void upload_fits(mars_hip_ctx *ctx, const Cands &cs, const float *xover, const f
         d.ldb = (long long)ctx->ldb;
         d.rsr = ctx->rbuf_cm ? 1 : (long long)fs.ldr;
         d.rsc = ctx->rbuf_cm ? (long long)fs.ldx : 1;
+        d.rbyl = 0;
+        d.pad_ = 0;
+        if (arena_off) {  // compact column-major residuals in the retry arena
+            d.rbuf = ctx->rarena.p + (*arena_off)[f];
+            d.rsr = 1;
+            d.rsc = (long long)fs.ldx;
+            d.rbyl = 1;
+        }
         d.m = (int)fs.m;
         d.c0 = cs.fit_c0.empty() ? 0 : cs.fit_c0[f];
         d.tol = fs.tol;
@@ -455,6 +466,104 @@ void reserve_step_buffers(mars_hip_ctx *ctx)
     ctx->spart.ensure(items * kSlots);
 }
 
+namespace {
+
+/* A few candidates per fit (the mixed form's rescoring, DGKS retries) run on 16-wide candidate
+ * tiles: the wide tiles would be mostly zero padding in the FP64 MFMA work that bounds these
+ * kernels. */
+bool narrow_for(const std::vector<int> &c)
+{
+    long long tot = 0, act = 0;
+    for (int v : c) {
+        tot += v;
+        act += v > 0;
+    }
+    return tot <= 24 * act;
+}
+
+/*
+ * DGKS retry of the plain form's linear scoring (kernels.cc:387-399), for the fired candidates
+ * fired[rb[f], re[f]) of each fit: store the first residual r = f32(Bx - Bo T), T2 = Bo^T r,
+ * r = f32(r - Bo T2) in place, s = sum r^2, then ybx = sum f32(r scale) y.  The residuals go to a
+ * shared arena, column-major and numbered by position in the fit's part of the fired list, in
+ * blocks of at most kRetryArena bytes: the memory is bounded whatever the number of fired
+ * candidates or rows (the reference bounds its candidate block the same way,
+ * orthonormalize.cu:476-489).
+ */
+void retry_compact(mars_hip_ctx *ctx, Cands &cs, const std::vector<int> &fired, const std::vector<int> &rb,
+                   const std::vector<int> &re, const std::vector<long long> &n, int mt, int ldT, double *ybx,
+                   double *lin)
+{
+    hipStream_t st = ctx->stream;
+    const size_t F = ctx->fits.size();
+    ctx->dgks_host += (long)fired.size();
+    // blocks of the fired list (grouped by fit), each within the arena budget
+    size_t k0 = 0;
+    std::vector<int> fit_of(fired.size());
+    for (size_t f = 0; f < F; ++f)
+        for (int i = rb[f]; i < re[f]; ++i) fit_of[i] = (int)f;
+    while (k0 < fired.size()) {
+        size_t k1 = k0, bytes = 0;
+        while (k1 < fired.size()) {
+            const size_t col = ctx->fits[fit_of[k1]].ldx * sizeof(float);
+            if (k1 > k0 && bytes + col > kRetryArena) break;
+            bytes += col;
+            ++k1;
+        }
+        const int nf = (int)(k1 - k0);
+        std::vector<int> bb(F, 0), be(F, 0), fcnt(F, 0);
+        std::vector<size_t> off(F, 0);
+        size_t used = 0;
+        for (size_t f = 0; f < F; ++f) {
+            const int b = std::max<int>(rb[f], (int)k0), e = std::min<int>(re[f], (int)k1);
+            bb[f] = std::min<int>(b, (int)k1) - (int)k0;
+            be[f] = std::max(b, e) - (int)k0;
+            fcnt[f] = be[f] - bb[f];
+            off[f] = used;
+            used += (size_t)fcnt[f] * ctx->fits[f].ldx;
+        }
+        ctx->rarena.ensure(std::max<size_t>(used, 1));
+        upload_fits(ctx, cs, nullptr, nullptr, &off);
+        h2d_defer(ctx, ctx->fired_list.p, fired.data() + k0, nf * sizeof(int));
+        const bool rnarrow = narrow_for(fcnt);
+        const int rtT = 16 * (rnarrow ? 1 : cg_tgemm(mt)), rtP = 16 * (rnarrow ? 1 : cg_project(mt));
+        Pass rT = make_pass(bb, be, n, split_rows(n, fcnt, rtT, 4096, 256), rtT, kWR);
+        Pass rP = make_pass(bb, be, n, split_rows(n, fcnt, rtP, 4096, 256), rtP, kWR);
+        Pass fF = make_pass(bb, be, n, split_rows(n, fcnt, 16, 2048, 512), 16, 32);
+        Uploaded vT, vP, wF;
+        ctx->d_ritems.ensure(rT.items.size() + rP.items.size() + fF.items.size() + 1);
+        ctx->d_rcmap.ensure(rT.cmap.size() + rP.cmap.size() + fF.cmap.size() + 1);
+        upload_pass(ctx, rT, ctx->d_ritems, ctx->d_rcmap, 0, 0, vT);
+        upload_pass(ctx, rP, ctx->d_ritems, ctx->d_rcmap, rT.items.size(), rT.cmap.size(), vP);
+        upload_pass(ctx, fF, ctx->d_ritems, ctx->d_rcmap, rT.items.size() + rP.items.size(),
+                    rT.cmap.size() + rP.cmap.size(), wF);
+        h2d_flush(ctx);
+        ctx->tpart.ensure(std::max<size_t>(rT.items.size(), 1) * kSlots * ldT);
+        ctx->spart.ensure(std::max(rP.items.size() * kSlots, fF.items.size() * kTileF) + 1);
+        ctx->T2.ensure((size_t)nf * ldT);
+        const int *fl = ctx->fired_list.p;
+        launch_project<kStoreR, false, true>(ctx, mt, vP, (int)rP.items.size(), cs, fl, ctx->T.p, ldT, rnarrow);
+        launch_tgemm<true>(ctx, mt, vT, (int)rT.items.size(), cs, fl, ldT, rnarrow);
+        hipLaunchKernelGGL(k_treduce, dim3(nf), dim3(64), 0, st, ctx->d_fits.p, cs.d_fit.p, fl, nf, vT.cmap,
+                           ctx->tpart.p, ldT, ctx->T2.p, nullptr);
+        MH_LAUNCH_CHECK();
+        launch_project<kStoreR, true, false>(ctx, mt, vP, (int)rP.items.size(), cs, fl, ctx->T2.p, ldT, rnarrow);
+        hipLaunchKernelGGL(k_sreduce, dim3((nf + 255) / 256), dim3(256), 0, st, fl, nf, vP.cmap, ctx->spart.p,
+                           kSlots, nullptr, ctx->d_fits.p, cs.d_fit.p, nullptr, ctx->ssum.p, nullptr);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_finalize_cm, dim3(grid_for(fF.items.size())), dim3(256), 0, st, ctx->d_fits.p,
+                           wF.items, (int)fF.items.size(), fl, nullptr, ctx->ssum.p, ctx->spart.p);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_yreduce, dim3((nf + 255) / 256), dim3(256), 0, st, nf, wF.cmap, ctx->spart.p, fl,
+                           ybx, lin);
+        MH_LAUNCH_CHECK();
+        k0 = k1;
+        if (k0 < fired.size()) ctx_sync(ctx);  // the next block reuses the arena and the work lists
+    }
+}
+
+} // namespace
+
 void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, double *ybx,
                  const float *xover, const float *sover, bool rescore, const int *keep)
 {
@@ -466,7 +575,9 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     hipStream_t st = ctx->stream;
     const size_t F = ctx->fits.size();
     index_cands(ctx, cs);
-    ensure_rbuf(ctx, cs);
+    // the plain form's linear scoring stores no residual in the fits' buffers: its DGKS-fired
+    // candidates keep theirs in the bounded retry arena (retry_compact)
+    if (store_r) ensure_rbuf(ctx, cs);
     // normalised columns kept for the hinge sweep or a download stay row-major; residuals that
     // only linear scoring reads are column-major (contiguous per candidate)
     ctx->rbuf_cm = !keep_bxn;
@@ -489,17 +600,6 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     const int mt = (int)std::min<size_t>(16, (mmax + 15) / 16);
     require(mmax <= 256, "the FP64 scoring path supports up to 256 basis functions");
     const int ldT = mt * 16;
-    // a few candidates per fit (the mixed form's rescoring, DGKS retries) run on
-    // 16-wide candidate tiles: the wide tiles would be mostly zero padding in
-    // the FP64 MFMA work that bounds these kernels
-    auto narrow_for = [&](const std::vector<int> &c) {
-        long long tot = 0, act = 0;
-        for (int v : c) {
-            tot += v;
-            act += v > 0;
-        }
-        return tot <= 24 * act;
-    };
     const bool narrow = narrow_for(cnt);
     const int tileT = 16 * (narrow ? 1 : cg_tgemm(mt)), tileP = 16 * (narrow ? 1 : cg_project(mt));
 
@@ -617,7 +717,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         le[f] = (int)live.size();
         te[f] = (int)late.size();
     }
-    if (!fired.empty()) {
+    if (!fired.empty() && store_r) {
         const int nf = (int)fired.size();
         ctx->dgks_host += nf;
         KTimer kt(ctx, K_DGKS, 0.0, 0.0);
@@ -637,7 +737,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         ctx->tpart.ensure(std::max<size_t>(rT.items.size(), 1) * kSlots * ldT);
         ctx->spart.ensure(rP.items.size() * kSlots + 1);
         ctx->T2.ensure((size_t)nf * ldT);
-        if (!store_r || fired_unstored)  // the first residual of the fired columns, for the retry
+        if (fired_unstored)  // the first residual of the fired columns, for the retry
             launch_project<kStoreR, false, true>(ctx, mt, vP, (int)rP.items.size(), cs, ctx->fired_list.p,
                                                  ctx->T.p, ldT, rnarrow);
         launch_tgemm<true>(ctx, mt, vT, (int)rT.items.size(), cs, ctx->fired_list.p, ldT, rnarrow);
@@ -713,9 +813,22 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
             hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, P, uF.cmap, ctx->spart.p,
                                nullptr, ybx, lin);
         } else {
-            // only candidates with s > tol can score above 0; the others are 0 exactly
+            // only candidates with s > tol can score above 0; the others are 0 exactly.  The
+            // fired ones are finished by their retry (retry_compact); the rest recompute r.
             hipLaunchKernelGGL(k_zero2, dim3((P + 255) / 256), dim3(256), 0, st, P, ybx, lin);
             MH_LAUNCH_CHECK();
+            {
+                std::vector<int> keep_live;
+                keep_live.reserve(live.size());
+                for (size_t f = 0; f < F; ++f) {
+                    const int b = (int)keep_live.size();
+                    for (int i = lb[f]; i < le[f]; ++i)
+                        if (!(flags[live[i]] & 1)) keep_live.push_back(live[i]);
+                    lb[f] = b;
+                    le[f] = (int)keep_live.size();
+                }
+                live.swap(keep_live);
+            }
             const int nl = (int)live.size();
             if (nl > 0) {
                 std::vector<int> lcnt(F);
@@ -735,6 +848,11 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                                                      ctx->T.p, ldT, lnarrow);
                 hipLaunchKernelGGL(k_yreduce, dim3((nl + 255) / 256), dim3(256), 0, st, nl, uL.cmap, ctx->spart.p,
                                    ctx->live_list.p, ybx, lin);
+            }
+            MH_LAUNCH_CHECK();
+            if (!fired.empty()) {
+                KTimer kt2(ctx, K_DGKS, 0.0, 0.0);
+                retry_compact(ctx, cs, fired, rb, re, n, mt, ldT, ybx, lin);
             }
         }
         MH_LAUNCH_CHECK();

@@ -57,6 +57,7 @@ struct LaneCand {
     const float *bc;   // B column of the parent (column-major)
     const float *xp;   // X column
     long long col;     // rbuf column
+    int cand;          // candidate index (position in the scored list)
 };
 
 /* Lane setup for item slot `slot`: position I.c0 + slot, candidate list[pos] (or pos itself).
@@ -65,11 +66,12 @@ template <bool RB>
 __device__ inline LaneCand lane_cand(const FitDev &F, const Item &I, int slot, const int *cxc, const int *cpar,
                                      const int *list)
 {
-    LaneCand c{slot < I.nc, 0.0f, F.BC, F.X, 0};
+    LaneCand c{slot < I.nc, 0.0f, F.BC, F.X, 0, 0};
     if (!c.ok) return c;
     const int pos = I.c0 + slot;
     const int cand = list ? list[pos] : pos;
-    c.col = cand - F.c0;
+    c.cand = cand;
+    c.col = F.rbyl ? pos - I.pad : cand - F.c0;
     if (!RB) {
         const int xc = cxc[cand];
         c.sx = F.s[xc];
@@ -293,9 +295,9 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_project_w(const FitDev *_
             acc[g] = 0.0;
             scale[g] = 0.0f;
             fromr[g] = false;
-            store[g] = MODE == kStoreR && c[g].ok && (!keep || keep[c[g].col + F.c0] != 0);
+            store[g] = MODE == kStoreR && c[g].ok && (!keep || keep[c[g].cand] != 0);
             if (MODE == kFinalY && c[g].ok) {
-                const int cand = (int)(c[g].col + F.c0);
+                const int cand = c[g].cand;
                 const double sv = ssum[cand];
                 scale[g] = sv > F.tol ? (float)(1.0 / sqrt(sv + F.tol)) : 0.0f;
                 fromr[g] = (fired[cand] & 1) != 0;

@@ -299,179 +299,89 @@ def gram(A, B=None, chunk_size=128):
     return out
 
 
-def _init_inverse_state(XX, XY, YY, active_idx, ridge=0.0):
-    """
-    (H, beta, SSE) of the active columns from a fresh Cholesky factor
-    (reference mars.py:493-534): H = (XX_a + ridge I)^-1 explicitly (the
-    backward pass downdates it), beta = H XY_a, SSE = max(YY - XY_a.beta, 0).
-    A factorization failure retries once with the diagonal loaded by
-    1e-8 (trace/m + 1).
-    """
-    import scipy.linalg
-
-    m = len(active_idx)
-    if m == 0:
-        return np.zeros((0, 0)), np.zeros(0), float(YY)
-    G = XX[np.ix_(active_idx, active_idx)]
-    b = XY[active_idx]
-    if ridge > 0:
-        G = G + ridge * np.eye(m)
-    try:
-        L = np.linalg.cholesky(G)
-    except np.linalg.LinAlgError:
-        L = np.linalg.cholesky(G + 1e-8 * (np.trace(G) / m + 1.0) * np.eye(m))
-    factor = (L, True)
-    beta = scipy.linalg.cho_solve(factor, b)
-    H = scipy.linalg.cho_solve(factor, np.eye(m))
-    return H, beta, max(float(YY - b @ beta), 0.0)
-
-
-def _backward_eliminate(XX, XY, YY, mask, penalty, n_true, ridge=0.0):
-    """
-    Backward stepwise elimination by rank-1 Schur downdates (reference
-    mars.py:537-604): each step drops the active column with the smallest
-    beta_j^2 / H_jj (the intercept, global column 0, is never dropped), adds
-    that to the SSE and downdates H and beta; every max(64, M/4) steps H is
-    rebuilt from scratch.  Returns the mask of the lowest-GCV subset seen
-    (strictly lower replaces, so the larger subset wins a tie).
-    """
-    M = len(XX)
-    idx = np.flatnonzero(mask)
-    m = len(idx)
-    if m == 0:
-        return mask.copy()
-    H, beta, SSE = _init_inverse_state(XX, XY, YY, idx, ridge)
-
-    def gcv(sse, k):
-        denom = 1.0 - (k + penalty * (k - 1)) / n_true
-        return sse / (denom * denom) if denom > 0 else np.inf
-
-    best = gcv(SSE, m)
-    best_mask = mask.copy()
-    period = max(64, M // 4)
-    steps = 0
-    while m > 0:
-        h = np.diag(H)
-        floor = 1e-12 * max(float(h.max()), 1.0)
-        cost = np.full_like(beta, np.inf)
-        np.divide(beta * beta, h, out=cost, where=h > floor)
-        if idx[0] == 0:
-            cost[0] = np.inf
-        j = int(np.argmin(cost))
-        if not np.isfinite(cost[j]):
-            break
-        SSE += max(float(cost[j]), 0.0)
-        col = H[:, j]
-        v = col / H[j, j]
-        beta = beta - beta[j] * v
-        H = H - np.outer(v, col)
-        H = 0.5 * (H + H.T)
-        np.fill_diagonal(H, np.maximum(np.diag(H), 0.0))
-        keep = np.arange(m) != j
-        beta = beta[keep]
-        H = H[np.ix_(keep, keep)]
-        mask[idx[j]] = False
-        idx = idx[keep]
-        m -= 1
-        steps += 1
-        if steps >= period and m > 0:
-            H, beta, SSE = _init_inverse_state(XX, XY, YY, idx, ridge)
-            steps = 0
-        g = gcv(SSE, m)
-        if g < best:
-            best = g
-            best_mask = mask.copy()
-    return best_mask
-
-
 def prune(B, y, w=None, n_true=None, penalty=3.0, ridge=0.0, mask=None):
     """
-    Linear coefficients by backward stepwise selection with GCV (Friedman
-    1991, section 3.4), as the reference's mars.prune (mars.py:607-698): the
-    intercept (column 0) is protected, columns are standardized to a unit
-    Gram diagonal (so `ridge` acts on that scale), non-finite y rows get
-    weight 0, weights fold in as sqrt(w).  The Gram products run on the
-    device (gram); the M x M elimination runs on the host.
+    Coefficients of the basis by backward stepwise selection under GCV
+    (Friedman 1991, section 3.4), with the reference's interface and
+    selection rule (mars.py:493-698).  Rows whose y is not finite get weight
+    0; weights fold in as sqrt(w); the intercept (column 0) is never removed;
+    columns are standardised to a unit Gram diagonal, where `ridge` applies.
 
-    Returns beta (M,), zero for pruned terms, in B's original scale.
+    The Gram products B^T B, B^T y, y^T y run on the device (gram); the
+    elimination and the final solve run in the library's host code
+    (mars_hip_prune_gram).  Returns beta (M,) in B's scale, 0 for pruned terms.
     """
-    n, M = B.shape
-    if n_true is None:
-        n_true = n
-    ok = np.isfinite(y)
-    if not ok.all():
-        if w is None:
-            w = np.ones(len(y), dtype=B.dtype)
-        w = np.where(ok, w, 0)
-        y = np.where(ok, y, 0)
-    if w is not None:
-        sw = np.sqrt(np.asarray(w, dtype=B.dtype))
-        B = B * sw[:, np.newaxis]
-        y = y * sw
+    B, y = _weighted_rows(B, y, w)
+    M = B.shape[1]
     XX = gram(B)
-    XY = gram(B, y)
-    YY = gram(y, y)
-    d = np.sqrt(np.diag(XX))
-    active = d > 0
-    scale = np.where(active, d, 1.0)
-    XX = XX / scale / scale[:, np.newaxis]
-    XY = XY / scale
-    mask = (np.array(mask) if mask is not None else np.ones(M, dtype="bool")) & active
-    chosen = _backward_eliminate(XX, XY, YY, mask, penalty, n_true, ridge)
-    if ridge > 0:
-        XX = XX + ridge * np.eye(len(XX))
-    beta = np.zeros(M)
-    if chosen.any():
-        beta[chosen] = np.linalg.lstsq(XX[np.ix_(chosen, chosen)], XY[chosen], rcond=None)[0]
-    return beta / scale
+    XY = np.atleast_1d(gram(B, y))
+    YY = float(gram(y, y))
+    keep = None if mask is None else np.asarray(mask, dtype=bool).reshape(M)
+    beta, _ = marslib.prune_gram(XX, XY, YY, keep, float(penalty), float(len(B) if n_true is None else n_true),
+                                 float(ridge))
+    return beta
+
+
+def _weighted_rows(B, y, w):
+    """(B, y) with each row scaled by sqrt(w); rows with a non-finite y weigh 0 (their y is set
+    to 0).  The products keep numpy's promotion: sqrt(w) is taken in B's dtype."""
+    finite = np.isfinite(y)
+    if not finite.all():
+        y = np.where(finite, y, 0)
+        w = np.where(finite, np.ones(len(y), dtype=B.dtype) if w is None else w, 0)
+    if w is None:
+        return B, y
+    root = np.sqrt(np.asarray(w, dtype=B.dtype))
+    return B * root[:, None], y * root
 
 
 def compact(model):
     """
-    Drop the nodes whose `beta` is zero, keeping every ancestor a surviving
-    node needs (kept with its zero beta), and remap the parent indices
+    The model without its zero-`beta` nodes, except those an earlier-kept node
+    needs as an ancestor (they stay, with beta 0); parent indices renumbered
     (reference mars.py:701-738).
     """
-    keep = model["beta"] != 0
-    # a parent always precedes its children: one backward sweep marks every ancestor
-    for i in range(len(model) - 1, 0, -1):
-        if keep[i] and model[i]["type"] != b"i":
-            keep[model[i]["basis"]] = True
-    renum = np.full(len(model), -1, dtype="i4")
-    renum[keep] = np.arange(int(keep.sum()))
-    out = model[keep].copy()
-    out["basis"] = renum[out["basis"]]
+    needed = set(np.flatnonzero(model["beta"] != 0).tolist())
+    stack = list(needed)
+    while stack:  # close the set under "parent of"
+        i = stack.pop()
+        if model[i]["type"] != b"i":
+            p = int(model[i]["basis"])
+            if p not in needed:
+                needed.add(p)
+                stack.append(p)
+    order = sorted(needed)
+    where = {old: new for new, old in enumerate(order)}
+    out = model[np.array(order, dtype=np.intp)].copy() if order else model[:0].copy()
+    for r in range(len(out)):
+        out[r]["basis"] = where.get(int(out[r]["basis"]), -1)
     return out
 
 
 def pprint(model, beta=None, labels=None):
     """
-    Print one line per node: its coefficient and the product of its inputs
-    (reference mars.py:741-776).  Coefficients default to model["beta"].
+    One line per node: its coefficient (model["beta"] unless `beta` is given)
+    and the product of the factors on its path to the intercept, the node's
+    own factor first (reference mars.py:741-776).
     """
-    if beta is None:
-        beta = model["beta"]
-
-    def name(i):
-        return labels[i] if labels is not None else "X[%d]" % i
-
-    def factors(i):
-        row = model[i]
-        if row["type"] == b"+":
-            f = "MAX(%s-%g,0)" % (name(row["input"]), row["hinge"])
-        elif row["type"] == b"-":
-            f = "MAX(%g-%s,0)" % (row["hinge"], name(row["input"]))
-        else:
-            f = name(row["input"])
-        return [f] + factors(row["basis"]) if row["basis"] > 0 else [f]
-
+    coef = model["beta"] if beta is None else beta
+    names = (lambda k: "X[%d]" % k) if labels is None else (lambda k: labels[k])
     for i in range(len(model)):
-        line = "  "
-        if beta is not None and hasattr(beta, "__getitem__"):
-            line += "%+9.4g" % beta[i]
-        if model[i]["type"] != b"i":
-            line += " * " + " * ".join(factors(i))
-        print(line)
+        parts = []
+        node = i
+        while model[node]["type"] != b"i":
+            row = model[node]
+            x, h = names(row["input"]), row["hinge"]
+            parts.append({b"+": "MAX(%s-%g,0)" % (x, h), b"-": "MAX(%g-%s,0)" % (h, x)}.get(row["type"], x))
+            if row["basis"] <= 0:
+                break
+            node = int(row["basis"])
+        text = "  "
+        if coef is not None and hasattr(coef, "__getitem__"):
+            text += "%+9.4g" % coef[i]
+        if parts:
+            text += " * " + " * ".join(parts)
+        print(text)
 
 
 def fit_batch(Xs, ys, ws=None, **kwargs):

@@ -11,7 +11,9 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
 try:
-    from ._marslib import Batch, MarsAlgo, __version__, expand_f32, expand_f64, gram_f32, gram_f64  # noqa: F401
+    from ._marslib import (  # noqa: F401
+        Batch, MarsAlgo, __version__, device_bytes, expand_f32, expand_f64, gram_f32, gram_f64, prune_gram,
+    )
 except ImportError as e:  # pragma: no cover - exercised only on unbuilt trees
     raise ImportError(
         "mars_amd native extension is not built (expected %s and libmars_hip.so next to it); "

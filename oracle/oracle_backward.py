@@ -9,6 +9,9 @@ mars_amd.expand / gram / prune:
                     at 0 and multiply in f64, then round once to X's dtype.
   gram_ref          A^T B with both operands widened to f64 first (exact f32
                     products), summed by numpy's f64 matmul.
+  prune_ref         mars.py:607-698 over given Gram products: standardise to a
+                    unit diagonal, eliminate_brute, then numpy's lstsq on the
+                    kept subset, back in the original scale.
   eliminate_brute   backward elimination of mars.py:537-604 by brute force:
                     every step refits each candidate subset from scratch with
                     lstsq and drops the column whose removal raises the SSE
@@ -92,3 +95,22 @@ def eliminate_brute(XX, XY, YY, mask, penalty, n_true, ridge=0.0):
     out = np.zeros(len(mask), dtype=bool)
     out[best_cols] = True
     return out
+
+
+def prune_ref(XX, XY, YY, mask, penalty, n_true, ridge=0.0):
+    """beta (M,) of mars.prune (mars.py:607-698) from raw Gram products, via eliminate_brute."""
+    XX = np.asarray(XX, dtype=np.float64)
+    XY = np.asarray(XY, dtype=np.float64).reshape(-1)
+    d = np.sqrt(np.diag(XX))
+    on = d > 0
+    s = np.where(on, d, 1.0)
+    S = XX / s / s[:, None]
+    b = XY / s
+    m = (np.ones(len(XX), dtype=bool) if mask is None else np.asarray(mask, dtype=bool)) & on
+    keep = eliminate_brute(S, b, YY, m, penalty, n_true, ridge)
+    if ridge > 0:
+        S = S + ridge * np.eye(len(S))
+    beta = np.zeros(len(XX))
+    if keep.any():
+        beta[keep] = np.linalg.lstsq(S[np.ix_(keep, keep)], b[keep], rcond=None)[0]
+    return beta / s, keep

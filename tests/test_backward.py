@@ -7,6 +7,7 @@ brute-force path (every candidate subset refit from scratch).  No device work.
 import numpy as np
 import pytest
 
+from mars_amd import marslib
 from mars_amd import mars as M
 from mars_amd.mars import MODEL_DTYPE
 from tests import abi
@@ -111,7 +112,7 @@ def test_eliminate_matches_brute_force(seed, n, m, ridge):
     coef[: max(3, m // 3)] = rng.uniform(0.5, 2.0, max(3, m // 3)) * rng.choice([-1, 1], max(3, m // 3))
     y = B @ coef + 0.5 * rng.randn(n)
     XX, XY, YY, active = _standardized(B, y)
-    got = M._backward_eliminate(XX, XY, YY, active.copy(), 3.0, n, ridge)
+    _, got = marslib.prune_gram(XX, XY, YY, active.copy(), 3.0, n, ridge)
     want = eliminate_brute(XX, XY, YY, active.copy(), 3.0, n, ridge)
     np.testing.assert_array_equal(got, want)
     assert got[0]
@@ -127,7 +128,7 @@ def test_eliminate_keeps_intercept_and_mask():
     XX, XY, YY, active = _standardized(B, y)
     mask = active.copy()
     mask[3] = False
-    got = M._backward_eliminate(XX, XY, YY, mask.copy(), 3.0, n)
+    _, got = marslib.prune_gram(XX, XY, YY, mask.copy(), 3.0, n)
     assert got[0] and not got[3]
     np.testing.assert_array_equal(got, eliminate_brute(XX, XY, YY, mask.copy(), 3.0, n))
 
@@ -143,3 +144,50 @@ def test_oracle_finds_true_subset():
     XX, XY, YY, active = _standardized(B, y)
     got = eliminate_brute(XX, XY, YY, active.copy(), 3.0, n)
     np.testing.assert_array_equal(got, [True, False, True, False, False, False])
+
+
+# ---- the library's prune (mars_hip_prune_gram, host C++) against the oracle's prune
+
+@pytest.mark.parametrize("seed,n,m,ridge,weights", [(11, 150, 6, 0.0, False), (12, 300, 12, 0.05, True),
+                                                    (13, 90, 9, 0.0, True)])
+def test_prune_gram_matches_oracle(seed, n, m, ridge, weights):
+    from oracle_backward import gram_ref, prune_ref
+
+    rng = np.random.RandomState(seed)
+    B = np.column_stack([np.ones(n), rng.randn(n, m - 1)])
+    B[:, 3] = 0.0                                   # zero-norm column: never active
+    y = 1.5 * B[:, 1] - 0.7 * B[:, 2] + 0.3 * rng.randn(n)
+    if weights:
+        sw = np.sqrt(rng.uniform(0.5, 2.0, n))
+        B, y = B * sw[:, None], y * sw
+    XX, XY, YY = gram_ref(B), gram_ref(B, y)[:, 0], float(gram_ref(y, y)[0, 0])
+    beta, keep = marslib.prune_gram(XX, XY, YY, None, 3.0, n, ridge)
+    want, wkeep = prune_ref(XX, XY, YY, None, 3.0, n, ridge)
+    np.testing.assert_array_equal(keep, wkeep)
+    np.testing.assert_allclose(beta, want, rtol=1e-10, atol=1e-12)
+    assert beta[3] == 0.0 and keep[0]
+
+
+def test_prune_gram_min_norm_on_duplicate_columns():
+    """Two identical columns cannot both be resolved: the final solve returns numpy lstsq's
+    minimum-norm split (equal halves), as an SVD-based least squares does."""
+    rng = np.random.RandomState(5)
+    n = 200
+    x = rng.randn(n)
+    B = np.column_stack([np.ones(n), x, x])
+    y = 2.0 * x + 0.01 * rng.randn(n)
+    XX, XY, YY = B.T @ B, B.T @ y, float(y @ y)
+    keep = np.ones(3, dtype=bool)
+    d = np.sqrt(np.diag(XX))
+    S, b = XX / d / d[:, None], XY / d
+    want = np.linalg.lstsq(S, b, rcond=None)[0] / d
+    # force both duplicates to stay: GCV with a zero penalty and n_true huge keeps everything
+    beta, chosen = marslib.prune_gram(XX, XY, YY, keep, 0.0, 1e12, 0.0)
+    if chosen.all():
+        np.testing.assert_allclose(beta, want, rtol=1e-9, atol=1e-12)
+        assert abs(beta[1] - beta[2]) <= 1e-9 * abs(beta[1])
+
+
+def test_prune_gram_errors():
+    with pytest.raises(RuntimeError, match="n_true"):
+        marslib.prune_gram(np.eye(2), np.ones(2), 1.0, None, 3.0, 0.0, 0.0)

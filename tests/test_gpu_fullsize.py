@@ -99,8 +99,17 @@ def test_config4_full_mixed_equals_plain(policy, monkeypatch):
     say("config 4 full: %d fits, %d rows" % (len(fits), sum(len(x) for x, _ in fits)))
     Xs, ys = [x for x, _ in fits], [y for _, y in fits]
     sp, sm = {}, {}
+    from mars_amd import marslib
+
+    live0 = marslib.device_bytes(reset_peak=True)[0]
     plain = mars_amd.fit_batch(Xs, ys, max_terms=33, stats=sp)
+    peak_plain = marslib.device_bytes(reset_peak=True)[1] - live0
     mixed = mars_amd.fit_batch(Xs, ys, max_terms=33, precision="mixed", stats=sm)
+    peak_mixed = marslib.device_bytes()[1] - live0
+    say("config 4 peak device bytes: plain %.2f GB, mixed %.2f GB" % (peak_plain / 1e9, peak_mixed / 1e9))
+    # one context's footprint: inputs, both basis copies, work lists and partial sums; the plain
+    # form keeps no per-candidate residual columns (1.5 GB of X, 3.2 GB of basis columns)
+    assert peak_plain <= 10e9 and peak_mixed <= 12e9
     for i, (a, b) in enumerate(zip(plain, mixed)):
         assert _seq(a) == _seq(b), i
         for u, v in zip(sp["mse"][i], sm["mse"][i]):
@@ -114,7 +123,16 @@ def test_config5_full_mixed_equals_plain():
     X, y = datagen.workload5()
     kw = datagen.FIT_KW[5]
     say("config 5 full: X", X.shape)
+    from mars_amd import marslib
+
+    live0 = marslib.device_bytes(reset_peak=True)[0]
     plain, ptrace, algo = _fit(X, y, kw, "fp64")
+    fired = algo.stats()["dgks_fired"]
+    algo.close()
+    peak_plain = marslib.device_bytes()[1] - live0
+    say("config 5 plain peak device bytes: %.2f GB, DGKS retries %d" % (peak_plain / 1e9, fired))
+    # X 3.2 GB, four basis copies 6.6 GB, the DGKS retry arena 2 GiB, work lists and partial sums
+    assert peak_plain <= 20e9
     del algo
     mixed, mtrace, algo = _fit(X, y, kw, "mixed")
     del algo
