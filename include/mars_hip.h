@@ -131,6 +131,20 @@ int mars_hip_fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t mask_
  * column is numerically degenerate (the basis is then unchanged). */
 int mars_hip_fit_append(mars_hip_ctx *ctx, char type, int xcol, int bcol, float h, double *mse);
 
+/* mars_hip_fit_append without allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device or pinned memory during a call
+ * (code under stream capture, serving loops, harnesses that forbid it).  Same
+ * arguments, checks, result and basis as mars_hip_fit_append (the same
+ * kernels), on a single-fit context from mars_hip_fit_create.  Its scratch is
+ * one fixed device array of 4 MiB per GPU, held until the process ends: it
+ * supports the append's limits (basis columns < 512) when
+ * 16 * ldb * min(256, ceil(n / 4096)) + 12288 <= 4 MiB, ldb = max_terms
+ * rounded up to 4; larger fits get MARS_HIP_EINVAL.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (several host threads or
+ * contexts at once) race and must be serialised by the caller.  Synchronous:
+ * returns after the result has been copied to *mse. */
+int mars_hip_fit_append_noalloc(mars_hip_ctx *ctx, char type, int xcol, int bcol, float h, double *mse);
+
 /* Copy the live basis out: B and Bo as (n, m) row-major f32 (either may be NULL). */
 int mars_hip_fit_basis(mars_hip_ctx *ctx, float *B, float *Bo, size_t *m_out);
 

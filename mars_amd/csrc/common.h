@@ -274,6 +274,7 @@ struct mars_hip_ctx {
                                      // so its residual is stored for the final pass
     bool rbuf_cm = false;            // candidate columns in rbuf are column-major (linear scoring)
     mh::DBuf<float> rarena;          // plain form's DGKS retry: compact residual columns (bounded)
+    char *scratch = nullptr;         // device address of this device's no-alloc append scratch
     int rsel_hint = 0;               // the previous rescoring list's length
     mh::DBuf<mh::FitDev> d_fits;
     mh::DBuf<mh::Item> d_items;

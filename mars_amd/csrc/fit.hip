@@ -5,6 +5,7 @@
 
 #include <atomic>
 #include <algorithm>
+#include <mutex>
 #include <functional>
 #include <cmath>
 #include <cstring>
@@ -561,6 +562,7 @@ void ctx_base_init(mars_hip_ctx *ctx, size_t max_terms)
     ctx->dgks.ensure(1);
     ctx->fired_count.ensure(2);
     MH_CHECK(hipMemsetAsync(ctx->dgks.p, 0, sizeof(long long), ctx->stream));
+    ctx->scratch = scratch_ptr();  // this device's no-alloc scratch (resolved once per device)
 }
 
 void fit_state_alloc(mars_hip_ctx *ctx, FitState &fs, size_t n, double tol)
@@ -977,6 +979,165 @@ void fit_append_batch(mars_hip_ctx *ctx, std::vector<AppendArg> &args, std::vect
         } else {
             out[f].mse = -1.0;
         }
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Append without allocation (mars_hip_fit_append_noalloc)                  */
+/* ---------------------------------------------------------------------- */
+
+/* The append's scratch for callers that cannot allocate: per-launch state (the fit's device
+ * view, column pointers, argument, result, DGKS gate, scalars) and the partial sums of the
+ * append's reductions.  Each GPU holds its own copy for the life of the process.  Host code
+ * never uses this name as an address: it uses the per-device pointer that hipGetSymbolAddress
+ * returns (scratch_ptr). */
+constexpr size_t kNoAllocBytes = size_t(4) << 20;
+static __device__ __align__(256) double scratch[kNoAllocBytes / sizeof(double)];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<char *> g_scratch_dev;  // per device: the device address of `scratch` (null: not resolved)
+
+/* Byte offsets of the no-alloc append's pieces inside `scratch` (single fit). */
+constexpr size_t kNaFit = 0, kNaFitW = 256, kNaArg = 512, kNaOut = 768, kNaGate = 1024, kNaScal = 1280,
+                 kNaRed = 2048, kNaCoef = 8192, kNaDots = 12288;
+static_assert(sizeof(FitDev) <= kNaFitW - kNaFit && sizeof(FitW) <= kNaArg - kNaFitW &&
+                  sizeof(AppendArg) <= kNaOut - kNaArg && sizeof(AppendOut) <= kNaGate - kNaOut &&
+                  S_COUNT * sizeof(double) <= kNaRed - kNaScal && 3 * 256 * sizeof(double) <= kNaCoef - kNaRed &&
+                  kMaxAppendCols * sizeof(double) <= kNaDots - kNaCoef,
+              "no-alloc scratch layout");
+
+__global__ void k_noalloc_args(FitDev fd, FitW fw, AppendArg a, FitDev *__restrict__ dfd, FitW *__restrict__ dfw,
+                               AppendArg *__restrict__ darg)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *dfd = fd;
+        *dfw = fw;
+        *darg = a;
+    }
+}
+
+} // namespace
+
+/* Resolve the current device's address of `scratch` (once per device; the device must be
+ * current).  Called when a context is created. */
+char *scratch_ptr()
+{
+    int dev = 0;
+    MH_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_dev.size() <= dev) g_scratch_dev.resize(dev + 1, nullptr);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        MH_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = static_cast<char *>(p);
+    }
+    return g_scratch_dev[dev];
+}
+
+/* The append of fit_append_batch for a single-fit context, with every scratch buffer inside
+ * `scratch` and the arguments passed by value: no device or pinned allocation, no staged upload.
+ * The result comes back by a copy to the caller's memory and a stream synchronisation. */
+void fit_append_noalloc(mars_hip_ctx *ctx, const AppendArg &a, double *mse)
+{
+    require(ctx->resident && ctx->fits.size() == 1, "mars_hip_fit_append_noalloc needs a single-fit resident context");
+    FitState &fs = ctx->fits[0];
+    if (fs.m >= ctx->max_terms) throw Error(MARS_HIP_EFULL, "basis matrix is full");
+    if (a.bcol < 0 || (size_t)a.bcol >= fs.m) {
+        char msg[80];
+        snprintf(msg, sizeof(msg), "invalid basis column number: %d", a.bcol);
+        throw Error(MARS_HIP_EINVAL, msg);
+    }
+    require(a.xcol >= 0 && (size_t)a.xcol < fs.px, "invalid X column index");
+    require(fs.m < (size_t)kMaxAppendCols, "the device append supports up to 512 basis functions");
+    const int nb = (int)std::min<size_t>(256, std::max<size_t>(1, (fs.n + 4095) / 4096));  // as fit_append_batch
+    const size_t need = kNaDots + 2 * ctx->ldb * (size_t)nb * sizeof(double);
+    if (need > kNoAllocBytes) {
+        char msg[160];
+        snprintf(msg, sizeof(msg),
+                 "mars_hip_fit_append_noalloc: needs %zu scratch bytes for max_terms %zu and %zu rows, above its "
+                 "%zu-byte maximum",
+                 need, ctx->max_terms, fs.n, kNoAllocBytes);
+        throw Error(MARS_HIP_EINVAL, msg);
+    }
+    char *base = ctx->scratch;
+    require(base != nullptr, "no-alloc scratch was not resolved for this context's device");
+    FitDev *dfit = reinterpret_cast<FitDev *>(base + kNaFit);
+    FitW *dfw = reinterpret_cast<FitW *>(base + kNaFitW);
+    AppendArg *darg = reinterpret_cast<AppendArg *>(base + kNaArg);
+    AppendOut *dout = reinterpret_cast<AppendOut *>(base + kNaOut);
+    int *gate = reinterpret_cast<int *>(base + kNaGate);
+    double *scal = reinterpret_cast<double *>(base + kNaScal);
+    double *pa = reinterpret_cast<double *>(base + kNaRed), *pb = pa + nb, *pc = pb + nb;
+    double *coef = reinterpret_cast<double *>(base + kNaCoef);
+    double *adot = reinterpret_cast<double *>(base + kNaDots);
+    double *agram = adot + ctx->ldb * (size_t)nb;
+
+    FitDev d{};
+    d.X = fs.X.p;
+    d.s = fs.s.p;
+    d.B = fs.B.p;
+    d.Bo = fs.Bo.p;
+    d.BoC = fs.BoC.p;
+    d.BC = fs.BC.p;
+    d.y = fs.y.p;
+    d.ybo = fs.ybo.p;
+    d.rbuf = fs.rbuf.p;
+    d.v = fs.v.p;
+    d.tc = fs.tc.p;
+    d.tcn = fs.tcn.p;
+    d.n = (long long)fs.n;
+    d.ldx = (long long)fs.ldx;
+    d.ldb = (long long)ctx->ldb;
+    d.rsr = 1;
+    d.rsc = (long long)fs.ldx;
+    d.m = (int)fs.m;
+    d.tol = fs.tol;
+    const FitW w{fs.B.p, fs.Bo.p, fs.BoC.p, fs.BC.p, fs.ybo.p, fs.G.p};
+    hipStream_t st = ctx->stream;
+    // every scratch entry a kernel reads is written earlier in this same sequence, so the contents
+    // a previous call left behind are never read
+    hipLaunchKernelGGL(k_noalloc_args, dim3(1), dim3(64), 0, st, d, w, a, dfit, dfw, darg);
+    MH_LAUNCH_CHECK();
+    const dim3 grid(nb, 1);
+    hipLaunchKernelGGL(k_app_build, grid, dim3(256), 0, st, (const FitDev *)dfit, (const AppendArg *)darg, nb, pa,
+                       scal);
+    MH_LAUNCH_CHECK();
+    for (int pass = 0; pass < 2; ++pass) {
+        const int first = pass == 0 ? 1 : 0;
+        const int *gp = pass == 0 ? nullptr : gate;
+        const unsigned kg = (unsigned)((fs.m + kAppG - 1) / kAppG);
+        hipLaunchKernelGGL(k_app_dots, dim3(nb, 1, kg), dim3(256), 0, st, (const FitDev *)dfit,
+                           (const AppendArg *)darg, nb, (const double *)pa, first, gp, adot, agram,
+                           (const FitW *)dfw);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_app_solve, dim3(1), dim3(256), 0, st, (const FitDev *)dfit, (const AppendArg *)darg, nb,
+                           first, gp, (const double *)adot, (const double *)agram, (const FitW *)dfw, coef, scal);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_app_sub, grid, dim3(256), 0, st, (const FitDev *)dfit, (const AppendArg *)darg, nb, gp,
+                           (const double *)coef, pb);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_app_gate, dim3(1), dim3(256), 0, st, (const FitDev *)dfit, (const AppendArg *)darg, nb,
+                           (const double *)pb, (const double *)pb, gate, scal, ctx->dgks.p, pass);
+        MH_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_app_store, grid, dim3(256), 0, st, (const FitDev *)dfit, (const AppendArg *)darg, nb,
+                       (const FitW *)dfw, (const double *)scal, pc);
+    MH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_app_final, dim3(1), dim3(256), 0, st, (const FitDev *)dfit, (const AppendArg *)darg, nb,
+                       (const double *)pc, (const double *)scal, (const FitW *)dfw, (const int *)gate, dout);
+    MH_LAUNCH_CHECK();
+    AppendOut o{};
+    MH_CHECK(hipMemcpyAsync(&o, dout, sizeof(AppendOut), hipMemcpyDeviceToHost, st));
+    MH_CHECK(hipStreamSynchronize(st));
+    if (o.fired) ctx->dgks_host += 1;
+    if (o.status == 1) {
+        fs.ybo_h.push_back(o.ybo_m);
+        fs.m += 1;
+        *mse = o.mse;
+    } else {
+        *mse = -1.0;
     }
 }
 
@@ -1410,6 +1571,19 @@ int mars_hip_fit_append(mars_hip_ctx *ctx, char type, int xcol, int bcol, float 
         std::vector<AppendArg> args{a};
         fit_append_batch(ctx, args, o);
         *mse = o[0].mse;
+    });
+}
+
+int mars_hip_fit_append_noalloc(mars_hip_ctx *ctx, char type, int xcol, int bcol, float h, double *mse)
+{
+    return guarded([&] {
+        require(ctx != nullptr && mse != nullptr, "null argument");
+        AppendArg a{0, xcol, bcol, 1, h, 0};
+        if (type == 'l') a.type = 0;
+        else if (type == '+') a.type = 1;
+        else if (type == '-') a.type = 2;
+        else throw Error(MARS_HIP_EINVAL, "invalid basis type");
+        fit_append_noalloc(ctx, a, mse);
     });
 }
 

@@ -89,6 +89,8 @@ void fit_append_batch(mars_hip_ctx *ctx, std::vector<AppendArg> &args, std::vect
  * on the stream: ctx->bout receives it, ctx->ival holds the fits' candidate offsets and counts. */
 void argmax_launch(mars_hip_ctx *ctx, const Cands &cs, const double *lin);
 double host_dsse(const FitState &fs);
+/* Device address of the no-alloc append's scratch on the current device (resolved once). */
+char *scratch_ptr();
 extern thread_local std::string g_last_error;
 
 /* Asynchronous host-to-device copy on the context's stream.  Small copies go through pinned
