@@ -201,6 +201,14 @@ int mars_hip_batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts *opts, mars_
  * bulk pass (both 0 for the plain form).  Any pointer may be NULL. */
 int mars_hip_score_stats(mars_hip_ctx *ctx, long *scored, long *rescored, long *gated);
 
+/* Checks of the mixed form since creation: window_violations counts candidates
+ * left unrescored whose score interval reached their fit's best exact score (0
+ * while the interval bounds hold: then every step's winner is an FP64 value);
+ * host_redo_steps counts batch steps whose device-driven rescoring ran out of
+ * residual room and were scored again with host-built lists.  No counterpart in
+ * the reference (its GPU path has no mixed form).  Either pointer may be NULL. */
+int mars_hip_check_stats(mars_hip_ctx *ctx, long *window_violations, long *host_redo_steps);
+
 /* Per-kernel-class timing with HIP events on the context's stream (off by
  * default).  Each record: launches, summed device milliseconds and the
  * summed algorithmic flops / HBM bytes of those launches. */

@@ -18,15 +18,19 @@ namespace {
 
 /* Append arguments from the device argmax: fit f appends the linear term of its best candidate
  * (X column, parent) when it had candidates and that candidate's score is above 0 (mars.py:345-346);
- * otherwise its argument is inactive.  One thread per fit. */
+ * otherwise its argument is inactive.  When the step's rescoring must be redone (*redo), every
+ * argument is inactive and marked (pad = 1): nothing is appended.  One thread per fit. */
 __global__ void k_batch_args(int F, const int *__restrict__ c0, const int *__restrict__ cnt,
                              const BestOut *__restrict__ best, const int *__restrict__ cxc,
-                             const int *__restrict__ cpar, AppendArg *__restrict__ args)
+                             const int *__restrict__ cpar, const int *__restrict__ redo,
+                             AppendArg *__restrict__ args)
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F) return;
     AppendArg a{0, 0, 0, 0, 0.0f, 0};
-    if (cnt[f] > 0 && best[f].v1 > 0.0) {
+    if (redo && *redo) {
+        a.pad = 1;
+    } else if (cnt[f] > 0 && best[f].v1 > 0.0) {
         const int g = c0[f] + best[f].c1;
         a.xcol = cxc[g];
         a.bcol = cpar[g];
@@ -66,6 +70,7 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
     int step = 0;
     std::vector<AppendArg> args(F);
     std::vector<BestOut> best;
+    bool device_lists = true;  // the mixed form's rescoring lists are built on the device
     while (true) {
         HostTimer step_timer(ctx, K_H_STEP);
         // the mixed form's cache update runs on the device while the host builds the candidates
@@ -108,7 +113,7 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
         if (P) {
             HostTimer t(ctx, K_H_SCORE);
             if (ctx->form == MARS_HIP_FORM_MIXED)
-                mixed_score(ctx);
+                mixed_score(ctx, !device_lists);
             else
                 plain_score(ctx, cs, false, ctx->lin.p, ctx->ybx.p);
             // no synchronisation here: the argmax that follows on the same stream returns the result
@@ -122,7 +127,7 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
             hipLaunchKernelGGL(k_batch_args, dim3((unsigned)((F + 255) / 256)), dim3(256), 0, ctx->stream, (int)F,
                                (const int *)ctx->ival.p, (const int *)ctx->ival.p + F,
                                (const BestOut *)ctx->bout.p, (const int *)cs.d_xc.p, (const int *)cs.d_par.p,
-                               ctx->aarg.p);
+                               device_lists ? (const int *)ctx->dredo.p : nullptr, ctx->aarg.p);
             MH_LAUNCH_CHECK();
         }
         std::vector<double> sse0(F, 0.0), m0(F, 0.0);
@@ -160,6 +165,15 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
                 if (fs.mc > fs.m) fs.mc = fs.m;
             }
         }
+        if (F > 0 && args[0].pad == 1) {
+            // the device-driven rescoring ran out of residual room: nothing was appended; score
+            // this step again with host-built lists
+            ctx->redo_steps += 1;
+            --step;
+            device_lists = false;
+            continue;
+        }
+        device_lists = true;
         for (size_t f = 0; f < F; ++f) {
             FitDriver &d = D[f];
             if (!d.active) continue;

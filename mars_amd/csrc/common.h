@@ -121,9 +121,10 @@ struct FitDev {
     double *tcn;       // mixed form: cached |Bx|^2, Bx.y per (xc, par)
     long long n, ldx, ldb, rsr, rsc;
     int m, c0;         // live basis count, first candidate of this fit
-    int rbyl;          // 1: a list pass addresses rbuf columns by position in the fit's part of the
-                       // list (compact residual storage), 0: by candidate (column cand - c0)
+    int rbyl;          // how a candidate's rbuf column is found: 0 column cand - c0, 1 (list passes)
+                       // position in the fit's part of the list, 2 float offset roff[cand]
     int pad_;
+    const long long *roff;  // rbyl == 2: per candidate, float offset of its column in rbuf (-1: none)
     double tol;
 };
 
@@ -208,6 +209,8 @@ struct BestOut {
     int knot2;
 };
 
+constexpr int kDStat = 8;  // device counters per context (mars_hip_ctx::dstat)
+
 /* Kernel classes timed by mars_hip_timing. */
 enum KClass { K_TGEMM = 0, K_PROJECT, K_FINALIZE, K_DGKS, K_SWEEP, K_SORT, K_APPEND, K_MIXED_A,
               K_MIXED_B, K_RECHECK, K_ARGMAX,
@@ -273,7 +276,17 @@ struct mars_hip_ctx {
     mh::DBuf<int> rkeep;             // 1: the candidate was ungated in the cache approximation,
                                      // so its residual is stored for the final pass
     bool rbuf_cm = false;            // candidate columns in rbuf are column-major (linear scoring)
-    mh::DBuf<float> rarena;          // plain form's DGKS retry: compact residual columns (bounded)
+    mh::DBuf<float> rarena;          // residual columns of the DGKS retry / device rescoring (bounded)
+    mh::DBuf<long long> roff;        // device rescoring: per list entry, its column's offset in rarena
+    mh::DBuf<mh::FitDev> d_fits2;    // device rescoring: fit views with arena-addressed residuals
+    mh::DBuf<int> dl_int;            // device rescoring: per-fit ranges, counts, compaction tiles
+    mh::DBuf<unsigned long long> dl_u64;  // device rescoring: arena bump counter
+    mh::DBuf<int> cgate, cdone;      // mixed form, per candidate: bulk gate tripped; rescored exactly
+    mh::DBuf<double> clo, chi;       // mixed form, per candidate: interval of the plain form's score
+    mh::DBuf<unsigned long long> dstat;  // device counters (kDStat): [1] rescored, [2] gated,
+                                         // [3] window violations, [4] DGKS retries of the rescoring
+    long redo_steps = 0;             // batch steps scored again on the host-driven path
+    mh::DBuf<int> dredo;             // 1: the step's device-driven rescoring must be redone on the host
     char *scratch = nullptr;         // device address of this device's no-alloc append scratch
     int rsel_hint = 0;               // the previous rescoring list's length
     mh::DBuf<mh::FitDev> d_fits;

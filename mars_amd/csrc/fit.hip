@@ -562,7 +562,20 @@ void ctx_base_init(mars_hip_ctx *ctx, size_t max_terms)
     ctx->dgks.ensure(1);
     ctx->fired_count.ensure(2);
     MH_CHECK(hipMemsetAsync(ctx->dgks.p, 0, sizeof(long long), ctx->stream));
+    ctx->dstat.ensure(kDStat);
+    MH_CHECK(hipMemsetAsync(ctx->dstat.p, 0, kDStat * sizeof(unsigned long long), ctx->stream));
+    ctx->dredo.ensure(1);
+    MH_CHECK(hipMemsetAsync(ctx->dredo.p, 0, sizeof(int), ctx->stream));
     ctx->scratch = scratch_ptr();  // this device's no-alloc scratch (resolved once per device)
+}
+
+void read_dstat(mars_hip_ctx *ctx, unsigned long long *out)
+{
+    std::fill(out, out + kDStat, 0ull);
+    if (!ctx->dstat.p) return;
+    MH_CHECK(hipMemcpyAsync(out, ctx->dstat.p, kDStat * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    MH_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
 void fit_state_alloc(mars_hip_ctx *ctx, FitState &fs, size_t n, double tol)
@@ -1217,7 +1230,7 @@ void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t 
         ctx->lin.ensure(P);
         ctx->ybx.ensure(P);
         if (ctx->form == MARS_HIP_FORM_MIXED) {
-            mixed_score(ctx);
+            mixed_score(ctx, true);  // one fit: the eval synchronises anyway
         } else {
             require(linear_only || ctx->row_major, "hinge sweeps need the row-major basis copies");
             plain_score(ctx, cs, !linear_only, ctx->lin.p, ctx->ybx.p);
@@ -1646,9 +1659,22 @@ int mars_hip_score_stats(mars_hip_ctx *ctx, long *scored, long *rescored, long *
 {
     return guarded([&] {
         require(ctx != nullptr, "null context");
+        unsigned long long d[kDStat];
+        read_dstat(ctx, d);
         if (scored) *scored = ctx->scored;
-        if (rescored) *rescored = ctx->rescored;
-        if (gated) *gated = ctx->gated;
+        if (rescored) *rescored = ctx->rescored + (long)d[1];
+        if (gated) *gated = ctx->gated + (long)d[2];
+    });
+}
+
+int mars_hip_check_stats(mars_hip_ctx *ctx, long *window_violations, long *host_redo_steps)
+{
+    return guarded([&] {
+        require(ctx != nullptr, "null context");
+        unsigned long long d[kDStat];
+        read_dstat(ctx, d);
+        if (window_violations) *window_violations = (long)d[3];
+        if (host_redo_steps) *host_redo_steps = ctx->redo_steps;
     });
 }
 
@@ -1656,7 +1682,9 @@ int mars_hip_stats(mars_hip_ctx *ctx, long *dgks_fired, double *eval_ms)
 {
     return guarded([&] {
         require(ctx != nullptr, "null context");
-        if (dgks_fired) *dgks_fired = ctx->dgks_host;
+        unsigned long long d[kDStat];
+        read_dstat(ctx, d);
+        if (dgks_fired) *dgks_fired = ctx->dgks_host + (long)d[4];
         if (eval_ms) *eval_ms = ctx->eval_ms;
     });
 }

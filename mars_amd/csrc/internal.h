@@ -34,11 +34,18 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                  const float *xover = nullptr, const float *sover = nullptr, bool rescore = false,
                  const int *keep = nullptr);
 
+/* Exact FP64 rescoring of cs[list[i]], i < *dcount (device-built lists and work items, no host
+ * synchronisation); lin[list[i]] = exact score, done[list[i]] = 1 (done may be null);
+ * keep[i]: store entry i's first residual; *redo = 1 when a fired entry found no arena room
+ * (the step must then be scored again on the host-driven path). */
+void plain_rescore_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, const int *dcount, int bound,
+                          int *keep, double *lin, int *done, int *redo);
+
 /* Mixed-precision scoring of ctx->cand (linear delta-SSE only) from each
  * fit's incrementally maintained T cache (FP32 MFMA, FP64 across row chunks),
  * then an exact FP64 rescoring of the DGKS-gated and near-winning
  * candidates.  Writes ctx->lin. */
-void mixed_score(mars_hip_ctx *ctx);
+void mixed_score(mars_hip_ctx *ctx, bool host_lists = false);
 /* Launch the mixed form's cache update for every fit's newest basis column; it needs no candidate
  * list, so a caller can build the list while the update runs (mixed_score then finds it done). */
 void mixed_prepare(mars_hip_ctx *ctx);
@@ -89,6 +96,8 @@ void fit_append_batch(mars_hip_ctx *ctx, std::vector<AppendArg> &args, std::vect
  * on the stream: ctx->bout receives it, ctx->ival holds the fits' candidate offsets and counts. */
 void argmax_launch(mars_hip_ctx *ctx, const Cands &cs, const double *lin);
 double host_dsse(const FitState &fs);
+/* Copy the context's device counters (kDStat entries) to `out` (synchronises the stream). */
+void read_dstat(mars_hip_ctx *ctx, unsigned long long *out);
 /* Device address of the no-alloc append's scratch on the current device (resolved once). */
 char *scratch_ptr();
 extern thread_local std::string g_last_error;

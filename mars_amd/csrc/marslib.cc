@@ -164,14 +164,17 @@ public:
         long fired = 0;
         double ms = 0;
         check(mars_hip_stats(ctx_, &fired, &ms));
-        long sc = 0, re = 0, ga = 0;
+        long sc = 0, re = 0, ga = 0, wv = 0, rd = 0;
         check(mars_hip_score_stats(ctx_, &sc, &re, &ga));
+        check(mars_hip_check_stats(ctx_, &wv, &rd));
         py::dict d;
         d["dgks_fired"] = fired;
         d["eval_ms"] = ms;
         d["scored"] = sc;
         d["rescored"] = re;
         d["gated"] = ga;
+        d["window_violations"] = wv;
+        d["host_redo_steps"] = rd;
         return d;
     }
 
@@ -304,14 +307,17 @@ public:
         long fired = 0;
         double ms = 0;
         check(mars_hip_stats(ctx_, &fired, &ms));
-        long sc = 0, re = 0, ga = 0;
+        long sc = 0, re = 0, ga = 0, wv = 0, rd = 0;
         check(mars_hip_score_stats(ctx_, &sc, &re, &ga));
+        check(mars_hip_check_stats(ctx_, &wv, &rd));
         py::dict d;
         d["dgks_fired"] = fired;
         d["eval_ms"] = ms;
         d["scored"] = sc;
         d["rescored"] = re;
         d["gated"] = ga;
+        d["window_violations"] = wv;
+        d["host_redo_steps"] = rd;
         return d;
     }
 

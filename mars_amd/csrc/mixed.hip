@@ -36,7 +36,6 @@ namespace {
 
 constexpr int kChunk = 32;         // rows per FP32 accumulation chunk (folded into FP64)
 constexpr int kXT = 64;            // X columns per update item (4 groups of 16)
-constexpr double kWindow = 1e-4;   // relative near-tie window rescored in FP64
 constexpr int kGroupInts = 7;      // fit, xc0, nc, j, a-tile, first list entry, item count
 
 /*
@@ -266,29 +265,48 @@ __global__ __launch_bounds__(256) void k_tc_reduce(const FitDev *__restrict__ fi
     }
 }
 
-/* Approximate score and gate flag per candidate from the cache; 16 lanes per candidate. */
+/*
+ * Approximate score, gate flag and an interval [lo, hi] that contains the plain form's score,
+ * per candidate, from the cache; 16 lanes per candidate.
+ *
+ * Interval (u = 2^-24, the f32 unit roundoff).  The cached T differs from the plain form's FP64
+ * T = Bo^T Bx by at most gamma sum_i |Bo_ik| |Bx_i| <= gamma |Bx| per entry (Cauchy-Schwarz,
+ * unit Bo columns), gamma = 40 u: the rounded products f32(Bo B) against Bo f32(B x) (2u) and
+ * at most 32 FP32 accumulations per row chunk (32u), with margin.  So with |dT| <= sqrt(m) gamma |Bx|:
+ *     |d(r.y)|  <= |dT| |ybo|,
+ *     |d|r|^2|  <= 2 |T| |dT| + |dT|^2 + m 8u |T|^2
+ * (the last term bounds T^T (Bo^T Bo - I) T, the orthogonality loss of the stored f32 columns,
+ * which the closed form |r|^2 = |Bx|^2 - |T|^2 omits).  That bounds the real
+ * |r.y| / sqrt(|r|^2 + tol); the plain form rounds r, the scale and each normalised entry to
+ * f32, which moves its ybx by at most 6u (|r.y| / |r| <= |y| = 1), taken as 8u.  The plain
+ * score ybx^2 then lies in [lo, hi].  A candidate whose hi is below the largest lo of its fit's
+ * ungated candidates cannot be the step's winner and keeps its approximate score.
+ */
 __global__ __launch_bounds__(256) void k_tc_score(const FitDev *__restrict__ fits, const int *__restrict__ cfit,
                                                   const int *__restrict__ cxc, const int *__restrict__ cpar,
-                                                  int count, double *__restrict__ score, int *__restrict__ flag)
+                                                  int count, double *__restrict__ score, int *__restrict__ flag,
+                                                  double *__restrict__ lo, double *__restrict__ hi)
 {
     const int c = blockIdx.x * 16 + (threadIdx.x >> 4), l = threadIdx.x & 15;
     const bool ok = c < count;
-    double tt = 0.0, ty = 0.0;
+    double tt = 0.0, ty = 0.0, yy = 0.0;
     const FitDev *F = ok ? fits + cfit[c] : fits;
     size_t row = 0;
     if (ok) {
         row = (size_t)cxc[c] * F->ldb + cpar[c];
         const double *T = F->tc + row * F->ldb;
         for (int k = l; k < F->m; k += 16) {
-            const double v = T[k];
+            const double v = T[k], yb = F->ybo[k];
             tt = fma(v, v, tt);
-            ty = fma(v, F->ybo[k], ty);
+            ty = fma(v, yb, ty);
+            yy = fma(yb, yb, yy);
         }
     }
 #pragma unroll
     for (int o = 8; o >= 1; o >>= 1) {
         tt += __shfl_xor(tt, o, 16);
         ty += __shfl_xor(ty, o, 16);
+        yy += __shfl_xor(yy, o, 16);
     }
     if (!ok || l != 0) return;
     const double bxx = F->tcn[row * 2], bxy = F->tcn[row * 2 + 1];
@@ -297,9 +315,20 @@ __global__ __launch_bounds__(256) void k_tc_score(const FitDev *__restrict__ fit
     const bool gate = !(s > F->tol) || s * kGate < tt;
     flag[c] = gate ? 1 : 0;
     score[c] = gate ? 0.0 : (ry * ry) / (s + F->tol);
+    constexpr double u = 0x1p-24;
+    const double m = (double)F->m;
+    const double dT = 40.0 * u * sqrt(m) * sqrt(fmax(bxx, 0.0));
+    const double dry = dT * sqrt(yy);
+    const double ds = 2.0 * sqrt(tt) * dT + dT * dT + 8.0 * u * m * tt;
+    const double slo = s - ds, shi = s + ds, ary = fabs(ry);
+    const double yhi = slo > 0.0 ? (ary + dry) / sqrt(slo + F->tol) : INFINITY;
+    const double ylo = slo > F->tol ? fmax(ary - dry, 0.0) / sqrt(shi + F->tol) : 0.0;
+    const double h = yhi + 8.0 * u, lw = fmax(ylo - 8.0 * u, 0.0);
+    hi[c] = gate ? INFINITY : h * h;
+    lo[c] = gate ? 0.0 : lw * lw;
 }
 
-/* mx[f] = largest ungated approximate score of fit f (0 if none); one block per fit. */
+/* mx[f] = largest lower score bound of fit f's ungated candidates (0 if none); one block per fit. */
 __global__ __launch_bounds__(256) void k_fit_max(const int *__restrict__ c0, const int *__restrict__ cnt,
                                                  const double *__restrict__ score, const int *__restrict__ flag,
                                                  double *__restrict__ mx)
@@ -320,7 +349,7 @@ __global__ __launch_bounds__(256) void k_fit_max(const int *__restrict__ c0, con
 
 /*
  * Rescoring list, in candidate order: every gated candidate and every ungated one whose
- * approximate score is within the window of its fit's best ungated one.  Two passes over
+ * score interval reaches its fit's largest lower bound (k_tc_score).  Two passes over
  * tiles of kSelTile candidates (256 threads, 4 coalesced candidates each): k_select_count
  * writes each tile's list and gated counts, k_select_write places the tile's entries after
  * those of the tiles before it; out_n[0] = list length, out_n[1] = gated count.  keep[i] = 1
@@ -329,11 +358,9 @@ __global__ __launch_bounds__(256) void k_fit_max(const int *__restrict__ c0, con
  */
 constexpr int kSelTile = 1024;
 
-__device__ inline bool sel_take(int c, const int *cfit, const double *score, const int *flag, const double *mx,
-                                double window)
+__device__ inline bool sel_take(int c, const int *cfit, const double *hi, const int *flag, const double *mx)
 {
-    const double m = mx[cfit[c]];
-    return flag[c] != 0 || (m > 0.0 && score[c] >= m * (1.0 - window));
+    return flag[c] != 0 || hi[c] >= mx[cfit[c]];
 }
 
 __device__ inline int wave_sum_i(int v)
@@ -344,9 +371,9 @@ __device__ inline int wave_sum_i(int v)
 }
 
 __global__ __launch_bounds__(256) void k_select_count(int P, const int *__restrict__ cfit,
-                                                      const double *__restrict__ score,
+                                                      const double *__restrict__ hi,
                                                       const int *__restrict__ flag, const double *__restrict__ mx,
-                                                      double window, int *__restrict__ blk)
+                                                      int *__restrict__ blk)
 {
     __shared__ int sn[4], sg[4];
     const int t = threadIdx.x, base = blockIdx.x * kSelTile;
@@ -355,7 +382,7 @@ __global__ __launch_bounds__(256) void k_select_count(int P, const int *__restri
     for (int i = 0; i < kSelTile / 256; ++i) {
         const int c = base + 256 * i + t;
         if (c < P) {
-            n += sel_take(c, cfit, score, flag, mx, window) ? 1 : 0;
+            n += sel_take(c, cfit, hi, flag, mx) ? 1 : 0;
             g += flag[c] != 0 ? 1 : 0;
         }
     }
@@ -373,9 +400,9 @@ __global__ __launch_bounds__(256) void k_select_count(int P, const int *__restri
 }
 
 __global__ __launch_bounds__(256) void k_select_write(int P, const int *__restrict__ cfit,
-                                                      const double *__restrict__ score,
+                                                      const double *__restrict__ hi,
                                                       const int *__restrict__ flag, const double *__restrict__ mx,
-                                                      double window, const int *__restrict__ blk,
+                                                      const int *__restrict__ blk,
                                                       int *__restrict__ list, int *__restrict__ keep,
                                                       int policy, int *__restrict__ out_n)
 {
@@ -405,7 +432,7 @@ __global__ __launch_bounds__(256) void k_select_write(int P, const int *__restri
 #pragma unroll 1
     for (int i = 0; i < kSelTile / 256; ++i) {
         const int c = base + 256 * i + t;
-        const bool tk = c < P && sel_take(c, cfit, score, flag, mx, window);
+        const bool tk = c < P && sel_take(c, cfit, hi, flag, mx);
         const unsigned long long m = __ballot(tk);
         if (lane == 0) sw[w] = __popcll(m);
         __syncthreads();
@@ -431,6 +458,60 @@ int store_policy()
     if (!std::strcmp(e, "all")) return 1;
     if (!std::strcmp(e, "none")) return 2;
     throw Error(MARS_HIP_EINVAL, std::string("MARS_HIP_RESCORE_STORE must be ungated, all or none, not '") + e + "'");
+}
+
+/* Check of the rescoring's premise, one block per fit: with E the largest exact score of the
+ * fit's rescored candidates, no candidate left unrescored may have an upper bound at or above
+ * E.  Violations (none while the interval bounds hold) are counted in stat[3]. */
+__global__ __launch_bounds__(256) void k_window_check(const int *__restrict__ c0, const int *__restrict__ cnt,
+                                                      const double *__restrict__ lin, const double *__restrict__ hi,
+                                                      const int *__restrict__ done,
+                                                      unsigned long long *__restrict__ stat)
+{
+    __shared__ double sh[256];
+    __shared__ int sn[256];
+    const int f = blockIdx.x, t = threadIdx.x;
+    double e = 0.0;
+    for (int c = c0[f] + t; c < c0[f] + cnt[f]; c += 256)
+        if (done[c]) e = fmax(e, lin[c]);
+    sh[t] = e;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) sh[t] = fmax(sh[t], sh[t + s]);
+        __syncthreads();
+    }
+    const double E = sh[0];
+    int v = 0;
+    for (int c = c0[f] + t; c < c0[f] + cnt[f]; c += 256)
+        if (!done[c] && hi[c] >= E && E > 0.0) ++v;
+    sn[t] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) sn[t] += sn[t + s];
+        __syncthreads();
+    }
+    if (t == 0 && sn[0]) atomicAdd(stat + 3, (unsigned long long)sn[0]);
+}
+
+/* stat[1] += rescored count, stat[2] += gated count (nsel: [count, gated]). */
+__global__ void k_stat_add(const int *__restrict__ nsel, unsigned long long *__restrict__ stat)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        stat[1] += (unsigned long long)nsel[0];
+        stat[2] += (unsigned long long)nsel[1];
+    }
+}
+
+__global__ void k_zero_i(int *__restrict__ p, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+/* done[idx[i]] = 1 */
+__global__ void k_mark(const int *__restrict__ idx, int n, int *__restrict__ done)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) done[idx[i]] = 1;
 }
 
 /* lin[idx[i]] = val[i] */
@@ -559,7 +640,7 @@ void mixed_prepare(mars_hip_ctx *ctx)
     tc_update(ctx);
 }
 
-void mixed_score(mars_hip_ctx *ctx)
+void mixed_score(mars_hip_ctx *ctx, bool host_lists)
 {
     PendingScope pending_scope(ctx);
     Cands &cs = ctx->cand;
@@ -573,15 +654,17 @@ void mixed_score(mars_hip_ctx *ctx)
     upload_cands(ctx, cs);
     h2d_flush(ctx);  // k_tc_score reads them whether or not the update below has work
     tc_update(ctx);
-    ctx->flag.ensure(P);
+    ctx->cgate.ensure(P);
     ctx->lin.ensure(P);
+    ctx->clo.ensure(P);
+    ctx->chi.ensure(P);
     {
         double bytes = 0;
         for (size_t f = 0; f < F; ++f) bytes += 8.0 * cs.fit_P[f] * (ctx->fits[f].m + 2);
         KTimer kt(ctx, K_MIXED_B, 0.0, bytes);
         hipLaunchKernelGGL(k_tc_score, dim3((unsigned)((P + 15) / 16)), dim3(256), 0, st,
                            (const FitDev *)ctx->d_fits.p, (const int *)cs.d_fit.p, (const int *)cs.d_xc.p,
-                           (const int *)cs.d_par.p, P, ctx->lin.p, ctx->flag.p);
+                           (const int *)cs.d_par.p, P, ctx->lin.p, ctx->cgate.p, ctx->clo.p, ctx->chi.p);
         MH_LAUNCH_CHECK();
     }
     // rescoring list on the device (candidate order), then its exact FP64 scores
@@ -596,56 +679,66 @@ void mixed_score(mars_hip_ctx *ctx)
     ctx->red.ensure(F);
     ctx->rsel.ensure(P);
     ctx->rkeep.ensure(P);
+    ctx->cdone.ensure(P);
     int *d_nsel = ctx->ival.p + 2 * F, *d_blk = d_nsel + 2;
     h2d_defer(ctx, ctx->ival.p, ranges.data(), 2 * F * sizeof(int));
     h2d_flush(ctx);
     hipLaunchKernelGGL(k_fit_max, dim3((unsigned)F), dim3(256), 0, st, (const int *)ctx->ival.p,
-                       (const int *)ctx->ival.p + F, (const double *)ctx->lin.p, (const int *)ctx->flag.p, ctx->red.p);
+                       (const int *)ctx->ival.p + F, (const double *)ctx->clo.p, (const int *)ctx->cgate.p, ctx->red.p);
     MH_LAUNCH_CHECK();
-    // An exact gated score above a fit's best ungated approximation only raises the maximum,
-    // so the window taken here contains the final one.
+    // An exact gated score above a fit's largest lower bound only raises the maximum, so every
+    // candidate that can win is on the list.
     hipLaunchKernelGGL(k_select_count, dim3((unsigned)nblk), dim3(256), 0, st, P, (const int *)cs.d_fit.p,
-                       (const double *)ctx->lin.p, (const int *)ctx->flag.p, (const double *)ctx->red.p, kWindow,
-                       d_blk);
+                       (const double *)ctx->chi.p, (const int *)ctx->cgate.p, (const double *)ctx->red.p, d_blk);
     MH_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_select_write, dim3((unsigned)nblk), dim3(256), 0, st, P, (const int *)cs.d_fit.p,
-                       (const double *)ctx->lin.p, (const int *)ctx->flag.p, (const double *)ctx->red.p, kWindow,
+                       (const double *)ctx->chi.p, (const int *)ctx->cgate.p, (const double *)ctx->red.p,
                        (const int *)d_blk, ctx->rsel.p, ctx->rkeep.p, store_policy(), d_nsel);
     MH_LAUNCH_CHECK();
-    // the list length and a prefix of the list sized from the previous step come back together;
-    // a longer list takes a second copy
-    const int guess = std::min(P, std::max(2 * ctx->rsel_hint, 1024));
-    int *pin = ctx->sel_pin.get<int>(2 + (size_t)P);  // [count, gated count, list...]
-    MH_CHECK(hipMemcpyAsync(pin, d_nsel, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-    MH_CHECK(hipMemcpyAsync(pin + 2, ctx->rsel.p, guess * sizeof(int), hipMemcpyDeviceToHost, st));
-    ctx_sync(ctx);
-    const int nsel[2] = {pin[0], pin[1]};
-    const int R = nsel[0];
-    require(R >= 0 && R <= P, "rescoring list length out of range");
-    ctx->rsel_hint = R;
-    ctx->gated += nsel[1];
-    ctx->rescored += R;
-    if (R == 0) return;
-    if (R > guess) {
-        MH_CHECK(hipMemcpyAsync(pin + 2 + guess, ctx->rsel.p + guess, (R - guess) * sizeof(int),
-                                hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(k_zero_i, dim3((unsigned)std::min(1024, (P + 255) / 256)), dim3(256), 0, st, ctx->cdone.p, P);
+    MH_LAUNCH_CHECK();
+    if (!host_lists) {
+        hipLaunchKernelGGL(k_stat_add, dim3(1), dim3(64), 0, st, (const int *)d_nsel, ctx->dstat.p);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, st, ctx->dredo.p, 1);
+        MH_LAUNCH_CHECK();
+        plain_rescore_device(ctx, cs, ctx->rsel.p, d_nsel, P, ctx->rkeep.p, ctx->lin.p, ctx->cdone.p, ctx->dredo.p);
+    } else {
+        // the list comes back to the host, which builds the rescoring's work lists
+        int *pin = ctx->sel_pin.get<int>(2 + (size_t)P);  // [count, gated count, list...]
+        MH_CHECK(hipMemcpyAsync(pin, d_nsel, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
         ctx_sync(ctx);
+        const int R = pin[0];
+        require(R >= 0 && R <= P, "rescoring list length out of range");
+        ctx->gated += pin[1];
+        ctx->rescored += R;
+        if (R > 0) {
+            MH_CHECK(hipMemcpyAsync(pin + 2, ctx->rsel.p, R * sizeof(int), hipMemcpyDeviceToHost, st));
+            ctx_sync(ctx);
+            const int *sub = pin + 2;
+            Cands &rc = ctx->rcand;
+            rc.fit.resize(R);
+            rc.xc.resize(R);
+            rc.par.resize(R);
+            for (int i = 0; i < R; ++i) {
+                rc.fit[i] = cs.fit[sub[i]];
+                rc.xc[i] = cs.xc[sub[i]];
+                rc.par[i] = cs.par[sub[i]];
+            }
+            ctx->rlin.ensure(R);
+            ctx->rybx.ensure(R);
+            plain_score(ctx, rc, false, ctx->rlin.p, ctx->rybx.p, nullptr, nullptr, true, ctx->rkeep.p);
+            hipLaunchKernelGGL(k_put, dim3((R + 255) / 256), dim3(256), 0, st, (const int *)ctx->rsel.p,
+                               (const double *)ctx->rlin.p, R, ctx->lin.p);
+            MH_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_mark, dim3((R + 255) / 256), dim3(256), 0, st, (const int *)ctx->rsel.p, R,
+                               ctx->cdone.p);
+            MH_LAUNCH_CHECK();
+        }
     }
-    const int *sub = pin + 2;
-    Cands &rc = ctx->rcand;
-    rc.fit.resize(R);
-    rc.xc.resize(R);
-    rc.par.resize(R);
-    for (int i = 0; i < R; ++i) {
-        rc.fit[i] = cs.fit[sub[i]];
-        rc.xc[i] = cs.xc[sub[i]];
-        rc.par[i] = cs.par[sub[i]];
-    }
-    ctx->rlin.ensure(R);
-    ctx->rybx.ensure(R);
-    plain_score(ctx, rc, false, ctx->rlin.p, ctx->rybx.p, nullptr, nullptr, true, ctx->rkeep.p);
-    hipLaunchKernelGGL(k_put, dim3((R + 255) / 256), dim3(256), 0, st, (const int *)ctx->rsel.p,
-                       (const double *)ctx->rlin.p, R, ctx->lin.p);
+    hipLaunchKernelGGL(k_window_check, dim3((unsigned)F), dim3(256), 0, st, (const int *)ctx->ival.p,
+                       (const int *)ctx->ival.p + F, (const double *)ctx->lin.p, (const double *)ctx->chi.p,
+                       (const int *)ctx->cdone.p, ctx->dstat.p);
     MH_LAUNCH_CHECK();
 }
 

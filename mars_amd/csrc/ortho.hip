@@ -31,6 +31,8 @@ constexpr double kPassItems = 32768;
 constexpr long long kMaxSplits = 2048;  // row ranges per candidate tile at most
 // residual bytes that the plain form's DGKS retry keeps at once (one arena per context)
 constexpr size_t kRetryArena = size_t(2) << 30;
+// residual bytes of the device-driven rescoring: half for stored entries, half for fired ones
+constexpr size_t kRescoreArena = size_t(2) << 30;
 
 /* T[c] = sum of the candidate's item partials in order; tnorm[c] = |T[c][:m]|^2.
  * One wave per candidate (or list position). */
@@ -39,22 +41,25 @@ __global__ __launch_bounds__(64) void k_treduce(const FitDev *__restrict__ fits,
                                                 const int *__restrict__ list, int count,
                                                 const int *__restrict__ cmap,
                                                 const double *__restrict__ tpart, int ldT,
-                                                double *__restrict__ T, double *__restrict__ tnorm)
+                                                double *__restrict__ T, double *__restrict__ tnorm,
+                                                const int *__restrict__ dcount)
 {
-    const int c = blockIdx.x, lane = threadIdx.x;
-    if (c >= count) return;
-    const int cid = list ? list[c] : c;
-    const int m = fits[cfit[cid]].m;
-    const int first = cmap[3 * c], cnt = cmap[3 * c + 1], slot = cmap[3 * c + 2];
-    double tt = 0.0;
-    for (int k = lane; k < ldT; k += 64) {
-        const double s =
-            ordered_sum(tpart, cnt, [&](int i) { return ((size_t)(first + i) * 64 + slot) * ldT + k; });
-        T[(size_t)c * ldT + k] = s;
-        if (k < m) tt = fma(s, s, tt);
+    if (dcount) count = *dcount;  // list length built on the device
+    const int lane = threadIdx.x;
+    for (int c = blockIdx.x; c < count; c += gridDim.x) {
+        const int cid = list ? list[c] : c;
+        const int m = fits[cfit[cid]].m;
+        const int first = cmap[3 * c], cnt = cmap[3 * c + 1], slot = cmap[3 * c + 2];
+        double tt = 0.0;
+        for (int k = lane; k < ldT; k += 64) {
+            const double s =
+                ordered_sum(tpart, cnt, [&](int i) { return ((size_t)(first + i) * 64 + slot) * ldT + k; });
+            T[(size_t)c * ldT + k] = s;
+            if (k < m) tt = fma(s, s, tt);
+        }
+        tt = wave_sum(tt);
+        if (lane == 0 && tnorm) tnorm[c] = tt;
     }
-    tt = wave_sum(tt);
-    if (lane == 0 && tnorm) tnorm[c] = tt;
 }
 
 /* s[c] = sum of item partials; the first pass also sets flag[c]: bit 0 the DGKS gate
@@ -64,19 +69,20 @@ __global__ void k_sreduce(const int *__restrict__ list, int count, const int *__
                           const double *__restrict__ spart, int tile,
                           const double *__restrict__ tnorm, const FitDev *__restrict__ fits,
                           const int *__restrict__ cfit, const int *__restrict__ keep,
-                          double *__restrict__ ssum, int *__restrict__ flag)
+                          double *__restrict__ ssum, int *__restrict__ flag, const int *__restrict__ dcount)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= count) return;
-    const int first = cmap[3 * c], cnt = cmap[3 * c + 1], slot = cmap[3 * c + 2];
-    const double s = ordered_sum(spart, cnt, [&](int i) { return (size_t)(first + i) * tile + slot; });
-    if (list) {
-        ssum[list[c]] = s;
-    } else {
-        const double tol = fits[cfit[c]].tol;
-        ssum[c] = s;
-        flag[c] = ((s > tol && s * kGate < tnorm[c]) ? 1 : 0) | (s > tol ? 2 : 0) |
-                  ((keep && keep[c] == 0) ? 4 : 0);
+    if (dcount) count = *dcount;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < count; c += gridDim.x * blockDim.x) {
+        const int first = cmap[3 * c], cnt = cmap[3 * c + 1], slot = cmap[3 * c + 2];
+        const double s = ordered_sum(spart, cnt, [&](int i) { return (size_t)(first + i) * tile + slot; });
+        if (list) {
+            ssum[list[c]] = s;
+        } else {
+            const double tol = fits[cfit[c]].tol;
+            ssum[c] = s;
+            flag[c] = ((s > tol && s * kGate < tnorm[c]) ? 1 : 0) | (s > tol ? 2 : 0) |
+                      ((keep && keep[c] == 0) ? 4 : 0);
+        }
     }
 }
 
@@ -151,8 +157,9 @@ __global__ __launch_bounds__(256) void k_finalize_cm(const FitDev *__restrict__ 
                                                      const int *__restrict__ list,
                                                      const int *__restrict__ keep,
                                                      const double *__restrict__ ssum,
-                                                     double *__restrict__ ypart)
+                                                     double *__restrict__ ypart, const int *__restrict__ dn)
 {
+    if (dn) nitems = *dn;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const Item I = items[it];
@@ -161,7 +168,8 @@ __global__ __launch_bounds__(256) void k_finalize_cm(const FitDev *__restrict__ 
             const int c = list ? list[I.c0 + slot] : I.c0 + slot;
             const double s = ssum[c];
             const float scale = (s > F.tol) ? (float)(1.0 / sqrt(s + F.tol)) : 0.0f;
-            const float *col = F.rbuf + (F.rbyl ? I.c0 + slot - I.pad : c - F.c0) * F.rsc;
+            const long long cc = F.rbyl == 2 ? max(F.roff[c], 0ll) : (F.rbyl ? I.c0 + slot - I.pad : c - F.c0);
+            const float *col = F.rbuf + cc * F.rsc;
             double acc = 0.0;
             // s <= tol: scale 0, so ybx = 0 exactly (kernels.cc:401-416) and r is not read; most
             // rescored candidates (duplicates of basis functions) end here
@@ -195,15 +203,17 @@ __global__ __launch_bounds__(256) void k_finalize_cm(const FitDev *__restrict__ 
 
 /* ybx = sum of item partials, lin = ybx^2, for candidate c (or list[c]). */
 __global__ void k_yreduce(int count, const int *__restrict__ cmap, const double *__restrict__ ypart,
-                          const int *__restrict__ list, double *__restrict__ ybx, double *__restrict__ lin)
+                          const int *__restrict__ list, double *__restrict__ ybx, double *__restrict__ lin,
+                          const int *__restrict__ dcount)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= count) return;
-    const int first = cmap[3 * c], cnt = cmap[3 * c + 1], slot = cmap[3 * c + 2];
-    const double s = ordered_sum(ypart, cnt, [&](int i) { return (size_t)(first + i) * kTileF + slot; });
-    const int cand = list ? list[c] : c;
-    if (ybx) ybx[cand] = s;
-    lin[cand] = s * s;
+    if (dcount) count = *dcount;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < count; c += gridDim.x * blockDim.x) {
+        const int first = cmap[3 * c], cnt = cmap[3 * c + 1], slot = cmap[3 * c + 2];
+        const double s = ordered_sum(ypart, cnt, [&](int i) { return (size_t)(first + i) * kTileF + slot; });
+        const int cand = list ? list[c] : c;
+        if (ybx) ybx[cand] = s;
+        lin[cand] = s * s;
+    }
 }
 
 /* lin = ybx = 0 (candidates whose s <= tol: scale 0, kernels.cc:401-416). */
@@ -319,14 +329,15 @@ int grid_w(size_t items)
 
 template <bool RETRY>
 void launch_tgemm(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, const Cands &cs,
-                  const int *list, int ldT, bool narrow)
+                  const int *list, int ldT, bool narrow, const int *dn = nullptr, const FitDev *fits = nullptr)
 {
     if (nitems == 0) return;
     const dim3 grid(grid_w(nitems)), block(64);
+    const FitDev *fv = fits ? fits : (const FitDev *)ctx->d_fits.p;
 #define MH_GO1(M, CG)                                                                                 \
-    hipLaunchKernelGGL((k_tgemm_w<M, CG, RETRY>), grid, block, 0, ctx->stream, (const FitDev *)ctx->d_fits.p, \
+    hipLaunchKernelGGL((k_tgemm_w<M, CG, RETRY>), grid, block, 0, ctx->stream, fv, \
                        (const Item *)u.items, nitems, (const int *)cs.d_xc.p, (const int *)cs.d_par.p, list, ldT, \
-                       ctx->tpart.p)
+                       ctx->tpart.p, dn)
 #define MH_GO(M)                      \
     if (narrow) MH_GO1(M, 1);         \
     else MH_GO1(M, cg_tgemm(M))
@@ -338,15 +349,17 @@ void launch_tgemm(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, cons
 
 template <int MODE, bool RB, bool TCAND>
 void launch_project(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, const Cands &cs,
-                    const int *list, const double *T, int ldT, bool narrow, const int *keep = nullptr)
+                    const int *list, const double *T, int ldT, bool narrow, const int *keep = nullptr,
+                    const int *dn = nullptr, const FitDev *fits = nullptr)
 {
     if (nitems == 0) return;
     const dim3 grid(grid_w(nitems)), block(64);
+    const FitDev *fv = fits ? fits : (const FitDev *)ctx->d_fits.p;
 #define MH_GO1(M, CG)                                                                                     \
     hipLaunchKernelGGL((k_project_w<M, CG, MODE, RB, TCAND>), grid, block, 0, ctx->stream,                \
-                       (const FitDev *)ctx->d_fits.p, (const Item *)u.items, nitems, (const int *)cs.d_xc.p, \
+                       fv, (const Item *)u.items, nitems, (const int *)cs.d_xc.p, \
                        (const int *)cs.d_par.p, list, T, ldT, (const double *)ctx->ssum.p,                \
-                       (const int *)ctx->flag.p, keep, ctx->spart.p)
+                       (const int *)ctx->flag.p, keep, ctx->spart.p, dn)
 #define MH_GO(M)                      \
     if (narrow) MH_GO1(M, 1);         \
     else MH_GO1(M, cg_project(M))
@@ -545,17 +558,17 @@ void retry_compact(mars_hip_ctx *ctx, Cands &cs, const std::vector<int> &fired, 
         launch_project<kStoreR, false, true>(ctx, mt, vP, (int)rP.items.size(), cs, fl, ctx->T.p, ldT, rnarrow);
         launch_tgemm<true>(ctx, mt, vT, (int)rT.items.size(), cs, fl, ldT, rnarrow);
         hipLaunchKernelGGL(k_treduce, dim3(nf), dim3(64), 0, st, ctx->d_fits.p, cs.d_fit.p, fl, nf, vT.cmap,
-                           ctx->tpart.p, ldT, ctx->T2.p, nullptr);
+                           ctx->tpart.p, ldT, ctx->T2.p, nullptr, nullptr);
         MH_LAUNCH_CHECK();
         launch_project<kStoreR, true, false>(ctx, mt, vP, (int)rP.items.size(), cs, fl, ctx->T2.p, ldT, rnarrow);
         hipLaunchKernelGGL(k_sreduce, dim3((nf + 255) / 256), dim3(256), 0, st, fl, nf, vP.cmap, ctx->spart.p,
-                           kSlots, nullptr, ctx->d_fits.p, cs.d_fit.p, nullptr, ctx->ssum.p, nullptr);
+                           kSlots, nullptr, ctx->d_fits.p, cs.d_fit.p, nullptr, ctx->ssum.p, nullptr, nullptr);
         MH_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_finalize_cm, dim3(grid_for(fF.items.size())), dim3(256), 0, st, ctx->d_fits.p,
-                           wF.items, (int)fF.items.size(), fl, nullptr, ctx->ssum.p, ctx->spart.p);
+                           wF.items, (int)fF.items.size(), fl, nullptr, ctx->ssum.p, ctx->spart.p, nullptr);
         MH_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_yreduce, dim3((nf + 255) / 256), dim3(256), 0, st, nf, wF.cmap, ctx->spart.p, fl,
-                           ybx, lin);
+                           ybx, lin, nullptr);
         MH_LAUNCH_CHECK();
         k0 = k1;
         if (k0 < fired.size()) ctx_sync(ctx);  // the next block reuses the arena and the work lists
@@ -563,6 +576,485 @@ void retry_compact(mars_hip_ctx *ctx, Cands &cs, const std::vector<int> &fired, 
 }
 
 } // namespace
+
+namespace {
+
+/* ---------------------------------------------------------------------- */
+/* Device-built lists and work items (the device-driven rescoring)         */
+/* ---------------------------------------------------------------------- */
+
+constexpr int kCompactTile = 1024;   // list entries per compaction block (256 threads x 4)
+constexpr int kRedGrid = 1024;       // blocks of the grid-stride reductions over device-sized lists
+
+/* rfit/rxc/rpar[i] = cfit/cxc/cpar[list[i]] for i < *dcount. */
+__global__ void k_rc_gather(const int *__restrict__ list, const int *__restrict__ dcount,
+                            const int *__restrict__ cfit, const int *__restrict__ cxc, const int *__restrict__ cpar,
+                            int *__restrict__ rfit, int *__restrict__ rxc, int *__restrict__ rpar)
+{
+    const int n = *dcount;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int c = list[i];
+        rfit[i] = cfit[c];
+        rxc[i] = cxc[c];
+        rpar[i] = cpar[c];
+    }
+}
+
+/* first[f], cnt[f] of a list grouped by fit in increasing fit order; the fit of entry i is
+ * lfit[sub ? sub[i] : i].  One thread per fit, binary search. */
+__global__ void k_list_ranges(const int *__restrict__ lfit, const int *__restrict__ sub,
+                              const int *__restrict__ dcount, int F, int *__restrict__ first, int *__restrict__ cnt)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const int n = *dcount;
+    auto fit_at = [&](int i) { return lfit[sub ? sub[i] : i]; };
+    auto lower = [&](int key) {
+        int lo = 0, hi = n;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (fit_at(mid) < key) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    const int b = lower(f), e = lower(f + 1);
+    first[f] = b;
+    cnt[f] = e - b;
+}
+
+/* Residual columns of the entries marked keep: ldx floats each from the arena (bump allocator
+ * shared by the call; the order of the offsets does not matter).  An entry that does not fit
+ * in `cap` floats is not stored (keep 0): its residual is recomputed by the final pass. */
+__global__ void k_alloc_kept(const FitDev *__restrict__ fits, const int *__restrict__ rfit,
+                             const int *__restrict__ dcount, int *__restrict__ keep, long long *__restrict__ roff,
+                             unsigned long long *__restrict__ used, unsigned long long cap)
+{
+    const int n = *dcount;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        long long off = -1;
+        if (keep[i]) {
+            const unsigned long long need = (unsigned long long)fits[rfit[i]].ldx;
+            const unsigned long long at = atomicAdd(used, need);
+            if (at + need <= cap) off = (long long)at;
+            else keep[i] = 0;
+        }
+        roff[i] = off;
+    }
+}
+
+/* Fired entries (list `fl`) without a stored residual take arena columns for the DGKS retry;
+ * when the arena is full, redo[0] = 1 (the caller scores the step again on the host-driven
+ * path: the retry cannot run without the residual). */
+__global__ void k_alloc_fired(const FitDev *__restrict__ fits, const int *__restrict__ rfit,
+                              const int *__restrict__ fl, const int *__restrict__ dnf, long long *__restrict__ roff,
+                              unsigned long long *__restrict__ used, unsigned long long cap, int *__restrict__ redo)
+{
+    const int n = *dnf;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const int i = fl[j];
+        if (roff[i] >= 0) continue;
+        const unsigned long long need = (unsigned long long)fits[rfit[i]].ldx;
+        const unsigned long long at = atomicAdd(used, need);
+        if (at + need <= cap) roff[i] = (long long)at;
+        else redo[0] = 1;
+    }
+}
+
+__device__ inline int wave_isum(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+/* Order-preserving compaction of the positions i < *dcount with (flag[i] & mask) == want:
+ * k_compact_count writes each tile's count, k_compact_write places the tile's entries after
+ * those of the tiles before it and writes the length to *out_n. */
+__global__ __launch_bounds__(256) void k_compact_count(const int *__restrict__ dcount, const int *__restrict__ flag,
+                                                       int mask, int want, int *__restrict__ blk)
+{
+    __shared__ int sn[4];
+    const int n = *dcount, t = threadIdx.x, base = blockIdx.x * kCompactTile;
+    int c = 0;
+#pragma unroll
+    for (int u = 0; u < kCompactTile / 256; ++u) {
+        const int i = base + 256 * u + t;
+        c += (i < n && (flag[i] & mask) == want) ? 1 : 0;
+    }
+    c = wave_isum(c);
+    if ((t & 63) == 0) sn[t >> 6] = c;
+    __syncthreads();
+    if (t == 0) blk[blockIdx.x] = sn[0] + sn[1] + sn[2] + sn[3];
+}
+
+__global__ __launch_bounds__(256) void k_compact_write(const int *__restrict__ dcount, const int *__restrict__ flag,
+                                                       int mask, int want, const int *__restrict__ blk,
+                                                       int *__restrict__ out, int *__restrict__ out_n)
+{
+    __shared__ int sw[4];
+    const int n = *dcount, t = threadIdx.x, w = t >> 6, lane = t & 63, b = blockIdx.x;
+    // every block of the grid has a count (0 past the list's end): the last one writes the total
+    int off = 0;
+    for (int j = t; j < b; j += 256) off += blk[j];
+    off = wave_isum(off);
+    if (lane == 0) sw[w] = off;
+    __syncthreads();
+    int run = sw[0] + sw[1] + sw[2] + sw[3];
+    __syncthreads();
+    if (b == (int)gridDim.x - 1 && t == 0) out_n[0] = run + blk[b];
+    if (b * kCompactTile >= n) return;
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll 1
+    for (int u = 0; u < kCompactTile / 256; ++u) {
+        const int i = b * kCompactTile + 256 * u + t;
+        const bool take = i < n && (flag[i] & mask) == want;
+        const unsigned long long m = __ballot(take);
+        if (lane == 0) sw[w] = __popcll(m);
+        __syncthreads();
+        int woff = run;
+        for (int k = 0; k < w; ++k) woff += sw[k];
+        if (take) out[woff + __popcll(m & below)] = i;
+        run += sw[0] + sw[1] + sw[2] + sw[3];
+        __syncthreads();
+    }
+}
+
+/*
+ * Work list of one pass over a device list, built on the device the way make_pass and
+ * split_rows build it on the host: each fit's positions [first, first + cnt) cut into tiles of
+ * `tile`, each tile into row ranges of about n * target / work rows (at least minrows, at most
+ * maxsplits ranges, multiples of align); cmap gets (first item, item count, slot) per position.
+ * One block of 1024 threads; fits are split into contiguous chunks per thread.
+ */
+__global__ __launch_bounds__(1024) void k_pass_build(const FitDev *__restrict__ fits, int F,
+                                                     const int *__restrict__ first, const int *__restrict__ cnt,
+                                                     int tile, double target, long long minrows,
+                                                     long long maxsplits, long long align, Item *__restrict__ items,
+                                                     int *__restrict__ cmap, int cap, int *__restrict__ nitems)
+{
+    __shared__ double sw[1024];
+    __shared__ long long ss[1024];
+    const int t = threadIdx.x;
+    const int chunk = (F + 1023) / 1024, f0 = min(F, t * chunk), f1 = min(F, f0 + chunk);
+    double w = 0.0;
+    for (int f = f0; f < f1; ++f) w += (double)fits[f].n * (double)((cnt[f] + tile - 1) / tile);
+    sw[t] = w;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s) sw[t] = sw[t] + sw[t + s];
+        __syncthreads();
+    }
+    const double work = sw[0];
+    auto row_len = [&](int f) {
+        const long long n = fits[f].n;
+        long long splits = work > 0 ? (long long)(target * (double)n / work) : 1;
+        splits = max(splits, 1ll);
+        splits = min(splits, max(1ll, n / minrows));
+        splits = min(splits, maxsplits);
+        const long long rows = (n + splits - 1) / splits;
+        return max(align, (rows + align - 1) / align * align);
+    };
+    long long mine = 0;
+    for (int f = f0; f < f1; ++f) {
+        if (cnt[f] <= 0) continue;
+        const long long n = fits[f].n, r = row_len(f);
+        mine += (long long)((cnt[f] + tile - 1) / tile) * ((n + r - 1) / r);
+    }
+    ss[t] = mine;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const long long v = t >= o ? ss[t - o] : 0;
+        __syncthreads();
+        ss[t] += v;
+        __syncthreads();
+    }
+    long long at = ss[t] - mine;
+    if (t == 1023) nitems[0] = (int)min(ss[1023], (long long)cap);
+    for (int f = f0; f < f1; ++f) {
+        if (cnt[f] <= 0) continue;
+        const long long n = fits[f].n, r = row_len(f);
+        const int nr = (int)((n + r - 1) / r);
+        for (int c0 = first[f]; c0 < first[f] + cnt[f]; c0 += tile) {
+            const int nc = min(tile, first[f] + cnt[f] - c0);
+            for (int c = c0; c < c0 + nc; ++c) {
+                cmap[3 * (size_t)c] = (int)min(at, (long long)cap);
+                cmap[3 * (size_t)c + 1] = at + nr <= cap ? nr : 0;
+                cmap[3 * (size_t)c + 2] = c - c0;
+            }
+            for (int q = 0; q < nr; ++q, ++at) {
+                if (at >= cap) continue;
+                Item it;
+                it.fit = f;
+                it.c0 = c0;
+                it.nc = nc;
+                it.pad = first[f];
+                it.r0 = (long long)q * r;
+                it.r1 = min(n, (long long)(q + 1) * r);
+                items[at] = it;
+            }
+        }
+    }
+}
+
+/* lin[list[i]] = val[i], done[list[i]] = 1 for i < *dcount. */
+__global__ void k_put_dev(const int *__restrict__ list, const int *__restrict__ dcount,
+                          const double *__restrict__ val, double *__restrict__ lin, int *__restrict__ done)
+{
+    const int n = *dcount;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        lin[list[i]] = val[i];
+        if (done) done[list[i]] = 1;
+    }
+}
+
+/* *acc += *n */
+__global__ void k_count_add(const int *__restrict__ n, unsigned long long *__restrict__ acc)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *acc += (unsigned long long)*n;
+}
+
+__global__ void k_fill_i(int *__restrict__ p, int n, int v)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long *__restrict__ p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+/* Device pass: items, cmap and the device item count. */
+struct DevPass {
+    Item *items;
+    int *cmap;
+    int *n;
+    int cap;
+};
+
+} // namespace
+
+/*
+ * Exact FP64 rescoring (the plain form's arithmetic and rounding points) of the candidates
+ * cs[list[i]], i < *dcount, with every list, work item and count built on the device: no
+ * host synchronisation.  list: candidate indices, increasing (so grouped by fit); bound: the
+ * host's upper bound on *dcount.  Results: lin[list[i]] = exact score, done[list[i]] = 1.
+ * keep (per list entry): store the entry's first residual for the final pass.  Residuals go to
+ * the context's arena (kRescoreArena bytes, half for stored entries, the rest for the DGKS
+ * retry of fired ones); an entry that does not fit is recomputed, and a fired one that does
+ * not fit sets *redo (the caller scores again on the host-driven path).
+ */
+void plain_rescore_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, const int *dcount, int bound,
+                          int *keep, double *lin, int *done, int *redo)
+{
+    PendingScope pending_scope(ctx);
+    if (bound <= 0) return;
+    hipStream_t st = ctx->stream;
+    const int F = (int)ctx->fits.size();
+    size_t mmax = 1;
+    for (const FitState &fs : ctx->fits) mmax = std::max(mmax, fs.m);
+    require(mmax <= 256, "the FP64 scoring path supports up to 256 basis functions");
+    const int mt = (int)std::min<size_t>(16, (mmax + 15) / 16), ldT = 16 * mt;
+    const size_t B = (size_t)bound;
+
+    // buffers, sized for the bound (no reallocation once the pipeline is queued)
+    Cands &rc = ctx->rcand;
+    rc.d_fit.ensure(B);
+    rc.d_xc.ensure(B);
+    rc.d_par.ensure(B);
+    ctx->roff.ensure(B);
+    ctx->T.ensure(B * ldT);
+    ctx->T2.ensure(B * ldT);
+    ctx->tnorm.ensure(B);
+    ctx->ssum.ensure(B);
+    ctx->flag.ensure(B);
+    ctx->rlin.ensure(B);
+    ctx->rybx.ensure(B);
+    ctx->fired_list.ensure(B);
+    ctx->live_list.ensure(B);
+    ctx->dl_int.ensure(6 * (size_t)F + 16 + 2 * ((B + kCompactTile - 1) / kCompactTile + 1));
+    ctx->dl_u64.ensure(2);
+    const size_t capT = (size_t)kPassItems + B / 16 + F + 64;   // T-GEMM / projection (narrow tiles)
+    const size_t capF = 2048 + B / 16 + F + 64;                 // finalize
+    const size_t capR = 4096 + B / 16 + F + 64;                 // DGKS retry passes
+    const size_t capL = (size_t)kPassItems + B / 16 + F + 64;   // final pass of unstored entries
+    ctx->d_items.ensure(capT + capF);
+    ctx->d_ritems.ensure(2 * capR + capF);
+    ctx->d_litems.ensure(capL);
+    ctx->d_cmap.ensure(6 * B + 6);
+    ctx->d_rcmap.ensure(9 * B + 9);
+    ctx->d_lcmap.ensure(3 * B + 3);
+    ctx->tpart.ensure(std::max(capT, capR) * kSlots * ldT);
+    ctx->spart.ensure(std::max(std::max(capT, capR) * kSlots, std::max(capF, capL) * kTileF) + 1);
+    const size_t arena = kRescoreArena / sizeof(float);
+    ctx->rarena.ensure(arena);
+
+    int *ip = ctx->dl_int.p;
+    int *rfirst = ip, *rcnt = ip + F, *ffirst = ip + 2 * F, *fcnt = ip + 3 * F, *lfirst = ip + 4 * F,
+        *lcnt = ip + 5 * F;
+    int *nT = ip + 6 * F, *nF = nT + 1, *nRT = nT + 2, *nRP = nT + 3, *nFF = nT + 4, *nL = nT + 5, *dnf = nT + 6,
+        *dnl = nT + 7;
+    int *blk = nT + 16;
+    unsigned long long *used = ctx->dl_u64.p;
+    DevPass pT{ctx->d_items.p, ctx->d_cmap.p, nT, (int)capT};
+    DevPass pF{ctx->d_items.p + capT, ctx->d_cmap.p + 3 * B + 3, nF, (int)capF};
+    DevPass rT{ctx->d_ritems.p, ctx->d_rcmap.p, nRT, (int)capR};
+    DevPass rP{ctx->d_ritems.p + capR, ctx->d_rcmap.p + 3 * B + 3, nRP, (int)capR};
+    DevPass fF{ctx->d_ritems.p + 2 * capR, ctx->d_rcmap.p + 6 * B + 6, nFF, (int)capF};
+    DevPass pL{ctx->d_litems.p, ctx->d_lcmap.p, nL, (int)capL};
+
+    // fits as seen by the rescoring: residual columns at per-entry arena offsets
+    {
+        std::vector<FitDev> h((size_t)F);
+        for (int f = 0; f < F; ++f) {
+            const FitState &fs = ctx->fits[f];
+            FitDev &d = h[f];
+            d.X = fs.X.p;
+            d.s = fs.s.p;
+            d.B = fs.B.p;
+            d.Bo = fs.Bo.p;
+            d.BoC = fs.BoC.p;
+            d.BC = fs.BC.p;
+            d.y = fs.y.p;
+            d.ybo = fs.ybo.p;
+            d.rbuf = ctx->rarena.p;
+            d.v = fs.v.p;
+            d.tc = fs.tc.p;
+            d.tcn = fs.tcn.p;
+            d.n = (long long)fs.n;
+            d.ldx = (long long)fs.ldx;
+            d.ldb = (long long)ctx->ldb;
+            d.rsr = 1;
+            d.rsc = 1;
+            d.m = (int)fs.m;
+            d.c0 = 0;
+            d.rbyl = 2;
+            d.roff = ctx->roff.p;
+            d.tol = fs.tol;
+        }
+        ctx->d_fits2.ensure((size_t)F);
+        h2d_defer(ctx, ctx->d_fits2.p, h.data(), h.size() * sizeof(FitDev));
+        h2d_flush(ctx);
+    }
+    const FitDev *fits = ctx->d_fits2.p;
+    const dim3 gB((unsigned)std::min<size_t>((B + 255) / 256, kRedGrid)), b256(256);
+    const dim3 gF((unsigned)((F + 255) / 256));
+    const unsigned nblk = (unsigned)((B + kCompactTile - 1) / kCompactTile);
+    auto build = [&](const DevPass &p, const int *first, const int *cnt, int tile, double target, long long minrows,
+                     long long align) {
+        hipLaunchKernelGGL(k_pass_build, dim3(1), dim3(1024), 0, st, fits, F, first, cnt, tile, target, minrows,
+                           (long long)kMaxSplits, align, p.items, p.cmap, p.cap, p.n);
+        MH_LAUNCH_CHECK();
+    };
+    Uploaded uT{pT.items, pT.cmap}, uF{pF.items, pF.cmap}, vT{rT.items, rT.cmap}, vP{rP.items, rP.cmap},
+        wF{fF.items, fF.cmap}, uL{pL.items, pL.cmap};
+    const int gT = (int)std::min<size_t>(capT, 65536), gR = (int)std::min<size_t>(capR, 65536),
+              gL = (int)std::min<size_t>(capL, 65536);
+
+    // candidates of the list, their ranges, residual columns
+    hipLaunchKernelGGL(k_rc_gather, gB, b256, 0, st, list, dcount, (const int *)cs.d_fit.p, (const int *)cs.d_xc.p,
+                       (const int *)cs.d_par.p, rc.d_fit.p, rc.d_xc.p, rc.d_par.p);
+    MH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_list_ranges, gF, b256, 0, st, (const int *)rc.d_fit.p, nullptr, dcount, F, rfirst, rcnt);
+    MH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, used, 1);
+    MH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_alloc_kept, gB, b256, 0, st, fits, (const int *)rc.d_fit.p, dcount, keep, ctx->roff.p, used,
+                       (unsigned long long)(arena / 2));
+    MH_LAUNCH_CHECK();
+    build(pT, rfirst, rcnt, 16, kPassItems, 256, kWR);
+    build(pF, rfirst, rcnt, 16, 2048, 512, 32);
+
+    // T = Bo^T Bx, s = sum r^2 (residuals stored for the entries kept), DGKS flags
+    {
+        KTimer kt(ctx, K_TGEMM, 0.0, 0.0);
+        launch_tgemm<false>(ctx, mt, uT, gT, rc, nullptr, ldT, true, nT, fits);
+        hipLaunchKernelGGL(k_treduce, dim3((unsigned)std::min<size_t>(B, 4 * kRedGrid)), dim3(64), 0, st, fits,
+                           (const int *)rc.d_fit.p, nullptr, bound, (const int *)pT.cmap, (const double *)ctx->tpart.p,
+                           ldT, ctx->T.p, ctx->tnorm.p, dcount);
+        MH_LAUNCH_CHECK();
+    }
+    {
+        KTimer kt(ctx, K_PROJECT, 0.0, 0.0);
+        launch_project<kStoreR, false, false>(ctx, mt, uT, gT, rc, nullptr, ctx->T.p, ldT, true, keep, nT, fits);
+        hipLaunchKernelGGL(k_sreduce, gB, b256, 0, st, nullptr, bound, (const int *)pT.cmap, (const double *)ctx->spart.p,
+                           kSlots, (const double *)ctx->tnorm.p, fits, (const int *)rc.d_fit.p, (const int *)keep,
+                           ctx->ssum.p, ctx->flag.p, dcount);
+        MH_LAUNCH_CHECK();
+    }
+    // final pass over the stored residuals (unstored entries with s > tol get 0 here)
+    {
+        KTimer kt(ctx, K_FINALIZE, 0.0, 0.0);
+        hipLaunchKernelGGL(k_finalize_cm, dim3((unsigned)std::min<size_t>(capF, 4096)), b256, 0, st, fits,
+                           (const Item *)pF.items, (int)capF, nullptr, (const int *)keep, (const double *)ctx->ssum.p,
+                           ctx->spart.p, (const int *)nF);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_yreduce, gB, b256, 0, st, bound, (const int *)pF.cmap, (const double *)ctx->spart.p,
+                           nullptr, ctx->rybx.p, ctx->rlin.p, dcount);
+        MH_LAUNCH_CHECK();
+    }
+    // DGKS retry of the fired entries (kernels.cc:387-399)
+    {
+        KTimer kt(ctx, K_DGKS, 0.0, 0.0);
+        hipLaunchKernelGGL(k_compact_count, dim3(nblk), b256, 0, st, dcount, (const int *)ctx->flag.p, 1, 1, blk);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_compact_write, dim3(nblk), b256, 0, st, dcount, (const int *)ctx->flag.p, 1, 1,
+                           (const int *)blk, ctx->fired_list.p, dnf);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_list_ranges, gF, b256, 0, st, (const int *)rc.d_fit.p, (const int *)ctx->fired_list.p,
+                           (const int *)dnf, F, ffirst, fcnt);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_count_add, dim3(1), dim3(64), 0, st, (const int *)dnf, ctx->dstat.p + 4);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_alloc_fired, dim3(4), b256, 0, st, fits, (const int *)rc.d_fit.p,
+                           (const int *)ctx->fired_list.p, (const int *)dnf, ctx->roff.p, used,
+                           (unsigned long long)arena, redo);
+        MH_LAUNCH_CHECK();
+        build(rT, ffirst, fcnt, 16, 4096, 256, kWR);
+        build(rP, ffirst, fcnt, 16, 4096, 256, kWR);
+        build(fF, ffirst, fcnt, 16, 2048, 512, 32);
+        const int *fl = ctx->fired_list.p;
+        // the first residual of every fired entry (recomputed where it was stored already)
+        launch_project<kStoreR, false, true>(ctx, mt, vP, gR, rc, fl, ctx->T.p, ldT, true, nullptr, nRP, fits);
+        launch_tgemm<true>(ctx, mt, vT, gR, rc, fl, ldT, true, nRT, fits);
+        hipLaunchKernelGGL(k_treduce, dim3(256), dim3(64), 0, st, fits, (const int *)rc.d_fit.p, fl, bound,
+                           (const int *)rT.cmap, (const double *)ctx->tpart.p, ldT, ctx->T2.p, nullptr,
+                           (const int *)dnf);
+        MH_LAUNCH_CHECK();
+        launch_project<kStoreR, true, false>(ctx, mt, vP, gR, rc, fl, ctx->T2.p, ldT, true, nullptr, nRP, fits);
+        hipLaunchKernelGGL(k_sreduce, dim3(64), b256, 0, st, fl, bound, (const int *)rP.cmap,
+                           (const double *)ctx->spart.p, kSlots, nullptr, fits, (const int *)rc.d_fit.p, nullptr,
+                           ctx->ssum.p, nullptr, (const int *)dnf);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_finalize_cm, dim3(256), b256, 0, st, fits, (const Item *)fF.items, (int)capF, fl, nullptr,
+                           (const double *)ctx->ssum.p, ctx->spart.p, (const int *)nFF);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_yreduce, dim3(64), b256, 0, st, bound, (const int *)fF.cmap, (const double *)ctx->spart.p,
+                           fl, ctx->rybx.p, ctx->rlin.p, (const int *)dnf);
+        MH_LAUNCH_CHECK();
+    }
+    // entries with s > tol whose residual was not stored and that did not fire: recompute r
+    {
+        KTimer kt(ctx, K_FINALIZE, 0.0, 0.0);
+        hipLaunchKernelGGL(k_compact_count, dim3(nblk), b256, 0, st, dcount, (const int *)ctx->flag.p, 7, 6, blk);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_compact_write, dim3(nblk), b256, 0, st, dcount, (const int *)ctx->flag.p, 7, 6,
+                           (const int *)blk, ctx->live_list.p, dnl);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_list_ranges, gF, b256, 0, st, (const int *)rc.d_fit.p, (const int *)ctx->live_list.p,
+                           (const int *)dnl, F, lfirst, lcnt);
+        MH_LAUNCH_CHECK();
+        build(pL, lfirst, lcnt, 16, kPassItems, 256, kWR);
+        launch_project<kFinalY, false, true>(ctx, mt, uL, gL, rc, ctx->live_list.p, ctx->T.p, ldT, true, nullptr, nL,
+                                             fits);
+        hipLaunchKernelGGL(k_yreduce, gB, b256, 0, st, bound, (const int *)pL.cmap, (const double *)ctx->spart.p,
+                           (const int *)ctx->live_list.p, ctx->rybx.p, ctx->rlin.p, (const int *)dnl);
+        MH_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_put_dev, gB, b256, 0, st, list, dcount, (const double *)ctx->rlin.p, lin, done);
+    MH_LAUNCH_CHECK();
+}
 
 void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, double *ybx,
                  const float *xover, const float *sover, bool rescore, const int *keep)
@@ -642,7 +1134,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         KTimer kt(ctx, K_TGEMM, flops, bytes);
         launch_tgemm<false>(ctx, mt, uT, (int)pT.items.size(), cs, nullptr, ldT, narrow);
         hipLaunchKernelGGL(k_treduce, dim3(P), dim3(64), 0, st, ctx->d_fits.p, cs.d_fit.p, nullptr, P,
-                           uT.cmap, ctx->tpart.p, ldT, ctx->T.p, ctx->tnorm.p);
+                           uT.cmap, ctx->tpart.p, ldT, ctx->T.p, ctx->tnorm.p, nullptr);
         MH_LAUNCH_CHECK();
     }
     // Pass 2: s = sum f32(Bx - Bo T)^2 and the DGKS gate; r itself is kept only for the
@@ -657,7 +1149,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                                                  narrow);
         hipLaunchKernelGGL(k_sreduce, dim3((P + 255) / 256), dim3(256), 0, st, nullptr, P, uP.cmap,
                            ctx->spart.p, kSlots, ctx->tnorm.p, ctx->d_fits.p, cs.d_fit.p, keep,
-                           ctx->ssum.p, ctx->flag.p);
+                           ctx->ssum.p, ctx->flag.p, nullptr);
         MH_LAUNCH_CHECK();
     }
     // Stored column-major residuals (the mixed form's rescoring) are finalized before the host
@@ -684,10 +1176,10 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
             for (size_t f = 0; f < F; ++f) fb2 += 4.0 * n[f] * cnt[f] + 4.0 * n[f];
             KTimer kt(ctx, K_FINALIZE, 0.0, fb2);
             hipLaunchKernelGGL(k_finalize_cm, dim3(grid_for(pF.items.size())), dim3(256), 0, st, ctx->d_fits.p,
-                               uF.items, nF, nullptr, keep, ctx->ssum.p, ctx->spart.p);
+                               uF.items, nF, nullptr, keep, ctx->ssum.p, ctx->spart.p, nullptr);
             MH_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, P, uF.cmap, ctx->spart.p,
-                               nullptr, ybx, lin);
+                               nullptr, ybx, lin, nullptr);
             MH_LAUNCH_CHECK();
         }
         MH_CHECK(hipEventSynchronize(ctx->ev_rb));
@@ -742,13 +1234,13 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                                                  ctx->T.p, ldT, rnarrow);
         launch_tgemm<true>(ctx, mt, vT, (int)rT.items.size(), cs, ctx->fired_list.p, ldT, rnarrow);
         hipLaunchKernelGGL(k_treduce, dim3(nf), dim3(64), 0, st, ctx->d_fits.p, cs.d_fit.p,
-                           ctx->fired_list.p, nf, vT.cmap, ctx->tpart.p, ldT, ctx->T2.p, nullptr);
+                           ctx->fired_list.p, nf, vT.cmap, ctx->tpart.p, ldT, ctx->T2.p, nullptr, nullptr);
         MH_LAUNCH_CHECK();
         launch_project<kStoreR, true, false>(ctx, mt, vP, (int)rP.items.size(), cs, ctx->fired_list.p, ctx->T2.p,
                                              ldT, rnarrow);
         hipLaunchKernelGGL(k_sreduce, dim3((nf + 255) / 256), dim3(256), 0, st, ctx->fired_list.p, nf,
                            vP.cmap, ctx->spart.p, kSlots, nullptr, ctx->d_fits.p, cs.d_fit.p, nullptr,
-                           ctx->ssum.p, nullptr);
+                           ctx->ssum.p, nullptr, nullptr);
         MH_LAUNCH_CHECK();
         if (spec) {  // pass 3 again for the fired candidates, over their retried residuals
             Pass fF = make_pass(rb, re, n, split_rows(n, fcnt, 16, 2048, 512), 16, 32);
@@ -760,10 +1252,10 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
             ctx->spart.ensure(fF.items.size() * kTileF + 1);
             hipLaunchKernelGGL(k_finalize_cm, dim3(grid_for(fF.items.size())), dim3(256), 0, st, ctx->d_fits.p,
                                wF.items, (int)fF.items.size(), (const int *)ctx->fired_list.p, nullptr,
-                               ctx->ssum.p, ctx->spart.p);
+                               ctx->ssum.p, ctx->spart.p, nullptr);
             MH_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_yreduce, dim3((nf + 255) / 256), dim3(256), 0, st, nf, wF.cmap, ctx->spart.p,
-                               ctx->fired_list.p, ybx, lin);
+                               ctx->fired_list.p, ybx, lin, nullptr);
             MH_LAUNCH_CHECK();
         }
     }
@@ -789,7 +1281,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
             launch_project<kFinalY, false, true>(ctx, mt, uL, (int)pL.items.size(), cs, ctx->live_list.p,
                                                  ctx->T.p, ldT, lnarrow);
             hipLaunchKernelGGL(k_yreduce, dim3((nl + 255) / 256), dim3(256), 0, st, nl, uL.cmap, ctx->spart.p,
-                               ctx->live_list.p, ybx, lin);
+                               ctx->live_list.p, ybx, lin, nullptr);
             MH_LAUNCH_CHECK();
         }
         return;
@@ -811,7 +1303,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                                    ctx->ssum.p, ctx->spart.p);
             MH_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, P, uF.cmap, ctx->spart.p,
-                               nullptr, ybx, lin);
+                               nullptr, ybx, lin, nullptr);
         } else {
             // only candidates with s > tol can score above 0; the others are 0 exactly.  The
             // fired ones are finished by their retry (retry_compact); the rest recompute r.
@@ -847,7 +1339,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                 launch_project<kFinalY, false, true>(ctx, mt, uL, (int)pL.items.size(), cs, ctx->live_list.p,
                                                      ctx->T.p, ldT, lnarrow);
                 hipLaunchKernelGGL(k_yreduce, dim3((nl + 255) / 256), dim3(256), 0, st, nl, uL.cmap, ctx->spart.p,
-                                   ctx->live_list.p, ybx, lin);
+                                   ctx->live_list.p, ybx, lin, nullptr);
             }
             MH_LAUNCH_CHECK();
             if (!fired.empty()) {

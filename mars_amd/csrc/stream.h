@@ -71,7 +71,8 @@ __device__ inline LaneCand lane_cand(const FitDev &F, const Item &I, int slot, c
     const int pos = I.c0 + slot;
     const int cand = list ? list[pos] : pos;
     c.cand = cand;
-    c.col = F.rbyl ? pos - I.pad : cand - F.c0;
+    // (an unstored candidate's offset is -1: clamped so an address formed for it stays valid)
+    c.col = F.rbyl == 2 ? max(F.roff[cand], 0ll) : (F.rbyl ? pos - I.pad : cand - F.c0);
     if (!RB) {
         const int xc = cxc[cand];
         c.sx = F.s[xc];
@@ -131,8 +132,10 @@ template <int MT, int CG, bool RETRY>
 __global__ __launch_bounds__(64, waves_for(MT)) void k_tgemm_w(const FitDev *__restrict__ fits, const Item *__restrict__ items,
                                                 int nitems, const int *__restrict__ cxc,
                                                 const int *__restrict__ cpar, const int *__restrict__ list,
-                                                int ldT, double *__restrict__ tpart)
+                                                int ldT, double *__restrict__ tpart,
+                                                const int *__restrict__ dn)
 {
+    if (dn) nitems = *dn;  // item count built on the device
     constexpr int HF = kHoist(MT, CG) ? 2 : 1;   // half-steps whose loads are issued together
     constexpr int NCH = kHoist(MT, CG) ? 2 : 1;  // 32-row chunks per iteration
     const int lane = threadIdx.x, kr = lane >> 4, kc = lane & 15;
@@ -250,8 +253,10 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_project_w(const FitDev *_
                                                                  const double *__restrict__ ssum,
                                                                  const int *__restrict__ fired,
                                                                  const int *__restrict__ keep,
-                                                                 double *__restrict__ spart)
+                                                                 double *__restrict__ spart,
+                                                                 const int *__restrict__ dn)
 {
+    if (dn) nitems = *dn;  // item count built on the device
     constexpr int KQ = 4 * MT;           // k steps of 4
     constexpr int NC = 16 * CG, ST = 16 * MT + 1;
     constexpr int HF = 1;  // 16-row blocks whose loads are issued together (two spill registers)

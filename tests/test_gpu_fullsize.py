@@ -115,6 +115,8 @@ def test_config4_full_mixed_equals_plain(policy, monkeypatch):
         for u, v in zip(sp["mse"][i], sm["mse"][i]):
             assert abs(u - v) <= 1e-6 * abs(u), (i, u, v)
     assert 0 < sm["rescored"] < sm["scored"]
+    assert sm["window_violations"] == 0
+    say("config 4 mixed: rescored %d of %d, host redo steps %d" % (sm["rescored"], sm["scored"], sm["host_redo_steps"]))
 
 
 def test_config5_full_mixed_equals_plain():

@@ -380,3 +380,35 @@ def test_batch_mixed_matches_reference(policy, monkeypatch):
         for a, b in zip(mse, [a["mse"] for a in gold["appends"]]):
             assert abs(a - b) <= 1e-10 * abs(b)
     assert 0 < stats["rescored"] < stats["scored"]
+    assert stats["window_violations"] == 0 and stats["host_redo_steps"] == 0
+
+
+def _rival_inputs(seed, n, p, eps):
+    """X columns 0 and 1 nearly equal (column 1 = column 0 times 1 + eps, plus eps-sized noise):
+    candidates on column 0 and column 1 score within a relative ~eps of each other, below the
+    f32 error of the mixed form's cached projections, while neither is in the basis's span."""
+    rng = np.random.RandomState(seed)
+    X = np.asfortranarray(rng.randn(n, p).astype(np.float32))
+    X[:, 1] = (X[:, 0].astype(np.float64) * (1 + eps) + eps * rng.randn(n)).astype(np.float32)
+    y = (1.5 * X[:, 0] + 0.8 * X[:, 2] - 0.5 * X[:, 3] * X[:, 4] + rng.randn(n)).astype(np.float32)
+    return X, y
+
+
+@pytest.mark.parametrize("eps", [1e-6, 3e-7])
+def test_mixed_resolves_rivals_below_fp32_error(eps):
+    """Near-tied rivals away from the DGKS gate: the mixed form's score intervals send both to
+    the exact rescoring, so it picks the plain form's term every step."""
+    import mars_amd
+
+    say("rivals eps", eps)
+    data = [_rival_inputs(s, 20000 + 3000 * s, 8, eps) for s in range(6)]
+    Xs, ys = [x for x, _ in data], [y for _, y in data]
+    sp, sm = {}, {}
+    plain = mars_amd.fit_batch(Xs, ys, max_terms=12, stats=sp)
+    mixed = mars_amd.fit_batch(Xs, ys, max_terms=12, precision="mixed", stats=sm)
+    for i, (a, b) in enumerate(zip(plain, mixed)):
+        sa = [(r["type"], int(r["basis"]), int(r["input"])) for r in a]
+        sb = [(r["type"], int(r["basis"]), int(r["input"])) for r in b]
+        assert sa == sb, (i, sa, sb)
+        assert {0, 1} & {int(r["input"]) for r in a[1:2]}  # the first pick is one of the rivals
+    assert sm["window_violations"] == 0
