@@ -45,7 +45,7 @@ FP32_KERNELS = ("mixed_update",)
 # rocprofv3 average to compare with): the exact rescoring and DGKS retry, the append (build,
 # one MGS kernel per basis column, gate, store, final) and the FP64 passes (each pass kernel
 # plus its reduce kernel).  The roofline is reported for the largest single-kernel class.
-COMPOSITE = ("mixed_recheck", "dgks_retry", "append", "tgemm_f64", "project_f64", "finalize")
+COMPOSITE = ("mixed_recheck", "dgks_retry", "append", "tgemm_f64", "project_f64", "finalize", "plain_graph")
 # Classes bound by HBM traffic by design (arithmetic intensity below the MFMA ridge point).
 HBM_BOUND = ("mixed_update", "append", "finalize", "sort", "argmax", "mixed_score")
 CPU_SAMPLE_S = 20.0
@@ -325,7 +325,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=CPU_SAMPLE_S)
     ap.add_argument("--copy-probe", action="store_true", help="run only a 1 GiB device copy (PMC calibration)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the batch step's scoring kernels directly instead of replaying captured graphs "
+                         "(profiling runs: kernel traces are taken on direct launches)")
     args = ap.parse_args()
+    if args.no_graph:
+        os.environ["MARS_HIP_GRAPH"] = "0"
     if args.copy_probe:
         copy_probe()
         return
@@ -360,7 +365,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if forms[-1] == "fp64" else "f32 (MFMA) + f64",
         "data": "synthetic (mars_amd.datagen, seeded)",
-        "config": dict(workload=workload, precision=forms[-1]),
+        "config": dict(workload=workload, precision=forms[-1],
+                       graphs=os.environ.get("MARS_HIP_GRAPH", "1") != "0"),
         "fit_s": {f: r["fit_s"] for f, r in res.items()},
         "upload_s": {f: r["upload_s"] for f, r in res.items()},
         "candidates_per_fit": head["cands"],

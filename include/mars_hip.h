@@ -201,13 +201,16 @@ int mars_hip_batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts *opts, mars_
  * bulk pass (both 0 for the plain form).  Any pointer may be NULL. */
 int mars_hip_score_stats(mars_hip_ctx *ctx, long *scored, long *rescored, long *gated);
 
-/* Checks of the mixed form since creation: window_violations counts candidates
- * left unrescored whose score interval reached their fit's best exact score (0
- * while the interval bounds hold: then every step's winner is an FP64 value);
- * host_redo_steps counts batch steps whose device-driven rescoring ran out of
- * residual room and were scored again with host-built lists.  No counterpart in
- * the reference (its GPU path has no mixed form).  Either pointer may be NULL. */
-int mars_hip_check_stats(mars_hip_ctx *ctx, long *window_violations, long *host_redo_steps);
+/* Checks and launch counts since creation: window_violations counts mixed-form
+ * candidates left unrescored whose score interval reached their fit's best
+ * exact score (0 while the interval bounds hold: then every step's winner is an
+ * FP64 value); host_redo_steps counts batch steps whose device-driven scoring
+ * ran out of residual room and were scored again with host-built lists;
+ * graph_captures / graph_launches count the batch step's scoring graphs
+ * captured and launched (MARS_HIP_GRAPH=0 turns them off).  No counterpart in
+ * the reference.  Any pointer may be NULL. */
+int mars_hip_check_stats(mars_hip_ctx *ctx, long *window_violations, long *host_redo_steps, long *graph_captures,
+                         long *graph_launches);
 
 /* Per-kernel-class timing with HIP events on the context's stream (off by
  * default).  Each record: launches, summed device milliseconds and the

@@ -5,6 +5,7 @@
 
 #include <chrono>
 #include <cstddef>
+#include <map>
 #include <cstdint>
 #include <algorithm>
 #include <stdexcept>
@@ -123,6 +124,8 @@ struct FitDev {
     int m, c0;         // live basis count, first candidate of this fit
     int rbyl;          // how a candidate's rbuf column is found: 0 column cand - c0, 1 (list passes)
                        // position in the fit's part of the list, 2 float offset roff[cand]
+    int px;            // X columns
+    int act;           // batch driver: the fit takes part in this step (candidates are generated)
     int pad_;
     const long long *roff;  // rbyl == 2: per candidate, float offset of its column in rbuf (-1: none)
     double tol;
@@ -213,12 +216,12 @@ constexpr int kDStat = 8;  // device counters per context (mars_hip_ctx::dstat)
 
 /* Kernel classes timed by mars_hip_timing. */
 enum KClass { K_TGEMM = 0, K_PROJECT, K_FINALIZE, K_DGKS, K_SWEEP, K_SORT, K_APPEND, K_MIXED_A,
-              K_MIXED_B, K_RECHECK, K_ARGMAX,
+              K_MIXED_B, K_RECHECK, K_ARGMAX, K_GRAPH_PLAIN,
               // host wall-clock sections of a batch step (include the waits they contain)
               K_H_CANDS, K_H_SCORE, K_H_ARGMAX, K_H_APPEND, K_H_STEP, K_NCLASS };
 constexpr const char *kClassName[K_NCLASS] = {"tgemm_f64", "project_f64", "finalize", "dgks_retry",
                                              "hinge_sweep", "sort", "append", "mixed_update",
-                                             "mixed_score", "mixed_recheck", "argmax",
+                                             "mixed_score", "mixed_recheck", "argmax", "plain_graph",
                                              "host_cands", "host_score", "host_argmax", "host_append",
                                              "host_step"};
 
@@ -287,6 +290,16 @@ struct mars_hip_ctx {
                                          // [3] window violations, [4] DGKS retries of the rescoring
     long redo_steps = 0;             // batch steps scored again on the host-driven path
     mh::DBuf<int> dredo;             // 1: the step's device-driven rescoring must be redone on the host
+    // batch driver, candidates generated on the device: per fit and basis function its degree and
+    // the bit set of X columns it contains (words of 32 bits, kw per basis function)
+    mh::DBuf<int> bdeg;
+    mh::DBuf<unsigned> bhas;
+    int kw = 1;
+    std::vector<int> act_h;          // the active flags the last upload_fits sent
+    mh::DBuf<int> dcand;             // batch driver: device count of the step's candidates
+    bool use_graphs = false;         // batch driver: replay captured graphs of the scoring launches
+    std::map<std::vector<unsigned long long>, hipGraphExec_t> graphs;  // by run_graph key
+    long graph_captures = 0, graph_launches = 0;
     char *scratch = nullptr;         // device address of this device's no-alloc append scratch
     int rsel_hint = 0;               // the previous rescoring list's length
     mh::DBuf<mh::FitDev> d_fits;

@@ -600,8 +600,9 @@ void fit_state_alloc(mars_hip_ctx *ctx, FitState &fs, size_t n, double tol)
     fs.v.ensure(n);
     // room for the residuals of 64 candidates (either layout), allocated with the fit so that a
     // forward pass does not begin with one large allocation per fit; grown when a step needs more.
-    // A plain-form batch stores none there (its DGKS retry uses the context's bounded arena).
-    if (ctx->row_major || ctx->form == MARS_HIP_FORM_MIXED) {
+    // A batch stores none there: its scoring keeps residuals in the context's bounded arena (the
+    // host-built fallback grows the buffer when it runs).
+    if (ctx->row_major) {
         fs.ldr = 64;
         fs.rbuf.ensure(fs.ldx * fs.ldr);
     }
@@ -737,6 +738,52 @@ void free_retired()
     g_retired_host.clear();
 }
 
+void run_graph(mars_hip_ctx *ctx, const std::vector<unsigned long long> &key, const std::function<void()> &launches)
+{
+    if (!ctx->use_graphs) {
+        launches();
+        return;
+    }
+    auto it = ctx->graphs.find(key);
+    if (it == ctx->graphs.end()) {
+        // capture: the launches' own timers are off (a graph replay runs none of their host code)
+        const bool timing = ctx->timing;
+        ctx->timing = false;
+        hipGraph_t g = nullptr;
+        MH_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            launches();
+        } catch (...) {
+            (void)hipStreamEndCapture(ctx->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            ctx->timing = timing;
+            throw;
+        }
+        MH_CHECK(hipStreamEndCapture(ctx->stream, &g));
+        ctx->timing = timing;
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        MH_CHECK(e);
+        it = ctx->graphs.emplace(key, ex).first;
+        ctx->graph_captures += 1;
+    }
+    MH_CHECK(hipGraphLaunch(it->second, ctx->stream));
+    ctx->graph_launches += 1;
+}
+
+void graph_key_buffers(mars_hip_ctx *ctx, std::vector<unsigned long long> &key)
+{
+    const void *p[] = {ctx->cand.d_fit.p, ctx->cand.d_xc.p, ctx->cand.d_par.p, ctx->rcand.d_fit.p, ctx->rcand.d_xc.p,
+                       ctx->rcand.d_par.p, ctx->lin.p, ctx->ybx.p, ctx->cgate.p, ctx->clo.p, ctx->chi.p, ctx->cdone.p,
+                       ctx->rsel.p, ctx->rkeep.p, ctx->ival.p, ctx->red.p, ctx->dstat.p, ctx->dredo.p, ctx->d_fits.p,
+                       ctx->d_fits2.p, ctx->roff.p, ctx->rarena.p, ctx->T.p, ctx->T2.p, ctx->tnorm.p, ctx->ssum.p,
+                       ctx->flag.p, ctx->rlin.p, ctx->rybx.p, ctx->fired_list.p, ctx->live_list.p, ctx->dl_int.p,
+                       ctx->dl_u64.p, ctx->d_items.p, ctx->d_ritems.p, ctx->d_litems.p, ctx->d_cmap.p,
+                       ctx->d_rcmap.p, ctx->d_lcmap.p, ctx->tpart.p, ctx->spart.p, ctx->dcand.p};
+    for (const void *q : p) key.push_back((unsigned long long)(uintptr_t)q);
+}
+
 void ctx_sync(mars_hip_ctx *ctx)
 {
     h2d_flush(ctx);
@@ -780,6 +827,17 @@ void ctx_free(mars_hip_ctx *ctx)
     ctx->aarg.release();
     ctx->d_fitw.release();
     ctx->bout.release();
+    ctx->rarena.release();
+    ctx->roff.release();
+    ctx->d_fits2.release();
+    for (DBuf<int> *b : {&ctx->dl_int, &ctx->cgate, &ctx->cdone, &ctx->dredo, &ctx->bdeg, &ctx->dcand})
+        b->release();
+    for (DBuf<double> *b : {&ctx->clo, &ctx->chi}) b->release();
+    ctx->dl_u64.release();
+    ctx->dstat.release();
+    ctx->bhas.release();
+    for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
+    ctx->graphs.clear();
     for (auto &r : ctx->pend) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
@@ -1106,6 +1164,8 @@ void fit_append_noalloc(mars_hip_ctx *ctx, const AppendArg &a, double *mse)
     d.rsr = 1;
     d.rsc = (long long)fs.ldx;
     d.m = (int)fs.m;
+    d.px = (int)fs.px;
+    d.act = 1;
     d.tol = fs.tol;
     const FitW w{fs.B.p, fs.Bo.p, fs.BoC.p, fs.BC.p, fs.ybo.p, fs.G.p};
     hipStream_t st = ctx->stream;
@@ -1181,6 +1241,16 @@ void argmax_launch(mars_hip_ctx *ctx, const Cands &cs, const double *lin)
                            ctx->ival.p + F, ctx->bout.p);
         MH_LAUNCH_CHECK();
     }
+}
+
+void argmax_launch_ranges(mars_hip_ctx *ctx, const double *lin, size_t bound)
+{
+    const size_t F = ctx->fits.size();
+    ctx->bout.ensure(F);
+    KTimer kt(ctx, K_ARGMAX, 0.0, 8.0 * (double)bound);
+    hipLaunchKernelGGL(k_argmax, dim3(F), dim3(1024), 0, ctx->stream, lin, nullptr, nullptr, ctx->ival.p,
+                       ctx->ival.p + F, ctx->bout.p);
+    MH_LAUNCH_CHECK();
 }
 
 void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t stride, int min_span,
@@ -1661,13 +1731,14 @@ int mars_hip_score_stats(mars_hip_ctx *ctx, long *scored, long *rescored, long *
         require(ctx != nullptr, "null context");
         unsigned long long d[kDStat];
         read_dstat(ctx, d);
-        if (scored) *scored = ctx->scored;
+        if (scored) *scored = ctx->scored + (long)d[0];
         if (rescored) *rescored = ctx->rescored + (long)d[1];
         if (gated) *gated = ctx->gated + (long)d[2];
     });
 }
 
-int mars_hip_check_stats(mars_hip_ctx *ctx, long *window_violations, long *host_redo_steps)
+int mars_hip_check_stats(mars_hip_ctx *ctx, long *window_violations, long *host_redo_steps, long *graph_captures,
+                         long *graph_launches)
 {
     return guarded([&] {
         require(ctx != nullptr, "null context");
@@ -1675,6 +1746,8 @@ int mars_hip_check_stats(mars_hip_ctx *ctx, long *window_violations, long *host_
         read_dstat(ctx, d);
         if (window_violations) *window_violations = (long)d[3];
         if (host_redo_steps) *host_redo_steps = ctx->redo_steps;
+        if (graph_captures) *graph_captures = ctx->graph_captures;
+        if (graph_launches) *graph_launches = ctx->graph_launches;
     });
 }
 

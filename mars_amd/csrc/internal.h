@@ -5,6 +5,7 @@
 
 #include <exception>
 #include <functional>
+#include <map>
 
 namespace mh {
 
@@ -34,18 +35,31 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                  const float *xover = nullptr, const float *sover = nullptr, bool rescore = false,
                  const int *keep = nullptr);
 
-/* Exact FP64 rescoring of cs[list[i]], i < *dcount (device-built lists and work items, no host
- * synchronisation); lin[list[i]] = exact score, done[list[i]] = 1 (done may be null);
- * keep[i]: store entry i's first residual; *redo = 1 when a fired entry found no arena room
- * (the step must then be scored again on the host-driven path). */
-void plain_rescore_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, const int *dcount, int bound,
-                          int *keep, double *lin, int *done, int *redo);
+/* FP64 scoring with device-built lists and work items (no host synchronisation).  With `list`:
+ * the candidates cs[list[i]], i < *dcount (bound: upper bound on the count), lin[list[i]] = score,
+ * done[list[i]] = 1 (done may be null), keep[i]: store entry i's first residual.  Without: every
+ * candidate of cs (bound = its count), lin[c] / ybx[c], keep null.  *redo = 1 when a fired entry
+ * found no arena room (the step must then be scored again with host-built lists).  narrow:
+ * 16-candidate tiles. */
+void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, const int *dcount, int bound,
+                        int *keep, double *lin, double *ybx, int *done, int *redo, bool narrow, int phase = 0);
+/* phase 1: size the buffers and upload the fit views only; phase 2: the kernel launches only (no
+ * allocation, upload or synchronisation: they can be captured into a graph); 0: both. */
+
+/* Run `launches` (kernel launches on the context's stream, nothing else) through a graph
+ * captured the first time `key` is seen and replayed afterwards; directly when graphs are off.
+ * The key must name every kernel argument the launches take (pointers, sizes, variants). */
+void run_graph(mars_hip_ctx *ctx, const std::vector<unsigned long long> &key, const std::function<void()> &launches);
+/* Key words for run_graph: the addresses of the context's scoring buffers. */
+void graph_key_buffers(mars_hip_ctx *ctx, std::vector<unsigned long long> &key);
+/* True when the per-fit counts favour 16-candidate tiles. */
+bool narrow_tiles(const std::vector<int> &cnt);
 
 /* Mixed-precision scoring of ctx->cand (linear delta-SSE only) from each
  * fit's incrementally maintained T cache (FP32 MFMA, FP64 across row chunks),
  * then an exact FP64 rescoring of the DGKS-gated and near-winning
  * candidates.  Writes ctx->lin. */
-void mixed_score(mars_hip_ctx *ctx, bool host_lists = false);
+void mixed_score(mars_hip_ctx *ctx, bool host_lists = false, const int *dP = nullptr, int Pb = 0);
 /* Launch the mixed form's cache update for every fit's newest basis column; it needs no candidate
  * list, so a caller can build the list while the update runs (mixed_score then finds it done). */
 void mixed_prepare(mars_hip_ctx *ctx);
@@ -95,6 +109,8 @@ void fit_append_batch(mars_hip_ctx *ctx, std::vector<AppendArg> &args, std::vect
 /* Per-fit argmax of `lin` over each fit's candidates (first maximum in candidate order), queued
  * on the stream: ctx->bout receives it, ctx->ival holds the fits' candidate offsets and counts. */
 void argmax_launch(mars_hip_ctx *ctx, const Cands &cs, const double *lin);
+/* The same over ranges already on the device (ctx->ival: first candidate, count per fit). */
+void argmax_launch_ranges(mars_hip_ctx *ctx, const double *lin, size_t bound);
 double host_dsse(const FitState &fs);
 /* Copy the context's device counters (kDStat entries) to `out` (synchronises the stream). */
 void read_dstat(mars_hip_ctx *ctx, unsigned long long *out);

@@ -12,6 +12,7 @@
 #include <pybind11/numpy.h>
 
 #include <cmath>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -164,9 +165,9 @@ public:
         long fired = 0;
         double ms = 0;
         check(mars_hip_stats(ctx_, &fired, &ms));
-        long sc = 0, re = 0, ga = 0, wv = 0, rd = 0;
+        long sc = 0, re = 0, ga = 0, wv = 0, rd = 0, gc = 0, gl = 0;
         check(mars_hip_score_stats(ctx_, &sc, &re, &ga));
-        check(mars_hip_check_stats(ctx_, &wv, &rd));
+        check(mars_hip_check_stats(ctx_, &wv, &rd, &gc, &gl));
         py::dict d;
         d["dgks_fired"] = fired;
         d["eval_ms"] = ms;
@@ -175,6 +176,8 @@ public:
         d["gated"] = ga;
         d["window_violations"] = wv;
         d["host_redo_steps"] = rd;
+        d["graph_captures"] = gc;
+        d["graph_launches"] = gl;
         return d;
     }
 
@@ -269,17 +272,38 @@ public:
             rc = mars_hip_batch_fit(ctx_, &o, t.data(), nt.data(), &steps);
         }
         check(rc);
-        py::list out;
+        // model rows as packed records of mars.MODEL_DTYPE (type S1, basis i4, input i4, hinge f8,
+        // beta f4, r2 f4, r2_cv f4, order i4, time f4: 37 bytes), fit by fit, plus the mse of each
+        // row: the caller views them with numpy without a Python loop over rows
+        constexpr size_t kRec = 37;
+        size_t total = 0;
+        for (size_t f = 0; f < nfits_; ++f) total += (size_t)nt[f];
+        std::string rec(total * kRec, '\0');
+        py::array_t<double> mse(total);
+        py::array_t<int> counts(nfits_);
+        double *pm = mse.mutable_data();
+        char *q = &rec[0];
+        const double nan = std::nan("");
+        const float zero = 0.0f, fnan = std::nanf("");
         for (size_t f = 0; f < nfits_; ++f) {
-            py::list rows;
+            counts.mutable_data()[f] = nt[f];
             for (int r = 0; r < nt[f]; ++r) {
                 const mars_hip_term &x = t[f * max_terms_ + r];
-                rows.append(py::make_tuple(std::string(1, (char)x.type), x.basis, x.input, x.order, x.mse,
-                                           x.r2, x.r2_cv));
+                const char ty = (char)x.type;
+                std::memcpy(q, &ty, 1);
+                std::memcpy(q + 1, &x.basis, 4);
+                std::memcpy(q + 5, &x.input, 4);
+                std::memcpy(q + 9, &nan, 8);
+                std::memcpy(q + 17, &fnan, 4);
+                std::memcpy(q + 21, &x.r2, 4);
+                std::memcpy(q + 25, &x.r2_cv, 4);
+                std::memcpy(q + 29, &x.order, 4);
+                std::memcpy(q + 33, &zero, 4);
+                q += kRec;
+                *pm++ = x.mse;
             }
-            out.append(rows);
         }
-        return py::make_tuple(out, steps);
+        return py::make_tuple(py::bytes(rec), counts, mse, steps);
     }
 
     void set_timing(bool on) { check(mars_hip_timing(ctx_, on ? 1 : 0)); }
@@ -307,9 +331,9 @@ public:
         long fired = 0;
         double ms = 0;
         check(mars_hip_stats(ctx_, &fired, &ms));
-        long sc = 0, re = 0, ga = 0, wv = 0, rd = 0;
+        long sc = 0, re = 0, ga = 0, wv = 0, rd = 0, gc = 0, gl = 0;
         check(mars_hip_score_stats(ctx_, &sc, &re, &ga));
-        check(mars_hip_check_stats(ctx_, &wv, &rd));
+        check(mars_hip_check_stats(ctx_, &wv, &rd, &gc, &gl));
         py::dict d;
         d["dgks_fired"] = fired;
         d["eval_ms"] = ms;
@@ -318,6 +342,8 @@ public:
         d["gated"] = ga;
         d["window_violations"] = wv;
         d["host_redo_steps"] = rd;
+        d["graph_captures"] = gc;
+        d["graph_launches"] = gl;
         return d;
     }
 

@@ -285,8 +285,10 @@ __global__ __launch_bounds__(256) void k_tc_reduce(const FitDev *__restrict__ fi
 __global__ __launch_bounds__(256) void k_tc_score(const FitDev *__restrict__ fits, const int *__restrict__ cfit,
                                                   const int *__restrict__ cxc, const int *__restrict__ cpar,
                                                   int count, double *__restrict__ score, int *__restrict__ flag,
-                                                  double *__restrict__ lo, double *__restrict__ hi)
+                                                  double *__restrict__ lo, double *__restrict__ hi,
+                                                  const int *__restrict__ dcount)
 {
+    if (dcount) count = *dcount;
     const int c = blockIdx.x * 16 + (threadIdx.x >> 4), l = threadIdx.x & 15;
     const bool ok = c < count;
     double tt = 0.0, ty = 0.0, yy = 0.0;
@@ -373,8 +375,9 @@ __device__ inline int wave_sum_i(int v)
 __global__ __launch_bounds__(256) void k_select_count(int P, const int *__restrict__ cfit,
                                                       const double *__restrict__ hi,
                                                       const int *__restrict__ flag, const double *__restrict__ mx,
-                                                      int *__restrict__ blk)
+                                                      int *__restrict__ blk, const int *__restrict__ dcount)
 {
+    if (dcount) P = *dcount;
     __shared__ int sn[4], sg[4];
     const int t = threadIdx.x, base = blockIdx.x * kSelTile;
     int n = 0, g = 0;
@@ -404,8 +407,10 @@ __global__ __launch_bounds__(256) void k_select_write(int P, const int *__restri
                                                       const int *__restrict__ flag, const double *__restrict__ mx,
                                                       const int *__restrict__ blk,
                                                       int *__restrict__ list, int *__restrict__ keep,
-                                                      int policy, int *__restrict__ out_n)
+                                                      int policy, int *__restrict__ out_n,
+                                                      const int *__restrict__ dcount)
 {
+    if (dcount) P = *dcount;
     __shared__ int sw[4], sg[4];
     const int t = threadIdx.x, w = t >> 6, lane = t & 63, b = blockIdx.x, base = b * kSelTile;
     // this tile's offset: the list counts of the tiles before it (the last tile also sums the gated counts)
@@ -640,70 +645,105 @@ void mixed_prepare(mars_hip_ctx *ctx)
     tc_update(ctx);
 }
 
-void mixed_score(mars_hip_ctx *ctx, bool host_lists)
+void mixed_score(mars_hip_ctx *ctx, bool host_lists, const int *dP, int Pb)
 {
     PendingScope pending_scope(ctx);
     Cands &cs = ctx->cand;
-    const int P = (int)cs.fit.size();
+    const bool dev_cands = dP != nullptr;  // candidates generated on the device (count *dP <= Pb)
+    const int P = dev_cands ? Pb : (int)cs.fit.size();
     if (P == 0) return;
     hipStream_t st = ctx->stream;
     const size_t F = ctx->fits.size();
-    index_cands(ctx, cs);
     ensure_tcache(ctx);
-    upload_fits(ctx, cs);
-    upload_cands(ctx, cs);
-    h2d_flush(ctx);  // k_tc_score reads them whether or not the update below has work
+    if (!dev_cands) {
+        index_cands(ctx, cs);
+        upload_fits(ctx, cs);
+        upload_cands(ctx, cs);
+        h2d_flush(ctx);  // k_tc_score reads them whether or not the update below has work
+    }
     tc_update(ctx);
     ctx->cgate.ensure(P);
     ctx->lin.ensure(P);
     ctx->clo.ensure(P);
     ctx->chi.ensure(P);
-    {
-        double bytes = 0;
-        for (size_t f = 0; f < F; ++f) bytes += 8.0 * cs.fit_P[f] * (ctx->fits[f].m + 2);
-        KTimer kt(ctx, K_MIXED_B, 0.0, bytes);
-        hipLaunchKernelGGL(k_tc_score, dim3((unsigned)((P + 15) / 16)), dim3(256), 0, st,
-                           (const FitDev *)ctx->d_fits.p, (const int *)cs.d_fit.p, (const int *)cs.d_xc.p,
-                           (const int *)cs.d_par.p, P, ctx->lin.p, ctx->cgate.p, ctx->clo.p, ctx->chi.p);
-        MH_LAUNCH_CHECK();
-    }
-    // rescoring list on the device (candidate order), then its exact FP64 scores
-    KTimer kt(ctx, K_RECHECK, 0.0, 0.0);
-    std::vector<int> ranges(2 * F);
-    for (size_t f = 0; f < F; ++f) {
-        ranges[f] = cs.fit_c0[f];
-        ranges[F + f] = cs.fit_P[f];
-    }
     const int nblk = (P + kSelTile - 1) / kSelTile;
-    ctx->ival.ensure(2 * F + 2 + 2 * (size_t)nblk);
+    ctx->ival.ensure(2 * F + 2 + 2 * (size_t)nblk);  // (with device candidates: already so sized)
     ctx->red.ensure(F);
     ctx->rsel.ensure(P);
     ctx->rkeep.ensure(P);
     ctx->cdone.ensure(P);
     int *d_nsel = ctx->ival.p + 2 * F, *d_blk = d_nsel + 2;
-    h2d_defer(ctx, ctx->ival.p, ranges.data(), 2 * F * sizeof(int));
-    h2d_flush(ctx);
-    hipLaunchKernelGGL(k_fit_max, dim3((unsigned)F), dim3(256), 0, st, (const int *)ctx->ival.p,
-                       (const int *)ctx->ival.p + F, (const double *)ctx->clo.p, (const int *)ctx->cgate.p, ctx->red.p);
-    MH_LAUNCH_CHECK();
-    // An exact gated score above a fit's largest lower bound only raises the maximum, so every
-    // candidate that can win is on the list.
-    hipLaunchKernelGGL(k_select_count, dim3((unsigned)nblk), dim3(256), 0, st, P, (const int *)cs.d_fit.p,
-                       (const double *)ctx->chi.p, (const int *)ctx->cgate.p, (const double *)ctx->red.p, d_blk);
-    MH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_select_write, dim3((unsigned)nblk), dim3(256), 0, st, P, (const int *)cs.d_fit.p,
-                       (const double *)ctx->chi.p, (const int *)ctx->cgate.p, (const double *)ctx->red.p,
-                       (const int *)d_blk, ctx->rsel.p, ctx->rkeep.p, store_policy(), d_nsel);
-    MH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_zero_i, dim3((unsigned)std::min(1024, (P + 255) / 256)), dim3(256), 0, st, ctx->cdone.p, P);
-    MH_LAUNCH_CHECK();
-    if (!host_lists) {
+    if (!dev_cands) {
+        std::vector<int> ranges(2 * F);
+        for (size_t f = 0; f < F; ++f) {
+            ranges[f] = cs.fit_c0[f];
+            ranges[F + f] = cs.fit_P[f];
+        }
+        h2d_defer(ctx, ctx->ival.p, ranges.data(), 2 * F * sizeof(int));
+        h2d_flush(ctx);
+    }
+    const int policy = store_policy();
+    const bool device = !host_lists;
+    if (device)  // buffers and fit views of the exact rescoring, before anything is queued below
+        plain_score_device(ctx, cs, ctx->rsel.p, d_nsel, P, ctx->rkeep.p, ctx->lin.p, nullptr, ctx->cdone.p,
+                           ctx->dredo.p, true, 1);
+    // approximate scores, the rescoring list, and (device lists) the exact rescoring itself
+    auto body = [&] {
+        {
+            double bytes = 0;
+            for (size_t f = 0; f < F; ++f) bytes += 8.0 * cs.fit_P[f] * (ctx->fits[f].m + 2);
+            KTimer kt(ctx, K_MIXED_B, 0.0, bytes);
+            hipLaunchKernelGGL(k_tc_score, dim3((unsigned)((P + 15) / 16)), dim3(256), 0, st,
+                               (const FitDev *)ctx->d_fits.p, (const int *)cs.d_fit.p, (const int *)cs.d_xc.p,
+                               (const int *)cs.d_par.p, P, ctx->lin.p, ctx->cgate.p, ctx->clo.p, ctx->chi.p, dP);
+            MH_LAUNCH_CHECK();
+        }
+        KTimer kt(ctx, K_RECHECK, 0.0, 0.0);
+        hipLaunchKernelGGL(k_fit_max, dim3((unsigned)F), dim3(256), 0, st, (const int *)ctx->ival.p,
+                           (const int *)ctx->ival.p + F, (const double *)ctx->clo.p, (const int *)ctx->cgate.p,
+                           ctx->red.p);
+        MH_LAUNCH_CHECK();
+        // An exact gated score above a fit's largest lower bound only raises the maximum, so every
+        // candidate that can win is on the list.
+        hipLaunchKernelGGL(k_select_count, dim3((unsigned)nblk), dim3(256), 0, st, P, (const int *)cs.d_fit.p,
+                           (const double *)ctx->chi.p, (const int *)ctx->cgate.p, (const double *)ctx->red.p, d_blk,
+                           dP);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_select_write, dim3((unsigned)nblk), dim3(256), 0, st, P, (const int *)cs.d_fit.p,
+                           (const double *)ctx->chi.p, (const int *)ctx->cgate.p, (const double *)ctx->red.p,
+                           (const int *)d_blk, ctx->rsel.p, ctx->rkeep.p, policy, d_nsel, dP);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_zero_i, dim3((unsigned)std::min(1024, (P + 255) / 256)), dim3(256), 0, st, ctx->cdone.p,
+                           P);
+        MH_LAUNCH_CHECK();
+        if (!device) return;
         hipLaunchKernelGGL(k_stat_add, dim3(1), dim3(64), 0, st, (const int *)d_nsel, ctx->dstat.p);
         MH_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, st, ctx->dredo.p, 1);
         MH_LAUNCH_CHECK();
-        plain_rescore_device(ctx, cs, ctx->rsel.p, d_nsel, P, ctx->rkeep.p, ctx->lin.p, ctx->cdone.p, ctx->dredo.p);
-    } else {
+        plain_score_device(ctx, cs, ctx->rsel.p, d_nsel, P, ctx->rkeep.p, ctx->lin.p, nullptr, ctx->cdone.p,
+                           ctx->dredo.p, true, 2);
+        hipLaunchKernelGGL(k_window_check, dim3((unsigned)F), dim3(256), 0, st, (const int *)ctx->ival.p,
+                           (const int *)ctx->ival.p + F, (const double *)ctx->lin.p, (const double *)ctx->chi.p,
+                           (const int *)ctx->cdone.p, ctx->dstat.p);
+        MH_LAUNCH_CHECK();
+    };
+    if (device && dev_cands && ctx->use_graphs) {
+        // one graph per basis-size class: every argument below is fixed for the fit
+        size_t mmax = 1;
+        for (const FitState &fs : ctx->fits) mmax = std::max(mmax, fs.m);
+        std::vector<unsigned long long> key = {1ull, (unsigned long long)std::min<size_t>(16, (mmax + 15) / 16),
+                                               (unsigned long long)P, (unsigned long long)F,
+                                               (unsigned long long)policy};
+        graph_key_buffers(ctx, key);
+        KTimer kt(ctx, K_RECHECK, 0.0, 0.0);
+        run_graph(ctx, key, body);
+        return;
+    }
+    body();
+    if (device) return;
+    {
+        require(!dev_cands, "internal: host-built rescoring lists need host-built candidates");
         // the list comes back to the host, which builds the rescoring's work lists
         int *pin = ctx->sel_pin.get<int>(2 + (size_t)P);  // [count, gated count, list...]
         MH_CHECK(hipMemcpyAsync(pin, d_nsel, 2 * sizeof(int), hipMemcpyDeviceToHost, st));

@@ -441,6 +441,8 @@ void upload_fits(mars_hip_ctx *ctx, const Cands &cs, const float *xover, const f
         d.rsr = ctx->rbuf_cm ? 1 : (long long)fs.ldr;
         d.rsc = ctx->rbuf_cm ? (long long)fs.ldx : 1;
         d.rbyl = 0;
+        d.px = (int)fs.px;
+        d.act = f < ctx->act_h.size() ? ctx->act_h[f] : 1;
         d.pad_ = 0;
         if (arena_off) {  // compact column-major residuals in the retry arena
             d.rbuf = ctx->rarena.p + (*arena_off)[f];
@@ -493,6 +495,12 @@ bool narrow_for(const std::vector<int> &c)
     }
     return tot <= 24 * act;
 }
+
+} // namespace
+
+bool narrow_tiles(const std::vector<int> &cnt) { return narrow_for(cnt); }
+
+namespace {
 
 /*
  * DGKS retry of the plain form's linear scoring (kernels.cc:387-399), for the fired candidates
@@ -633,7 +641,7 @@ __global__ void k_alloc_kept(const FitDev *__restrict__ fits, const int *__restr
     const int n = *dcount;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         long long off = -1;
-        if (keep[i]) {
+        if (keep && keep[i]) {
             const unsigned long long need = (unsigned long long)fits[rfit[i]].ldx;
             const unsigned long long at = atomicAdd(used, need);
             if (at + need <= cap) off = (long long)at;
@@ -836,17 +844,20 @@ struct DevPass {
 } // namespace
 
 /*
- * Exact FP64 rescoring (the plain form's arithmetic and rounding points) of the candidates
- * cs[list[i]], i < *dcount, with every list, work item and count built on the device: no
- * host synchronisation.  list: candidate indices, increasing (so grouped by fit); bound: the
- * host's upper bound on *dcount.  Results: lin[list[i]] = exact score, done[list[i]] = 1.
- * keep (per list entry): store the entry's first residual for the final pass.  Residuals go to
- * the context's arena (kRescoreArena bytes, half for stored entries, the rest for the DGKS
- * retry of fired ones); an entry that does not fit is recomputed, and a fired one that does
- * not fit sets *redo (the caller scores again on the host-driven path).
+ * FP64 scoring (the plain form's arithmetic and rounding points) with every list, work item
+ * and count built on the device: no host synchronisation.
+ *   list != null: the candidates cs[list[i]], i < *dcount (increasing, so grouped by fit);
+ *                 bound: the host's upper bound on *dcount; lin[list[i]] = score, done[list[i]] = 1;
+ *                 keep[i]: store entry i's first residual for the final pass.
+ *   list == null: every candidate of cs (bound = its size); lin[c], ybx[c] = score, ybx; no
+ *                 residual is stored except the DGKS-fired ones'.
+ * narrow: 16-candidate tiles (few candidates per fit) instead of the T-GEMM / projection widths.
+ * Residuals go to the context's arena (kRescoreArena bytes: at most half for stored entries, the
+ * rest for the DGKS retry of fired ones); an entry that does not fit is recomputed, and a fired
+ * one that does not fit sets *redo (the caller scores again with host-built lists).
  */
-void plain_rescore_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, const int *dcount, int bound,
-                          int *keep, double *lin, int *done, int *redo)
+void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, const int *dcount, int bound,
+                        int *keep, double *lin, double *ybx, int *done, int *redo, bool narrow, int phase)
 {
     PendingScope pending_scope(ctx);
     if (bound <= 0) return;
@@ -857,12 +868,18 @@ void plain_rescore_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, c
     require(mmax <= 256, "the FP64 scoring path supports up to 256 basis functions");
     const int mt = (int)std::min<size_t>(16, (mmax + 15) / 16), ldT = 16 * mt;
     const size_t B = (size_t)bound;
+    const bool ident = list == nullptr;
+    const bool none_stored = keep == nullptr;
+    const int tileT = narrow ? 16 : 16 * cg_tgemm(mt), tileP = narrow ? 16 : 16 * cg_project(mt);
 
     // buffers, sized for the bound (no reallocation once the pipeline is queued)
     Cands &rc = ctx->rcand;
-    rc.d_fit.ensure(B);
-    rc.d_xc.ensure(B);
-    rc.d_par.ensure(B);
+    if (!ident) {
+        rc.d_fit.ensure(B);
+        rc.d_xc.ensure(B);
+        rc.d_par.ensure(B);
+    }
+    const Cands &sc = ident ? cs : rc;  // the candidates as the passes see them
     ctx->roff.ensure(B);
     ctx->T.ensure(B * ldT);
     ctx->T2.ensure(B * ldT);
@@ -875,18 +892,19 @@ void plain_rescore_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, c
     ctx->live_list.ensure(B);
     ctx->dl_int.ensure(6 * (size_t)F + 16 + 2 * ((B + kCompactTile - 1) / kCompactTile + 1));
     ctx->dl_u64.ensure(2);
-    const size_t capT = (size_t)kPassItems + B / 16 + F + 64;   // T-GEMM / projection (narrow tiles)
+    const size_t capT = (size_t)kPassItems + B / 16 + F + 64;   // T-GEMM
+    const size_t capP = (size_t)kPassItems + B / 16 + F + 64;   // projection
     const size_t capF = 2048 + B / 16 + F + 64;                 // finalize
     const size_t capR = 4096 + B / 16 + F + 64;                 // DGKS retry passes
     const size_t capL = (size_t)kPassItems + B / 16 + F + 64;   // final pass of unstored entries
-    ctx->d_items.ensure(capT + capF);
+    ctx->d_items.ensure(capT + capP + capF);
     ctx->d_ritems.ensure(2 * capR + capF);
     ctx->d_litems.ensure(capL);
-    ctx->d_cmap.ensure(6 * B + 6);
+    ctx->d_cmap.ensure(9 * B + 9);
     ctx->d_rcmap.ensure(9 * B + 9);
     ctx->d_lcmap.ensure(3 * B + 3);
     ctx->tpart.ensure(std::max(capT, capR) * kSlots * ldT);
-    ctx->spart.ensure(std::max(std::max(capT, capR) * kSlots, std::max(capF, capL) * kTileF) + 1);
+    ctx->spart.ensure(std::max(std::max(capP, capR) * kSlots, std::max(capF, capL) * kTileF) + 1);
     const size_t arena = kRescoreArena / sizeof(float);
     ctx->rarena.ensure(arena);
 
@@ -894,17 +912,18 @@ void plain_rescore_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, c
     int *rfirst = ip, *rcnt = ip + F, *ffirst = ip + 2 * F, *fcnt = ip + 3 * F, *lfirst = ip + 4 * F,
         *lcnt = ip + 5 * F;
     int *nT = ip + 6 * F, *nF = nT + 1, *nRT = nT + 2, *nRP = nT + 3, *nFF = nT + 4, *nL = nT + 5, *dnf = nT + 6,
-        *dnl = nT + 7;
+        *dnl = nT + 7, *nP = nT + 8, *dall = nT + 9;
     int *blk = nT + 16;
     unsigned long long *used = ctx->dl_u64.p;
     DevPass pT{ctx->d_items.p, ctx->d_cmap.p, nT, (int)capT};
-    DevPass pF{ctx->d_items.p + capT, ctx->d_cmap.p + 3 * B + 3, nF, (int)capF};
+    DevPass pP{ctx->d_items.p + capT, ctx->d_cmap.p + 3 * B + 3, nP, (int)capP};
+    DevPass pF{ctx->d_items.p + capT + capP, ctx->d_cmap.p + 6 * B + 6, nF, (int)capF};
     DevPass rT{ctx->d_ritems.p, ctx->d_rcmap.p, nRT, (int)capR};
     DevPass rP{ctx->d_ritems.p + capR, ctx->d_rcmap.p + 3 * B + 3, nRP, (int)capR};
     DevPass fF{ctx->d_ritems.p + 2 * capR, ctx->d_rcmap.p + 6 * B + 6, nFF, (int)capF};
     DevPass pL{ctx->d_litems.p, ctx->d_lcmap.p, nL, (int)capL};
 
-    // fits as seen by the rescoring: residual columns at per-entry arena offsets
+    // fits as seen by these passes: residual columns at per-entry arena offsets
     {
         std::vector<FitDev> h((size_t)F);
         for (int f = 0; f < F; ++f) {
@@ -930,13 +949,18 @@ void plain_rescore_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, c
             d.m = (int)fs.m;
             d.c0 = 0;
             d.rbyl = 2;
+            d.px = (int)fs.px;
+            d.act = f < (int)ctx->act_h.size() ? ctx->act_h[f] : 1;
             d.roff = ctx->roff.p;
             d.tol = fs.tol;
         }
         ctx->d_fits2.ensure((size_t)F);
-        h2d_defer(ctx, ctx->d_fits2.p, h.data(), h.size() * sizeof(FitDev));
-        h2d_flush(ctx);
+        if (phase != 2) {  // (phase 2 may be captured into a graph: nothing but kernel launches)
+            h2d_defer(ctx, ctx->d_fits2.p, h.data(), h.size() * sizeof(FitDev));
+            h2d_flush(ctx);
+        }
     }
+    if (phase == 1) return;
     const FitDev *fits = ctx->d_fits2.p;
     const dim3 gB((unsigned)std::min<size_t>((B + 255) / 256, kRedGrid)), b256(256);
     const dim3 gF((unsigned)((F + 255) / 256));
@@ -947,67 +971,100 @@ void plain_rescore_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, c
                            (long long)kMaxSplits, align, p.items, p.cmap, p.cap, p.n);
         MH_LAUNCH_CHECK();
     };
-    Uploaded uT{pT.items, pT.cmap}, uF{pF.items, pF.cmap}, vT{rT.items, rT.cmap}, vP{rP.items, rP.cmap},
+    Uploaded uT{pT.items, pT.cmap}, uP{pP.items, pP.cmap}, vT{rT.items, rT.cmap}, vP{rP.items, rP.cmap},
         wF{fF.items, fF.cmap}, uL{pL.items, pL.cmap};
-    const int gT = (int)std::min<size_t>(capT, 65536), gR = (int)std::min<size_t>(capR, 65536),
-              gL = (int)std::min<size_t>(capL, 65536);
+    const int gT = (int)std::min<size_t>(capT, 65536), gP = (int)std::min<size_t>(capP, 65536),
+              gR = (int)std::min<size_t>(capR, 65536), gL = (int)std::min<size_t>(capL, 65536);
+    double *rl = ident ? lin : ctx->rlin.p;   // scores by entry
+    double *ry = ident && ybx ? ybx : ctx->rybx.p;
+    const int *cnt_dev = dcount;
+    if (ident && !dcount) {  // the count is the host's: written on the device so every kernel reads it alike
+        hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, st, dall, 1, bound);
+        MH_LAUNCH_CHECK();
+        cnt_dev = dall;
+    }
+    double flops = 0, bytes = 0;
+    if (ident) {
+        for (int f = 0; f < F; ++f) {
+            const FitState &fs = ctx->fits[f];
+            const double nf = (double)fs.n;
+            flops += 2.0 * nf * fs.m * (f < (int)cs.fit_P.size() ? cs.fit_P[f] : 0);
+            bytes += 4.0 * nf * (2.0 * fs.m + 1);
+        }
+    }
 
     // candidates of the list, their ranges, residual columns
-    hipLaunchKernelGGL(k_rc_gather, gB, b256, 0, st, list, dcount, (const int *)cs.d_fit.p, (const int *)cs.d_xc.p,
-                       (const int *)cs.d_par.p, rc.d_fit.p, rc.d_xc.p, rc.d_par.p);
-    MH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_list_ranges, gF, b256, 0, st, (const int *)rc.d_fit.p, nullptr, dcount, F, rfirst, rcnt);
+    if (!ident) {
+        hipLaunchKernelGGL(k_rc_gather, gB, b256, 0, st, list, dcount, (const int *)cs.d_fit.p,
+                           (const int *)cs.d_xc.p, (const int *)cs.d_par.p, rc.d_fit.p, rc.d_xc.p, rc.d_par.p);
+        MH_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_list_ranges, gF, b256, 0, st, (const int *)sc.d_fit.p, nullptr, cnt_dev, F, rfirst, rcnt);
     MH_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, used, 1);
     MH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_alloc_kept, gB, b256, 0, st, fits, (const int *)rc.d_fit.p, dcount, keep, ctx->roff.p, used,
-                       (unsigned long long)(arena / 2));
+    hipLaunchKernelGGL(k_alloc_kept, gB, b256, 0, st, fits, (const int *)sc.d_fit.p, cnt_dev, keep, ctx->roff.p,
+                       used, (unsigned long long)(arena / 2));
     MH_LAUNCH_CHECK();
-    build(pT, rfirst, rcnt, 16, kPassItems, 256, kWR);
-    build(pF, rfirst, rcnt, 16, 2048, 512, 32);
+    build(pT, rfirst, rcnt, tileT, kPassItems, 256, kWR);
+    if (tileP != tileT) build(pP, rfirst, rcnt, tileP, kPassItems, 256, kWR);
+    const DevPass &pPr = tileP != tileT ? pP : pT;
+    const Uploaded &uPr = tileP != tileT ? uP : uT;
+    const int gPr = tileP != tileT ? gP : gT;
+    if (!none_stored) build(pF, rfirst, rcnt, 16, 2048, 512, 32);
 
     // T = Bo^T Bx, s = sum r^2 (residuals stored for the entries kept), DGKS flags
     {
-        KTimer kt(ctx, K_TGEMM, 0.0, 0.0);
-        launch_tgemm<false>(ctx, mt, uT, gT, rc, nullptr, ldT, true, nT, fits);
+        KTimer kt(ctx, K_TGEMM, flops, bytes);
+        launch_tgemm<false>(ctx, mt, uT, gT, sc, nullptr, ldT, narrow, nT, fits);
         hipLaunchKernelGGL(k_treduce, dim3((unsigned)std::min<size_t>(B, 4 * kRedGrid)), dim3(64), 0, st, fits,
-                           (const int *)rc.d_fit.p, nullptr, bound, (const int *)pT.cmap, (const double *)ctx->tpart.p,
-                           ldT, ctx->T.p, ctx->tnorm.p, dcount);
+                           (const int *)sc.d_fit.p, nullptr, bound, (const int *)pT.cmap, (const double *)ctx->tpart.p,
+                           ldT, ctx->T.p, ctx->tnorm.p, cnt_dev);
         MH_LAUNCH_CHECK();
     }
     {
-        KTimer kt(ctx, K_PROJECT, 0.0, 0.0);
-        launch_project<kStoreR, false, false>(ctx, mt, uT, gT, rc, nullptr, ctx->T.p, ldT, true, keep, nT, fits);
-        hipLaunchKernelGGL(k_sreduce, gB, b256, 0, st, nullptr, bound, (const int *)pT.cmap, (const double *)ctx->spart.p,
-                           kSlots, (const double *)ctx->tnorm.p, fits, (const int *)rc.d_fit.p, (const int *)keep,
-                           ctx->ssum.p, ctx->flag.p, dcount);
+        KTimer kt(ctx, K_PROJECT, flops, bytes);
+        if (none_stored)
+            launch_project<kSumSq, false, false>(ctx, mt, uPr, gPr, sc, nullptr, ctx->T.p, ldT, narrow, nullptr,
+                                                 pPr.n, fits);
+        else
+            launch_project<kStoreR, false, false>(ctx, mt, uPr, gPr, sc, nullptr, ctx->T.p, ldT, narrow, keep,
+                                                  pPr.n, fits);
+        hipLaunchKernelGGL(k_sreduce, gB, b256, 0, st, nullptr, bound, (const int *)pPr.cmap,
+                           (const double *)ctx->spart.p, kSlots, (const double *)ctx->tnorm.p, fits,
+                           (const int *)sc.d_fit.p, (const int *)keep, ctx->ssum.p, ctx->flag.p, cnt_dev);
         MH_LAUNCH_CHECK();
     }
     // final pass over the stored residuals (unstored entries with s > tol get 0 here)
     {
         KTimer kt(ctx, K_FINALIZE, 0.0, 0.0);
-        hipLaunchKernelGGL(k_finalize_cm, dim3((unsigned)std::min<size_t>(capF, 4096)), b256, 0, st, fits,
-                           (const Item *)pF.items, (int)capF, nullptr, (const int *)keep, (const double *)ctx->ssum.p,
-                           ctx->spart.p, (const int *)nF);
-        MH_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_yreduce, gB, b256, 0, st, bound, (const int *)pF.cmap, (const double *)ctx->spart.p,
-                           nullptr, ctx->rybx.p, ctx->rlin.p, dcount);
-        MH_LAUNCH_CHECK();
+        if (none_stored) {
+            hipLaunchKernelGGL(k_zero2, dim3((unsigned)((B + 255) / 256)), b256, 0, st, bound, ry, rl);
+            MH_LAUNCH_CHECK();
+        } else {
+            hipLaunchKernelGGL(k_finalize_cm, dim3((unsigned)std::min<size_t>(capF, 4096)), b256, 0, st, fits,
+                               (const Item *)pF.items, (int)capF, nullptr, (const int *)keep,
+                               (const double *)ctx->ssum.p, ctx->spart.p, (const int *)nF);
+            MH_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_yreduce, gB, b256, 0, st, bound, (const int *)pF.cmap, (const double *)ctx->spart.p,
+                               nullptr, ry, rl, cnt_dev);
+            MH_LAUNCH_CHECK();
+        }
     }
     // DGKS retry of the fired entries (kernels.cc:387-399)
     {
         KTimer kt(ctx, K_DGKS, 0.0, 0.0);
-        hipLaunchKernelGGL(k_compact_count, dim3(nblk), b256, 0, st, dcount, (const int *)ctx->flag.p, 1, 1, blk);
+        hipLaunchKernelGGL(k_compact_count, dim3(nblk), b256, 0, st, cnt_dev, (const int *)ctx->flag.p, 1, 1, blk);
         MH_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_compact_write, dim3(nblk), b256, 0, st, dcount, (const int *)ctx->flag.p, 1, 1,
+        hipLaunchKernelGGL(k_compact_write, dim3(nblk), b256, 0, st, cnt_dev, (const int *)ctx->flag.p, 1, 1,
                            (const int *)blk, ctx->fired_list.p, dnf);
         MH_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_list_ranges, gF, b256, 0, st, (const int *)rc.d_fit.p, (const int *)ctx->fired_list.p,
+        hipLaunchKernelGGL(k_list_ranges, gF, b256, 0, st, (const int *)sc.d_fit.p, (const int *)ctx->fired_list.p,
                            (const int *)dnf, F, ffirst, fcnt);
         MH_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_count_add, dim3(1), dim3(64), 0, st, (const int *)dnf, ctx->dstat.p + 4);
         MH_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_alloc_fired, dim3(4), b256, 0, st, fits, (const int *)rc.d_fit.p,
+        hipLaunchKernelGGL(k_alloc_fired, dim3(4), b256, 0, st, fits, (const int *)sc.d_fit.p,
                            (const int *)ctx->fired_list.p, (const int *)dnf, ctx->roff.p, used,
                            (unsigned long long)arena, redo);
         MH_LAUNCH_CHECK();
@@ -1016,44 +1073,50 @@ void plain_rescore_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, c
         build(fF, ffirst, fcnt, 16, 2048, 512, 32);
         const int *fl = ctx->fired_list.p;
         // the first residual of every fired entry (recomputed where it was stored already)
-        launch_project<kStoreR, false, true>(ctx, mt, vP, gR, rc, fl, ctx->T.p, ldT, true, nullptr, nRP, fits);
-        launch_tgemm<true>(ctx, mt, vT, gR, rc, fl, ldT, true, nRT, fits);
-        hipLaunchKernelGGL(k_treduce, dim3(256), dim3(64), 0, st, fits, (const int *)rc.d_fit.p, fl, bound,
+        launch_project<kStoreR, false, true>(ctx, mt, vP, gR, sc, fl, ctx->T.p, ldT, true, nullptr, nRP, fits);
+        launch_tgemm<true>(ctx, mt, vT, gR, sc, fl, ldT, true, nRT, fits);
+        hipLaunchKernelGGL(k_treduce, dim3(256), dim3(64), 0, st, fits, (const int *)sc.d_fit.p, fl, bound,
                            (const int *)rT.cmap, (const double *)ctx->tpart.p, ldT, ctx->T2.p, nullptr,
                            (const int *)dnf);
         MH_LAUNCH_CHECK();
-        launch_project<kStoreR, true, false>(ctx, mt, vP, gR, rc, fl, ctx->T2.p, ldT, true, nullptr, nRP, fits);
+        launch_project<kStoreR, true, false>(ctx, mt, vP, gR, sc, fl, ctx->T2.p, ldT, true, nullptr, nRP, fits);
         hipLaunchKernelGGL(k_sreduce, dim3(64), b256, 0, st, fl, bound, (const int *)rP.cmap,
-                           (const double *)ctx->spart.p, kSlots, nullptr, fits, (const int *)rc.d_fit.p, nullptr,
+                           (const double *)ctx->spart.p, kSlots, nullptr, fits, (const int *)sc.d_fit.p, nullptr,
                            ctx->ssum.p, nullptr, (const int *)dnf);
         MH_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_finalize_cm, dim3(256), b256, 0, st, fits, (const Item *)fF.items, (int)capF, fl, nullptr,
                            (const double *)ctx->ssum.p, ctx->spart.p, (const int *)nFF);
         MH_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_yreduce, dim3(64), b256, 0, st, bound, (const int *)fF.cmap, (const double *)ctx->spart.p,
-                           fl, ctx->rybx.p, ctx->rlin.p, (const int *)dnf);
+                           fl, ry, rl, (const int *)dnf);
         MH_LAUNCH_CHECK();
     }
     // entries with s > tol whose residual was not stored and that did not fire: recompute r
     {
         KTimer kt(ctx, K_FINALIZE, 0.0, 0.0);
-        hipLaunchKernelGGL(k_compact_count, dim3(nblk), b256, 0, st, dcount, (const int *)ctx->flag.p, 7, 6, blk);
+        const int mask = none_stored ? 3 : 7, want = none_stored ? 2 : 6;
+        hipLaunchKernelGGL(k_compact_count, dim3(nblk), b256, 0, st, cnt_dev, (const int *)ctx->flag.p, mask, want,
+                           blk);
         MH_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_compact_write, dim3(nblk), b256, 0, st, dcount, (const int *)ctx->flag.p, 7, 6,
+        hipLaunchKernelGGL(k_compact_write, dim3(nblk), b256, 0, st, cnt_dev, (const int *)ctx->flag.p, mask, want,
                            (const int *)blk, ctx->live_list.p, dnl);
         MH_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_list_ranges, gF, b256, 0, st, (const int *)rc.d_fit.p, (const int *)ctx->live_list.p,
+        hipLaunchKernelGGL(k_list_ranges, gF, b256, 0, st, (const int *)sc.d_fit.p, (const int *)ctx->live_list.p,
                            (const int *)dnl, F, lfirst, lcnt);
         MH_LAUNCH_CHECK();
-        build(pL, lfirst, lcnt, 16, kPassItems, 256, kWR);
-        launch_project<kFinalY, false, true>(ctx, mt, uL, gL, rc, ctx->live_list.p, ctx->T.p, ldT, true, nullptr, nL,
-                                             fits);
+        // the live entries of a full scoring are most of the list: they take the wide tiles
+        const int tileL = none_stored ? tileP : 16;
+        build(pL, lfirst, lcnt, tileL, kPassItems, 256, kWR);
+        launch_project<kFinalY, false, true>(ctx, mt, uL, gL, sc, ctx->live_list.p, ctx->T.p, ldT, tileL == 16,
+                                             nullptr, nL, fits);
         hipLaunchKernelGGL(k_yreduce, gB, b256, 0, st, bound, (const int *)pL.cmap, (const double *)ctx->spart.p,
-                           (const int *)ctx->live_list.p, ctx->rybx.p, ctx->rlin.p, (const int *)dnl);
+                           (const int *)ctx->live_list.p, ry, rl, (const int *)dnl);
         MH_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_put_dev, gB, b256, 0, st, list, dcount, (const double *)ctx->rlin.p, lin, done);
-    MH_LAUNCH_CHECK();
+    if (!ident) {
+        hipLaunchKernelGGL(k_put_dev, gB, b256, 0, st, list, dcount, (const double *)rl, lin, done);
+        MH_LAUNCH_CHECK();
+    }
 }
 
 void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, double *ybx,

@@ -426,18 +426,16 @@ def fit_batch(Xs, ys, ws=None, **kwargs):
         batch = marslib.Batch(Xs, ys, ws, int(max_terms), precision=precision)
     batch.set_timing(bool(timing))
     t0 = time.perf_counter()
-    rows, steps = batch.fit(int(max_degree), float(penalty), bool(self_interact), int(r2_window), float(r2_thresh))
+    rec, counts, mse, steps = batch.fit(int(max_degree), float(penalty), bool(self_interact), int(r2_window),
+                                        float(r2_thresh))
     dt = time.perf_counter() - t0
-    models = []
-    for r in rows:
-        model = np.zeros(len(r), dtype=MODEL_DTYPE)
-        for i, (t, b, x, o, mse, r2, r2cv) in enumerate(r):
-            model[i] = (t, b, x, np.nan, np.nan, r2, r2cv, o, 0.0)
-        models.append(model)
+    rows = np.frombuffer(rec, dtype=np.dtype(MODEL_DTYPE))
+    ends = np.cumsum(counts)
+    models = [m.copy() for m in np.split(rows, ends[:-1])]
     if stats is not None:
         stats.update(batch.stats())
         stats["steps"] = steps
         stats["seconds"] = dt
-        stats["mse"] = [[row[4] for row in r[1:]] for r in rows]
+        stats["mse"] = [list(v[1:]) for v in np.split(mse, ends[:-1])]
         stats["kernels"] = batch.kernel_stats()
     return models

@@ -412,3 +412,27 @@ def test_mixed_resolves_rivals_below_fp32_error(eps):
         assert sa == sb, (i, sa, sb)
         assert {0, 1} & {int(r["input"]) for r in a[1:2]}  # the first pick is one of the rivals
     assert sm["window_violations"] == 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_graph_replays_match_direct_launches(precision, monkeypatch):
+    """The batch step's scoring runs as a captured graph replayed every step (eager work in
+    between); the same fits with direct launches give bitwise the same terms and mse."""
+    import mars_amd
+
+    say("graph vs direct", precision)
+    golds = [json.load(open(os.path.join(HERE, "golden", "w4_fit%d.json" % i))) for i in range(8)]
+    inputs = [make_inputs(CASES[g["name"]]) for g in golds]
+    Xs, ys = [x for x, _, _ in inputs], [y for _, y, _ in inputs]
+    runs = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("MARS_HIP_GRAPH", mode)
+        st = {}
+        models = mars_amd.fit_batch(Xs, ys, max_terms=33, precision=precision, stats=st)
+        runs[mode] = ([[(r["type"], int(r["basis"]), int(r["input"])) for r in m] for m in models], st)
+    (sa, a), (sb, b) = runs["1"], runs["0"]
+    assert sa == sb
+    assert a["mse"] == b["mse"]  # bitwise
+    assert a["graph_captures"] >= 1 and a["graph_launches"] == a["steps"]
+    assert a["graph_captures"] <= 3  # one per basis-size class (16 columns each)
+    assert b["graph_launches"] == 0
