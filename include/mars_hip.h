@@ -212,6 +212,28 @@ int mars_hip_score_stats(mars_hip_ctx *ctx, long *scored, long *rescored, long *
 int mars_hip_check_stats(mars_hip_ctx *ctx, long *window_violations, long *host_redo_steps, long *graph_captures,
                          long *graph_launches);
 
+/* Launch shapes of a context, tunable per workload (mars_amd/autotune.py writes
+ * the table the Python layer applies).  None of them changes a result beyond
+ * the summation order of FP64 sums.  No counterpart in the reference (its
+ * tile and budget knobs are environment variables, orthonormalize.cu:476-501). */
+typedef struct mars_hip_tuning {
+    double pass_items;    /* work items per FP64 T-GEMM / projection pass (default 32768) */
+    double retry_items;   /* work items per DGKS-retry pass (default 4096) */
+    double update_waves;  /* waves per mixed-form cache update (default 16384) */
+    double sweep_waves;   /* waves per hinge-sweep pass (default 65536) */
+    long sweep_min_rows;  /* sorted rows per hinge-sweep chunk, at least (default 512) */
+    long append_rows;     /* rows per append block; 0: 8192 from 64 fits on, else 4096 */
+    int narrow_avg;       /* 16-candidate tiles when fits average at most this many (default 24) */
+    int pad;
+} mars_hip_tuning;
+
+int mars_hip_default_tuning(mars_hip_tuning *out);
+int mars_hip_get_tuning(mars_hip_ctx *ctx, mars_hip_tuning *out);
+/* Validated: pass_items, retry_items, update_waves, sweep_waves in [256, 1e6];
+ * sweep_min_rows in [32, 1 << 20]; append_rows 0 or a multiple of 256 in
+ * [256, 1 << 20]; narrow_avg in [0, 4096]. */
+int mars_hip_set_tuning(mars_hip_ctx *ctx, const mars_hip_tuning *t);
+
 /* Per-kernel-class timing with HIP events on the context's stream (off by
  * default).  Each record: launches, summed device milliseconds and the
  * summed algorithmic flops / HBM bytes of those launches. */

@@ -320,7 +320,7 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
                 mixed_score(ctx, !device_lists, dP, (int)(device_lists ? Pl : P));
             } else if (device_lists) {
                 // every list after the DGKS gate (retries, the final pass) is built on the device
-                const bool narrow = narrow_tiles(cs.fit_P);
+                const bool narrow = narrow_tiles(ctx, cs.fit_P);
                 plain_score_device(ctx, cs, nullptr, dP, (int)Pl, nullptr, ctx->lin.p, ctx->ybx.p, nullptr,
                                    ctx->dredo.p, narrow, 1);
                 auto body = [&] {
@@ -342,6 +342,7 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
                     std::vector<unsigned long long> key = {2ull, (unsigned long long)std::min<size_t>(16, (mmax + 15) / 16),
                                                            (unsigned long long)narrow, (unsigned long long)Pl,
                                                            (unsigned long long)F};
+                    tuning_key(ctx, key);
                     graph_key_buffers(ctx, key);
                     KTimer kt(ctx, K_GRAPH_PLAIN, flops, bytes);
                     run_graph(ctx, key, body);

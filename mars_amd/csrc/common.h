@@ -297,6 +297,7 @@ struct mars_hip_ctx {
     int kw = 1;
     std::vector<int> act_h;          // the active flags the last upload_fits sent
     mh::DBuf<int> dcand;             // batch driver: device count of the step's candidates
+    mars_hip_tuning tune{32768.0, 4096.0, 16384.0, 65536.0, 512, 0, 24, 0};  // launch shapes
     bool use_graphs = false;         // batch driver: replay captured graphs of the scoring launches
     std::map<std::vector<unsigned long long>, hipGraphExec_t> graphs;  // by run_graph key
     long graph_captures = 0, graph_launches = 0;

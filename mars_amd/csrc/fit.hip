@@ -772,6 +772,16 @@ void run_graph(mars_hip_ctx *ctx, const std::vector<unsigned long long> &key, co
     ctx->graph_launches += 1;
 }
 
+void tuning_key(const mars_hip_ctx *ctx, std::vector<unsigned long long> &key)
+{
+    const mars_hip_tuning &t = ctx->tune;
+    for (double v : {t.pass_items, t.retry_items, t.update_waves, t.sweep_waves})
+        key.push_back(__builtin_bit_cast(unsigned long long, v));
+    key.push_back((unsigned long long)t.sweep_min_rows);
+    key.push_back((unsigned long long)t.append_rows);
+    key.push_back((unsigned long long)t.narrow_avg);
+}
+
 void graph_key_buffers(mars_hip_ctx *ctx, std::vector<unsigned long long> &key)
 {
     const void *p[] = {ctx->cand.d_fit.p, ctx->cand.d_xc.p, ctx->cand.d_par.p, ctx->rcand.d_fit.p, ctx->rcand.d_xc.p,
@@ -949,7 +959,7 @@ void fit_append_batch(mars_hip_ctx *ctx, std::vector<AppendArg> &args, std::vect
     hipStream_t st = ctx->stream;
     // rows per block: 8192 for large batches (4096 and 16384 measured slower on 256 fits), 4096
     // when few fits must fill the device
-    const size_t rpb = F >= 64 ? 8192 : 4096;
+    const size_t rpb = ctx->tune.append_rows > 0 ? (size_t)ctx->tune.append_rows : (F >= 64 ? 8192 : 4096);
     const int nb = (int)std::min<size_t>(256, std::max<size_t>(1, (nmax + rpb - 1) / rpb));
     Cands none;
     none.fit_c0.assign(F, 0);
@@ -1759,6 +1769,40 @@ int mars_hip_stats(mars_hip_ctx *ctx, long *dgks_fired, double *eval_ms)
         read_dstat(ctx, d);
         if (dgks_fired) *dgks_fired = ctx->dgks_host + (long)d[4];
         if (eval_ms) *eval_ms = ctx->eval_ms;
+    });
+}
+
+int mars_hip_default_tuning(mars_hip_tuning *out)
+{
+    return guarded([&] {
+        require(out != nullptr, "null argument");
+        *out = mars_hip_ctx().tune;
+    });
+}
+
+int mars_hip_get_tuning(mars_hip_ctx *ctx, mars_hip_tuning *out)
+{
+    return guarded([&] {
+        require(ctx != nullptr && out != nullptr, "null argument");
+        *out = ctx->tune;
+    });
+}
+
+int mars_hip_set_tuning(mars_hip_ctx *ctx, const mars_hip_tuning *t)
+{
+    return guarded([&] {
+        require(ctx != nullptr && t != nullptr, "null argument");
+        auto in = [](double v, double lo, double hi) { return v >= lo && v <= hi; };
+        require(in(t->pass_items, 256, 1e6) && in(t->retry_items, 256, 1e6) && in(t->update_waves, 256, 1e6) &&
+                    in(t->sweep_waves, 256, 1e6),
+                "tuning: item and wave targets must be in [256, 1e6]");
+        require(t->sweep_min_rows >= 32 && t->sweep_min_rows <= (1l << 20), "tuning: sweep_min_rows out of range");
+        require(t->append_rows == 0 || (t->append_rows % 256 == 0 && t->append_rows >= 256 &&
+                                        t->append_rows <= (1l << 20)),
+                "tuning: append_rows must be 0 or a multiple of 256 in [256, 2^20]");
+        require(t->narrow_avg >= 0 && t->narrow_avg <= 4096, "tuning: narrow_avg out of range");
+        ctx->tune = *t;
+        ctx->tune.pad = 0;
     });
 }
 

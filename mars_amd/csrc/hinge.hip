@@ -644,10 +644,11 @@ void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span)
     const int NC = maxnp <= 8 ? 8 : (maxnp <= 16 ? 16 : 32);
     const int pgroups = (maxnp + NC - 1) / NC;
     const int wpb = NC / (64 / (PT / (PT == 32 ? 4 : 8)));   // waves per block
-    // about 65536 waves, chunks of at least 512 sorted rows; a multiple of 8
+    // about sweep_waves waves (65536), chunks of at least sweep_min_rows (512) sorted rows; a multiple of 8
     // chunks puts the blocks of one (column, chunk) on one XCD (shared L2)
-    const int want = std::max(1, (65536 + ng * pgroups * wpb - 1) / (ng * pgroups * wpb));
-    const int maxc = (int)std::max<size_t>(1, tail / 512);
+    const int waves = (int)ctx->tune.sweep_waves;
+    const int want = std::max(1, (waves + ng * pgroups * wpb - 1) / (ng * pgroups * wpb));
+    const int maxc = (int)std::max<size_t>(1, tail / (size_t)ctx->tune.sweep_min_rows);
     int C = std::min(want, maxc);
     if (C >= 8) C -= C % 8;
     const size_t clen = (tail + C - 1) / C;

@@ -50,10 +50,11 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
  * captured the first time `key` is seen and replayed afterwards; directly when graphs are off.
  * The key must name every kernel argument the launches take (pointers, sizes, variants). */
 void run_graph(mars_hip_ctx *ctx, const std::vector<unsigned long long> &key, const std::function<void()> &launches);
-/* Key words for run_graph: the addresses of the context's scoring buffers. */
+/* Key words for run_graph: the addresses of the context's scoring buffers; the launch shapes. */
 void graph_key_buffers(mars_hip_ctx *ctx, std::vector<unsigned long long> &key);
+void tuning_key(const mars_hip_ctx *ctx, std::vector<unsigned long long> &key);
 /* True when the per-fit counts favour 16-candidate tiles. */
-bool narrow_tiles(const std::vector<int> &cnt);
+bool narrow_tiles(const mars_hip_ctx *ctx, const std::vector<int> &cnt);
 
 /* Mixed-precision scoring of ctx->cand (linear delta-SSE only) from each
  * fit's incrementally maintained T cache (FP32 MFMA, FP64 across row chunks),

@@ -41,6 +41,37 @@ int parse_form(const std::string &f)
     throw std::runtime_error("unknown precision form '" + f + "' (expected 'fp64' or 'mixed')");
 }
 
+py::dict tuning_dict(const mars_hip_tuning &t)
+{
+    py::dict d;
+    d["pass_items"] = t.pass_items;
+    d["retry_items"] = t.retry_items;
+    d["update_waves"] = t.update_waves;
+    d["sweep_waves"] = t.sweep_waves;
+    d["sweep_min_rows"] = t.sweep_min_rows;
+    d["append_rows"] = t.append_rows;
+    d["narrow_avg"] = t.narrow_avg;
+    return d;
+}
+
+/* Start from `base`, override the keys `d` names (unknown keys are an error). */
+mars_hip_tuning tuning_from(const mars_hip_tuning &base, const py::dict &d)
+{
+    mars_hip_tuning t = base;
+    for (auto kv : d) {
+        const std::string k = py::cast<std::string>(kv.first);
+        if (k == "pass_items") t.pass_items = py::cast<double>(kv.second);
+        else if (k == "retry_items") t.retry_items = py::cast<double>(kv.second);
+        else if (k == "update_waves") t.update_waves = py::cast<double>(kv.second);
+        else if (k == "sweep_waves") t.sweep_waves = py::cast<double>(kv.second);
+        else if (k == "sweep_min_rows") t.sweep_min_rows = py::cast<long>(kv.second);
+        else if (k == "append_rows") t.append_rows = py::cast<long>(kv.second);
+        else if (k == "narrow_avg") t.narrow_avg = py::cast<int>(kv.second);
+        else throw std::runtime_error("unknown tuning parameter '" + k + "'");
+    }
+    return t;
+}
+
 class MarsAlgo {
 public:
     /* new_algo (marslib.cc:12-37) + MarsAlgo::MarsAlgo (marsalgo.cc:266-370) on the device */
@@ -71,6 +102,21 @@ public:
     }
 
     ~MarsAlgo() { close(); }
+
+    py::dict get_tuning() const
+    {
+        mars_hip_tuning t;
+        check(mars_hip_get_tuning(ctx_, &t));
+        return tuning_dict(t);
+    }
+
+    void set_tuning(py::dict d)
+    {
+        mars_hip_tuning t;
+        check(mars_hip_get_tuning(ctx_, &t));
+        t = tuning_from(t, d);
+        check(mars_hip_set_tuning(ctx_, &t));
+    }
 
     /* Free the device state now (the object is unusable afterwards). */
     void close()
@@ -252,6 +298,21 @@ public:
     }
 
     ~Batch() { close(); }
+
+    py::dict get_tuning() const
+    {
+        mars_hip_tuning t;
+        check(mars_hip_get_tuning(ctx_, &t));
+        return tuning_dict(t);
+    }
+
+    void set_tuning(py::dict d)
+    {
+        mars_hip_tuning t;
+        check(mars_hip_get_tuning(ctx_, &t));
+        t = tuning_from(t, d);
+        check(mars_hip_set_tuning(ctx_, &t));
+    }
 
     /* Free the device state now (the object is unusable afterwards). */
     void close()
@@ -451,7 +512,9 @@ PYBIND11_MODULE(_marslib, m)
         .def("stats", &MarsAlgo::stats)
         .def("set_timing", &MarsAlgo::set_timing)
         .def("kernel_stats", &MarsAlgo::kernel_stats)
-        .def("close", &MarsAlgo::close);
+        .def("close", &MarsAlgo::close)
+        .def("get_tuning", &MarsAlgo::get_tuning)
+        .def("set_tuning", &MarsAlgo::set_tuning);
     py::class_<Batch>(m, "Batch")
         .def(py::init<py::list, py::list, py::object, int, const std::string &>(), py::arg("Xs"), py::arg("ys"),
              py::arg("ws") = py::none(), py::arg("max_terms"), py::arg("precision") = "fp64")
@@ -460,7 +523,14 @@ PYBIND11_MODULE(_marslib, m)
         .def("set_timing", &Batch::set_timing)
         .def("kernel_stats", &Batch::kernel_stats)
         .def("stats", &Batch::stats)
-        .def("close", &Batch::close);
+        .def("close", &Batch::close)
+        .def("get_tuning", &Batch::get_tuning)
+        .def("set_tuning", &Batch::set_tuning);
+    m.def("default_tuning", [] {
+        mars_hip_tuning t;
+        check(mars_hip_default_tuning(&t));
+        return tuning_dict(t);
+    });
     m.def(
         "prune_gram",
         [](py::array_t<double, py::array::c_style | py::array::forcecast> XX,

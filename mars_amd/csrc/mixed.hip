@@ -575,7 +575,7 @@ void tc_update(mars_hip_ctx *ctx)
             const int tiles = (2 * jf[f] + 1 + 15) / 16;
             bwork += (double)ctx->fits[f].n * ((tiles + W - 1) / W) * ((ctx->fits[f].px + kXT - 1) / kXT);
         }
-        const double target = 16384.0 / W;   // about 16384 waves (8192 measured 10% slower)
+        const double target = ctx->tune.update_waves / W;   // waves per launch (default 16384)
         std::vector<Item> items;
         std::vector<int> groups, glist;
         for (size_t f = 0; f < F; ++f) {
@@ -736,6 +736,7 @@ void mixed_score(mars_hip_ctx *ctx, bool host_lists, const int *dP, int Pb)
                                                (unsigned long long)P, (unsigned long long)F,
                                                (unsigned long long)policy};
         graph_key_buffers(ctx, key);
+        tuning_key(ctx, key);
         KTimer kt(ctx, K_RECHECK, 0.0, 0.0);
         run_graph(ctx, key, body);
         return;

@@ -25,9 +25,8 @@ namespace mh {
 namespace {
 
 constexpr int kTileF = 64;   // candidates per finalize item (x 4 row phases)
-// work items per projection / T-GEMM pass: about 32 one-wave items per SIMD, so a pass keeps
-// every CU busy however few candidates each fit has (8192 measured 10% slower on the mixed form)
-constexpr double kPassItems = 32768;
+// work items per projection / T-GEMM pass (ctx->tune.pass_items, default 32768): about 32 one-wave
+// items per SIMD, so a pass keeps every CU busy however few candidates each fit has
 constexpr long long kMaxSplits = 2048;  // row ranges per candidate tile at most
 // residual bytes that the plain form's DGKS retry keeps at once (one arena per context)
 constexpr size_t kRetryArena = size_t(2) << 30;
@@ -466,7 +465,7 @@ void reserve_step_buffers(mars_hip_ctx *ctx)
     P = std::min<size_t>(P, size_t(1) << 22);  // beyond this the buffers grow when needed
     const size_t F = ctx->fits.size();
     const size_t ldT = 16 * std::min<size_t>(16, (ctx->max_terms + 15) / 16);
-    const size_t items = 3 * ((size_t)kPassItems + P / 16 + F) + 1;  // the three passes' work lists
+    const size_t items = 3 * ((size_t)ctx->tune.pass_items + P / 16 + F) + 1;  // the three passes' work lists
     for (Cands *c : {&ctx->cand, &ctx->rcand}) {
         c->d_fit.ensure(P);
         c->d_xc.ensure(P);
@@ -486,19 +485,19 @@ namespace {
 /* A few candidates per fit (the mixed form's rescoring, DGKS retries) run on 16-wide candidate
  * tiles: the wide tiles would be mostly zero padding in the FP64 MFMA work that bounds these
  * kernels. */
-bool narrow_for(const std::vector<int> &c)
+bool narrow_for(const std::vector<int> &c, int avg)
 {
     long long tot = 0, act = 0;
     for (int v : c) {
         tot += v;
         act += v > 0;
     }
-    return tot <= 24 * act;
+    return tot <= (long long)avg * act;
 }
 
 } // namespace
 
-bool narrow_tiles(const std::vector<int> &cnt) { return narrow_for(cnt); }
+bool narrow_tiles(const mars_hip_ctx *ctx, const std::vector<int> &cnt) { return narrow_for(cnt, ctx->tune.narrow_avg); }
 
 namespace {
 
@@ -546,10 +545,10 @@ void retry_compact(mars_hip_ctx *ctx, Cands &cs, const std::vector<int> &fired, 
         ctx->rarena.ensure(std::max<size_t>(used, 1));
         upload_fits(ctx, cs, nullptr, nullptr, &off);
         h2d_defer(ctx, ctx->fired_list.p, fired.data() + k0, nf * sizeof(int));
-        const bool rnarrow = narrow_for(fcnt);
+        const bool rnarrow = narrow_for(fcnt, ctx->tune.narrow_avg);
         const int rtT = 16 * (rnarrow ? 1 : cg_tgemm(mt)), rtP = 16 * (rnarrow ? 1 : cg_project(mt));
-        Pass rT = make_pass(bb, be, n, split_rows(n, fcnt, rtT, 4096, 256), rtT, kWR);
-        Pass rP = make_pass(bb, be, n, split_rows(n, fcnt, rtP, 4096, 256), rtP, kWR);
+        Pass rT = make_pass(bb, be, n, split_rows(n, fcnt, rtT, ctx->tune.retry_items, 256), rtT, kWR);
+        Pass rP = make_pass(bb, be, n, split_rows(n, fcnt, rtP, ctx->tune.retry_items, 256), rtP, kWR);
         Pass fF = make_pass(bb, be, n, split_rows(n, fcnt, 16, 2048, 512), 16, 32);
         Uploaded vT, vP, wF;
         ctx->d_ritems.ensure(rT.items.size() + rP.items.size() + fF.items.size() + 1);
@@ -892,11 +891,11 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
     ctx->live_list.ensure(B);
     ctx->dl_int.ensure(6 * (size_t)F + 16 + 2 * ((B + kCompactTile - 1) / kCompactTile + 1));
     ctx->dl_u64.ensure(2);
-    const size_t capT = (size_t)kPassItems + B / 16 + F + 64;   // T-GEMM
-    const size_t capP = (size_t)kPassItems + B / 16 + F + 64;   // projection
+    const size_t capT = (size_t)ctx->tune.pass_items + B / 16 + F + 64;   // T-GEMM
+    const size_t capP = (size_t)ctx->tune.pass_items + B / 16 + F + 64;   // projection
     const size_t capF = 2048 + B / 16 + F + 64;                 // finalize
-    const size_t capR = 4096 + B / 16 + F + 64;                 // DGKS retry passes
-    const size_t capL = (size_t)kPassItems + B / 16 + F + 64;   // final pass of unstored entries
+    const size_t capR = (size_t)ctx->tune.retry_items + B / 16 + F + 64;  // DGKS retry passes
+    const size_t capL = (size_t)ctx->tune.pass_items + B / 16 + F + 64;   // final pass of unstored entries
     ctx->d_items.ensure(capT + capP + capF);
     ctx->d_ritems.ensure(2 * capR + capF);
     ctx->d_litems.ensure(capL);
@@ -1006,8 +1005,8 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
     hipLaunchKernelGGL(k_alloc_kept, gB, b256, 0, st, fits, (const int *)sc.d_fit.p, cnt_dev, keep, ctx->roff.p,
                        used, (unsigned long long)(arena / 2));
     MH_LAUNCH_CHECK();
-    build(pT, rfirst, rcnt, tileT, kPassItems, 256, kWR);
-    if (tileP != tileT) build(pP, rfirst, rcnt, tileP, kPassItems, 256, kWR);
+    build(pT, rfirst, rcnt, tileT, ctx->tune.pass_items, 256, kWR);
+    if (tileP != tileT) build(pP, rfirst, rcnt, tileP, ctx->tune.pass_items, 256, kWR);
     const DevPass &pPr = tileP != tileT ? pP : pT;
     const Uploaded &uPr = tileP != tileT ? uP : uT;
     const int gPr = tileP != tileT ? gP : gT;
@@ -1068,8 +1067,8 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
                            (const int *)ctx->fired_list.p, (const int *)dnf, ctx->roff.p, used,
                            (unsigned long long)arena, redo);
         MH_LAUNCH_CHECK();
-        build(rT, ffirst, fcnt, 16, 4096, 256, kWR);
-        build(rP, ffirst, fcnt, 16, 4096, 256, kWR);
+        build(rT, ffirst, fcnt, 16, ctx->tune.retry_items, 256, kWR);
+        build(rP, ffirst, fcnt, 16, ctx->tune.retry_items, 256, kWR);
         build(fF, ffirst, fcnt, 16, 2048, 512, 32);
         const int *fl = ctx->fired_list.p;
         // the first residual of every fired entry (recomputed where it was stored already)
@@ -1106,7 +1105,7 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
         MH_LAUNCH_CHECK();
         // the live entries of a full scoring are most of the list: they take the wide tiles
         const int tileL = none_stored ? tileP : 16;
-        build(pL, lfirst, lcnt, tileL, kPassItems, 256, kWR);
+        build(pL, lfirst, lcnt, tileL, ctx->tune.pass_items, 256, kWR);
         launch_project<kFinalY, false, true>(ctx, mt, uL, gL, sc, ctx->live_list.p, ctx->T.p, ldT, tileL == 16,
                                              nullptr, nL, fits);
         hipLaunchKernelGGL(k_yreduce, gB, b256, 0, st, bound, (const int *)pL.cmap, (const double *)ctx->spart.p,
@@ -1155,14 +1154,14 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     const int mt = (int)std::min<size_t>(16, (mmax + 15) / 16);
     require(mmax <= 256, "the FP64 scoring path supports up to 256 basis functions");
     const int ldT = mt * 16;
-    const bool narrow = narrow_for(cnt);
+    const bool narrow = narrow_for(cnt, ctx->tune.narrow_avg);
     const int tileT = 16 * (narrow ? 1 : cg_tgemm(mt)), tileP = 16 * (narrow ? 1 : cg_project(mt));
 
     // equal tile widths give the two passes the same work list: it is built and sent once
     const bool shared = tileT == tileP;
-    Pass pT = make_pass(fb, fe, n, split_rows(n, cnt, tileT, kPassItems, 256), tileT, kWR);
+    Pass pT = make_pass(fb, fe, n, split_rows(n, cnt, tileT, ctx->tune.pass_items, 256), tileT, kWR);
     Pass pPown;
-    if (!shared) pPown = make_pass(fb, fe, n, split_rows(n, cnt, tileP, kPassItems, 256), tileP, kWR);
+    if (!shared) pPown = make_pass(fb, fe, n, split_rows(n, cnt, tileP, ctx->tune.pass_items, 256), tileP, kWR);
     const Pass &pP = shared ? pT : pPown;
     Pass pF;  // finalize work list, over stored residuals (hinge path, rescoring)
     const int tileF = narrow ? 16 : kTileF;
@@ -1279,10 +1278,10 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         h2d_defer(ctx, ctx->fired_list.p, fired.data(), nf * sizeof(int));
         std::vector<int> fcnt(F);
         for (size_t f = 0; f < F; ++f) fcnt[f] = re[f] - rb[f];
-        const bool rnarrow = narrow_for(fcnt);
+        const bool rnarrow = narrow_for(fcnt, ctx->tune.narrow_avg);
         const int rtT = 16 * (rnarrow ? 1 : cg_tgemm(mt)), rtP = 16 * (rnarrow ? 1 : cg_project(mt));
-        Pass rT = make_pass(rb, re, n, split_rows(n, fcnt, rtT, 4096, 256), rtT, kWR);
-        Pass rP = make_pass(rb, re, n, split_rows(n, fcnt, rtP, 4096, 256), rtP, kWR);
+        Pass rT = make_pass(rb, re, n, split_rows(n, fcnt, rtT, ctx->tune.retry_items, 256), rtT, kWR);
+        Pass rP = make_pass(rb, re, n, split_rows(n, fcnt, rtP, ctx->tune.retry_items, 256), rtP, kWR);
         Uploaded vT, vP;
         ctx->d_ritems.ensure(rT.items.size() + rP.items.size() + 1);
         ctx->d_rcmap.ensure(rT.cmap.size() + rP.cmap.size() + 1);
@@ -1330,9 +1329,9 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
             KTimer kt(ctx, K_FINALIZE, 0.0, 0.0);
             std::vector<int> lcnt(F);
             for (size_t f = 0; f < F; ++f) lcnt[f] = te[f] - tb[f];
-            const bool lnarrow = narrow_for(lcnt);
+            const bool lnarrow = narrow_for(lcnt, ctx->tune.narrow_avg);
             const int ltP = 16 * (lnarrow ? 1 : cg_project(mt));
-            Pass pL = make_pass(tb, te, n, split_rows(n, lcnt, ltP, kPassItems, 256), ltP, kWR);
+            Pass pL = make_pass(tb, te, n, split_rows(n, lcnt, ltP, ctx->tune.pass_items, 256), ltP, kWR);
             Uploaded uL;
             ctx->live_list.ensure(nl);
             ctx->d_litems.ensure(pL.items.size() + 1);
@@ -1388,9 +1387,9 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
             if (nl > 0) {
                 std::vector<int> lcnt(F);
                 for (size_t f = 0; f < F; ++f) lcnt[f] = le[f] - lb[f];
-                const bool lnarrow = narrow_for(lcnt);
+                const bool lnarrow = narrow_for(lcnt, ctx->tune.narrow_avg);
                 const int ltP = 16 * (lnarrow ? 1 : cg_project(mt));
-                Pass pL = make_pass(lb, le, n, split_rows(n, lcnt, ltP, kPassItems, 256), ltP, kWR);
+                Pass pL = make_pass(lb, le, n, split_rows(n, lcnt, ltP, ctx->tune.pass_items, 256), ltP, kWR);
                 Uploaded uL;
                 ctx->live_list.ensure(nl);
                 ctx->d_litems.ensure(pL.items.size() + 1);

@@ -68,6 +68,9 @@ def fit(X, y, w=None, **kwargs):
         (chosen term, knot index, delta-SSEs, candidate count, append mse values).
     algo : a fresh marslib.MarsAlgo already built from (X, y, w) -- lets a caller
         upload the inputs to the GPU before timing the forward pass.
+    tuning : launch shapes of a context this call creates -- None (default) the
+        tuning table's entry for the workload (mars_amd/tuning.json), a dict
+        those shapes, False the library defaults.
     """
     if w is None:
         w = np.ones(len(X), dtype="f")
@@ -106,6 +109,7 @@ def fit(X, y, w=None, **kwargs):
     precision     = kwargs.pop("precision", "fp64")
     trace         = kwargs.pop("trace", None)
     algo          = kwargs.pop("algo", None)
+    tuning        = kwargs.pop("tuning", None)
     # fmt: on
     if kwargs:
         raise TypeError("unknown argument: %s" % kwargs)
@@ -135,6 +139,7 @@ def fit(X, y, w=None, **kwargs):
     tail = max(int(n * tail_span), 1)
     if algo is None:
         algo = marslib.MarsAlgo(X, y, w, int(max_terms), precision=precision)
+        _set_tuning(algo, tuning, "single", precision, not linear_only, n)
     elif algo.nbasis() != 1:
         raise ValueError("algo must be a fresh MarsAlgo built from the same X, y, w")
     else:
@@ -384,6 +389,19 @@ def pprint(model, beta=None, labels=None):
         print(text)
 
 
+def _set_tuning(ctx, tuning, kind, form, hinge, rows):
+    """Launch shapes of a new context: None takes the tuning table's entry for the workload class
+    (mars_amd/tuning.json), a dict sets those shapes, False keeps the context's."""
+    from . import tuning as table
+
+    if tuning is False:
+        return
+    if tuning is None:
+        table.apply(ctx, kind, form, hinge, rows)
+    else:
+        ctx.set_tuning(dict(tuning))
+
+
 def fit_batch(Xs, ys, ws=None, **kwargs):
     """
     Batching form: fit many independent problems (each its own X, y, w of any
@@ -396,7 +414,9 @@ def fit_batch(Xs, ys, ws=None, **kwargs):
     linear_only (must be True: the batch scores linear terms), stats (a dict
     that receives step count, DGKS retries and per-kernel timing when given),
     batch (a marslib.Batch already built on these same inputs, max_terms and
-    precision, so that device upload happens outside the call).
+    precision, so that device upload happens outside the call), tuning (None:
+    the launch shapes of the tuning table for this workload class; a dict:
+    those shapes; False: leave the context's shapes as they are).
     Returns one model array per problem, as `fit` would.
     """
     Xs = [np.asarray(X) for X in Xs]
@@ -418,12 +438,14 @@ def fit_batch(Xs, ys, ws=None, **kwargs):
     stats = kwargs.pop("stats", None)
     timing = kwargs.pop("timing", False)
     batch = kwargs.pop("batch", None)
+    tuning = kwargs.pop("tuning", None)
     if kwargs:
         raise TypeError("unknown argument: %s" % kwargs)
     if not linear_only:
         raise ValueError("fit_batch scores linear_only fits; use fit() for hinge sweeps")
     if batch is None:
         batch = marslib.Batch(Xs, ys, ws, int(max_terms), precision=precision)
+    _set_tuning(batch, tuning, "batch", precision, False, sum(len(X) for X in Xs))
     batch.set_timing(bool(timing))
     t0 = time.perf_counter()
     rec, counts, mse, steps = batch.fit(int(max_degree), float(penalty), bool(self_interact), int(r2_window),

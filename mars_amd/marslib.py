@@ -12,7 +12,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 
 try:
     from ._marslib import (  # noqa: F401
-        Batch, MarsAlgo, __version__, device_bytes, expand_f32, expand_f64, gram_f32, gram_f64, prune_gram,
+        Batch, MarsAlgo, __version__, default_tuning, device_bytes, expand_f32, expand_f64, gram_f32, gram_f64,
+        prune_gram,
     )
 except ImportError as e:  # pragma: no cover - exercised only on unbuilt trees
     raise ImportError(
