@@ -216,12 +216,13 @@ constexpr int kDStat = 8;  // device counters per context (mars_hip_ctx::dstat)
 
 /* Kernel classes timed by mars_hip_timing. */
 enum KClass { K_TGEMM = 0, K_PROJECT, K_FINALIZE, K_DGKS, K_SWEEP, K_SORT, K_APPEND, K_MIXED_A,
-              K_MIXED_B, K_RECHECK, K_ARGMAX, K_GRAPH_PLAIN,
+              K_MIXED_B, K_RECHECK, K_ARGMAX, K_GRAPH_PLAIN, K_TGEMM32, K_PROJECT32, K_HRECHECK,
               // host wall-clock sections of a batch step (include the waits they contain)
               K_H_CANDS, K_H_SCORE, K_H_ARGMAX, K_H_APPEND, K_H_STEP, K_NCLASS };
 constexpr const char *kClassName[K_NCLASS] = {"tgemm_f64", "project_f64", "finalize", "dgks_retry",
                                              "hinge_sweep", "sort", "append", "mixed_update",
                                              "mixed_score", "mixed_recheck", "argmax", "plain_graph",
+                                             "tgemm_f32", "project_f32", "hinge_recheck",
                                              "host_cands", "host_score", "host_argmax", "host_append",
                                              "host_step"};
 
@@ -289,6 +290,8 @@ struct mars_hip_ctx {
     mh::DBuf<unsigned long long> dstat;  // device counters (kDStat): [1] rescored, [2] gated,
                                          // [3] window violations, [4] DGKS retries of the rescoring
     long redo_steps = 0;             // batch steps scored again on the host-driven path
+    long hinge_rounds = 0;           // precision-form hinge evals that needed a further rescoring round
+    long window_misses = 0;          // ... candidates still approximate after the last round
     mh::DBuf<int> dredo;             // 1: the step's device-driven rescoring must be redone on the host
     // batch driver, candidates generated on the device: per fit and basis function its degree and
     // the bit set of X columns it contains (words of 32 bits, kw per basis function)
@@ -326,6 +329,8 @@ struct mars_hip_ctx {
     mh::DBuf<double> hstate;         // hinge chunk summaries / carries
     mh::DBuf<double> hbest;
     mh::DBuf<int> hbest_i;
+    mh::DBuf<double> hbest2;         // precision-form hinge eval: exact hinge scores of the rescored
+    mh::DBuf<int> rknot;             // ... and their knot positions and rows
 
     long dgks_host = 0;
     double eval_ms = 0;

@@ -842,7 +842,8 @@ void ctx_free(mars_hip_ctx *ctx)
     ctx->d_fits2.release();
     for (DBuf<int> *b : {&ctx->dl_int, &ctx->cgate, &ctx->cdone, &ctx->dredo, &ctx->bdeg, &ctx->dcand})
         b->release();
-    for (DBuf<double> *b : {&ctx->clo, &ctx->chi}) b->release();
+    for (DBuf<double> *b : {&ctx->clo, &ctx->chi, &ctx->hbest2}) b->release();
+    ctx->rknot.release();
     ctx->dl_u64.release();
     ctx->dstat.release();
     ctx->bhas.release();
@@ -1253,6 +1254,190 @@ void argmax_launch(mars_hip_ctx *ctx, const Cands &cs, const double *lin)
     }
 }
 
+/* ---------------------------------------------------------------------- */
+/* Precision form with hinges: exact rescoring of the near winners         */
+/* ---------------------------------------------------------------------- */
+constexpr double kHingeWindow = 2e-3;  // relative window around the approximate best scores
+
+/* mx[0] = largest linear, mx[1] = largest hinge score (one block). */
+__global__ __launch_bounds__(1024) void k_two_max(const double *__restrict__ lin, const double *__restrict__ hin,
+                                                  int P, double *__restrict__ mx)
+{
+    __shared__ double s1[1024], s2[1024];
+    const int t = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (int c = t; c < P; c += 1024) {
+        a = fmax(a, lin[c]);
+        b = fmax(b, hin[c]);
+    }
+    s1[t] = a;
+    s2[t] = b;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s) {
+            s1[t] = fmax(s1[t], s1[t + s]);
+            s2[t] = fmax(s2[t], s2[t + s]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        mx[0] = s1[0];
+        mx[1] = s2[0];
+    }
+}
+
+/* take[c] = 1 when the candidate's approximate linear or hinge score is within the window of the
+ * best approximate one, its DGKS gate fired (flag bit 0), or its residual energy is within a
+ * factor 1000 of the degeneracy threshold (the approximation may sit on either side of it). */
+__global__ void k_hsel(const double *__restrict__ lin, const double *__restrict__ hin, const int *__restrict__ flag,
+                       const double *__restrict__ ssum, double tol, int P, const double *__restrict__ mx, double w,
+                       int *__restrict__ take)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= P) return;
+    take[c] = ((flag[c] & 1) || ssum[c] <= 1e3 * tol || (mx[0] > 0 && lin[c] >= mx[0] * (1.0 - w)) ||
+               (mx[1] > 0 && hin[c] >= mx[1] * (1.0 - w)))
+                  ? 1
+                  : 0;
+}
+
+/* Exact values of the rescored candidates into the full arrays; they are marked done. */
+__global__ void k_hput(const int *__restrict__ idx, int R, const double *__restrict__ rl, const double *__restrict__ rh,
+                       const int *__restrict__ rk, const int *__restrict__ rr, double *__restrict__ lin,
+                       double *__restrict__ hin, int *__restrict__ knot, int *__restrict__ krow, int *__restrict__ done)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R) return;
+    const int c = idx[i];
+    lin[c] = rl[i];
+    hin[c] = rh[i];
+    knot[c] = rk[i];
+    krow[c] = rr[i];
+    done[c] = 1;
+}
+
+/* With approximate scores within a relative w / 2 of the exact ones (the window's premise), a
+ * candidate left approximate, below (1 - w) times the best approximate score, stays below the
+ * exact best by a factor 1 + w / 2.  take[c] = 1 for every approximate candidate that does not
+ * (it may beat the exact best: the approximation erred by more than w / 2 somewhere, as a noise
+ * column can); *more = their count.  One block. */
+__global__ __launch_bounds__(1024) void k_hmore(const double *__restrict__ lin, const double *__restrict__ hin,
+                                                const int *__restrict__ done, int P, double w, int *__restrict__ take,
+                                                int *__restrict__ more)
+{
+    __shared__ double s1[1024], s2[1024];
+    __shared__ int sv[1024];
+    const int t = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (int c = t; c < P; c += 1024)
+        if (done[c]) {
+            a = fmax(a, lin[c]);
+            b = fmax(b, hin[c]);
+        }
+    s1[t] = a;
+    s2[t] = b;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s) {
+            s1[t] = fmax(s1[t], s1[t + s]);
+            s2[t] = fmax(s2[t], s2[t + s]);
+        }
+        __syncthreads();
+    }
+    const double e1 = s1[0], e2 = s2[0];
+    int v = 0;
+    for (int c = t; c < P; c += 1024) {
+        const bool m = !done[c] && ((lin[c] > 0 && lin[c] * (1.0 + 0.5 * w) >= e1) ||
+                                    (hin[c] > 0 && hin[c] * (1.0 + 0.5 * w) >= e2));
+        take[c] = m ? 1 : 0;
+        v += m ? 1 : 0;
+    }
+    sv[t] = v;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s) sv[t] += sv[t + s];
+        __syncthreads();
+    }
+    if (t == 0) *more = sv[0];
+}
+
+/*
+ * Precision form of a hinge eval: the candidates' first projection runs on FP32 MFMA (DGKS retry,
+ * norms and the FP64 hinge sweep as in the plain form), then every candidate near the best
+ * approximate linear or hinge score, every one whose DGKS gate fired and every near-degenerate
+ * one is scored again by the plain form (FP64 projection and sweep), and its exact values replace
+ * the approximate ones.  Rounds of rescoring follow while an approximate score could still beat
+ * the exact best (k_hmore), so the step's winner and its scores are plain-form values.
+ */
+void hinge_eval_mixed(mars_hip_ctx *ctx, Cands &cs, int min_span, int end_span)
+{
+    hipStream_t st = ctx->stream;
+    const int P = (int)cs.fit.size();
+    const double tol = ctx->fits[0].tol;
+    plain_score(ctx, cs, true, ctx->lin.p, ctx->ybx.p, nullptr, nullptr, false, nullptr, true);
+    hinge_sweep(ctx, min_span, end_span);
+    KTimer kt(ctx, K_HRECHECK, 0.0, 0.0);
+    ctx->red.ensure(2);
+    ctx->rsel.ensure(P);
+    ctx->cdone.ensure(P);
+    ctx->dcand.ensure(2);
+    hipLaunchKernelGGL(k_two_max, dim3(1), dim3(1024), 0, st, (const double *)ctx->lin.p, (const double *)ctx->hin.p, P,
+                       ctx->red.p);
+    MH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_hsel, dim3((P + 255) / 256), dim3(256), 0, st, (const double *)ctx->lin.p,
+                       (const double *)ctx->hin.p, (const int *)ctx->flag.p, (const double *)ctx->ssum.p, tol, P,
+                       (const double *)ctx->red.p, kHingeWindow, ctx->rsel.p);
+    MH_LAUNCH_CHECK();
+    std::vector<int> zeros(P, 0);
+    h2d(ctx, ctx->cdone.p, zeros.data(), P * sizeof(int));
+    std::vector<int> take(P);
+    int more = 0;
+    for (int round = 0;; ++round) {
+        MH_CHECK(hipMemcpyAsync(take.data(), ctx->rsel.p, P * sizeof(int), hipMemcpyDeviceToHost, st));
+        ctx_sync(ctx);
+        std::vector<int> idx;
+        Cands &rc = ctx->rcand;
+        rc.fit.clear();
+        rc.xc.clear();
+        rc.par.clear();
+        for (int c = 0; c < P; ++c)
+            if (take[c]) {
+                idx.push_back(c);
+                rc.fit.push_back(0);
+                rc.xc.push_back(cs.xc[c]);
+                rc.par.push_back(cs.par[c]);
+            }
+        const int R = (int)idx.size();
+        if (R == 0) break;
+        if (round == 4) {  // (has not happened: the approximation errs beyond w / 2 only for noise columns)
+            ctx->window_misses += R;
+            break;
+        }
+        ctx->rescored += R;
+        ctx->rlin.ensure(R);
+        ctx->rybx.ensure(R);
+        ctx->hbest2.ensure(R);
+        ctx->rknot.ensure(2 * (size_t)R + 1);
+        ctx->live_list.ensure(R);
+        plain_score(ctx, rc, true, ctx->rlin.p, ctx->rybx.p);
+        hinge_sweep_list(ctx, rc, ctx->rlin.p, ctx->rybx.p, ctx->hbest2.p, ctx->rknot.p, ctx->rknot.p + R, min_span,
+                         end_span);
+        h2d(ctx, ctx->live_list.p, idx.data(), R * sizeof(int));
+        hipLaunchKernelGGL(k_hput, dim3((R + 255) / 256), dim3(256), 0, st, (const int *)ctx->live_list.p, R,
+                           (const double *)ctx->rlin.p, (const double *)ctx->hbest2.p, (const int *)ctx->rknot.p,
+                           (const int *)ctx->rknot.p + R, ctx->lin.p, ctx->hin.p, ctx->knot.p, ctx->krow.p,
+                           ctx->cdone.p);
+        MH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_hmore, dim3(1), dim3(1024), 0, st, (const double *)ctx->lin.p, (const double *)ctx->hin.p,
+                           (const int *)ctx->cdone.p, P, kHingeWindow, ctx->rsel.p, ctx->dcand.p);
+        MH_LAUNCH_CHECK();
+        MH_CHECK(hipMemcpyAsync(&more, ctx->dcand.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        ctx_sync(ctx);
+        if (more == 0) break;
+        ctx->hinge_rounds += 1;
+    }
+}
+
 void argmax_launch_ranges(mars_hip_ctx *ctx, const double *lin, size_t bound)
 {
     const size_t F = ctx->fits.size();
@@ -1276,8 +1461,7 @@ void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t 
     require(stride >= fs.m, "invalid mask row stride");
     require(min_span >= 1, "min_span must be >= 1");
     require(end_span >= 0 && (size_t)end_span < fs.n, "end_span must be in [0, n)");
-    require(linear_only || ctx->form == MARS_HIP_FORM_PLAIN,
-            "the mixed-precision form scores linear_only fits (use precision='fp64' for hinges)");
+
     const size_t m = fs.m, total = rows * m;
     hipStream_t st = ctx->stream;
 
@@ -1309,8 +1493,11 @@ void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t 
     if (P > 0) {
         ctx->lin.ensure(P);
         ctx->ybx.ensure(P);
-        if (ctx->form == MARS_HIP_FORM_MIXED) {
+        if (ctx->form == MARS_HIP_FORM_MIXED && linear_only) {
             mixed_score(ctx, true);  // one fit: the eval synchronises anyway
+        } else if (ctx->form == MARS_HIP_FORM_MIXED) {
+            require(ctx->row_major, "hinge sweeps need the row-major basis copies");
+            hinge_eval_mixed(ctx, cs, min_span, end_span);
         } else {
             require(linear_only || ctx->row_major, "hinge sweeps need the row-major basis copies");
             plain_score(ctx, cs, !linear_only, ctx->lin.p, ctx->ybx.p);
@@ -1754,7 +1941,7 @@ int mars_hip_check_stats(mars_hip_ctx *ctx, long *window_violations, long *host_
         require(ctx != nullptr, "null context");
         unsigned long long d[kDStat];
         read_dstat(ctx, d);
-        if (window_violations) *window_violations = (long)d[3];
+        if (window_violations) *window_violations = (long)d[3] + ctx->window_misses;
         if (host_redo_steps) *host_redo_steps = ctx->redo_steps;
         if (graph_captures) *graph_captures = ctx->graph_captures;
         if (graph_launches) *graph_launches = ctx->graph_launches;

@@ -610,15 +610,22 @@ static void ensure_sorted(mars_hip_ctx *ctx, int xc)
 
 void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span)
 {
-    FitState &fs = ctx->fits[0];
     Cands &cs = ctx->cand;
+    const size_t P = std::max<size_t>(cs.fit.size(), 1);
+    ctx->hin.ensure(P);
+    ctx->knot.ensure(P);
+    ctx->krow.ensure(P);
+    hinge_sweep_list(ctx, cs, ctx->lin.p, ctx->ybx.p, ctx->hin.p, ctx->knot.p, ctx->krow.p, min_span, end_span);
+}
+
+void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const double *ybx, double *hin, int *knot,
+                      int *krow, int min_span, int end_span)
+{
+    FitState &fs = ctx->fits[0];
     const int P = (int)cs.fit.size();
     const size_t n = fs.n;
     const int m = (int)fs.m;
     hipStream_t st = ctx->stream;
-    ctx->hin.ensure(P);
-    ctx->knot.ensure(P);
-    ctx->krow.ensure(P);
 
     // X-column groups of this block (candidates are contiguous per column)
     std::vector<int> grp, cgrp(P);
@@ -673,7 +680,7 @@ void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span)
     a.ldr = fs.ldr;
     a.n = n;
     a.ybo = fs.ybo.p;
-    a.ybx = ctx->ybx.p;
+    a.ybx = ybx;
     a.grp = dgrp.p;
     a.cpar = cs.d_par.p;
     a.order = fs.order.p;
@@ -712,8 +719,8 @@ void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span)
             default: launch_sweep<256>(a, grid, NC, st, eval); break;
             }
         }
-        hipLaunchKernelGGL(k_sweep_final, dim3((P + 255) / 256), dim3(256), 0, st, a, P, cs.d_xc.p,
-                           ctx->lin.p, ctx->hin.p, ctx->knot.p, ctx->krow.p);
+        hipLaunchKernelGGL(k_sweep_final, dim3((P + 255) / 256), dim3(256), 0, st, a, P, cs.d_xc.p, lin, hin, knot,
+                           krow);
         MH_LAUNCH_CHECK();
     }
     ctx_sync(ctx);

@@ -33,7 +33,7 @@ struct Pass {
  */
 void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, double *ybx,
                  const float *xover = nullptr, const float *sover = nullptr, bool rescore = false,
-                 const int *keep = nullptr);
+                 const int *keep = nullptr, bool f32 = false);
 
 /* FP64 scoring with device-built lists and work items (no host synchronisation).  With `list`:
  * the candidates cs[list[i]], i < *dcount (bound: upper bound on the count), lin[list[i]] = score,
@@ -74,6 +74,10 @@ void reserve_step_buffers(mars_hip_ctx *ctx);
 
 /* Hinge sweep of fit 0's candidates, whose normalized columns sit in its rbuf. */
 void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span);
+/* The same over the candidates `cs` (columns c of rbuf), scores lin / ybx in, hinge score, knot
+ * position and knot row per candidate out. */
+void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const double *ybx, double *hin, int *knot,
+                      int *krow, int min_span, int end_span);
 
 /* Upload the per-fit device views (current m, rbuf, first candidate of `cs`). */
 void upload_fits(mars_hip_ctx *ctx, const Cands &cs, const float *xover = nullptr,

@@ -326,7 +326,7 @@ int grid_w(size_t items)
     return (int)std::max<size_t>(1, std::min<size_t>(items, 65536));
 }
 
-template <bool RETRY>
+template <bool RETRY, bool F32 = false>
 void launch_tgemm(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, const Cands &cs,
                   const int *list, int ldT, bool narrow, const int *dn = nullptr, const FitDev *fits = nullptr)
 {
@@ -334,7 +334,7 @@ void launch_tgemm(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, cons
     const dim3 grid(grid_w(nitems)), block(64);
     const FitDev *fv = fits ? fits : (const FitDev *)ctx->d_fits.p;
 #define MH_GO1(M, CG)                                                                                 \
-    hipLaunchKernelGGL((k_tgemm_w<M, CG, RETRY>), grid, block, 0, ctx->stream, fv, \
+    hipLaunchKernelGGL((k_tgemm_w<M, CG, RETRY, F32>), grid, block, 0, ctx->stream, fv, \
                        (const Item *)u.items, nitems, (const int *)cs.d_xc.p, (const int *)cs.d_par.p, list, ldT, \
                        ctx->tpart.p, dn)
 #define MH_GO(M)                      \
@@ -346,7 +346,7 @@ void launch_tgemm(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, cons
     MH_LAUNCH_CHECK();
 }
 
-template <int MODE, bool RB, bool TCAND>
+template <int MODE, bool RB, bool TCAND, bool F32 = false>
 void launch_project(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, const Cands &cs,
                     const int *list, const double *T, int ldT, bool narrow, const int *keep = nullptr,
                     const int *dn = nullptr, const FitDev *fits = nullptr)
@@ -355,7 +355,7 @@ void launch_project(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, co
     const dim3 grid(grid_w(nitems)), block(64);
     const FitDev *fv = fits ? fits : (const FitDev *)ctx->d_fits.p;
 #define MH_GO1(M, CG)                                                                                     \
-    hipLaunchKernelGGL((k_project_w<M, CG, MODE, RB, TCAND>), grid, block, 0, ctx->stream,                \
+    hipLaunchKernelGGL((k_project_w<M, CG, MODE, RB, TCAND, F32>), grid, block, 0, ctx->stream,           \
                        fv, (const Item *)u.items, nitems, (const int *)cs.d_xc.p, \
                        (const int *)cs.d_par.p, list, T, ldT, (const double *)ctx->ssum.p,                \
                        (const int *)ctx->flag.p, keep, ctx->spart.p, dn)
@@ -1119,8 +1119,9 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
 }
 
 void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, double *ybx,
-                 const float *xover, const float *sover, bool rescore, const int *keep)
+                 const float *xover, const float *sover, bool rescore, const int *keep, bool f32)
 {
+    require(!f32 || (keep_bxn && !rescore), "internal: the FP32 first pass serves the hinge path");
     PendingScope pending_scope(ctx);
     const bool store_r = keep_bxn || rescore;  // residuals stay in rbuf (all, or those `keep` marks)
     if (!rescore) keep = nullptr;
@@ -1191,10 +1192,13 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     ctx->flag.ensure(P);
     ctx->fired_list.ensure(P);
 
-    // Pass 1: T = Bo^T Bx
+    // Pass 1: T = Bo^T Bx (FP32 MFMA in the precision form's approximate pass)
     {
-        KTimer kt(ctx, K_TGEMM, flops, bytes);
-        launch_tgemm<false>(ctx, mt, uT, (int)pT.items.size(), cs, nullptr, ldT, narrow);
+        KTimer kt(ctx, f32 ? K_TGEMM32 : K_TGEMM, flops, bytes);
+        if (f32)
+            launch_tgemm<false, true>(ctx, mt, uT, (int)pT.items.size(), cs, nullptr, ldT, narrow);
+        else
+            launch_tgemm<false>(ctx, mt, uT, (int)pT.items.size(), cs, nullptr, ldT, narrow);
         hipLaunchKernelGGL(k_treduce, dim3(P), dim3(64), 0, st, ctx->d_fits.p, cs.d_fit.p, nullptr, P,
                            uT.cmap, ctx->tpart.p, ldT, ctx->T.p, ctx->tnorm.p, nullptr);
         MH_LAUNCH_CHECK();
@@ -1202,8 +1206,11 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     // Pass 2: s = sum f32(Bx - Bo T)^2 and the DGKS gate; r itself is kept only for the
     // hinge sweep (which needs the normalized columns)
     {
-        KTimer kt(ctx, K_PROJECT, flops, bytes + 0.0);
-        if (store_r)
+        KTimer kt(ctx, f32 ? K_PROJECT32 : K_PROJECT, flops, bytes + 0.0);
+        if (f32)
+            launch_project<kStoreR, false, false, true>(ctx, mt, uP, (int)pP.items.size(), cs, nullptr, ctx->T.p,
+                                                        ldT, narrow);
+        else if (store_r)
             launch_project<kStoreR, false, false>(ctx, mt, uP, (int)pP.items.size(), cs, nullptr, ctx->T.p, ldT,
                                                   narrow, keep);
         else

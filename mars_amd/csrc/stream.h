@@ -12,6 +12,8 @@
 // sets (the mixed form's rescoring) cost in proportion.
 //
 //   k_tgemm_w:   T = Bo^T Bx over the item's rows                  (kernels.cc:332-345)
+//                (F32: FP32 MFMA over each 64-row span, spans added in FP64 -- the precision
+//                form's approximate first pass; its DGKS retry and norms stay FP64)
 //   k_project_w: r = f32(Bx - Bo T) and sum r^2, r stored row-major or not,
 //                or the normalised column's projection on y    (kernels.cc:347-416)
 //                (T tile staged in LDS once per item)
@@ -128,7 +130,7 @@ __device__ inline float4 bx4(const FitDev &F, const LaneCand &c, long long row, 
  * takes rows r + 8 kr + q (kr = lane / 16), so each lane reads 4 consecutive
  * rows of its Bo columns 16u + lane%16 and of its candidates' Bx.
  */
-template <int MT, int CG, bool RETRY>
+template <int MT, int CG, bool RETRY, bool F32 = false>
 __global__ __launch_bounds__(64, waves_for(MT)) void k_tgemm_w(const FitDev *__restrict__ fits, const Item *__restrict__ items,
                                                 int nitems, const int *__restrict__ cxc,
                                                 const int *__restrict__ cpar, const int *__restrict__ list,
@@ -163,6 +165,14 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_tgemm_w(const FitDev *__r
             for (int u = 0; u < MT; ++u) acc[g][u] = v4d{0, 0, 0, 0};
 #pragma unroll 1
         for (long long r = I.r0; r < I.r1; r += kWR * NCH) {
+            // F32: this iteration's rows accumulate in FP32 (v_mfma_f32_16x16x4f32), folded into FP64
+            v4f acc32[CG][MT];
+            if (F32) {
+#pragma unroll
+                for (int g = 0; g < CG; ++g)
+#pragma unroll
+                    for (int u = 0; u < MT; ++u) acc32[g][u] = v4f{0, 0, 0, 0};
+            }
 #pragma unroll 1
             for (int h0 = 0; h0 < 2; h0 += HF) {
                 // all loads of HF half-steps of NCH chunks first, unconditionally within a chunk
@@ -206,10 +216,22 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_tgemm_w(const FitDev *__r
                             for (int g = 0; g < CG; ++g)
                                 if (16 * g < I.nc)
 #pragma unroll
-                                    for (int u = 0; u < MT; ++u)
-                                        acc[g][u] = mfma_f64(a[u][q], b[g][q], acc[g][u]);
+                                    for (int u = 0; u < MT; ++u) {
+                                        if (F32)
+                                            acc32[g][u] = mfma_f32((float)a[u][q], (float)b[g][q], acc32[g][u]);
+                                        else
+                                            acc[g][u] = mfma_f64(a[u][q], b[g][q], acc[g][u]);
+                                    }
                     }
                 }
+            }
+            if (F32) {
+#pragma unroll
+                for (int g = 0; g < CG; ++g)
+#pragma unroll
+                    for (int u = 0; u < MT; ++u)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) acc[g][u][v] += (double)acc32[g][u][v];
             }
         }
 #pragma unroll
@@ -220,7 +242,8 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_tgemm_w(const FitDev *__r
 #pragma unroll
             for (int u = 0; u < MT; ++u)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) dst[16 * u + kr + 4 * v] = acc[g][u][v];  // D row l/16 + 4v
+                for (int v = 0; v < 4; ++v)  // D row l/16 + 4v (FP64 MFMA), 4 (l/16) + v (FP32 MFMA)
+                    dst[16 * u + (F32 ? 4 * kr + v : kr + 4 * v)] = acc[g][u][v];
         }
     }
 }
@@ -243,7 +266,7 @@ enum { kStoreR = 0, kSumSq = 1, kFinalY = 2 };
  * elements a lane holds (M = l/16 + 4v) are the 4 consecutive rows
  * 4 (l/16) + v, whose Bx it loads as one float4 each of B and X.
  */
-template <int MT, int CG, int MODE, bool RB, bool TCAND>
+template <int MT, int CG, int MODE, bool RB, bool TCAND, bool F32 = false>
 __global__ __launch_bounds__(64, waves_for(MT)) void k_project_w(const FitDev *__restrict__ fits,
                                                                  const Item *__restrict__ items, int nitems,
                                                                  const int *__restrict__ cxc,
@@ -262,7 +285,8 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_project_w(const FitDev *_
     constexpr int HF = 1;  // 16-row blocks whose loads are issued together (two spill registers)
     __shared__ double sT[NC][ST];
     const int lane = threadIdx.x, kr = lane >> 4, kc = lane & 15;
-    const int prow = 4 * (kc & 3) + (kc >> 2);  // memory row of this lane's A-operand M index
+    // memory row of this lane's A-operand M index: the lane's D elements are then rows 4 (l/16) + v
+    const int prow = F32 ? kc : 4 * (kc & 3) + (kc >> 2);
     for (int round = 0; round < nitems; round += gridDim.x) {
         const int cnt = min((int)gridDim.x, nitems - round);
         if ((int)blockIdx.x >= cnt) break;
@@ -339,8 +363,16 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_project_w(const FitDev *_
                     for (int g = 0; g < CG; ++g) {
                         if (16 * g >= I.nc) continue;
                         v4d d = v4d{0, 0, 0, 0};
+                        if (F32) {  // Bo T in FP32 (the approximate first pass of the precision form)
+                            v4f d32 = v4f{0, 0, 0, 0};
 #pragma unroll
-                        for (int q = 0; q < KQ; ++q) d = mfma_f64((double)av[q], sT[16 * g + kc][4 * q + kr], d);
+                            for (int q = 0; q < KQ; ++q)
+                                d32 = mfma_f32(av[q], (float)sT[16 * g + kc][4 * q + kr], d32);
+                            d = v4d{(double)d32[0], (double)d32[1], (double)d32[2], (double)d32[3]};
+                        } else {
+#pragma unroll
+                            for (int q = 0; q < KQ; ++q) d = mfma_f64((double)av[q], sT[16 * g + kc][4 * q + kr], d);
+                        }
                         if (!c[g].ok) continue;
                         const float bv[4] = {bxv[g].x, bxv[g].y, bxv[g].z, bxv[g].w};
                         float rf[4];

@@ -48,8 +48,7 @@ def test_accuracy_per_config(config, name):
         gold = json.load(f)
     want = [a["mse"] for a in gold["appends"]]
     want_seq = [(r["type"], r["basis"], r["input"]) for r in gold["model"]]
-    forms = ["fp64", "mixed"] if case["fit"].get("linear_only", False) else ["fp64"]
-    for prec in forms:
+    for prec in ("fp64", "mixed"):
         model, mse, orth = _fit(case, prec)
         seq = [(r["type"].decode(), int(r["basis"]), int(r["input"])) for r in model]
         assert seq == want_seq, "%s: term sequence differs" % prec

@@ -72,11 +72,24 @@ def _seq(model):
 
 
 def test_config3_full_properties():
-    """1M rows x 100 columns, hinge fit to 65 terms (plain form)."""
+    """1M rows x 100 columns, hinge fit to 65 terms (plain form), and the precision form: the
+    same terms and knots, mse within 1e-6."""
     X, y = datagen.workload3()
     kw = datagen.FIT_KW[3]
     say("config 3 full: X", X.shape)
+    mixed, mtrace, malgo = _fit(X, y, kw, "mixed")
+    st = malgo.stats()
+    _, Bo = malgo.basis()
+    Qm = np.asarray(Bo, dtype=np.float64)
+    malgo.close()
     model, trace, algo = _fit(X, y, kw, "fp64")
+    assert _seq(model) == _seq(mixed)
+    pm = [v for s in trace for v in s["mse"]]
+    mm = [v for s in mtrace for v in s["mse"]]
+    assert len(pm) == len(mm) and all(abs(u - v) <= 1e-6 * abs(u) for u, v in zip(pm, mm))
+    say("config 3 mixed: rescored %d of %d, window violations %d, max|QtQ-I| %.3g"
+        % (st["rescored"], st["scored"], st["window_violations"], np.abs(Qm.T @ Qm - np.eye(Qm.shape[1])).max()))
+    assert st["window_violations"] == 0
     n = len(X)
     assert len(model) == kw["max_terms"]
     assert _check_scores_realised(trace, n) >= len(trace) - 1

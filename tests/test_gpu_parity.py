@@ -300,6 +300,25 @@ def test_batch_of_one_equals_single_fit():
 STORE_POLICIES = ["ungated", "none"]
 
 
+@pytest.mark.parametrize("case", ["w2_small", "w3_small", "hinge_edges", "deg3", "weighted", "w2"])
+def test_mixed_hinge_form_matches_reference(case):
+    """Precision form with hinge sweeps (FP32 first projection, FP64 retry and sweep, exact
+    rescoring of the near winners): the reference's terms, knots and per-step mse."""
+    import mars_amd
+    from mars_amd import marslib
+
+    gold = json.load(open(os.path.join(HERE, "golden", case + ".json")))
+    say("mixed hinge fit", case)
+    X, y, w = make_inputs(CASES[case])
+    w = np.ones(len(X), dtype=np.float32) if w is None else w
+    algo = marslib.MarsAlgo(X, y, w, CASES[case]["fit"]["max_terms"], precision="mixed")
+    trace = []
+    model = mars_amd.fit(X, y, w, algo=algo, trace=trace, precision="mixed", **CASES[case]["fit"])
+    _compare_to_golden(gold, model, trace, 1e-10)
+    st = algo.stats()
+    assert st["window_violations"] == 0 and st["rescored"] > 0
+
+
 @pytest.mark.parametrize("policy", STORE_POLICIES)
 @pytest.mark.parametrize("case", ["w1", "w5_small", "w4_fit2"])
 def test_mixed_form_matches_reference(case, policy, monkeypatch):
