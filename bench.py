@@ -145,14 +145,17 @@ def run_form(X, y, fit_kw, precision, steps, warmup):
             for f in ("launches", "ms", "flops", "bytes"):
                 d[f] += r[f]
         if dgks is None:
-            dgks = a.stats()["dgks_fired"]
+            st0 = a.stats()
+            dgks = st0["dgks_fired"]
         a.close()
         upload += t1 - t0
         wall += t2 - t1
     cands = sum(s["n_cand"] for s in traces[0])
     seq = [(r["type"].decode(), int(r["basis"]), int(r["input"]), float(r["hinge"])) for r in models[0]]
     return dict(wall=wall, fit_s=wall / steps, upload_s=upload / steps, cands=cands, kstats=ks, seq=seq,
-                trace=traces[0], epochs=len(traces[0]), dgks=dgks, peak_device_bytes=marslib.device_bytes()[1])
+                trace=traces[0], epochs=len(traces[0]), dgks=dgks, peak_device_bytes=marslib.device_bytes()[1],
+                mse=[[v for s in traces[0] for v in s["mse"]]], rescored=st0["rescored"], gated=st0["gated"],
+                terms=len(models[0]))
 
 
 def roofline(ks):
@@ -320,8 +323,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", type=int, default=4, choices=[2, 4])
-    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--workload", type=int, default=4, choices=[2, 3, 4, 5])
+    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "mixed-bf16", "both", "all"],
+                    help="both: fp64 and mixed; all: fp64, mixed and mixed-bf16 (BF16-split products on the "
+                         "hinge path)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=CPU_SAMPLE_S)
     ap.add_argument("--copy-probe", action="store_true", help="run only a 1 GiB device copy (PMC calibration)")
@@ -340,7 +345,7 @@ def main():
         sys.exit(2)
 
     data, fit_kw = workload_inputs(args.workload)
-    forms = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    forms = {"both": ["fp64", "mixed"], "all": ["fp64", "mixed-bf16", "mixed"]}.get(args.precision, [args.precision])
     if args.workload == 4:
         res = {f: run_batch(data, fit_kw, f, args.steps, args.warmup) for f in forms}
         workload = ("256 independent fits, N log-uniform [2000, 200000], p uniform [5, 64], "
@@ -363,7 +368,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if forms[-1] == "fp64" else "f32 (MFMA) + f64",
+        "dtype": {"fp64": "f64", "mixed": "f32 (MFMA) + f64", "mixed-bf16": "bf16x3 (MFMA) + f32 + f64"}[forms[-1]],
         "data": "synthetic (mars_amd.datagen, seeded)",
         "config": dict(workload=workload, precision=forms[-1],
                        graphs=os.environ.get("MARS_HIP_GRAPH", "1") != "0"),
@@ -381,11 +386,14 @@ def main():
         "class_rooflines": {f: class_rooflines(r["kstats"]) for f, r in res.items()},
         "peak_device_bytes": {f: r["peak_device_bytes"] for f, r in res.items()},
     }
-    if len(res) > 1:
+    if "fp64" in res and "mixed" in res:
         out["speedup_mixed_vs_fp64"] = res["fp64"]["fit_s"] / res["mixed"]["fit_s"]
-        if args.workload == 4 and out["same_term_sequence"]:
-            out["mse_rel_diff"] = max(abs(a - b) / b for ma, mb in zip(res["mixed"]["mse"], res["fp64"]["mse"])
-                                      for a, b in zip(ma, mb))
+    if "fp64" in res and out["same_term_sequence"]:
+        diffs = {f: max((abs(a - b) / b for ma, mb in zip(r["mse"], res["fp64"]["mse"]) for a, b in zip(ma, mb)),
+                        default=0.0) for f, r in res.items() if f != "fp64"}
+        if diffs:
+            out["mse_rel_diff"] = max(diffs.values())
+            out["mse_rel_diff_by_form"] = diffs
     if "rescored" in head:
         out["rescored"] = head["rescored"]
         out["rescored_gated"] = head["gated"]

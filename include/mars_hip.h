@@ -42,8 +42,10 @@ extern "C" {
 
 /* Precision forms of the candidate scoring. */
 #define MARS_HIP_FORM_PLAIN 0  /* all FP64 arithmetic on f32 storage, CPU rounding points */
-#define MARS_HIP_FORM_MIXED 1  /* FP32-MFMA incremental projection cache + exact FP64 rescoring of DGKS-gated
-                                  and near-tie candidates (linear delta-SSE only) */
+#define MARS_HIP_FORM_MIXED 1  /* linear fits: FP32-MFMA incremental projection cache; hinge fits: FP32-MFMA
+                                  first projection; then exact FP64 rescoring of every candidate that can win */
+#define MARS_HIP_FORM_MIXED_BF16 2  /* the mixed form with the hinge path's first T-GEMM on BF16-split
+                                       hi/lo products (three BF16 MFMAs per product, FP32 sums) */
 
 typedef struct mars_hip_ctx mars_hip_ctx;
 

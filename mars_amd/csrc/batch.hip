@@ -502,14 +502,15 @@ int mars_hip_batch_create(int nfits, const float *const *X, const size_t *ldX, c
         require(out != nullptr, "null output pointer");
         require(nfits >= 1, "need at least one fit");
         require(X && ldX && y && n && p_x, "null argument");
-        require(form == MARS_HIP_FORM_PLAIN || form == MARS_HIP_FORM_MIXED, "unknown precision form");
+        require(valid_form(form), "unknown precision form");
         *out = nullptr;
         auto *ctx = new mars_hip_ctx;
         try {
             ctx_base_init(ctx, max_terms);
             ctx->resident = true;
             ctx->row_major = false;  // the batch scores linear terms only: column-major copies suffice
-            ctx->form = form;
+            set_form(ctx, form);
+            form = ctx->form;
             ctx->fits.resize(nfits);
             for (int f = 0; f < nfits; ++f) {
                 std::vector<float> ones;

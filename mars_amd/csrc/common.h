@@ -216,13 +216,13 @@ constexpr int kDStat = 8;  // device counters per context (mars_hip_ctx::dstat)
 
 /* Kernel classes timed by mars_hip_timing. */
 enum KClass { K_TGEMM = 0, K_PROJECT, K_FINALIZE, K_DGKS, K_SWEEP, K_SORT, K_APPEND, K_MIXED_A,
-              K_MIXED_B, K_RECHECK, K_ARGMAX, K_GRAPH_PLAIN, K_TGEMM32, K_PROJECT32, K_HRECHECK,
+              K_MIXED_B, K_RECHECK, K_ARGMAX, K_GRAPH_PLAIN, K_TGEMM32, K_PROJECT32, K_HRECHECK, K_TGEMM_BF16,
               // host wall-clock sections of a batch step (include the waits they contain)
               K_H_CANDS, K_H_SCORE, K_H_ARGMAX, K_H_APPEND, K_H_STEP, K_NCLASS };
 constexpr const char *kClassName[K_NCLASS] = {"tgemm_f64", "project_f64", "finalize", "dgks_retry",
                                              "hinge_sweep", "sort", "append", "mixed_update",
                                              "mixed_score", "mixed_recheck", "argmax", "plain_graph",
-                                             "tgemm_f32", "project_f32", "hinge_recheck",
+                                             "tgemm_f32", "project_f32", "hinge_recheck", "tgemm_bf16x3",
                                              "host_cands", "host_score", "host_argmax", "host_append",
                                              "host_step"};
 
@@ -268,6 +268,7 @@ struct mars_hip_ctx {
     size_t max_terms = 0;  // basis capacity (all fits)
     size_t ldb = 0;        // row stride of B and Bo (>= max_terms, multiple of 4)
     int form = MARS_HIP_FORM_PLAIN;
+    bool split_bf16 = false;         // precision form: BF16-split products in the hinge path's first T-GEMM
     bool resident = false;
     bool row_major = true;  // keep row-major B and Bo (hinge sweep, basis download); batch contexts do not
     std::vector<mh::FitState> fits;
@@ -329,6 +330,8 @@ struct mars_hip_ctx {
     mh::DBuf<double> hstate;         // hinge chunk summaries / carries
     mh::DBuf<double> hbest;
     mh::DBuf<int> hbest_i;
+    mh::DBuf<double> hbest_ub;       // hinge chunk upper bounds (precision form)
+    mh::DBuf<double> hub;            // per candidate: linear + upper bound of its best hinge score
     mh::DBuf<double> hbest2;         // precision-form hinge eval: exact hinge scores of the rescored
     mh::DBuf<int> rknot;             // ... and their knot positions and rows
 

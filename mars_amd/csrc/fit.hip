@@ -842,7 +842,7 @@ void ctx_free(mars_hip_ctx *ctx)
     ctx->d_fits2.release();
     for (DBuf<int> *b : {&ctx->dl_int, &ctx->cgate, &ctx->cdone, &ctx->dredo, &ctx->bdeg, &ctx->dcand})
         b->release();
-    for (DBuf<double> *b : {&ctx->clo, &ctx->chi, &ctx->hbest2}) b->release();
+    for (DBuf<double> *b : {&ctx->clo, &ctx->chi, &ctx->hbest2, &ctx->hbest_ub, &ctx->hub}) b->release();
     ctx->rknot.release();
     ctx->dl_u64.release();
     ctx->dstat.release();
@@ -1257,7 +1257,14 @@ void argmax_launch(mars_hip_ctx *ctx, const Cands &cs, const double *lin)
 /* ---------------------------------------------------------------------- */
 /* Precision form with hinges: exact rescoring of the near winners         */
 /* ---------------------------------------------------------------------- */
-constexpr double kHingeWindow = 2e-3;  // relative window around the approximate best scores
+constexpr double kHingeWindow = 2e-3;  // relative window around the approximate best linear score
+// relative error of the approximate normalised candidate columns taken by the screen: the FP32
+// projection's T error, summed over 64-row spans whose rounding errors add like a random walk,
+// is about u sqrt(64 m) |Bx| against |r| >= |Bx| / sqrt(10) for an ungated candidate: 1.5e-6 at
+// m = 64 (taken as 3e-6); the BF16 split's products err by 2^-17 each: 3e-5.  The worst case
+// (every rounding error in one direction) is 20x wider; the rescoring rounds that follow catch
+// a candidate whose approximation erred beyond its bound.
+constexpr double kEpsF32 = 3e-6, kEpsBf16 = 3e-5;
 
 /* mx[0] = largest linear, mx[1] = largest hinge score (one block). */
 __global__ __launch_bounds__(1024) void k_two_max(const double *__restrict__ lin, const double *__restrict__ hin,
@@ -1286,17 +1293,18 @@ __global__ __launch_bounds__(1024) void k_two_max(const double *__restrict__ lin
     }
 }
 
-/* take[c] = 1 when the candidate's approximate linear or hinge score is within the window of the
- * best approximate one, its DGKS gate fired (flag bit 0), or its residual energy is within a
- * factor 1000 of the degeneracy threshold (the approximation may sit on either side of it). */
-__global__ void k_hsel(const double *__restrict__ lin, const double *__restrict__ hin, const int *__restrict__ flag,
+/* take[c] = 1 when the candidate's approximate linear score is within the window of the best one,
+ * the upper bound of its hinge scores (hub) reaches the window of the best approximate hinge
+ * score, its DGKS gate fired (flag bit 0), or its residual energy is within a factor 1000 of the
+ * degeneracy threshold (the approximation may sit on either side of it). */
+__global__ void k_hsel(const double *__restrict__ lin, const double *__restrict__ hub, const int *__restrict__ flag,
                        const double *__restrict__ ssum, double tol, int P, const double *__restrict__ mx, double w,
                        int *__restrict__ take)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= P) return;
     take[c] = ((flag[c] & 1) || ssum[c] <= 1e3 * tol || (mx[0] > 0 && lin[c] >= mx[0] * (1.0 - w)) ||
-               (mx[1] > 0 && hin[c] >= mx[1] * (1.0 - w)))
+               (mx[1] > 0 && hub[c] >= mx[1] * (1.0 - w)))
                   ? 1
                   : 0;
 }
@@ -1322,8 +1330,8 @@ __global__ void k_hput(const int *__restrict__ idx, int R, const double *__restr
  * (it may beat the exact best: the approximation erred by more than w / 2 somewhere, as a noise
  * column can); *more = their count.  One block. */
 __global__ __launch_bounds__(1024) void k_hmore(const double *__restrict__ lin, const double *__restrict__ hin,
-                                                const int *__restrict__ done, int P, double w, int *__restrict__ take,
-                                                int *__restrict__ more)
+                                                const double *__restrict__ hub, const int *__restrict__ done, int P,
+                                                double w, int *__restrict__ take, int *__restrict__ more)
 {
     __shared__ double s1[1024], s2[1024];
     __shared__ int sv[1024];
@@ -1348,7 +1356,7 @@ __global__ __launch_bounds__(1024) void k_hmore(const double *__restrict__ lin, 
     int v = 0;
     for (int c = t; c < P; c += 1024) {
         const bool m = !done[c] && ((lin[c] > 0 && lin[c] * (1.0 + 0.5 * w) >= e1) ||
-                                    (hin[c] > 0 && hin[c] * (1.0 + 0.5 * w) >= e2));
+                                    (hub[c] > 0 && hub[c] >= e2));
         take[c] = m ? 1 : 0;
         v += m ? 1 : 0;
     }
@@ -1366,8 +1374,11 @@ __global__ __launch_bounds__(1024) void k_hmore(const double *__restrict__ lin, 
  * norms and the FP64 hinge sweep as in the plain form), then every candidate near the best
  * approximate linear or hinge score, every one whose DGKS gate fired and every near-degenerate
  * one is scored again by the plain form (FP64 projection and sweep), and its exact values replace
- * the approximate ones.  Rounds of rescoring follow while an approximate score could still beat
- * the exact best (k_hmore), so the step's winner and its scores are plain-form values.
+ * the approximate ones.  A hinge candidate's screen is the upper bound of its knots' scores under
+ * a relative error eps of its covariates (the sweep computes it; an ill-conditioned knot, whose
+ * denominator k - |f|^2 cancels, gets a wide bound).  Rounds of rescoring follow while an
+ * approximate candidate could still beat the exact best (k_hmore), so the step's winner and its
+ * scores are plain-form values.
  */
 void hinge_eval_mixed(mars_hip_ctx *ctx, Cands &cs, int min_span, int end_span)
 {
@@ -1375,7 +1386,8 @@ void hinge_eval_mixed(mars_hip_ctx *ctx, Cands &cs, int min_span, int end_span)
     const int P = (int)cs.fit.size();
     const double tol = ctx->fits[0].tol;
     plain_score(ctx, cs, true, ctx->lin.p, ctx->ybx.p, nullptr, nullptr, false, nullptr, true);
-    hinge_sweep(ctx, min_span, end_span);
+    ctx->hub.ensure(std::max(P, 1));
+    hinge_sweep(ctx, min_span, end_span, ctx->split_bf16 ? kEpsBf16 : kEpsF32, ctx->hub.p);
     KTimer kt(ctx, K_HRECHECK, 0.0, 0.0);
     ctx->red.ensure(2);
     ctx->rsel.ensure(P);
@@ -1385,7 +1397,7 @@ void hinge_eval_mixed(mars_hip_ctx *ctx, Cands &cs, int min_span, int end_span)
                        ctx->red.p);
     MH_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_hsel, dim3((P + 255) / 256), dim3(256), 0, st, (const double *)ctx->lin.p,
-                       (const double *)ctx->hin.p, (const int *)ctx->flag.p, (const double *)ctx->ssum.p, tol, P,
+                       (const double *)ctx->hub.p, (const int *)ctx->flag.p, (const double *)ctx->ssum.p, tol, P,
                        (const double *)ctx->red.p, kHingeWindow, ctx->rsel.p);
     MH_LAUNCH_CHECK();
     std::vector<int> zeros(P, 0);
@@ -1429,13 +1441,25 @@ void hinge_eval_mixed(mars_hip_ctx *ctx, Cands &cs, int min_span, int end_span)
                            ctx->cdone.p);
         MH_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_hmore, dim3(1), dim3(1024), 0, st, (const double *)ctx->lin.p, (const double *)ctx->hin.p,
-                           (const int *)ctx->cdone.p, P, kHingeWindow, ctx->rsel.p, ctx->dcand.p);
+                           (const double *)ctx->hub.p, (const int *)ctx->cdone.p, P, kHingeWindow, ctx->rsel.p,
+                           ctx->dcand.p);
         MH_LAUNCH_CHECK();
         MH_CHECK(hipMemcpyAsync(&more, ctx->dcand.p, sizeof(int), hipMemcpyDeviceToHost, st));
         ctx_sync(ctx);
         if (more == 0) break;
         ctx->hinge_rounds += 1;
     }
+}
+
+bool valid_form(int form)
+{
+    return form == MARS_HIP_FORM_PLAIN || form == MARS_HIP_FORM_MIXED || form == MARS_HIP_FORM_MIXED_BF16;
+}
+
+void set_form(mars_hip_ctx *ctx, int form)
+{
+    ctx->split_bf16 = form == MARS_HIP_FORM_MIXED_BF16;
+    ctx->form = form == MARS_HIP_FORM_PLAIN ? MARS_HIP_FORM_PLAIN : MARS_HIP_FORM_MIXED;
 }
 
 void argmax_launch_ranges(mars_hip_ctx *ctx, const double *lin, size_t bound)
@@ -1786,14 +1810,15 @@ int mars_hip_fit_create(const float *X, size_t ldX, const float *y, const float 
 {
     return guarded([&] {
         require(out != nullptr, "null output pointer");
-        require(form == MARS_HIP_FORM_PLAIN || form == MARS_HIP_FORM_MIXED, "unknown precision form");
+        require(valid_form(form), "unknown precision form");
         *out = nullptr;
         auto *ctx = new mars_hip_ctx;
         try {
             ctx_base_init(ctx, max_terms);
             ctx->fits.resize(1);
             ctx->resident = true;
-            ctx->form = form;
+            set_form(ctx, form);
+            form = ctx->form;
             fit_init(ctx, ctx->fits[0], X, ldX, y, w, n, p_x);
             if (form == MARS_HIP_FORM_MIXED) ensure_tcache(ctx);
             reserve_step_buffers(ctx);
@@ -1821,8 +1846,8 @@ int mars_hip_fit_set_form(mars_hip_ctx *ctx, int form)
 {
     return guarded([&] {
         require(ctx != nullptr, "null context");
-        require(form == MARS_HIP_FORM_PLAIN || form == MARS_HIP_FORM_MIXED, "unknown precision form");
-        ctx->form = form;
+        require(valid_form(form), "unknown precision form");
+        set_form(ctx, form);
     });
 }
 

@@ -164,6 +164,8 @@ struct SweepArgs {
     double *dsum;        // [ngroups][C]    chunk delta sums
     double *best;        // [P][C]
     int *besti;          // [P][C]
+    double eps;          // > 0: also bound each knot's score from above (the precision form's screen)
+    double *bestub;      // [P][C]  largest upper bound over the chunk's knots (eps > 0)
 };
 
 constexpr int kTR = 16;    // sorted rows per tile
@@ -322,7 +324,7 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
         for (int o = 32; o > 0; o >>= 1) mine = max(mine, __shfl_xor(mine, o, 64));
         lastnz = mine;
     }
-    double bsse = 0.0, dacc = 0.0;
+    double bsse = 0.0, dacc = 0.0, bub = 0.0;
     int bidx = -1;
     auto grid_mask = [&](int tt) {
         const size_t tb = r0 + (size_t)tt * kTR, te = tb + tile_rows(tt);
@@ -435,6 +437,19 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
                             bsse = sse;
                             bidx = (int)i;
                         }
+                        if (a.eps > 0) {
+                            // relative errors eps of the covariates move den = k - |f|^2 by at most
+                            // eps (k + |f|^2) and fy - w by eps sum |f_s Y_s|: the score's bound
+                            double ua = 0.0;
+#pragma unroll
+                            for (int s = 0; s < J; ++s) ua = fma(fabs(f[s]), fabs(Y[s]), ua);
+                            const double ffs = group_sum<G>(ff);
+                            ua = group_sum<G>(ua);
+                            const double rel = a.eps * ((den + 2.0 * ffs) / fmax(den, a.tol) +
+                                                        2.0 * ua / fmax(fabs(uw), 1e-300));
+                            const double ub = den > a.tol ? sse * (1.0 + rel) : 0.0;
+                            bub = fmax(bub, ub);
+                        }
                     }
                 }
             }
@@ -468,6 +483,7 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
         if (t == 0) {
             a.best[cs] = bsse;
             a.besti[cs] = bidx;
+            if (a.eps > 0) a.bestub[cs] = bub;
         }
     } else {
         double *dst = a.sum + cs * a.L2 * 2;
@@ -508,14 +524,15 @@ __global__ void k_sweep_carry(SweepArgs a, const int *__restrict__ cgrp, int P)
     }
 }
 
-/* Best knot over chunks (first maximum in row order), hinge = linear + best. */
+/* Best knot over chunks (first maximum in row order), hinge = linear + best; with eps > 0 also
+ * hub = linear + the largest upper bound over every knot. */
 __global__ void k_sweep_final(SweepArgs a, int P, const int *__restrict__ cxc,
                               const double *__restrict__ lin, double *__restrict__ hin,
-                              int *__restrict__ knot, int *__restrict__ krow)
+                              int *__restrict__ knot, int *__restrict__ krow, double *__restrict__ hub)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= P) return;
-    double b = 0.0;
+    double b = 0.0, ub = 0.0;
     int bi = -1;
     for (int ch = 0; ch < a.C; ++ch) {
         const double v = a.best[(size_t)c * a.C + ch];
@@ -524,7 +541,9 @@ __global__ void k_sweep_final(SweepArgs a, int P, const int *__restrict__ cxc,
             b = v;
             bi = vi;
         }
+        if (hub) ub = fmax(ub, a.bestub[(size_t)c * a.C + ch]);
     }
+    if (hub) hub[c] = lin[c] + ub;
     knot[c] = bi;
     if (bi >= 0) {
         hin[c] = lin[c] + b;
@@ -608,18 +627,19 @@ static void ensure_sorted(mars_hip_ctx *ctx, int xc)
     fs.sorted[xc] = 1;
 }
 
-void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span)
+void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span, double eps, double *hub)
 {
     Cands &cs = ctx->cand;
     const size_t P = std::max<size_t>(cs.fit.size(), 1);
     ctx->hin.ensure(P);
     ctx->knot.ensure(P);
     ctx->krow.ensure(P);
-    hinge_sweep_list(ctx, cs, ctx->lin.p, ctx->ybx.p, ctx->hin.p, ctx->knot.p, ctx->krow.p, min_span, end_span);
+    hinge_sweep_list(ctx, cs, ctx->lin.p, ctx->ybx.p, ctx->hin.p, ctx->knot.p, ctx->krow.p, min_span, end_span, eps,
+                     hub);
 }
 
 void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const double *ybx, double *hin, int *knot,
-                      int *krow, int min_span, int end_span)
+                      int *krow, int min_span, int end_span, double eps, double *hub)
 {
     FitState &fs = ctx->fits[0];
     const int P = (int)cs.fit.size();
@@ -668,6 +688,7 @@ void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const dou
     ctx->hstate.ensure((size_t)P * C * L2 * 2 + (size_t)P * C + (size_t)ng * C);
     ctx->hbest.ensure((size_t)P * C);
     ctx->hbest_i.ensure((size_t)P * C);
+    if (hub) ctx->hbest_ub.ensure((size_t)P * C);
 
     SweepArgs a;
     a.Bo = fs.Bo.p;
@@ -698,6 +719,8 @@ void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const dou
     a.dsum = a.ksum + (size_t)P * C;
     a.best = ctx->hbest.p;
     a.besti = ctx->hbest_i.p;
+    a.eps = hub ? eps : 0.0;
+    a.bestub = hub ? ctx->hbest_ub.p : nullptr;
 
     const dim3 grid(C, pgroups, ng);
     // reference algorithmic work: n (m+1) (4 + 4/min_span) flops per candidate (marsalgo.cc:576-649)
@@ -720,7 +743,7 @@ void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const dou
             }
         }
         hipLaunchKernelGGL(k_sweep_final, dim3((P + 255) / 256), dim3(256), 0, st, a, P, cs.d_xc.p, lin, hin, knot,
-                           krow);
+                           krow, hub);
         MH_LAUNCH_CHECK();
     }
     ctx_sync(ctx);

@@ -72,12 +72,14 @@ void ensure_tcache(mars_hip_ctx *ctx);
  * does not grow them step by step; called when a resident context is created. */
 void reserve_step_buffers(mars_hip_ctx *ctx);
 
-/* Hinge sweep of fit 0's candidates, whose normalized columns sit in its rbuf. */
-void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span);
+/* Hinge sweep of fit 0's candidates, whose normalized columns sit in its rbuf.  With hub: also,
+ * per candidate, linear + an upper bound of every knot's score when each covariate carries a
+ * relative error up to eps (the precision form's screen). */
+void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span, double eps = 0.0, double *hub = nullptr);
 /* The same over the candidates `cs` (columns c of rbuf), scores lin / ybx in, hinge score, knot
  * position and knot row per candidate out. */
 void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const double *ybx, double *hin, int *knot,
-                      int *krow, int min_span, int end_span);
+                      int *krow, int min_span, int end_span, double eps = 0.0, double *hub = nullptr);
 
 /* Upload the per-fit device views (current m, rbuf, first candidate of `cs`). */
 void upload_fits(mars_hip_ctx *ctx, const Cands &cs, const float *xover = nullptr,
@@ -117,6 +119,9 @@ void argmax_launch(mars_hip_ctx *ctx, const Cands &cs, const double *lin);
 /* The same over ranges already on the device (ctx->ival: first candidate, count per fit). */
 void argmax_launch_ranges(mars_hip_ctx *ctx, const double *lin, size_t bound);
 double host_dsse(const FitState &fs);
+/* MARS_HIP_FORM_* checks; set_form maps MIXED_BF16 to the mixed form with BF16-split products. */
+bool valid_form(int form);
+void set_form(mars_hip_ctx *ctx, int form);
 /* Copy the context's device counters (kDStat entries) to `out` (synchronises the stream). */
 void read_dstat(mars_hip_ctx *ctx, unsigned long long *out);
 /* Device address of the no-alloc append's scratch on the current device (resolved once). */

@@ -38,7 +38,8 @@ int parse_form(const std::string &f)
 {
     if (f == "fp64" || f == "plain") return MARS_HIP_FORM_PLAIN;
     if (f == "mixed") return MARS_HIP_FORM_MIXED;
-    throw std::runtime_error("unknown precision form '" + f + "' (expected 'fp64' or 'mixed')");
+    if (f == "mixed-bf16") return MARS_HIP_FORM_MIXED_BF16;
+    throw std::runtime_error("unknown precision form '" + f + "' (expected 'fp64', 'mixed' or 'mixed-bf16')");
 }
 
 py::dict tuning_dict(const mars_hip_tuning &t)

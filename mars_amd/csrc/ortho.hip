@@ -346,6 +346,24 @@ void launch_tgemm(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, cons
     MH_LAUNCH_CHECK();
 }
 
+void launch_tgemm_bf16(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, const Cands &cs, int ldT,
+                       bool narrow)
+{
+    if (nitems == 0) return;
+    const dim3 grid(grid_w(nitems)), block(64);
+#define MH_GO1(M, CG)                                                                                           \
+    hipLaunchKernelGGL((k_tgemm_bf16<M, CG>), grid, block, 0, ctx->stream, (const FitDev *)ctx->d_fits.p,       \
+                       (const Item *)u.items, nitems, (const int *)cs.d_xc.p, (const int *)cs.d_par.p, ldT,     \
+                       ctx->tpart.p)
+#define MH_GO(M)                      \
+    if (narrow) MH_GO1(M, 1);         \
+    else MH_GO1(M, cg_tgemm(M))
+    MH_MT_SWITCH(mt, MH_GO)
+#undef MH_GO
+#undef MH_GO1
+    MH_LAUNCH_CHECK();
+}
+
 template <int MODE, bool RB, bool TCAND, bool F32 = false>
 void launch_project(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, const Cands &cs,
                     const int *list, const double *T, int ldT, bool narrow, const int *keep = nullptr,
@@ -1194,8 +1212,10 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
 
     // Pass 1: T = Bo^T Bx (FP32 MFMA in the precision form's approximate pass)
     {
-        KTimer kt(ctx, f32 ? K_TGEMM32 : K_TGEMM, flops, bytes);
-        if (f32)
+        KTimer kt(ctx, f32 ? (ctx->split_bf16 ? K_TGEMM_BF16 : K_TGEMM32) : K_TGEMM, flops, bytes);
+        if (f32 && ctx->split_bf16)
+            launch_tgemm_bf16(ctx, mt, uT, (int)pT.items.size(), cs, ldT, narrow);
+        else if (f32)
             launch_tgemm<false, true>(ctx, mt, uT, (int)pT.items.size(), cs, nullptr, ldT, narrow);
         else
             launch_tgemm<false>(ctx, mt, uT, (int)pT.items.size(), cs, nullptr, ldT, narrow);

@@ -248,6 +248,107 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_tgemm_w(const FitDev *__r
     }
 }
 
+/*
+ * T-GEMM partials by BF16-split products: each f32 operand x = hi + lo with hi = bf16(x) and
+ * lo = bf16(x - hi), and x y ~ hi_x hi_y + hi_x lo_y + lo_x hi_y (the lo_x lo_y term and the
+ * rounding of lo are below 2^-17 of |x y|), three v_mfma_f32_16x16x32_bf16 per 32-row chunk with
+ * FP32 accumulation, chunks added in FP64.  Operand layout of the 16x16x32 BF16 MFMA: lane l holds
+ * A[l%16][8 (l/16) .. 8 (l/16) + 7] and B[8 (l/16) .. + 7][l%16]; D as the FP32 16x16x4 MFMA.
+ * So lane (kr, kc) supplies rows r + 8 kr .. r + 8 kr + 7 of its Bo column 16u + kc and of its
+ * candidates' Bx: two float4 loads each, as k_tgemm_w.  Same partial layout as k_tgemm_w<F32>.
+ */
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+__device__ inline void split_bf16(const float4 &x0, const float4 &x1, v8bf &hi, v8bf &lo)
+{
+    const float v[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const __bf16 h = (__bf16)v[j];
+        hi[j] = h;
+        lo[j] = (__bf16)(v[j] - (float)h);
+    }
+}
+
+template <int MT, int CG>
+__global__ __launch_bounds__(64, waves_for(MT)) void k_tgemm_bf16(const FitDev *__restrict__ fits,
+                                                                   const Item *__restrict__ items, int nitems,
+                                                                   const int *__restrict__ cxc,
+                                                                   const int *__restrict__ cpar, int ldT,
+                                                                   double *__restrict__ tpart)
+{
+    const int lane = threadIdx.x, kr = lane >> 4, kc = lane & 15;
+    for (int round = 0; round < nitems; round += gridDim.x) {
+        const int cnt = min((int)gridDim.x, nitems - round);
+        if ((int)blockIdx.x >= cnt) break;
+        const int it = round + xcd_item(blockIdx.x, cnt);
+        const Item I = items[it];
+        const FitDev F = fits[I.fit];
+        LaneCand c[CG];
+#pragma unroll
+        for (int g = 0; g < CG; ++g) c[g] = lane_cand<false>(F, I, 16 * g + kc, cxc, cpar, nullptr);
+        const float *bo[MT];
+        bool kok[MT];
+#pragma unroll
+        for (int u = 0; u < MT; ++u) {
+            const int k = 16 * u + kc;
+            kok[u] = k < F.m;
+            bo[u] = F.BoC + (size_t)(kok[u] ? k : 0) * F.ldx;
+        }
+        v4d acc[CG][MT];
+#pragma unroll
+        for (int g = 0; g < CG; ++g)
+#pragma unroll
+            for (int u = 0; u < MT; ++u) acc[g][u] = v4d{0, 0, 0, 0};
+#pragma unroll 1
+        for (long long r = I.r0; r < I.r1; r += kWR) {
+            const long long row = r + 8 * kr;
+            float4 av[MT][2], bv[CG][2];
+#pragma unroll
+            for (int u = 0; u < MT; ++u) {
+                av[u][0] = gld4(bo[u] + row);
+                av[u][1] = gld4(bo[u] + row + 4);
+            }
+#pragma unroll
+            for (int g = 0; g < CG; ++g) {
+                bv[g][0] = bx4<false>(F, c[g], row, I.r1);
+                bv[g][1] = bx4<false>(F, c[g], row + 4, I.r1);
+            }
+            v8bf ah[MT], al[MT], bh[CG], bl[CG];
+#pragma unroll
+            for (int u = 0; u < MT; ++u) {
+                if (!kok[u]) av[u][0] = av[u][1] = float4{0.f, 0.f, 0.f, 0.f};
+                split_bf16(av[u][0], av[u][1], ah[u], al[u]);
+            }
+#pragma unroll
+            for (int g = 0; g < CG; ++g) split_bf16(bv[g][0], bv[g][1], bh[g], bl[g]);
+#pragma unroll
+            for (int g = 0; g < CG; ++g) {
+                if (16 * g >= I.nc) continue;
+#pragma unroll
+                for (int u = 0; u < MT; ++u) {
+                    v4f d = v4f{0, 0, 0, 0};
+                    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[u], bh[g], d, 0, 0, 0);
+                    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[u], bl[g], d, 0, 0, 0);
+                    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[u], bh[g], d, 0, 0, 0);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[g][u][v] += (double)d[v];
+                }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < CG; ++g) {
+            const int slot = 16 * g + kc;
+            if (slot >= I.nc) continue;
+            double *dst = tpart + ((size_t)it * kSlots + slot) * ldT;
+#pragma unroll
+            for (int u = 0; u < MT; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) dst[16 * u + 4 * kr + v] = acc[g][u][v];  // D row 4 (l/16) + v
+        }
+    }
+}
+
 /* Projection modes. */
 enum { kStoreR = 0, kSumSq = 1, kFinalY = 2 };
 

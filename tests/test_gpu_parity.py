@@ -300,20 +300,24 @@ def test_batch_of_one_equals_single_fit():
 STORE_POLICIES = ["ungated", "none"]
 
 
+HINGE_FORMS = ["mixed", "mixed-bf16"]
+
+
+@pytest.mark.parametrize("precision", HINGE_FORMS)
 @pytest.mark.parametrize("case", ["w2_small", "w3_small", "hinge_edges", "deg3", "weighted", "w2"])
-def test_mixed_hinge_form_matches_reference(case):
-    """Precision form with hinge sweeps (FP32 first projection, FP64 retry and sweep, exact
-    rescoring of the near winners): the reference's terms, knots and per-step mse."""
+def test_mixed_hinge_form_matches_reference(case, precision):
+    """Precision form with hinge sweeps (FP32 or BF16-split first projection, FP64 retry and
+    sweep, exact rescoring of the near winners): the reference's terms, knots and per-step mse."""
     import mars_amd
     from mars_amd import marslib
 
     gold = json.load(open(os.path.join(HERE, "golden", case + ".json")))
-    say("mixed hinge fit", case)
+    say("mixed hinge fit", case, precision)
     X, y, w = make_inputs(CASES[case])
     w = np.ones(len(X), dtype=np.float32) if w is None else w
-    algo = marslib.MarsAlgo(X, y, w, CASES[case]["fit"]["max_terms"], precision="mixed")
+    algo = marslib.MarsAlgo(X, y, w, CASES[case]["fit"]["max_terms"], precision=precision)
     trace = []
-    model = mars_amd.fit(X, y, w, algo=algo, trace=trace, precision="mixed", **CASES[case]["fit"])
+    model = mars_amd.fit(X, y, w, algo=algo, trace=trace, precision=precision, **CASES[case]["fit"])
     _compare_to_golden(gold, model, trace, 1e-10)
     st = algo.stats()
     assert st["window_violations"] == 0 and st["rescored"] > 0
@@ -455,3 +459,52 @@ def test_batch_graph_replays_match_direct_launches(precision, monkeypatch):
     assert a["graph_captures"] >= 1 and a["graph_launches"] == a["steps"]
     assert a["graph_captures"] <= 3  # one per basis-size class (16 columns each)
     assert b["graph_launches"] == 0
+
+
+
+@pytest.mark.parametrize("precision,rtol", [("mixed", 1e-4), ("mixed-bf16", 2e-3)])
+def test_mixed_hinge_eval_close_to_plain(precision, rtol):
+    """The approximate scores themselves (most candidates are not rescored): every linear and
+    hinge score of a mid-fit eval within rtol of the plain form's (checks the FP32 and the
+    BF16-split MFMA operand layouts and their accuracy, not only the rescored winners), and the
+    same winners.  The mask leaves out self-interactions, as fit() does by default: a hinge on a
+    variable its parent already holds can cancel the sweep's denominator, and its approximate
+    score is then far from the exact one (the precision form rescores such candidates, whose
+    upper bound is wide; its scores are then exact, which this comparison does not need)."""
+    import mars_amd
+
+    say("mixed hinge eval", precision)
+    case = CASES["w2_small"]
+    X, y, w = make_inputs(case)
+    w = np.ones(len(X), dtype=np.float32) if w is None else w
+    out, best = {}, {}
+    # (no hinge pair on one knot: with its linear term it spans x exactly, and a candidate in the
+    # span scores rounding noise, which no two summation orders reproduce)
+    terms = [("l", 3, 0, 0.0), ("+", 7, 0, 0.25), ("l", 0, 2, 0.0), ("-", 12, 1, -0.3), ("+", 5, 0, 0.1)]
+    holds = [set()]
+    for t, xc, bc, h in terms:
+        holds.append(holds[bc] | {xc})
+    for prec in ("fp64", precision):
+        a = mars_amd.marslib.MarsAlgo(X, y, w, 21, precision=prec)
+        for t, xc, bc, h in terms:
+            a.append(t, xc, bc, h)
+        mask = np.array([[xc not in holds[b] for b in range(a.nbasis())] for xc in range(X.shape[1])])
+        _, lin, hin, _, bst = a.eval_best(mask, 5, 200, False)
+        out[prec] = (lin, hin)
+        best[prec] = (int(bst["j1"][0]), int(bst["j1"][1]), int(bst["j2"][0]), int(bst["j2"][1]),
+                      float(bst["sse1"]), float(bst["sse2"]))
+        B, Bo = a.basis()
+        a.close()
+    assert best[precision] == best["fp64"]
+    # candidates in the basis's span (a duplicate of a basis function) score rounding noise in
+    # either form: they are left out, by the residual of each candidate column against Bo
+    Q = np.asarray(Bo, dtype=np.float64)
+    live = np.zeros(out["fp64"][0].shape, dtype=bool)
+    for xc in range(X.shape[1]):
+        for b in range(Q.shape[1]):
+            v = np.asarray(B[:, b], dtype=np.float64) * X[:, xc]
+            r = v - Q @ (Q.T @ v)
+            live[xc, b] = r @ r > 1e-8 * (v @ v) and xc not in holds[b]
+    for exact, approx in zip(out["fp64"], out[precision]):
+        scale = float(exact.max())
+        np.testing.assert_allclose(approx[live], exact[live], rtol=rtol, atol=1e-7 * scale)
