@@ -153,11 +153,23 @@ struct FitState {
     size_t m_synced = 0;
     const float *y_key = nullptr;
     std::vector<char> xres;
+    // a view (the sorted hinge path's per-X-column fit): its buffers alias storage owned elsewhere
+    bool view = false;
     void release()
     {
+        if (view) {
+            detach();
+            return;
+        }
         X.release(); s.release(); B.release(); Bo.release(); BoC.release(); BC.release(); y.release();
         rbuf.release();
         ybo.release(); v.release(); G.release(); order.release(); delta.release(); tc.release(); tcn.release();
+    }
+    void detach()
+    {
+        for (DBuf<float> *b : {&X, &s, &B, &Bo, &BoC, &BC, &y, &rbuf, &delta}) *b = DBuf<float>{};
+        for (DBuf<double> *b : {&ybo, &v, &G, &tc, &tcn}) *b = DBuf<double>{};
+        order = DBuf<int>{};
     }
 };
 
@@ -334,6 +346,17 @@ struct mars_hip_ctx {
     mh::DBuf<double> hub;            // per candidate: linear + upper bound of its best hinge score
     mh::DBuf<double> hbest2;         // precision-form hinge eval: exact hinge scores of the rescored
     mh::DBuf<int> rknot;             // ... and their knot positions and rows
+    // sorted hinge path: per X column xc, the block rep + xc * rep_stride holds that column's
+    // descending order of x, y, B[:, 0..rep_mcap) and Bo[:, 0..rep_mcap), column-major (ld = ldx,
+    // rows past n zero); its first rep_m[xc] basis columns are current
+    mh::DBuf<float> rep;
+    size_t rep_mcap = 0, rep_stride = 0;
+    std::vector<int> rep_m;
+    int rep_state = 0;               // 0 undecided, 1 in use, -1 the gather sweep instead
+    bool keep_cm = false;            // plain_score keeps normalised columns column-major
+    mh::DBuf<float> hr;              // sorted hinge path: normalised candidate column c at c * ldx
+    mh::Cands vcand;                 // sorted hinge path: the list over the per-column fit views
+    mh::DBuf<int> rep_todo;          // replica extension list
 
     long dgks_host = 0;
     double eval_ms = 0;

@@ -844,6 +844,9 @@ void ctx_free(mars_hip_ctx *ctx)
         b->release();
     for (DBuf<double> *b : {&ctx->clo, &ctx->chi, &ctx->hbest2, &ctx->hbest_ub, &ctx->hub}) b->release();
     ctx->rknot.release();
+    ctx->rep.release();
+    ctx->hr.release();
+    for (DBuf<int> *b : {&ctx->vcand.d_fit, &ctx->vcand.d_xc, &ctx->vcand.d_par, &ctx->rep_todo}) b->release();
     ctx->dl_u64.release();
     ctx->dstat.release();
     ctx->bhas.release();
@@ -1385,9 +1388,12 @@ void hinge_eval_mixed(mars_hip_ctx *ctx, Cands &cs, int min_span, int end_span)
     hipStream_t st = ctx->stream;
     const int P = (int)cs.fit.size();
     const double tol = ctx->fits[0].tol;
-    plain_score(ctx, cs, true, ctx->lin.p, ctx->ybx.p, nullptr, nullptr, false, nullptr, true);
     ctx->hub.ensure(std::max(P, 1));
-    hinge_sweep(ctx, min_span, end_span, ctx->split_bf16 ? kEpsBf16 : kEpsF32, ctx->hub.p);
+    ctx->hin.ensure(std::max(P, 1));
+    ctx->knot.ensure(std::max(P, 1));
+    ctx->krow.ensure(std::max(P, 1));
+    hinge_score(ctx, cs, ctx->lin.p, ctx->ybx.p, ctx->hin.p, ctx->knot.p, ctx->krow.p, min_span, end_span, true,
+                ctx->split_bf16 ? kEpsBf16 : kEpsF32, ctx->hub.p);
     KTimer kt(ctx, K_HRECHECK, 0.0, 0.0);
     ctx->red.ensure(2);
     ctx->rsel.ensure(P);
@@ -1431,9 +1437,8 @@ void hinge_eval_mixed(mars_hip_ctx *ctx, Cands &cs, int min_span, int end_span)
         ctx->hbest2.ensure(R);
         ctx->rknot.ensure(2 * (size_t)R + 1);
         ctx->live_list.ensure(R);
-        plain_score(ctx, rc, true, ctx->rlin.p, ctx->rybx.p);
-        hinge_sweep_list(ctx, rc, ctx->rlin.p, ctx->rybx.p, ctx->hbest2.p, ctx->rknot.p, ctx->rknot.p + R, min_span,
-                         end_span);
+        hinge_score(ctx, rc, ctx->rlin.p, ctx->rybx.p, ctx->hbest2.p, ctx->rknot.p, ctx->rknot.p + R, min_span,
+                    end_span);
         h2d(ctx, ctx->live_list.p, idx.data(), R * sizeof(int));
         hipLaunchKernelGGL(k_hput, dim3((R + 255) / 256), dim3(256), 0, st, (const int *)ctx->live_list.p, R,
                            (const double *)ctx->rlin.p, (const double *)ctx->hbest2.p, (const int *)ctx->rknot.p,
@@ -1524,8 +1529,15 @@ void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t 
             hinge_eval_mixed(ctx, cs, min_span, end_span);
         } else {
             require(linear_only || ctx->row_major, "hinge sweeps need the row-major basis copies");
-            plain_score(ctx, cs, !linear_only, ctx->lin.p, ctx->ybx.p);
-            if (!linear_only) hinge_sweep(ctx, min_span, end_span);
+            if (linear_only) {
+                plain_score(ctx, cs, false, ctx->lin.p, ctx->ybx.p);
+            } else {
+                ctx->hin.ensure(P);
+                ctx->knot.ensure(P);
+                ctx->krow.ensure(P);
+                hinge_score(ctx, cs, ctx->lin.p, ctx->ybx.p, ctx->hin.p, ctx->knot.p, ctx->krow.p, min_span,
+                            end_span);
+            }
         }
         upload_cands(ctx, cs);
         h2d_flush(ctx);

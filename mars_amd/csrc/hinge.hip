@@ -22,6 +22,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace mh {
 namespace {
@@ -153,9 +154,15 @@ struct SweepArgs {
     size_t ldb, ldr, n;
     const double *ybo, *ybx;
     const int *grp;      // per X-column group: xc, first candidate, count
+    const int *cgrp;     // group per candidate
     const int *cpar;     // parent per candidate
     const int *order;    // [px][n]
     const float *delta;  // [px][n]
+    // sorted sweep: X column xc's replica block rep + xc * rstride holds [x | y | B (mcap) | Bo (mcap)]
+    // in the column's order (ld = ldx); candidate c's normalised column, in the same order, is hr + c * ldx
+    const float *rep, *hr;
+    size_t rstride, ldx;
+    int mcap;
     int m, L2, C;        // basis count, covariate slots per candidate, chunks
     size_t tail, head, span, clen;
     double tol;
@@ -210,29 +217,38 @@ __device__ __forceinline__ double group_sum(double v)
 /*
  * One block = NC candidates (parents c0.. of one X column) swept over the
  * sorted rows [r0, r1) of its chunk (marsalgo.cc:576-649).  Tiles of 16 sorted
- * rows are gathered into LDS: row-major [0 | y | 0 | Bo row | 0...] (PT
- * floats), plus b (f64) and Bx per candidate, and the deltas (d, d^2).  The
- * buffers are doubled: the global gathers of tile t+1 are in flight while the
- * unrolled recurrence runs over tile t.  Rows past the chunk end are zero
- * (d = b = 0 leaves every accumulator unchanged), so every tile runs 16 rows.
- * Grid rows and nonzero deltas of a tile are bit masks in scalar registers.
- * At a grid row each lane sums its slots' f.Y (Y = -1 on the y slot, so the
- * sum is fy - w) and k - |f|^2 over the bx and basis slots, and the group
- * reduces both with DPP.
+ * rows go to LDS: row-major [0 | y | 0 | Bo row | 0...] (PT floats, rows
+ * padded by 4), plus b (f64) and Bx per candidate, and the deltas (d, d^2).
+ * SORTED: the tiles are read from the X column's sorted replicas (y, B, Bo in
+ * the column's order) and from the candidates' columns that the projection
+ * produced in that order: every source is a contiguous run of 16 rows (float4
+ * per lane).  Otherwise each tile row is gathered from the row-major B, Bo and
+ * candidate columns through the sort order.  The buffers are doubled: the
+ * global loads of tile t+1 are in flight while the unrolled recurrence runs
+ * over tile t.  Rows past the chunk end are zero (d = b = 0 leaves every
+ * accumulator unchanged), so every tile runs 16 rows.  Grid rows and nonzero
+ * deltas of a tile are bit masks in scalar registers.  At a grid row each lane
+ * sums its slots' f.Y (Y = -1 on the y slot, so the sum is fy - w) and
+ * k - |f|^2 over the bx and basis slots, and the group reduces both with DPP.
  */
-template <int PT, int NC, bool EVAL>
+template <int PT, int NC, bool EVAL, bool SORTED>
 __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepArgs a)
 {
     using S = SweepShape<PT, NC>;
     constexpr int J = S::J, G = S::G, CW = S::CW, NW = S::NW, RPS = S::RPS;
+    constexpr int NT = 64 * NW;                            // threads per block
     static_assert(kTR % RPS == 0, "tile rows");
-    constexpr int NB = kTR / RPS;                          // Bo-row loads per lane and tile
-    constexpr int NPC = 2 * NC * kTR / (64 * NW);          // per-candidate loads per lane and tile
+    constexpr int NB = kTR / RPS;                          // gather: Bo-row loads per lane and tile
+    constexpr int NPC = 2 * NC * kTR / NT;                 // gather: per-candidate loads per lane and tile
+    constexpr int QC = kTR / 4;                            // float4 runs per column and tile
+    constexpr int NQ = (PT * QC + NT - 1) / NT;            // sorted: y / Bo runs per lane and tile
+    constexpr int NQC = (2 * NC * QC + NT - 1) / NT;       // sorted: b / Bx runs per lane and tile
+    constexpr int SQ = PT + 4;                             // LDS row stride of the y / Bo tile
     __shared__ double sbv[2][kTR][NC];                     // b (f64)
     __shared__ double2 sdd[2][kTR];                        // (d, d*d)
-    __shared__ __attribute__((aligned(16))) float sq[2][kTR][PT];
+    __shared__ __attribute__((aligned(16))) float sq[2][kTR][SQ];
     __shared__ float sbx[2][kTR][NC];
-    __shared__ int srow[2][kTR];
+    __shared__ int srow[2][kTR];                           // gather: sorted rows' memory rows
 
     const int m = a.m;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -258,36 +274,112 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
     const int ro = tid / PT;    // row within a block-wide load
     const float *bp = nullptr;
     size_t bld = 0;
-    if (pos == 1) {
-        bp = a.y;
-        bld = 1;
-    } else if (pos >= 3 && pos < m + 3) {
-        bp = a.Bo + (pos - 3);
-        bld = a.ldb;
-    }
     const float *pp[NPC];
     size_t pld[NPC];
     int prr[NPC], pj[NPC];
     bool pisb[NPC];
+    // ---- sorted sources: float4 run i of a tile is column qcol[i] (or candidate slot), rows 4 qq[i]..
+    const float *qs[NQ], *cq[NQC];
+    int qcol[NQ], qq[NQ], cj[NQC], cqq[NQC];
+    bool cisb[NQC];
+    if constexpr (SORTED) {
+        const float *blk = a.rep + (size_t)xc * a.rstride;  // [x | y | B (mcap) | Bo (mcap)]
 #pragma unroll
-    for (int i = 0; i < NPC; ++i) {
-        const int e = tid + 64 * NW * i;
-        pisb[i] = e < NC * kTR;
-        const int rem = e % (NC * kTR);
-        prr[i] = rem / NC;
-        pj[i] = rem % NC;
-        pp[i] = nullptr;
-        pld[i] = 0;
-        if (pj[i] < nact) {
-            if (pisb[i]) {
-                pp[i] = a.B + a.cpar[c0 + sbase + pj[i]];
-                pld[i] = a.ldb;
-            } else {
-                pp[i] = a.rbuf + (c0 + sbase + pj[i]);
-                pld[i] = a.ldr;
+        for (int i = 0; i < NQ; ++i) {
+            const int e = tid + NT * i;
+            const int col = e / QC;
+            qcol[i] = e < PT * QC ? col : -1;
+            qq[i] = e % QC;
+            qs[i] = nullptr;
+            if (col == 1)
+                qs[i] = blk + a.ldx;
+            else if (col >= 3 && col < m + 3)
+                qs[i] = blk + (size_t)(2 + a.mcap + col - 3) * a.ldx;
+        }
+#pragma unroll
+        for (int i = 0; i < NQC; ++i) {
+            const int e = tid + NT * i;
+            const bool ok = e < 2 * NC * QC;
+            cisb[i] = e < NC * QC;
+            const int rem = e % (NC * QC);
+            cj[i] = ok ? rem / QC : -1;
+            cqq[i] = rem % QC;
+            cq[i] = nullptr;
+            if (ok && cj[i] < nact) {
+                const int cc = c0 + sbase + cj[i];
+                cq[i] = cisb[i] ? blk + (size_t)(2 + a.cpar[cc]) * a.ldx : a.hr + (size_t)cc * a.ldx;
+            }
+        }
+    } else {
+        if (pos == 1) {
+            bp = a.y;
+            bld = 1;
+        } else if (pos >= 3 && pos < m + 3) {
+            bp = a.Bo + (pos - 3);
+            bld = a.ldb;
+        }
+#pragma unroll
+        for (int i = 0; i < NPC; ++i) {
+            const int e = tid + NT * i;
+            pisb[i] = e < NC * kTR;
+            const int rem = e % (NC * kTR);
+            prr[i] = rem / NC;
+            pj[i] = rem % NC;
+            pp[i] = nullptr;
+            pld[i] = 0;
+            if (pj[i] < nact) {
+                if (pisb[i]) {
+                    pp[i] = a.B + a.cpar[c0 + sbase + pj[i]];
+                    pld[i] = a.ldb;
+                } else {
+                    pp[i] = a.rbuf + (c0 + sbase + pj[i]);
+                    pld[i] = a.ldr;
+                }
             }
         }
     }
+    // a float4 run of a contiguous source at sorted row r (a multiple of 4), rows past r1 zero
+    auto run4 = [&](const float *p, size_t r) {
+        if (!p) return float4{0.f, 0.f, 0.f, 0.f};
+        const float4 v = *(const float4 *)(p + r);
+        return float4{r < r1 ? v.x : 0.f, r + 1 < r1 ? v.y : 0.f, r + 2 < r1 ? v.z : 0.f,
+                      r + 3 < r1 ? v.w : 0.f};
+    };
+    float4 qv[NQ], cv[NQC];
+    auto sorted_load = [&](int tt) {
+        const size_t tb = r0 + (size_t)tt * kTR;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) qv[i] = run4(qs[i], tb + 4 * qq[i]);
+#pragma unroll
+        for (int i = 0; i < NQC; ++i) cv[i] = run4(cq[i], tb + 4 * cqq[i]);
+    };
+    auto sorted_store = [&](int b) {
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            if (qcol[i] < 0) continue;
+            const int rr = 4 * qq[i];
+            sq[b][rr][qcol[i]] = qv[i].x;
+            sq[b][rr + 1][qcol[i]] = qv[i].y;
+            sq[b][rr + 2][qcol[i]] = qv[i].z;
+            sq[b][rr + 3][qcol[i]] = qv[i].w;
+        }
+#pragma unroll
+        for (int i = 0; i < NQC; ++i) {
+            if (cj[i] < 0) continue;
+            const int rr = 4 * cqq[i];
+            if (cisb[i]) {
+                sbv[b][rr][cj[i]] = (double)cv[i].x;
+                sbv[b][rr + 1][cj[i]] = (double)cv[i].y;
+                sbv[b][rr + 2][cj[i]] = (double)cv[i].z;
+                sbv[b][rr + 3][cj[i]] = (double)cv[i].w;
+            } else {
+                sbx[b][rr][cj[i]] = cv[i].x;
+                sbx[b][rr + 1][cj[i]] = cv[i].y;
+                sbx[b][rr + 2][cj[i]] = cv[i].z;
+                sbx[b][rr + 3][cj[i]] = cv[i].w;
+            }
+        }
+    };
 
     // ---- lane state
     double f[J], g[J], Y[J];
@@ -333,27 +425,34 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
         return km;
     };
 
-    // ---- prologue: row indices of tiles 0 and 1, tile 0 into buffer 0
-    if (tid < kTR) {
-        if (T > 0 && tid < tile_rows(0)) srow[0][tid] = ord[r0 + tid];
-        if (T > 1 && tid < tile_rows(1)) srow[1][tid] = ord[r0 + kTR + tid];
+    // ---- prologue: tile 0 into buffer 0 (gather: with the row indices of tiles 0 and 1)
+    if constexpr (!SORTED) {
+        if (tid < kTR) {
+            if (T > 0 && tid < tile_rows(0)) srow[0][tid] = ord[r0 + tid];
+            if (T > 1 && tid < tile_rows(1)) srow[1][tid] = ord[r0 + kTR + tid];
+        }
+        __syncthreads();
     }
-    __syncthreads();
     unsigned kmask = 0, nzmask = 0;
     if (T > 0) {
         const int nr = tile_rows(0);
+        if constexpr (SORTED) {
+            sorted_load(0);
+            sorted_store(0);
+        } else {
 #pragma unroll
-        for (int i = 0; i < NB; ++i) {
-            const int rr = i * RPS + ro;
-            const bool in = rr < nr && bp;
-            sq[0][rr][pos] = in ? bp[(size_t)srow[0][rr] * bld] : 0.0f;
-        }
+            for (int i = 0; i < NB; ++i) {
+                const int rr = i * RPS + ro;
+                const bool in = rr < nr && bp;
+                sq[0][rr][pos] = in ? bp[(size_t)srow[0][rr] * bld] : 0.0f;
+            }
 #pragma unroll
-        for (int i = 0; i < NPC; ++i) {
-            const bool in = prr[i] < nr && pp[i];
-            const float v = in ? pp[i][(size_t)srow[0][prr[i]] * pld[i]] : 0.0f;
-            if (pisb[i]) sbv[0][prr[i]][pj[i]] = (double)v;
-            else sbx[0][prr[i]][pj[i]] = v;
+            for (int i = 0; i < NPC; ++i) {
+                const bool in = prr[i] < nr && pp[i];
+                const float v = in ? pp[i][(size_t)srow[0][prr[i]] * pld[i]] : 0.0f;
+                if (pisb[i]) sbv[0][prr[i]][pj[i]] = (double)v;
+                else sbx[0][prr[i]][pj[i]] = v;
+            }
         }
         const float dv = lane < nr ? dl[r0 + lane] : 0.0f;
         if (tid < kTR) {
@@ -372,22 +471,26 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
         const size_t tb0 = r0 + (size_t)tt * kTR;
         const int nr1 = tt + 1 < T ? tile_rows(tt + 1) : 0;
         const int nr2 = tt + 2 < T ? tile_rows(tt + 2) : 0;
-        // ---- prefetch tile t+1 (its row indices are in srow[nxt]) and tile t+2's indices
+        // ---- prefetch tile t+1 (gather: its row indices are in srow[nxt]; and tile t+2's indices)
         float bov[NB], pcv[NPC];
+        int nrow = 0;
+        if constexpr (SORTED) {
+            if (nr1 > 0) sorted_load(tt + 1);
+        } else {
 #pragma unroll
-        for (int i = 0; i < NB; ++i) {
-            const int rr = i * RPS + ro;
-            const bool in = rr < nr1 && bp;
-            bov[i] = in ? bp[(size_t)srow[nxt][rr] * bld] : 0.0f;
-        }
+            for (int i = 0; i < NB; ++i) {
+                const int rr = i * RPS + ro;
+                const bool in = rr < nr1 && bp;
+                bov[i] = in ? bp[(size_t)srow[nxt][rr] * bld] : 0.0f;
+            }
 #pragma unroll
-        for (int i = 0; i < NPC; ++i) {
-            const bool in = prr[i] < nr1 && pp[i];
-            pcv[i] = in ? pp[i][(size_t)srow[nxt][prr[i]] * pld[i]] : 0.0f;
+            for (int i = 0; i < NPC; ++i) {
+                const bool in = prr[i] < nr1 && pp[i];
+                pcv[i] = in ? pp[i][(size_t)srow[nxt][prr[i]] * pld[i]] : 0.0f;
+            }
+            if (tid < nr2) nrow = ord[tb0 + 2 * kTR + tid];
         }
         const float nd = lane < nr1 ? dl[tb0 + kTR + lane] : 0.0f;
-        int nrow = 0;
-        if (tid < nr2) nrow = ord[tb0 + 2 * kTR + tid];
 
         // ---- the recurrence over tile t (marsalgo.cc:576-649)
         if (wave_active) {
@@ -463,17 +566,21 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
         }
         // buffer nxt last held tile t-1, which every wave finished before the
         // previous iteration's barrier: store tile t+1 into it
+        if constexpr (SORTED) {
+            if (nr1 > 0) sorted_store(nxt);
+        } else {
 #pragma unroll
-        for (int i = 0; i < NB; ++i) sq[nxt][i * RPS + ro][pos] = bov[i];
+            for (int i = 0; i < NB; ++i) sq[nxt][i * RPS + ro][pos] = bov[i];
 #pragma unroll
-        for (int i = 0; i < NPC; ++i) {
-            if (pisb[i]) sbv[nxt][prr[i]][pj[i]] = (double)pcv[i];
-            else sbx[nxt][prr[i]][pj[i]] = pcv[i];
+            for (int i = 0; i < NPC; ++i) {
+                if (pisb[i]) sbv[nxt][prr[i]][pj[i]] = (double)pcv[i];
+                else sbx[nxt][prr[i]][pj[i]] = pcv[i];
+            }
         }
         if (tid < kTR) {
             const double d = (double)nd;
             sdd[nxt][tid] = make_double2(d, d * d);
-            if (tid < nr2) srow[cur][tid] = nrow;
+            if (!SORTED && tid < nr2) srow[cur][tid] = nrow;
         }
         __syncthreads();
     }
@@ -526,8 +633,7 @@ __global__ void k_sweep_carry(SweepArgs a, const int *__restrict__ cgrp, int P)
 
 /* Best knot over chunks (first maximum in row order), hinge = linear + best; with eps > 0 also
  * hub = linear + the largest upper bound over every knot. */
-__global__ void k_sweep_final(SweepArgs a, int P, const int *__restrict__ cxc,
-                              const double *__restrict__ lin, double *__restrict__ hin,
+__global__ void k_sweep_final(SweepArgs a, int P, const double *__restrict__ lin, double *__restrict__ hin,
                               int *__restrict__ knot, int *__restrict__ krow, double *__restrict__ hub)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -547,31 +653,65 @@ __global__ void k_sweep_final(SweepArgs a, int P, const int *__restrict__ cxc,
     knot[c] = bi;
     if (bi >= 0) {
         hin[c] = lin[c] + b;
-        krow[c] = a.order[(size_t)cxc[c] * a.n + bi];
+        krow[c] = a.order[(size_t)a.grp[3 * a.cgrp[c]] * a.n + bi];
     } else {
         hin[c] = 0.0;
         krow[c] = -1;
     }
 }
 
-template <int PT, int NC>
+template <int PT, int NC, bool SORTED>
 void launch_sweep_nc(const SweepArgs &a, dim3 grid, hipStream_t st, bool eval)
 {
     const dim3 block(64 * SweepShape<PT, NC>::NW);
     if (eval)
-        hipLaunchKernelGGL((k_sweep<PT, NC, true>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((k_sweep<PT, NC, true, SORTED>), grid, block, 0, st, a);
     else
-        hipLaunchKernelGGL((k_sweep<PT, NC, false>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((k_sweep<PT, NC, false, SORTED>), grid, block, 0, st, a);
     MH_LAUNCH_CHECK();
 }
 
-template <int PT>
+template <int PT, bool SORTED>
 void launch_sweep(const SweepArgs &a, dim3 grid, int nc, hipStream_t st, bool eval)
 {
     switch (nc) {
-    case 8: launch_sweep_nc<PT, 8>(a, grid, st, eval); break;
-    case 16: launch_sweep_nc<PT, 16>(a, grid, st, eval); break;
-    default: launch_sweep_nc<PT, 32>(a, grid, st, eval); break;
+    case 8: launch_sweep_nc<PT, 8, SORTED>(a, grid, st, eval); break;
+    case 16: launch_sweep_nc<PT, 16, SORTED>(a, grid, st, eval); break;
+    default: launch_sweep_nc<PT, 32, SORTED>(a, grid, st, eval); break;
+    }
+}
+
+template <bool SORTED>
+void launch_sweep_pt(const SweepArgs &a, dim3 grid, int pt, int nc, hipStream_t st, bool eval)
+{
+    switch (pt) {
+    case 32: launch_sweep<32, SORTED>(a, grid, nc, st, eval); break;
+    case 64: launch_sweep<64, SORTED>(a, grid, nc, st, eval); break;
+    case 128: launch_sweep<128, SORTED>(a, grid, nc, st, eval); break;
+    default: launch_sweep<256, SORTED>(a, grid, nc, st, eval); break;
+    }
+}
+
+/* Extend the sorted replicas: for each listed (xc, k0), rows i < n of column xc's block get
+ * B[:, k] and Bo[:, k] at row order[i] for k0 <= k < m (and x, y when k0 == 0). */
+__global__ __launch_bounds__(256) void k_rep_gather(const int *__restrict__ todo, const int *__restrict__ order,
+                                                    const float *__restrict__ X, const float *__restrict__ y,
+                                                    const float *__restrict__ BC, const float *__restrict__ BoC,
+                                                    size_t ldx, size_t n, int m, int mcap, size_t rstride,
+                                                    float *__restrict__ rep)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int xc = todo[2 * blockIdx.y], k0 = todo[2 * blockIdx.y + 1];
+    const size_t o = (size_t)order[(size_t)xc * n + i];
+    float *blk = rep + (size_t)xc * rstride;
+    if (k0 == 0) {
+        blk[i] = X[(size_t)xc * ldx + o];
+        blk[ldx + i] = y[o];
+    }
+    for (int k = k0; k < m; ++k) {
+        blk[(size_t)(2 + k) * ldx + i] = BC[(size_t)k * ldx + o];
+        blk[(size_t)(2 + mcap + k) * ldx + i] = BoC[(size_t)k * ldx + o];
     }
 }
 
@@ -627,19 +767,8 @@ static void ensure_sorted(mars_hip_ctx *ctx, int xc)
     fs.sorted[xc] = 1;
 }
 
-void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span, double eps, double *hub)
-{
-    Cands &cs = ctx->cand;
-    const size_t P = std::max<size_t>(cs.fit.size(), 1);
-    ctx->hin.ensure(P);
-    ctx->knot.ensure(P);
-    ctx->krow.ensure(P);
-    hinge_sweep_list(ctx, cs, ctx->lin.p, ctx->ybx.p, ctx->hin.p, ctx->knot.p, ctx->krow.p, min_span, end_span, eps,
-                     hub);
-}
-
 void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const double *ybx, double *hin, int *knot,
-                      int *krow, int min_span, int end_span, double eps, double *hub)
+                      int *krow, int min_span, int end_span, double eps, double *hub, const int *dpar, bool sorted)
 {
     FitState &fs = ctx->fits[0];
     const int P = (int)cs.fit.size();
@@ -672,13 +801,14 @@ void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const dou
     const int pgroups = (maxnp + NC - 1) / NC;
     const int wpb = NC / (64 / (PT / (PT == 32 ? 4 : 8)));   // waves per block
     // about sweep_waves waves (65536), chunks of at least sweep_min_rows (512) sorted rows; a multiple of 8
-    // chunks puts the blocks of one (column, chunk) on one XCD (shared L2)
+    // chunks puts the blocks of one (column, chunk) on one XCD (shared L2).  Chunks start at multiples of
+    // the 16-row tile (aligned float4 runs of the sorted sources).
     const int waves = (int)ctx->tune.sweep_waves;
     const int want = std::max(1, (waves + ng * pgroups * wpb - 1) / (ng * pgroups * wpb));
     const int maxc = (int)std::max<size_t>(1, tail / (size_t)ctx->tune.sweep_min_rows);
     int C = std::min(want, maxc);
     if (C >= 8) C -= C % 8;
-    const size_t clen = (tail + C - 1) / C;
+    const size_t clen = ((tail + C - 1) / C + kTR - 1) / kTR * kTR;
 
     DBuf<int> dgrp, dcgrp;
     dgrp.ensure(grp.size());
@@ -690,7 +820,7 @@ void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const dou
     ctx->hbest_i.ensure((size_t)P * C);
     if (hub) ctx->hbest_ub.ensure((size_t)P * C);
 
-    SweepArgs a;
+    SweepArgs a{};
     a.Bo = fs.Bo.p;
     a.B = fs.B.p;
     a.rbuf = fs.rbuf.p;
@@ -703,9 +833,17 @@ void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const dou
     a.ybo = fs.ybo.p;
     a.ybx = ybx;
     a.grp = dgrp.p;
-    a.cpar = cs.d_par.p;
+    a.cgrp = dcgrp.p;
+    a.cpar = dpar ? dpar : cs.d_par.p;
     a.order = fs.order.p;
     a.delta = fs.delta.p;
+    if (sorted) {
+        a.rep = ctx->rep.p;
+        a.hr = ctx->hr.p;
+        a.rstride = ctx->rep_stride;
+        a.ldx = fs.ldx;
+        a.mcap = (int)ctx->rep_mcap;
+    }
     a.m = m;
     a.L2 = L2;
     a.C = C;
@@ -723,10 +861,11 @@ void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const dou
     a.bestub = hub ? ctx->hbest_ub.p : nullptr;
 
     const dim3 grid(C, pgroups, ng);
-    // reference algorithmic work: n (m+1) (4 + 4/min_span) flops per candidate (marsalgo.cc:576-649)
+    // reference algorithmic work: n (m+1) (4 + 4/min_span) flops per candidate (marsalgo.cc:576-649);
+    // bytes per pass: y, the m basis columns and the deltas per X column, b and Bx per candidate
     {
         KTimer kt(ctx, K_SWEEP, (double)P * (double)tail * (m + 1) * (4.0 + 4.0 / min_span),
-                  (double)ng * tail * (4.0 * m + 12.0) + (double)P * tail * 8.0);
+                  (double)ng * tail * (4.0 * m + 8.0) + (double)P * tail * 8.0);
         for (int pass = 0; pass < 2; ++pass) {
             const bool eval = pass == 1;
             if (eval) {
@@ -735,20 +874,154 @@ void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const dou
                                    dcgrp.p, P);
                 MH_LAUNCH_CHECK();
             }
-            switch (PT) {
-            case 32: launch_sweep<32>(a, grid, NC, st, eval); break;
-            case 64: launch_sweep<64>(a, grid, NC, st, eval); break;
-            case 128: launch_sweep<128>(a, grid, NC, st, eval); break;
-            default: launch_sweep<256>(a, grid, NC, st, eval); break;
-            }
+            if (sorted)
+                launch_sweep_pt<true>(a, grid, PT, NC, st, eval);
+            else
+                launch_sweep_pt<false>(a, grid, PT, NC, st, eval);
         }
-        hipLaunchKernelGGL(k_sweep_final, dim3((P + 255) / 256), dim3(256), 0, st, a, P, cs.d_xc.p, lin, hin, knot,
-                           krow, hub);
+        hipLaunchKernelGGL(k_sweep_final, dim3((P + 255) / 256), dim3(256), 0, st, a, P, lin, hin, knot, krow, hub);
         MH_LAUNCH_CHECK();
     }
     ctx_sync(ctx);
     dgrp.release();
     dcgrp.release();
+}
+
+/*
+ * The sorted replicas: decided once per context (they take (2 + 2 max_terms) columns of n floats
+ * per X column, and the candidates' columns up to max_terms per X column; they are used when both
+ * fit in half the free device memory, and never with MARS_HIP_SORTED=0), then brought up to the
+ * current basis for the listed X columns.  Returns false when the gather sweep serves instead.
+ */
+static bool replicas_ready(mars_hip_ctx *ctx, const std::vector<int> &xcs)
+{
+    FitState &fs = ctx->fits[0];
+    hipStream_t st = ctx->stream;
+    if (ctx->rep_state == 0) {
+        ctx->rep_state = -1;
+        const char *env = std::getenv("MARS_HIP_SORTED");
+        if (!(env && env[0] == '0')) {
+            const size_t mcap = std::max<size_t>(ctx->max_terms, 1);
+            const size_t stride = (2 + 2 * mcap) * fs.ldx;
+            const double need = 4.0 * (double)stride * fs.px + 4.0 * (double)fs.ldx * fs.px * mcap;
+            size_t fr = 0, tot = 0;
+            MH_CHECK(hipMemGetInfo(&fr, &tot));
+            if (need <= 0.5 * (double)fr) {
+                ctx->rep.ensure(stride * fs.px);
+                MH_CHECK(hipMemsetAsync(ctx->rep.p, 0, stride * fs.px * sizeof(float), st));
+                ctx->rep_mcap = mcap;
+                ctx->rep_stride = stride;
+                ctx->rep_m.assign(fs.px, 0);
+                ctx->rep_state = 1;
+            }
+        }
+    }
+    if (ctx->rep_state < 0) return false;
+    require(fs.m <= ctx->rep_mcap, "basis exceeds the sorted replicas' capacity");
+    std::vector<int> todo;
+    for (int xc : xcs)
+        if (ctx->rep_m[xc] < (int)fs.m) {
+            todo.push_back(xc);
+            todo.push_back(ctx->rep_m[xc]);
+            ctx->rep_m[xc] = (int)fs.m;
+        }
+    if (todo.empty()) return true;
+    const int cnt = (int)(todo.size() / 2);
+    double cols = 0;
+    for (int i = 0; i < cnt; ++i) cols += 2.0 * ((double)fs.m - todo[2 * i + 1]) + (todo[2 * i + 1] == 0 ? 2 : 0);
+    ctx->rep_todo.ensure(todo.size());
+    h2d(ctx, ctx->rep_todo.p, todo.data(), todo.size() * sizeof(int));
+    KTimer kt(ctx, K_SORT, 0.0, (8.0 * cols + 4.0 * cnt) * (double)fs.n);
+    for (int i0 = 0; i0 < cnt; i0 += 65535) {
+        const int c = std::min(cnt - i0, 65535);
+        hipLaunchKernelGGL(k_rep_gather, dim3((unsigned)((fs.n + 255) / 256), c), dim3(256), 0, st,
+                           (const int *)ctx->rep_todo.p + 2 * i0, (const int *)fs.order.p, (const float *)fs.X.p,
+                           (const float *)fs.y.p, (const float *)fs.BC.p, (const float *)fs.BoC.p, fs.ldx, fs.n,
+                           (int)fs.m, (int)ctx->rep_mcap, ctx->rep_stride, ctx->rep.p);
+        MH_LAUNCH_CHECK();
+    }
+    return true;
+}
+
+/*
+ * ctx->fits replaced, for the scope, by one view per X-column group of the list: a fit of one X
+ * column (x, y, B, Bo from that column's sorted replica, its scale, Bo^T y) whose kept candidate
+ * columns land at hr + c * ldx, column-major.  Scoring through the views runs the projection on
+ * the sorted rows, so the normalised columns come out in the order the sweep walks.
+ */
+struct SortedViews {
+    mars_hip_ctx *ctx;
+    std::vector<FitState> saved;
+    SortedViews(mars_hip_ctx *c, const std::vector<int> &gxc, const std::vector<int> &gc0) : ctx(c)
+    {
+        const FitState &fs = ctx->fits[0];
+        std::vector<FitState> v(gxc.size());
+        const size_t big = size_t(1) << 50;  // the views never grow their buffers
+        for (size_t g = 0; g < gxc.size(); ++g) {
+            FitState &f = v[g];
+            f.view = true;
+            f.n = fs.n;
+            f.px = 1;
+            f.ldx = fs.ldx;
+            f.tol = fs.tol;
+            f.yvar = fs.yvar;
+            f.m = fs.m;
+            float *blk = ctx->rep.p + (size_t)gxc[g] * ctx->rep_stride;
+            f.X.p = blk;
+            f.y.p = blk + fs.ldx;
+            f.BC.p = blk + 2 * fs.ldx;
+            f.BoC.p = blk + (2 + ctx->rep_mcap) * fs.ldx;
+            f.s.p = fs.s.p + gxc[g];
+            f.ybo.p = fs.ybo.p;
+            f.rbuf.p = ctx->hr.p + (size_t)gc0[g] * fs.ldx;
+            for (DBuf<float> *b : {&f.X, &f.y, &f.BC, &f.BoC, &f.s, &f.rbuf}) b->cap = big;
+            f.ybo.cap = big;
+            f.ldr = big;
+        }
+        saved.swap(ctx->fits);
+        ctx->fits.swap(v);
+        ctx->keep_cm = true;
+    }
+    ~SortedViews()
+    {
+        for (FitState &f : ctx->fits) f.detach();
+        ctx->fits.swap(saved);
+        ctx->keep_cm = false;
+    }
+};
+
+void hinge_score(mars_hip_ctx *ctx, Cands &cs, double *lin, double *ybx, double *hin, int *knot, int *krow,
+                 int min_span, int end_span, bool f32, double eps, double *hub)
+{
+    const int P = (int)cs.fit.size();
+    if (P == 0) return;
+    std::vector<int> gxc, gc0;
+    for (int c = 0; c < P; ++c)
+        if (c == 0 || cs.xc[c] != cs.xc[c - 1]) {
+            gxc.push_back(cs.xc[c]);
+            gc0.push_back(c);
+        }
+    for (int xc : gxc) ensure_sorted(ctx, xc);
+    if (!replicas_ready(ctx, gxc)) {
+        plain_score(ctx, cs, true, lin, ybx, nullptr, nullptr, false, nullptr, f32);
+        hinge_sweep_list(ctx, cs, lin, ybx, hin, knot, krow, min_span, end_span, eps, hub);
+        return;
+    }
+    const FitState &fs = ctx->fits[0];
+    ctx->hr.ensure(fs.ldx * (size_t)P);
+    Cands &vc = ctx->vcand;
+    vc.fit.resize(P);
+    vc.xc.assign(P, 0);
+    vc.par = cs.par;
+    for (int g = 0, c = 0; c < P; ++c) {
+        if (g + 1 < (int)gc0.size() && c == gc0[g + 1]) ++g;
+        vc.fit[c] = g;
+    }
+    {
+        SortedViews views(ctx, gxc, gc0);
+        plain_score(ctx, vc, true, lin, ybx, nullptr, nullptr, false, nullptr, f32);
+    }
+    hinge_sweep_list(ctx, cs, lin, ybx, hin, knot, krow, min_span, end_span, eps, hub, vc.d_par.p, true);
 }
 
 } // namespace mh

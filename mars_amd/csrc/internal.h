@@ -72,14 +72,21 @@ void ensure_tcache(mars_hip_ctx *ctx);
  * does not grow them step by step; called when a resident context is created. */
 void reserve_step_buffers(mars_hip_ctx *ctx);
 
-/* Hinge sweep of fit 0's candidates, whose normalized columns sit in its rbuf.  With hub: also,
- * per candidate, linear + an upper bound of every knot's score when each covariate carries a
- * relative error up to eps (the precision form's screen). */
-void hinge_sweep(mars_hip_ctx *ctx, int min_span, int end_span, double eps = 0.0, double *hub = nullptr);
-/* The same over the candidates `cs` (columns c of rbuf), scores lin / ybx in, hinge score, knot
- * position and knot row per candidate out. */
+/* Hinge sweep of fit 0's candidates `cs`, whose normalized columns sit in its rbuf (column c),
+ * scores lin / ybx in, hinge score, knot position and knot row per candidate out.  With hub:
+ * also, per candidate, linear + an upper bound of every knot's score when each covariate carries
+ * a relative error up to eps (the precision form's screen).  dpar: device parents of cs (default cs.d_par).
+ * sorted: the columns are ctx->hr's, in each X column's sort order, and the sweep streams the
+ * sorted replicas instead of gathering rows. */
 void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const double *ybx, double *hin, int *knot,
-                      int *krow, int min_span, int end_span, double eps = 0.0, double *hub = nullptr);
+                      int *krow, int min_span, int end_span, double eps = 0.0, double *hub = nullptr,
+                      const int *dpar = nullptr, bool sorted = false);
+/* Hinge eval of fit 0's candidates `cs` (grouped by X column): projection (FP32 first pass when
+ * f32) and sweep, lin / ybx / hin / knot / krow per candidate out, ctx->flag and ctx->ssum per
+ * candidate as plain_score leaves them.  Runs on the per-column sorted replicas when they fit
+ * the device, else as plain_score + hinge_sweep_list over row-major columns. */
+void hinge_score(mars_hip_ctx *ctx, Cands &cs, double *lin, double *ybx, double *hin, int *knot, int *krow,
+                 int min_span, int end_span, bool f32 = false, double eps = 0.0, double *hub = nullptr);
 
 /* Upload the per-fit device views (current m, rbuf, first candidate of `cs`). */
 void upload_fits(mars_hip_ctx *ctx, const Cands &cs, const float *xover = nullptr,

@@ -150,13 +150,17 @@ __global__ __launch_bounds__(256) void k_finalize(const FitDev *__restrict__ fit
  * in flight), and the wave's sum is formed in a fixed order.  With `list`, item positions
  * index the list of candidates.  A candidate whose residual was not stored (keep entry 0)
  * gets 0 here; if its s > tol, a later pass recomputes it.
+ * wb: the normalised column Bxn = f32(r scale) also replaces r (the sorted hinge path keeps it;
+ * a candidate with s <= tol gets a zero column), except for candidates whose `skip` flag has
+ * bit 0 set (DGKS fired: the retry still needs r; their ybx partial is 0 until it runs).
  */
 __global__ __launch_bounds__(256) void k_finalize_cm(const FitDev *__restrict__ fits,
                                                      const Item *__restrict__ items, int nitems,
                                                      const int *__restrict__ list,
                                                      const int *__restrict__ keep,
                                                      const double *__restrict__ ssum,
-                                                     double *__restrict__ ypart, const int *__restrict__ dn)
+                                                     double *__restrict__ ypart, const int *__restrict__ dn,
+                                                     const int *__restrict__ skip, int wb)
 {
     if (dn) nitems = *dn;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -165,22 +169,27 @@ __global__ __launch_bounds__(256) void k_finalize_cm(const FitDev *__restrict__ 
         const FitDev F = fits[I.fit];
         for (int slot = w; slot < I.nc; slot += 4) {
             const int c = list ? list[I.c0 + slot] : I.c0 + slot;
+            if (skip && (skip[c] & 1)) {
+                if (lane == 0) ypart[(size_t)it * kTileF + slot] = 0.0;
+                continue;
+            }
             const double s = ssum[c];
             const float scale = (s > F.tol) ? (float)(1.0 / sqrt(s + F.tol)) : 0.0f;
             const long long cc = F.rbyl == 2 ? max(F.roff[c], 0ll) : (F.rbyl ? I.c0 + slot - I.pad : c - F.c0);
-            const float *col = F.rbuf + cc * F.rsc;
+            float *col = F.rbuf + cc * F.rsc;
             double acc = 0.0;
             // s <= tol: scale 0, so ybx = 0 exactly (kernels.cc:401-416) and r is not read; most
             // rescored candidates (duplicates of basis functions) end here
-            const long long rend = (s > F.tol && (!keep || keep[c] != 0)) ? I.r1 : I.r0;
+            const bool live = s > F.tol && (!keep || keep[c] != 0);
+            const long long rend = (live || wb) ? I.r1 : I.r0;
             for (long long r = I.r0 + 4 * lane; r < rend; r += 1024) {
                 float4 rv[4], yv[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const long long rr = r + 256 * u;
                     const long long ra = rr < rend ? rr : r;  // clamped: stays inside the column
-                    rv[u] = gld4(col + ra);
-                    yv[u] = gld4(F.y + ra);
+                    rv[u] = live ? gld4(col + ra) : float4{0.f, 0.f, 0.f, 0.f};
+                    yv[u] = live ? gld4(F.y + ra) : float4{0.f, 0.f, 0.f, 0.f};
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -188,9 +197,21 @@ __global__ __launch_bounds__(256) void k_finalize_cm(const FitDev *__restrict__ 
                     if (rr >= rend) continue;
                     const float a[4] = {rv[u].x, rv[u].y, rv[u].z, rv[u].w};
                     const float b[4] = {yv[u].x, yv[u].y, yv[u].z, yv[u].w};
+                    float o[4];
 #pragma unroll
-                    for (int v = 0; v < 4; ++v)
-                        if (rr + v < rend) acc = fma((double)__fmul_rn(a[v], scale), (double)b[v], acc);
+                    for (int v = 0; v < 4; ++v) {
+                        o[v] = __fmul_rn(a[v], scale);
+                        if (rr + v < rend) acc = fma((double)o[v], (double)b[v], acc);
+                    }
+                    if (wb) {
+                        if (rr + 3 < rend) {
+                            *(float4 *)(col + rr) = float4{o[0], o[1], o[2], o[3]};
+                        } else {
+#pragma unroll
+                            for (int v = 0; v < 4; ++v)
+                                if (rr + v < rend) col[rr + v] = o[v];
+                        }
+                    }
                 }
             }
 #pragma unroll
@@ -590,7 +611,8 @@ void retry_compact(mars_hip_ctx *ctx, Cands &cs, const std::vector<int> &fired, 
                            kSlots, nullptr, ctx->d_fits.p, cs.d_fit.p, nullptr, ctx->ssum.p, nullptr, nullptr);
         MH_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_finalize_cm, dim3(grid_for(fF.items.size())), dim3(256), 0, st, ctx->d_fits.p,
-                           wF.items, (int)fF.items.size(), fl, nullptr, ctx->ssum.p, ctx->spart.p, nullptr);
+                           wF.items, (int)fF.items.size(), fl, nullptr, ctx->ssum.p, ctx->spart.p, nullptr, nullptr,
+                           0);
         MH_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_yreduce, dim3((nf + 255) / 256), dim3(256), 0, st, nf, wF.cmap, ctx->spart.p, fl,
                            ybx, lin, nullptr);
@@ -1061,7 +1083,7 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
         } else {
             hipLaunchKernelGGL(k_finalize_cm, dim3((unsigned)std::min<size_t>(capF, 4096)), b256, 0, st, fits,
                                (const Item *)pF.items, (int)capF, nullptr, (const int *)keep,
-                               (const double *)ctx->ssum.p, ctx->spart.p, (const int *)nF);
+                               (const double *)ctx->ssum.p, ctx->spart.p, (const int *)nF, nullptr, 0);
             MH_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_yreduce, gB, b256, 0, st, bound, (const int *)pF.cmap, (const double *)ctx->spart.p,
                                nullptr, ry, rl, cnt_dev);
@@ -1102,7 +1124,7 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
                            ctx->ssum.p, nullptr, (const int *)dnf);
         MH_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_finalize_cm, dim3(256), b256, 0, st, fits, (const Item *)fF.items, (int)capF, fl, nullptr,
-                           (const double *)ctx->ssum.p, ctx->spart.p, (const int *)nFF);
+                           (const double *)ctx->ssum.p, ctx->spart.p, (const int *)nFF, nullptr, 0);
         MH_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_yreduce, dim3(64), b256, 0, st, bound, (const int *)fF.cmap, (const double *)ctx->spart.p,
                            fl, ry, rl, (const int *)dnf);
@@ -1151,9 +1173,10 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
     // the plain form's linear scoring stores no residual in the fits' buffers: its DGKS-fired
     // candidates keep theirs in the bounded retry arena (retry_compact)
     if (store_r) ensure_rbuf(ctx, cs);
-    // normalised columns kept for the hinge sweep or a download stay row-major; residuals that
-    // only linear scoring reads are column-major (contiguous per candidate)
-    ctx->rbuf_cm = !keep_bxn;
+    // normalised columns kept for the gather sweep or a download stay row-major; residuals that
+    // only linear scoring reads, and the sorted hinge path's columns (keep_cm), are column-major
+    // (contiguous per candidate)
+    ctx->rbuf_cm = !keep_bxn || ctx->keep_cm;
     upload_fits(ctx, cs, xover, sover);
     upload_cands(ctx, cs);
 
@@ -1241,9 +1264,10 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                            ctx->ssum.p, ctx->flag.p, nullptr);
         MH_LAUNCH_CHECK();
     }
-    // Stored column-major residuals (the mixed form's rescoring) are finalized before the host
-    // has read the DGKS flags: the device runs pass 3 over every candidate while the flags come
-    // back, and the few fired candidates are finalized again after their retry.
+    // Stored column-major residuals (the mixed form's rescoring, the sorted hinge path) are
+    // finalized before the host has read the DGKS flags: the device runs pass 3 over every
+    // candidate while the flags come back, and the few fired candidates are finalized again after
+    // their retry (a kept column is normalised in place only once its residual is final).
     const bool spec = store_r && ctx->rbuf_cm;
     const int nF = (int)pF.items.size();
     std::vector<int> flag_v;
@@ -1265,7 +1289,8 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
             for (size_t f = 0; f < F; ++f) fb2 += 4.0 * n[f] * cnt[f] + 4.0 * n[f];
             KTimer kt(ctx, K_FINALIZE, 0.0, fb2);
             hipLaunchKernelGGL(k_finalize_cm, dim3(grid_for(pF.items.size())), dim3(256), 0, st, ctx->d_fits.p,
-                               uF.items, nF, nullptr, keep, ctx->ssum.p, ctx->spart.p, nullptr);
+                               uF.items, nF, nullptr, keep, ctx->ssum.p, ctx->spart.p, nullptr,
+                               keep_bxn ? (const int *)ctx->flag.p : nullptr, keep_bxn ? 1 : 0);
             MH_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, P, uF.cmap, ctx->spart.p,
                                nullptr, ybx, lin, nullptr);
@@ -1341,7 +1366,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
             ctx->spart.ensure(fF.items.size() * kTileF + 1);
             hipLaunchKernelGGL(k_finalize_cm, dim3(grid_for(fF.items.size())), dim3(256), 0, st, ctx->d_fits.p,
                                wF.items, (int)fF.items.size(), (const int *)ctx->fired_list.p, nullptr,
-                               ctx->ssum.p, ctx->spart.p, nullptr);
+                               ctx->ssum.p, ctx->spart.p, nullptr, nullptr, keep_bxn ? 1 : 0);
             MH_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_yreduce, dim3((nf + 255) / 256), dim3(256), 0, st, nf, wF.cmap, ctx->spart.p,
                                ctx->fired_list.p, ybx, lin, nullptr);

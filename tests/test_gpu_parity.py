@@ -172,11 +172,18 @@ def test_fit_matches_reference(path):
     _compare_to_golden(gold, model, trace, 1e-10)
 
 
-def test_eval_arrays_vs_oracle(oracle):
-    """Per-candidate linear / hinge / cut arrays of one mid-fit step against the oracle."""
+@pytest.mark.parametrize("sweep", ["sorted", "gather"])
+def test_eval_arrays_vs_oracle(oracle, sweep, monkeypatch):
+    """Per-candidate linear / hinge / cut arrays of one mid-fit step against the oracle, with the
+    hinge sweep streaming the per-column sorted replicas (the default) and gathering rows through
+    the sort order (MARS_HIP_SORTED=0, the path for devices the replicas do not fit)."""
     import mars_amd
 
-    say("eval arrays")
+    say("eval arrays", sweep)
+    if sweep == "gather":
+        monkeypatch.setenv("MARS_HIP_SORTED", "0")
+    else:
+        monkeypatch.delenv("MARS_HIP_SORTED", raising=False)
     case = CASES["w2_small"]
     X, y, w = make_inputs(case)
     w = np.ones(len(X), dtype=np.float32) if w is None else w
@@ -206,8 +213,9 @@ def test_eval_arrays_vs_oracle(oracle):
     assert best["knot2"] == k0[j2]
 
 
+@pytest.mark.parametrize("sweep", ["sorted", "gather"])
 @pytest.mark.parametrize("n,p,mt,widest", [(6000, 6, 61, 32), (8000, 6, 118, 64)])
-def test_wide_parent_blocks_vs_oracle(oracle, n, p, mt, widest):
+def test_wide_parent_blocks_vs_oracle(oracle, n, p, mt, widest, sweep, monkeypatch):
     """Hinge fits whose late steps score more than 32 parents per X column (two 32-candidate
     sweep blocks per column; beyond 64, three) against the oracle: same terms and knots, mse
     within 1e-10.  The stopping rules are off so that the fit runs to max_terms.  The second
@@ -219,7 +227,11 @@ def test_wide_parent_blocks_vs_oracle(oracle, n, p, mt, widest):
     from mars_amd import datagen
     from oracle import fit as F
 
-    say("wide parent blocks n=%d p=%d max_terms=%d" % (n, p, mt))
+    say("wide parent blocks n=%d p=%d max_terms=%d %s" % (n, p, mt, sweep))
+    if sweep == "gather":
+        monkeypatch.setenv("MARS_HIP_SORTED", "0")
+    else:
+        monkeypatch.delenv("MARS_HIP_SORTED", raising=False)
     X, y = datagen.workload2(n=n, p=p)
     kw = dict(max_terms=mt, linear_only=False, max_degree=5, r2_thresh=0.0, r2_window=1000)
     want, wtrace = F.fit(oracle, X, y, None, **kw)
