@@ -28,7 +28,8 @@ namespace mh {
 namespace {
 
 /* ---------------------------------------------------------------------- */
-/* Stable descending sort of one X column: LSD radix, 8-bit digits        */
+/* Stable descending sort of X columns: LSD radix, 8-bit digits, a batch   */
+/* of columns per launch (blockIdx.y = the column's place in the batch)    */
 /* ---------------------------------------------------------------------- */
 constexpr int kSortTile = 4096;  // 256 threads x 16 elements
 
@@ -40,19 +41,24 @@ __device__ inline unsigned desc_key(float v)
     return ~u;                                       // descending order of v
 }
 
-__global__ __launch_bounds__(256) void k_sort_keys(const float *__restrict__ x, size_t n,
-                                                   unsigned *__restrict__ key, int *__restrict__ val)
+/* Column y of the batch: keys and values of X column cols[y] at key + 2 n y, val + 2 n y. */
+__global__ __launch_bounds__(256) void k_sort_keys(const float *__restrict__ X, size_t ldx, const int *__restrict__ cols,
+                                                   size_t n, unsigned *__restrict__ key, int *__restrict__ val)
 {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    key[i] = desc_key(x[i]);
-    val[i] = (int)i;
+    const size_t y = blockIdx.y;
+    key[2 * n * y + i] = desc_key(X[(size_t)cols[y] * ldx + i]);
+    val[2 * n * y + i] = (int)i;
 }
 
-__global__ __launch_bounds__(256) void k_sort_hist(const unsigned *__restrict__ key, size_t n,
-                                                   int shift, int nblocks, unsigned *__restrict__ hist)
+/* Digit histogram per tile: hist[y][digit][tile]. */
+__global__ __launch_bounds__(256) void k_sort_hist(const unsigned *__restrict__ key, size_t n, int half, int shift,
+                                                   int nblocks, unsigned *__restrict__ hist)
 {
     __shared__ unsigned h[256];
+    const size_t y = blockIdx.y;
+    key += 2 * n * y + (size_t)half * n;
     h[threadIdx.x] = 0;
     __syncthreads();
     const size_t b0 = (size_t)blockIdx.x * kSortTile;
@@ -61,13 +67,14 @@ __global__ __launch_bounds__(256) void k_sort_hist(const unsigned *__restrict__ 
         if (i < n) atomicAdd(&h[(key[i] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    hist[(y * 256 + threadIdx.x) * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-/* Exclusive scan of the digit-major histogram (single block). */
+/* Exclusive scan of each column's digit-major histogram (one block per column). */
 __global__ __launch_bounds__(1024) void k_sort_scan(unsigned *__restrict__ hist, int count)
 {
     __shared__ unsigned sh[1024];
+    hist += (size_t)blockIdx.x * count;
     const int t = threadIdx.x;
     const int chunk = (count + 1023) / 1024;
     const int b = t * chunk, e = min(count, b + chunk);
@@ -89,18 +96,21 @@ __global__ __launch_bounds__(1024) void k_sort_scan(unsigned *__restrict__ hist,
     }
 }
 
-/* Stable scatter: rank within the tile = earlier elements with the same digit. */
-__global__ __launch_bounds__(256) void k_sort_scatter(const unsigned *__restrict__ kin,
-                                                      const int *__restrict__ vin, size_t n,
-                                                      int shift, int nblocks,
-                                                      const unsigned *__restrict__ offs,
-                                                      unsigned *__restrict__ kout,
-                                                      int *__restrict__ vout)
+/* Stable scatter from half `half` to the other: rank within the tile = earlier elements with the
+ * same digit. */
+__global__ __launch_bounds__(256) void k_sort_scatter(unsigned *__restrict__ key, int *__restrict__ val, size_t n,
+                                                      int half, int shift, int nblocks,
+                                                      const unsigned *__restrict__ offs)
 {
     __shared__ unsigned base[256];
     __shared__ unsigned wcnt[4][256];
+    const size_t y = blockIdx.y;
+    const unsigned *kin = key + 2 * n * y + (size_t)half * n;
+    const int *vin = val + 2 * n * y + (size_t)half * n;
+    unsigned *kout = key + 2 * n * y + (size_t)(1 - half) * n;
+    int *vout = val + 2 * n * y + (size_t)(1 - half) * n;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    base[t] = offs[(size_t)t * nblocks + blockIdx.x];
+    base[t] = offs[(y * 256 + t) * nblocks + blockIdx.x];
     const size_t b0 = (size_t)blockIdx.x * kSortTile;
     const unsigned long long lt = (1ull << lane) - 1ull;
     for (int r = 0; r < kSortTile / 256; ++r) {
@@ -130,20 +140,24 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const unsigned *__restrict
     }
 }
 
-/* d[i] = f32(xs[k[i-1]] - xs[k[i]]) on the scaled column (marsalgo.cc:526-529), d[0] = 0. */
-__global__ __launch_bounds__(256) void k_deltas(const float *__restrict__ x, float s,
-                                                const int *__restrict__ ord, size_t n,
-                                                float *__restrict__ d)
+/* The sorted rows of column cols[y] (half 0 of its values) into order[xc][.], and the deltas
+ * d[i] = f32(xs[k[i-1]] - xs[k[i]]) on the scaled column (marsalgo.cc:526-529), d[0] = 0. */
+__global__ __launch_bounds__(256) void k_sort_out(const int *__restrict__ val, const float *__restrict__ X, size_t ldx,
+                                                  const float *__restrict__ scale, const int *__restrict__ cols,
+                                                  size_t n, int *__restrict__ order, float *__restrict__ delta)
 {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    if (i == 0) {
-        d[0] = 0.0f;
-        return;
-    }
-    const float a = __fmul_rn(x[ord[i - 1]], s);
-    const float b = __fmul_rn(x[ord[i]], s);
-    d[i] = __fsub_rn(a, b);
+    const size_t y = blockIdx.y;
+    const int xc = cols[y];
+    const int *v = val + 2 * n * y;
+    const float *x = X + (size_t)xc * ldx;
+    const float s = scale[xc];
+    const int k = v[i];
+    order[(size_t)xc * n + i] = k;
+    float d = 0.0f;
+    if (i > 0) d = __fsub_rn(__fmul_rn(x[v[i - 1]], s), __fmul_rn(x[k], s));
+    delta[(size_t)xc * n + i] = d;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -160,7 +174,7 @@ struct SweepArgs {
     const float *delta;  // [px][n]
     // sorted sweep: X column xc's replica block rep + xc * rstride holds [x | y | B (mcap) | Bo (mcap)]
     // in the column's order (ld = ldx); candidate c's normalised column, in the same order, is hr + c * ldx
-    const float *rep, *hr;
+    const float *rep, *hr, *zero;   // zero: a column of ldx zeros (runs of absent columns)
     size_t rstride, ldx;
     int mcap;
     int m, L2, C;        // basis count, covariate slots per candidate, chunks
@@ -183,13 +197,20 @@ constexpr int kTR = 16;    // sorted rows per tile
  * u = 2 the candidate's Bx column, u = 3 .. m+2 the orthonormal basis, the
  * rest zero padding up to PT = G J slots.
  */
+/* Slots per lane: 8 or 12, with 4 lanes per candidate up to 48 slots (blocks of 16 or 32
+ * candidates), then 8, 16 or 32 lanes; blocks of 8 candidates keep 8 lanes below 64 slots. */
+constexpr int sweep_j(int pt, int nc)
+{
+    return pt == 32 ? (nc >= 16 ? 8 : 4) : (pt == 48 ? (nc >= 16 ? 12 : 6) : (pt == 96 || pt == 192 ? 12 : 8));
+}
+
 template <int PT, int NC>
 struct SweepShape {
-    static constexpr int J = PT == 32 ? 4 : 8;          // slots per lane
+    static constexpr int J = sweep_j(PT, NC);
     static constexpr int G = PT / J;                    // lanes per candidate
     static constexpr int CW = 64 / G;                   // candidates per wave
     static constexpr int NW = NC / CW;                  // waves per block
-    static constexpr int RPS = 64 * NW / PT;            // tile rows per block-wide load
+    static constexpr int RPS = 64 * NW / PT;            // gather: tile rows per block-wide load
 };
 
 /* v from the lane DPP control CTRL names (a permutation within each row of 16 lanes). */
@@ -208,7 +229,7 @@ __device__ __forceinline__ double group_sum(double v)
 {
     v = v + dpp_perm<0xB1>(v);   // quad_perm [1,0,3,2]
     v = v + dpp_perm<0x4E>(v);   // quad_perm [2,3,0,1]
-    v = v + dpp_perm<0x141>(v);  // row_half_mirror: lane i <-> 7-i
+    if (G >= 8) v = v + dpp_perm<0x141>(v);  // row_half_mirror: lane i <-> 7-i
     if (G >= 16) v = v + dpp_perm<0x140>(v);  // row_mirror: lane i <-> 15-i
     if (G >= 32) v = v + __shfl_xor(v, 16, 64);
     return v;
@@ -217,19 +238,22 @@ __device__ __forceinline__ double group_sum(double v)
 /*
  * One block = NC candidates (parents c0.. of one X column) swept over the
  * sorted rows [r0, r1) of its chunk (marsalgo.cc:576-649).  Tiles of 16 sorted
- * rows go to LDS: row-major [0 | y | 0 | Bo row | 0...] (PT floats, rows
- * padded by 4), plus b (f64) and Bx per candidate, and the deltas (d, d^2).
+ * rows go to LDS in FP64: row-major [0 | y | 0 | Bo row | 0...] (PT values,
+ * rows padded by 2), (b, Bx) per candidate, and the deltas d.
  * SORTED: the tiles are read from the X column's sorted replicas (y, B, Bo in
  * the column's order) and from the candidates' columns that the projection
  * produced in that order: every source is a contiguous run of 16 rows (float4
- * per lane).  Otherwise each tile row is gathered from the row-major B, Bo and
- * candidate columns through the sort order.  The buffers are doubled: the
- * global loads of tile t+1 are in flight while the unrolled recurrence runs
- * over tile t.  Rows past the chunk end are zero (d = b = 0 leaves every
- * accumulator unchanged), so every tile runs 16 rows.  Grid rows and nonzero
- * deltas of a tile are bit masks in scalar registers.  At a grid row each lane
- * sums its slots' f.Y (Y = -1 on the y slot, so the sum is fy - w) and
- * k - |f|^2 over the bx and basis slots, and the group reduces both with DPP.
+ * per lane; absent columns read a zero column).  Otherwise each tile row is
+ * gathered from the row-major B, Bo and candidate columns through the sort
+ * order.  The buffers are doubled: the global loads of tile t+1 are in flight
+ * while the unrolled recurrence runs over tile t.  Rows past the chunk end are
+ * zero (d = b = 0 leaves every accumulator unchanged), so every tile runs 16
+ * rows; chunks start at multiples of 16, so only the last tile before the tail
+ * is partial.  Grid rows and nonzero deltas of a tile are bit masks in scalar
+ * registers.  At a grid row each lane sums its slots' f.Y (Y = -1 on the y
+ * slot, so the sum is fy - w) and k - |f|^2 over the bx and basis slots, the
+ * group reduces both with DPP, and the quotient is formed only where it can
+ * exceed the chunk's best so far.
  */
 template <int PT, int NC, bool EVAL, bool SORTED>
 __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepArgs a)
@@ -237,17 +261,19 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
     using S = SweepShape<PT, NC>;
     constexpr int J = S::J, G = S::G, CW = S::CW, NW = S::NW, RPS = S::RPS;
     constexpr int NT = 64 * NW;                            // threads per block
-    static_assert(kTR % RPS == 0, "tile rows");
-    constexpr int NB = kTR / RPS;                          // gather: Bo-row loads per lane and tile
+    static_assert(SORTED || (RPS > 0 && kTR % RPS == 0 && 64 * NW == RPS * PT), "tile rows");
+    static_assert(J % 2 == 0, "slot pairs");
+    constexpr int NB = SORTED ? 1 : kTR / RPS;             // gather: Bo-row loads per lane and tile
     constexpr int NPC = 2 * NC * kTR / NT;                 // gather: per-candidate loads per lane and tile
     constexpr int QC = kTR / 4;                            // float4 runs per column and tile
     constexpr int NQ = (PT * QC + NT - 1) / NT;            // sorted: y / Bo runs per lane and tile
     constexpr int NQC = (2 * NC * QC + NT - 1) / NT;       // sorted: b / Bx runs per lane and tile
-    constexpr int SQ = PT + 4;                             // LDS row stride of the y / Bo tile
-    __shared__ double sbv[2][kTR][NC];                     // b (f64)
-    __shared__ double2 sdd[2][kTR];                        // (d, d*d)
-    __shared__ __attribute__((aligned(16))) float sq[2][kTR][SQ];
-    __shared__ float sbx[2][kTR][NC];
+    constexpr bool QPART = NQ * NT > PT * QC;              // some lanes have no run
+    constexpr bool CPART = NQC * NT > 2 * NC * QC;
+    constexpr int SQ = PT + 2;                             // LDS row stride of the y / Bo tile
+    __shared__ __attribute__((aligned(16))) double sq[2][kTR][SQ];
+    __shared__ double2 sbx[2][kTR][NC];                    // (b, Bx) per candidate
+    __shared__ double sd[2][kTR];                          // d
     __shared__ int srow[2][kTR];                           // gather: sorted rows' memory rows
 
     const int m = a.m;
@@ -278,23 +304,25 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
     size_t pld[NPC];
     int prr[NPC], pj[NPC];
     bool pisb[NPC];
-    // ---- sorted sources: float4 run i of a tile is column qcol[i] (or candidate slot), rows 4 qq[i]..
+    // ---- sorted sources: float4 run i of a tile is column qcol[i] (or candidate value cj[i]), rows 4 qq[i]..
     const float *qs[NQ], *cq[NQC];
     int qcol[NQ], qq[NQ], cj[NQC], cqq[NQC];
     bool cisb[NQC];
     if constexpr (SORTED) {
         const float *blk = a.rep + (size_t)xc * a.rstride;  // [x | y | B (mcap) | Bo (mcap)]
+        const size_t rb = r0;                                // runs are addressed from the chunk start
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
             const int e = tid + NT * i;
             const int col = e / QC;
             qcol[i] = e < PT * QC ? col : -1;
             qq[i] = e % QC;
-            qs[i] = nullptr;
+            const float *p = a.zero;
             if (col == 1)
-                qs[i] = blk + a.ldx;
+                p = blk + a.ldx;
             else if (col >= 3 && col < m + 3)
-                qs[i] = blk + (size_t)(2 + a.mcap + col - 3) * a.ldx;
+                p = blk + (size_t)(2 + a.mcap + col - 3) * a.ldx;
+            qs[i] = p + rb + 4 * qq[i];
         }
 #pragma unroll
         for (int i = 0; i < NQC; ++i) {
@@ -304,11 +332,12 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
             const int rem = e % (NC * QC);
             cj[i] = ok ? rem / QC : -1;
             cqq[i] = rem % QC;
-            cq[i] = nullptr;
+            const float *p = a.zero;
             if (ok && cj[i] < nact) {
                 const int cc = c0 + sbase + cj[i];
-                cq[i] = cisb[i] ? blk + (size_t)(2 + a.cpar[cc]) * a.ldx : a.hr + (size_t)cc * a.ldx;
+                p = cisb[i] ? blk + (size_t)(2 + a.cpar[cc]) * a.ldx : a.hr + (size_t)cc * a.ldx;
             }
+            cq[i] = p + rb + 4 * cqq[i];
         }
     } else {
         if (pos == 1) {
@@ -338,53 +367,52 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
             }
         }
     }
-    // a float4 run of a contiguous source at sorted row r (a multiple of 4), rows past r1 zero
-    auto run4 = [&](const float *p, size_t r) {
-        if (!p) return float4{0.f, 0.f, 0.f, 0.f};
-        const float4 v = *(const float4 *)(p + r);
-        return float4{r < r1 ? v.x : 0.f, r + 1 < r1 ? v.y : 0.f, r + 2 < r1 ? v.z : 0.f,
-                      r + 3 < r1 ? v.w : 0.f};
-    };
+    // sorted: the runs of tile tt; rows at or past r1 (only in the tile before the tail) read as zero
     float4 qv[NQ], cv[NQC];
     auto sorted_load = [&](int tt) {
-        const size_t tb = r0 + (size_t)tt * kTR;
+        const size_t off = (size_t)tt * kTR;
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) qv[i] = run4(qs[i], tb + 4 * qq[i]);
+        for (int i = 0; i < NQ; ++i) qv[i] = *(const float4 *)(qs[i] + off);
 #pragma unroll
-        for (int i = 0; i < NQC; ++i) cv[i] = run4(cq[i], tb + 4 * cqq[i]);
+        for (int i = 0; i < NQC; ++i) cv[i] = *(const float4 *)(cq[i] + off);
+        const size_t tb = r0 + off;
+        if (tb + kTR > r1) {   // (uniform)
+            auto cut = [&](float4 v, int q) {
+                const size_t r = tb + 4 * q;
+                return float4{r < r1 ? v.x : 0.f, r + 1 < r1 ? v.y : 0.f, r + 2 < r1 ? v.z : 0.f,
+                              r + 3 < r1 ? v.w : 0.f};
+            };
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) qv[i] = cut(qv[i], qq[i]);
+#pragma unroll
+            for (int i = 0; i < NQC; ++i) cv[i] = cut(cv[i], cqq[i]);
+        }
     };
     auto sorted_store = [&](int b) {
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
-            if (qcol[i] < 0) continue;
+            if (QPART && qcol[i] < 0) continue;
             const int rr = 4 * qq[i];
-            sq[b][rr][qcol[i]] = qv[i].x;
-            sq[b][rr + 1][qcol[i]] = qv[i].y;
-            sq[b][rr + 2][qcol[i]] = qv[i].z;
-            sq[b][rr + 3][qcol[i]] = qv[i].w;
+            sq[b][rr][qcol[i]] = (double)qv[i].x;
+            sq[b][rr + 1][qcol[i]] = (double)qv[i].y;
+            sq[b][rr + 2][qcol[i]] = (double)qv[i].z;
+            sq[b][rr + 3][qcol[i]] = (double)qv[i].w;
         }
 #pragma unroll
         for (int i = 0; i < NQC; ++i) {
-            if (cj[i] < 0) continue;
+            if (CPART && cj[i] < 0) continue;
             const int rr = 4 * cqq[i];
-            if (cisb[i]) {
-                sbv[b][rr][cj[i]] = (double)cv[i].x;
-                sbv[b][rr + 1][cj[i]] = (double)cv[i].y;
-                sbv[b][rr + 2][cj[i]] = (double)cv[i].z;
-                sbv[b][rr + 3][cj[i]] = (double)cv[i].w;
-            } else {
-                sbx[b][rr][cj[i]] = cv[i].x;
-                sbx[b][rr + 1][cj[i]] = cv[i].y;
-                sbx[b][rr + 2][cj[i]] = cv[i].z;
-                sbx[b][rr + 3][cj[i]] = cv[i].w;
-            }
+            double *dst = cisb[i] ? &sbx[b][rr][cj[i]].x : &sbx[b][rr][cj[i]].y;
+            dst[0] = (double)cv[i].x;
+            dst[2 * NC] = (double)cv[i].y;
+            dst[4 * NC] = (double)cv[i].z;
+            dst[6 * NC] = (double)cv[i].w;
         }
     };
 
     // ---- lane state
     double f[J], g[J], Y[J];
     const double isb = t == 0 ? 1.0 : 0.0;   // this lane holds the b, y and bx slots
-    const float isbf = t == 0 ? 1.0f : 0.0f;
     double k2 = 0.0;
 #pragma unroll
     for (int s = 0; s < J; ++s) {
@@ -416,7 +444,7 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
         for (int o = 32; o > 0; o >>= 1) mine = max(mine, __shfl_xor(mine, o, 64));
         lastnz = mine;
     }
-    double bsse = 0.0, dacc = 0.0, bub = 0.0;
+    double bsse = 0.0, bub = 0.0;
     int bidx = -1;
     auto grid_mask = [&](int tt) {
         const size_t tb = r0 + (size_t)tt * kTR, te = tb + tile_rows(tt);
@@ -444,21 +472,18 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
             for (int i = 0; i < NB; ++i) {
                 const int rr = i * RPS + ro;
                 const bool in = rr < nr && bp;
-                sq[0][rr][pos] = in ? bp[(size_t)srow[0][rr] * bld] : 0.0f;
+                sq[0][rr][pos] = in ? (double)bp[(size_t)srow[0][rr] * bld] : 0.0;
             }
 #pragma unroll
             for (int i = 0; i < NPC; ++i) {
                 const bool in = prr[i] < nr && pp[i];
-                const float v = in ? pp[i][(size_t)srow[0][prr[i]] * pld[i]] : 0.0f;
-                if (pisb[i]) sbv[0][prr[i]][pj[i]] = (double)v;
-                else sbx[0][prr[i]][pj[i]] = v;
+                const double v = in ? (double)pp[i][(size_t)srow[0][prr[i]] * pld[i]] : 0.0;
+                if (pisb[i]) sbx[0][prr[i]][pj[i]].x = v;
+                else sbx[0][prr[i]][pj[i]].y = v;
             }
         }
         const float dv = lane < nr ? dl[r0 + lane] : 0.0f;
-        if (tid < kTR) {
-            const double d = (double)dv;
-            sdd[0][tid] = make_double2(d, d * d);
-        }
+        if (tid < kTR) sd[0][tid] = (double)dv;
         if (EVAL) {
             kmask = grid_mask(0);
             nzmask = (unsigned)__ballot(dv != 0.0f);
@@ -496,28 +521,25 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
         if (wave_active) {
 #pragma unroll
             for (int rr = 0; rr < kTR; ++rr) {
-                const double2 dd = sdd[cur][rr];
-                const double b = sbv[cur][rr][slot];
-                float qf[J];
+                const double d = sd[cur][rr];
+                const double2 bx = sbx[cur][rr][slot];   // (b, Bx)
+                double q[J];
 #pragma unroll
-                for (int s = 0; s < J; s += 4) {
-                    const float4 v4 = *(const float4 *)&sq[cur][rr][t * J + s];
-                    qf[s] = v4.x;
-                    qf[s + 1] = v4.y;
-                    qf[s + 2] = v4.z;
-                    qf[s + 3] = v4.w;
+                for (int s = 0; s < J; s += 2) {
+                    const double2 v2 = *(const double2 *)&sq[cur][rr][t * J + s];
+                    q[s] = v2.x;
+                    q[s + 1] = v2.y;
                 }
-                qf[2] = fmaf(isbf, sbx[cur][rr][slot], qf[2]);   // bx on the group's first lane
-                // k uses the b slot's (f, g) before this row's update
-                k2 = fma(dd.y, g[0], k2);
-                k2 = fma(dd.x * 2, f[0], k2);
+                q[0] = fma(isb, bx.x, q[0]);   // b and Bx on the group's first lane (its tile slots 0, 2 are 0)
+                q[2] = fma(isb, bx.y, q[2]);
+                const double fb = f[0];
 #pragma unroll
                 for (int s = 0; s < J; ++s) {
-                    const double q = s == 0 ? fma(isb, b, (double)qf[0]) : (double)qf[s];
-                    f[s] = fma(dd.x, g[s], f[s]);
-                    g[s] = fma(b, q, g[s]);
+                    f[s] = fma(d, g[s], f[s]);
+                    g[s] = fma(bx.x, q[s], g[s]);
                 }
-                if (!EVAL) dacc += dd.x;
+                // k += d^2 g_b + 2 d f_b with the b slot's (f, g) before the row = d (f_b + f_b')
+                k2 = fma(d, fb + f[0], k2);
                 if (EVAL && ((kmask >> rr) & 1u)) {
                     const size_t i = tb0 + rr;
                     const unsigned nzb = nzmask & (0xffffffffu >> (31 - rr));
@@ -535,23 +557,27 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
                         double den = fma(isb, k2, -ff);
                         uw = group_sum<G>(uw);
                         den = group_sum<G>(den);
-                        const double sse = den > a.tol ? (uw * uw) / (den + a.tol) : 0.0;
-                        if (sse > bsse) {
-                            bsse = sse;
-                            bidx = (int)i;
-                        }
-                        if (a.eps > 0) {
-                            // relative errors eps of the covariates move den = k - |f|^2 by at most
-                            // eps (k + |f|^2) and fy - w by eps sum |f_s Y_s|: the score's bound
-                            double ua = 0.0;
+                        // the quotient only where it can beat the best so far (the product test has
+                        // a margin far beyond its rounding); the winner's value is the quotient
+                        const double u2 = uw * uw, dt = den + a.tol;
+                        if (den > a.tol && (a.eps > 0 || u2 >= bsse * dt * (1.0 - 1e-12))) {
+                            const double sse = u2 / dt;
+                            if (sse > bsse) {
+                                bsse = sse;
+                                bidx = (int)i;
+                            }
+                            if (a.eps > 0) {
+                                // relative errors eps of the covariates move den = k - |f|^2 by at most
+                                // eps (k + |f|^2) and fy - w by eps sum |f_s Y_s|: the score's bound
+                                double ua = 0.0;
 #pragma unroll
-                            for (int s = 0; s < J; ++s) ua = fma(fabs(f[s]), fabs(Y[s]), ua);
-                            const double ffs = group_sum<G>(ff);
-                            ua = group_sum<G>(ua);
-                            const double rel = a.eps * ((den + 2.0 * ffs) / fmax(den, a.tol) +
-                                                        2.0 * ua / fmax(fabs(uw), 1e-300));
-                            const double ub = den > a.tol ? sse * (1.0 + rel) : 0.0;
-                            bub = fmax(bub, ub);
+                                for (int s = 0; s < J; ++s) ua = fma(fabs(f[s]), fabs(Y[s]), ua);
+                                const double ffs = group_sum<G>(ff);
+                                ua = group_sum<G>(ua);
+                                const double rel = a.eps * ((den + 2.0 * ffs) / fmax(den, a.tol) +
+                                                            2.0 * ua / fmax(fabs(uw), 1e-300));
+                                bub = fmax(bub, sse * (1.0 + rel));
+                            }
                         }
                     }
                 }
@@ -570,19 +596,26 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
             if (nr1 > 0) sorted_store(nxt);
         } else {
 #pragma unroll
-            for (int i = 0; i < NB; ++i) sq[nxt][i * RPS + ro][pos] = bov[i];
+            for (int i = 0; i < NB; ++i) sq[nxt][i * RPS + ro][pos] = (double)bov[i];
 #pragma unroll
             for (int i = 0; i < NPC; ++i) {
-                if (pisb[i]) sbv[nxt][prr[i]][pj[i]] = (double)pcv[i];
-                else sbx[nxt][prr[i]][pj[i]] = pcv[i];
+                if (pisb[i]) sbx[nxt][prr[i]][pj[i]].x = (double)pcv[i];
+                else sbx[nxt][prr[i]][pj[i]].y = (double)pcv[i];
             }
         }
         if (tid < kTR) {
-            const double d = (double)nd;
-            sdd[nxt][tid] = make_double2(d, d * d);
+            sd[nxt][tid] = (double)nd;
             if (!SORTED && tid < nr2) srow[cur][tid] = nrow;
         }
         __syncthreads();
+    }
+    if (!EVAL && pg == 0 && wv == 0) {
+        // the chunk's delta sum (carry shift), lane-strided then a fixed butterfly
+        double dacc = 0.0;
+        for (size_t l = r0 + lane; l < r1; l += 64) dacc += (double)dl[l];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) dacc += __shfl_xor(dacc, o, 64);
+        if (lane == 0) a.dsum[(size_t)gi * a.C + chunk] = dacc;
     }
     if (!active) return;
     const size_t cs = (size_t)c * a.C + chunk;
@@ -600,7 +633,6 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
             dst[2 * (t * J + s) + 1] = g[s];
         }
         if (t == 0) a.ksum[cs] = k2;
-        if (t == 0 && pg == 0 && slot == 0) a.dsum[(size_t)gi * a.C + chunk] = dacc;
     }
 }
 
@@ -681,24 +713,47 @@ void launch_sweep(const SweepArgs &a, dim3 grid, int nc, hipStream_t st, bool ev
     }
 }
 
-template <bool SORTED>
-void launch_sweep_pt(const SweepArgs &a, dim3 grid, int pt, int nc, hipStream_t st, bool eval)
+/* Slots per candidate (b, y, Bx and the m basis columns, padded): the sorted sweep has the finer
+ * steps (its loads need no row-to-lane fit of the tile width). */
+int sweep_slots(int m, bool sorted)
+{
+    static const int fine[] = {32, 48, 64, 96, 128, 192, 256}, coarse[] = {32, 64, 128, 256};
+    for (int pt : sorted ? std::vector<int>(fine, fine + 7) : std::vector<int>(coarse, coarse + 4))
+        if (m + 3 <= pt) return pt;
+    return 256;
+}
+
+void launch_sweep_sorted(const SweepArgs &a, dim3 grid, int pt, int nc, hipStream_t st, bool eval)
 {
     switch (pt) {
-    case 32: launch_sweep<32, SORTED>(a, grid, nc, st, eval); break;
-    case 64: launch_sweep<64, SORTED>(a, grid, nc, st, eval); break;
-    case 128: launch_sweep<128, SORTED>(a, grid, nc, st, eval); break;
-    default: launch_sweep<256, SORTED>(a, grid, nc, st, eval); break;
+    case 32: launch_sweep<32, true>(a, grid, nc, st, eval); break;
+    case 48: launch_sweep<48, true>(a, grid, nc, st, eval); break;
+    case 64: launch_sweep<64, true>(a, grid, nc, st, eval); break;
+    case 96: launch_sweep<96, true>(a, grid, nc, st, eval); break;
+    case 128: launch_sweep<128, true>(a, grid, nc, st, eval); break;
+    case 192: launch_sweep<192, true>(a, grid, nc, st, eval); break;
+    default: launch_sweep<256, true>(a, grid, nc, st, eval); break;
+    }
+}
+
+void launch_sweep_gather(const SweepArgs &a, dim3 grid, int pt, int nc, hipStream_t st, bool eval)
+{
+    switch (pt) {
+    case 32: launch_sweep<32, false>(a, grid, nc, st, eval); break;
+    case 64: launch_sweep<64, false>(a, grid, nc, st, eval); break;
+    case 128: launch_sweep<128, false>(a, grid, nc, st, eval); break;
+    default: launch_sweep<256, false>(a, grid, nc, st, eval); break;
     }
 }
 
 /* Extend the sorted replicas: for each listed (xc, k0), rows i < n of column xc's block get
- * B[:, k] and Bo[:, k] at row order[i] for k0 <= k < m (and x, y when k0 == 0). */
+ * B[:, k] and Bo[:, k] at row order[i] for k0 <= k < m (and x, y when k0 == 0).  The sources are
+ * the row-major copies (ld = ldb): one sorted row reads one run of each, however many columns. */
 __global__ __launch_bounds__(256) void k_rep_gather(const int *__restrict__ todo, const int *__restrict__ order,
                                                     const float *__restrict__ X, const float *__restrict__ y,
-                                                    const float *__restrict__ BC, const float *__restrict__ BoC,
-                                                    size_t ldx, size_t n, int m, int mcap, size_t rstride,
-                                                    float *__restrict__ rep)
+                                                    const float *__restrict__ B, const float *__restrict__ Bo,
+                                                    size_t ldb, size_t ldx, size_t n, int m, int mcap,
+                                                    size_t rstride, float *__restrict__ rep)
 {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
@@ -709,62 +764,74 @@ __global__ __launch_bounds__(256) void k_rep_gather(const int *__restrict__ todo
         blk[i] = X[(size_t)xc * ldx + o];
         blk[ldx + i] = y[o];
     }
+    const float *br = B + o * ldb, *bor = Bo + o * ldb;
     for (int k = k0; k < m; ++k) {
-        blk[(size_t)(2 + k) * ldx + i] = BC[(size_t)k * ldx + o];
-        blk[(size_t)(2 + mcap + k) * ldx + i] = BoC[(size_t)k * ldx + o];
+        blk[(size_t)(2 + k) * ldx + i] = br[k];
+        blk[(size_t)(2 + mcap + k) * ldx + i] = bor[k];
     }
 }
 
 } // namespace
 
-/* Make the stable descending order and f32 deltas of X column xc of fit 0 resident. */
-static void ensure_sorted(mars_hip_ctx *ctx, int xc)
+/* Make the stable descending order and f32 deltas of fit 0's X columns `xcs` resident (the columns
+ * not sorted yet, in batches of at most 64 columns and about 1.5 GB of key / value space). */
+static void ensure_sorted(mars_hip_ctx *ctx, const std::vector<int> &xcs)
 {
     FitState &fs = ctx->fits[0];
     if (fs.sorted.size() != fs.px) fs.sorted.assign(fs.px, 0);
-    if (fs.sorted[xc]) return;
+    std::vector<int> todo;
+    for (int xc : xcs)
+        if (!fs.sorted[xc]) {
+            fs.sorted[xc] = 1;
+            todo.push_back(xc);
+        }
+    if (todo.empty()) return;
     const size_t n = fs.n;
     hipStream_t st = ctx->stream;
     fs.order.ensure(fs.px * n);
     fs.delta.ensure(fs.px * n);
     const int nblocks = (int)((n + kSortTile - 1) / kSortTile);
+    const size_t per = std::max<size_t>(1, std::min<size_t>(64, (size_t(3) << 29) / (16 * n + 1)));
+    const size_t batch = std::min(per, todo.size());
     DBuf<unsigned> keys, hist;
-    DBuf<int> vals;
-    keys.ensure(2 * n);
-    vals.ensure(n);
-    hist.ensure((size_t)256 * nblocks);
-    int *out = fs.order.p + (size_t)xc * n;
-    unsigned *ka = keys.p, *kb = keys.p + n;
-    int *va = vals.p, *vb = out;
-    KTimer kt(ctx, K_SORT, 0.0, 16.0 * 4 * n);
-    hipLaunchKernelGGL(k_sort_keys, dim3((n + 255) / 256), dim3(256), 0, st, fs.X.p + (size_t)xc * fs.ldx,
-                       n, ka, va);
-    MH_LAUNCH_CHECK();
-    for (int pass = 0; pass < 4; ++pass) {
-        const int shift = 8 * pass;
-        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(256), 0, st, ka, n, shift, nblocks, hist.p);
+    DBuf<int> vals, cols;
+    keys.ensure(2 * n * batch);
+    vals.ensure(2 * n * batch);
+    hist.ensure((size_t)256 * nblocks * batch);
+    cols.ensure(todo.size());
+    h2d(ctx, cols.p, todo.data(), todo.size() * sizeof(int));
+    KTimer kt(ctx, K_SORT, 0.0, 16.0 * 4 * n * todo.size());
+    for (size_t b0 = 0; b0 < todo.size(); b0 += batch) {
+        const unsigned nb = (unsigned)std::min(batch, todo.size() - b0);
+        const int *cb = cols.p + b0;
+        hipLaunchKernelGGL(k_sort_keys, dim3((unsigned)((n + 255) / 256), nb), dim3(256), 0, st,
+                           (const float *)fs.X.p, fs.ldx, cb, n, keys.p, vals.p);
         MH_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, hist.p, 256 * nblocks);
+        for (int pass = 0; pass < 4; ++pass) {
+            const int shift = 8 * pass, half = pass & 1;
+            hipLaunchKernelGGL(k_sort_hist, dim3(nblocks, nb), dim3(256), 0, st, (const unsigned *)keys.p, n, half,
+                               shift, nblocks, hist.p);
+            MH_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_sort_scan, dim3(nb), dim3(1024), 0, st, hist.p, 256 * nblocks);
+            MH_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks, nb), dim3(256), 0, st, keys.p, vals.p, n, half, shift,
+                               nblocks, (const unsigned *)hist.p);
+            MH_LAUNCH_CHECK();
+        }
+        // after 4 passes the values are back in half 0
+        hipLaunchKernelGGL(k_sort_out, dim3((unsigned)((n + 255) / 256), nb), dim3(256), 0, st, (const int *)vals.p,
+                           (const float *)fs.X.p, fs.ldx, (const float *)fs.s.p, cb, n, fs.order.p, fs.delta.p);
         MH_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(256), 0, st, ka, va, n, shift, nblocks,
-                           hist.p, kb, vb);
-        MH_LAUNCH_CHECK();
-        std::swap(ka, kb);
-        std::swap(va, vb);
     }
-    // after 4 passes the values are back in `vals` (even pass count): copy into the cache
-    MH_CHECK(hipMemcpyAsync(out, va, n * sizeof(int), hipMemcpyDeviceToDevice, st));
-    float sc = 1.0f;
-    MH_CHECK(hipMemcpyAsync(&sc, fs.s.p + xc, sizeof(float), hipMemcpyDeviceToHost, st));
-    ctx_sync(ctx);
-    hipLaunchKernelGGL(k_deltas, dim3((n + 255) / 256), dim3(256), 0, st, fs.X.p + (size_t)xc * fs.ldx,
-                       sc, out, n, fs.delta.p + (size_t)xc * n);
-    MH_LAUNCH_CHECK();
-    ctx_sync(ctx);
-    keys.release();
-    vals.release();
-    hist.release();
-    fs.sorted[xc] = 1;
+    // the temporaries are released once the stream has drained (queued kernels still use them)
+    retire_device(keys.p, keys.cap * sizeof(unsigned));
+    retire_device(vals.p, vals.cap * sizeof(int));
+    retire_device(hist.p, hist.cap * sizeof(unsigned));
+    retire_device(cols.p, cols.cap * sizeof(int));
+    keys = DBuf<unsigned>{};
+    vals = DBuf<int>{};
+    hist = DBuf<unsigned>{};
+    cols = DBuf<int>{};
 }
 
 void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const double *ybx, double *hin, int *knot,
@@ -787,19 +854,22 @@ void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const dou
         grp.push_back(c);
         grp.push_back(e - c);
         maxnp = std::max(maxnp, e - c);
-        ensure_sorted(ctx, cs.xc[c]);
         c = e;
     }
     const int ng = (int)(grp.size() / 3);
+    {
+        std::vector<int> gx(ng);
+        for (int g = 0; g < ng; ++g) gx[g] = grp[3 * g];
+        ensure_sorted(ctx, gx);
+    }
     const size_t tail = n - (size_t)end_span;
     require(m <= 253, "hinge sweep supports up to 253 basis functions");
-    // covariate slots per candidate: b, y, bx and the m basis columns, padded
-    const int PT = m + 3 <= 32 ? 32 : (m + 3 <= 64 ? 64 : (m + 3 <= 128 ? 128 : 256));
+    const int PT = sweep_slots(m, sorted);
     const int L2 = PT;
     // candidates per block: one block takes all parents of an X column, up to 32
     const int NC = maxnp <= 8 ? 8 : (maxnp <= 16 ? 16 : 32);
     const int pgroups = (maxnp + NC - 1) / NC;
-    const int wpb = NC / (64 / (PT / (PT == 32 ? 4 : 8)));   // waves per block
+    const int wpb = NC / (64 / (PT / sweep_j(PT, NC)));   // waves per block
     // about sweep_waves waves (65536), chunks of at least sweep_min_rows (512) sorted rows; a multiple of 8
     // chunks puts the blocks of one (column, chunk) on one XCD (shared L2).  Chunks start at multiples of
     // the 16-row tile (aligned float4 runs of the sorted sources).
@@ -841,6 +911,7 @@ void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const dou
         a.rep = ctx->rep.p;
         a.hr = ctx->hr.p;
         a.rstride = ctx->rep_stride;
+        a.zero = ctx->rep.p + ctx->rep_stride * fs.px;
         a.ldx = fs.ldx;
         a.mcap = (int)ctx->rep_mcap;
     }
@@ -875,9 +946,9 @@ void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const dou
                 MH_LAUNCH_CHECK();
             }
             if (sorted)
-                launch_sweep_pt<true>(a, grid, PT, NC, st, eval);
+                launch_sweep_sorted(a, grid, PT, NC, st, eval);
             else
-                launch_sweep_pt<false>(a, grid, PT, NC, st, eval);
+                launch_sweep_gather(a, grid, PT, NC, st, eval);
         }
         hipLaunchKernelGGL(k_sweep_final, dim3((P + 255) / 256), dim3(256), 0, st, a, P, lin, hin, knot, krow, hub);
         MH_LAUNCH_CHECK();
@@ -907,8 +978,9 @@ static bool replicas_ready(mars_hip_ctx *ctx, const std::vector<int> &xcs)
             size_t fr = 0, tot = 0;
             MH_CHECK(hipMemGetInfo(&fr, &tot));
             if (need <= 0.5 * (double)fr) {
-                ctx->rep.ensure(stride * fs.px);
-                MH_CHECK(hipMemsetAsync(ctx->rep.p, 0, stride * fs.px * sizeof(float), st));
+                // (one more column of zeros after the blocks: the sweep's runs of absent columns)
+                ctx->rep.ensure(stride * fs.px + fs.ldx);
+                MH_CHECK(hipMemsetAsync(ctx->rep.p, 0, (stride * fs.px + fs.ldx) * sizeof(float), st));
                 ctx->rep_mcap = mcap;
                 ctx->rep_stride = stride;
                 ctx->rep_m.assign(fs.px, 0);
@@ -918,6 +990,7 @@ static bool replicas_ready(mars_hip_ctx *ctx, const std::vector<int> &xcs)
     }
     if (ctx->rep_state < 0) return false;
     require(fs.m <= ctx->rep_mcap, "basis exceeds the sorted replicas' capacity");
+    require(ctx->row_major, "the sorted replicas are gathered from the row-major basis copies");
     std::vector<int> todo;
     for (int xc : xcs)
         if (ctx->rep_m[xc] < (int)fs.m) {
@@ -936,8 +1009,8 @@ static bool replicas_ready(mars_hip_ctx *ctx, const std::vector<int> &xcs)
         const int c = std::min(cnt - i0, 65535);
         hipLaunchKernelGGL(k_rep_gather, dim3((unsigned)((fs.n + 255) / 256), c), dim3(256), 0, st,
                            (const int *)ctx->rep_todo.p + 2 * i0, (const int *)fs.order.p, (const float *)fs.X.p,
-                           (const float *)fs.y.p, (const float *)fs.BC.p, (const float *)fs.BoC.p, fs.ldx, fs.n,
-                           (int)fs.m, (int)ctx->rep_mcap, ctx->rep_stride, ctx->rep.p);
+                           (const float *)fs.y.p, (const float *)fs.B.p, (const float *)fs.Bo.p, ctx->ldb, fs.ldx,
+                           fs.n, (int)fs.m, (int)ctx->rep_mcap, ctx->rep_stride, ctx->rep.p);
         MH_LAUNCH_CHECK();
     }
     return true;
@@ -1001,7 +1074,7 @@ void hinge_score(mars_hip_ctx *ctx, Cands &cs, double *lin, double *ybx, double 
             gxc.push_back(cs.xc[c]);
             gc0.push_back(c);
         }
-    for (int xc : gxc) ensure_sorted(ctx, xc);
+    ensure_sorted(ctx, gxc);
     if (!replicas_ready(ctx, gxc)) {
         plain_score(ctx, cs, true, lin, ybx, nullptr, nullptr, false, nullptr, f32);
         hinge_sweep_list(ctx, cs, lin, ybx, hin, knot, krow, min_span, end_span, eps, hub);
