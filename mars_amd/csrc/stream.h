@@ -35,12 +35,8 @@ constexpr int kSlots = 64;   // candidate slots per item in the partial-sum layo
 /* Candidate groups of 16 per item: bounds registers (T-GEMM accumulators) and
  * LDS (projection T tile, at most 2048 doubles for m <= 128). */
 constexpr int cg_tgemm(int mt) { return mt <= 2 ? 4 : (mt <= 8 ? 2 : 1); }
-constexpr int cg_project(int mt) { return mt <= 2 ? 4 : (mt <= 4 ? 2 : 1); }
+constexpr int cg_project(int mt) { return mt <= 4 ? 2 : 1; }
 constexpr int waves_for(int mt) { return mt == 1 ? 2 : (mt <= 4 ? 3 : 2); }
-/* Narrow T-GEMM tiles (one 16-candidate group, the mixed form's rescoring) issue the loads of
- * two whole 32-row chunks before their MFMAs: their few loads per half-step alone leave HBM
- * latency exposed, and each column then streams 256-byte runs. */
-constexpr bool kHoist(int mt, int cg) { return cg == 1 && mt <= 2; }
 
 /* Work items are listed fit by fit and tile by tile in row order, so neighbouring items read
  * the same columns and share their address translations.  Workgroups are dispatched round-robin
@@ -124,22 +120,86 @@ __device__ inline float4 bx4(const FitDev &F, const LaneCand &c, long long row, 
     return v;
 }
 
+/* Raw loads behind one float4 of Bx (bx_load), turned into Bx once they have arrived (bx_make):
+ * the loads of later row steps can then be in flight while the MFMAs of this one run. */
+struct BxRaw {
+    float4 b, x;  // B and X runs, or (RB) the stored run in b
+};
+
+template <bool RB>
+__device__ inline BxRaw bx_load(const FitDev &F, const LaneCand &c, long long row, long long r1)
+{
+    BxRaw w;
+    if (RB) {
+        const float *p = F.rbuf + c.col * F.rsc;
+        if (F.rsr == 1) {
+            w.b = gld4(p + row);  // column-major: rows row..row+3 lie inside the column's ldx rows
+        } else {
+            const long long last = r1 - 1;
+            w.b.x = gld(p + (row < r1 ? row : last) * F.rsr);
+            w.b.y = gld(p + (row + 1 < r1 ? row + 1 : last) * F.rsr);
+            w.b.z = gld(p + (row + 2 < r1 ? row + 2 : last) * F.rsr);
+            w.b.w = gld(p + (row + 3 < r1 ? row + 3 : last) * F.rsr);
+        }
+        w.x = w.b;
+    } else {
+        w.b = gld4(c.bc + row);
+        w.x = gld4(c.xp + row);
+    }
+    return w;
+}
+
+template <bool RB>
+__device__ inline float4 bx_make(const BxRaw &w, const LaneCand &c, long long row, long long r1)
+{
+    float4 v;
+    if (RB) {
+        v.x = row < r1 ? w.b.x : 0.f;
+        v.y = row + 1 < r1 ? w.b.y : 0.f;
+        v.z = row + 2 < r1 ? w.b.z : 0.f;
+        v.w = row + 3 < r1 ? w.b.w : 0.f;
+    } else {
+        v.x = __fmul_rn(w.b.x, __fmul_rn(w.x.x, c.sx));
+        v.y = __fmul_rn(w.b.y, __fmul_rn(w.x.y, c.sx));
+        v.z = __fmul_rn(w.b.z, __fmul_rn(w.x.z, c.sx));
+        v.w = __fmul_rn(w.b.w, __fmul_rn(w.x.w, c.sx));
+    }
+    if (!c.ok) v = float4{0.f, 0.f, 0.f, 0.f};
+    return v;
+}
+
+/* Row steps of the T-GEMM whose loads are in flight at once: narrow tiles (little MFMA work per
+ * step) keep four, wide ones two; tiles of more than 64 basis columns none ahead (registers). */
+constexpr int tg_depth(int mt, int cg)
+{
+    return (mt >= 8 || (mt >= 5 && cg >= 2)) ? 1 : (cg == 1 && mt <= 4 ? 4 : 2);
+}
+constexpr int tg_waves(int mt, int cg) { return tg_depth(mt, cg) > 1 ? 2 : waves_for(mt); }
+/* The projection's blocks hold more per step (a scalar Bo load per k step, Bx and y): two in
+ * flight for 32-candidate tiles of at most 32 basis columns (the plain form's wide passes); the
+ * narrow tiles of the rescoring measured faster at three waves per SIMD without a pipeline. */
+constexpr int pr_depth(int mt, int cg) { return cg == 2 && mt <= 2 ? 2 : 1; }
+constexpr int pr_waves(int mt, int cg) { return pr_depth(mt, cg) > 1 ? 2 : waves_for(mt); }
+
 /*
  * T-GEMM partials: tpart[(item * 64 + slot) * ldT + k] = sum over the item's
  * rows of Bo[i][k] Bx[i][slot].  MFMA step q of a 4-row half-step at row r
  * takes rows r + 8 kr + q (kr = lane / 16), so each lane reads 4 consecutive
- * rows of its Bo columns 16u + lane%16 and of its candidates' Bx.
+ * rows of its Bo columns 16u + lane%16 and of its candidates' Bx.  The item's
+ * half-steps form a software pipeline: the loads of step s + PD - 1 are issued
+ * before the MFMAs of step s (PD register sets, indexed at compile time).
  */
 template <int MT, int CG, bool RETRY, bool F32 = false>
-__global__ __launch_bounds__(64, waves_for(MT)) void k_tgemm_w(const FitDev *__restrict__ fits, const Item *__restrict__ items,
-                                                int nitems, const int *__restrict__ cxc,
-                                                const int *__restrict__ cpar, const int *__restrict__ list,
-                                                int ldT, double *__restrict__ tpart,
-                                                const int *__restrict__ dn)
+__global__ __launch_bounds__(64, tg_waves(MT, CG)) void k_tgemm_w(const FitDev *__restrict__ fits,
+                                                                  const Item *__restrict__ items, int nitems,
+                                                                  const int *__restrict__ cxc,
+                                                                  const int *__restrict__ cpar,
+                                                                  const int *__restrict__ list, int ldT,
+                                                                  double *__restrict__ tpart,
+                                                                  const int *__restrict__ dn)
 {
     if (dn) nitems = *dn;  // item count built on the device
-    constexpr int HF = kHoist(MT, CG) ? 2 : 1;   // half-steps whose loads are issued together
-    constexpr int NCH = kHoist(MT, CG) ? 2 : 1;  // 32-row chunks per iteration
+    constexpr int PD = tg_depth(MT, CG);
     const int lane = threadIdx.x, kr = lane >> 4, kc = lane & 15;
     for (int round = 0; round < nitems; round += gridDim.x) {
         const int cnt = min((int)gridDim.x, nitems - round);
@@ -159,79 +219,78 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_tgemm_w(const FitDev *__r
             bo[u] = F.BoC + (size_t)(kok[u] ? k : 0) * F.ldx;
         }
         v4d acc[CG][MT];
+        v4f acc32[CG][MT];  // F32: one 32-row chunk (two steps) in FP32, then folded into FP64
 #pragma unroll
         for (int g = 0; g < CG; ++g)
 #pragma unroll
-            for (int u = 0; u < MT; ++u) acc[g][u] = v4d{0, 0, 0, 0};
-#pragma unroll 1
-        for (long long r = I.r0; r < I.r1; r += kWR * NCH) {
-            // F32: this iteration's rows accumulate in FP32 (v_mfma_f32_16x16x4f32), folded into FP64
-            v4f acc32[CG][MT];
-            if (F32) {
+            for (int u = 0; u < MT; ++u) {
+                acc[g][u] = v4d{0, 0, 0, 0};
+                acc32[g][u] = v4f{0, 0, 0, 0};
+            }
+        // step s: rows r0 + 32 (s / 2) + 8 kr + 4 (s % 2) .. + 3 of this lane (a multiple of PD apart
+        // from the loop's step, so its parity is the unrolled index's)
+        const int nsteps = 2 * (int)((I.r1 - I.r0 + kWR - 1) / kWR);
+        auto srow = [&](int st) { return I.r0 + (long long)kWR * (st >> 1) + 8 * kr + 4 * (st & 1); };
+        float4 av[PD][MT];
+        BxRaw bw[PD][CG];
+        auto issue = [&](int j, int st) {
+            if (st >= nsteps) return;  // (uniform)
+            const long long row = srow(st);
+#pragma unroll
+            for (int u = 0; u < MT; ++u) av[j][u] = gld4(bo[u] + row);
+#pragma unroll
+            for (int g = 0; g < CG; ++g) bw[j][g] = bx_load<RETRY>(F, c[g], row, I.r1);
+        };
+        auto compute = [&](int j, int st) {
+            if (st >= nsteps) return;  // (uniform)
+            const long long row = srow(st);
+            double a[MT][4], b[CG][4];
+#pragma unroll
+            for (int u = 0; u < MT; ++u) {
+                a[u][0] = kok[u] ? av[j][u].x : 0.f;
+                a[u][1] = kok[u] ? av[j][u].y : 0.f;
+                a[u][2] = kok[u] ? av[j][u].z : 0.f;
+                a[u][3] = kok[u] ? av[j][u].w : 0.f;
+            }
+#pragma unroll
+            for (int g = 0; g < CG; ++g) {
+                const float4 v = bx_make<RETRY>(bw[j][g], c[g], row, I.r1);
+                b[g][0] = v.x;
+                b[g][1] = v.y;
+                b[g][2] = v.z;
+                b[g][3] = v.w;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
 #pragma unroll
                 for (int g = 0; g < CG; ++g)
-#pragma unroll
-                    for (int u = 0; u < MT; ++u) acc32[g][u] = v4f{0, 0, 0, 0};
-            }
-#pragma unroll 1
-            for (int h0 = 0; h0 < 2; h0 += HF) {
-                // all loads of HF half-steps of NCH chunks first, unconditionally within a chunk
-                // (see bx4); a chunk past the item's end is skipped whole (uniform branch)
-                float4 av4[NCH][HF][MT], bv4[NCH][HF][CG];
-#pragma unroll
-                for (int cc = 0; cc < NCH; ++cc) {
-                    if (cc > 0 && r + kWR * cc >= I.r1) continue;
-#pragma unroll
-                    for (int hh = 0; hh < HF; ++hh) {
-                        const long long row = r + kWR * cc + 8 * kr + 4 * (h0 + hh);
-#pragma unroll
-                        for (int u = 0; u < MT; ++u) av4[cc][hh][u] = gld4(bo[u] + row);
-#pragma unroll
-                        for (int g = 0; g < CG; ++g) bv4[cc][hh][g] = bx4<RETRY>(F, c[g], row, I.r1);
-                    }
-                }
-#pragma unroll
-                for (int cc = 0; cc < NCH; ++cc) {
-                    if (cc > 0 && r + kWR * cc >= I.r1) continue;
-#pragma unroll
-                    for (int hh = 0; hh < HF; ++hh) {
-                        double a[MT][4], b[CG][4];
+                    if (16 * g < I.nc)
 #pragma unroll
                         for (int u = 0; u < MT; ++u) {
-                            a[u][0] = kok[u] ? av4[cc][hh][u].x : 0.f;
-                            a[u][1] = kok[u] ? av4[cc][hh][u].y : 0.f;
-                            a[u][2] = kok[u] ? av4[cc][hh][u].z : 0.f;
-                            a[u][3] = kok[u] ? av4[cc][hh][u].w : 0.f;
+                            if (F32)
+                                acc32[g][u] = mfma_f32((float)a[u][q], (float)b[g][q], acc32[g][u]);
+                            else
+                                acc[g][u] = mfma_f64(a[u][q], b[g][q], acc[g][u]);
                         }
-#pragma unroll
-                        for (int g = 0; g < CG; ++g) {
-                            b[g][0] = bv4[cc][hh][g].x;
-                            b[g][1] = bv4[cc][hh][g].y;
-                            b[g][2] = bv4[cc][hh][g].z;
-                            b[g][3] = bv4[cc][hh][g].w;
-                        }
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-#pragma unroll
-                            for (int g = 0; g < CG; ++g)
-                                if (16 * g < I.nc)
-#pragma unroll
-                                    for (int u = 0; u < MT; ++u) {
-                                        if (F32)
-                                            acc32[g][u] = mfma_f32((float)a[u][q], (float)b[g][q], acc32[g][u]);
-                                        else
-                                            acc[g][u] = mfma_f64(a[u][q], b[g][q], acc[g][u]);
-                                    }
-                    }
-                }
-            }
-            if (F32) {
+            if (F32 && (st & 1)) {
 #pragma unroll
                 for (int g = 0; g < CG; ++g)
 #pragma unroll
-                    for (int u = 0; u < MT; ++u)
+                    for (int u = 0; u < MT; ++u) {
 #pragma unroll
                         for (int v = 0; v < 4; ++v) acc[g][u][v] += (double)acc32[g][u][v];
+                        acc32[g][u] = v4f{0, 0, 0, 0};
+                    }
+            }
+        };
+#pragma unroll
+        for (int j = 0; j + 1 < PD; ++j) issue(j, j);
+#pragma unroll 1
+        for (int s0 = 0; s0 < nsteps; s0 += PD) {
+#pragma unroll
+            for (int j = 0; j < PD; ++j) {
+                issue((j + PD - 1) % PD, s0 + j + PD - 1);
+                compute(j, s0 + j);
             }
         }
 #pragma unroll
@@ -365,25 +424,26 @@ enum { kStoreR = 0, kSumSq = 1, kFinalY = 2 };
  * The item's T tile is staged once in LDS (MFMA B operand).  The A operand's
  * M index M stands for row 4 (M % 4) + M / 4 of the 16-row block, so the D
  * elements a lane holds (M = l/16 + 4v) are the 4 consecutive rows
- * 4 (l/16) + v, whose Bx it loads as one float4 each of B and X.
+ * 4 (l/16) + v, whose Bx it loads as one float4 each of B and X.  The 16-row
+ * blocks are software-pipelined like the T-GEMM's half-steps.
  */
 template <int MT, int CG, int MODE, bool RB, bool TCAND, bool F32 = false>
-__global__ __launch_bounds__(64, waves_for(MT)) void k_project_w(const FitDev *__restrict__ fits,
-                                                                 const Item *__restrict__ items, int nitems,
-                                                                 const int *__restrict__ cxc,
-                                                                 const int *__restrict__ cpar,
-                                                                 const int *__restrict__ list,
-                                                                 const double *__restrict__ T, int ldT,
-                                                                 const double *__restrict__ ssum,
-                                                                 const int *__restrict__ fired,
-                                                                 const int *__restrict__ keep,
-                                                                 double *__restrict__ spart,
-                                                                 const int *__restrict__ dn)
+__global__ __launch_bounds__(64, pr_waves(MT, CG)) void k_project_w(const FitDev *__restrict__ fits,
+                                                                    const Item *__restrict__ items, int nitems,
+                                                                    const int *__restrict__ cxc,
+                                                                    const int *__restrict__ cpar,
+                                                                    const int *__restrict__ list,
+                                                                    const double *__restrict__ T, int ldT,
+                                                                    const double *__restrict__ ssum,
+                                                                    const int *__restrict__ fired,
+                                                                    const int *__restrict__ keep,
+                                                                    double *__restrict__ spart,
+                                                                    const int *__restrict__ dn)
 {
     if (dn) nitems = *dn;  // item count built on the device
     constexpr int KQ = 4 * MT;           // k steps of 4
     constexpr int NC = 16 * CG, ST = 16 * MT + 1;
-    constexpr int HF = 1;  // 16-row blocks whose loads are issued together (two spill registers)
+    constexpr int PD = pr_depth(MT, CG);  // 16-row blocks whose loads are in flight at once
     __shared__ double sT[NC][ST];
     const int lane = threadIdx.x, kr = lane >> 4, kc = lane & 15;
     // memory row of this lane's A-operand M index: the lane's D elements are then rows 4 (l/16) + v
@@ -433,89 +493,92 @@ __global__ __launch_bounds__(64, waves_for(MT)) void k_project_w(const FitDev *_
                 fromr[g] = (fired[cand] & 1) != 0;
             }
         }
-#pragma unroll 1
-        for (long long r = I.r0; r < I.r1; r += kWR) {
-#pragma unroll 1
-            for (int h0 = 0; h0 < kWR / 16; h0 += HF) {
-                // all loads of HF 16-row blocks first, unconditionally (see bx4)
-                float avh[HF][KQ];
-                float4 bxh[HF][CG], yh[HF];
+        // step s: the 16-row block at r0 + 16 s (item ranges start at multiples of 32, and the rows
+        // of a block past the item's end lie past the fit's end, in the zero padding)
+        const int nsteps = 2 * (int)((I.r1 - I.r0 + kWR - 1) / kWR);
+        float avh[PD][KQ];
+        BxRaw bw[PD][CG];
+        float4 yh[PD];
+        auto issue = [&](int j, int st) {
+            if (st >= nsteps) return;  // (uniform)
+            const long long r16 = I.r0 + 16LL * st;
+            const long long row = r16 + 4 * kr;
 #pragma unroll
-                for (int hh = 0; hh < HF; ++hh) {
-                    const long long r16 = r + 16 * (h0 + hh);
-                    const long long row = r16 + 4 * kr;
+            for (int q = 0; q < KQ; ++q) avh[j][q] = gld(bo[q] + r16 + prow);
 #pragma unroll
-                    for (int q = 0; q < KQ; ++q) avh[hh][q] = gld(bo[q] + r16 + prow);
+            for (int g = 0; g < CG; ++g) bw[j][g] = bx_load<RB>(F, c[g], row, I.r1);
+            yh[j] = float4{0.f, 0.f, 0.f, 0.f};
+            if (MODE == kFinalY) yh[j] = gld4(F.y + row);
+        };
+        auto compute = [&](int j, int st) {
+            if (st >= nsteps) return;  // (uniform)
+            const long long row = I.r0 + 16LL * st + 4 * kr;
+            float av[KQ];
 #pragma unroll
-                    for (int g = 0; g < CG; ++g) bxh[hh][g] = bx4<RB>(F, c[g], row, I.r1);
-                    yh[hh] = float4{0.f, 0.f, 0.f, 0.f};
-                    if (MODE == kFinalY) yh[hh] = gld4(F.y + row);
+            for (int q = 0; q < KQ; ++q) av[q] = 4 * q + kr >= F.m ? 0.0f : avh[j][q];
+            const float yy[4] = {yh[j].x, yh[j].y, yh[j].z, yh[j].w};
+#pragma unroll
+            for (int g = 0; g < CG; ++g) {
+                if (16 * g >= I.nc) continue;
+                v4d d = v4d{0, 0, 0, 0};
+                if (F32) {  // Bo T in FP32 (the approximate first pass of the precision form)
+                    v4f d32 = v4f{0, 0, 0, 0};
+#pragma unroll
+                    for (int q = 0; q < KQ; ++q) d32 = mfma_f32(av[q], (float)sT[16 * g + kc][4 * q + kr], d32);
+                    d = v4d{(double)d32[0], (double)d32[1], (double)d32[2], (double)d32[3]};
+                } else {
+#pragma unroll
+                    for (int q = 0; q < KQ; ++q) d = mfma_f64((double)av[q], sT[16 * g + kc][4 * q + kr], d);
                 }
+                if (!c[g].ok) continue;
+                const float4 bx = bx_make<RB>(bw[j][g], c[g], row, I.r1);
+                const float bv[4] = {bx.x, bx.y, bx.z, bx.w};
+                float rf[4];
 #pragma unroll
-                for (int hh = 0; hh < HF; ++hh) {
-                    const long long row = r + 16 * (h0 + hh) + 4 * kr;
-                    float *av = avh[hh];
-                    const float4 *bxv = bxh[hh];
-                    const float yy[4] = {yh[hh].x, yh[hh].y, yh[hh].z, yh[hh].w};
+                for (int v = 0; v < 4; ++v) rf[v] = (float)((double)bv[v] - d[v]);
+                float *rp = F.rbuf + c[g].col * F.rsc;
+                if (MODE == kFinalY && fromr[g]) {
+                    if (F.rsr == 1) {
+                        const float4 q = gld4(rp + row);
+                        rf[0] = q.x;
+                        rf[1] = q.y;
+                        rf[2] = q.z;
+                        rf[3] = q.w;
+                    } else {
 #pragma unroll
-                    for (int q = 0; q < KQ; ++q)
-                        if (4 * q + kr >= F.m) av[q] = 0.0f;
-#pragma unroll
-                    for (int g = 0; g < CG; ++g) {
-                        if (16 * g >= I.nc) continue;
-                        v4d d = v4d{0, 0, 0, 0};
-                        if (F32) {  // Bo T in FP32 (the approximate first pass of the precision form)
-                            v4f d32 = v4f{0, 0, 0, 0};
-#pragma unroll
-                            for (int q = 0; q < KQ; ++q)
-                                d32 = mfma_f32(av[q], (float)sT[16 * g + kc][4 * q + kr], d32);
-                            d = v4d{(double)d32[0], (double)d32[1], (double)d32[2], (double)d32[3]};
-                        } else {
-#pragma unroll
-                            for (int q = 0; q < KQ; ++q) d = mfma_f64((double)av[q], sT[16 * g + kc][4 * q + kr], d);
-                        }
-                        if (!c[g].ok) continue;
-                        const float bv[4] = {bxv[g].x, bxv[g].y, bxv[g].z, bxv[g].w};
-                        float rf[4];
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) rf[v] = (float)((double)bv[v] - d[v]);
-                        // rows past the item's end occur only past the fit's end (item ranges
-                        // start at multiples of 32), where r is 0 from the zero padding
-                        float *rp = F.rbuf + c[g].col * F.rsc;
-                        if (MODE == kFinalY && fromr[g]) {
-                            if (F.rsr == 1) {
-                                const float4 q = gld4(rp + row);
-                                rf[0] = q.x;
-                                rf[1] = q.y;
-                                rf[2] = q.z;
-                                rf[3] = q.w;
-                            } else {
-#pragma unroll
-                                for (int v = 0; v < 4; ++v)
-                                    if (row + v < I.r1) rf[v] = rp[(row + v) * F.rsr];
-                            }
-                        }
-                        if (MODE == kStoreR && store[g]) {
-                            if (F.rsr == 1) {
-                                *reinterpret_cast<float4 *>(rp + row) = float4{rf[0], rf[1], rf[2], rf[3]};
-                            } else {
-#pragma unroll
-                                for (int v = 0; v < 4; ++v)
-                                    if (row + v < I.r1) rp[(row + v) * F.rsr] = rf[v];
-                            }
-                        }
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) {
-                            if (row + v >= I.r1) continue;
-                            if (MODE == kFinalY) {
-                                const float bxn = __fmul_rn(rf[v], scale[g]);
-                                acc[g] = fma((double)bxn, (double)yy[v], acc[g]);
-                            } else {
-                                acc[g] = fma((double)rf[v], (double)rf[v], acc[g]);
-                            }
-                        }
+                        for (int v = 0; v < 4; ++v)
+                            if (row + v < I.r1) rf[v] = rp[(row + v) * F.rsr];
                     }
                 }
+                if (MODE == kStoreR && store[g]) {
+                    if (F.rsr == 1) {
+                        *reinterpret_cast<float4 *>(rp + row) = float4{rf[0], rf[1], rf[2], rf[3]};
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            if (row + v < I.r1) rp[(row + v) * F.rsr] = rf[v];
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    if (row + v >= I.r1) continue;
+                    if (MODE == kFinalY) {
+                        const float bxn = __fmul_rn(rf[v], scale[g]);
+                        acc[g] = fma((double)bxn, (double)yy[v], acc[g]);
+                    } else {
+                        acc[g] = fma((double)rf[v], (double)rf[v], acc[g]);
+                    }
+                }
+            }
+        };
+#pragma unroll
+        for (int j = 0; j + 1 < PD; ++j) issue(j, j);
+#pragma unroll 1
+        for (int s0 = 0; s0 < nsteps; s0 += PD) {
+#pragma unroll
+            for (int j = 0; j < PD; ++j) {
+                issue((j + PD - 1) % PD, s0 + j + PD - 1);
+                compute(j, s0 + j);
             }
         }
 #pragma unroll
