@@ -104,6 +104,7 @@ def fit(lib, X, y, w=None, **kwargs):
     while algo.nbasis() < max_terms:
         if max_epochs is not None and epoch >= max_epochs:
             break
+        t_epoch = time.perf_counter()
         basis_to_use = np.argsort(basis_sse)[::-1][:max_basis]
         input_to_use = np.argsort(ranks(input_sse) + aging_factor * (epoch - input_age) * (input_sse > 0))
         n_take = None if max_inputs is None else max_inputs + int(np.isinf(input_sse).sum())
@@ -156,6 +157,7 @@ def fit(lib, X, y, w=None, **kwargs):
                             1.0 - gcv(mse, m + 1) / var_y, len(basis[-1]), dt)
                 basis_sse = np.append(basis_sse, [0.0])
                 m += 1
+        step["secs"] = time.perf_counter() - t_epoch  # eval + append of this epoch (timing only)
         trace.append(step)
         assert algo.nbasis() == len(basis) == m
 
