@@ -357,6 +357,7 @@ struct mars_hip_ctx {
     mh::DBuf<float> hr;              // sorted hinge path: normalised candidate column c at c * ldx
     mh::Cands vcand;                 // sorted hinge path: the list over the per-column fit views
     mh::DBuf<int> rep_todo;          // replica extension list
+    mh::DBuf<float> rep_stage;       // an append's new basis columns, row-major, for the replica gather
 
     long dgks_host = 0;
     double eval_ms = 0;

@@ -846,6 +846,7 @@ void ctx_free(mars_hip_ctx *ctx)
     ctx->rknot.release();
     ctx->rep.release();
     ctx->hr.release();
+    ctx->rep_stage.release();
     for (DBuf<int> *b : {&ctx->vcand.d_fit, &ctx->vcand.d_xc, &ctx->vcand.d_par, &ctx->rep_todo}) b->release();
     ctx->dl_u64.release();
     ctx->dstat.release();

@@ -748,11 +748,13 @@ void launch_sweep_gather(const SweepArgs &a, dim3 grid, int pt, int nc, hipStrea
 
 /* Extend the sorted replicas: for each listed (xc, k0), rows i < n of column xc's block get
  * B[:, k] and Bo[:, k] at row order[i] for k0 <= k < m (and x, y when k0 == 0).  The sources are
- * the row-major copies (ld = ldb): one sorted row reads one run of each, however many columns. */
+ * row-major, element (o, k) at sB / sBo + o sld + k - kbase: the basis's row-major copies (ld =
+ * ldb, kbase 0), or, for an append's few new columns, a compact staging copy of just those columns
+ * (a few MB: a sorted row's random read then hits the Infinity Cache instead of HBM). */
 __global__ __launch_bounds__(256) void k_rep_gather(const int *__restrict__ todo, const int *__restrict__ order,
                                                     const float *__restrict__ X, const float *__restrict__ y,
-                                                    const float *__restrict__ B, const float *__restrict__ Bo,
-                                                    size_t ldb, size_t ldx, size_t n, int m, int mcap,
+                                                    const float *__restrict__ sB, const float *__restrict__ sBo,
+                                                    size_t sld, int kbase, size_t ldx, size_t n, int m, int mcap,
                                                     size_t rstride, float *__restrict__ rep)
 {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -764,10 +766,22 @@ __global__ __launch_bounds__(256) void k_rep_gather(const int *__restrict__ todo
         blk[i] = X[(size_t)xc * ldx + o];
         blk[ldx + i] = y[o];
     }
-    const float *br = B + o * ldb, *bor = Bo + o * ldb;
+    const float *br = sB + o * sld - kbase, *bor = sBo + o * sld - kbase;
     for (int k = k0; k < m; ++k) {
         blk[(size_t)(2 + k) * ldx + i] = br[k];
         blk[(size_t)(2 + mcap + k) * ldx + i] = bor[k];
+    }
+}
+
+/* Staging copy of basis columns k0 .. k0 + nk - 1: S[o][j] = B[o][k0 + j], S[o][nk + j] = Bo[o][k0 + j]. */
+__global__ __launch_bounds__(256) void k_rep_stage(const float *__restrict__ BC, const float *__restrict__ BoC,
+                                                   size_t ldx, size_t n, int k0, int nk, float *__restrict__ S)
+{
+    const size_t o = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (o >= n) return;
+    for (int j = 0; j < nk; ++j) {
+        S[o * 2 * nk + j] = BC[(size_t)(k0 + j) * ldx + o];
+        S[o * 2 * nk + nk + j] = BoC[(size_t)(k0 + j) * ldx + o];
     }
 }
 
@@ -1005,12 +1019,30 @@ static bool replicas_ready(mars_hip_ctx *ctx, const std::vector<int> &xcs)
     ctx->rep_todo.ensure(todo.size());
     h2d(ctx, ctx->rep_todo.p, todo.data(), todo.size() * sizeof(int));
     KTimer kt(ctx, K_SORT, 0.0, (8.0 * cols + 4.0 * cnt) * (double)fs.n);
+    // an append's new columns, the same for every listed column: gathered from a staging copy
+    int k0 = todo[1];
+    for (int i = 0; i < cnt; ++i)
+        if (todo[2 * i + 1] != k0) k0 = -1;
+    const int nk = (int)fs.m - k0;
+    const float *sB = fs.B.p, *sBo = fs.Bo.p;
+    size_t sld = ctx->ldb;
+    int kbase = 0;
+    if (k0 > 0 && nk <= 8) {
+        ctx->rep_stage.ensure(2 * (size_t)nk * fs.n);
+        hipLaunchKernelGGL(k_rep_stage, dim3((unsigned)((fs.n + 255) / 256)), dim3(256), 0, st, (const float *)fs.BC.p,
+                           (const float *)fs.BoC.p, fs.ldx, fs.n, k0, nk, ctx->rep_stage.p);
+        MH_LAUNCH_CHECK();
+        sB = ctx->rep_stage.p;
+        sBo = ctx->rep_stage.p + nk;
+        sld = 2 * (size_t)nk;
+        kbase = k0;
+    }
     for (int i0 = 0; i0 < cnt; i0 += 65535) {
         const int c = std::min(cnt - i0, 65535);
         hipLaunchKernelGGL(k_rep_gather, dim3((unsigned)((fs.n + 255) / 256), c), dim3(256), 0, st,
                            (const int *)ctx->rep_todo.p + 2 * i0, (const int *)fs.order.p, (const float *)fs.X.p,
-                           (const float *)fs.y.p, (const float *)fs.B.p, (const float *)fs.Bo.p, ctx->ldb, fs.ldx,
-                           fs.n, (int)fs.m, (int)ctx->rep_mcap, ctx->rep_stride, ctx->rep.p);
+                           (const float *)fs.y.p, sB, sBo, sld, kbase, fs.ldx, fs.n, (int)fs.m, (int)ctx->rep_mcap,
+                           ctx->rep_stride, ctx->rep.p);
         MH_LAUNCH_CHECK();
     }
     return true;
