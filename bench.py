@@ -361,7 +361,12 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the batch step's scoring kernels directly instead of replaying captured graphs "
                          "(profiling runs: kernel traces are taken on direct launches)")
+    ap.add_argument("--cache-f64", action="store_true",
+                    help="run the mixed form's cache update on FP64 MFMA instead of FP32 (MARS_HIP_CACHE=f64): "
+                         "separates the FP32 arithmetic's share of the mixed form's speed from the cache's")
     args = ap.parse_args()
+    if args.cache_f64:
+        os.environ["MARS_HIP_CACHE"] = "f64"
     if args.no_graph:
         os.environ["MARS_HIP_GRAPH"] = "0"
     if args.copy_probe:
@@ -399,7 +404,8 @@ def main():
         "dtype": {"fp64": "f64", "mixed": "f32 (MFMA) + f64", "mixed-bf16": "bf16x3 (MFMA) + f32 + f64"}[forms[-1]],
         "data": "synthetic (mars_amd.datagen, seeded)",
         "config": dict(workload=workload, precision=forms[-1],
-                       graphs=os.environ.get("MARS_HIP_GRAPH", "1") != "0"),
+                       graphs=os.environ.get("MARS_HIP_GRAPH", "1") != "0",
+                       cache="f64" if os.environ.get("MARS_HIP_CACHE") == "f64" else "f32"),
         "fit_s": {f: r["fit_s"] for f, r in res.items()},
         "upload_s": {f: r["upload_s"] for f, r in res.items()},
         "candidates_per_fit": head["cands"],

@@ -57,7 +57,7 @@ constexpr int kGroupInts = 7;      // fit, xc0, nc, j, a-tile, first list entry,
 constexpr int kTW = 8;         // A tiles (waves) per block at most
 constexpr int kXP = 36;        // LDS floats per staged X column (32 rows + pad)
 
-template <int W>
+template <int W, bool F64 = false>
 __global__ __launch_bounds__(64 * W, W >= 4 ? 3 : 1) void k_tc_update(const FitDev *__restrict__ fits,
                                                       const Item *__restrict__ items, int nitems,
                                                       double *__restrict__ tpart, double *__restrict__ npart)
@@ -146,7 +146,12 @@ __global__ __launch_bounds__(64 * W, W >= 4 ? 3 : 1) void k_tc_update(const FitD
                     const float4 x4 = *(const float4 *)&sx[buf][16 * g + kc][8 * kr + 4 * hf];
                     const float xs[4] = {x4.x, x4.y, x4.z, x4.w};
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) part[g] = mfma_f32(av[q], xs[q], part[g]);
+                    for (int q = 0; q < 4; ++q) {
+                        if (F64)
+                            acc[g] = mfma_f64((double)av[q], (double)xs[q], acc[g]);
+                        else
+                            part[g] = mfma_f32(av[q], xs[q], part[g]);
+                    }
                     if (norms) {
                         const float4 j4 = *(const float4 *)&sjy[buf][0][8 * kr + 4 * hf];
                         const float4 y4 = *(const float4 *)&sjy[buf][1][8 * kr + 4 * hf];
@@ -161,10 +166,12 @@ __global__ __launch_bounds__(64 * W, W >= 4 ? 3 : 1) void k_tc_update(const FitD
                     }
                 }
             }
+            if (!F64) {
 #pragma unroll
-            for (int g = 0; g < 4; ++g)
+                for (int g = 0; g < 4; ++g)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) acc[g][v] += (double)part[g][v];
+                    for (int v = 0; v < 4; ++v) acc[g][v] += (double)part[g][v];
+            }
         };
         // prologue: chunk 0 into buffer 0, A of chunks 0 and 1 into register sets 0 and 1
         float4 xv[NXL], jy;
@@ -214,11 +221,11 @@ __global__ __launch_bounds__(64 * W, W >= 4 ? 3 : 1) void k_tc_update(const FitD
         for (int g = 0; g < 4; ++g) {
             if (g >= G) continue;
             const int slot = 16 * g + kc;
-            // D row 4(l/16)+v is A column 16u + 4 kr + v
-            double *dst = tpart + ((size_t)it * kXT + slot) * 16 + 4 * kr;
+            // D row 4(l/16)+v (FP32 MFMA) or l/16 + 4v (FP64 MFMA) is A column 16u + that row
+            double *dst = tpart + ((size_t)it * kXT + slot) * 16;
             if (slot < I.nc)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) dst[v] = acc[g][v];
+                for (int v = 0; v < 4; ++v) dst[F64 ? kr + 4 * v : 4 * kr + v] = acc[g][v];
             if (norms) {
                 double s0 = bxx[g], s1 = bxy[g];
                 s0 = s0 + __shfl_xor(s0, 16, 64);
@@ -616,11 +623,25 @@ void tc_update(mars_hip_ctx *ctx)
             const dim3 grid((unsigned)std::max<size_t>(1, std::min<size_t>(ni, 65536)));
             const Item *di = (const Item *)ctx->d_items.p;
             const FitDev *dfit = (const FitDev *)ctx->d_fits.p;
+            // MARS_HIP_CACHE=f64: the same update on FP64 MFMA (a measurement variant: it shows how much
+            // of the mixed form's gain is the FP32 arithmetic rather than the cache)
+            const char *ce = std::getenv("MARS_HIP_CACHE");
+            const bool f64 = ce && std::strcmp(ce, "f64") == 0;
+            auto go = [&](auto w, auto d) {
+                constexpr int WW = decltype(w)::value;
+                constexpr bool DD = decltype(d)::value;
+                hipLaunchKernelGGL((k_tc_update<WW, DD>), grid, dim3(64 * WW), 0, st, dfit, di, (int)ni, ctx->tpart.p,
+                                   ctx->aux.p);
+            };
+            using t_ = std::true_type;
+            using f_ = std::false_type;
             switch (W) {
-            case 1: hipLaunchKernelGGL(k_tc_update<1>, grid, dim3(64), 0, st, dfit, di, (int)ni, ctx->tpart.p, ctx->aux.p); break;
-            case 2: hipLaunchKernelGGL(k_tc_update<2>, grid, dim3(128), 0, st, dfit, di, (int)ni, ctx->tpart.p, ctx->aux.p); break;
-            case 4: hipLaunchKernelGGL(k_tc_update<4>, grid, dim3(256), 0, st, dfit, di, (int)ni, ctx->tpart.p, ctx->aux.p); break;
-            default: hipLaunchKernelGGL(k_tc_update<kTW>, grid, dim3(64 * kTW), 0, st, dfit, di, (int)ni, ctx->tpart.p, ctx->aux.p); break;
+            case 1: f64 ? go(std::integral_constant<int, 1>{}, t_{}) : go(std::integral_constant<int, 1>{}, f_{}); break;
+            case 2: f64 ? go(std::integral_constant<int, 2>{}, t_{}) : go(std::integral_constant<int, 2>{}, f_{}); break;
+            case 4: f64 ? go(std::integral_constant<int, 4>{}, t_{}) : go(std::integral_constant<int, 4>{}, f_{}); break;
+            default:
+                f64 ? go(std::integral_constant<int, kTW>{}, t_{}) : go(std::integral_constant<int, kTW>{}, f_{});
+                break;
             }
             MH_LAUNCH_CHECK();
         }

@@ -418,6 +418,27 @@ def test_batch_mixed_matches_reference(policy, monkeypatch):
     assert stats["window_violations"] == 0 and stats["host_redo_steps"] == 0
 
 
+def test_batch_mixed_fp64_cache_matches_reference(monkeypatch):
+    """The cache update on FP64 MFMA (MARS_HIP_CACHE=f64, the bench's --cache-f64 variant) chooses
+    the reference's terms with the same mse: the rescoring keeps every winner exact either way."""
+    import mars_amd
+
+    monkeypatch.setenv("MARS_HIP_CACHE", "f64")
+    say("batch mixed, FP64 cache")
+    golds = [json.load(open(os.path.join(HERE, "golden", "w4_fit%d.json" % i))) for i in range(8)]
+    inputs = [make_inputs(CASES[g["name"]]) for g in golds]
+    stats = {}
+    models = mars_amd.fit_batch([x for x, _, _ in inputs], [y for _, y, _ in inputs], max_terms=33,
+                                precision="mixed", stats=stats)
+    for gold, model, mse in zip(golds, models, stats["mse"]):
+        got = [(r["type"].decode(), int(r["basis"]), int(r["input"])) for r in model]
+        want = [(r["type"], r["basis"], r["input"]) for r in gold["model"]]
+        assert got == want, gold["name"]
+        for a, b in zip(mse, [a["mse"] for a in gold["appends"]]):
+            assert abs(a - b) <= 1e-10 * abs(b)
+    assert stats["window_violations"] == 0 and stats["host_redo_steps"] == 0
+
+
 def _rival_inputs(seed, n, p, eps):
     """X columns 0 and 1 nearly equal (column 1 = column 0 times 1 + eps, plus eps-sized noise):
     candidates on column 0 and column 1 score within a relative ~eps of each other, below the
