@@ -4,7 +4,8 @@ of the golden fixtures): orthogonality loss of the final basis, max |QᵀQ - I|,
 and the per-step RSS error against the reference's FP64 outputs (the golden
 fixtures written by the reference built from its own sources).
 
-Bars: plain form RSS within 1e-10 relative, mixed form within 1e-6, and
+Bars: plain form RSS within 1e-10 relative, the mixed forms (FP32 MFMA, and on
+hinge fits the BF16-split form) within 1e-6, and
 |QᵀQ - I| at the f32 storage floor (Q is stored in f32: 1e-5).
 """
 
@@ -48,7 +49,8 @@ def test_accuracy_per_config(config, name):
         gold = json.load(f)
     want = [a["mse"] for a in gold["appends"]]
     want_seq = [(r["type"], r["basis"], r["input"]) for r in gold["model"]]
-    for prec in ("fp64", "mixed"):
+    hinge = not case["fit"].get("linear_only", True)
+    for prec in ("fp64", "mixed") + (("mixed-bf16",) if hinge else ()):
         model, mse, orth = _fit(case, prec)
         seq = [(r["type"].decode(), int(r["basis"]), int(r["input"])) for r in model]
         assert seq == want_seq, "%s: term sequence differs" % prec

@@ -72,8 +72,8 @@ def _seq(model):
 
 
 def test_config3_full_properties():
-    """1M rows x 100 columns, hinge fit to 65 terms (plain form), and the precision form: the
-    same terms and knots, mse within 1e-6."""
+    """1M rows x 100 columns, hinge fit to 65 terms (plain form), and the precision forms (FP32
+    and BF16-split first projections): the same terms and knots, mse within 1e-6."""
     X, y = datagen.workload3()
     kw = datagen.FIT_KW[3]
     say("config 3 full: X", X.shape)
@@ -82,9 +82,19 @@ def test_config3_full_properties():
     _, Bo = malgo.basis()
     Qm = np.asarray(Bo, dtype=np.float64)
     malgo.close()
+    bf, btrace, balgo = _fit(X, y, kw, "mixed-bf16")
+    bst = balgo.stats()
+    _, Bo = balgo.basis()
+    Qb = np.asarray(Bo, dtype=np.float64)
+    balgo.close()
     model, trace, algo = _fit(X, y, kw, "fp64")
-    assert _seq(model) == _seq(mixed)
+    assert _seq(model) == _seq(mixed) == _seq(bf)
     pm = [v for s in trace for v in s["mse"]]
+    bm = [v for s in btrace for v in s["mse"]]
+    assert len(pm) == len(bm) and all(abs(u - v) <= 1e-6 * abs(u) for u, v in zip(pm, bm))
+    say("config 3 mixed-bf16: rescored %d of %d, window violations %d, max|QtQ-I| %.3g"
+        % (bst["rescored"], bst["scored"], bst["window_violations"], np.abs(Qb.T @ Qb - np.eye(Qb.shape[1])).max()))
+    assert bst["window_violations"] == 0
     mm = [v for s in mtrace for v in s["mse"]]
     assert len(pm) == len(mm) and all(abs(u - v) <= 1e-6 * abs(u) for u, v in zip(pm, mm))
     say("config 3 mixed: rescored %d of %d, window violations %d, max|QtQ-I| %.3g"
