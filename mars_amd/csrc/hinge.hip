@@ -947,10 +947,11 @@ void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const dou
 
     const dim3 grid(C, pgroups, ng);
     // reference algorithmic work: n (m+1) (4 + 4/min_span) flops per candidate (marsalgo.cc:576-649);
-    // bytes per pass: y, the m basis columns and the deltas per X column, b and Bx per candidate
+    // bytes, each of the two passes: y, the m basis columns and the deltas per X column, b and Bx
+    // per candidate
     {
         KTimer kt(ctx, K_SWEEP, (double)P * (double)tail * (m + 1) * (4.0 + 4.0 / min_span),
-                  (double)ng * tail * (4.0 * m + 8.0) + (double)P * tail * 8.0);
+                  2.0 * ((double)ng * tail * (4.0 * m + 8.0) + (double)P * tail * 8.0));
         for (int pass = 0; pass < 2; ++pass) {
             const bool eval = pass == 1;
             if (eval) {
