@@ -236,6 +236,110 @@ __device__ __forceinline__ double group_sum(double v)
 }
 
 /*
+ * The sorted sweep's tile loads: a tile of TR sorted rows is PT slot columns (y and the Bo
+ * columns of X column xc's replica; the others a zero column) and, per candidate, b (its
+ * parent's B column) and Bx (its normalised column), each a float4 run of 4 rows per lane
+ * (NT threads per block).  load() fetches tile tt into registers, store() puts them into one
+ * LDS buffer as FP64: sq[row][slot] and sbx[row][candidate] = (b, Bx).
+ */
+template <int PT, int NC, int NT, int TR = kTR, int NS = 1>
+struct SortedRuns {
+    static constexpr int QC = TR / 4;                         // float4 runs per column and tile
+    static constexpr int NQ = (PT * QC + NT - 1) / NT;        // slot runs per lane
+    static constexpr int NQC = (2 * NC * QC + NT - 1) / NT;   // candidate runs per lane
+    static constexpr bool QPART = NQ * NT > PT * QC;          // some lanes have no slot run
+    static constexpr bool CPART = NQC * NT > 2 * NC * QC;
+    const float *qs[NQ], *cq[NQC];
+    int qcol[NQ], qq[NQ], cj[NQC], cqq[NQC];
+    bool cisb[NQC];
+    float4 qv[NS][NQ], cv[NS][NQC];   // NS register sets: loads of NS tiles in flight
+    size_t r0, r1;
+
+    /* Runs are addressed from the chunk start r0 (a multiple of 16); candidates c0 + sbase + j,
+     * j < nact, of X column xc. */
+    __device__ void init(const SweepArgs &a, int tid, int xc, int m, int c0, int sbase, int nact, size_t cr0,
+                         size_t cr1)
+    {
+        r0 = cr0;
+        r1 = cr1;
+        const float *blk = a.rep + (size_t)xc * a.rstride;  // [x | y | B (mcap) | Bo (mcap)]
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            const int e = tid + NT * i;
+            const int col = e / QC;
+            qcol[i] = e < PT * QC ? col : -1;
+            qq[i] = e % QC;
+            const float *p = a.zero;
+            if (col == 1)
+                p = blk + a.ldx;
+            else if (col >= 3 && col < m + 3)
+                p = blk + (size_t)(2 + a.mcap + col - 3) * a.ldx;
+            qs[i] = p + r0 + 4 * qq[i];
+        }
+#pragma unroll
+        for (int i = 0; i < NQC; ++i) {
+            const int e = tid + NT * i;
+            const bool ok = e < 2 * NC * QC;
+            cisb[i] = e < NC * QC;
+            const int rem = e % (NC * QC);
+            cj[i] = ok ? rem / QC : -1;
+            cqq[i] = rem % QC;
+            const float *p = a.zero;
+            if (ok && cj[i] < nact) {
+                const int cc = c0 + sbase + cj[i];
+                p = cisb[i] ? blk + (size_t)(2 + a.cpar[cc]) * a.ldx : a.hr + (size_t)cc * a.ldx;
+            }
+            cq[i] = p + r0 + 4 * cqq[i];
+        }
+    }
+    /* Tile tt into register set S; rows at or past r1 (only in the chunk's last tile) read as zero. */
+    template <int S = 0>
+    __device__ void load(int tt)
+    {
+        const size_t off = (size_t)tt * TR;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) qv[S][i] = *(const float4 *)(qs[i] + off);
+#pragma unroll
+        for (int i = 0; i < NQC; ++i) cv[S][i] = *(const float4 *)(cq[i] + off);
+        const size_t tb = r0 + off;
+        if (tb + TR > r1) {   // (uniform)
+            auto cut = [&](float4 v, int q) {
+                const size_t r = tb + 4 * q;
+                return float4{r < r1 ? v.x : 0.f, r + 1 < r1 ? v.y : 0.f, r + 2 < r1 ? v.z : 0.f,
+                              r + 3 < r1 ? v.w : 0.f};
+            };
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) qv[S][i] = cut(qv[S][i], qq[i]);
+#pragma unroll
+            for (int i = 0; i < NQC; ++i) cv[S][i] = cut(cv[S][i], cqq[i]);
+        }
+    }
+    template <int S = 0, int SQ = PT + 2>
+    __device__ void store(double (*sq)[SQ], double2 (*sbx)[NC])
+    {
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            if (QPART && qcol[i] < 0) continue;
+            const int rr = 4 * qq[i];
+            sq[rr][qcol[i]] = (double)qv[S][i].x;
+            sq[rr + 1][qcol[i]] = (double)qv[S][i].y;
+            sq[rr + 2][qcol[i]] = (double)qv[S][i].z;
+            sq[rr + 3][qcol[i]] = (double)qv[S][i].w;
+        }
+#pragma unroll
+        for (int i = 0; i < NQC; ++i) {
+            if (CPART && cj[i] < 0) continue;
+            const int rr = 4 * cqq[i];
+            double *dst = cisb[i] ? &sbx[rr][cj[i]].x : &sbx[rr][cj[i]].y;
+            dst[0] = (double)cv[S][i].x;
+            dst[2 * NC] = (double)cv[S][i].y;
+            dst[4 * NC] = (double)cv[S][i].z;
+            dst[6 * NC] = (double)cv[S][i].w;
+        }
+    }
+};
+
+/*
  * One block = NC candidates (parents c0.. of one X column) swept over the
  * sorted rows [r0, r1) of its chunk (marsalgo.cc:576-649).  Tiles of 16 sorted
  * rows go to LDS in FP64: row-major [0 | y | 0 | Bo row | 0...] (PT values,
@@ -265,11 +369,6 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
     static_assert(J % 2 == 0, "slot pairs");
     constexpr int NB = SORTED ? 1 : kTR / RPS;             // gather: Bo-row loads per lane and tile
     constexpr int NPC = 2 * NC * kTR / NT;                 // gather: per-candidate loads per lane and tile
-    constexpr int QC = kTR / 4;                            // float4 runs per column and tile
-    constexpr int NQ = (PT * QC + NT - 1) / NT;            // sorted: y / Bo runs per lane and tile
-    constexpr int NQC = (2 * NC * QC + NT - 1) / NT;       // sorted: b / Bx runs per lane and tile
-    constexpr bool QPART = NQ * NT > PT * QC;              // some lanes have no run
-    constexpr bool CPART = NQC * NT > 2 * NC * QC;
     constexpr int SQ = PT + 2;                             // LDS row stride of the y / Bo tile
     __shared__ __attribute__((aligned(16))) double sq[2][kTR][SQ];
     __shared__ double2 sbx[2][kTR][NC];                    // (b, Bx) per candidate
@@ -304,41 +403,9 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
     size_t pld[NPC];
     int prr[NPC], pj[NPC];
     bool pisb[NPC];
-    // ---- sorted sources: float4 run i of a tile is column qcol[i] (or candidate value cj[i]), rows 4 qq[i]..
-    const float *qs[NQ], *cq[NQC];
-    int qcol[NQ], qq[NQ], cj[NQC], cqq[NQC];
-    bool cisb[NQC];
+    SortedRuns<PT, NC, NT> R;   // sorted: the tile loads
     if constexpr (SORTED) {
-        const float *blk = a.rep + (size_t)xc * a.rstride;  // [x | y | B (mcap) | Bo (mcap)]
-        const size_t rb = r0;                                // runs are addressed from the chunk start
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-            const int e = tid + NT * i;
-            const int col = e / QC;
-            qcol[i] = e < PT * QC ? col : -1;
-            qq[i] = e % QC;
-            const float *p = a.zero;
-            if (col == 1)
-                p = blk + a.ldx;
-            else if (col >= 3 && col < m + 3)
-                p = blk + (size_t)(2 + a.mcap + col - 3) * a.ldx;
-            qs[i] = p + rb + 4 * qq[i];
-        }
-#pragma unroll
-        for (int i = 0; i < NQC; ++i) {
-            const int e = tid + NT * i;
-            const bool ok = e < 2 * NC * QC;
-            cisb[i] = e < NC * QC;
-            const int rem = e % (NC * QC);
-            cj[i] = ok ? rem / QC : -1;
-            cqq[i] = rem % QC;
-            const float *p = a.zero;
-            if (ok && cj[i] < nact) {
-                const int cc = c0 + sbase + cj[i];
-                p = cisb[i] ? blk + (size_t)(2 + a.cpar[cc]) * a.ldx : a.hr + (size_t)cc * a.ldx;
-            }
-            cq[i] = p + rb + 4 * cqq[i];
-        }
+        R.init(a, tid, xc, m, c0, sbase, nact, r0, r1);
     } else {
         if (pos == 1) {
             bp = a.y;
@@ -367,49 +434,6 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
             }
         }
     }
-    // sorted: the runs of tile tt; rows at or past r1 (only in the tile before the tail) read as zero
-    float4 qv[NQ], cv[NQC];
-    auto sorted_load = [&](int tt) {
-        const size_t off = (size_t)tt * kTR;
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) qv[i] = *(const float4 *)(qs[i] + off);
-#pragma unroll
-        for (int i = 0; i < NQC; ++i) cv[i] = *(const float4 *)(cq[i] + off);
-        const size_t tb = r0 + off;
-        if (tb + kTR > r1) {   // (uniform)
-            auto cut = [&](float4 v, int q) {
-                const size_t r = tb + 4 * q;
-                return float4{r < r1 ? v.x : 0.f, r + 1 < r1 ? v.y : 0.f, r + 2 < r1 ? v.z : 0.f,
-                              r + 3 < r1 ? v.w : 0.f};
-            };
-#pragma unroll
-            for (int i = 0; i < NQ; ++i) qv[i] = cut(qv[i], qq[i]);
-#pragma unroll
-            for (int i = 0; i < NQC; ++i) cv[i] = cut(cv[i], cqq[i]);
-        }
-    };
-    auto sorted_store = [&](int b) {
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-            if (QPART && qcol[i] < 0) continue;
-            const int rr = 4 * qq[i];
-            sq[b][rr][qcol[i]] = (double)qv[i].x;
-            sq[b][rr + 1][qcol[i]] = (double)qv[i].y;
-            sq[b][rr + 2][qcol[i]] = (double)qv[i].z;
-            sq[b][rr + 3][qcol[i]] = (double)qv[i].w;
-        }
-#pragma unroll
-        for (int i = 0; i < NQC; ++i) {
-            if (CPART && cj[i] < 0) continue;
-            const int rr = 4 * cqq[i];
-            double *dst = cisb[i] ? &sbx[b][rr][cj[i]].x : &sbx[b][rr][cj[i]].y;
-            dst[0] = (double)cv[i].x;
-            dst[2 * NC] = (double)cv[i].y;
-            dst[4 * NC] = (double)cv[i].z;
-            dst[6 * NC] = (double)cv[i].w;
-        }
-    };
-
     // ---- lane state
     double f[J], g[J], Y[J];
     const double isb = t == 0 ? 1.0 : 0.0;   // this lane holds the b, y and bx slots
@@ -465,8 +489,8 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
     if (T > 0) {
         const int nr = tile_rows(0);
         if constexpr (SORTED) {
-            sorted_load(0);
-            sorted_store(0);
+            R.template load<0>(0);
+            R.template store<0>(sq[0], sbx[0]);
         } else {
 #pragma unroll
             for (int i = 0; i < NB; ++i) {
@@ -500,7 +524,7 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
         float bov[NB], pcv[NPC];
         int nrow = 0;
         if constexpr (SORTED) {
-            if (nr1 > 0) sorted_load(tt + 1);
+            if (nr1 > 0) R.template load<0>(tt + 1);
         } else {
 #pragma unroll
             for (int i = 0; i < NB; ++i) {
@@ -593,7 +617,7 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
         // buffer nxt last held tile t-1, which every wave finished before the
         // previous iteration's barrier: store tile t+1 into it
         if constexpr (SORTED) {
-            if (nr1 > 0) sorted_store(nxt);
+            if (nr1 > 0) R.template store<0>(sq[nxt], sbx[nxt]);
         } else {
 #pragma unroll
             for (int i = 0; i < NB; ++i) sq[nxt][i * RPS + ro][pos] = (double)bov[i];
@@ -633,6 +657,139 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
             dst[2 * (t * J + s) + 1] = g[s];
         }
         if (t == 0) a.ksum[cs] = k2;
+    }
+}
+
+/*
+ * Chunk summaries of the sorted sweep (its first pass) on FP64 MFMA.  Run from the zero state,
+ * the recurrence f += d g, g += b q over the chunk's rows l ends at
+ *     g = G = sum_l b_l q_l,      f = sum_l b_l q_l (D - W_l) = D G - H,   H = sum_l b_l W_l q_l,
+ * W_l = the chunk's deltas up to and including row l, D = all of them.  For the slots shared by
+ * the X column's parents (y, the Bo columns) G and H are two contractions [b; b W] (candidates x
+ * rows) x q (rows x slots): v_mfma_f64_16x16x4f64 with candidates as M, slots as N and 4 sorted
+ * rows as K, wave w taking slots 16 w .. 16 w + 15.  The candidate's own slots (b, Bx) and k
+ * run the recurrence itself on one lane per candidate of wave 0.  Tiles as in k_sweep (16 rows,
+ * LDS in FP64, double-buffered); the prefix W of each tile is scanned by wave 0.  (Measured
+ * slower: a separate wave for the candidates' own slots, and tile loads issued three tiles
+ * ahead.)
+ */
+template <int PT, int NC>
+__global__ __launch_bounds__(64 * (PT / 16)) void k_sweep_sum(SweepArgs a)
+{
+    constexpr int NW = PT / 16, NT = 64 * NW, SQ = PT + 2, MT = (NC + 15) / 16;
+    __shared__ __attribute__((aligned(16))) double sq[2][kTR][SQ];
+    __shared__ double2 sbx[2][kTR][NC];
+    __shared__ double sd[2][kTR], sw[2][kTR];
+
+    const int m = a.m;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, kr = lane >> 4, kc = lane & 15;
+    const int chunk = blockIdx.x, pg = blockIdx.y, gi = blockIdx.z;
+    const int xc = a.grp[3 * gi], c0 = a.grp[3 * gi + 1], np = a.grp[3 * gi + 2];
+    const int sbase = pg * NC;
+    if (sbase >= np) return;   // whole block idle (uniform)
+    const int nact = min(NC, np - sbase);
+    const size_t r0 = (size_t)chunk * a.clen;
+    const size_t r1 = min(a.tail, r0 + a.clen);
+    const float *dl = a.delta + (size_t)xc * a.n;
+    const int T = r1 > r0 ? (int)((r1 - r0 + kTR - 1) / kTR) : 0;
+    auto tile_rows = [&](int tt) { return (int)min((size_t)kTR, r1 - (r0 + (size_t)tt * kTR)); };
+    const bool own = wv == 0 && lane < NC;  // the lane of candidate `lane`'s own slots
+
+    SortedRuns<PT, NC, NT> R;
+    R.init(a, tid, xc, m, c0, sbase, nact, r0, r1);
+    v4d accG[MT], accH[MT];
+#pragma unroll
+    for (int u = 0; u < MT; ++u) accG[u] = accH[u] = v4d{0, 0, 0, 0};
+    double fb = 0.0, gb = 0.0, fx = 0.0, gx = 0.0, k2 = 0.0;
+    double wbase = 0.0;   // W at the end of the previous tile (wave 0)
+    // deltas of tile tt into sd[b], their running prefix into sw[b] (wave 0)
+    auto deltas = [&](int tt, int b) {
+        if (wv != 0) return;
+        const int nr = tt < T ? tile_rows(tt) : 0;
+        const double d = lane < nr ? (double)dl[r0 + (size_t)tt * kTR + lane] : 0.0;
+        double w = d;
+#pragma unroll
+        for (int o = 1; o < kTR; o <<= 1) {
+            const double v = __shfl_up(w, o, 64);
+            if (lane >= o) w += v;
+        }
+        if (lane < kTR) {
+            sd[b][lane] = d;
+            sw[b][lane] = wbase + w;
+        }
+        wbase += __shfl(w, kTR - 1, 64);
+    };
+    if (T > 0) {
+        R.template load<0>(0);
+        R.template store<0>(sq[0], sbx[0]);
+        deltas(0, 0);
+    }
+    __syncthreads();
+    for (int tt = 0; tt < T; ++tt) {
+        const int cur = tt & 1, nxt = cur ^ 1;
+        const bool more = tt + 1 < T;
+        if (more) R.template load<0>(tt + 1);
+        {
+#pragma unroll
+            for (int ks = 0; ks < kTR / 4; ++ks) {
+                const int row = 4 * ks + kr;   // K step ks takes rows 4 ks .. 4 ks + 3
+                const double q = sq[cur][row][16 * wv + kc];
+                const double w = sw[cur][row];
+#pragma unroll
+                for (int u = 0; u < MT; ++u) {
+                    const int j = 16 * u + kc;
+                    const double b = j < NC ? sbx[cur][row][j < NC ? j : 0].x : 0.0;
+                    accG[u] = mfma_f64(b, q, accG[u]);
+                    accH[u] = mfma_f64(b * w, q, accH[u]);
+                }
+            }
+        }
+        if (own) {
+#pragma unroll
+            for (int rr = 0; rr < kTR; ++rr) {
+                const double d = sd[cur][rr];
+                const double2 bx = sbx[cur][rr][lane];
+                const double fb1 = fma(d, gb, fb);
+                k2 = fma(d, fb + fb1, k2);
+                fb = fb1;
+                gb = fma(bx.x, bx.x, gb);
+                fx = fma(d, gx, fx);
+                gx = fma(bx.x, bx.y, gx);
+            }
+        }
+        if (more) {
+            R.template store<0>(sq[nxt], sbx[nxt]);
+            deltas(tt + 1, nxt);
+        }
+        __syncthreads();
+    }
+    // D: the chunk's delta sum (the carry's shift), from the same prefix chain
+    __shared__ double sD;
+    if (tid == 0) sD = wbase;
+    __syncthreads();
+    const double Dc = sD;
+    if (pg == 0 && tid == 0) a.dsum[(size_t)gi * a.C + chunk] = Dc;
+    {
+        // D layout of the FP64 MFMA: lane l holds rows (candidates) l/16 + 4 v, column (slot) l % 16
+#pragma unroll
+        for (int u = 0; u < MT; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int j = 16 * u + kr + 4 * v, sl = 16 * wv + kc;
+                if (j >= nact || sl == 0 || sl == 2) continue;
+                double *dst = a.sum + (((size_t)(c0 + sbase + j) * a.C + chunk) * a.L2 + sl) * 2;
+                dst[0] = fma(Dc, accG[u][v], -accH[u][v]);
+                dst[1] = accG[u][v];
+            }
+    }
+    if (own && lane < nact) {
+        const size_t cs = (size_t)(c0 + sbase + lane) * a.C + chunk;
+        double *dst = a.sum + cs * a.L2 * 2;
+        dst[0] = fb;
+        dst[1] = gb;
+        dst[4] = fx;
+        dst[5] = gx;
+        a.ksum[cs] = k2;
     }
 }
 
@@ -723,8 +880,32 @@ int sweep_slots(int m, bool sorted)
     return 256;
 }
 
+template <int PT>
+void launch_sweep_sum(const SweepArgs &a, dim3 grid, int nc, hipStream_t st)
+{
+    const dim3 block(64 * (PT / 16));
+    switch (nc) {
+    case 8: hipLaunchKernelGGL((k_sweep_sum<PT, 8>), grid, block, 0, st, a); break;
+    case 16: hipLaunchKernelGGL((k_sweep_sum<PT, 16>), grid, block, 0, st, a); break;
+    default: hipLaunchKernelGGL((k_sweep_sum<PT, 32>), grid, block, 0, st, a); break;
+    }
+    MH_LAUNCH_CHECK();
+}
+
 void launch_sweep_sorted(const SweepArgs &a, dim3 grid, int pt, int nc, hipStream_t st, bool eval)
 {
+    if (!eval) {  // the summary pass on MFMA
+        switch (pt) {
+        case 32: launch_sweep_sum<32>(a, grid, nc, st); break;
+        case 48: launch_sweep_sum<48>(a, grid, nc, st); break;
+        case 64: launch_sweep_sum<64>(a, grid, nc, st); break;
+        case 96: launch_sweep_sum<96>(a, grid, nc, st); break;
+        case 128: launch_sweep_sum<128>(a, grid, nc, st); break;
+        case 192: launch_sweep_sum<192>(a, grid, nc, st); break;
+        default: launch_sweep_sum<256>(a, grid, nc, st); break;
+        }
+        return;
+    }
     switch (pt) {
     case 32: launch_sweep<32, true>(a, grid, nc, st, eval); break;
     case 48: launch_sweep<48, true>(a, grid, nc, st, eval); break;
