@@ -65,6 +65,7 @@ def fit(lib, X, y, w=None, **kwargs):
     aging_factor = kwargs.pop("aging_factor", 1.0)
     threads = kwargs.pop("threads", -1)
     max_epochs = kwargs.pop("max_epochs", None)  # oracle-only: truncate long fits
+    cuda = kwargs.pop("cuda", False)  # the binding's cuda= flag (mars.py:156-160, 330)
     if kwargs:
         raise TypeError("unknown argument: %s" % kwargs)
 
@@ -119,7 +120,7 @@ def fit(lib, X, y, w=None, **kwargs):
         if has_knots:
             sse0, sse1, sse2, cut, knots, _ = algo.eval_knots(bmask, min_span, tail, linear_only, threads)
         else:
-            sse0, sse1, sse2, cut = algo.eval(bmask, min_span, tail, linear_only, threads)
+            sse0, sse1, sse2, cut = algo.eval(bmask, min_span, tail, linear_only, threads, cuda)
             knots = None
 
         active = bmask.any(axis=0)

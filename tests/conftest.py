@@ -15,12 +15,9 @@ def pytest_configure(config):
 
 
 def gpu_available():
-    try:
-        import torch  # noqa: F401  plumbing only: device presence
-
-        return torch.cuda.is_available()
-    except Exception:
-        return False
+    """Whether this host exposes an AMD GPU (the kernel driver's device node).  No GPU runtime is
+    loaded to find out."""
+    return os.path.exists("/dev/kfd")
 
 
 @pytest.fixture(scope="session")
