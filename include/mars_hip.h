@@ -147,6 +147,61 @@ int mars_hip_fit_append(mars_hip_ctx *ctx, char type, int xcol, int bcol, float 
  * returns after the result has been copied to *mse. */
 int mars_hip_fit_append_noalloc(mars_hip_ctx *ctx, char type, int xcol, int bcol, float h, double *mse);
 
+/* Forward-pass options of mars.fit (mars.py:190-247) for mars_hip_fit_forward, resolved by the
+ * caller as mars.fit resolves them: max_terms after the dof clamp (mars.py:241-243), min_span
+ * after Friedman's rule (mars.py:225-228), end_span = max(int(n * tail_span), 1) (mars.py:247). */
+typedef struct mars_hip_forward_opts {
+    int max_terms;
+    int max_degree;
+    double penalty;
+    int min_span;
+    int end_span;
+    int self_interact;
+    int linear_only;
+    double n_true;
+    int r2_window;
+    double r2_thresh;
+    double max_runtime;  /* seconds */
+} mars_hip_forward_opts;
+
+/* One model row (mars.py:261-275, 366-375). */
+typedef struct mars_hip_model_row {
+    int type;       /* 'i', 'l', '+' or '-' */
+    int basis;
+    int input;
+    int order;
+    double hinge;   /* knot (NaN for 'i' and 'l') */
+    double mse;     /* append mse (normalized units); 0 for the intercept */
+    float r2;
+    float r2_cv;
+    float time;     /* seconds since the call started, at this term's step */
+    int pad;
+} mars_hip_model_row;
+
+/* One forward step: the chosen candidate, its scores, the appends' mse values. */
+typedef struct mars_hip_step {
+    int kind;       /* 'l' (linear term) or 'h' (hinge pair) */
+    int xcol, bcol;
+    int knot;       /* sorted position of the knot (-1 for 'l') */
+    int n_cand;     /* candidates scored */
+    int n_mse;      /* appends attempted (1 or 2; fewer when the term budget ran out) */
+    double cut;     /* knot value (NaN for 'l') */
+    double sse0, sse1, sse2;
+    double mse[2];  /* each append's mse (-1: degenerate column, not added) */
+} mars_hip_step;
+
+/* The whole forward pass of a fresh single-fit context (nbasis 1), driven in
+ * C++: mars.fit's loop (mars.py:285-406) with Fast-MARS ranking off (its
+ * default, max_basis = max_inputs = None) and no per-candidate filter -- per
+ * step the degree / self-interaction mask, mars_hip_fit_eval's scoring and
+ * argmax, the GCV choice between the linear term and the hinge pair, the
+ * appends and the stopping rules (float32 model fields compared in float32, as
+ * numpy 2 compares them).  rows: at least opts->max_terms entries; *nrows rows
+ * written (the intercept first).  steps (may be NULL): up to steps_capacity
+ * step records; *nsteps (may be NULL) receives the step count. */
+int mars_hip_fit_forward(mars_hip_ctx *ctx, const mars_hip_forward_opts *opts, mars_hip_model_row *rows,
+                         int *nrows, mars_hip_step *steps, int steps_capacity, int *nsteps);
+
 /* Copy the live basis out: B and Bo as (n, m) row-major f32 (either may be NULL). */
 int mars_hip_fit_basis(mars_hip_ctx *ctx, float *B, float *Bo, size_t *m_out);
 

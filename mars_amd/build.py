@@ -14,7 +14,7 @@ import sysconfig
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-HIP_SOURCES = ["ortho.hip", "hinge.hip", "mixed.hip", "fit.hip", "batch.hip", "post.hip"]
+HIP_SOURCES = ["ortho.hip", "hinge.hip", "mixed.hip", "fit.hip", "forward.hip", "batch.hip", "post.hip"]
 CXX_SOURCES = ["prune.cc"]  # host-only parts of libmars_hip
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MARS_HIP_ARCH", "gfx950")

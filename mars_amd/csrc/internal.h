@@ -122,6 +122,9 @@ void fit_append_batch(mars_hip_ctx *ctx, std::vector<AppendArg> &args, std::vect
                       std::vector<BestOut> *best = nullptr);
 /* Per-fit argmax of `lin` over each fit's candidates (first maximum in candidate order), queued
  * on the stream: ctx->bout receives it, ctx->ival holds the fits' candidate offsets and counts. */
+void fit_eval(mars_hip_ctx *ctx, const unsigned char *mask, size_t rows, size_t stride, int min_span,
+              int end_span, int linear_only, double *lin_out, double *hin_out, double *cut_out,
+              mars_hip_best *best);
 void argmax_launch(mars_hip_ctx *ctx, const Cands &cs, const double *lin);
 /* The same over ranges already on the device (ctx->ival: first candidate, count per fit). */
 void argmax_launch_ranges(mars_hip_ctx *ctx, const double *lin, size_t bound);

@@ -11,6 +11,7 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/numpy.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -182,6 +183,66 @@ public:
         best["knot2"] = b.knot2;
         best["cut2"] = b.cut2;
         return py::make_tuple(b.sse0, d1, d2, hc, best);
+    }
+
+    /* The whole forward pass in C++ (mars_hip_fit_forward, mars.py:285-406 with Fast-MARS ranking
+     * off).  Returns (records, mse, steps): the model rows as packed records of mars.MODEL_DTYPE
+     * (type S1, basis i4, input i4, hinge f8, beta f4, r2 f4, r2_cv f4, order i4, time f4: 37
+     * bytes), each row's append mse, and one dict per step (mars.fit's trace entries). */
+    py::tuple forward(int max_terms, int max_degree, double penalty, int min_span, int end_span,
+                      bool self_interact, bool linear_only, double n_true, int r2_window, double r2_thresh,
+                      double max_runtime)
+    {
+        verify(max_terms >= 1, "max_terms must be >= 1");
+        mars_hip_forward_opts o{max_terms,   max_degree, penalty,   min_span,  end_span,   self_interact ? 1 : 0,
+                                linear_only ? 1 : 0, n_true, r2_window, r2_thresh, max_runtime};
+        std::vector<mars_hip_model_row> rows((size_t)max_terms);
+        std::vector<mars_hip_step> steps((size_t)max_terms);
+        int nr = 0, ns = 0, rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = mars_hip_fit_forward(ctx_, &o, rows.data(), &nr, steps.data(), (int)steps.size(), &ns);
+        }
+        check(rc);
+        constexpr size_t kRec = 37;
+        std::string rec((size_t)nr * kRec, '\0');
+        py::array_t<double> mse(nr);
+        char *q = &rec[0];
+        const float fnan = std::nanf("");
+        for (int r = 0; r < nr; ++r, q += kRec) {
+            const mars_hip_model_row &x = rows[(size_t)r];
+            const char ty = (char)x.type;
+            std::memcpy(q, &ty, 1);
+            std::memcpy(q + 1, &x.basis, 4);
+            std::memcpy(q + 5, &x.input, 4);
+            std::memcpy(q + 9, &x.hinge, 8);
+            std::memcpy(q + 17, &fnan, 4);
+            std::memcpy(q + 21, &x.r2, 4);
+            std::memcpy(q + 25, &x.r2_cv, 4);
+            std::memcpy(q + 29, &x.order, 4);
+            std::memcpy(q + 33, &x.time, 4);
+            mse.mutable_data()[r] = x.mse;
+        }
+        py::list sl;
+        for (int k = 0; k < std::min(ns, (int)steps.size()); ++k) {
+            const mars_hip_step &t = steps[(size_t)k];
+            py::dict d;
+            d["epoch"] = k + 1;
+            d["xcol"] = t.xcol;
+            d["bcol"] = t.bcol;
+            d["cut"] = t.cut;
+            d["knot"] = t.knot;
+            d["kind"] = std::string(1, (char)t.kind);
+            d["sse0"] = t.sse0;
+            d["sse1"] = t.sse1;
+            d["sse2"] = t.sse2;
+            d["n_cand"] = t.n_cand;
+            py::list ms;
+            for (int j = 0; j < t.n_mse; ++j) ms.append(t.mse[j]);
+            d["mse"] = ms;
+            sl.append(d);
+        }
+        return py::make_tuple(py::bytes(rec), mse, sl);
     }
 
     /* MarsAlgo::append (marsalgo.cc:760-875) */
@@ -508,6 +569,9 @@ PYBIND11_MODULE(_marslib, m)
         .def("yvar", &MarsAlgo::yvar)
         .def("dsse", &MarsAlgo::dsse)
         .def("append", &MarsAlgo::append)
+        .def("forward", &MarsAlgo::forward, py::arg("max_terms"), py::arg("max_degree"), py::arg("penalty"),
+             py::arg("min_span"), py::arg("end_span"), py::arg("self_interact"), py::arg("linear_only"),
+             py::arg("n_true"), py::arg("r2_window"), py::arg("r2_thresh"), py::arg("max_runtime"))
         .def("set_precision", &MarsAlgo::set_precision)
         .def("basis", &MarsAlgo::basis)
         .def("stats", &MarsAlgo::stats)

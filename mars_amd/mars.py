@@ -100,7 +100,7 @@ def fit(X, y, w=None, **kwargs):
     r2_window     = kwargs.pop("r2_window", 32)
     r2_thresh     = kwargs.pop("r2_thresh", 3e-5)
     labels        = kwargs.pop("labels", None)
-    aux_filter    = kwargs.pop("xfilter", lambda x, b: True)
+    aux_filter    = kwargs.pop("xfilter", None)
     max_basis     = kwargs.pop("max_basis", None)
     max_inputs    = kwargs.pop("max_inputs", None)
     aging_factor  = kwargs.pop("aging_factor", 1.0)
@@ -129,7 +129,6 @@ def fit(X, y, w=None, **kwargs):
 
     get_dof = lambda m: m + penalty * (m - 1)  # noqa: E731
     gcv_adj = lambda mse, m: mse / (1.0 - get_dof(m) / n_true) ** 2  # noqa: E731
-    basic_filter = lambda i, b: (len(b) < max_degree) and (self_interact or (i not in b))  # noqa: E731
 
     terms = np.arange(1, max_terms + 1)
     max_terms = terms[get_dof(terms) < n_true].max()
@@ -146,6 +145,21 @@ def fit(X, y, w=None, **kwargs):
         algo.set_precision(precision)
     var_y = algo.yvar()
 
+    if max_basis is None and max_inputs is None and aux_filter is None and logger is None and callback is None:
+        # the default configuration: the whole loop runs in C++ (mars_hip_fit_forward)
+        offset = time.time() - start_t
+        recs, _, steps = algo.forward(int(max_terms), int(max_degree), float(penalty), int(min_span), int(tail),
+                                      bool(self_interact), bool(linear_only), float(n_true), int(r2_window),
+                                      float(r2_thresh), float(max_runtime) - offset)
+        model = np.frombuffer(recs, dtype=MODEL_DTYPE).copy()
+        model["time"][1:] += np.float32(offset)
+        if trace is not None:
+            trace.extend(steps)
+        if profile:
+            sys.stderr.write("[mars_amd] fit: %.3fs total (C++ driver) | %d epochs | %s\n"
+                             % (time.time() - start_t, len(steps), algo.stats()))
+        return model
+
     epoch = 0
     basis_sse = np.array([0.0])
     input_sse = np.full(p, np.inf)
@@ -159,17 +173,26 @@ def fit(X, y, w=None, **kwargs):
         r[np.argsort(v)] = np.arange(len(v))
         return r
 
+    # the degree / self-interaction filter (mars.py:237-239) as arrays: each basis function's degree
+    # and variable set
+    degree = np.zeros(max_terms, dtype=np.intp)
+    uses = np.zeros((max_terms, p), dtype=bool)
+
     while algo.nbasis() < max_terms:
         basis_to_use = np.argsort(basis_sse)[::-1][:max_basis]
         input_to_use = np.argsort(_ranks(input_sse) + aging_factor * (epoch - input_age) * (input_sse > 0))
         n_take = None if max_inputs is None else max_inputs + int(np.isinf(input_sse).sum())
         input_to_use = np.sort(input_to_use[::-1][:n_take])
 
-        bmask = np.zeros((p, len(basis)), dtype="bool")
+        nb = len(basis)
+        bmask = np.zeros((p, nb), dtype="bool")
         bmask[np.ix_(input_to_use, basis_to_use)] = True
-        for i in input_to_use:
-            bmask[i] &= np.array([basic_filter(i, b) for b in basis])
-            bmask[i] &= np.array([aux_filter(i, b) for b in basis])
+        bmask &= degree[None, :nb] < max_degree
+        if not self_interact:
+            bmask &= ~uses[:nb].T
+        if aux_filter is not None:
+            for i in input_to_use:
+                bmask[i] &= np.array([aux_filter(i, b) for b in basis])
 
         e0 = time.perf_counter()
         sse0, sse1, sse2, cut, best = algo.eval_best(bmask, min_span, tail, linear_only)
@@ -205,6 +228,9 @@ def fit(X, y, w=None, **kwargs):
             step["mse"].append(float(mse))
             if mse >= 0:
                 basis.append(basis[bcol] + [xcol])
+                degree[m] = degree[bcol] + 1
+                uses[m] = uses[bcol]
+                uses[m, xcol] = True
                 model[m] = (htype, bcol, xcol, hcut, np.nan, 1.0 - mse / var_y,
                             1.0 - gcv_adj(mse, m + 1) / var_y, len(basis[-1]), dt)
                 basis_sse = np.append(basis_sse, [0.0])
