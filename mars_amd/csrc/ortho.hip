@@ -772,7 +772,8 @@ __global__ __launch_bounds__(256) void k_compact_write(const int *__restrict__ d
  * split_rows build it on the host: each fit's positions [first, first + cnt) cut into tiles of
  * `tile`, each tile into row ranges of about n * target / work rows (at least minrows, at most
  * maxsplits ranges, multiples of align); cmap gets (first item, item count, slot) per position.
- * One block of 1024 threads; fits are split into contiguous chunks per thread.
+ * One block of 1024 threads: the fits are split into contiguous ranges per thread for the item
+ * counts, and the items themselves are written by the whole block in index order.
  */
 __global__ __launch_bounds__(1024) void k_pass_build(const FitDev *__restrict__ fits, int F,
                                                      const int *__restrict__ first, const int *__restrict__ cnt,
@@ -781,7 +782,7 @@ __global__ __launch_bounds__(1024) void k_pass_build(const FitDev *__restrict__ 
                                                      int *__restrict__ cmap, int cap, int *__restrict__ nitems)
 {
     __shared__ double sw[1024];
-    __shared__ long long ss[1024];
+    __shared__ long long ss[1024];  // inclusive prefix of the item counts of the threads' fit ranges
     const int t = threadIdx.x;
     const int chunk = (F + 1023) / 1024, f0 = min(F, t * chunk), f1 = min(F, f0 + chunk);
     double w = 0.0;
@@ -816,31 +817,50 @@ __global__ __launch_bounds__(1024) void k_pass_build(const FitDev *__restrict__ 
         ss[t] += v;
         __syncthreads();
     }
-    long long at = ss[t] - mine;
-    if (t == 1023) nitems[0] = (int)min(ss[1023], (long long)cap);
-    for (int f = f0; f < f1; ++f) {
-        if (cnt[f] <= 0) continue;
-        const long long n = fits[f].n, r = row_len(f);
-        const int nr = (int)((n + r - 1) / r);
-        for (int c0 = first[f]; c0 < first[f] + cnt[f]; c0 += tile) {
-            const int nc = min(tile, first[f] + cnt[f] - c0);
+    const long long total = ss[1023];
+    if (t == 0) nitems[0] = (int)min(total, (long long)cap);
+    // Items are numbered fit by fit, tile by tile, row range by row range.  The whole block writes
+    // them: item i belongs to the first thread range whose prefix exceeds i, and to the fit of that
+    // range whose items cover i; the first row range of each tile also maps the tile's candidates.
+    for (long long i = t; i < total; i += 1024) {
+        int lo = 0, hi = 1023;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (ss[mid] > i) hi = mid;
+            else lo = mid + 1;
+        }
+        long long at = lo ? ss[lo - 1] : 0, r = 1;
+        int f = min(F, lo * chunk), nr = 1;
+        const int fe = min(F, f + chunk);
+        for (; f < fe; ++f) {
+            if (cnt[f] <= 0) continue;
+            const long long n = fits[f].n;
+            r = row_len(f);
+            nr = (int)((n + r - 1) / r);
+            const long long k = (long long)((cnt[f] + tile - 1) / tile) * nr;
+            if (i < at + k) break;
+            at += k;
+        }
+        if (f >= fe) continue;  // (unreachable: the range's counts cover i)
+        const long long local = i - at;
+        const int tl = (int)(local / nr), q = (int)(local - (long long)tl * nr);
+        const int c0 = first[f] + tl * tile, nc = min(tile, first[f] + cnt[f] - c0);
+        if (i < cap) {
+            Item it;
+            it.fit = f;
+            it.c0 = c0;
+            it.nc = nc;
+            it.pad = first[f];
+            it.r0 = (long long)q * r;
+            it.r1 = min((long long)fits[f].n, (long long)(q + 1) * r);
+            items[i] = it;
+        }
+        if (q == 0)
             for (int c = c0; c < c0 + nc; ++c) {
-                cmap[3 * (size_t)c] = (int)min(at, (long long)cap);
-                cmap[3 * (size_t)c + 1] = at + nr <= cap ? nr : 0;
+                cmap[3 * (size_t)c] = (int)min(i, (long long)cap);
+                cmap[3 * (size_t)c + 1] = i + nr <= cap ? nr : 0;
                 cmap[3 * (size_t)c + 2] = c - c0;
             }
-            for (int q = 0; q < nr; ++q, ++at) {
-                if (at >= cap) continue;
-                Item it;
-                it.fit = f;
-                it.c0 = c0;
-                it.nc = nc;
-                it.pad = first[f];
-                it.r0 = (long long)q * r;
-                it.r1 = min(n, (long long)(q + 1) * r);
-                items[at] = it;
-            }
-        }
     }
 }
 
