@@ -46,8 +46,6 @@ FP32_KERNELS = ("mixed_update",)
 # one MGS kernel per basis column, gate, store, final) and the FP64 passes (each pass kernel
 # plus its reduce kernel).  The roofline is reported for the largest single-kernel class.
 COMPOSITE = ("mixed_recheck", "dgks_retry", "append", "tgemm_f64", "project_f64", "finalize", "plain_graph")
-# Classes bound by HBM traffic by design (arithmetic intensity below the MFMA ridge point).
-HBM_BOUND = ("mixed_update", "append", "finalize", "sort", "argmax", "mixed_score")
 CPU_SAMPLE_S = 20.0
 
 
@@ -158,21 +156,35 @@ def run_form(X, y, fit_kw, precision, steps, warmup):
                 terms=len(models[0]))
 
 
+def binding(name, flops, nbytes):
+    """The roof that bounds a kernel class at its own arithmetic intensity, as the roofline model's
+    attainable rate min(peak flops, intensity x peak HBM bandwidth) picks it: the matrix peak of its
+    arithmetic (FP32 for the cache update, FP64 otherwise) when its algorithmic flops per byte reach
+    that peak's ridge point, HBM below it; a class that counts only one of the two is bound by it."""
+    peak = PEAK_FP32 if name in FP32_KERNELS else PEAK_FP64
+    if flops > 0 and (nbytes <= 0 or flops / nbytes >= peak / PEAK_HBM):
+        return "mfma", peak
+    if nbytes > 0:
+        return "hbm", PEAK_HBM
+    return None, None
+
+
 def roofline(ks):
     single = [(k, v) for k, v in ks.items() if k not in COMPOSITE and not k.startswith("host_")]
     if not single or max(v["ms"] for _, v in single) <= 0:  # a form without single-kernel classes
         single = [(k, v) for k, v in ks.items() if not k.startswith("host_")]
     name, d = max(single, key=lambda kv: kv[1]["ms"])
     avg_s = d["ms"] / 1e3 / d["launches"]
-    if d["flops"] > 0 and name not in HBM_BOUND:
-        peak = PEAK_FP32 if name in FP32_KERNELS else PEAK_FP64
+    bound, peak = binding(name, d["flops"], d["bytes"])
+    common = dict(traffic=None, avg_launch_ms=d["ms"] / d["launches"], launches=d["launches"],
+                  intensity_flop_per_byte=d["flops"] / d["bytes"] if d["bytes"] > 0 else None)
+    if bound == "mfma":
         ach = d["flops"] / d["launches"] / avg_s
-        return name, dict(bound="mfma", achieved=ach / 1e12, peak=peak / 1e12, unit="TFLOP/s",
-                          frac=ach / peak, traffic=None, avg_launch_ms=d["ms"] / d["launches"],
-                          launches=d["launches"])
+        return name, dict(bound="mfma", achieved=ach / 1e12, peak=peak / 1e12, unit="TFLOP/s", frac=ach / peak,
+                          **common)
     ach = d["bytes"] / d["launches"] / avg_s
     return name, dict(bound="hbm", achieved=ach / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s", frac=ach / PEAK_HBM,
-                      traffic=None, avg_launch_ms=d["ms"] / d["launches"], launches=d["launches"])
+                      **common)
 
 
 def cpu_threads():
@@ -209,10 +221,10 @@ def class_rooflines(ks):
             continue
         t = d["ms"] / 1e3
         e = dict(ms=d["ms"], launches=d["launches"], composite=name in COMPOSITE)
-        if d["flops"] > 0 and name not in HBM_BOUND:
-            peak = PEAK_FP32 if name in FP32_KERNELS else PEAK_FP64
+        bound, peak = binding(name, d["flops"], d["bytes"])
+        if bound == "mfma":
             e.update(bound="mfma", achieved_tflops=d["flops"] / t / 1e12, frac=d["flops"] / t / peak)
-        elif d["bytes"] > 0:
+        elif bound == "hbm":
             e.update(bound="hbm", achieved_gbs=d["bytes"] / t / 1e9, frac=d["bytes"] / t / PEAK_HBM)
         out[name] = e
     return out
@@ -414,7 +426,9 @@ def main():
         "dgks_fired": head["dgks"],
         "same_term_sequence": (len({json.dumps(r["seq"]) for r in res.values()}) == 1) if len(res) > 1 else None,
         "roofline_kernel": kname,
-        "roofline_rule": "largest kernel class whose timer brackets exactly one kernel launch",
+        "roofline_rule": ("largest kernel class whose timer brackets exactly one kernel launch; bound: the "
+                          "roof its algorithmic flops per byte reach (MFMA peak at or above the ridge point, "
+                          "else HBM)"),
         "roofline": roof,
         "kernels": {k: dict(v, avg_ms=v["ms"] / max(v["launches"], 1)) for k, v in head["kstats"].items()},
         "class_rooflines": {f: class_rooflines(r["kstats"]) for f, r in res.items()},

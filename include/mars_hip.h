@@ -293,7 +293,12 @@ int mars_hip_set_tuning(mars_hip_ctx *ctx, const mars_hip_tuning *t);
 
 /* Per-kernel-class timing with HIP events on the context's stream (off by
  * default).  Each record: launches, summed device milliseconds and the
- * summed algorithmic flops / HBM bytes of those launches. */
+ * summed algorithmic flops / HBM bytes of those launches.  The mixed form's
+ * exact rescoring with device-built lists ("mixed_recheck") reports the work
+ * counted on the device since the context was created: 4 n m flops per
+ * rescored candidate and per DGKS retry (one T-GEMM and one projection pass
+ * each); bytes: each candidate's own columns over its passes and its fit's Bo
+ * columns once per pass. */
 typedef struct mars_hip_kstat {
     char name[24];
     long launches;

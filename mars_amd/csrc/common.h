@@ -301,7 +301,9 @@ struct mars_hip_ctx {
     mh::DBuf<int> cgate, cdone;      // mixed form, per candidate: bulk gate tripped; rescored exactly
     mh::DBuf<double> clo, chi;       // mixed form, per candidate: interval of the plain form's score
     mh::DBuf<unsigned long long> dstat;  // device counters (kDStat): [1] rescored, [2] gated,
-                                         // [3] window violations, [4] DGKS retries of the rescoring
+                                         // [3] window violations, [4] DGKS retries of the rescoring,
+                                         // [5], [6] algorithmic flops and HBM bytes of the device-driven
+                                         // exact rescoring
     long redo_steps = 0;             // batch steps scored again on the host-driven path
     long hinge_rounds = 0;           // precision-form hinge evals that needed a further rescoring round
     long window_misses = 0;          // ... candidates still approximate after the last round

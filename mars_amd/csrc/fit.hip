@@ -1942,6 +1942,8 @@ int mars_hip_kernel_stats(mars_hip_ctx *ctx, mars_hip_kstat *out, int capacity, 
         require(ctx != nullptr && count != nullptr, "null argument");
         ctx_sync(ctx);
         resolve_timing(ctx);
+        unsigned long long d[kDStat];
+        read_dstat(ctx, d);
         int k = 0;
         for (int c = 0; c < K_NCLASS; ++c) {
             if (ctx->kacc[c].launches == 0) continue;
@@ -1953,6 +1955,11 @@ int mars_hip_kernel_stats(mars_hip_ctx *ctx, mars_hip_kstat *out, int capacity, 
                 r.ms = ctx->kacc[c].ms;
                 r.flops = ctx->kacc[c].flops;
                 r.bytes = ctx->kacc[c].bytes;
+                // the device-driven exact rescoring: its list lengths, and so its flops, are the device's
+                if (c == K_RECHECK) {
+                    r.flops += (double)d[5];
+                    r.bytes += (double)d[6];
+                }
             }
             ++k;
         }

@@ -864,6 +864,43 @@ __global__ __launch_bounds__(1024) void k_pass_build(const FitDev *__restrict__ 
     }
 }
 
+/* Algorithmic work of one exact-scoring stage over a device list, added to stat[0] (flops) and
+ * stat[1] (HBM bytes): 4 n m flops per entry (one T-GEMM and one projection pass,
+ * the orthonormalization's 4 n m per candidate), `row_bytes` n bytes per entry (its own columns
+ * over the stage's passes), and 8 n m bytes per fit with entries (its Bo columns, once per pass).
+ * Entry i's fit is cfit[list ? list[i] : i]; fcnt (F entries) counts each fit's entries.
+ * Integer sums: the totals do not depend on the order of the atomic adds. */
+__global__ __launch_bounds__(256) void k_work_count(const FitDev *__restrict__ fits, const int *__restrict__ cfit,
+                                                    const int *__restrict__ list, const int *__restrict__ dcount,
+                                                    const int *__restrict__ fcnt, int F, int row_bytes,
+                                                    unsigned long long *__restrict__ stat)
+{
+    __shared__ unsigned long long sh[2][256];
+    const int n = *dcount, stride = gridDim.x * blockDim.x;
+    unsigned long long fl = 0, by = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const FitDev &Fv = fits[cfit[list ? list[i] : i]];
+        fl += 4ull * (unsigned long long)Fv.n * (unsigned long long)Fv.m;
+        by += (unsigned long long)row_bytes * (unsigned long long)Fv.n;
+    }
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < F; f += stride)
+        if (fcnt[f] > 0) by += 8ull * (unsigned long long)fits[f].n * (unsigned long long)fits[f].m;
+    sh[0][threadIdx.x] = fl;
+    sh[1][threadIdx.x] = by;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + o];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (sh[0][0]) atomicAdd(stat, sh[0][0]);
+        if (sh[1][0]) atomicAdd(stat + 1, sh[1][0]);
+    }
+}
+
 /* lin[list[i]] = val[i], done[list[i]] = 1 for i < *dcount. */
 __global__ void k_put_dev(const int *__restrict__ list, const int *__restrict__ dcount,
                           const double *__restrict__ val, double *__restrict__ lin, int *__restrict__ done)
@@ -1060,6 +1097,14 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
     }
     hipLaunchKernelGGL(k_list_ranges, gF, b256, 0, st, (const int *)sc.d_fit.p, nullptr, cnt_dev, F, rfirst, rcnt);
     MH_LAUNCH_CHECK();
+    if (!ident) {
+        // the rescoring's algorithmic work, counted on the device (its list is the device's): per entry
+        // B and X columns in the T-GEMM and the projection (16 n bytes) and its stored residual
+        // written and read back by the final pass (8 n bytes)
+        hipLaunchKernelGGL(k_work_count, gB, b256, 0, st, fits, (const int *)sc.d_fit.p, nullptr, cnt_dev,
+                           (const int *)rcnt, F, 24, ctx->dstat.p + 5);
+        MH_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, used, 1);
     MH_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_alloc_kept, gB, b256, 0, st, fits, (const int *)sc.d_fit.p, cnt_dev, keep, ctx->roff.p,
@@ -1123,6 +1168,12 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
         MH_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_count_add, dim3(1), dim3(64), 0, st, (const int *)dnf, ctx->dstat.p + 4);
         MH_LAUNCH_CHECK();
+        if (!ident) {  // the retry of each fired entry: its first residual read twice, the second written
+            hipLaunchKernelGGL(k_work_count, gB, b256, 0, st, fits, (const int *)sc.d_fit.p,
+                               (const int *)ctx->fired_list.p, (const int *)dnf, (const int *)fcnt, F, 12,
+                               ctx->dstat.p + 5);
+            MH_LAUNCH_CHECK();
+        }
         hipLaunchKernelGGL(k_alloc_fired, dim3(4), b256, 0, st, fits, (const int *)sc.d_fit.p,
                            (const int *)ctx->fired_list.p, (const int *)dnf, ctx->roff.p, used,
                            (unsigned long long)arena, redo);
