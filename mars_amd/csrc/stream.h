@@ -496,6 +496,9 @@ __global__ __launch_bounds__(64, pr_waves(MT, CG)) void k_project_w(const FitDev
         // step s: the 16-row block at r0 + 16 s (item ranges start at multiples of 32, and the rows
         // of a block past the item's end lie past the fit's end, in the zero padding)
         const int nsteps = 2 * (int)((I.r1 - I.r0 + kWR - 1) / kWR);
+        // k steps of 4 basis columns that hold live columns: the tile's padding past m is neither
+        // loaded nor multiplied (uniform over the item)
+        const int kq = (F.m + 3) >> 2;
         float avh[PD][KQ];
         BxRaw bw[PD][CG];
         float4 yh[PD];
@@ -504,7 +507,8 @@ __global__ __launch_bounds__(64, pr_waves(MT, CG)) void k_project_w(const FitDev
             const long long r16 = I.r0 + 16LL * st;
             const long long row = r16 + 4 * kr;
 #pragma unroll
-            for (int q = 0; q < KQ; ++q) avh[j][q] = gld(bo[q] + r16 + prow);
+            for (int q = 0; q < KQ; ++q)
+                if (q < kq) avh[j][q] = gld(bo[q] + r16 + prow);
 #pragma unroll
             for (int g = 0; g < CG; ++g) bw[j][g] = bx_load<RB>(F, c[g], row, I.r1);
             yh[j] = float4{0.f, 0.f, 0.f, 0.f};
@@ -515,7 +519,7 @@ __global__ __launch_bounds__(64, pr_waves(MT, CG)) void k_project_w(const FitDev
             const long long row = I.r0 + 16LL * st + 4 * kr;
             float av[KQ];
 #pragma unroll
-            for (int q = 0; q < KQ; ++q) av[q] = 4 * q + kr >= F.m ? 0.0f : avh[j][q];
+            for (int q = 0; q < KQ; ++q) av[q] = q >= kq || 4 * q + kr >= F.m ? 0.0f : avh[j][q];
             const float yy[4] = {yh[j].x, yh[j].y, yh[j].z, yh[j].w};
 #pragma unroll
             for (int g = 0; g < CG; ++g) {
@@ -524,11 +528,13 @@ __global__ __launch_bounds__(64, pr_waves(MT, CG)) void k_project_w(const FitDev
                 if (F32) {  // Bo T in FP32 (the approximate first pass of the precision form)
                     v4f d32 = v4f{0, 0, 0, 0};
 #pragma unroll
-                    for (int q = 0; q < KQ; ++q) d32 = mfma_f32(av[q], (float)sT[16 * g + kc][4 * q + kr], d32);
+                    for (int q = 0; q < KQ; ++q)
+                        if (q < kq) d32 = mfma_f32(av[q], (float)sT[16 * g + kc][4 * q + kr], d32);
                     d = v4d{(double)d32[0], (double)d32[1], (double)d32[2], (double)d32[3]};
                 } else {
 #pragma unroll
-                    for (int q = 0; q < KQ; ++q) d = mfma_f64((double)av[q], sT[16 * g + kc][4 * q + kr], d);
+                    for (int q = 0; q < KQ; ++q)
+                        if (q < kq) d = mfma_f64((double)av[q], sT[16 * g + kc][4 * q + kr], d);
                 }
                 if (!c[g].ok) continue;
                 const float4 bx = bx_make<RB>(bw[j][g], c[g], row, I.r1);
