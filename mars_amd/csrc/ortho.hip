@@ -767,20 +767,32 @@ __global__ __launch_bounds__(256) void k_compact_write(const int *__restrict__ d
     }
 }
 
+/* One work list for k_pass_build. */
+struct PassSpec {
+    const int *first, *cnt;  // each fit's positions [first, first + cnt) in the device list
+    Item *items;
+    int *cmap, *nitems;
+    double target;           // work items the pass aims at
+    long long minrows, maxsplits, align;
+    int tile, cap;
+};
+
 /*
- * Work list of one pass over a device list, built on the device the way make_pass and
- * split_rows build it on the host: each fit's positions [first, first + cnt) cut into tiles of
+ * Work lists of passes over a device list, built on the device the way make_pass and
+ * split_rows build them on the host: each fit's positions [first, first + cnt) cut into tiles of
  * `tile`, each tile into row ranges of about n * target / work rows (at least minrows, at most
  * maxsplits ranges, multiples of align); cmap gets (first item, item count, slot) per position.
- * One block of 1024 threads: the fits are split into contiguous ranges per thread for the item
- * counts, and the items themselves are written by the whole block in index order.
+ * One block of 1024 threads per list (blockIdx.x selects s0, s1 or s2): the fits are split into
+ * contiguous ranges per thread for the item counts, and the items themselves are written by the
+ * whole block in index order.
  */
-__global__ __launch_bounds__(1024) void k_pass_build(const FitDev *__restrict__ fits, int F,
-                                                     const int *__restrict__ first, const int *__restrict__ cnt,
-                                                     int tile, double target, long long minrows,
-                                                     long long maxsplits, long long align, Item *__restrict__ items,
-                                                     int *__restrict__ cmap, int cap, int *__restrict__ nitems)
+__global__ __launch_bounds__(1024) void k_pass_build(const FitDev *__restrict__ fits, int F, PassSpec s0,
+                                                     PassSpec s1, PassSpec s2)
 {
+    const PassSpec S = blockIdx.x == 0 ? s0 : (blockIdx.x == 1 ? s1 : s2);
+    const int *__restrict__ first = S.first;
+    const int *__restrict__ cnt = S.cnt;
+    const int tile = S.tile, cap = S.cap;
     __shared__ double sw[1024];
     __shared__ long long ss[1024];  // inclusive prefix of the item counts of the threads' fit ranges
     const int t = threadIdx.x;
@@ -796,12 +808,12 @@ __global__ __launch_bounds__(1024) void k_pass_build(const FitDev *__restrict__ 
     const double work = sw[0];
     auto row_len = [&](int f) {
         const long long n = fits[f].n;
-        long long splits = work > 0 ? (long long)(target * (double)n / work) : 1;
+        long long splits = work > 0 ? (long long)(S.target * (double)n / work) : 1;
         splits = max(splits, 1ll);
-        splits = min(splits, max(1ll, n / minrows));
-        splits = min(splits, maxsplits);
+        splits = min(splits, max(1ll, n / S.minrows));
+        splits = min(splits, S.maxsplits);
         const long long rows = (n + splits - 1) / splits;
-        return max(align, (rows + align - 1) / align * align);
+        return max(S.align, (rows + S.align - 1) / S.align * S.align);
     };
     long long mine = 0;
     for (int f = f0; f < f1; ++f) {
@@ -818,10 +830,10 @@ __global__ __launch_bounds__(1024) void k_pass_build(const FitDev *__restrict__ 
         __syncthreads();
     }
     const long long total = ss[1023];
-    if (t == 0) nitems[0] = (int)min(total, (long long)cap);
-    // Items are numbered fit by fit, tile by tile, row range by row range.  The whole block writes
-    // them: item i belongs to the first thread range whose prefix exceeds i, and to the fit of that
-    // range whose items cover i; the first row range of each tile also maps the tile's candidates.
+    if (t == 0) S.nitems[0] = (int)min(total, (long long)cap);
+    // Items are numbered fit by fit, tile by tile, row range by row range.  Item i belongs to the
+    // first thread range whose prefix exceeds i, and to the fit of that range whose items cover i;
+    // the first row range of each tile also maps the tile's candidates.
     for (long long i = t; i < total; i += 1024) {
         int lo = 0, hi = 1023;
         while (lo < hi) {
@@ -829,12 +841,12 @@ __global__ __launch_bounds__(1024) void k_pass_build(const FitDev *__restrict__ 
             if (ss[mid] > i) hi = mid;
             else lo = mid + 1;
         }
-        long long at = lo ? ss[lo - 1] : 0, r = 1;
+        long long at = lo ? ss[lo - 1] : 0, r = 1, n = 0;
         int f = min(F, lo * chunk), nr = 1;
         const int fe = min(F, f + chunk);
         for (; f < fe; ++f) {
             if (cnt[f] <= 0) continue;
-            const long long n = fits[f].n;
+            n = fits[f].n;
             r = row_len(f);
             nr = (int)((n + r - 1) / r);
             const long long k = (long long)((cnt[f] + tile - 1) / tile) * nr;
@@ -842,24 +854,25 @@ __global__ __launch_bounds__(1024) void k_pass_build(const FitDev *__restrict__ 
             at += k;
         }
         if (f >= fe) continue;  // (unreachable: the range's counts cover i)
+        const int fst = first[f], fc = cnt[f];
         const long long local = i - at;
         const int tl = (int)(local / nr), q = (int)(local - (long long)tl * nr);
-        const int c0 = first[f] + tl * tile, nc = min(tile, first[f] + cnt[f] - c0);
+        const int c0 = fst + tl * tile, nc = min(tile, fst + fc - c0);
         if (i < cap) {
             Item it;
             it.fit = f;
             it.c0 = c0;
             it.nc = nc;
-            it.pad = first[f];
+            it.pad = fst;
             it.r0 = (long long)q * r;
-            it.r1 = min((long long)fits[f].n, (long long)(q + 1) * r);
-            items[i] = it;
+            it.r1 = min(n, (long long)(q + 1) * r);
+            S.items[i] = it;
         }
         if (q == 0)
             for (int c = c0; c < c0 + nc; ++c) {
-                cmap[3 * (size_t)c] = (int)min(i, (long long)cap);
-                cmap[3 * (size_t)c + 1] = i + nr <= cap ? nr : 0;
-                cmap[3 * (size_t)c + 2] = c - c0;
+                S.cmap[3 * (size_t)c] = (int)min(i, (long long)cap);
+                S.cmap[3 * (size_t)c + 1] = i + nr <= cap ? nr : 0;
+                S.cmap[3 * (size_t)c + 2] = c - c0;
             }
     }
 }
@@ -1061,10 +1074,27 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
     const dim3 gB((unsigned)std::min<size_t>((B + 255) / 256, kRedGrid)), b256(256);
     const dim3 gF((unsigned)((F + 255) / 256));
     const unsigned nblk = (unsigned)((B + kCompactTile - 1) / kCompactTile);
-    auto build = [&](const DevPass &p, const int *first, const int *cnt, int tile, double target, long long minrows,
-                     long long align) {
-        hipLaunchKernelGGL(k_pass_build, dim3(1), dim3(1024), 0, st, fits, F, first, cnt, tile, target, minrows,
-                           (long long)kMaxSplits, align, p.items, p.cmap, p.cap, p.n);
+    auto spec = [&](const DevPass &p, const int *first, const int *cnt, int tile, double target, long long minrows,
+                    long long align) {
+        PassSpec q;
+        q.first = first;
+        q.cnt = cnt;
+        q.items = p.items;
+        q.cmap = p.cmap;
+        q.nitems = p.n;
+        q.target = target;
+        q.minrows = minrows;
+        q.maxsplits = (long long)kMaxSplits;
+        q.align = align;
+        q.tile = tile;
+        q.cap = p.cap;
+        return q;
+    };
+    // independent lists are built by one launch, a block each
+    auto build = [&](const std::vector<PassSpec> &v) {
+        require(!v.empty() && v.size() <= 3, "internal: 1 to 3 work lists per build");
+        const PassSpec &a = v[0], &b = v.size() > 1 ? v[1] : v[0], &c = v.size() > 2 ? v[2] : v[0];
+        hipLaunchKernelGGL(k_pass_build, dim3((unsigned)v.size()), dim3(1024), 0, st, fits, F, a, b, c);
         MH_LAUNCH_CHECK();
     };
     Uploaded uT{pT.items, pT.cmap}, uP{pP.items, pP.cmap}, vT{rT.items, rT.cmap}, vP{rP.items, rP.cmap},
@@ -1110,12 +1140,15 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
     hipLaunchKernelGGL(k_alloc_kept, gB, b256, 0, st, fits, (const int *)sc.d_fit.p, cnt_dev, keep, ctx->roff.p,
                        used, (unsigned long long)(arena / 2));
     MH_LAUNCH_CHECK();
-    build(pT, rfirst, rcnt, tileT, ctx->tune.pass_items, 256, kWR);
-    if (tileP != tileT) build(pP, rfirst, rcnt, tileP, ctx->tune.pass_items, 256, kWR);
+    {
+        std::vector<PassSpec> v{spec(pT, rfirst, rcnt, tileT, ctx->tune.pass_items, 256, kWR)};
+        if (tileP != tileT) v.push_back(spec(pP, rfirst, rcnt, tileP, ctx->tune.pass_items, 256, kWR));
+        if (!none_stored) v.push_back(spec(pF, rfirst, rcnt, 16, 2048, 512, 32));
+        build(v);
+    }
     const DevPass &pPr = tileP != tileT ? pP : pT;
     const Uploaded &uPr = tileP != tileT ? uP : uT;
     const int gPr = tileP != tileT ? gP : gT;
-    if (!none_stored) build(pF, rfirst, rcnt, 16, 2048, 512, 32);
 
     // T = Bo^T Bx, s = sum r^2 (residuals stored for the entries kept), DGKS flags
     {
@@ -1178,9 +1211,8 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
                            (const int *)ctx->fired_list.p, (const int *)dnf, ctx->roff.p, used,
                            (unsigned long long)arena, redo);
         MH_LAUNCH_CHECK();
-        build(rT, ffirst, fcnt, 16, ctx->tune.retry_items, 256, kWR);
-        build(rP, ffirst, fcnt, 16, ctx->tune.retry_items, 256, kWR);
-        build(fF, ffirst, fcnt, 16, 2048, 512, 32);
+        build({spec(rT, ffirst, fcnt, 16, ctx->tune.retry_items, 256, kWR),
+               spec(rP, ffirst, fcnt, 16, ctx->tune.retry_items, 256, kWR), spec(fF, ffirst, fcnt, 16, 2048, 512, 32)});
         const int *fl = ctx->fired_list.p;
         // the first residual of every fired entry (recomputed where it was stored already)
         launch_project<kStoreR, false, true>(ctx, mt, vP, gR, sc, fl, ctx->T.p, ldT, true, nullptr, nRP, fits);
@@ -1216,7 +1248,7 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
         MH_LAUNCH_CHECK();
         // the live entries of a full scoring are most of the list: they take the wide tiles
         const int tileL = none_stored ? tileP : 16;
-        build(pL, lfirst, lcnt, tileL, ctx->tune.pass_items, 256, kWR);
+        build({spec(pL, lfirst, lcnt, tileL, ctx->tune.pass_items, 256, kWR)});
         launch_project<kFinalY, false, true>(ctx, mt, uL, gL, sc, ctx->live_list.p, ctx->T.p, ldT, tileL == 16,
                                              nullptr, nL, fits);
         hipLaunchKernelGGL(k_yreduce, gB, b256, 0, st, bound, (const int *)pL.cmap, (const double *)ctx->spart.p,

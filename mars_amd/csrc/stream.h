@@ -496,9 +496,9 @@ __global__ __launch_bounds__(64, pr_waves(MT, CG)) void k_project_w(const FitDev
         // step s: the 16-row block at r0 + 16 s (item ranges start at multiples of 32, and the rows
         // of a block past the item's end lie past the fit's end, in the zero padding)
         const int nsteps = 2 * (int)((I.r1 - I.r0 + kWR - 1) / kWR);
-        // k steps of 4 basis columns that hold live columns: the tile's padding past m is neither
-        // loaded nor multiplied (uniform over the item)
-        const int kq = (F.m + 3) >> 2;
+        // k steps of 4 basis columns that hold live columns: the MFMAs of the tile's padding past m
+        // are skipped (uniform over the item; its loads read column 0, a cache hit)
+        const int kq = MT <= 8 ? (F.m + 3) >> 2 : KQ;  // (wide tiles: padding at most 15 of 16 MT columns)
         float avh[PD][KQ];
         BxRaw bw[PD][CG];
         float4 yh[PD];
@@ -507,8 +507,7 @@ __global__ __launch_bounds__(64, pr_waves(MT, CG)) void k_project_w(const FitDev
             const long long r16 = I.r0 + 16LL * st;
             const long long row = r16 + 4 * kr;
 #pragma unroll
-            for (int q = 0; q < KQ; ++q)
-                if (q < kq) avh[j][q] = gld(bo[q] + r16 + prow);
+            for (int q = 0; q < KQ; ++q) avh[j][q] = gld(bo[q] + r16 + prow);
 #pragma unroll
             for (int g = 0; g < CG; ++g) bw[j][g] = bx_load<RB>(F, c[g], row, I.r1);
             yh[j] = float4{0.f, 0.f, 0.f, 0.f};
