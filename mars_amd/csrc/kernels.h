@@ -72,6 +72,31 @@ __device__ inline double ordered_sum(const double *p, int cnt, Idx idx)
     return s;
 }
 
+/* Sum of p[idx(i)] for i in [lo, hi), left to right, eight loads in flight. */
+template <typename Idx>
+__device__ inline double range_sum(const double *p, int lo, int hi, Idx idx)
+{
+    double s = 0.0;
+    int i = lo;
+    for (; i + 8 <= hi; i += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = gld(p + idx(i + u));
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; i < hi; ++i) s += gld(p + idx(i));
+    return s;
+}
+
+/* Fixed-order butterfly sum over the aligned group of L lanes of a wave (L a power of two, at
+ * most 64); every lane of the group gets the result. */
+__device__ inline double group_sum(double v, int L)
+{
+    for (int o = L >> 1; o > 0; o >>= 1) v = v + __shfl_xor(v, o, kWave);
+    return v;
+}
+
 /* Fixed-order block sum of one double per thread (blockDim.x == 256). */
 __device__ inline double block_sum_256(double v, double *sh)
 {

@@ -242,33 +242,50 @@ __global__ __launch_bounds__(64 * W, W >= 4 ? 3 : 1) void k_tc_update(const FitD
     }
 }
 
-/* Sum each (fit, A tile, X-column block) group's row partials in row order and write the new cache entries. */
+/* Sum each (fit, A tile, X-column block) group's row partials and write the new cache entries.
+ * Block (group, 64-entry slice): lane l takes entry 64 y + l of the group (X column e / 16, A
+ * column e % 16), wave w the w-th of four equal segments of the group's row partials (each summed
+ * in row order), and the segment sums are added in wave order; slice 0 of the u == 0 groups also
+ * sums |Bx|^2 and Bx.y of the group's X columns the same way. */
 __global__ __launch_bounds__(256) void k_tc_reduce(const FitDev *__restrict__ fits, const int *__restrict__ groups,
                                                    const int *__restrict__ glist, const double *__restrict__ tpart,
                                                    const double *__restrict__ npart)
 {
+    __shared__ double sh[4][64][3];
     const int *G = groups + (size_t)blockIdx.x * kGroupInts;
     const int fit = G[0], xc0 = G[1], nc = G[2], j = G[3], u = G[4], first = G[5], count = G[6];
     const FitDev F = fits[fit];
     const int na = 2 * j + 1;
     const long long ldb = F.ldb;
-    for (int e = threadIdx.x; e < nc * 16; e += 256) {
-        const int xl = e >> 4, al = e & 15, a = 16 * u + al;
-        if (a >= na) continue;
-        const double s = ordered_sum(
-            tpart, count, [&](int i) { return ((size_t)glist[first + i] * kXT + xl) * 16 + al; });
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int seg = (count + 3) / 4, lo = min(count, w * seg), hi = min(count, lo + seg);
+    const int e = (int)blockIdx.y * 64 + lane, xl = e >> 4, al = e & 15, a = 16 * u + al;
+    const bool ok = xl < nc && a < na;
+    const bool nok = u == 0 && blockIdx.y == 0 && lane < nc;
+    double s = 0.0, s0 = 0.0, s1 = 0.0;
+    if (ok) s = range_sum(tpart, lo, hi, [&](int i) { return ((size_t)glist[first + i] * kXT + xl) * 16 + al; });
+    if (nok) {
+        s0 = range_sum(npart, lo, hi, [&](int i) { return ((size_t)glist[first + i] * kXT + lane) * 2; });
+        s1 = range_sum(npart, lo, hi, [&](int i) { return ((size_t)glist[first + i] * kXT + lane) * 2 + 1; });
+    }
+    sh[w][lane][0] = s;
+    sh[w][lane][1] = s0;
+    sh[w][lane][2] = s1;
+    __syncthreads();
+    if (w != 0) return;
+    for (int v = 1; v < 4; ++v) {
+        s += sh[v][lane][0];
+        s0 += sh[v][lane][1];
+        s1 += sh[v][lane][2];
+    }
+    if (ok) {
         const long long xc = xc0 + xl;
         const int k = a <= j ? a : j, par = a <= j ? j : a - j - 1;
         F.tc[(xc * ldb + par) * ldb + k] = s;
     }
-    if (u != 0) return;
-    for (int xl = threadIdx.x; xl < nc; xl += 256) {
-        const double s0 =
-            ordered_sum(npart, count, [&](int i) { return ((size_t)glist[first + i] * kXT + xl) * 2; });
-        const double s1 =
-            ordered_sum(npart, count, [&](int i) { return ((size_t)glist[first + i] * kXT + xl) * 2 + 1; });
-        F.tcn[((xc0 + xl) * ldb + j) * 2] = s0;
-        F.tcn[((xc0 + xl) * ldb + j) * 2 + 1] = s1;
+    if (nok) {
+        F.tcn[((xc0 + lane) * ldb + j) * 2] = s0;
+        F.tcn[((xc0 + lane) * ldb + j) * 2 + 1] = s1;
     }
 }
 
@@ -645,9 +662,9 @@ void tc_update(mars_hip_ctx *ctx)
             }
             MH_LAUNCH_CHECK();
         }
-        hipLaunchKernelGGL(k_tc_reduce, dim3((unsigned)ng), dim3(256), 0, st, (const FitDev *)ctx->d_fits.p,
-                           (const int *)ctx->d_cmap.p, (const int *)ctx->d_cmap.p + nh, (const double *)ctx->tpart.p,
-                           (const double *)ctx->aux.p);
+        hipLaunchKernelGGL(k_tc_reduce, dim3((unsigned)ng, kXT * 16 / 64), dim3(256), 0, st,
+                           (const FitDev *)ctx->d_fits.p, (const int *)ctx->d_cmap.p, (const int *)ctx->d_cmap.p + nh,
+                           (const double *)ctx->tpart.p, (const double *)ctx->aux.p);
         MH_LAUNCH_CHECK();
         for (size_t f = 0; f < F; ++f)
             if (jf[f] >= 0) ctx->fits[f].mc += 1;

@@ -33,47 +33,69 @@ constexpr size_t kRetryArena = size_t(2) << 30;
 // residual bytes of the device-driven rescoring: half for stored entries, half for fired ones
 constexpr size_t kRescoreArena = size_t(2) << 30;
 
-/* T[c] = sum of the candidate's item partials in order; tnorm[c] = |T[c][:m]|^2.
- * One wave per candidate (or list position). */
-__global__ __launch_bounds__(64) void k_treduce(const FitDev *__restrict__ fits,
-                                                const int *__restrict__ cfit,
-                                                const int *__restrict__ list, int count,
-                                                const int *__restrict__ cmap,
-                                                const double *__restrict__ tpart, int ldT,
-                                                double *__restrict__ T, double *__restrict__ tnorm,
-                                                const int *__restrict__ dcount)
+/* T[c] = sum of the candidate's item partials; tnorm[c] = |T[c][:m]|^2.  One block of S waves per
+ * candidate (or list position): lane l takes basis columns l, l + 64, ..., wave w the w-th of S
+ * equal segments of the item partials (each summed left to right), and the segment sums are added
+ * in wave order: a fixed order for a given S (S = 1: the partials left to right). */
+__global__ __launch_bounds__(1024) void k_treduce(const FitDev *__restrict__ fits,
+                                                  const int *__restrict__ cfit,
+                                                  const int *__restrict__ list, int count,
+                                                  const int *__restrict__ cmap,
+                                                  const double *__restrict__ tpart, int ldT,
+                                                  double *__restrict__ T, double *__restrict__ tnorm,
+                                                  const int *__restrict__ dcount)
 {
+    __shared__ double sh[16][64];
     if (dcount) count = *dcount;  // list length built on the device
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, S = blockDim.x >> 6;
     for (int c = blockIdx.x; c < count; c += gridDim.x) {
         const int cid = list ? list[c] : c;
         const int m = fits[cfit[cid]].m;
         const int first = cmap[3 * c], cnt = cmap[3 * c + 1], slot = cmap[3 * c + 2];
+        const int seg = (cnt + S - 1) / S, lo = min(cnt, w * seg), hi = min(cnt, lo + seg);
         double tt = 0.0;
-        for (int k = lane; k < ldT; k += 64) {
-            const double s =
-                ordered_sum(tpart, cnt, [&](int i) { return ((size_t)(first + i) * 64 + slot) * ldT + k; });
-            T[(size_t)c * ldT + k] = s;
-            if (k < m) tt = fma(s, s, tt);
+        for (int k0 = 0; k0 < ldT; k0 += 64) {
+            const int k = k0 + lane;
+            double s = 0.0;
+            if (k < ldT) s = range_sum(tpart, lo, hi, [&](int i) { return ((size_t)(first + i) * 64 + slot) * ldT + k; });
+            if (S > 1) {
+                sh[w][lane] = s;
+                __syncthreads();
+                if (w == 0)
+                    for (int v = 1; v < S; ++v) s += sh[v][lane];
+                __syncthreads();
+            }
+            if (w == 0 && k < ldT) {
+                T[(size_t)c * ldT + k] = s;
+                if (k < m) tt = fma(s, s, tt);
+            }
         }
-        tt = wave_sum(tt);
-        if (lane == 0 && tnorm) tnorm[c] = tt;
+        if (w == 0) {
+            tt = wave_sum(tt);
+            if (lane == 0 && tnorm) tnorm[c] = tt;
+        }
     }
 }
 
 /* s[c] = sum of item partials; the first pass also sets flag[c]: bit 0 the DGKS gate
  * fired (kernels.cc:387-388), bit 1 s > tol (the candidate can score above 0), bit 2 its
- * residual was not stored (keep[c] == 0). */
+ * residual was not stored (keep[c] == 0).  L lanes per candidate (a power of two dividing 64):
+ * lane j of the group sums the j-th of L equal segments of the partials left to right and the
+ * group adds the segment sums by a fixed butterfly (L = 1: the partials left to right). */
 __global__ void k_sreduce(const int *__restrict__ list, int count, const int *__restrict__ cmap,
                           const double *__restrict__ spart, int tile,
                           const double *__restrict__ tnorm, const FitDev *__restrict__ fits,
                           const int *__restrict__ cfit, const int *__restrict__ keep,
-                          double *__restrict__ ssum, int *__restrict__ flag, const int *__restrict__ dcount)
+                          double *__restrict__ ssum, int *__restrict__ flag, const int *__restrict__ dcount, int L)
 {
     if (dcount) count = *dcount;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < count; c += gridDim.x * blockDim.x) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, sub = t % L, ng = gridDim.x * blockDim.x / L;
+    for (int c = t / L; c < count; c += ng) {
         const int first = cmap[3 * c], cnt = cmap[3 * c + 1], slot = cmap[3 * c + 2];
-        const double s = ordered_sum(spart, cnt, [&](int i) { return (size_t)(first + i) * tile + slot; });
+        const int seg = (cnt + L - 1) / L, lo = min(cnt, sub * seg), hi = min(cnt, lo + seg);
+        double s = range_sum(spart, lo, hi, [&](int i) { return (size_t)(first + i) * tile + slot; });
+        s = group_sum(s, L);
+        if (sub != 0) continue;
         if (list) {
             ssum[list[c]] = s;
         } else {
@@ -224,12 +246,16 @@ __global__ __launch_bounds__(256) void k_finalize_cm(const FitDev *__restrict__ 
 /* ybx = sum of item partials, lin = ybx^2, for candidate c (or list[c]). */
 __global__ void k_yreduce(int count, const int *__restrict__ cmap, const double *__restrict__ ypart,
                           const int *__restrict__ list, double *__restrict__ ybx, double *__restrict__ lin,
-                          const int *__restrict__ dcount)
+                          const int *__restrict__ dcount, int L)
 {
     if (dcount) count = *dcount;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < count; c += gridDim.x * blockDim.x) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, sub = t % L, ng = gridDim.x * blockDim.x / L;
+    for (int c = t / L; c < count; c += ng) {
         const int first = cmap[3 * c], cnt = cmap[3 * c + 1], slot = cmap[3 * c + 2];
-        const double s = ordered_sum(ypart, cnt, [&](int i) { return (size_t)(first + i) * kTileF + slot; });
+        const int seg = (cnt + L - 1) / L, lo = min(cnt, sub * seg), hi = min(cnt, lo + seg);
+        double s = range_sum(ypart, lo, hi, [&](int i) { return (size_t)(first + i) * kTileF + slot; });
+        s = group_sum(s, L);
+        if (sub != 0) continue;
         const int cand = list ? list[c] : c;
         if (ybx) ybx[cand] = s;
         lin[cand] = s * s;
@@ -404,6 +430,43 @@ void launch_project(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, co
     MH_MT_SWITCH(mt, MH_GO)
 #undef MH_GO
 #undef MH_GO1
+    MH_LAUNCH_CHECK();
+}
+
+
+constexpr int kRedGrid = 1024;       // blocks of the grid-stride reductions over device-sized lists
+
+/* Threads per candidate of the ordered reductions, from the list's length bound: the passes split
+ * each candidate tile's rows to fill the GPU, so few candidates have many partials each and get
+ * more threads; many candidates get one lane (L) or one wave (S) each. */
+int red_lanes(size_t bound) { return bound >= 8192 ? 1 : bound >= 1024 ? 8 : bound >= 128 ? 32 : 64; }
+int red_waves(size_t bound) { return bound >= 1024 ? 1 : bound >= 128 ? 4 : 16; }
+
+void launch_treduce(hipStream_t st, int S, size_t bound, const FitDev *fits, const int *cfit, const int *list,
+                    int count, const int *cmap, const double *tpart, int ldT, double *T, double *tnorm,
+                    const int *dcount)
+{
+    const unsigned g = (unsigned)std::max<size_t>(1, std::min<size_t>(bound, 4 * kRedGrid));
+    hipLaunchKernelGGL(k_treduce, dim3(g), dim3(64 * S), 0, st, fits, cfit, list, count, cmap, tpart, ldT, T, tnorm,
+                       dcount);
+    MH_LAUNCH_CHECK();
+}
+
+void launch_sreduce(hipStream_t st, int L, size_t bound, const int *list, int count, const int *cmap,
+                    const double *spart, int tile, const double *tnorm, const FitDev *fits, const int *cfit,
+                    const int *keep, double *ssum, int *flag, const int *dcount)
+{
+    const unsigned g = (unsigned)std::max<size_t>(1, std::min<size_t>((bound * L + 255) / 256, 4 * kRedGrid));
+    hipLaunchKernelGGL(k_sreduce, dim3(g), dim3(256), 0, st, list, count, cmap, spart, tile, tnorm, fits, cfit, keep,
+                       ssum, flag, dcount, L);
+    MH_LAUNCH_CHECK();
+}
+
+void launch_yreduce(hipStream_t st, int L, size_t bound, int count, const int *cmap, const double *ypart,
+                    const int *list, double *ybx, double *lin, const int *dcount)
+{
+    const unsigned g = (unsigned)std::max<size_t>(1, std::min<size_t>((bound * L + 255) / 256, 4 * kRedGrid));
+    hipLaunchKernelGGL(k_yreduce, dim3(g), dim3(256), 0, st, count, cmap, ypart, list, ybx, lin, dcount, L);
     MH_LAUNCH_CHECK();
 }
 
@@ -603,20 +666,16 @@ void retry_compact(mars_hip_ctx *ctx, Cands &cs, const std::vector<int> &fired, 
         const int *fl = ctx->fired_list.p;
         launch_project<kStoreR, false, true>(ctx, mt, vP, (int)rP.items.size(), cs, fl, ctx->T.p, ldT, rnarrow);
         launch_tgemm<true>(ctx, mt, vT, (int)rT.items.size(), cs, fl, ldT, rnarrow);
-        hipLaunchKernelGGL(k_treduce, dim3(nf), dim3(64), 0, st, ctx->d_fits.p, cs.d_fit.p, fl, nf, vT.cmap,
-                           ctx->tpart.p, ldT, ctx->T2.p, nullptr, nullptr);
-        MH_LAUNCH_CHECK();
+        launch_treduce(st, red_waves(nf), nf, ctx->d_fits.p, cs.d_fit.p, fl, (int)nf, vT.cmap, ctx->tpart.p, ldT,
+                       ctx->T2.p, nullptr, nullptr);
         launch_project<kStoreR, true, false>(ctx, mt, vP, (int)rP.items.size(), cs, fl, ctx->T2.p, ldT, rnarrow);
-        hipLaunchKernelGGL(k_sreduce, dim3((nf + 255) / 256), dim3(256), 0, st, fl, nf, vP.cmap, ctx->spart.p,
-                           kSlots, nullptr, ctx->d_fits.p, cs.d_fit.p, nullptr, ctx->ssum.p, nullptr, nullptr);
-        MH_LAUNCH_CHECK();
+        launch_sreduce(st, red_lanes(nf), nf, fl, (int)nf, vP.cmap, ctx->spart.p, kSlots, nullptr, ctx->d_fits.p,
+                       cs.d_fit.p, nullptr, ctx->ssum.p, nullptr, nullptr);
         hipLaunchKernelGGL(k_finalize_cm, dim3(grid_for(fF.items.size())), dim3(256), 0, st, ctx->d_fits.p,
                            wF.items, (int)fF.items.size(), fl, nullptr, ctx->ssum.p, ctx->spart.p, nullptr, nullptr,
                            0);
         MH_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_yreduce, dim3((nf + 255) / 256), dim3(256), 0, st, nf, wF.cmap, ctx->spart.p, fl,
-                           ybx, lin, nullptr);
-        MH_LAUNCH_CHECK();
+        launch_yreduce(st, red_lanes(nf), nf, (int)nf, wF.cmap, ctx->spart.p, fl, ybx, lin, nullptr);
         k0 = k1;
         if (k0 < fired.size()) ctx_sync(ctx);  // the next block reuses the arena and the work lists
     }
@@ -631,7 +690,6 @@ namespace {
 /* ---------------------------------------------------------------------- */
 
 constexpr int kCompactTile = 1024;   // list entries per compaction block (256 threads x 4)
-constexpr int kRedGrid = 1024;       // blocks of the grid-stride reductions over device-sized lists
 
 /* rfit/rxc/rpar[i] = cfit/cxc/cpar[list[i]] for i < *dcount. */
 __global__ void k_rc_gather(const int *__restrict__ list, const int *__restrict__ dcount,
@@ -1154,10 +1212,8 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
     {
         KTimer kt(ctx, K_TGEMM, flops, bytes);
         launch_tgemm<false>(ctx, mt, uT, gT, sc, nullptr, ldT, narrow, nT, fits);
-        hipLaunchKernelGGL(k_treduce, dim3((unsigned)std::min<size_t>(B, 4 * kRedGrid)), dim3(64), 0, st, fits,
-                           (const int *)sc.d_fit.p, nullptr, bound, (const int *)pT.cmap, (const double *)ctx->tpart.p,
-                           ldT, ctx->T.p, ctx->tnorm.p, cnt_dev);
-        MH_LAUNCH_CHECK();
+        launch_treduce(st, red_waves(B), B, fits, (const int *)sc.d_fit.p, nullptr, bound, (const int *)pT.cmap,
+                       (const double *)ctx->tpart.p, ldT, ctx->T.p, ctx->tnorm.p, cnt_dev);
     }
     {
         KTimer kt(ctx, K_PROJECT, flops, bytes);
@@ -1167,10 +1223,9 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
         else
             launch_project<kStoreR, false, false>(ctx, mt, uPr, gPr, sc, nullptr, ctx->T.p, ldT, narrow, keep,
                                                   pPr.n, fits);
-        hipLaunchKernelGGL(k_sreduce, gB, b256, 0, st, nullptr, bound, (const int *)pPr.cmap,
-                           (const double *)ctx->spart.p, kSlots, (const double *)ctx->tnorm.p, fits,
-                           (const int *)sc.d_fit.p, (const int *)keep, ctx->ssum.p, ctx->flag.p, cnt_dev);
-        MH_LAUNCH_CHECK();
+        launch_sreduce(st, red_lanes(B), B, nullptr, bound, (const int *)pPr.cmap, (const double *)ctx->spart.p, kSlots,
+                       (const double *)ctx->tnorm.p, fits, (const int *)sc.d_fit.p, (const int *)keep, ctx->ssum.p,
+                       ctx->flag.p, cnt_dev);
     }
     // final pass over the stored residuals (unstored entries with s > tol get 0 here)
     {
@@ -1183,9 +1238,8 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
                                (const Item *)pF.items, (int)capF, nullptr, (const int *)keep,
                                (const double *)ctx->ssum.p, ctx->spart.p, (const int *)nF, nullptr, 0);
             MH_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_yreduce, gB, b256, 0, st, bound, (const int *)pF.cmap, (const double *)ctx->spart.p,
-                               nullptr, ry, rl, cnt_dev);
-            MH_LAUNCH_CHECK();
+            launch_yreduce(st, red_lanes(B), B, bound, (const int *)pF.cmap, (const double *)ctx->spart.p, nullptr, ry,
+                           rl, cnt_dev);
         }
     }
     // DGKS retry of the fired entries (kernels.cc:387-399)
@@ -1217,21 +1271,17 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
         // the first residual of every fired entry (recomputed where it was stored already)
         launch_project<kStoreR, false, true>(ctx, mt, vP, gR, sc, fl, ctx->T.p, ldT, true, nullptr, nRP, fits);
         launch_tgemm<true>(ctx, mt, vT, gR, sc, fl, ldT, true, nRT, fits);
-        hipLaunchKernelGGL(k_treduce, dim3(256), dim3(64), 0, st, fits, (const int *)sc.d_fit.p, fl, bound,
-                           (const int *)rT.cmap, (const double *)ctx->tpart.p, ldT, ctx->T2.p, nullptr,
-                           (const int *)dnf);
-        MH_LAUNCH_CHECK();
+        // (the fired count is the device's and usually small: a few waves or lanes per entry)
+        launch_treduce(st, 4, 256, fits, (const int *)sc.d_fit.p, fl, bound, (const int *)rT.cmap,
+                       (const double *)ctx->tpart.p, ldT, ctx->T2.p, nullptr, (const int *)dnf);
         launch_project<kStoreR, true, false>(ctx, mt, vP, gR, sc, fl, ctx->T2.p, ldT, true, nullptr, nRP, fits);
-        hipLaunchKernelGGL(k_sreduce, dim3(64), b256, 0, st, fl, bound, (const int *)rP.cmap,
-                           (const double *)ctx->spart.p, kSlots, nullptr, fits, (const int *)sc.d_fit.p, nullptr,
-                           ctx->ssum.p, nullptr, (const int *)dnf);
-        MH_LAUNCH_CHECK();
+        launch_sreduce(st, 16, 1024, fl, bound, (const int *)rP.cmap, (const double *)ctx->spart.p, kSlots, nullptr,
+                       fits, (const int *)sc.d_fit.p, nullptr, ctx->ssum.p, nullptr, (const int *)dnf);
         hipLaunchKernelGGL(k_finalize_cm, dim3(256), b256, 0, st, fits, (const Item *)fF.items, (int)capF, fl, nullptr,
                            (const double *)ctx->ssum.p, ctx->spart.p, (const int *)nFF, nullptr, 0);
         MH_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_yreduce, dim3(64), b256, 0, st, bound, (const int *)fF.cmap, (const double *)ctx->spart.p,
-                           fl, ry, rl, (const int *)dnf);
-        MH_LAUNCH_CHECK();
+        launch_yreduce(st, 16, 1024, bound, (const int *)fF.cmap, (const double *)ctx->spart.p, fl, ry, rl,
+                       (const int *)dnf);
     }
     // entries with s > tol whose residual was not stored and that did not fire: recompute r
     {
@@ -1251,9 +1301,8 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
         build({spec(pL, lfirst, lcnt, tileL, ctx->tune.pass_items, 256, kWR)});
         launch_project<kFinalY, false, true>(ctx, mt, uL, gL, sc, ctx->live_list.p, ctx->T.p, ldT, tileL == 16,
                                              nullptr, nL, fits);
-        hipLaunchKernelGGL(k_yreduce, gB, b256, 0, st, bound, (const int *)pL.cmap, (const double *)ctx->spart.p,
-                           (const int *)ctx->live_list.p, ry, rl, (const int *)dnl);
-        MH_LAUNCH_CHECK();
+        launch_yreduce(st, red_lanes(B), B, bound, (const int *)pL.cmap, (const double *)ctx->spart.p,
+                       (const int *)ctx->live_list.p, ry, rl, (const int *)dnl);
     }
     if (!ident) {
         hipLaunchKernelGGL(k_put_dev, gB, b256, 0, st, list, dcount, (const double *)rl, lin, done);
@@ -1345,9 +1394,8 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
             launch_tgemm<false, true>(ctx, mt, uT, (int)pT.items.size(), cs, nullptr, ldT, narrow);
         else
             launch_tgemm<false>(ctx, mt, uT, (int)pT.items.size(), cs, nullptr, ldT, narrow);
-        hipLaunchKernelGGL(k_treduce, dim3(P), dim3(64), 0, st, ctx->d_fits.p, cs.d_fit.p, nullptr, P,
-                           uT.cmap, ctx->tpart.p, ldT, ctx->T.p, ctx->tnorm.p, nullptr);
-        MH_LAUNCH_CHECK();
+        launch_treduce(st, red_waves(P), P, ctx->d_fits.p, cs.d_fit.p, nullptr, P, uT.cmap, ctx->tpart.p, ldT, ctx->T.p,
+                       ctx->tnorm.p, nullptr);
     }
     // Pass 2: s = sum f32(Bx - Bo T)^2 and the DGKS gate; r itself is kept only for the
     // hinge sweep (which needs the normalized columns)
@@ -1362,10 +1410,8 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         else
             launch_project<kSumSq, false, false>(ctx, mt, uP, (int)pP.items.size(), cs, nullptr, ctx->T.p, ldT,
                                                  narrow);
-        hipLaunchKernelGGL(k_sreduce, dim3((P + 255) / 256), dim3(256), 0, st, nullptr, P, uP.cmap,
-                           ctx->spart.p, kSlots, ctx->tnorm.p, ctx->d_fits.p, cs.d_fit.p, keep,
-                           ctx->ssum.p, ctx->flag.p, nullptr);
-        MH_LAUNCH_CHECK();
+        launch_sreduce(st, red_lanes(P), P, nullptr, P, uP.cmap, ctx->spart.p, kSlots, ctx->tnorm.p, ctx->d_fits.p,
+                       cs.d_fit.p, keep, ctx->ssum.p, ctx->flag.p, nullptr);
     }
     // Stored column-major residuals (the mixed form's rescoring, the sorted hinge path) are
     // finalized before the host has read the DGKS flags: the device runs pass 3 over every
@@ -1395,9 +1441,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                                uF.items, nF, nullptr, keep, ctx->ssum.p, ctx->spart.p, nullptr,
                                keep_bxn ? (const int *)ctx->flag.p : nullptr, keep_bxn ? 1 : 0);
             MH_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, P, uF.cmap, ctx->spart.p,
-                               nullptr, ybx, lin, nullptr);
-            MH_LAUNCH_CHECK();
+            launch_yreduce(st, red_lanes(P), P, P, uF.cmap, ctx->spart.p, nullptr, ybx, lin, nullptr);
         }
         MH_CHECK(hipEventSynchronize(ctx->ev_rb));
         flags = ctx->rb_pin;
@@ -1450,15 +1494,12 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
             launch_project<kStoreR, false, true>(ctx, mt, vP, (int)rP.items.size(), cs, ctx->fired_list.p,
                                                  ctx->T.p, ldT, rnarrow);
         launch_tgemm<true>(ctx, mt, vT, (int)rT.items.size(), cs, ctx->fired_list.p, ldT, rnarrow);
-        hipLaunchKernelGGL(k_treduce, dim3(nf), dim3(64), 0, st, ctx->d_fits.p, cs.d_fit.p,
-                           ctx->fired_list.p, nf, vT.cmap, ctx->tpart.p, ldT, ctx->T2.p, nullptr, nullptr);
-        MH_LAUNCH_CHECK();
+        launch_treduce(st, red_waves(nf), nf, ctx->d_fits.p, cs.d_fit.p, ctx->fired_list.p, nf, vT.cmap, ctx->tpart.p,
+                       ldT, ctx->T2.p, nullptr, nullptr);
         launch_project<kStoreR, true, false>(ctx, mt, vP, (int)rP.items.size(), cs, ctx->fired_list.p, ctx->T2.p,
                                              ldT, rnarrow);
-        hipLaunchKernelGGL(k_sreduce, dim3((nf + 255) / 256), dim3(256), 0, st, ctx->fired_list.p, nf,
-                           vP.cmap, ctx->spart.p, kSlots, nullptr, ctx->d_fits.p, cs.d_fit.p, nullptr,
-                           ctx->ssum.p, nullptr, nullptr);
-        MH_LAUNCH_CHECK();
+        launch_sreduce(st, red_lanes(nf), nf, ctx->fired_list.p, nf, vP.cmap, ctx->spart.p, kSlots, nullptr,
+                       ctx->d_fits.p, cs.d_fit.p, nullptr, ctx->ssum.p, nullptr, nullptr);
         if (spec) {  // pass 3 again for the fired candidates, over their retried residuals
             Pass fF = make_pass(rb, re, n, split_rows(n, fcnt, 16, 2048, 512), 16, 32);
             Uploaded wF;
@@ -1471,9 +1512,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                                wF.items, (int)fF.items.size(), (const int *)ctx->fired_list.p, nullptr,
                                ctx->ssum.p, ctx->spart.p, nullptr, nullptr, keep_bxn ? 1 : 0);
             MH_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_yreduce, dim3((nf + 255) / 256), dim3(256), 0, st, nf, wF.cmap, ctx->spart.p,
-                               ctx->fired_list.p, ybx, lin, nullptr);
-            MH_LAUNCH_CHECK();
+            launch_yreduce(st, red_lanes(nf), nf, nf, wF.cmap, ctx->spart.p, ctx->fired_list.p, ybx, lin, nullptr);
         }
     }
     if (spec) {
@@ -1497,9 +1536,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
             ctx->spart.ensure(pL.items.size() * kSlots + 1);
             launch_project<kFinalY, false, true>(ctx, mt, uL, (int)pL.items.size(), cs, ctx->live_list.p,
                                                  ctx->T.p, ldT, lnarrow);
-            hipLaunchKernelGGL(k_yreduce, dim3((nl + 255) / 256), dim3(256), 0, st, nl, uL.cmap, ctx->spart.p,
-                               ctx->live_list.p, ybx, lin, nullptr);
-            MH_LAUNCH_CHECK();
+            launch_yreduce(st, red_lanes(nl), nl, nl, uL.cmap, ctx->spart.p, ctx->live_list.p, ybx, lin, nullptr);
         }
         return;
     }
@@ -1519,8 +1556,7 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                 hipLaunchKernelGGL((k_finalize<kTileF>), gF, dim3(256), 0, st, ctx->d_fits.p, uF.items, nF,
                                    ctx->ssum.p, ctx->spart.p);
             MH_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_yreduce, dim3((P + 255) / 256), dim3(256), 0, st, P, uF.cmap, ctx->spart.p,
-                               nullptr, ybx, lin, nullptr);
+            launch_yreduce(st, red_lanes(P), P, P, uF.cmap, ctx->spart.p, nullptr, ybx, lin, nullptr);
         } else {
             // only candidates with s > tol can score above 0; the others are 0 exactly.  The
             // fired ones are finished by their retry (retry_compact); the rest recompute r.
@@ -1555,10 +1591,8 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
                 ctx->spart.ensure(pL.items.size() * kSlots + 1);
                 launch_project<kFinalY, false, true>(ctx, mt, uL, (int)pL.items.size(), cs, ctx->live_list.p,
                                                      ctx->T.p, ldT, lnarrow);
-                hipLaunchKernelGGL(k_yreduce, dim3((nl + 255) / 256), dim3(256), 0, st, nl, uL.cmap, ctx->spart.p,
-                                   ctx->live_list.p, ybx, lin, nullptr);
+                launch_yreduce(st, red_lanes(nl), nl, nl, uL.cmap, ctx->spart.p, ctx->live_list.p, ybx, lin, nullptr);
             }
-            MH_LAUNCH_CHECK();
             if (!fired.empty()) {
                 KTimer kt2(ctx, K_DGKS, 0.0, 0.0);
                 retry_compact(ctx, cs, fired, rb, re, n, mt, ldT, ybx, lin);
