@@ -33,9 +33,11 @@ constexpr int kWR = 32;      // rows per chunk (item row ranges start at multipl
 constexpr int kSlots = 64;   // candidate slots per item in the partial-sum layouts
 
 /* Candidate groups of 16 per item: bounds registers (T-GEMM accumulators) and
- * LDS (projection T tile, at most 2048 doubles for m <= 128). */
+ * LDS (projection T tile: 32 candidates x at most 129 doubles for m <= 128; wider tiles halve the
+ * basis reads of the projection, which outweighs their lower occupancy: config 5's plain fit
+ * 32.5 -> 29.9 s with 32-candidate tiles up to 128 columns). */
 constexpr int cg_tgemm(int mt) { return mt <= 2 ? 4 : (mt <= 8 ? 2 : 1); }
-constexpr int cg_project(int mt) { return mt <= 4 ? 2 : 1; }
+constexpr int cg_project(int mt) { return mt <= 8 ? 2 : 1; }
 constexpr int waves_for(int mt) { return mt == 1 ? 2 : (mt <= 4 ? 3 : 2); }
 
 /* Work items are listed fit by fit and tile by tile in row order, so neighbouring items read
