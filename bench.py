@@ -257,8 +257,8 @@ def _epoch_sizes(trace):
 
 
 def cpu_baseline(X, y, fit_kw, budget_s, gpu_trace=None, full=False):
-    """The CPU path on the same inputs.  full: the whole fit.  Otherwise the first epochs, doubled
-    until about a third of budget_s; with the GPU fit's trace, the whole fit's CPU time is then
+    """The CPU path on the same inputs.  full: the whole fit.  Otherwise the epochs that start
+    within budget_s of the fit's start; with the GPU fit's trace, the whole fit's CPU time is then
     extrapolated: each sampled epoch's seconds are fitted as a P m + c (P candidates, m basis
     functions: the orthonormalization and the sweep both cost O(n m) per candidate; c the per-epoch
     overhead), and the model is summed over the GPU fit's epochs (the same term sequence).  On
@@ -279,16 +279,10 @@ def cpu_baseline(X, y, fit_kw, budget_s, gpu_trace=None, full=False):
         return dict(value=cands / spent, unit="candidates/s", cores=cores, kind=kind, fit_s=spent,
                     sample="the whole fit (%d epochs, %d candidates, %.1f s)" % (len(tr), cands, spent),
                     **host_info())
-    epochs, result = 2, None
-    while True:
-        t0 = time.perf_counter()
-        _, tr = F.fit(lib, X, y, w, threads=-1, max_epochs=epochs, **fit_kw)
-        spent = time.perf_counter() - t0
-        result = (tr, spent)
-        if spent > budget_s / 3 or len(tr) < epochs:
-            break
-        epochs *= 2
-    tr, spent = result
+    # one run that starts no new epoch after the budget: every epoch that fits is sampled
+    t0 = time.perf_counter()
+    _, tr = F.fit(lib, X, y, w, threads=-1, max_seconds=budget_s, **fit_kw)
+    spent = time.perf_counter() - t0
     cands = sum(int(s.get("n_cand", 0)) for s in tr)
     out = dict(value=cands / spent if spent > 0 else None, unit="candidates/s", cores=cores, kind=kind,
                sample="first %d epochs of the same fit (%d candidates, %.1f s)" % (len(tr), cands, spent),

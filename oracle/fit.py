@@ -65,6 +65,7 @@ def fit(lib, X, y, w=None, **kwargs):
     aging_factor = kwargs.pop("aging_factor", 1.0)
     threads = kwargs.pop("threads", -1)
     max_epochs = kwargs.pop("max_epochs", None)  # oracle-only: truncate long fits
+    max_seconds = kwargs.pop("max_seconds", None)  # oracle-only: no new epoch after this much time
     cuda = kwargs.pop("cuda", False)  # the binding's cuda= flag (mars.py:156-160, 330)
     if kwargs:
         raise TypeError("unknown argument: %s" % kwargs)
@@ -102,8 +103,11 @@ def fit(lib, X, y, w=None, **kwargs):
         r[np.argsort(v)] = np.arange(len(v))
         return r
 
+    t_start = time.perf_counter()
     while algo.nbasis() < max_terms:
         if max_epochs is not None and epoch >= max_epochs:
+            break
+        if max_seconds is not None and epoch > 0 and time.perf_counter() - t_start >= max_seconds:
             break
         t_epoch = time.perf_counter()
         basis_to_use = np.argsort(basis_sse)[::-1][:max_basis]
