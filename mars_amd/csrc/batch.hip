@@ -339,7 +339,7 @@ void batch_fit(mars_hip_ctx *ctx, const mars_hip_batch_opts &o, mars_hip_term *t
                         flops += 4.0 * fs.n * fs.m * cs.fit_P[f];
                         bytes += ctx->act_h[f] ? 4.0 * fs.n * (2.0 * fs.m + fs.px + 1) : 0.0;
                     }
-                    std::vector<unsigned long long> key = {2ull, (unsigned long long)std::min<size_t>(16, (mmax + 15) / 16),
+                    std::vector<unsigned long long> key = {2ull, (unsigned long long)((mmax + 15) / 16),
                                                            (unsigned long long)narrow, (unsigned long long)Pl,
                                                            (unsigned long long)F};
                     tuning_key(ctx, key);

@@ -1282,6 +1282,8 @@ void hinge_score(mars_hip_ctx *ctx, Cands &cs, double *lin, double *ybx, double 
 {
     const int P = (int)cs.fit.size();
     if (P == 0) return;
+    // the sweep's limit, checked before the scoring passes run
+    require(ctx->fits[0].m <= 253, "hinge sweep supports up to 253 basis functions");
     std::vector<int> gxc, gc0;
     for (int c = 0; c < P; ++c)
         if (c == 0 || cs.xc[c] != cs.xc[c - 1]) {

@@ -550,7 +550,7 @@ __global__ void k_put(const int *__restrict__ idx, const double *__restrict__ va
     if (i < n) lin[idx[i]] = val[i];
 }
 
-constexpr size_t kMaxMixedTerms = 256;  // cache: p_x * ldb^2 doubles per fit
+constexpr size_t kMaxMixedTerms = 512;  // cache: p_x * ldb^2 doubles per fit (the append's limit too)
 
 } // namespace
 
@@ -558,7 +558,7 @@ constexpr size_t kMaxMixedTerms = 256;  // cache: p_x * ldb^2 doubles per fit
 void ensure_tcache(mars_hip_ctx *ctx)
 {
     for (FitState &fs : ctx->fits) {
-        require(fs.m <= kMaxMixedTerms, "the mixed-precision form supports up to 256 basis functions");
+        require(fs.m <= kMaxMixedTerms, "the mixed-precision form supports up to 512 basis functions");
         if (fs.tc.p) continue;
         fs.tc.ensure(fs.px * ctx->ldb * ctx->ldb);
         fs.tcn.ensure(fs.px * ctx->ldb * 2);
@@ -770,7 +770,7 @@ void mixed_score(mars_hip_ctx *ctx, bool host_lists, const int *dP, int Pb)
         // one graph per basis-size class: every argument below is fixed for the fit
         size_t mmax = 1;
         for (const FitState &fs : ctx->fits) mmax = std::max(mmax, fs.m);
-        std::vector<unsigned long long> key = {1ull, (unsigned long long)std::min<size_t>(16, (mmax + 15) / 16),
+        std::vector<unsigned long long> key = {1ull, (unsigned long long)((mmax + 15) / 16),
                                                (unsigned long long)P, (unsigned long long)F,
                                                (unsigned long long)policy};
         graph_key_buffers(ctx, key);

@@ -25,6 +25,10 @@ namespace mh {
 namespace {
 
 constexpr int kTileF = 64;   // candidates per finalize item (x 4 row phases)
+// basis functions the FP64 scoring passes take: 256 per T-GEMM launch and in the projection's
+// LDS tile, the columns beyond in further launches and straight from memory; 512 is also the
+// device append's limit
+constexpr size_t kMaxScoreTerms = 512;
 // work items per projection / T-GEMM pass (ctx->tune.pass_items, default 32768): about 32 one-wave
 // items per SIMD, so a pass keeps every CU busy however few candidates each fit has
 constexpr long long kMaxSplits = 2048;  // row ranges per candidate tile at most
@@ -380,23 +384,27 @@ void launch_tgemm(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, cons
     if (nitems == 0) return;
     const dim3 grid(grid_w(nitems)), block(64);
     const FitDev *fv = fits ? fits : (const FitDev *)ctx->d_fits.p;
+    // basis columns [kb, kb + 256) per launch: fits of more than 256 basis functions take several
+    for (int kb = 0; kb < 16 * mt; kb += 256) {
 #define MH_GO1(M, CG)                                                                                 \
     hipLaunchKernelGGL((k_tgemm_w<M, CG, RETRY, F32>), grid, block, 0, ctx->stream, fv, \
                        (const Item *)u.items, nitems, (const int *)cs.d_xc.p, (const int *)cs.d_par.p, list, ldT, \
-                       ctx->tpart.p, dn)
+                       ctx->tpart.p, dn, kb)
 #define MH_GO(M)                      \
     if (narrow) MH_GO1(M, 1);         \
     else MH_GO1(M, cg_tgemm(M))
-    MH_MT_SWITCH(mt, MH_GO)
+        MH_MT_SWITCH(std::min(mt - kb / 16, 16), MH_GO)
 #undef MH_GO
 #undef MH_GO1
-    MH_LAUNCH_CHECK();
+        MH_LAUNCH_CHECK();
+    }
 }
 
 void launch_tgemm_bf16(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, const Cands &cs, int ldT,
                        bool narrow)
 {
     if (nitems == 0) return;
+    require(mt <= 16, "the BF16-split T-GEMM covers up to 256 basis columns");
     const dim3 grid(grid_w(nitems)), block(64);
 #define MH_GO1(M, CG)                                                                                           \
     hipLaunchKernelGGL((k_tgemm_bf16<M, CG>), grid, block, 0, ctx->stream, (const FitDev *)ctx->d_fits.p,       \
@@ -419,17 +427,21 @@ void launch_project(mars_hip_ctx *ctx, int mt, const Uploaded &u, int nitems, co
     if (nitems == 0) return;
     const dim3 grid(grid_w(nitems)), block(64);
     const FitDev *fv = fits ? fits : (const FitDev *)ctx->d_fits.p;
-#define MH_GO1(M, CG)                                                                                     \
-    hipLaunchKernelGGL((k_project_w<M, CG, MODE, RB, TCAND, F32>), grid, block, 0, ctx->stream,           \
+#define MH_GO2(M, CG, W)                                                                                  \
+    hipLaunchKernelGGL((k_project_w<M, CG, MODE, RB, TCAND, F32, W>), grid, block, 0, ctx->stream,        \
                        fv, (const Item *)u.items, nitems, (const int *)cs.d_xc.p, \
                        (const int *)cs.d_par.p, list, T, ldT, (const double *)ctx->ssum.p,                \
                        (const int *)ctx->flag.p, keep, ctx->spart.p, dn)
+#define MH_GO1(M, CG) MH_GO2(M, CG, false)
 #define MH_GO(M)                      \
     if (narrow) MH_GO1(M, 1);         \
     else MH_GO1(M, cg_project(M))
-    MH_MT_SWITCH(mt, MH_GO)
+    // more than 256 basis functions: the 256-column tile plus the columns past it from memory
+    if (mt > 16) MH_GO2(16, 1, true);
+    else MH_MT_SWITCH(mt, MH_GO)
 #undef MH_GO
 #undef MH_GO1
+#undef MH_GO2
     MH_LAUNCH_CHECK();
 }
 
@@ -566,7 +578,7 @@ void reserve_step_buffers(mars_hip_ctx *ctx)
     for (const FitState &fs : ctx->fits) P += fs.px * parents;
     P = std::min<size_t>(P, size_t(1) << 22);  // beyond this the buffers grow when needed
     const size_t F = ctx->fits.size();
-    const size_t ldT = 16 * std::min<size_t>(16, (ctx->max_terms + 15) / 16);
+    const size_t ldT = 16 * ((std::min<size_t>(ctx->max_terms, kMaxScoreTerms) + 15) / 16);
     const size_t items = 3 * ((size_t)ctx->tune.pass_items + P / 16 + F) + 1;  // the three passes' work lists
     for (Cands *c : {&ctx->cand, &ctx->rcand}) {
         c->d_fit.ensure(P);
@@ -1032,8 +1044,8 @@ void plain_score_device(mars_hip_ctx *ctx, const Cands &cs, const int *list, con
     const int F = (int)ctx->fits.size();
     size_t mmax = 1;
     for (const FitState &fs : ctx->fits) mmax = std::max(mmax, fs.m);
-    require(mmax <= 256, "the FP64 scoring path supports up to 256 basis functions");
-    const int mt = (int)std::min<size_t>(16, (mmax + 15) / 16), ldT = 16 * mt;
+    require(mmax <= kMaxScoreTerms, "the FP64 scoring path supports up to 512 basis functions");
+    const int mt = (int)((mmax + 15) / 16), ldT = 16 * mt;
     const size_t B = (size_t)bound;
     const bool ident = list == nullptr;
     const bool none_stored = keep == nullptr;
@@ -1345,8 +1357,8 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
         flops += 2.0 * n[f] * ctx->fits[f].m * cnt[f];
         bytes += 4.0 * n[f] * (2.0 * ctx->fits[f].m + 1);
     }
-    const int mt = (int)std::min<size_t>(16, (mmax + 15) / 16);
-    require(mmax <= 256, "the FP64 scoring path supports up to 256 basis functions");
+    const int mt = (int)((mmax + 15) / 16);
+    require(mmax <= kMaxScoreTerms, "the FP64 scoring path supports up to 512 basis functions");
     const int ldT = mt * 16;
     const bool narrow = narrow_for(cnt, ctx->tune.narrow_avg);
     const int tileT = 16 * (narrow ? 1 : cg_tgemm(mt)), tileP = 16 * (narrow ? 1 : cg_project(mt));

@@ -189,7 +189,9 @@ constexpr int pr_waves(int mt, int cg) { return pr_depth(mt, cg) > 1 ? 2 : waves
  * takes rows r + 8 kr + q (kr = lane / 16), so each lane reads 4 consecutive
  * rows of its Bo columns 16u + lane%16 and of its candidates' Bx.  The item's
  * half-steps form a software pipeline: the loads of step s + PD - 1 are issued
- * before the MFMAs of step s (PD register sets, indexed at compile time).
+ * before the MFMAs of step s (PD register sets, indexed at compile time).  A
+ * launch covers basis columns kb .. kb + 16 MT - 1: fits of more than 256 basis
+ * functions take one launch per 256 columns.
  */
 template <int MT, int CG, bool RETRY, bool F32 = false>
 __global__ __launch_bounds__(64, tg_waves(MT, CG)) void k_tgemm_w(const FitDev *__restrict__ fits,
@@ -198,7 +200,7 @@ __global__ __launch_bounds__(64, tg_waves(MT, CG)) void k_tgemm_w(const FitDev *
                                                                   const int *__restrict__ cpar,
                                                                   const int *__restrict__ list, int ldT,
                                                                   double *__restrict__ tpart,
-                                                                  const int *__restrict__ dn)
+                                                                  const int *__restrict__ dn, int kb)
 {
     if (dn) nitems = *dn;  // item count built on the device
     constexpr int PD = tg_depth(MT, CG);
@@ -216,7 +218,7 @@ __global__ __launch_bounds__(64, tg_waves(MT, CG)) void k_tgemm_w(const FitDev *
         bool kok[MT];
 #pragma unroll
         for (int u = 0; u < MT; ++u) {
-            const int k = 16 * u + kc;
+            const int k = kb + 16 * u + kc;
             kok[u] = k < F.m;
             bo[u] = F.BoC + (size_t)(kok[u] ? k : 0) * F.ldx;
         }
@@ -304,7 +306,7 @@ __global__ __launch_bounds__(64, tg_waves(MT, CG)) void k_tgemm_w(const FitDev *
             for (int u = 0; u < MT; ++u)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)  // D row l/16 + 4v (FP64 MFMA), 4 (l/16) + v (FP32 MFMA)
-                    dst[16 * u + (F32 ? 4 * kr + v : kr + 4 * v)] = acc[g][u][v];
+                    dst[kb + 16 * u + (F32 ? 4 * kr + v : kr + 4 * v)] = acc[g][u][v];
         }
     }
 }
@@ -427,9 +429,13 @@ enum { kStoreR = 0, kSumSq = 1, kFinalY = 2 };
  * M index M stands for row 4 (M % 4) + M / 4 of the 16-row block, so the D
  * elements a lane holds (M = l/16 + 4v) are the 4 consecutive rows
  * 4 (l/16) + v, whose Bx it loads as one float4 each of B and X.  The 16-row
- * blocks are software-pipelined like the T-GEMM's half-steps.
+ * blocks are software-pipelined like the T-GEMM's half-steps.  The 256-column
+ * build (MT = 16) also serves fits of more than 256 basis functions: the
+ * columns past the first 256 are added to Bo T in the same FP64 sum, their Bo
+ * and T values read straight from memory (a tile's T, 16 x ldT doubles, stays in
+ * the L2), before the residual is rounded.
  */
-template <int MT, int CG, int MODE, bool RB, bool TCAND, bool F32 = false>
+template <int MT, int CG, int MODE, bool RB, bool TCAND, bool F32 = false, bool WIDE = false>
 __global__ __launch_bounds__(64, pr_waves(MT, CG)) void k_project_w(const FitDev *__restrict__ fits,
                                                                     const Item *__restrict__ items, int nitems,
                                                                     const int *__restrict__ cxc,
@@ -475,6 +481,8 @@ __global__ __launch_bounds__(64, pr_waves(MT, CG)) void k_project_w(const FitDev
         double acc[CG];
         float scale[CG];
         bool fromr[CG], store[CG];
+        static_assert(!WIDE || MT == 16, "columns past 256 follow a full 256-column tile");
+        const double *trow[CG];  // WIDE: this lane's candidate's T row (columns past 256)
         const float *bo[KQ];
 #pragma unroll
         for (int q = 0; q < KQ; ++q) {
@@ -484,6 +492,12 @@ __global__ __launch_bounds__(64, pr_waves(MT, CG)) void k_project_w(const FitDev
 #pragma unroll
         for (int g = 0; g < CG; ++g) {
             c[g] = lane_cand<RB>(F, I, 16 * g + kc, cxc, cpar, list);
+            if constexpr (WIDE) {
+                const int sl = 16 * g + kc < I.nc ? 16 * g + kc : 0;
+                const int pos = I.c0 + sl;
+                const int row = TCAND ? (list ? gld(list + pos) : pos) : pos;
+                trow[g] = T + (size_t)row * ldT;
+            }
             acc[g] = 0.0;
             scale[g] = 0.0f;
             fromr[g] = false;
@@ -536,6 +550,28 @@ __global__ __launch_bounds__(64, pr_waves(MT, CG)) void k_project_w(const FitDev
 #pragma unroll
                     for (int q = 0; q < KQ; ++q)
                         if (q < kq) d = mfma_f64((double)av[q], sT[16 * g + kc][4 * q + kr], d);
+                }
+                if constexpr (WIDE) {
+                    // basis columns 256 .. m - 1 (uniform over the item)
+                    const long long r16 = I.r0 + 16LL * st;
+                    for (int k0 = 256; k0 < F.m; k0 += 256) {
+#pragma unroll 8
+                        for (int q = 0; q < KQ; ++q) {
+                            const int k = k0 + 4 * q + kr;
+                            const bool ok = k < F.m;
+                            const int kk = ok ? k : 0;  // (clamped: every address formed is valid)
+                            const float av1 = gld(F.BoC + (size_t)kk * F.ldx + r16 + prow);
+                            const double bv1 = gld(trow[g] + kk);
+                            const float a = ok ? av1 : 0.0f;
+                            const double b = ok ? bv1 : 0.0;
+                            if (F32) {
+                                const v4f e = mfma_f32(a, (float)b, v4f{0, 0, 0, 0});
+                                d = v4d{d[0] + e[0], d[1] + e[1], d[2] + e[2], d[3] + e[3]};
+                            } else {
+                                d = mfma_f64((double)a, b, d);
+                            }
+                        }
+                    }
                 }
                 if (!c[g].ok) continue;
                 const float4 bx = bx_make<RB>(bw[j][g], c[g], row, I.r1);

@@ -528,6 +528,26 @@ public:
         return -1.;
     }
 
+    /* Test hook: replace the basis by m columns given as B (n, m) and Bo (n, m), both
+     * C-order, and recompute y . Bo[:, k] as append does, so that eval scores candidates
+     * against a basis built elsewhere (the device's). */
+    void load_basis(const float *B, const float *Bo, size_t m)
+    {
+        check(m >= 1 && m <= max_terms_, "basis size out of range");
+        const size_t n = n_, L = max_terms_;
+        ybo_.clear();
+        for (size_t j = 0; j < m; ++j) {
+            double yb = 0.0;
+            for (size_t i = 0; i < n; ++i) {
+                B_[j * n + i] = B[i * m + j];
+                Bo_[i * L + j] = Bo[i * m + j];
+                yb = std::fma((double)Bo_[i * L + j], (double)y_[i], yb);
+            }
+            ybo_.push_back(yb);
+        }
+        m_ = m;
+    }
+
     const std::vector<float> &B() const { return B_; }
     const std::vector<float> &Bo() const { return Bo_; }
     const std::vector<float> &y() const { return y_; }
@@ -670,6 +690,14 @@ PYBIND11_MODULE(oracle_marslib, m)
         .def("y", [](PyAlgo &pa) { return vec_copy(pa.a->y()); })
         .def("ybo", [](PyAlgo &pa) { return vec_copy(pa.a->ybo()); })
         .def("scales", [](PyAlgo &pa) { return vec_copy(pa.a->s()); })
+        .def("load_basis",
+             [](PyAlgo &pa, py::array_t<float, py::array::c_style | py::array::forcecast> B,
+                py::array_t<float, py::array::c_style | py::array::forcecast> Bo) {
+                 oracle::check(B.ndim() == 2 && Bo.ndim() == 2 && B.shape(0) == Bo.shape(0) &&
+                                   B.shape(1) == Bo.shape(1) && (size_t)B.shape(0) == pa.a->n(),
+                               "load_basis: B and Bo must both be (n, m)");
+                 pa.a->load_basis(B.data(), Bo.data(), B.shape(1));
+             })
         .def("basis",
              [](PyAlgo &pa) {
                  /* (B, Bo) restricted to the live columns, both (n, m) C-order. */
