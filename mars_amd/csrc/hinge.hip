@@ -198,7 +198,8 @@ constexpr int kTR = 16;    // sorted rows per tile
  * rest zero padding up to PT = G J slots.
  */
 /* Slots per lane: 8 or 12, with 4 lanes per candidate up to 48 slots (blocks of 16 or 32
- * candidates), then 8, 16 or 32 lanes; blocks of 8 candidates keep 8 lanes below 64 slots. */
+ * candidates), then 8, 16, 32 or 64 lanes; blocks of 8 candidates keep 8 lanes below 64 slots.
+ * 512 slots take a whole wave per candidate. */
 constexpr int sweep_j(int pt, int nc)
 {
     return pt == 32 ? (nc >= 16 ? 8 : 4) : (pt == 48 ? (nc >= 16 ? 12 : 6) : (pt == 96 || pt == 192 ? 12 : 8));
@@ -211,6 +212,7 @@ struct SweepShape {
     static constexpr int CW = 64 / G;                   // candidates per wave
     static constexpr int NW = NC / CW;                  // waves per block
     static constexpr int RPS = 64 * NW / PT;            // gather: tile rows per block-wide load
+    static_assert(G <= 64 && NW <= 16, "a candidate within one wave, at most 1024 threads per block");
 };
 
 /* v from the lane DPP control CTRL names (a permutation within each row of 16 lanes). */
@@ -232,6 +234,7 @@ __device__ __forceinline__ double group_sum(double v)
     if (G >= 8) v = v + dpp_perm<0x141>(v);  // row_half_mirror: lane i <-> 7-i
     if (G >= 16) v = v + dpp_perm<0x140>(v);  // row_mirror: lane i <-> 15-i
     if (G >= 32) v = v + __shfl_xor(v, 16, 64);
+    if (G >= 64) v = v + __shfl_xor(v, 32, 64);
     return v;
 }
 
@@ -368,7 +371,8 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
     static_assert(SORTED || (RPS > 0 && kTR % RPS == 0 && 64 * NW == RPS * PT), "tile rows");
     static_assert(J % 2 == 0, "slot pairs");
     constexpr int NB = SORTED ? 1 : kTR / RPS;             // gather: Bo-row loads per lane and tile
-    constexpr int NPC = 2 * NC * kTR / NT;                 // gather: per-candidate loads per lane and tile
+    constexpr int NPC = (2 * NC * kTR + NT - 1) / NT;      // gather: per-candidate loads per lane and tile
+    constexpr bool PPART = NPC * NT > 2 * NC * kTR;        // (64 lanes per candidate: half the threads load)
     constexpr int SQ = PT + 2;                             // LDS row stride of the y / Bo tile
     __shared__ __attribute__((aligned(16))) double sq[2][kTR][SQ];
     __shared__ double2 sbx[2][kTR][NC];                    // (b, Bx) per candidate
@@ -402,7 +406,7 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
     const float *pp[NPC];
     size_t pld[NPC];
     int prr[NPC], pj[NPC];
-    bool pisb[NPC];
+    bool pisb[NPC], pok[NPC];
     SortedRuns<PT, NC, NT> R;   // sorted: the tile loads
     if constexpr (SORTED) {
         R.init(a, tid, xc, m, c0, sbase, nact, r0, r1);
@@ -417,13 +421,14 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
 #pragma unroll
         for (int i = 0; i < NPC; ++i) {
             const int e = tid + NT * i;
+            pok[i] = !PPART || e < 2 * NC * kTR;
             pisb[i] = e < NC * kTR;
             const int rem = e % (NC * kTR);
             prr[i] = rem / NC;
             pj[i] = rem % NC;
             pp[i] = nullptr;
             pld[i] = 0;
-            if (pj[i] < nact) {
+            if (pok[i] && pj[i] < nact) {
                 if (pisb[i]) {
                     pp[i] = a.B + a.cpar[c0 + sbase + pj[i]];
                     pld[i] = a.ldb;
@@ -500,6 +505,7 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
             }
 #pragma unroll
             for (int i = 0; i < NPC; ++i) {
+                if (PPART && !pok[i]) continue;
                 const bool in = prr[i] < nr && pp[i];
                 const double v = in ? (double)pp[i][(size_t)srow[0][prr[i]] * pld[i]] : 0.0;
                 if (pisb[i]) sbx[0][prr[i]][pj[i]].x = v;
@@ -623,6 +629,7 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
             for (int i = 0; i < NB; ++i) sq[nxt][i * RPS + ro][pos] = (double)bov[i];
 #pragma unroll
             for (int i = 0; i < NPC; ++i) {
+                if (PPART && !pok[i]) continue;
                 if (pisb[i]) sbx[nxt][prr[i]][pj[i]].x = (double)pcv[i];
                 else sbx[nxt][prr[i]][pj[i]].y = (double)pcv[i];
             }
@@ -800,7 +807,6 @@ __global__ void k_sweep_carry(SweepArgs a, const int *__restrict__ cgrp, int P)
     const int c = (int)(t / a.L2), k = (int)(t % a.L2);
     if (c >= P) return;
     const int gi = cgrp[c];
-    const int m = a.m;
     double f = 0.0, g = 0.0, kk = 0.0;
     // the thread of lane slot 0 (the parent b) also carries k and needs its own (f, g) = (bd, b2)
     for (int ch = 0; ch < a.C; ++ch) {
@@ -874,10 +880,20 @@ void launch_sweep(const SweepArgs &a, dim3 grid, int nc, hipStream_t st, bool ev
  * steps (its loads need no row-to-lane fit of the tile width). */
 int sweep_slots(int m, bool sorted)
 {
-    static const int fine[] = {32, 48, 64, 96, 128, 192, 256}, coarse[] = {32, 64, 128, 256};
-    for (int pt : sorted ? std::vector<int>(fine, fine + 7) : std::vector<int>(coarse, coarse + 4))
+    static const int fine[] = {32, 48, 64, 96, 128, 192, 256, 512}, coarse[] = {32, 64, 128, 256, 512};
+    for (int pt : sorted ? std::vector<int>(fine, fine + 8) : std::vector<int>(coarse, coarse + 5))
         if (m + 3 <= pt) return pt;
-    return 256;
+    return 512;
+}
+
+/* 512 slots: a wave per candidate, blocks of 8 candidates (16 would spill the scoring pass's
+ * registers); both passes run the recurrence (the MFMA summary pass keeps one wave per 16 slots,
+ * at most 256 slots per block). */
+template <bool SORTED>
+void launch_sweep_wide(const SweepArgs &a, dim3 grid, int nc, hipStream_t st, bool eval)
+{
+    require(nc == 8, "512-slot sweeps take blocks of 8 candidates");
+    launch_sweep_nc<512, 8, SORTED>(a, grid, st, eval);
 }
 
 template <int PT>
@@ -902,7 +918,8 @@ void launch_sweep_sorted(const SweepArgs &a, dim3 grid, int pt, int nc, hipStrea
         case 96: launch_sweep_sum<96>(a, grid, nc, st); break;
         case 128: launch_sweep_sum<128>(a, grid, nc, st); break;
         case 192: launch_sweep_sum<192>(a, grid, nc, st); break;
-        default: launch_sweep_sum<256>(a, grid, nc, st); break;
+        case 256: launch_sweep_sum<256>(a, grid, nc, st); break;
+        default: launch_sweep_wide<true>(a, grid, nc, st, false); break;
         }
         return;
     }
@@ -913,7 +930,8 @@ void launch_sweep_sorted(const SweepArgs &a, dim3 grid, int pt, int nc, hipStrea
     case 96: launch_sweep<96, true>(a, grid, nc, st, eval); break;
     case 128: launch_sweep<128, true>(a, grid, nc, st, eval); break;
     case 192: launch_sweep<192, true>(a, grid, nc, st, eval); break;
-    default: launch_sweep<256, true>(a, grid, nc, st, eval); break;
+    case 256: launch_sweep<256, true>(a, grid, nc, st, eval); break;
+    default: launch_sweep_wide<true>(a, grid, nc, st, eval); break;
     }
 }
 
@@ -923,7 +941,8 @@ void launch_sweep_gather(const SweepArgs &a, dim3 grid, int pt, int nc, hipStrea
     case 32: launch_sweep<32, false>(a, grid, nc, st, eval); break;
     case 64: launch_sweep<64, false>(a, grid, nc, st, eval); break;
     case 128: launch_sweep<128, false>(a, grid, nc, st, eval); break;
-    default: launch_sweep<256, false>(a, grid, nc, st, eval); break;
+    case 256: launch_sweep<256, false>(a, grid, nc, st, eval); break;
+    default: launch_sweep_wide<false>(a, grid, nc, st, eval); break;
     }
 }
 
@@ -1058,11 +1077,11 @@ void hinge_sweep_list(mars_hip_ctx *ctx, Cands &cs, const double *lin, const dou
         ensure_sorted(ctx, gx);
     }
     const size_t tail = n - (size_t)end_span;
-    require(m <= 253, "hinge sweep supports up to 253 basis functions");
+    require(m <= 509, "hinge sweep supports up to 509 basis functions");
     const int PT = sweep_slots(m, sorted);
     const int L2 = PT;
-    // candidates per block: one block takes all parents of an X column, up to 32
-    const int NC = maxnp <= 8 ? 8 : (maxnp <= 16 ? 16 : 32);
+    // candidates per block: one block takes all parents of an X column, up to 32 (8 with 512 slots)
+    const int NC = maxnp <= 8 || PT > 256 ? 8 : (maxnp <= 16 ? 16 : 32);
     const int pgroups = (maxnp + NC - 1) / NC;
     const int wpb = NC / (64 / (PT / sweep_j(PT, NC)));   // waves per block
     // about sweep_waves waves (65536), chunks of at least sweep_min_rows (512) sorted rows; a multiple of 8
@@ -1283,7 +1302,7 @@ void hinge_score(mars_hip_ctx *ctx, Cands &cs, double *lin, double *ybx, double 
     const int P = (int)cs.fit.size();
     if (P == 0) return;
     // the sweep's limit, checked before the scoring passes run
-    require(ctx->fits[0].m <= 253, "hinge sweep supports up to 253 basis functions");
+    require(ctx->fits[0].m <= 509, "hinge sweep supports up to 509 basis functions");
     std::vector<int> gxc, gc0;
     for (int c = 0; c < P; ++c)
         if (c == 0 || cs.xc[c] != cs.xc[c - 1]) {

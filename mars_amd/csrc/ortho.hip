@@ -1399,8 +1399,10 @@ void plain_score(mars_hip_ctx *ctx, Cands &cs, bool keep_bxn, double *lin, doubl
 
     // Pass 1: T = Bo^T Bx (FP32 MFMA in the precision form's approximate pass)
     {
-        KTimer kt(ctx, f32 ? (ctx->split_bf16 ? K_TGEMM_BF16 : K_TGEMM32) : K_TGEMM, flops, bytes);
-        if (f32 && ctx->split_bf16)
+        KTimer kt(ctx, f32 ? (ctx->split_bf16 && mt <= 16 ? K_TGEMM_BF16 : K_TGEMM32) : K_TGEMM, flops, bytes);
+        // (past 256 basis columns the FP32 T-GEMM serves the BF16-split form: its error is smaller
+        // than the split's, which the screening bound assumes)
+        if (f32 && ctx->split_bf16 && mt <= 16)
             launch_tgemm_bf16(ctx, mt, uT, (int)pT.items.size(), cs, ldT, narrow);
         else if (f32)
             launch_tgemm<false, true>(ctx, mt, uT, (int)pT.items.size(), cs, nullptr, ldT, narrow);
