@@ -120,3 +120,31 @@ def test_input_errors_match_reference(bad):
     msg = "not all columns in X are finite" if bad == "x_inf" else "target Y is all zero or NANs"
     with pytest.raises(Exception, match=msg):
         mars_amd.fit(X, y, w, max_terms=5)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_edge_batch_matches_single_fits(precision):
+    """One batch of the linear versions of the edge inputs (non-finite targets and zero weights,
+    degenerate columns, 20 rows, ragged rows, one X column, ties): each fit's terms and mse equal
+    the same fit run alone."""
+    import mars_amd
+
+    names = ["nan_y_zero_w", "const_dup_cols", "tiny_n", "ragged_linear", "p1", "ties"]
+    say("edge batch", precision)
+    cases = [_case(nm) for nm in names]
+    ws = [np.ones(len(y), dtype=np.float32) if w is None else w for _, y, w, _ in cases]
+    stats = {}
+    models = mars_amd.fit_batch([c[0] for c in cases], [c[1] for c in cases], ws, max_terms=13,
+                                precision=precision, stats=stats)
+    for (X, y, _, _), w, got, mse in zip(cases, ws, models, stats["mse"]):
+        trace = []
+        # (min_span is unused by linear fits; given, as the default's formula fails at p = 1)
+        want = mars_amd.fit(X, y, w, max_terms=13, linear_only=True, min_span=1, trace=trace,
+                            precision=precision)
+        assert [(r["type"], int(r["basis"]), int(r["input"])) for r in got] == \
+            [(r["type"], int(r["basis"]), int(r["input"])) for r in want]
+        want_mse = [v for s in trace for v in s["mse"] if v >= 0]
+        got_mse = [v for v in mse if v >= 0]
+        assert len(got_mse) == len(want_mse)
+        for a, b in zip(got_mse, want_mse):
+            assert abs(a - b) <= 1e-10 * max(abs(b), 1e-300), (a, b)
