@@ -674,19 +674,26 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
  * W_l = the chunk's deltas up to and including row l, D = all of them.  For the slots shared by
  * the X column's parents (y, the Bo columns) G and H are two contractions [b; b W] (candidates x
  * rows) x q (rows x slots): v_mfma_f64_16x16x4f64 with candidates as M, slots as N and 4 sorted
- * rows as K, wave w taking slots 16 w .. 16 w + 15.  The candidate's own slots (b, Bx) and k
- * run the recurrence itself on one lane per candidate of wave 0.  Tiles as in k_sweep (16 rows,
- * LDS in FP64, double-buffered); the prefix W of each tile is scanned by wave 0.  (Measured
- * slower: a separate wave for the candidates' own slots, and tile loads issued three tiles
- * ahead.)
+ * rows as K, wave w taking slots 16 w .. 16 w + 15.  The candidate's own slots have the same
+ * closed forms with q = b (slot 0) and q = Bx (slot 2), and k = sum_l b_l^2 (D - W_l)^2; their
+ * five sums (b^2, b^2 W, b Bx, b Bx W, b^2 (D - W)^2, every product of two f32 values exact)
+ * are spread over the block, one candidate per thread and a fixed set of rows each, then
+ * reduced in a fixed order.  D, needed from the first tile on, is summed over the chunk's
+ * deltas before the tiles.  Tiles as in k_sweep (16 rows, LDS in FP64, double-buffered); the
+ * prefix W of each tile is scanned by wave 0.  (Measured slower: a separate wave for the
+ * candidates' own slots, and tile loads issued three tiles ahead.)
  */
 template <int PT, int NC>
 __global__ __launch_bounds__(64 * (PT / 16)) void k_sweep_sum(SweepArgs a)
 {
     constexpr int NW = PT / 16, NT = 64 * NW, SQ = PT + 2, MT = (NC + 15) / 16;
+    constexpr int NP = (NC * kTR + NT - 1) / NT;   // own-slot (candidate, row) pairs per thread and tile
+    static_assert(NT % NC == 0, "a thread's pairs share one candidate");
+    static_assert(5 * NW * NC <= 2 * kTR * SQ, "the own-slot reduction fits the tile buffers");
     __shared__ __attribute__((aligned(16))) double sq[2][kTR][SQ];
     __shared__ double2 sbx[2][kTR][NC];
-    __shared__ double sd[2][kTR], sw[2][kTR];
+    __shared__ double sw[2][kTR];
+    __shared__ double sdp[NW];
 
     const int m = a.m;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, kr = lane >> 4, kc = lane & 15;
@@ -700,42 +707,69 @@ __global__ __launch_bounds__(64 * (PT / 16)) void k_sweep_sum(SweepArgs a)
     const float *dl = a.delta + (size_t)xc * a.n;
     const int T = r1 > r0 ? (int)((r1 - r0 + kTR - 1) / kTR) : 0;
     auto tile_rows = [&](int tt) { return (int)min((size_t)kTR, r1 - (r0 + (size_t)tt * kTR)); };
-    const bool own = wv == 0 && lane < NC;  // the lane of candidate `lane`'s own slots
 
-    SortedRuns<PT, NC, NT> R;
+    // D: the chunk's delta sum (the carry's shift and the own slots' k), lane-strided per
+    // thread, a fixed butterfly per wave, the waves added in order
+    double Dc;
+    {
+        double dacc = 0.0;
+        for (size_t l = r0 + tid; l < r1; l += NT) dacc += (double)dl[l];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) dacc += __shfl_xor(dacc, o, 64);
+        if (lane == 0) sdp[wv] = dacc;
+        __syncthreads();
+        Dc = sdp[0];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) Dc += sdp[w];
+    }
+    // own slots: this thread's candidate and its rows of each tile
+    const int oj = tid % NC;
+    double oGb = 0.0, oHb = 0.0, oGx = 0.0, oHx = 0.0, oK = 0.0;
+
+    // tiles and their deltas are fetched two tiles ahead (register sets 0 and 1)
+    SortedRuns<PT, NC, NT, kTR, 2> R;
     R.init(a, tid, xc, m, c0, sbase, nact, r0, r1);
     v4d accG[MT], accH[MT];
 #pragma unroll
     for (int u = 0; u < MT; ++u) accG[u] = accH[u] = v4d{0, 0, 0, 0};
-    double fb = 0.0, gb = 0.0, fx = 0.0, gx = 0.0, k2 = 0.0;
     double wbase = 0.0;   // W at the end of the previous tile (wave 0)
-    // deltas of tile tt into sd[b], their running prefix into sw[b] (wave 0)
-    auto deltas = [&](int tt, int b) {
-        if (wv != 0) return;
+    float dv[2];          // wave 0, lanes < 16: deltas of the tiles in flight
+    auto load_d = [&](int tt) {
         const int nr = tt < T ? tile_rows(tt) : 0;
-        const double d = lane < nr ? (double)dl[r0 + (size_t)tt * kTR + lane] : 0.0;
-        double w = d;
+        return wv == 0 && lane < nr ? dl[r0 + (size_t)tt * kTR + lane] : 0.0f;
+    };
+    // the running prefix W of a tile's deltas d into sw[b] (wave 0)
+    auto deltas = [&](float df, int b) {
+        if (wv != 0) return;
+        double w = (double)df;
 #pragma unroll
         for (int o = 1; o < kTR; o <<= 1) {
             const double v = __shfl_up(w, o, 64);
             if (lane >= o) w += v;
         }
-        if (lane < kTR) {
-            sd[b][lane] = d;
-            sw[b][lane] = wbase + w;
-        }
+        if (lane < kTR) sw[b][lane] = wbase + w;
         wbase += __shfl(w, kTR - 1, 64);
     };
     if (T > 0) {
         R.template load<0>(0);
+        const float d0 = load_d(0);
         R.template store<0>(sq[0], sbx[0]);
-        deltas(0, 0);
+        deltas(d0, 0);
+        if (T > 1) {
+            R.template load<1>(1);
+            dv[1] = load_d(1);
+        }
     }
     __syncthreads();
-    for (int tt = 0; tt < T; ++tt) {
+    // tile tt: in LDS buffer tt & 1; tile tt + 1 in register set S (loaded a tile earlier)
+    auto body = [&](int tt, auto sc) {
+        constexpr int S = decltype(sc)::value, S2 = S ^ 1;
         const int cur = tt & 1, nxt = cur ^ 1;
         const bool more = tt + 1 < T;
-        if (more) R.template load<0>(tt + 1);
+        if (tt + 2 < T) {
+            R.template load<S2>(tt + 2);
+            dv[S2] = load_d(tt + 2);
+        }
         {
 #pragma unroll
             for (int ks = 0; ks < kTR / 4; ++ks) {
@@ -751,30 +785,32 @@ __global__ __launch_bounds__(64 * (PT / 16)) void k_sweep_sum(SweepArgs a)
                 }
             }
         }
-        if (own) {
 #pragma unroll
-            for (int rr = 0; rr < kTR; ++rr) {
-                const double d = sd[cur][rr];
-                const double2 bx = sbx[cur][rr][lane];
-                const double fb1 = fma(d, gb, fb);
-                k2 = fma(d, fb + fb1, k2);
-                fb = fb1;
-                gb = fma(bx.x, bx.x, gb);
-                fx = fma(d, gx, fx);
-                gx = fma(bx.x, bx.y, gx);
-            }
+        for (int i = 0; i < NP; ++i) {
+            const int e = tid + NT * i;
+            if (NP * NT > NC * kTR && e >= NC * kTR) break;
+            const int rr = e / NC;
+            const double2 bx = sbx[cur][rr][oj];   // (b, Bx)
+            const double w = sw[cur][rr];
+            const double b2 = bx.x * bx.x, bb = bx.x * bx.y, dw = Dc - w;
+            oGb += b2;
+            oHb = fma(b2, w, oHb);
+            oGx += bb;
+            oHx = fma(bb, w, oHx);
+            oK = fma(b2 * dw, dw, oK);
         }
         if (more) {
-            R.template store<0>(sq[nxt], sbx[nxt]);
-            deltas(tt + 1, nxt);
+            R.template store<S>(sq[nxt], sbx[nxt]);
+            deltas(dv[S], nxt);
         }
         __syncthreads();
+    };
+    for (int tt = 0; tt < T;) {
+        body(tt, std::integral_constant<int, 1>{});
+        if (++tt >= T) break;
+        body(tt, std::integral_constant<int, 0>{});
+        ++tt;
     }
-    // D: the chunk's delta sum (the carry's shift), from the same prefix chain
-    __shared__ double sD;
-    if (tid == 0) sD = wbase;
-    __syncthreads();
-    const double Dc = sD;
     if (pg == 0 && tid == 0) a.dsum[(size_t)gi * a.C + chunk] = Dc;
     {
         // D layout of the FP64 MFMA: lane l holds rows (candidates) l/16 + 4 v, column (slot) l % 16
@@ -789,14 +825,34 @@ __global__ __launch_bounds__(64 * (PT / 16)) void k_sweep_sum(SweepArgs a)
                 dst[1] = accG[u][v];
             }
     }
-    if (own && lane < nact) {
-        const size_t cs = (size_t)(c0 + sbase + lane) * a.C + chunk;
-        double *dst = a.sum + cs * a.L2 * 2;
-        dst[0] = fb;
-        dst[1] = gb;
-        dst[4] = fx;
-        dst[5] = gx;
-        a.ksum[cs] = k2;
+    {
+        // own slots: the threads of one candidate (lanes l = j mod NC of each wave) by a fixed
+        // butterfly, then the waves in order through the (now free) tile buffers
+        double v[5] = {oGb, oHb, oGx, oHx, oK};
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+#pragma unroll
+            for (int o = 32; o >= NC; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+        double *red = &sq[0][0][0];   // [5][NW][NC]
+        if (lane < NC)
+#pragma unroll
+            for (int q = 0; q < 5; ++q) red[(q * NW + wv) * NC + lane] = v[q];
+        __syncthreads();
+        if (tid < nact) {
+            double u[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                u[q] = red[(q * NW) * NC + tid];
+                for (int w = 1; w < NW; ++w) u[q] += red[(q * NW + w) * NC + tid];
+            }
+            const size_t cs = (size_t)(c0 + sbase + tid) * a.C + chunk;
+            double *dst = a.sum + cs * a.L2 * 2;
+            dst[0] = fma(Dc, u[0], -u[1]);
+            dst[1] = u[0];
+            dst[4] = fma(Dc, u[2], -u[3]);
+            dst[5] = u[2];
+            a.ksum[cs] = u[4];
+        }
     }
 }
 
