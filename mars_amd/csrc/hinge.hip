@@ -352,8 +352,10 @@ struct SortedRuns {
  * produced in that order: every source is a contiguous run of 16 rows (float4
  * per lane; absent columns read a zero column).  Otherwise each tile row is
  * gathered from the row-major B, Bo and candidate columns through the sort
- * order.  The buffers are doubled: the global loads of tile t+1 are in flight
- * while the unrolled recurrence runs over tile t.  Rows past the chunk end are
+ * order.  The LDS buffers are doubled: while the unrolled recurrence runs over
+ * tile t, the global loads of tile t+1 are in flight (sorted at 32 slots: of
+ * tiles t+1 and t+2, in two register sets; the deltas of both in every form).
+ * Rows past the chunk end are
  * zero (d = b = 0 leaves every accumulator unchanged), so every tile runs 16
  * rows; chunks start at multiples of 16, so only the last tile before the tail
  * is partial.  Grid rows and nonzero deltas of a tile are bit masks in scalar
@@ -407,7 +409,9 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
     size_t pld[NPC];
     int prr[NPC], pj[NPC];
     bool pisb[NPC], pok[NPC];
-    SortedRuns<PT, NC, NT> R;   // sorted: the tile loads
+    // sorted: the tile loads, two tiles ahead at 32 slots (wider tiles: one, which keeps their occupancy)
+    constexpr int NSR = SORTED && PT <= 32 ? 2 : 1;
+    SortedRuns<PT, NC, NT, kTR, NSR> R;
     if constexpr (SORTED) {
         R.init(a, tid, xc, m, c0, sbase, nact, r0, r1);
     } else {
@@ -519,9 +523,17 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
             nzmask = (unsigned)__ballot(dv != 0.0f);
         }
     }
+    // deltas of the tiles in flight (sorted: with the tile loads, two tiles ahead)
+    float ndv[2] = {0.0f, 0.0f};
+    if (T > 1) {
+        if constexpr (NSR == 2) R.template load<1>(1);
+        ndv[1] = lane < tile_rows(1) ? dl[r0 + kTR + lane] : 0.0f;
+    }
     __syncthreads();
 
-    for (int tt = 0; tt < T; ++tt) {
+    // tile tt: LDS buffer tt & 1; tile tt + 1's loads in register set S
+    auto body = [&](int tt, auto sc) {
+        constexpr int S = decltype(sc)::value, S2 = S ^ 1;
         const int cur = tt & 1, nxt = cur ^ 1;
         const size_t tb0 = r0 + (size_t)tt * kTR;
         const int nr1 = tt + 1 < T ? tile_rows(tt + 1) : 0;
@@ -529,7 +541,9 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
         // ---- prefetch tile t+1 (gather: its row indices are in srow[nxt]; and tile t+2's indices)
         float bov[NB], pcv[NPC];
         int nrow = 0;
-        if constexpr (SORTED) {
+        if constexpr (NSR == 2) {
+            if (nr2 > 0) R.template load<S2>(tt + 2);
+        } else if constexpr (SORTED) {
             if (nr1 > 0) R.template load<0>(tt + 1);
         } else {
 #pragma unroll
@@ -545,7 +559,8 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
             }
             if (tid < nr2) nrow = ord[tb0 + 2 * kTR + tid];
         }
-        const float nd = lane < nr1 ? dl[tb0 + kTR + lane] : 0.0f;
+        const float nd = ndv[S];
+        ndv[S2] = lane < nr2 ? dl[tb0 + 2 * kTR + lane] : 0.0f;
 
         // ---- the recurrence over tile t (marsalgo.cc:576-649)
         if (wave_active) {
@@ -623,7 +638,7 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
         // buffer nxt last held tile t-1, which every wave finished before the
         // previous iteration's barrier: store tile t+1 into it
         if constexpr (SORTED) {
-            if (nr1 > 0) R.template store<0>(sq[nxt], sbx[nxt]);
+            if (nr1 > 0) R.template store<NSR == 2 ? S : 0>(sq[nxt], sbx[nxt]);
         } else {
 #pragma unroll
             for (int i = 0; i < NB; ++i) sq[nxt][i * RPS + ro][pos] = (double)bov[i];
@@ -639,6 +654,12 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
             if (!SORTED && tid < nr2) srow[cur][tid] = nrow;
         }
         __syncthreads();
+    };
+    for (int tt = 0; tt < T;) {
+        body(tt, std::integral_constant<int, 1>{});
+        if (++tt >= T) break;
+        body(tt, std::integral_constant<int, 0>{});
+        ++tt;
     }
     if (!EVAL && pg == 0 && wv == 0) {
         // the chunk's delta sum (carry shift), lane-strided then a fixed butterfly
