@@ -429,11 +429,12 @@ enum { kStoreR = 0, kSumSq = 1, kFinalY = 2 };
  * M index M stands for row 4 (M % 4) + M / 4 of the 16-row block, so the D
  * elements a lane holds (M = l/16 + 4v) are the 4 consecutive rows
  * 4 (l/16) + v, whose Bx it loads as one float4 each of B and X.  The 16-row
- * blocks are software-pipelined like the T-GEMM's half-steps.  The 256-column
- * build (MT = 16) also serves fits of more than 256 basis functions: the
- * columns past the first 256 are added to Bo T in the same FP64 sum, their Bo
- * and T values read straight from memory (a tile's T, 16 x ldT doubles, stays in
- * the L2), before the residual is rounded.
+ * blocks are software-pipelined like the T-GEMM's half-steps.  WIDE (MT = 16
+ * only) serves fits of more than 256 basis functions: the columns past the
+ * first 256 are added to Bo T in the same FP64 sum, their Bo and T values read
+ * straight from memory (a tile's T, 16 x ldT doubles, stays in the L2), before
+ * the residual is rounded; a separate instantiation, so the kernels of smaller
+ * bases keep their registers.
  */
 template <int MT, int CG, int MODE, bool RB, bool TCAND, bool F32 = false, bool WIDE = false>
 __global__ __launch_bounds__(64, pr_waves(MT, CG)) void k_project_w(const FitDev *__restrict__ fits,
