@@ -57,8 +57,9 @@ constexpr int kGroupInts = 7;      // fit, xc0, nc, j, a-tile, first list entry,
 constexpr int kTW = 8;         // A tiles (waves) per block at most
 constexpr int kXP = 36;        // LDS floats per staged X column (32 rows + pad)
 
+// (8 waves: a block holds a CU's 2 waves per SIMD at any register count up to 256, so no bound)
 template <int W, bool F64 = false>
-__global__ __launch_bounds__(64 * W, W >= 4 ? 3 : 1) void k_tc_update(const FitDev *__restrict__ fits,
+__global__ __launch_bounds__(64 * W, W == 4 ? 3 : 1) void k_tc_update(const FitDev *__restrict__ fits,
                                                       const Item *__restrict__ items, int nitems,
                                                       double *__restrict__ tpart, double *__restrict__ npart)
 {
@@ -73,7 +74,16 @@ __global__ __launch_bounds__(64 * W, W >= 4 ? 3 : 1) void k_tc_update(const FitD
         const int a = 16 * u + kc;
         const bool aok = a < na;
         const bool wact = 16 * u < na;   // this wave's tile holds A columns (uniform)
+        // |Bx|^2 and Bx.y of the new parent (blocks with the first A tile): at 8 waves X-column group
+        // g goes to active wave g mod (active waves), so no single wave carries them between
+        // barriers; fewer waves keep them on wave 0 (their register budget is tighter)
         const bool norms = u == 0;
+        const bool nblk = (I.pad & 255) == 0;
+        const int nwa = max(1, min(W, (na + 15) / 16 - (I.pad & 255)));   // active waves of the block
+        auto mine = [&](int g) {
+            if constexpr (W >= 8) return nblk && g % nwa == wv;
+            else return norms;
+        };
         const int ka = a <= j ? a : j, pa = a <= j ? j : a - j - 1;
         const float *pbo = F.BoC + (size_t)(aok ? ka : 0) * F.ldx;
         const float *pb = F.BC + (size_t)(aok ? pa : 0) * F.ldx;
@@ -152,7 +162,7 @@ __global__ __launch_bounds__(64 * W, W >= 4 ? 3 : 1) void k_tc_update(const FitD
                         else
                             part[g] = mfma_f32(av[q], xs[q], part[g]);
                     }
-                    if (norms) {
+                    if (mine(g)) {
                         const float4 j4 = *(const float4 *)&sjy[buf][0][8 * kr + 4 * hf];
                         const float4 y4 = *(const float4 *)&sjy[buf][1][8 * kr + 4 * hf];
                         const float jv[4] = {j4.x, j4.y, j4.z, j4.w};
@@ -226,15 +236,16 @@ __global__ __launch_bounds__(64 * W, W >= 4 ? 3 : 1) void k_tc_update(const FitD
             if (slot < I.nc)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) dst[F64 ? kr + 4 * v : 4 * kr + v] = acc[g][v];
-            if (norms) {
+            if (mine(g)) {
                 double s0 = bxx[g], s1 = bxy[g];
                 s0 = s0 + __shfl_xor(s0, 16, 64);
                 s0 = s0 + __shfl_xor(s0, 32, 64);
                 s1 = s1 + __shfl_xor(s1, 16, 64);
                 s1 = s1 + __shfl_xor(s1, 32, 64);
-                if (kr == 0 && slot < I.nc) {
-                    npart[((size_t)it * kXT + slot) * 2] = s0;
-                    npart[((size_t)it * kXT + slot) * 2 + 1] = s1;
+                if (kr == 0 && slot < I.nc) {   // at the first A tile's sub-item, whichever wave summed them
+                    const size_t nit = W >= 8 ? (size_t)bi * W : (size_t)it;
+                    npart[(nit * kXT + slot) * 2] = s0;
+                    npart[(nit * kXT + slot) * 2 + 1] = s1;
                 }
             }
         }
