@@ -605,23 +605,37 @@ __global__ __launch_bounds__((64 * SweepShape<PT, NC>::NW)) void k_sweep(SweepAr
                         // the quotient only where it can beat the best so far (the product test has
                         // a margin far beyond its rounding); the winner's value is the quotient
                         const double u2 = uw * uw, dt = den + a.tol;
-                        if (den > a.tol && (a.eps > 0 || u2 >= bsse * dt * (1.0 - 1e-12))) {
-                            const double sse = u2 / dt;
-                            if (sse > bsse) {
-                                bsse = sse;
-                                bidx = (int)i;
-                            }
-                            if (a.eps > 0) {
-                                // relative errors eps of the covariates move den = k - |f|^2 by at most
-                                // eps (k + |f|^2) and fy - w by eps sum |f_s Y_s|: the score's bound
+                        if (a.eps > 0) {
+                            if (den > a.tol) {
+                                // the screen: relative errors eps of the covariates move den = k - |f|^2
+                                // by at most eps (k + |f|^2) and fy - w by eps sum |f_s Y_s|, which
+                                // bounds the score.  Its quotients are reciprocals with one Newton
+                                // step (relative error far below 2^-38 from any seed better than
+                                // 2^-20), and the bound carries a relative margin of 2^-36 for them;
+                                // the screen's own best is approximate (exact rescoring follows).
+                                auto rcp = [](double x) {
+                                    const double r = __builtin_amdgcn_rcp(x);
+                                    return fma(r, fma(-x, r, 1.0), r);
+                                };
+                                const double sse = u2 * rcp(dt);
+                                if (sse > bsse) {
+                                    bsse = sse;
+                                    bidx = (int)i;
+                                }
                                 double ua = 0.0;
 #pragma unroll
                                 for (int s = 0; s < J; ++s) ua = fma(fabs(f[s]), fabs(Y[s]), ua);
                                 const double ffs = group_sum<G>(ff);
                                 ua = group_sum<G>(ua);
-                                const double rel = a.eps * ((den + 2.0 * ffs) / fmax(den, a.tol) +
-                                                            2.0 * ua / fmax(fabs(uw), 1e-300));
-                                bub = fmax(bub, sse * (1.0 + rel));
+                                const double rel = a.eps * ((den + 2.0 * ffs) * rcp(fmax(den, a.tol)) +
+                                                            2.0 * ua * rcp(fmax(fabs(uw), 1e-300)));
+                                bub = fmax(bub, sse * (1.0 + rel) * (1.0 + 0x1p-36));
+                            }
+                        } else if (den > a.tol && u2 >= bsse * dt * (1.0 - 1e-12)) {
+                            const double sse = u2 / dt;
+                            if (sse > bsse) {
+                                bsse = sse;
+                                bidx = (int)i;
                             }
                         }
                     }
