@@ -74,14 +74,14 @@ __global__ __launch_bounds__(64 * W, W == 4 ? 3 : 1) void k_tc_update(const FitD
         const int a = 16 * u + kc;
         const bool aok = a < na;
         const bool wact = 16 * u < na;   // this wave's tile holds A columns (uniform)
-        // |Bx|^2 and Bx.y of the new parent (blocks with the first A tile): at 8 waves X-column group
-        // g goes to active wave g mod (active waves), so no single wave carries them between
-        // barriers; fewer waves keep them on wave 0 (their register budget is tighter)
+        // |Bx|^2 and Bx.y of the new parent (blocks with the first A tile): at 4 and 8 waves X-column
+        // group g goes to active wave g mod (active waves), so no single wave carries them between
+        // barriers; 1 and 2 waves keep them on wave 0
         const bool norms = u == 0;
         const bool nblk = (I.pad & 255) == 0;
         const int nwa = max(1, min(W, (na + 15) / 16 - (I.pad & 255)));   // active waves of the block
         auto mine = [&](int g) {
-            if constexpr (W >= 8) return nblk && g % nwa == wv;
+            if constexpr (W >= 4) return nblk && g % nwa == wv;
             else return norms;
         };
         const int ka = a <= j ? a : j, pa = a <= j ? j : a - j - 1;
@@ -122,7 +122,10 @@ __global__ __launch_bounds__(64 * W, W == 4 ? 3 : 1) void k_tc_update(const FitD
             }
             if (jyl) *(float4 *)&sjy[buf][tid >> 3][4 * (tid & 7)] = jy;
         };
-        float4 ao[3][2], ab[3][2];   // [register set][half]: A columns are fetched two chunks ahead
+        // [register set][half]: A columns are fetched AS - 1 chunks ahead (4 waves: one, which leaves
+        // the registers for spreading the norms over the waves)
+        constexpr int AS = W == 4 ? 2 : 3;
+        float4 ao[AS][2], ab[AS][2];
         auto load_a = [&](int set, long long r) {
 #pragma unroll
             for (int hf = 0; hf < 2; ++hf) {
@@ -183,25 +186,25 @@ __global__ __launch_bounds__(64 * W, W == 4 ? 3 : 1) void k_tc_update(const FitD
                     for (int v = 0; v < 4; ++v) acc[g][v] += (double)part[g][v];
             }
         };
-        // prologue: chunk 0 into buffer 0, A of chunks 0 and 1 into register sets 0 and 1
+        // prologue: chunk 0 into buffer 0, A of chunks 0 (and 1) into register sets 0 (and 1)
         float4 xv[NXL], jy;
         const long long r0 = I.r0, r1 = I.r1;
         if (r0 < r1) {
             load_x(xv, jy, r0);
             if (wact) {
                 load_a(0, r0);
-                load_a(1, r0 + kChunk < r1 ? r0 + kChunk : r0);
+                if constexpr (AS == 3) load_a(1, r0 + kChunk < r1 ? r0 + kChunk : r0);
             }
             store_x(xv, jy, 0);
         }
         __syncthreads();
-        // chunk c at row r: A in register set c % 3 (a constant in each unrolled copy)
+        // chunk c at row r: A in register set c % AS (a constant in each unrolled copy)
         auto step = [&](long long r, int c, auto setc) {
-            constexpr int set = decltype(setc)::value, ahead = (set + 2) % 3;
+            constexpr int set = decltype(setc)::value, ahead = (set + AS - 1) % AS;
             const int cur = c & 1, nxt = cur ^ 1;
             // a missing chunk ahead reloads the current one (in bounds, unused)
             const long long rn = r + kChunk < r1 ? r + kChunk : r;
-            const long long ra = r + 2 * kChunk < r1 ? r + 2 * kChunk : r;
+            const long long ra = r + (AS - 1) * kChunk < r1 ? r + (AS - 1) * kChunk : r;
             load_x(xv, jy, rn);
             if (wact) {
                 load_a(ahead, ra);
@@ -221,9 +224,11 @@ __global__ __launch_bounds__(64 * W, W == 4 ? 3 : 1) void k_tc_update(const FitD
                 if (r >= r1) break;
                 step(r, c, std::integral_constant<int, 1>{});
                 r += kChunk, ++c;
-                if (r >= r1) break;
-                step(r, c, std::integral_constant<int, 2>{});
-                r += kChunk, ++c;
+                if constexpr (AS == 3) {
+                    if (r >= r1) break;
+                    step(r, c, std::integral_constant<int, 2 % AS>{});
+                    r += kChunk, ++c;
+                }
             }
         }
         const int it = bi * W + wv;   // sub-item of this wave
@@ -243,7 +248,7 @@ __global__ __launch_bounds__(64 * W, W == 4 ? 3 : 1) void k_tc_update(const FitD
                 s1 = s1 + __shfl_xor(s1, 16, 64);
                 s1 = s1 + __shfl_xor(s1, 32, 64);
                 if (kr == 0 && slot < I.nc) {   // at the first A tile's sub-item, whichever wave summed them
-                    const size_t nit = W >= 8 ? (size_t)bi * W : (size_t)it;
+                    const size_t nit = W >= 4 ? (size_t)bi * W : (size_t)it;
                     npart[(nit * kXT + slot) * 2] = s0;
                     npart[(nit * kXT + slot) * 2 + 1] = s1;
                 }
