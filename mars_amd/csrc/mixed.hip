@@ -74,14 +74,13 @@ __global__ __launch_bounds__(64 * W, W == 4 ? 3 : 1) void k_tc_update(const FitD
         const int a = 16 * u + kc;
         const bool aok = a < na;
         const bool wact = 16 * u < na;   // this wave's tile holds A columns (uniform)
-        // |Bx|^2 and Bx.y of the new parent (blocks with the first A tile): at 4 and 8 waves X-column
-        // group g goes to active wave g mod (active waves), so no single wave carries them between
-        // barriers; 1 and 2 waves keep them on wave 0
+        // |Bx|^2 and Bx.y of the new parent (blocks with the first A tile): X-column group g goes to
+        // active wave g mod (active waves), so no single wave carries them between barriers
         const bool norms = u == 0;
         const bool nblk = (I.pad & 255) == 0;
         const int nwa = max(1, min(W, (na + 15) / 16 - (I.pad & 255)));   // active waves of the block
         auto mine = [&](int g) {
-            if constexpr (W >= 4) return nblk && g % nwa == wv;
+            if constexpr (W >= 2) return nblk && g % nwa == wv;
             else return norms;
         };
         const int ka = a <= j ? a : j, pa = a <= j ? j : a - j - 1;
@@ -248,7 +247,7 @@ __global__ __launch_bounds__(64 * W, W == 4 ? 3 : 1) void k_tc_update(const FitD
                 s1 = s1 + __shfl_xor(s1, 16, 64);
                 s1 = s1 + __shfl_xor(s1, 32, 64);
                 if (kr == 0 && slot < I.nc) {   // at the first A tile's sub-item, whichever wave summed them
-                    const size_t nit = W >= 4 ? (size_t)bi * W : (size_t)it;
+                    const size_t nit = W >= 2 ? (size_t)bi * W : (size_t)it;
                     npart[(nit * kXT + slot) * 2] = s0;
                     npart[(nit * kXT + slot) * 2 + 1] = s1;
                 }
