@@ -774,16 +774,21 @@ __global__ __launch_bounds__(64 * (PT / 16)) void k_sweep_sum(SweepArgs a)
         return wv == 0 && lane < nr ? dl[r0 + (size_t)tt * kTR + lane] : 0.0f;
     };
     // the running prefix W of a tile's deltas d into sw[b] (wave 0)
+    // (the 16 rows are one DPP row of lanes: row_shr moves, which shift in zeros, and a readlane,
+    // instead of cross-lane permutes through the LDS unit while the other waves wait)
+    static_assert(kTR == 16, "one tile is one DPP row");
     auto deltas = [&](float df, int b) {
         if (wv != 0) return;
         double w = (double)df;
-#pragma unroll
-        for (int o = 1; o < kTR; o <<= 1) {
-            const double v = __shfl_up(w, o, 64);
-            if (lane >= o) w += v;
-        }
+        w += dpp_perm<0x111>(w);   // row_shr:1
+        w += dpp_perm<0x112>(w);   // row_shr:2
+        w += dpp_perm<0x114>(w);   // row_shr:4
+        w += dpp_perm<0x118>(w);   // row_shr:8
         if (lane < kTR) sw[b][lane] = wbase + w;
-        wbase += __shfl(w, kTR - 1, 64);
+        const unsigned long long x = __builtin_bit_cast(unsigned long long, w);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, kTR - 1);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), kTR - 1);
+        wbase += __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
     };
     if (T > 0) {
         R.template load<0>(0);
