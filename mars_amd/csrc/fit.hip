@@ -988,15 +988,17 @@ void fit_append_batch(mars_hip_ctx *ctx, std::vector<AppendArg> &args, std::vect
     ctx->agram.ensure(F * ctx->ldb * nb);
     ctx->acoef.ensure(F * ctx->ldb);
     int *gate = ctx->flag.p;
-    double nrows = 0, mm = 0;
+    // algorithmic bytes of one sweep, each array once: the m f32 Bo columns read by the dots and
+    // by the subtraction (8·n·m), the FP64 residual built, read by the dots, the subtraction and
+    // the store and rewritten (40·n), the parent and X columns, the new B and Bo columns, the
+    // newest Bo column and y (24·n).  The retry sweep, which runs only where the DGKS gate
+    // fires, is not counted.
+    double abytes = 0;
     for (size_t f = 0; f < F; ++f)
-        if (args[f].active) {
-            nrows += (double)ctx->fits[f].n;
-            mm = std::max(mm, (double)ctx->fits[f].m);
-        }
+        if (args[f].active) abytes += 8.0 * (double)ctx->fits[f].n * ((double)ctx->fits[f].m + 8.0);
     {
         h2d_flush(ctx);  // fits, arguments and column pointers in one copy
-        KTimer kt(ctx, K_APPEND, 0.0, 8.0 * nrows * (3.0 * (mm + 1) + 4));
+        KTimer kt(ctx, K_APPEND, 0.0, abytes);
         const dim3 grid(nb, F);
         hipLaunchKernelGGL(k_app_build, grid, dim3(256), 0, st, ctx->d_fits.p, ctx->aarg.p, nb, pa, ctx->scal.p);
         MH_LAUNCH_CHECK();
