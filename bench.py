@@ -379,8 +379,9 @@ def main():
         copy_probe()
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design "
-              "is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (a multi-GPU run "
+              "would shard the candidate columns, or the independent fits of a batch, one process "
+              "per GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     data, fit_kw = workload_inputs(args.workload)
